@@ -1,0 +1,215 @@
+// builder.cpp -- host-side ExRaBitQ HA index quantizer (OpenMP).
+//
+// Produces the on-disk/in-HBM HA layout that the search path reads:
+//   short blocks : per 32 vectors, 4D bytes of packed 1-bit FastScan codes followed by
+//                  32 float factors ||o - c||          (Quantizer.hpp:86-125, :277-290)
+//   long codes   : per vector D*EX/8 bytes of the compacted EX-bit code (:491-631)
+//   ex factors   : per vector xipnorm = 2*||o - c|| / <o_bar + 1/2, |u|> (:457-489)
+// The caller supplies, per vector, the rotated residual r = o*P - c*P and ||o - c||.
+// The EX-bit code search is the critical-point enumeration of DataQuantizer::fast_quantize
+// (Quantizer.hpp:138-219): it has to choose the same codes, so it follows the same steps.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <queue>
+#include <utility>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// Position of vector v%16 inside a codebook's 16 bytes (pack_codes.hpp:44: perm0 places
+// vector w < 8 at byte 2w and vector w+8 at byte 2w+1).
+inline int byte_of_vec(int v) {
+    int w = v & 15;
+    return w < 8 ? 2 * w : 2 * (w - 8) + 1;
+}
+
+// Critical-point search for the rescale factor t maximising <o_bar(t)+1/2, o>/||o_bar(t)+1/2||
+// over o_bar(t)_i = floor(t*o_i + eps) clipped to 2^EX - 1; o holds |u| (all >= 0).
+// Writes the codes and returns 1/<o_bar + 1/2, o> (1.0 if not finite).
+float quantize_magnitudes(const float* o, size_t D, int ex_bits, uint8_t* code,
+                          std::vector<int>& cur, std::vector<std::pair<double, size_t>>& heap_buf) {
+    const double eps = 1e-5;
+    const int n_enum = 10;
+    const int top = (1 << ex_bits) - 1;
+    double max_o = -1;
+    for (size_t i = 0; i < D; i++)
+        if (o[i] > max_o) max_o = o[i];
+    double t_start = (double)(top / 3) / max_o;
+    double t_end = (double)(top + n_enum) / max_o;
+
+    double sqr_den = D * 0.25;
+    double num = 0;
+    for (size_t i = 0; i < D; i++) {
+        cur[i] = int(t_start * o[i] + eps);
+        sqr_den += cur[i] * cur[i] + cur[i];
+        num += (cur[i] + 0.5) * o[i];
+    }
+    using Ev = std::pair<double, size_t>;
+    heap_buf.clear();
+    for (size_t i = 0; i < D; i++) heap_buf.emplace_back((double)(cur[i] + 1) / o[i], i);
+    std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> events(std::greater<Ev>(),
+                                                                       std::move(heap_buf));
+    double best_ip = 0, best_t = 0;
+    while (!events.empty()) {
+        Ev e = events.top();
+        events.pop();
+        size_t i = e.second;
+        int v = ++cur[i];
+        sqr_den += 2 * v;
+        num += o[i];
+        double ip = num / std::sqrt(sqr_den);
+        if (ip > best_ip) {
+            best_ip = ip;
+            best_t = e.first;
+        }
+        if (v < top) {
+            double tn = (double)(v + 1) / o[i];
+            if (tn < t_end) events.emplace(tn, i);
+        }
+    }
+    double numer = 0;
+    for (size_t i = 0; i < D; i++) {
+        int c = int(best_t * o[i] + eps);
+        if (c > top) c = top;
+        code[i] = (uint8_t)c;
+        numer += (c + 0.5) * o[i];
+    }
+    float ipn = (float)(1.0 / numer);
+    if (!std::isfinite(ipn)) ipn = 1.f;
+    return ipn;
+}
+
+// Compacted EX-bit layouts (Quantizer.hpp:491-631).
+void compact_code(const uint8_t* raw, size_t D, int ex_bits, uint8_t* out) {
+    switch (ex_bits) {
+    case 8:
+        std::memcpy(out, raw, D);
+        break;
+    case 4:
+        for (size_t d = 0; d < D; d += 32, out += 16)
+            for (int j = 0; j < 16; j++) out[j] = (uint8_t)(raw[d + j] | (raw[d + 16 + j] << 4));
+        break;
+    case 2:
+    case 3:
+        for (size_t d = 0; d < D; d += 64) {
+            for (int j = 0; j < 16; j++)
+                out[j] = (uint8_t)((raw[d + j] & 3) | ((raw[d + 16 + j] & 3) << 2) |
+                                   ((raw[d + 32 + j] & 3) << 4) | ((raw[d + 48 + j] & 3) << 6));
+            out += 16;
+            if (ex_bits == 3) {
+                uint64_t T = 0;
+                for (int t = 0; t < 64; t++)
+                    T |= (uint64_t)((raw[d + t] >> 2) & 1) << (8 * (t & 7) + (t >> 3));
+                std::memcpy(out, &T, 8);
+                out += 8;
+            }
+        }
+        break;
+    case 6:
+    case 7:
+        for (size_t d = 0; d < D; d += 64) {
+            for (int j = 0; j < 16; j++) {
+                out[j] = (uint8_t)((raw[d + j] & 63) | (((raw[d + 32 + j] >> 4) & 3) << 6));
+                out[16 + j] =
+                    (uint8_t)((raw[d + 16 + j] & 63) | (((raw[d + 48 + j] >> 4) & 3) << 6));
+                out[32 + j] = (uint8_t)((raw[d + 32 + j] & 15) | ((raw[d + 48 + j] & 15) << 4));
+            }
+            out += 48;
+            if (ex_bits == 7) {
+                uint64_t T = 0;
+                for (int t = 0; t < 64; t++)
+                    T |= (uint64_t)((raw[d + t] >> 6) & 1) << (8 * (t & 7) + (t >> 3));
+                std::memcpy(out, &T, 8);
+                out += 8;
+            }
+        }
+        break;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Quantize one cluster: r[n x D] rotated residuals (cluster-local order), onorm[n].
+// blocks: ceil(n/32) * (4D + 128) bytes; long_code: n * D*EX/8; xipnorm: n.
+int erqb_quantize_cluster(const float* r, const float* onorm, size_t n, size_t D, int ex_bits,
+                          uint8_t* blocks, uint8_t* long_code, float* xipnorm) {
+    if (D % 64 != 0) return -1;
+    if (!(ex_bits == 2 || ex_bits == 3 || ex_bits == 4 || ex_bits == 6 || ex_bits == 7 ||
+          ex_bits == 8))
+        return -1;
+    const size_t M = D / 4;
+    const size_t block_bytes = 4 * D + 128;
+    const size_t code_len = D * (size_t)ex_bits / 8;
+    const int mask = (1 << ex_bits) - 1;
+    const size_t nblocks = (n + 31) / 32;
+    std::memset(blocks, 0, nblocks * block_bytes);
+
+    std::vector<float> absu(D);
+    std::vector<uint8_t> raw(D);
+    std::vector<int> cur(D);
+    std::vector<std::pair<double, size_t>> heap_buf;
+    heap_buf.reserve(D);
+    for (size_t v = 0; v < n; v++) {
+        const float* x = r + v * D;
+        double ss = 0;
+        for (size_t d = 0; d < D; d++) ss += (double)x[d] * x[d];
+        float inv = ss > 0 ? (float)(1.0 / std::sqrt(ss)) : 0.f;
+        for (size_t d = 0; d < D; d++) absu[d] = std::fabs(x[d] * inv);
+
+        // 1-bit code: sign of the rotated residual (Quantizer.hpp:356-361), packed into
+        // the FastScan block (nibble m = dims 4m..4m+3, dim 4m at bit 3).
+        uint8_t* blk = blocks + (v / 32) * block_bytes;
+        int vb = (int)(v % 32);
+        int j = byte_of_vec(vb);
+        for (size_t m = 0; m < M; m++) {
+            const float* xm = x + 4 * m;
+            int nib = ((xm[0] > 0) << 3) | ((xm[1] > 0) << 2) | ((xm[2] > 0) << 1) | (xm[3] > 0);
+            uint8_t* cb = blk + (m >> 1) * 32 + (m & 1) * 16;
+            cb[j] |= (uint8_t)(vb < 16 ? nib : nib << 4);
+        }
+        reinterpret_cast<float*>(blk + 4 * D)[vb] = onorm[v];
+
+        // EX-bit code on |u|, flipped where the sign bit is 0 (Quantizer.hpp:470-488).
+        float ipn = quantize_magnitudes(absu.data(), D, ex_bits, raw.data(), cur, heap_buf);
+        xipnorm[v] = ipn * 2 * onorm[v];
+        for (size_t d = 0; d < D; d++)
+            if (!(x[d] > 0)) raw[d] = (uint8_t)((~raw[d]) & mask);
+        compact_code(raw.data(), D, ex_bits, long_code + v * code_len);
+    }
+    return 0;
+}
+
+// All clusters, OpenMP over clusters.  r/onorm are in cluster order (cluster 0's
+// vectors first); outputs are the contiguous SHORT_DATA / LONG_CODE / EX_FACTOR arrays.
+int erqb_quantize_all(const float* r, const float* onorm, const uint64_t* cluster_sizes, size_t K,
+                      size_t D, int ex_bits, uint8_t* short_data, uint8_t* long_code,
+                      float* xipnorm, int threads) {
+    std::vector<size_t> vstart(K + 1, 0), bstart(K + 1, 0);
+    for (size_t k = 0; k < K; k++) {
+        vstart[k + 1] = vstart[k] + cluster_sizes[k];
+        bstart[k + 1] = bstart[k] + (cluster_sizes[k] + 31) / 32;
+    }
+    const size_t block_bytes = 4 * D + 128;
+    const size_t code_len = D * (size_t)ex_bits / 8;
+    int err = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+    for (long k = 0; k < (long)K; k++) {
+        size_t v0 = vstart[k];
+        err |= erqb_quantize_cluster(r + v0 * D, onorm + v0, cluster_sizes[k], D, ex_bits,
+                                     short_data + bstart[k] * block_bytes,
+                                     long_code + v0 * code_len, xipnorm + v0) != 0;
+    }
+    return err ? -1 : 0;
+}
+
+}  // extern "C"

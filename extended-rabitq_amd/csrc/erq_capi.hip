@@ -1,0 +1,517 @@
+// erq_capi.hip -- the extern "C" boundary of liberq.so (declared in include/erq.h).
+//
+// Owns the resident index (the HA arrays of IVF::load, IVF.hpp:254-293, copied to HBM
+// unchanged), a per-handle stream and a grow-only workspace, and sequences the search
+// stages: rotation GEMM -> centroid GEMM -> probe selection -> per-query scan.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "erq.h"
+#include "erq_internal.h"
+
+namespace erq {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? ERQ_ENOMEM : ERQ_EHIP;
+}
+
+int workspace(erq_index* h, size_t bytes, void** out) {
+    if (bytes > h->ws_bytes) {
+        if (h->ws) {
+            // kernels of earlier calls may still read it on a caller's stream
+            hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) return hip_fail(e, "workspace sync");
+            (void)hipFree(h->ws);
+            h->ws = nullptr;
+            h->ws_bytes = 0;
+        }
+        size_t want = bytes + bytes / 4;
+        hipError_t e = hipMalloc(&h->ws, want);
+        if (e != hipSuccess) return hip_fail(e, "workspace alloc");
+        h->ws_bytes = want;
+    }
+    *out = h->ws;
+    return ERQ_OK;
+}
+
+}  // namespace erq
+
+using namespace erq;
+
+namespace {
+
+constexpr size_t kMaxK = 8192;   // probe selection keeps all centroid keys in LDS
+constexpr size_t kMaxD = 4096;   // per-wave LDS slice of the scan kernel
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+bool ex_ok(size_t ex) { return ex == 2 || ex == 3 || ex == 4 || ex == 6 || ex == 7 || ex == 8; }
+
+template <typename T>
+int upload(T** dst, const T* src, size_t count, const char* what) {
+    if (count == 0) {
+        *dst = nullptr;
+        return ERQ_OK;
+    }
+    hipError_t e = hipMalloc(dst, count * sizeof(T));
+    if (e != hipSuccess) return hip_fail(e, what);
+    e = hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, what);
+    return ERQ_OK;
+}
+
+void release(erq_index* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void* ptrs[] = {h->short_data, h->long_code, h->ex_factor, h->ids, h->centroids,
+                    h->cnorm,      h->P,         h->csize,     h->bstart, h->vstart, h->ws};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+struct Ws {
+    char* base;
+    size_t off = 0;
+    template <typename T>
+    T* take(size_t n) {
+        off = align_up(off, 256);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += n * sizeof(T);
+        return p;
+    }
+};
+
+hipStream_t pick(erq_index* h, void* s) { return s ? static_cast<hipStream_t>(s) : h->stream; }
+
+}  // namespace
+
+extern "C" {
+
+const char* erq_last_error(void) { return g_err.c_str(); }
+
+int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint64_t* cluster_sizes,
+                     const float* P, const float* centroids, const uint8_t* short_data,
+                     const uint8_t* long_code, const float* ex_factor, const uint32_t* ids,
+                     erq_index** out) {
+    if (!out || !cluster_sizes || !centroids || !short_data || !long_code || !ex_factor || !ids) {
+        set_error("erq_index_create: null argument");
+        return ERQ_EINVAL;
+    }
+    *out = nullptr;
+    if (!ex_ok(EX_BITS)) {
+        set_error("erq_index_create: EX_BITS must be one of 2,3,4,6,7,8");
+        return ERQ_EFORMAT;
+    }
+    if (K == 0 || DIM == 0) {
+        set_error("erq_index_create: empty index");
+        return ERQ_EFORMAT;
+    }
+    const size_t D = align_up(DIM, 64);
+    if (K > kMaxK || D > kMaxD) {
+        set_error("erq_index_create: K > 8192 or padded dim > 4096 is not supported");
+        return ERQ_EUNSUPPORTED;
+    }
+    if (N > 0xFFFFFFFFull) {
+        set_error("erq_index_create: N exceeds 32-bit ids");
+        return ERQ_EUNSUPPORTED;
+    }
+    size_t total = 0, nblocks = 0;
+    std::vector<uint32_t> cs(K), bs(K + 1), vs(K + 1);
+    for (size_t k = 0; k < K; k++) {
+        cs[k] = (uint32_t)cluster_sizes[k];
+        bs[k] = (uint32_t)nblocks;
+        vs[k] = (uint32_t)total;
+        total += cluster_sizes[k];
+        nblocks += (cluster_sizes[k] + 31) / 32;
+    }
+    bs[K] = (uint32_t)nblocks;
+    vs[K] = (uint32_t)total;
+    if (total != N) {
+        set_error("erq_index_create: cluster sizes do not sum to N");
+        return ERQ_EFORMAT;
+    }
+    erq_index* h = new erq_index();
+    hipError_t e = hipGetDevice(&h->device);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(e, "hipGetDevice");
+    }
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    h->N = N;
+    h->DIM = DIM;
+    h->D = D;
+    h->K = K;
+    h->EX = EX_BITS;
+    h->nblocks = nblocks;
+    h->sizes.assign(cluster_sizes, cluster_sizes + K);
+    const size_t BB = 4 * D + 128;
+    int rc;
+    if ((rc = upload(&h->short_data, short_data, nblocks * BB, "upload short data")) ||
+        (rc = upload(&h->long_code, long_code, N * D * EX_BITS / 8, "upload long codes")) ||
+        (rc = upload(&h->ex_factor, ex_factor, N, "upload ex factors")) ||
+        (rc = upload(&h->ids, ids, N, "upload ids")) ||
+        (rc = upload(&h->centroids, centroids, K * D, "upload centroids")) ||
+        (rc = upload(&h->csize, cs.data(), K, "upload sizes")) ||
+        (rc = upload(&h->bstart, bs.data(), K + 1, "upload block starts")) ||
+        (rc = upload(&h->vstart, vs.data(), K + 1, "upload vector starts"))) {
+        release(h);
+        return rc;
+    }
+    if (P && (rc = upload(&h->P, P, D * D, "upload rotation"))) {
+        release(h);
+        return rc;
+    }
+    e = hipMalloc(&h->cnorm, K * sizeof(float));
+    if (e != hipSuccess) {
+        release(h);
+        return hip_fail(e, "alloc centroid norms");
+    }
+    e = launch_row_sqnorm(h->centroids, (int)K, (int)D, (int)D, h->cnorm, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        release(h);
+        return hip_fail(e, "centroid norms");
+    }
+    *out = h;
+    return ERQ_OK;
+}
+
+int erq_index_load(const char* path, erq_index** out) {
+    if (!path || !out) {
+        set_error("erq_index_load: null argument");
+        return ERQ_EINVAL;
+    }
+    *out = nullptr;
+    std::ifstream in(path, std::ios::binary);
+    if (!in) {
+        set_error(std::string("erq_index_load: cannot open ") + path);
+        return ERQ_EIO;
+    }
+    uint64_t hdr[4];
+    if (!in.read(reinterpret_cast<char*>(hdr), sizeof(hdr))) {
+        set_error("erq_index_load: truncated header");
+        return ERQ_EIO;
+    }
+    const size_t N = hdr[0], DIM = hdr[1], K = hdr[2], EX = hdr[3];
+    if (!ex_ok(EX) || K == 0 || DIM == 0 || K > (1u << 24) || DIM > (1u << 16)) {
+        set_error("erq_index_load: bad header");
+        return ERQ_EFORMAT;
+    }
+    const size_t D = align_up(DIM, 64);
+    std::vector<uint64_t> sizes(K);
+    if (!in.read(reinterpret_cast<char*>(sizes.data()), K * 8)) {
+        set_error("erq_index_load: truncated cluster sizes");
+        return ERQ_EIO;
+    }
+    size_t total = 0, nblocks = 0;
+    for (auto s : sizes) {
+        total += s;
+        nblocks += (s + 31) / 32;
+    }
+    if (total != N) {
+        set_error("erq_index_load: cluster sizes do not sum to N");
+        return ERQ_EFORMAT;
+    }
+    std::vector<float> P(D * D), C(K * D), ef(N);
+    std::vector<uint8_t> sd(nblocks * (4 * D + 128)), lc(N * D * EX / 8);
+    std::vector<uint32_t> ids(N);
+    bool ok = in.read(reinterpret_cast<char*>(P.data()), P.size() * 4) &&
+              in.read(reinterpret_cast<char*>(C.data()), C.size() * 4) &&
+              in.read(reinterpret_cast<char*>(sd.data()), sd.size()) &&
+              in.read(reinterpret_cast<char*>(lc.data()), lc.size()) &&
+              in.read(reinterpret_cast<char*>(ef.data()), ef.size() * 4) &&
+              in.read(reinterpret_cast<char*>(ids.data()), ids.size() * 4);
+    if (!ok) {
+        set_error("erq_index_load: truncated index body");
+        return ERQ_EIO;
+    }
+    return erq_index_create(N, DIM, K, EX, sizes.data(), P.data(), C.data(), sd.data(), lc.data(),
+                            ef.data(), ids.data(), out);
+}
+
+void erq_index_free(erq_index* h) { release(h); }
+
+size_t erq_padded_dim(const erq_index* h) { return h ? h->D : 0; }
+size_t erq_num_clusters(const erq_index* h) { return h ? h->K : 0; }
+size_t erq_dim(const erq_index* h) { return h ? h->DIM : 0; }
+size_t erq_num_vectors(const erq_index* h) { return h ? h->N : 0; }
+size_t erq_ex_bits(const erq_index* h) { return h ? h->EX : 0; }
+
+const void* erq_device_array(const erq_index* h, int which) {
+    if (!h) return nullptr;
+    switch (which) {
+    case 0: return h->short_data;
+    case 1: return h->long_code;
+    case 2: return h->ex_factor;
+    case 3: return h->ids;
+    case 4: return h->centroids;
+    case 5: return h->P;
+    case 6: return h->csize;
+    case 7: return h->bstart;
+    case 8: return h->vstart;
+    default: return nullptr;
+    }
+}
+
+int erq_rotate_dev(erq_index* h, const float* q_dev, size_t nq, float* q_rot_dev, void* stream) {
+    if (!h || (!q_dev && nq) || (!q_rot_dev && nq)) {
+        set_error("erq_rotate_dev: null argument");
+        return ERQ_EINVAL;
+    }
+    if (!h->P) {
+        set_error("erq_rotate_dev: index has no rotation matrix");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t s = pick(h, stream);
+    const float* A = q_dev;
+    if (h->DIM != h->D) {
+        void* ws;
+        int rc = workspace(h, nq * h->D * sizeof(float), &ws);
+        if (rc) return rc;
+        hipError_t e = launch_pad_rows(q_dev, (int)nq, (int)h->DIM, (float*)ws, (int)h->D, s);
+        if (e != hipSuccess) return hip_fail(e, "pad queries");
+        A = (const float*)ws;
+    }
+    hipError_t e = launch_gemm(A, (int)h->D, h->P, (int)h->D, 0, q_rot_dev, (int)h->D, (int)nq,
+                               (int)h->D, (int)h->D, 0, nullptr, nullptr, s);
+    if (e != hipSuccess) return hip_fail(e, "rotation gemm");
+    return ERQ_OK;
+}
+
+int erq_centroid_dist_dev(erq_index* h, const float* q_rot_dev, size_t nq, float* out_dev,
+                          void* stream) {
+    if (!h || (nq && (!q_rot_dev || !out_dev))) {
+        set_error("erq_centroid_dist_dev: null argument");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t s = pick(h, stream);
+    void* wsp;
+    int rc = workspace(h, align_up(nq * 4, 256), &wsp);
+    if (rc) return rc;
+    float* qn = (float*)wsp;
+    hipError_t e = launch_row_sqnorm(q_rot_dev, (int)nq, (int)h->D, (int)h->D, qn, s);
+    if (e == hipSuccess)
+        e = launch_gemm(q_rot_dev, (int)h->D, h->centroids, (int)h->D, 1, out_dev, (int)h->K,
+                        (int)nq, (int)h->K, (int)h->D, 1, qn, h->cnorm, s);
+    if (e != hipSuccess) return hip_fail(e, "centroid gemm");
+    return ERQ_OK;
+}
+
+static int probe_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
+                        uint32_t* pid, float* pd, hipStream_t s) {
+    const size_t K = h->K, D = h->D;
+    void* wsp;
+    size_t need = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
+    int rc = workspace(h, need, &wsp);
+    if (rc) return rc;
+    Ws ws{(char*)wsp};
+    float* approx = ws.take<float>(nq * K);
+    float* qn = ws.take<float>(nq);
+    hipError_t e = launch_row_sqnorm(q_rot_dev, (int)nq, (int)D, (int)D, qn, s);
+    if (e == hipSuccess)
+        e = launch_gemm(q_rot_dev, (int)D, h->centroids, (int)D, 1, approx, (int)K, (int)nq,
+                        (int)K, (int)D, 1, qn, h->cnorm, s);
+    if (e != hipSuccess) return hip_fail(e, "centroid gemm");
+    e = launch_probe_select(approx, qn, h->cnorm, q_rot_dev, h->centroids, (int)nq, (int)K, (int)D,
+                            (int)nprobe, pid, pd, s);
+    if (e != hipSuccess) return hip_fail(e, "probe selection");
+    return ERQ_OK;
+}
+
+int erq_probe_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
+                  uint32_t* probe_ids_dev, float* probe_sqr_y_dev, void* stream) {
+    if (!h || (nq && (!q_rot_dev || !probe_ids_dev || !probe_sqr_y_dev))) {
+        set_error("erq_probe_dev: null argument");
+        return ERQ_EINVAL;
+    }
+    if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
+        set_error("erq_probe_dev: nprobe must be in [1, min(K, 2048)]");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return probe_locked(h, q_rot_dev, nq, nprobe, probe_ids_dev, probe_sqr_y_dev,
+                        pick(h, stream));
+}
+
+static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                       const uint32_t* off, const uint32_t* pid, const float* pd, uint32_t* ids,
+                       float* dists, uint64_t* stats, hipStream_t s) {
+    ScanArgs a;
+    a.q_rot = q_rot_dev;
+    a.probe_off = off;
+    a.probe_ids = pid;
+    a.probe_sqr_y = pd;
+    a.centroids = h->centroids;
+    a.short_data = h->short_data;
+    a.long_code = h->long_code;
+    a.ex_factor = h->ex_factor;
+    a.ids = h->ids;
+    a.csize = h->csize;
+    a.bstart = h->bstart;
+    a.vstart = h->vstart;
+    a.out_ids = ids;
+    a.out_dists = dists;
+    a.stats = reinterpret_cast<unsigned long long*>(stats);
+    a.D = (int)h->D;
+    a.EX = (int)h->EX;
+    a.k = (int)k;
+    a.nq = (int)nq;
+    a.K = (int)h->K;
+    // HASearcher.hpp:56
+    a.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
+    hipError_t e = launch_scan(a, s);
+    if (e != hipSuccess) return hip_fail(e, "scan");
+    return ERQ_OK;
+}
+
+int erq_scan_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                 const uint32_t* probe_off_dev, const uint32_t* probe_ids_dev,
+                 const float* probe_sqr_y_dev, uint32_t* ids_dev, float* dists_dev,
+                 uint64_t* stats_dev, void* stream) {
+    if (!h || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev || !probe_sqr_y_dev ||
+                      !ids_dev))) {
+        set_error("erq_scan_dev: null argument");
+        return ERQ_EINVAL;
+    }
+    if (k == 0 || k > 1024) {
+        set_error("erq_scan_dev: k must be in [1, 1024]");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return scan_locked(h, q_rot_dev, nq, k, probe_off_dev, probe_ids_dev, probe_sqr_y_dev, ids_dev,
+                       dists_dev, stats_dev, pick(h, stream));
+}
+
+__global__ static void fill_offsets(uint32_t* off, size_t nq, uint32_t nprobe) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i <= nq) off[i] = (uint32_t)(i * nprobe);
+}
+
+int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
+                   uint32_t* ids_dev, float* dists_dev, uint64_t* stats_dev, void* stream) {
+    if (!h || (nq && (!q_rot_dev || !ids_dev))) {
+        set_error("erq_search_dev: null argument");
+        return ERQ_EINVAL;
+    }
+    if (k == 0 || k > 1024) {
+        set_error("erq_search_dev: k must be in [1, 1024]");
+        return ERQ_EINVAL;
+    }
+    if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
+        set_error("erq_search_dev: nprobe must be in [1, min(K, 2048)]");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    if (nq * nprobe >= 0xFFFFFFFFull) {
+        set_error("erq_search_dev: nq * nprobe exceeds 32-bit probe offsets");
+        return ERQ_EUNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t s = pick(h, stream);
+    // probe lists live in the second part of the workspace; size the whole thing first
+    const size_t K = h->K;
+    size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
+    size_t list_part = align_up(nq * nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) + 512;
+    void* wsp;
+    int rc = workspace(h, probe_part + list_part, &wsp);
+    if (rc) return rc;
+    Ws ws{(char*)wsp};
+    ws.off = probe_part;
+    uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
+    float* pd = ws.take<float>(nq * nprobe);
+    uint32_t* off = ws.take<uint32_t>(nq + 1);
+    rc = probe_locked(h, q_rot_dev, nq, nprobe, pid, pd, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off, nq,
+                       (uint32_t)nprobe);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "offsets");
+    return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s);
+}
+
+int erq_search_batch(erq_index* h, const float* q_rot, size_t nq, size_t k, size_t nprobe,
+                     uint32_t* results, float* dists) {
+    if (!h || (nq && (!q_rot || !results))) {
+        set_error("erq_search_batch: null argument");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    float* dq = nullptr;
+    uint32_t* di = nullptr;
+    float* dd = nullptr;
+    hipError_t e = hipMalloc(&dq, nq * h->D * 4);
+    if (e == hipSuccess) e = hipMalloc(&di, nq * k * 4);
+    if (e == hipSuccess) e = hipMalloc(&dd, nq * k * 4);
+    int rc = ERQ_OK;
+    if (e != hipSuccess) rc = hip_fail(e, "erq_search_batch alloc");
+    if (!rc) {
+        e = hipMemcpyAsync(dq, q_rot, nq * h->D * 4, hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "copy queries");
+    }
+    if (!rc) rc = erq_search_dev(h, dq, nq, k, nprobe, di, dd, nullptr, nullptr);
+    if (!rc) {
+        e = hipMemcpyAsync(results, di, nq * k * 4, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess && dists)
+            e = hipMemcpyAsync(dists, dd, nq * k * 4, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "copy results");
+    }
+    if (dq) (void)hipFree(dq);
+    if (di) (void)hipFree(di);
+    if (dd) (void)hipFree(dd);
+    return rc;
+}
+
+int erq_search(erq_index* h, const float* q_rot, size_t k, size_t nprobe, uint32_t* results,
+               float* dists) {
+    return erq_search_batch(h, q_rot, 1, k, nprobe, results, dists);
+}
+
+int erq_rotate(erq_index* h, const float* q, size_t nq, float* q_rot) {
+    if (!h || (nq && (!q || !q_rot))) {
+        set_error("erq_rotate: null argument");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    float *dq = nullptr, *dr = nullptr;
+    hipError_t e = hipMalloc(&dq, nq * h->DIM * 4);
+    if (e == hipSuccess) e = hipMalloc(&dr, nq * h->D * 4);
+    int rc = ERQ_OK;
+    if (e != hipSuccess) rc = hip_fail(e, "erq_rotate alloc");
+    if (!rc) {
+        e = hipMemcpyAsync(dq, q, nq * h->DIM * 4, hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "copy queries");
+    }
+    if (!rc) rc = erq_rotate_dev(h, dq, nq, dr, nullptr);
+    if (!rc) {
+        e = hipMemcpyAsync(q_rot, dr, nq * h->D * 4, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "copy rotated queries");
+    }
+    if (dq) (void)hipFree(dq);
+    if (dr) (void)hipFree(dr);
+    return rc;
+}
+
+}  // extern "C"

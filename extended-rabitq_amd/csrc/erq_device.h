@@ -1,0 +1,102 @@
+// erq_device.h -- device helpers shared by the liberq kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cfloat>
+
+namespace erq {
+
+constexpr int kWave = 64;
+
+// Make this wave's LDS writes visible to its other lanes (no workgroup barrier, so
+// waves of one workgroup may run independent queries and exit independently).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Sum over each 16-lane group in the order of _mm512_reduce_add_ps: lanes i and i^8,
+// then ^4, ^2, ^1 (IEEE addition is commutative, so every lane of the group ends with
+// the same, reference-ordered, value).
+__device__ __forceinline__ float butterfly16(float v) {
+    v = v + __shfl_xor(v, 8);
+    v = v + __shfl_xor(v, 4);
+    v = v + __shfl_xor(v, 2);
+    v = v + __shfl_xor(v, 1);
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Byte position (0..15) inside a codebook's 16 bytes that holds vector v%16 in the
+// FastScan block (reference pack_codes.hpp:44-65, perm0 = {0,8,1,9,...,7,15}).
+__device__ __forceinline__ int byte_of_vec(int v) {
+    int w = v & 15;
+    return w < 8 ? 2 * w : 2 * (w - 8) + 1;
+}
+// Inverse: vector (0..15) whose low nibble is at byte j.
+__device__ __forceinline__ int vec_of_byte(int j) { return (j & 1) ? (j >> 1) + 8 : (j >> 1); }
+
+// Ex-code values of dims 16s + l (s = 0..3) of one 64-dim chunk (8*EX bytes) for lane
+// l = 0..15 of a 16-lane group: layouts of DataQuantizer::store_compacted_code
+// (reference Quantizer.hpp:491-631), read as IP16_fxu8/IP32_fxu4/IP64_fxu{2,3,6,7} do.
+template <int EX>
+__device__ __forceinline__ void decode4(const uint8_t* __restrict__ c, int l, float e[4]) {
+    if constexpr (EX == 8) {
+        for (int s = 0; s < 4; s++) e[s] = (float)c[16 * s + l];
+    } else if constexpr (EX == 4) {
+        uint32_t b0 = c[l], b1 = c[16 + l];
+        e[0] = (float)(b0 & 15u);
+        e[1] = (float)(b0 >> 4);
+        e[2] = (float)(b1 & 15u);
+        e[3] = (float)(b1 >> 4);
+    } else if constexpr (EX == 2 || EX == 3) {
+        uint32_t b = c[l];
+        uint32_t tb = 0;
+        if constexpr (EX == 3) tb = c[16 + (l & 7)] >> (l >> 3);
+        for (int s = 0; s < 4; s++) {
+            uint32_t v = (b >> (2 * s)) & 3u;
+            if constexpr (EX == 3) v |= ((tb >> (2 * s)) & 1u) << 2;
+            e[s] = (float)v;
+        }
+    } else {  // 6, 7
+        uint32_t b0 = c[l], b1 = c[16 + l], b2 = c[32 + l];
+        uint32_t v[4];
+        v[0] = b0 & 63u;
+        v[1] = b1 & 63u;
+        v[2] = (b2 & 15u) | ((b0 >> 6) << 4);
+        v[3] = (b2 >> 4) | ((b1 >> 6) << 4);
+        if constexpr (EX == 7) {
+            uint32_t tb = c[48 + (l & 7)] >> (l >> 3);
+            for (int s = 0; s < 4; s++) v[s] |= ((tb >> (2 * s)) & 1u) << 6;
+        }
+        for (int s = 0; s < 4; s++) e[s] = (float)v[s];
+    }
+}
+
+// Float key with the same order as the float (for radix selection / sorting).
+__device__ __forceinline__ uint32_t float_key(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+
+}  // namespace erq

@@ -1,0 +1,74 @@
+// erq_internal.h -- handle layout and kernel launchers shared by the liberq sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+struct erq_index {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t N = 0, DIM = 0, D = 0, K = 0, EX = 0;
+    size_t nblocks = 0;
+    std::vector<uint64_t> sizes;
+    // resident arrays (HA layout, reference inc/index/IVF.hpp:28-40)
+    uint8_t* short_data = nullptr;  // nblocks * (4D + 128)
+    uint8_t* long_code = nullptr;   // N * D*EX/8
+    float* ex_factor = nullptr;     // N
+    uint32_t* ids = nullptr;        // N
+    float* centroids = nullptr;     // K x D
+    float* cnorm = nullptr;         // K, ||c||^2
+    float* P = nullptr;             // D x D
+    uint32_t* csize = nullptr;      // K
+    uint32_t* bstart = nullptr;     // K + 1
+    uint32_t* vstart = nullptr;     // K + 1
+    // grow-only workspace
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    std::mutex mu;
+};
+
+namespace erq {
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int workspace(erq_index* h, size_t bytes, void** out);
+
+struct ScanArgs {
+    const float* q_rot;
+    const uint32_t* probe_off;
+    const uint32_t* probe_ids;
+    const float* probe_sqr_y;
+    const float* centroids;
+    const uint8_t* short_data;
+    const uint8_t* long_code;
+    const float* ex_factor;
+    const uint32_t* ids;
+    const uint32_t* csize;
+    const uint32_t* bstart;
+    const uint32_t* vstart;
+    uint32_t* out_ids;
+    float* out_dists;
+    unsigned long long* stats;
+    int D, EX, k, nq, K;
+    float one_over_sqrtD;
+};
+hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
+
+// C[m][n] = sum_k A[m][k] * B(k, n); B row-major [K x N] (b_trans = 0) or stored as
+// B^T row-major [N x K] (b_trans = 1).  epi = 0: store; epi = 1: rn[m] + cn[n] - 2*acc.
+hipError_t launch_gemm(const float* A, int lda, const float* B, int ldb, int b_trans, float* C,
+                       int ldc, int M, int N, int K, int epi, const float* rn, const float* cn,
+                       hipStream_t s);
+hipError_t launch_row_sqnorm(const float* X, int rows, int cols, int ld, float* out, hipStream_t s);
+hipError_t launch_pad_rows(const float* src, int rows, int cols, float* dst, int ldd,
+                           hipStream_t s);
+
+// Ranked probe lists from approximate distances (see erq_probe.hip).
+hipError_t launch_probe_select(const float* approx, const float* qnorm, const float* cnorm,
+                               const float* q_rot, const float* centroids, int nq, int K, int D,
+                               int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s);
+
+}  // namespace erq

@@ -1,0 +1,185 @@
+// erq_probe.hip -- top-nprobe cluster selection with reference-exact distances.
+//
+// The MFMA contraction gives a[c] = ||q||^2 + ||c||^2 - 2 q.c, which ranks clusters but
+// is not the value the reference uses: FlatInitializer::centroids_distances
+// (Initializer.hpp:55-68) ranks and returns L2Sqr16 (space.hpp:18-39), 16 lane-strided
+// FMA chains of (q-c)^2 reduced by _mm512_reduce_add_ps.  Each query therefore:
+//   1. bounds |a[c] - L2Sqr16(c)| <= E[c] = (4D + 64) * 2^-24 * (||q||^2 + ||c||^2),
+//   2. finds U = the nprobe-th smallest a[c] + E[c] (radix select on float keys),
+//   3. keeps every c with a[c] - E[c] <= U (a superset of the exact top-nprobe),
+//   4. recomputes L2Sqr16 for those in the reference's order, sorts by (distance, id)
+//      and returns the first nprobe.
+// Equal distances are ordered by id (std::partial_sort leaves their order unspecified).
+#include "erq_device.h"
+#include "erq_internal.h"
+
+namespace erq {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kCapC = 2048;  // candidate capacity per query
+
+__device__ __forceinline__ bool cand_less(float da, uint32_t ia, float db, uint32_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+// Block-wide radix select: the key of rank `rank` (1-based) among keys[0..n).
+__device__ uint32_t radix_select(const uint32_t* keys, int n, uint32_t rank, uint32_t* hist,
+                                 uint32_t* sh_sel, uint32_t* sh_rank) {
+    const int tid = threadIdx.x;
+    uint32_t prefix = 0, pmask = 0;
+    if (tid == 0) *sh_rank = rank;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = tid; i < 256; i += kThreads) hist[i] = 0;
+        __syncthreads();
+        for (int c = tid; c < n; c += kThreads) {
+            uint32_t kv = keys[c];
+            if ((kv & pmask) == prefix) atomicAdd(&hist[(kv >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t r = *sh_rank, cum = 0, sel = 255;
+            for (int dgt = 0; dgt < 256; dgt++) {
+                if (cum + hist[dgt] >= r) {
+                    sel = (uint32_t)dgt;
+                    break;
+                }
+                cum += hist[dgt];
+            }
+            *sh_sel = sel;
+            *sh_rank = r - cum;
+        }
+        __syncthreads();
+        prefix |= *sh_sel << shift;
+        pmask |= 255u << shift;
+        __syncthreads();
+    }
+    return prefix;
+}
+
+// L2Sqr16 of q (in LDS) against centroid c by the 16 lanes of this thread's group.
+__device__ __forceinline__ float l2sqr16_group(const float* qs, const float* c, int D, int l,
+                                               bool active) {
+    float p = 0.f;
+    if (active)
+        for (int d = l; d < D; d += 16) {
+            float t = qs[d] - c[d];
+            p = fmaf(t, t, p);
+        }
+    return butterfly16(p);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    probe_select_kernel(const float* __restrict__ approx, const float* __restrict__ qnorm,
+                        const float* __restrict__ cnorm, const float* __restrict__ q_rot,
+                        const float* __restrict__ centroids, int K, int D, int nprobe,
+                        uint32_t* __restrict__ probe_ids, float* __restrict__ probe_d) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* keys = reinterpret_cast<uint32_t*>(smem);          // K
+    float* cd = reinterpret_cast<float*>(smem + 4 * (size_t)K);  // kCapC
+    uint32_t* ci = reinterpret_cast<uint32_t*>(cd + kCapC);      // kCapC
+    float* qs = reinterpret_cast<float*>(ci + kCapC);            // D
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
+
+    const int tid = threadIdx.x;
+    const int qi = blockIdx.x;
+    const int grp = tid >> 4, l = tid & 15;
+    const float* arow = approx + (size_t)qi * K;
+    const float qn = qnorm[qi];
+    const float coef = (float)(4 * D + 64) * 5.9604644775390625e-8f;  // 2^-24
+    const float* q = q_rot + (size_t)qi * D;
+    for (int d = tid; d < D; d += kThreads) qs[d] = q[d];
+    for (int c = tid; c < K; c += kThreads) keys[c] = float_key(arow[c] + coef * (qn + cnorm[c]));
+    if (tid == 0) sh_cnt = 0;
+    __syncthreads();
+
+    const float U = key_float(radix_select(keys, K, (uint32_t)nprobe, hist, &sh_sel, &sh_rank));
+    for (int c = tid; c < K; c += kThreads) {
+        if (arow[c] - coef * (qn + cnorm[c]) <= U) {
+            uint32_t slot = atomicAdd(&sh_cnt, 1u);
+            if (slot < (uint32_t)kCapC) ci[slot] = (uint32_t)c;
+        }
+    }
+    __syncthreads();
+    int cnt = (int)sh_cnt;
+    if (cnt <= kCapC) {
+        for (int base = 0; base < cnt; base += kThreads / 16) {
+            const int i = base + grp;
+            const bool act = i < cnt;
+            float p = l2sqr16_group(qs, centroids + (size_t)(act ? ci[i] : 0) * D, D, l, act);
+            if (act && l == 0) cd[i] = p;
+        }
+    } else {
+        // bound too loose for the candidate buffer: rank all K by the exact distance
+        for (int base = 0; base < K; base += kThreads / 16) {
+            const int c = base + grp;
+            const bool act = c < K;
+            float p = l2sqr16_group(qs, centroids + (size_t)(act ? c : 0) * D, D, l, act);
+            if (act && l == 0) keys[c] = float_key(p);
+        }
+        __syncthreads();
+        if (tid == 0) sh_cnt = 0;
+        const uint32_t U2 = radix_select(keys, K, (uint32_t)nprobe, hist, &sh_sel, &sh_rank);
+        for (int c = tid; c < K; c += kThreads) {
+            if (keys[c] <= U2) {
+                uint32_t slot = atomicAdd(&sh_cnt, 1u);
+                if (slot < (uint32_t)kCapC) {
+                    ci[slot] = (uint32_t)c;
+                    cd[slot] = key_float(keys[c]);
+                }
+            }
+        }
+        __syncthreads();
+        cnt = min((int)sh_cnt, kCapC);
+    }
+    int P2 = 1;
+    while (P2 < cnt) P2 <<= 1;
+    for (int i = cnt + tid; i < P2; i += kThreads) {
+        cd[i] = FLT_MAX;
+        ci[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+
+    // bitonic sort by (distance, id)
+    for (int size = 2; size <= P2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = tid; t < (P2 >> 1); t += kThreads) {
+                int lo = 2 * t - (t & (stride - 1));
+                int hi = lo + stride;
+                bool up = (lo & size) == 0;
+                float dl = cd[lo], dh = cd[hi];
+                uint32_t il = ci[lo], ih = ci[hi];
+                bool swap = up ? cand_less(dh, ih, dl, il) : cand_less(dl, il, dh, ih);
+                if (swap) {
+                    cd[lo] = dh;
+                    cd[hi] = dl;
+                    ci[lo] = ih;
+                    ci[hi] = il;
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = tid; i < nprobe; i += kThreads) {
+        probe_ids[(size_t)qi * nprobe + i] = ci[i];
+        probe_d[(size_t)qi * nprobe + i] = cd[i];
+    }
+}
+
+}  // namespace
+
+size_t probe_select_lds(int K, int D) { return 4 * (size_t)K + 8 * (size_t)kCapC + 4 * (size_t)D; }
+
+hipError_t launch_probe_select(const float* approx, const float* qnorm, const float* cnorm,
+                               const float* q_rot, const float* centroids, int nq, int K, int D,
+                               int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    if (nprobe < 1 || nprobe > kCapC || nprobe > K) return hipErrorInvalidValue;
+    size_t lds = probe_select_lds(K, D);
+    hipLaunchKernelGGL(probe_select_kernel, dim3(nq), dim3(kThreads), lds, s, approx, qnorm, cnorm,
+                       q_rot, centroids, K, D, nprobe, probe_ids, probe_d);
+    return hipGetLastError();
+}
+
+}  // namespace erq

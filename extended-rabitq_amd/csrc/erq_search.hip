@@ -1,0 +1,323 @@
+// erq_search.hip -- plain (per-query) ExRaBitQ HA scan on gfx950.
+//
+// One wavefront owns one query and replays the reference's sequential semantics
+// exactly: clusters in the given (ascending-distance) order, blocks in storage order,
+// each block's candidate mask taken against the k-th distance at block start, and
+// candidates inserted in ascending lane order (HASearcher.hpp:82-112, :187-237;
+// IVF.hpp:313-323).  Per cluster the wave builds, in its LDS slice:
+//   unit_q[D]      (q - c)/||q - c||                 normalize_query16, space.hpp:749-778
+//   lut[M][16]     u16 shifted subset sums           pack_high_acc_LUT, HASearcher.hpp:128-185
+// then scans 32-vector blocks straight from the HA layout in HBM (4D code bytes +
+// 32 float factors per block) and re-ranks survivors from their EX-bit long codes.
+#include "erq_device.h"
+#include "erq_internal.h"
+
+namespace erq {
+
+namespace {
+
+struct WaveLds {
+    float* uq;
+    uint16_t* lut;
+    int* sums;
+    float* pd;
+    uint32_t* pid;
+};
+
+__device__ __forceinline__ size_t wave_lds_bytes(int D, int k) {
+    int kcap = (k + 63) & ~63;
+    return (size_t)12 * D + 128 + (size_t)8 * kcap;
+}
+
+// ResultPool::insert + distk (Pool.hpp:15-31) with the pool in LDS, wave-parallel.
+// dist/id are wave-uniform.  New entries go before equal ones; a full pool rejects
+// only strictly larger distances.
+__device__ __forceinline__ void pool_insert(const WaveLds& w, int& psize, int k, int lane,
+                                            float dist, uint32_t id) {
+    if (psize == k && dist > w.pd[k - 1]) return;
+    int pos = 0;
+    for (int base = 0; base < psize; base += kWave) {
+        int i = base + lane;
+        bool lt = (i < psize) && (w.pd[i] < dist);
+        pos += __popcll(ballot(lt));
+    }
+    const int lim = psize < k - 1 ? psize : k - 1;
+    for (int base = lim & ~(kWave - 1); base >= 0; base -= kWave) {
+        int i = base + lane;
+        bool mv = (i > pos) && (i <= lim);
+        float vd = 0.f;
+        uint32_t vid = 0;
+        if (mv) {
+            vd = w.pd[i - 1];
+            vid = w.pid[i - 1];
+        }
+        wave_sync();
+        if (mv) {
+            w.pd[i] = vd;
+            w.pid[i] = vid;
+        }
+        wave_sync();
+    }
+    if (lane == 0) {
+        w.pd[pos] = dist;
+        w.pid[pos] = id;
+    }
+    wave_sync();
+    psize += (psize < k);
+}
+
+template <int EX, bool WRAP>
+__global__ void __launch_bounds__(64) scan_kernel(ScanArgs a, int per_wave_bytes) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int qi = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (qi >= a.nq) return;
+    const int D = a.D, M = D >> 2, k = a.k;
+    const int kcap = (k + 63) & ~63;
+    unsigned char* base = smem + (size_t)wid * per_wave_bytes;
+    WaveLds w;
+    w.uq = reinterpret_cast<float*>(base);
+    w.lut = reinterpret_cast<uint16_t*>(base + 4 * D);
+    w.sums = reinterpret_cast<int*>(base + 12 * D);
+    w.pd = reinterpret_cast<float*>(base + 12 * D + 128);
+    w.pid = reinterpret_cast<uint32_t*>(w.pd + kcap);
+    for (int i = lane; i < kcap; i += kWave) {
+        w.pd[i] = FLT_MAX;
+        w.pid[i] = 0;
+    }
+    int psize = 0;
+    wave_sync();
+
+    const float* q = a.q_rot + (size_t)qi * D;
+    const size_t BB = 4 * (size_t)D + 128;
+    const size_t CL = (size_t)D * EX / 8;
+    const int R = 1 << EX;
+    const float const_bound = 0.58f;  // FastScan.hpp:295
+    unsigned long long st_blocks = 0, st_vec = 0, st_surv = 0;
+    const int t4 = lane & 3, s16 = lane >> 2;
+    const int g = lane >> 4, l16 = lane & 15;
+
+    const uint32_t p0 = a.probe_off[qi], p1 = a.probe_off[qi + 1];
+    for (uint32_t p = p0; p < p1; ++p) {
+        const uint32_t cid = a.probe_ids[p];
+        if (cid >= (uint32_t)a.K) continue;  // not a cluster of this index
+        const float sqr_y = a.probe_sqr_y[p];
+        const float y = sqrtf(sqr_y);  // HASearcher.hpp:85
+        const float* c = a.centroids + (size_t)cid * D;
+
+        // ---- normalize_query16: unit residual + reference-ordered sum ----
+        float sumq;
+        if (y > 1e-5f) {
+            const float fac = (float)(1.0 / (double)y);
+            for (int d = lane; d < D; d += kWave) w.uq[d] = (q[d] - c[d]) * fac;
+            wave_sync();
+            float part = 0.f;
+            if (lane < 16)
+                for (int d = lane; d < D; d += 16) part = part + w.uq[d];
+            part = butterfly16(part);
+            sumq = __shfl(part, 0);
+        } else {
+            const float value = (float)(1.0 / (double)sqrtf((float)D));
+            for (int d = lane; d < D; d += kWave) w.uq[d] = value;
+            wave_sync();
+            sumq = (float)D * value;
+        }
+
+        // ---- high_acc_quantize16 + pack_high_acc_LUT ----
+        float mx = 0.f, mn = 0.f;
+        for (int d = lane; d < D; d += kWave) {
+            float u = w.uq[d];
+            mx = fmaxf(mx, u);
+            mn = fminf(mn, u);
+        }
+        mx = wave_max(mx);
+        mn = wave_min(mn);
+        const float vmax = fmaxf(fabsf(mn), fabsf(mx));
+        const float delta = vmax / 8191.0f;
+        int shift_part = 0;
+        for (int m = lane; m < M; m += kWave) {
+            int qv[4];
+            for (int t = 0; t < 4; t++) {
+                float tmp = w.uq[4 * m + t] / delta;
+                double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
+                qv[t] = (int)r;
+            }
+            // min over the 16 subset sums = sum of the negative terms
+            const int vmin = min(0, qv[0]) + min(0, qv[1]) + min(0, qv[2]) + min(0, qv[3]);
+            uint32_t wv[8];
+            for (int j = 0; j < 16; j += 2) {
+                // entry j: bit 3 <-> dim 4m, bit 0 <-> dim 4m+3 (HASearcher.hpp:130-158)
+                int L0 = ((j & 8) ? qv[0] : 0) + ((j & 4) ? qv[1] : 0) + ((j & 2) ? qv[2] : 0) - vmin;
+                int L1 = L0 + qv[3];
+                wv[j >> 1] = (uint32_t)L0 | ((uint32_t)L1 << 16);
+            }
+            uint4* dst = reinterpret_cast<uint4*>(w.lut + m * 16);
+            dst[0] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+            dst[1] = make_uint4(wv[4], wv[5], wv[6], wv[7]);
+            shift_part += vmin;
+        }
+        const int shift = wave_sum_i(shift_part);
+        wave_sync();
+
+        // ---- block scan (accumulate_one_block_high_acc + scan_one_block) ----
+        const int n = (int)a.csize[cid];
+        const int nb = (n + 31) >> 5;
+        const uint8_t* blk = a.short_data + (size_t)a.bstart[cid] * BB;
+        const uint32_t v0 = a.vstart[cid];
+        float distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+        const float c1 = (float)(0.5 * (double)sumq - (double)const_bound);
+        const float c2 = (-5.0f * y) * a.one_over_sqrtD;
+        const float qs = y * y;
+        for (int b = 0; b < nb; ++b, blk += BB) {
+            const int np = min(32, n - b * 32);
+            // Lane (t4, s16) reads dword t4 (bytes 4t4..4t4+3, 8 vectors) of codebooks
+            // s16, s16+16, ...; 64 lanes cover 256 contiguous code bytes per step.
+            uint32_t acc[8], acch[8];
+            for (int kk = 0; kk < 8; kk++) acc[kk] = acch[kk] = 0;
+            const uint8_t* cbase = blk + t4 * 4;
+            for (int i = 0; i < (M >> 4); ++i) {
+                const int cb = s16 + 16 * i;
+                const uint32_t wd =
+                    *reinterpret_cast<const uint32_t*>(cbase + (cb >> 1) * 32 + (cb & 1) * 16);
+                const uint16_t* L = w.lut + cb * 16;
+                for (int kk = 0; kk < 4; kk++) {
+                    const uint32_t byte = (wd >> (8 * kk)) & 0xFFu;
+                    const uint32_t e0 = L[byte & 15u], e1 = L[byte >> 4];
+                    acc[kk] += e0;
+                    acc[4 + kk] += e1;
+                    if (WRAP) {
+                        acch[kk] += e0 >> 8;
+                        acch[4 + kk] += e1 >> 8;
+                    }
+                }
+            }
+            if (WRAP) {
+                // even and odd codebooks (lane bit 2) wrap separately, per byte plane
+                for (int o = 8; o <= 32; o <<= 1)
+                    for (int kk = 0; kk < 8; kk++) {
+                        acc[kk] += __shfl_xor(acc[kk], o);
+                        acch[kk] += __shfl_xor(acch[kk], o);
+                    }
+                for (int kk = 0; kk < 8; kk++) {
+                    uint32_t lo = acc[kk] - (acch[kk] << 8);
+                    acc[kk] = (lo & 0xFFFFu) + ((acch[kk] & 0xFFFFu) << 8);
+                }
+                for (int kk = 0; kk < 8; kk++) acc[kk] += __shfl_xor(acc[kk], 4);
+            } else {
+                for (int o = 4; o <= 32; o <<= 1)
+                    for (int kk = 0; kk < 8; kk++) acc[kk] += __shfl_xor(acc[kk], o);
+            }
+            if (lane < 4)
+                for (int kk = 0; kk < 4; kk++) {
+                    int v = vec_of_byte(4 * lane + kk);
+                    w.sums[v] = (int)acc[kk];
+                    w.sums[v + 16] = (int)acc[4 + kk];
+                }
+            wave_sync();
+
+            const float* onorm = reinterpret_cast<const float*>(blk + 4 * (size_t)D);
+            bool cand = false;
+            float ipq_l = 0.f;
+            if (lane < 32) {
+                const int I = w.sums[lane] + shift;
+                ipq_l = (float)I * delta;
+                const float on = onorm[lane];
+                float t = ipq_l - c1;
+                t = t * c2;
+                t = t * on;
+                t = t + qs;
+                t = t + on * on;
+                cand = (lane < np) && (t < distk);
+            }
+            uint64_t mask = ballot(cand);
+            st_blocks++;
+            st_vec += np;
+
+            // ---- survivors, ascending lane order, 4 re-estimated at a time ----
+            while (mask) {
+                uint64_t mm = mask;
+                for (int r = 0; r < g; r++) mm &= mm - 1;
+                const int j = mm ? (__ffsll((unsigned long long)mm) - 1) : -1;
+                const int jj = j < 0 ? 0 : j;
+                const uint32_t vi = v0 + (uint32_t)(b * 32 + jj);
+                float part = 0.f;
+                if (j >= 0) {
+                    const uint8_t* code = a.long_code + (size_t)vi * CL;
+                    for (int ch = 0; ch < (D >> 6); ch++) {
+                        float e[4];
+                        decode4<EX>(code + ch * 8 * EX, l16, e);
+                        for (int s4 = 0; s4 < 4; s4++)
+                            part = fmaf(w.uq[64 * ch + 16 * s4 + l16], e[s4], part);
+                    }
+                }
+                const float ip = butterfly16(part);
+                const float ipq_j = __shfl(ipq_l, jj);
+                float exd = 0.f;
+                uint32_t idv = 0;
+                if (j >= 0) {
+                    // HASearcher.hpp:225-232 with C's float -> double promotions
+                    const float on = onorm[jj];
+                    const float sqr_x = on * on;
+                    const float A = sqr_x + sqr_y;
+                    const float B = a.ex_factor[vi] * y;
+                    const float C = (float)R * ipq_j + ip;
+                    const double G = (double)C - ((double)R - 0.5) * (double)sumq;
+                    const double H = (double)B * G;
+                    exd = (float)((double)A - H);
+                    idv = a.ids[vi];
+                }
+                const int cnt = min(4, __popcll(mask));
+                for (int gg = 0; gg < cnt; gg++) {
+                    const float dd = __shfl(exd, 16 * gg);
+                    const uint32_t ii = (uint32_t)__shfl((int)idv, 16 * gg);
+                    pool_insert(w, psize, k, lane, dd, ii);
+                }
+                for (int r = 0; r < cnt; r++) mask &= mask - 1;
+                st_surv += cnt;
+            }
+            distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+        }
+    }
+    uint32_t* oi = a.out_ids + (size_t)qi * k;
+    float* od = a.out_dists ? a.out_dists + (size_t)qi * k : nullptr;
+    for (int i = lane; i < k; i += kWave) {
+        oi[i] = i < psize ? w.pid[i] : 0u;
+        if (od) od[i] = i < psize ? w.pd[i] : FLT_MAX;
+    }
+    if (a.stats && lane == 0) {
+        atomicAdd(&a.stats[0], st_blocks);
+        atomicAdd(&a.stats[1], st_vec);
+        atomicAdd(&a.stats[2], st_surv);
+        atomicAdd(&a.stats[3], (unsigned long long)(p1 - p0));
+    }
+}
+
+template <int EX>
+hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
+    const int waves = 1;
+    const size_t per_wave = (size_t)12 * a.D + 128 + (size_t)8 * ((a.k + 63) & ~63);
+    const size_t lds = per_wave * waves;
+    dim3 grid((a.nq + waves - 1) / waves), block(64 * waves);
+    if (a.D > 2048)
+        hipLaunchKernelGGL((scan_kernel<EX, true>), grid, block, lds, s, a, (int)per_wave);
+    else
+        hipLaunchKernelGGL((scan_kernel<EX, false>), grid, block, lds, s, a, (int)per_wave);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_scan(const ScanArgs& a, hipStream_t s) {
+    switch (a.EX) {
+    case 2: return launch_ex<2>(a, s);
+    case 3: return launch_ex<3>(a, s);
+    case 4: return launch_ex<4>(a, s);
+    case 6: return launch_ex<6>(a, s);
+    case 7: return launch_ex<7>(a, s);
+    case 8: return launch_ex<8>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace erq

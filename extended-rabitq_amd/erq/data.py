@@ -1,0 +1,198 @@
+"""Synthetic workloads (Gaussian mixtures) and k-means IVF partitioning.
+
+Shapes follow the benchmark configurations:
+  mixture_isotropic   centres ~ N(0, I), x = centre + sigma * N(0, I)
+  mixture_decay       per-dim std s_i = i^-0.25: centre = s*z, x = centre + sigma * s*z'
+  mixture_powerlaw    eigen-spectrum lambda_i ~ i^-alpha in a random orthonormal basis U:
+                      centre = U(sqrt(lambda)*z), x = centre + sigma*U(sqrt(lambda)*z'),
+                      then L2-normalised
+numpy generators are used for CPU-sized cases; the *_torch variants generate on the GPU
+(torch is used for device memory and RNG only).
+"""
+import numpy as np
+
+
+def mixture_isotropic(n, dim, ncomp, sigma, seed, centres=None):
+    rng = np.random.default_rng(seed)
+    if centres is None:
+        centres = rng.standard_normal((ncomp, dim)).astype(np.float32)
+    lab = rng.integers(0, ncomp, size=n)
+    x = centres[lab] + sigma * rng.standard_normal((n, dim)).astype(np.float32)
+    return x.astype(np.float32), centres
+
+
+def mixture_decay(n, dim, ncomp, sigma, seed, centres=None):
+    rng = np.random.default_rng(seed)
+    s = (np.arange(1, dim + 1, dtype=np.float64) ** -0.25).astype(np.float32)
+    if centres is None:
+        centres = (rng.standard_normal((ncomp, dim)).astype(np.float32) * s)
+    lab = rng.integers(0, ncomp, size=n)
+    x = centres[lab] + sigma * s * rng.standard_normal((n, dim)).astype(np.float32)
+    return x.astype(np.float32), centres
+
+
+def powerlaw_basis(dim, alpha, seed):
+    rng = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(rng.standard_normal((dim, dim)))
+    lam = np.arange(1, dim + 1, dtype=np.float64) ** -alpha
+    lam = lam / lam.sum() * dim
+    return U.astype(np.float32), np.sqrt(lam).astype(np.float32)
+
+
+def mixture_powerlaw(n, dim, ncomp, sigma, seed, alpha=1.0, basis=None, centres=None):
+    rng = np.random.default_rng(seed)
+    if basis is None:
+        basis = powerlaw_basis(dim, alpha, seed + 1000)
+    U, sl = basis
+    if centres is None:
+        centres = (rng.standard_normal((ncomp, dim)).astype(np.float32) * sl) @ U.T
+    lab = rng.integers(0, ncomp, size=n)
+    x = centres[lab] + sigma * ((rng.standard_normal((n, dim)).astype(np.float32) * sl) @ U.T)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    return x.astype(np.float32), (basis, centres)
+
+
+def kmeans(x, k, iters, seed, chunk=65536):
+    """Lloyd k-means (numpy): init from k distinct sampled points."""
+    rng = np.random.default_rng(seed)
+    n = x.shape[0]
+    c = x[rng.choice(n, size=k, replace=False)].copy()
+    assign = np.zeros(n, np.int64)
+    for _ in range(iters):
+        cn = (c * c).sum(1)
+        for s in range(0, n, chunk):
+            xs = x[s:s + chunk]
+            d = cn[None, :] - 2.0 * (xs @ c.T)
+            assign[s:s + chunk] = d.argmin(1)
+        sums = np.zeros_like(c, dtype=np.float64)
+        np.add.at(sums, assign, x)
+        cnt = np.bincount(assign, minlength=k)
+        nz = cnt > 0
+        c[nz] = (sums[nz] / cnt[nz, None]).astype(np.float32)
+    cn = (c * c).sum(1)
+    for s in range(0, n, chunk):
+        xs = x[s:s + chunk]
+        assign[s:s + chunk] = (cn[None, :] - 2.0 * (xs @ c.T)).argmin(1)
+    return c.astype(np.float32), assign
+
+
+# ---------------- GPU (torch) variants for the 1M-vector configurations ----------------
+
+def torch_generator(seed, device):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    return g
+
+
+def mixture_powerlaw_torch(n, dim, ncomp, sigma, seed, alpha=1.0, device="cuda", centres=None,
+                           basis=None):
+    import torch
+    g = torch_generator(seed, device)
+    if basis is None:
+        gb = torch_generator(seed + 1000, device)
+        A = torch.randn(dim, dim, generator=gb, device=device, dtype=torch.float32)
+        U, _ = torch.linalg.qr(A)
+        lam = torch.arange(1, dim + 1, device=device, dtype=torch.float64) ** -alpha
+        lam = lam / lam.sum() * dim
+        basis = (U.contiguous(), lam.sqrt().float())
+    U, sl = basis
+    if centres is None:
+        centres = (torch.randn(ncomp, dim, generator=g, device=device) * sl) @ U.T
+    lab = torch.randint(0, ncomp, (n,), generator=g, device=device)
+    x = torch.empty(n, dim, device=device, dtype=torch.float32)
+    step = 131072
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        z = torch.randn(e - s, dim, generator=g, device=device) * sl
+        xs = centres[lab[s:e]] + sigma * (z @ U.T)
+        x[s:e] = xs / xs.norm(dim=1, keepdim=True)
+    return x, (basis, centres)
+
+
+def mixture_decay_torch(n, dim, ncomp, sigma, seed, device="cuda", centres=None):
+    import torch
+    g = torch_generator(seed, device)
+    s = torch.arange(1, dim + 1, device=device, dtype=torch.float32) ** -0.25
+    if centres is None:
+        centres = torch.randn(ncomp, dim, generator=g, device=device) * s
+    lab = torch.randint(0, ncomp, (n,), generator=g, device=device)
+    x = torch.empty(n, dim, device=device, dtype=torch.float32)
+    step = 131072
+    for st in range(0, n, step):
+        e = min(n, st + step)
+        x[st:e] = centres[lab[st:e]] + sigma * s * torch.randn(e - st, dim, generator=g,
+                                                               device=device)
+    return x, centres
+
+
+def kmeans_torch(x, k, iters, seed, chunk=65536):
+    """Lloyd k-means on the GPU (assignment by argmin of ||c||^2 - 2 x.c)."""
+    import torch
+    g = torch_generator(seed, x.device)
+    n = x.shape[0]
+    perm = torch.randperm(n, generator=g, device=x.device)[:k]
+    c = x[perm].clone()
+    assign = torch.empty(n, dtype=torch.int64, device=x.device)
+
+    def assign_all():
+        cn = (c * c).sum(1)
+        for s in range(0, n, chunk):
+            xs = x[s:s + chunk]
+            assign[s:s + chunk] = (cn[None, :] - 2.0 * (xs @ c.T)).argmin(1)
+
+    for _ in range(iters):
+        assign_all()
+        sums = torch.zeros_like(c, dtype=torch.float64)
+        sums.index_add_(0, assign, x.double())
+        cnt = torch.bincount(assign, minlength=k)
+        nz = cnt > 0
+        c[nz] = (sums[nz] / cnt[nz, None]).float()
+    assign_all()
+    return c, assign
+
+
+def exact_knn_torch(base, queries, k, chunk_q=1024, chunk_n=262144):
+    """Exact L2 top-k in FP32 on the GPU (ground truth), ids ascending by distance."""
+    import torch
+    nq = queries.shape[0]
+    bn = (base * base).sum(1)
+    out_i = torch.empty(nq, k, dtype=torch.int64, device=base.device)
+    out_d = torch.empty(nq, k, dtype=torch.float32, device=base.device)
+    for s in range(0, nq, chunk_q):
+        q = queries[s:s + chunk_q]
+        qn = (q * q).sum(1, keepdim=True)
+        best_d = None
+        best_i = None
+        for t in range(0, base.shape[0], chunk_n):
+            d = qn + bn[None, t:t + chunk_n] - 2.0 * (q @ base[t:t + chunk_n].T)
+            dk, ik = torch.topk(d, min(k, d.shape[1]), dim=1, largest=False)
+            ik = ik + t
+            if best_d is None:
+                best_d, best_i = dk, ik
+            else:
+                cd = torch.cat([best_d, dk], 1)
+                ci = torch.cat([best_i, ik], 1)
+                best_d, sel = torch.topk(cd, k, dim=1, largest=False)
+                best_i = torch.gather(ci, 1, sel)
+        order = torch.argsort(best_d, dim=1)
+        out_d[s:s + chunk_q] = torch.gather(best_d, 1, order)
+        out_i[s:s + chunk_q] = torch.gather(best_i, 1, order)
+    return out_i, out_d
+
+
+def exact_knn_np(base, queries, k):
+    """Exact L2 top-k in FP64 on the CPU (spot checks)."""
+    b = base.astype(np.float64)
+    q = queries.astype(np.float64)
+    d = (q * q).sum(1)[:, None] + (b * b).sum(1)[None, :] - 2.0 * q @ b.T
+    idx = np.argsort(d, axis=1, kind="stable")[:, :k]
+    return idx, np.take_along_axis(d, idx, 1)
+
+
+def recall_at_k(result_ids, gt_ids, k):
+    """test_search.cpp:113-123: |result[0:k] intersect gt[0:k]| / k, averaged."""
+    tot = 0
+    for r, g in zip(np.asarray(result_ids), np.asarray(gt_ids)):
+        tot += len(set(r[:k].tolist()) & set(g[:k].tolist()))
+    return tot / (len(result_ids) * k)

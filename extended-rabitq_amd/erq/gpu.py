@@ -1,0 +1,147 @@
+"""Host-side mirror of the reference's IVF class (inc/index/IVF.hpp:26-91) over liberq.so.
+
+``IVF.load(path)`` / ``IVF.search(query, k, nprobe)`` / ``padded_dim()`` / ``k()`` keep the
+reference's names and argument meaning; the ``*_dev`` methods take torch CUDA tensors
+(device memory and streams only -- every computation runs in liberq.so's HIP kernels).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class IVF:
+    def __init__(self, handle):
+        self._h = handle
+
+    # ---- construction (IVF::load, IVF.hpp:254-293) ----
+    @staticmethod
+    def load(path):
+        h = ctypes.c_void_p()
+        check(_lib.lib().erq_index_load(str(path).encode(), ctypes.byref(h)))
+        return IVF(h)
+
+    @staticmethod
+    def from_index(idx):
+        """Upload an in-memory erq.IVFIndex (same arrays IVF::load reads)."""
+        idx.validate()
+        arrs = [np.ascontiguousarray(idx.cluster_sizes, dtype=np.uint64),
+                np.ascontiguousarray(idx.P, dtype=np.float32),
+                np.ascontiguousarray(idx.centroids, dtype=np.float32),
+                np.ascontiguousarray(idx.short_data, dtype=np.uint8),
+                np.ascontiguousarray(idx.long_code, dtype=np.uint8),
+                np.ascontiguousarray(idx.ex_factor, dtype=np.float32),
+                np.ascontiguousarray(idx.ids, dtype=np.uint32)]
+        h = ctypes.c_void_p()
+        check(_lib.lib().erq_index_create(idx.N, idx.DIM, idx.K, idx.EX,
+                                          *[_ptr(a) for a in arrs], ctypes.byref(h)))
+        return IVF(h)
+
+    def close(self):
+        if self._h:
+            _lib.lib().erq_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- header (IVF.hpp:86-90) ----
+    def padded_dim(self):
+        return _lib.lib().erq_padded_dim(self._h)
+
+    def k(self):
+        return _lib.lib().erq_num_clusters(self._h)
+
+    def dim(self):
+        return _lib.lib().erq_dim(self._h)
+
+    def ex_bits(self):
+        return _lib.lib().erq_ex_bits(self._h)
+
+    # ---- host-pointer path (Rotator::rotate, IVF::search) ----
+    def rotate(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        if q.ndim == 1:
+            q = q[None]
+        out = np.empty((q.shape[0], self.padded_dim()), np.float32)
+        check(_lib.lib().erq_rotate(self._h, _ptr(q), q.shape[0], _ptr(out)))
+        return out
+
+    def search(self, query, k, nprobe, with_dists=False):
+        """IVF::search(query, data, k, nprobe, results) on one rotated query (D floats)."""
+        q = np.ascontiguousarray(query, dtype=np.float32).reshape(-1)
+        if q.size != self.padded_dim():
+            raise ValueError("query must be the rotated, padded D-vector")
+        ids = np.empty(k, np.uint32)
+        d = np.empty(k, np.float32)
+        check(_lib.lib().erq_search(self._h, _ptr(q), k, nprobe, _ptr(ids), _ptr(d)))
+        return (ids, d) if with_dists else ids
+
+    def search_batch(self, q_rot, k, nprobe):
+        q = np.ascontiguousarray(q_rot, dtype=np.float32)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        check(_lib.lib().erq_search_batch(self._h, _ptr(q), nq, k, nprobe, _ptr(ids), _ptr(d)))
+        return ids, d
+
+    # ---- device path (torch CUDA tensors; enqueued on torch's current stream) ----
+    def rotate_dev(self, q):
+        import torch
+        out = torch.empty((q.shape[0], self.padded_dim()), dtype=torch.float32, device=q.device)
+        check(_lib.lib().erq_rotate_dev(self._h, _ptr(q.contiguous()), q.shape[0], _ptr(out),
+                                        _stream()))
+        return out
+
+    def centroid_dist_dev(self, q_rot):
+        import torch
+        out = torch.empty((q_rot.shape[0], self.k()), dtype=torch.float32, device=q_rot.device)
+        check(_lib.lib().erq_centroid_dist_dev(self._h, _ptr(q_rot), q_rot.shape[0], _ptr(out),
+                                               _stream()))
+        return out
+
+    def probe_dev(self, q_rot, nprobe):
+        import torch
+        nq = q_rot.shape[0]
+        pid = torch.empty((nq, nprobe), dtype=torch.int32, device=q_rot.device)
+        pd = torch.empty((nq, nprobe), dtype=torch.float32, device=q_rot.device)
+        check(_lib.lib().erq_probe_dev(self._h, _ptr(q_rot), nq, nprobe, _ptr(pid), _ptr(pd),
+                                       _stream()))
+        return pid, pd
+
+    def scan_dev(self, q_rot, k, probe_off, probe_ids, probe_d, stats=None):
+        import torch
+        nq = q_rot.shape[0]
+        ids = torch.empty((nq, k), dtype=torch.int32, device=q_rot.device)
+        d = torch.empty((nq, k), dtype=torch.float32, device=q_rot.device)
+        check(_lib.lib().erq_scan_dev(self._h, _ptr(q_rot), nq, k, _ptr(probe_off),
+                                      _ptr(probe_ids), _ptr(probe_d), _ptr(ids), _ptr(d),
+                                      _ptr(stats), _stream()))
+        return ids, d
+
+    def search_dev(self, q_rot, k, nprobe, stats=None):
+        import torch
+        nq = q_rot.shape[0]
+        ids = torch.empty((nq, k), dtype=torch.int32, device=q_rot.device)
+        d = torch.empty((nq, k), dtype=torch.float32, device=q_rot.device)
+        check(_lib.lib().erq_search_dev(self._h, _ptr(q_rot), nq, k, nprobe, _ptr(ids), _ptr(d),
+                                        _ptr(stats), _stream()))
+        return ids, d

@@ -1,0 +1,111 @@
+/*
+ * erq.h -- C-ABI of liberq.so, the MI355X (gfx950) Extended-RaBitQ IVF search library.
+ *
+ * The reference exposes a C++ class, IVF (inc/index/IVF.hpp:26-91); these entry points
+ * are what a binding of that class's search path needs: load an HA index, rotate
+ * queries, search.  Plain pointers and sizes only; every call returns ERQ_OK or an
+ * error code, with a message in erq_last_error().  No exceptions cross the ABI.
+ *
+ * Pointer conventions: functions without a _dev suffix take host pointers; _dev
+ * functions take device pointers and enqueue on `stream` (a hipStream_t; NULL means
+ * the handle's own stream) and return without synchronising.
+ * Thread safety: calls on one handle are serialised by a per-handle mutex.
+ */
+#ifndef ERQ_H
+#define ERQ_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ERQ_OK = 0,
+    ERQ_EINVAL = 1,      /* bad argument (k == 0, nprobe > K, null pointer, ...) */
+    ERQ_EIO = 2,         /* file missing / unreadable / truncated */
+    ERQ_EFORMAT = 3,     /* header or sizes inconsistent */
+    ERQ_EHIP = 4,        /* HIP runtime error */
+    ERQ_ENOMEM = 5,      /* device allocation failed */
+    ERQ_EUNSUPPORTED = 6 /* shape outside what the kernels handle */
+};
+
+typedef struct erq_index erq_index;
+
+/* Message of the last error on the calling thread ("" if none). */
+const char* erq_last_error(void);
+
+/* Replaces IVF::load (IVF.hpp:254-293): reads an HA-format index file
+ * (HIGH_ACC_FAST_SCAN layout, 1 factor per vector) and uploads it to device 0
+ * (or the current HIP device). */
+int erq_index_load(const char* path, erq_index** out);
+
+/* The same arrays IVF::load reads (IVF.hpp:261-286), from host memory. */
+int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint64_t* cluster_sizes,
+                     const float* P, const float* centroids, const uint8_t* short_data,
+                     const uint8_t* long_code, const float* ex_factor, const uint32_t* ids,
+                     erq_index** out);
+
+/* Replaces IVF::~IVF (IVF.hpp:107). */
+void erq_index_free(erq_index* h);
+
+/* IVF::padded_dim (IVF.hpp:86), IVF::k (IVF.hpp:90) and the other header fields. */
+size_t erq_padded_dim(const erq_index* h);
+size_t erq_num_clusters(const erq_index* h);
+size_t erq_dim(const erq_index* h);
+size_t erq_num_vectors(const erq_index* h);
+size_t erq_ex_bits(const erq_index* h);
+
+/* Replaces Rotator::rotate on zero-padded queries (Rotator.hpp:60-63 as used by
+ * test_search.cpp:83-92): q is nq x DIM, q_rot is nq x D (host pointers). */
+int erq_rotate(erq_index* h, const float* q, size_t nq, float* q_rot);
+
+/* Replaces IVF::search (IVF.hpp:84, :295-324): q_rot is one rotated, padded query of D
+ * floats; writes k ids in ascending estimated distance (0 past the pool size), and k
+ * distances if dists != NULL (FLT_MAX past the pool size).  Host pointers. */
+int erq_search(erq_index* h, const float* q_rot, size_t k, size_t nprobe, uint32_t* results,
+               float* dists);
+
+/* IVF::search over nq queries (host pointers), fixed nprobe. */
+int erq_search_batch(erq_index* h, const float* q_rot, size_t nq, size_t k, size_t nprobe,
+                     uint32_t* results, float* dists);
+
+/* ---- device-pointer stages of the same path (enqueue on `stream`) ---- */
+
+/* Rotator::rotate: q_dev nq x DIM -> q_rot_dev nq x D (FP32 MFMA GEMM). */
+int erq_rotate_dev(erq_index* h, const float* q_dev, size_t nq, float* q_rot_dev, void* stream);
+
+/* The query-to-centroid contraction on MFMA: out[i][c] = ||q_i||^2 + ||c||^2 - 2 q_i.c
+ * (nq x K), the ranking key used before the exact distances of Initializer.hpp:59-62. */
+int erq_centroid_dist_dev(erq_index* h, const float* q_rot_dev, size_t nq, float* out_dev,
+                          void* stream);
+
+/* FlatInitializer::centroids_distances (Initializer.hpp:55-68): for each query the
+ * nprobe nearest clusters in ascending squared distance, distances computed exactly as
+ * the reference's L2Sqr16 (space.hpp:18-39).  Outputs nq x nprobe. */
+int erq_probe_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
+                  uint32_t* probe_ids_dev, float* probe_sqr_y_dev, void* stream);
+
+/* The per-query cluster loop of IVF::search (IVF.hpp:313-323) with HASearcher's scan
+ * (HASearcher.hpp:82-237): query i scans clusters probe_ids[off[i] .. off[i+1]) in that
+ * order.  Results nq x k.  stats_dev (nullable) accumulates 4 uint64 counters:
+ * blocks scanned, vectors scanned, survivors re-ranked, probes processed. */
+int erq_scan_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                 const uint32_t* probe_off_dev, const uint32_t* probe_ids_dev,
+                 const float* probe_sqr_y_dev, uint32_t* ids_dev, float* dists_dev,
+                 uint64_t* stats_dev, void* stream);
+
+/* erq_probe_dev followed by erq_scan_dev with a fixed nprobe. */
+int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
+                   uint32_t* ids_dev, float* dists_dev, uint64_t* stats_dev, void* stream);
+
+/* Device pointers of the resident index arrays (for tools that read the HBM layout).
+ * which: 0 short blocks, 1 long codes, 2 ex factors, 3 ids, 4 rotated centroids,
+ * 5 rotation P, 6 cluster sizes (u32), 7 first block per cluster (u32),
+ * 8 first vector per cluster (u32). */
+const void* erq_device_array(const erq_index* h, int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ERQ_H */

@@ -1,0 +1,90 @@
+/*
+ * erq_oracle.h -- CPU restatement of the Extended-RaBitQ "high-accuracy FastScan"
+ * IVF search path (HASearcher), used ONLY as the checker of the HIP implementation.
+ *
+ * TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library.  The product path never calls it.
+ *
+ * Floating point follows the reference source literally (IEEE single/double ops in
+ * source order, FMA only where the source writes an FMA intrinsic, C promotion
+ * rules for mixed float/double expressions).  Reductions reproduce
+ * _mm512_reduce_add_ps's xor-8/4/2/1 butterfly over 16 lane-strided partial sums.
+ *
+ * Pinning status of each function is stated in oracle/README.md.
+ */
+#ifndef ERQ_ORACLE_H
+#define ERQ_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* In-memory view of an HA-format IVF index (inc/index/IVF.hpp:28-40, :254-293). */
+typedef struct {
+    size_t N, DIM, D, K, EX;
+    const uint64_t* cluster_sizes; /* K */
+    const float* centroids;        /* K x D rotated centroids */
+    const uint8_t* short_data;     /* sum_k ceil(n_k/32) blocks of (4D + 128) bytes */
+    const uint8_t* long_code;      /* N x D*EX/8 */
+    const float* ex_factor;        /* N xipnorm */
+    const uint32_t* ids;           /* N */
+    const uint64_t* block_start;   /* K+1: first block of cluster k (prefix sums of ceil(n/32)) */
+    const uint64_t* vec_start;     /* K+1: first vector of cluster k (prefix sums of n) */
+} orc_index;
+
+float orc_reduce16(const float p[16]);
+float orc_l2sqr16(const float* x, const float* y, size_t D);
+float orc_normalize_query(float* unit_q, const float* q, const float* c, float norm, size_t D);
+void orc_data_range(const float* q, size_t D, float* vl, float* vr);
+void orc_quantize_query(int16_t* out, const float* u, size_t D, float* delta);
+int32_t orc_pack_lut(uint16_t* lut, const int16_t* q, size_t D);
+uint32_t orc_fastscan_block(const uint8_t* codes, const uint16_t* lut, const float* onorm,
+                            float sumq, float qnorm, float delta, int32_t shift,
+                            float one_over_sqrtD, float distk, size_t D,
+                            float* ipq_out, int32_t* sum_out, float* lower_out);
+int orc_ex_decode(const uint8_t* code, size_t D, int ex_bits, uint8_t* out);
+float orc_ip_ex(const float* u, const uint8_t* code, size_t D, int ex_bits);
+float orc_one_over_sqrtD(size_t D);
+
+/* ResultPool (inc/index/Pool.hpp:10-50). */
+typedef struct {
+    uint32_t* ids;
+    float* dist;
+    size_t size, cap;
+} orc_pool;
+void orc_pool_init(orc_pool* p, uint32_t* ids_buf, float* dist_buf, size_t cap);
+void orc_pool_insert(orc_pool* p, uint32_t id, float dist);
+float orc_pool_distk(const orc_pool* p);
+
+/* Centroid ranking (inc/index/Initializer.hpp:55-68). Ties broken by id. */
+void orc_centroid_topn(const float* q, const float* centroids, size_t K, size_t D, size_t nprobe,
+                       uint32_t* out_ids, float* out_dist);
+
+/* Statistics of one search, for tests and the CPU baseline. */
+typedef struct {
+    uint64_t blocks, vectors, survivors, inserts;
+} orc_stats;
+
+/* IVF::search (inc/index/IVF.hpp:295-324) on a rotated query.  Writes k ids
+ * (0-filled past the pool size) and, if dists != NULL, k distances (FLT_MAX-filled). */
+void orc_ivf_search(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
+                    uint32_t* ids, float* dists, orc_stats* st);
+/* Same, with an explicit probe list (cluster ids + squared distances) already ranked. */
+void orc_ivf_search_probes(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
+                           const uint32_t* probe_ids, const float* probe_sqr_y,
+                           uint32_t* ids, float* dists, orc_stats* st);
+/* Batch over nq queries with OpenMP (threads<=0: all). */
+void orc_ivf_search_batch(const orc_index* idx, const float* q_rot, size_t nq, size_t k,
+                          const uint32_t* nprobe_per_q, uint32_t* ids, float* dists, int threads);
+
+/* Format helpers the reference's builder defines (pack_codes.hpp:33-98,
+ * Quantizer.hpp:371-384, :491-631). */
+void orc_pack_block_codes(const uint8_t* bits, size_t n, size_t D, uint8_t* block_codes);
+int orc_ex_encode(const uint8_t* raw, size_t D, int ex_bits, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

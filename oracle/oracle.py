@@ -1,0 +1,248 @@
+"""Python handle on the CPU oracle (liberq_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg.  The product (extended-rabitq_amd) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+c_size = ctypes.c_size_t
+c_vp = ctypes.c_void_p
+
+
+class OrcIndex(ctypes.Structure):
+    _fields_ = [("N", c_size), ("DIM", c_size), ("D", c_size), ("K", c_size), ("EX", c_size),
+                ("cluster_sizes", c_vp), ("centroids", c_vp), ("short_data", c_vp),
+                ("long_code", c_vp), ("ex_factor", c_vp), ("ids", c_vp),
+                ("block_start", c_vp), ("vec_start", c_vp)]
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [("blocks", ctypes.c_uint64), ("vectors", ctypes.c_uint64),
+                ("survivors", ctypes.c_uint64), ("inserts", ctypes.c_uint64)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liberq_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: make -C oracle")
+        L = ctypes.CDLL(path)
+        f = ctypes.c_float
+        L.orc_reduce16.restype = f
+        L.orc_reduce16.argtypes = [c_vp]
+        L.orc_l2sqr16.restype = f
+        L.orc_l2sqr16.argtypes = [c_vp, c_vp, c_size]
+        L.orc_normalize_query.restype = f
+        L.orc_normalize_query.argtypes = [c_vp, c_vp, c_vp, f, c_size]
+        L.orc_quantize_query.restype = None
+        L.orc_quantize_query.argtypes = [c_vp, c_vp, c_size, c_vp]
+        L.orc_pack_lut.restype = ctypes.c_int32
+        L.orc_pack_lut.argtypes = [c_vp, c_vp, c_size]
+        L.orc_fastscan_block.restype = ctypes.c_uint32
+        L.orc_fastscan_block.argtypes = [c_vp, c_vp, c_vp, f, f, f, ctypes.c_int32, f, f, c_size,
+                                         c_vp, c_vp, c_vp]
+        L.orc_ex_decode.restype = ctypes.c_int
+        L.orc_ex_decode.argtypes = [c_vp, c_size, ctypes.c_int, c_vp]
+        L.orc_ex_encode.restype = ctypes.c_int
+        L.orc_ex_encode.argtypes = [c_vp, c_size, ctypes.c_int, c_vp]
+        L.orc_ip_ex.restype = f
+        L.orc_ip_ex.argtypes = [c_vp, c_vp, c_size, ctypes.c_int]
+        L.orc_one_over_sqrtD.restype = f
+        L.orc_one_over_sqrtD.argtypes = [c_size]
+        L.orc_pool_init.restype = None
+        L.orc_pool_insert.restype = None
+        L.orc_pool_distk.restype = f
+        L.orc_centroid_topn.restype = None
+        L.orc_centroid_topn.argtypes = [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp]
+        L.orc_ivf_search.restype = None
+        L.orc_ivf_search.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size, c_vp, c_vp,
+                                     ctypes.POINTER(OrcStats)]
+        L.orc_ivf_search_probes.restype = None
+        L.orc_ivf_search_probes.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size, c_vp,
+                                            c_vp, c_vp, c_vp, ctypes.POINTER(OrcStats)]
+        L.orc_ivf_search_batch.restype = None
+        L.orc_ivf_search_batch.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size, c_vp,
+                                           c_vp, c_vp, ctypes.c_int]
+        L.orc_pack_block_codes.restype = None
+        L.orc_pack_block_codes.argtypes = [c_vp, c_size, c_size, c_vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(c_vp)
+
+
+def reduce16(p):
+    p = np.ascontiguousarray(p, np.float32)
+    return float(lib().orc_reduce16(_p(p)))
+
+
+def l2sqr16(x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return float(lib().orc_l2sqr16(_p(x), _p(y), x.size))
+
+
+def normalize_query(q, c, norm):
+    q = np.ascontiguousarray(q, np.float32)
+    c = np.ascontiguousarray(c, np.float32)
+    u = np.empty_like(q)
+    s = lib().orc_normalize_query(_p(u), _p(q), _p(c), ctypes.c_float(norm), q.size)
+    return u, float(s)
+
+
+def quantize_query(u):
+    u = np.ascontiguousarray(u, np.float32)
+    out = np.empty(u.size, np.int16)
+    delta = ctypes.c_float()
+    lib().orc_quantize_query(_p(out), _p(u), u.size, ctypes.byref(delta))
+    return out, float(delta.value)
+
+
+def pack_lut(q16):
+    q16 = np.ascontiguousarray(q16, np.int16)
+    lut = np.empty(q16.size * 4, np.uint16)
+    shift = lib().orc_pack_lut(_p(lut), _p(q16), q16.size)
+    return lut.reshape(-1, 16), int(shift)
+
+
+def fastscan_block(codes, lut, onorm, sumq, qnorm, delta, shift, oosd, distk, D):
+    codes = np.ascontiguousarray(codes, np.uint8)
+    lut = np.ascontiguousarray(lut, np.uint16)
+    onorm = np.ascontiguousarray(onorm, np.float32)
+    ipq = np.empty(32, np.float32)
+    sums = np.empty(32, np.int32)
+    lower = np.empty(32, np.float32)
+    f = ctypes.c_float
+    mask = lib().orc_fastscan_block(_p(codes), _p(lut), _p(onorm), f(sumq), f(qnorm), f(delta),
+                                    ctypes.c_int32(shift), f(oosd), f(distk), D, _p(ipq),
+                                    _p(sums), _p(lower))
+    return int(mask), ipq, sums, lower
+
+
+def ex_decode(code, D, ex):
+    code = np.ascontiguousarray(code, np.uint8)
+    out = np.empty(D, np.uint8)
+    if lib().orc_ex_decode(_p(code), D, ex, _p(out)) != 0:
+        raise ValueError("bad ex_bits")
+    return out
+
+
+def ex_encode(raw, ex):
+    raw = np.ascontiguousarray(raw, np.uint8)
+    out = np.zeros(raw.size * ex // 8, np.uint8)
+    if lib().orc_ex_encode(_p(raw), raw.size, ex, _p(out)) != 0:
+        raise ValueError("bad ex_bits")
+    return out
+
+
+def ip_ex(u, code, ex):
+    u = np.ascontiguousarray(u, np.float32)
+    code = np.ascontiguousarray(code, np.uint8)
+    return float(lib().orc_ip_ex(_p(u), _p(code), u.size, ex))
+
+
+def one_over_sqrtD(D):
+    return float(lib().orc_one_over_sqrtD(D))
+
+
+def pack_block_codes(bits, D):
+    bits = np.ascontiguousarray(bits, np.uint8)
+    out = np.empty(4 * D, np.uint8)
+    lib().orc_pack_block_codes(_p(bits), bits.shape[0], D, _p(out))
+    return out
+
+
+class Pool:
+    """ResultPool (Pool.hpp:10-50) through the oracle."""
+
+    def __init__(self, cap):
+        self.ids = np.zeros(cap + 1, np.uint32)
+        self.dist = np.zeros(cap + 1, np.float32)
+        self.st = (ctypes.c_uint64 * 4)()
+        L = lib()
+        L.orc_pool_init.argtypes = [c_vp, c_vp, c_vp, c_size]
+        L.orc_pool_insert.argtypes = [c_vp, ctypes.c_uint32, ctypes.c_float]
+        L.orc_pool_distk.argtypes = [c_vp]
+        L.orc_pool_init(ctypes.byref(self.st), _p(self.ids), _p(self.dist), cap)
+
+    def insert(self, pid, d):
+        lib().orc_pool_insert(ctypes.byref(self.st), pid, ctypes.c_float(d))
+
+    def distk(self):
+        return float(lib().orc_pool_distk(ctypes.byref(self.st)))
+
+    @property
+    def size(self):
+        return int(self.st[2])
+
+    def contents(self):
+        n = self.size
+        return self.ids[:n].copy(), self.dist[:n].copy()
+
+
+class OracleIndex:
+    """Holds numpy arrays of an erq.IVFIndex and the ctypes view the oracle reads."""
+
+    def __init__(self, idx):
+        self.idx = idx
+        self.keep = dict(
+            cs=np.ascontiguousarray(idx.cluster_sizes, np.uint64),
+            C=np.ascontiguousarray(idx.centroids, np.float32),
+            sd=np.ascontiguousarray(idx.short_data, np.uint8),
+            lc=np.ascontiguousarray(idx.long_code, np.uint8),
+            ef=np.ascontiguousarray(idx.ex_factor, np.float32),
+            ids=np.ascontiguousarray(idx.ids, np.uint32),
+            bs=np.ascontiguousarray(idx.block_start(), np.uint64),
+            vs=np.ascontiguousarray(idx.vec_start(), np.uint64))
+        k = self.keep
+        self.view = OrcIndex(idx.N, idx.DIM, idx.D, idx.K, idx.EX, _p(k["cs"]), _p(k["C"]),
+                             _p(k["sd"]), _p(k["lc"]), _p(k["ef"]), _p(k["ids"]), _p(k["bs"]),
+                             _p(k["vs"]))
+
+    def search(self, q_rot, k, nprobe, stats=False):
+        q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)
+        ids = np.empty(k, np.uint32)
+        d = np.empty(k, np.float32)
+        st = OrcStats()
+        lib().orc_ivf_search(ctypes.byref(self.view), _p(q), k, nprobe, _p(ids), _p(d),
+                             ctypes.byref(st))
+        if stats:
+            return ids, d, (st.blocks, st.vectors, st.survivors)
+        return ids, d
+
+    def search_probes(self, q_rot, k, probe_ids, probe_d):
+        q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)
+        pid = np.ascontiguousarray(probe_ids, np.uint32)
+        pd = np.ascontiguousarray(probe_d, np.float32)
+        ids = np.empty(k, np.uint32)
+        d = np.empty(k, np.float32)
+        lib().orc_ivf_search_probes(ctypes.byref(self.view), _p(q), k, pid.size, _p(pid), _p(pd),
+                                    _p(ids), _p(d), None)
+        return ids, d
+
+    def search_batch(self, q_rot, k, nprobe, threads=0):
+        q = np.ascontiguousarray(q_rot, np.float32)
+        nq = q.shape[0]
+        npq = np.ascontiguousarray(np.broadcast_to(np.asarray(nprobe, np.uint32), (nq,)))
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        lib().orc_ivf_search_batch(ctypes.byref(self.view), _p(q), nq, k, _p(npq), _p(ids),
+                                   _p(d), threads)
+        return ids, d
+
+    def centroid_topn(self, q_rot, nprobe):
+        q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)
+        ids = np.empty(nprobe, np.uint32)
+        d = np.empty(nprobe, np.float32)
+        lib().orc_centroid_topn(_p(q), _p(self.keep["C"]), self.idx.K, self.idx.D, nprobe,
+                                _p(ids), _p(d))
+        return ids, d
