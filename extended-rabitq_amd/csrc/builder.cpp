@@ -39,6 +39,13 @@ float quantize_magnitudes(const float* o, size_t D, int ex_bits, uint8_t* code,
     double max_o = -1;
     for (size_t i = 0; i < D; i++)
         if (o[i] > max_o) max_o = o[i];
+    if (!(max_o > 0)) {
+        // zero residual (a vector equal to its centroid, e.g. a singleton cluster): the
+        // reference divides by max_o = 0 here; every magnitude code is 0 and the factor
+        // takes the reference's non-finite fallback (Quantizer.hpp:210-213).
+        std::memset(code, 0, D);
+        return 1.f;
+    }
     double t_start = (double)(top / 3) / max_o;
     double t_end = (double)(top + n_enum) / max_o;
 
@@ -156,6 +163,10 @@ int erqb_quantize_cluster(const float* r, const float* onorm, size_t n, size_t D
     std::vector<int> cur(D);
     std::vector<std::pair<double, size_t>> heap_buf;
     heap_buf.reserve(D);
+    for (size_t v = 0; v < n * D; v++)
+        if (!std::isfinite(r[v])) return -2;
+    for (size_t v = 0; v < n; v++)
+        if (!std::isfinite(onorm[v])) return -2;
     for (size_t v = 0; v < n; v++) {
         const float* x = r + v * D;
         double ss = 0;

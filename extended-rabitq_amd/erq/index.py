@@ -163,12 +163,14 @@ def build_index(data, centroids, assign, bits, P=None, seed=3, matmul=None, thre
     short = np.zeros(nblocks * (4 * D + 128), np.uint8)
     longc = np.zeros(N * D * EX // 8, np.uint8)
     xip = np.zeros(N, np.float32)
-    _lib.builder().erqb_quantize_all(
+    rc = _lib.builder().erqb_quantize_all(
         resid.ctypes.data_as(ctypes.c_void_p), onorm.ctypes.data_as(ctypes.c_void_p),
         sizes.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(K), ctypes.c_size_t(D),
         ctypes.c_int(EX), short.ctypes.data_as(ctypes.c_void_p),
         longc.ctypes.data_as(ctypes.c_void_p), xip.ctypes.data_as(ctypes.c_void_p),
         ctypes.c_int(threads))
+    if rc != 0:
+        raise ValueError("index build failed: non-finite vectors or unsupported shape")
     return IVFIndex(N, DIM, K, EX, sizes, P, crot, short, longc, xip,
                     order.astype(np.uint32))
 
