@@ -1,0 +1,50 @@
+// erq_batch.h -- arguments of the cluster-major batched search (erq_batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace erq {
+
+struct BatchArgs {
+    const float* q_rot;        // nq x D
+    const uint32_t* probe_off; // nq + 1 (CSR over probe slots)
+    const float* probe_sqr_y;  // per slot: squared distance to the cluster centroid
+    const float* centroids;
+    const uint8_t* short_data; // HA blocks (factors are read from here)
+    const uint8_t* bp;         // bit-plane copy of the 1-bit codes
+    const uint8_t* long_code;
+    const float* ex_factor;
+    const uint32_t* ids;
+    const uint32_t* csize;
+    const uint32_t* bstart;
+    const uint32_t* vstart;
+    const uint32_t* items;     // 2 x n_items: cluster, (first entry << 4) | count
+    const uint32_t* entries;   // probe slots grouped by cluster
+    uint32_t* n_items;         // [0] items of this round, [1] work counter
+    float* pool_d;             // nq x k, ascending (FLT_MAX padded)
+    uint32_t* cand_cnt;        // nq
+    float* cand_d;             // nq x cand_cap
+    uint32_t* cand_i;          // nq x cand_cap
+    unsigned long long* stats; // 8 counters (see erq.h)
+    int cand_cap;
+    int D, EX, k, nq, nchunk;
+    float one_over_sqrtD;
+};
+
+struct RoundBufs {
+    const uint32_t* probe_ids;
+    uint32_t* counts;  // K
+    uint32_t* cursor;  // K
+    uint32_t* pool_i;  // nq x k
+    int K;
+};
+
+int bp_nchunk(int D);
+size_t batch_lds_bytes(int D);
+hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
+                           hipStream_t s);
+hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s);
+hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1, int grid,
+                        hipStream_t s);
+
+}  // namespace erq

@@ -1,0 +1,510 @@
+// erq_batch.hip -- cluster-major batched ExRaBitQ search on gfx950.
+//
+// Many queries with uneven probe lists are searched together.  Probe ranks are split into
+// rounds ([0,1) [1,4) [4,16) [16,64) ...).  Round 0 is the exact per-query scan
+// (erq_search.hip).  Every later round:
+//   1. buckets the round's (query, rank) pairs by cluster (count / scan / scatter),
+//   2. runs one persistent kernel over (cluster, <= 8 queries) work items, so each
+//      cluster's codes are read once per round for all the queries that probe it,
+//   3. merges each query's new candidates into its top-k pool.
+// Within a round the pruning threshold of a query is fixed (its k-th distance at round
+// start), so the candidate set -- and the merged result -- does not depend on scheduling.
+//
+// The FastScan integer I = <x_b, q> (q: the 14-bit quantized query residual of
+// high_acc_quantize16, space.hpp:695-711) is computed with v_mfma_i32_16x16x64_i8:
+//   * at load, the 1-bit codes are re-laid out per 512-dim chunk as 64 bytes per vector,
+//     byte B bit p = dim 512c + 8B + p ("bit-plane" layout);
+//   * a lane holds 16 code bytes of one vector; bit-plane p of them, masked in place
+//     (one v_and per dword, values 0 or 2^p), is the B operand of MFMA p;
+//   * the A operand holds, for 8 queries x 2 planes (q = 128*hi + lo, lo in [0,127]),
+//     the int8 digits of the dims that bit-plane p of those bytes covers;
+//   * I = sum_p (acc_p >> p) (p = 7 enters with the sign flipped: 0x80 is -128).
+// The integer equals the reference's S + shift whenever its u16 lanes do not wrap
+// (always for D <= 2048).  Lower bounds and EX-bit re-estimates use the same formulas,
+// in the same order, as the per-query kernel.
+#include "erq_batch.h"
+#include "erq_device.h"
+#include "erq_internal.h"
+
+namespace erq {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kGroup = 8;      // queries per work item (16 MFMA rows = 8 x 2 planes)
+constexpr int kThreads = 256;  // 4 waves
+
+// ---------------------------------------------------------------- relayout
+// One thread per output byte: (block, chunk, vector, byte B).
+__global__ void relayout_kernel(const uint8_t* __restrict__ short_data, size_t nblocks, int D,
+                                int nchunk, uint8_t* __restrict__ bp) {
+    const size_t per_block = (size_t)nchunk * 32 * 64;
+    const size_t total = nblocks * per_block;
+    const size_t BB = 4 * (size_t)D + 128;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        size_t blk = i / per_block;
+        int rem = (int)(i % per_block);
+        int c = rem / (32 * 64);
+        int v = (rem / 64) % 32;
+        int B = rem % 64;
+        int d0 = 512 * c + 8 * B;
+        uint32_t out = 0;
+        if (d0 < D) {
+            const uint8_t* codes = short_data + blk * BB;
+            int j = byte_of_vec(v);
+            for (int h = 0; h < 2; h++) {
+                int m = (d0 >> 2) + h;  // codebook: dims 4m..4m+3, dim 4m at nibble bit 3
+                uint8_t b = codes[(m >> 1) * 32 + (m & 1) * 16 + j];
+                uint32_t nib = v < 16 ? (b & 15u) : (b >> 4);
+                for (int k = 0; k < 4; k++) out |= ((nib >> (3 - k)) & 1u) << (4 * h + k);
+            }
+        }
+        bp[i] = (uint8_t)out;
+    }
+}
+
+// ---------------------------------------------------------------- work lists
+__global__ void zero_u32_kernel(uint32_t* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__device__ __forceinline__ void round_range(const uint32_t* off, int q, int r0, int r1,
+                                            uint32_t& a, uint32_t& b) {
+    uint32_t s = off[q], e = off[q + 1];
+    a = min(e, s + (uint32_t)r0);
+    b = min(e, s + (uint32_t)r1);
+}
+
+__global__ void count_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pid,
+                             int nq, int r0, int r1, int K, uint32_t* __restrict__ counts) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    uint32_t a, b;
+    round_range(off, q, r0, r1, a, b);
+    for (uint32_t p = a; p < b; p++) {
+        uint32_t c = pid[p];
+        if (c < (uint32_t)K) atomicAdd(&counts[c], 1u);
+    }
+}
+
+// Single workgroup: exclusive scans of counts and of items per cluster, then items.
+__global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __restrict__ counts,
+                                                          int K, uint32_t* __restrict__ cursor,
+                                                          uint32_t* __restrict__ items,
+                                                          uint32_t* __restrict__ n_items) {
+    __shared__ uint32_t s_cnt[1024], s_it[1024];
+    __shared__ uint32_t carry_c, carry_i;
+    const int t = threadIdx.x;
+    if (t == 0) carry_c = carry_i = 0;
+    __syncthreads();
+    for (int base = 0; base < K; base += 1024) {
+        int c = base + t;
+        uint32_t cnt = c < K ? counts[c] : 0;
+        uint32_t it = (cnt + kGroup - 1) / kGroup;
+        s_cnt[t] = cnt;
+        s_it[t] = it;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+            uint32_t a = t >= o ? s_cnt[t - o] : 0, b = t >= o ? s_it[t - o] : 0;
+            __syncthreads();
+            s_cnt[t] += a;
+            s_it[t] += b;
+            __syncthreads();
+        }
+        uint32_t ex_c = carry_c + s_cnt[t] - cnt, ex_i = carry_i + s_it[t] - it;
+        if (c < K) {
+            cursor[c] = ex_c;
+            for (uint32_t j = 0; j < it; j++) {
+                uint32_t len = min((uint32_t)kGroup, cnt - j * kGroup);
+                items[2 * (ex_i + j)] = (uint32_t)c;
+                items[2 * (ex_i + j) + 1] = ((ex_c + j * kGroup) << 4) | len;
+            }
+        }
+        __syncthreads();
+        if (t == 1023) {
+            carry_c += s_cnt[1023];
+            carry_i += s_it[1023];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        n_items[0] = carry_i;
+        n_items[1] = 0;  // work counter for the persistent kernel
+    }
+}
+
+__global__ void scatter_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pid,
+                               int nq, int r0, int r1, int K, uint32_t* __restrict__ cursor,
+                               uint32_t* __restrict__ entries) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    uint32_t a, b;
+    round_range(off, q, r0, r1, a, b);
+    for (uint32_t p = a; p < b; p++) {
+        uint32_t c = pid[p];
+        if (c >= (uint32_t)K) continue;
+        uint32_t pos = atomicAdd(&cursor[c], 1u);
+        entries[pos] = p;  // the probe slot; its query is found by the consumer
+    }
+}
+
+// query of a probe slot: binary search in the CSR offsets
+__device__ __forceinline__ int query_of_slot(const uint32_t* off, int nq, uint32_t p) {
+    int lo = 0, hi = nq;  // off[lo] <= p < off[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= p) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------- batched scan
+struct QParam {
+    float y, sqr_y, fac, sumq, delta, c1, c2, qs, thr;
+    int q, valid;
+};
+
+template <int EX>
+__global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int D = a.D, M = D >> 2, nchunk = a.nchunk;
+    // LDS: A digits [nchunk][8 p][4 g][16 m][16 B], per-wave unit_q scratch [4][D],
+    // survivor list, per-query parameters.
+    int8_t* A = reinterpret_cast<int8_t*>(smem);
+    float* uq_all = reinterpret_cast<float*>(smem + (size_t)nchunk * 8192);
+    uint32_t* surv = reinterpret_cast<uint32_t*>(uq_all + 4 * (size_t)D);
+    __shared__ QParam qp[kGroup];
+    __shared__ uint32_t s_item, s_nsurv;
+    __shared__ float s_ipq[4 * 16 * kGroup];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    float* uq = uq_all + (size_t)wid * D;
+    const size_t BB = 4 * (size_t)D + 128;
+    const size_t CL = (size_t)D * EX / 8;
+    const int R = 1 << EX;
+    const float const_bound = 0.58f;
+    unsigned long long st_blocks = 0, st_surv = 0, st_cand = 0;
+
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&a.n_items[1], 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= a.n_items[0]) break;
+        const uint32_t cid = a.items[2 * item];
+        const uint32_t e0 = a.items[2 * item + 1] >> 4, elen = a.items[2 * item + 1] & 15u;
+        const float* cen = a.centroids + (size_t)cid * D;
+
+        // ---- per-query preparation: 2 queries per wave ----
+        for (int qq = wid; qq < kGroup; qq += 4) {
+            QParam P;
+            P.valid = qq < (int)elen;
+            if (P.valid) {
+                const uint32_t slot = a.entries[e0 + qq];
+                const int q = query_of_slot(a.probe_off, a.nq, slot);
+                const float* qv = a.q_rot + (size_t)q * D;
+                P.q = q;
+                P.sqr_y = a.probe_sqr_y[slot];
+                P.y = sqrtf(P.sqr_y);
+                float sumq;
+                if (P.y > 1e-5f) {
+                    P.fac = (float)(1.0 / (double)P.y);
+                    for (int d = lane; d < D; d += 64) uq[d] = (qv[d] - cen[d]) * P.fac;
+                    wave_sync();
+                    float part = 0.f;
+                    if (lane < 16)
+                        for (int d = lane; d < D; d += 16) part = part + uq[d];
+                    part = butterfly16(part);
+                    sumq = __shfl(part, 0);
+                } else {
+                    P.fac = 0.f;
+                    const float value = (float)(1.0 / (double)sqrtf((float)D));
+                    for (int d = lane; d < D; d += 64) uq[d] = value;
+                    wave_sync();
+                    sumq = (float)D * value;
+                }
+                float mx = 0.f, mn = 0.f;
+                for (int d = lane; d < D; d += 64) {
+                    float u = uq[d];
+                    mx = fmaxf(mx, u);
+                    mn = fminf(mn, u);
+                }
+                mx = wave_max(mx);
+                mn = wave_min(mn);
+                const float delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
+                // digits into the MFMA A layout; rows 2qq (low 7 bits) and 2qq+1 (high)
+                for (int d = lane; d < nchunk * 512; d += 64) {
+                    int qi = 0;
+                    if (d < D) {
+                        float tmp = uq[d] / delta;
+                        double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
+                        qi = (int)r;
+                    }
+                    const int c = d >> 9, dd = d & 511;
+                    const int g = dd >> 7, jw = (dd >> 5) & 3, b = (dd >> 3) & 3, p = dd & 7;
+                    const int j = 4 * jw + b;
+                    const int lo = qi & 127, hi = (qi - lo) >> 7;
+                    int8_t* base = A + ((((size_t)c * 8 + p) * 4 + g) * 16) * 16 + j;
+                    base[(2 * qq) * 16] = (int8_t)lo;
+                    base[(2 * qq + 1) * 16] = (int8_t)hi;
+                }
+                P.sumq = sumq;
+                P.delta = delta;
+                P.c1 = (float)(0.5 * (double)sumq - (double)const_bound);
+                P.c2 = (-5.0f * P.y) * a.one_over_sqrtD;
+                P.qs = P.y * P.y;
+                const float* pd = a.pool_d + (size_t)q * a.k;
+                P.thr = pd[a.k - 1];
+            } else {
+                // idle rows: zero digits so the MFMA rows contribute nothing
+                for (int d = lane; d < nchunk * 512; d += 64) {
+                    const int c = d >> 9, dd = d & 511;
+                    const int g = dd >> 7, jw = (dd >> 5) & 3, b = (dd >> 3) & 3, p = dd & 7;
+                    int8_t* base = A + ((((size_t)c * 8 + p) * 4 + g) * 16) * 16 + 4 * jw + b;
+                    base[(2 * qq) * 16] = 0;
+                    base[(2 * qq + 1) * 16] = 0;
+                }
+                P.q = 0;
+                P.thr = -FLT_MAX;
+            }
+            if (lane == 0) qp[qq] = P;
+            wave_sync();
+        }
+        __syncthreads();
+
+        // ---- scan: 16-vector tiles, one tile per wave per pass ----
+        const int n = (int)a.csize[cid];
+        const int ntile = (n + 15) >> 4;
+        const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
+        const int ln = lane & 15, lg = lane >> 4;
+        for (int tb = 0; tb < ntile; tb += 4) {
+            if (tid == 0) s_nsurv = 0;
+            __syncthreads();
+            const int t = tb + wid;
+            if (t < ntile) {
+                const uint32_t blk = b0 + (uint32_t)(t >> 1);
+                const int vib = 16 * (t & 1) + ln;  // vector within the 32-block
+                v4i acc[8];
+                for (int p = 0; p < 8; p++) acc[p] = v4i{0, 0, 0, 0};
+                for (int c = 0; c < nchunk; c++) {
+                    const uint4 w = *reinterpret_cast<const uint4*>(
+                        a.bp + (((size_t)blk * nchunk + c) * 32 + vib) * 64 + 16 * lg);
+                    for (int p = 0; p < 8; p++) {
+                        const uint32_t m = 0x01010101u << p;
+                        v4i bv = v4i{(int)(w.x & m), (int)(w.y & m), (int)(w.z & m),
+                                     (int)(w.w & m)};
+                        const v4i av = *reinterpret_cast<const v4i*>(
+                            A + ((((size_t)c * 8 + p) * 4 + lg) * 16 + ln) * 16);
+                        acc[p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[p], 0, 0, 0);
+                    }
+                }
+                // rows 4*lg + r of column ln: (query 2lg, planes 0/1), (query 2lg+1, 0/1)
+                int X[4];
+                for (int r = 0; r < 4; r++) {
+                    int s = -(acc[7][r] >> 7);
+                    for (int p = 0; p < 7; p++) s += acc[p][r] >> p;
+                    X[r] = s;
+                }
+                const int vloc = 16 * t + ln;  // vector within the cluster
+                const float on = reinterpret_cast<const float*>(
+                    a.short_data + (size_t)blk * BB + 4 * (size_t)D)[vib];
+                for (int h = 0; h < 2; h++) {
+                    const int qq = 2 * lg + h;
+                    const QParam& P = qp[qq];
+                    const int I = 128 * X[2 * h + 1] + X[2 * h];
+                    const float ipq = (float)I * P.delta;
+                    float tl = ipq - P.c1;
+                    tl = tl * P.c2;
+                    tl = tl * on;
+                    tl = tl + P.qs;
+                    tl = tl + on * on;
+                    if (P.valid && vloc < n && tl < P.thr) {
+                        uint32_t s = atomicAdd(&s_nsurv, 1u);
+                        surv[s] = ((uint32_t)qq << 16) | (uint32_t)vloc;
+                        s_ipq[s] = ipq;
+                    }
+                }
+                if (lane == 0) st_blocks++;
+            }
+            __syncthreads();
+
+            // ---- re-rank survivors: 16 lanes each, reference-ordered FP32 chains ----
+            const int ns = (int)s_nsurv;
+            const int grp = tid >> 4, l16 = tid & 15;
+            for (int sb = 0; sb < ns; sb += kThreads / 16) {
+                const int si = sb + grp;
+                const bool act = si < ns;
+                uint32_t sv = act ? surv[si] : 0u;
+                const int qq = (int)(sv >> 16), vloc = (int)(sv & 0xFFFFu);
+                const QParam& P = qp[qq];
+                const uint32_t vi = v0 + (uint32_t)vloc;
+                float part = 0.f;
+                if (act) {
+                    const float* qv = a.q_rot + (size_t)P.q * D;
+                    const uint8_t* code = a.long_code + (size_t)vi * CL;
+                    const float value = (float)(1.0 / (double)sqrtf((float)D));
+                    for (int ch = 0; ch < (D >> 6); ch++) {
+                        float e[4];
+                        decode4<EX>(code + ch * 8 * EX, l16, e);
+                        for (int s4 = 0; s4 < 4; s4++) {
+                            const int d = 64 * ch + 16 * s4 + l16;
+                            const float u = P.y > 1e-5f ? (qv[d] - cen[d]) * P.fac : value;
+                            part = fmaf(u, e[s4], part);
+                        }
+                    }
+                }
+                const float ip = butterfly16(part);
+                if (act && l16 == 0) {
+                    const uint32_t blk = b0 + (uint32_t)(vloc >> 5);
+                    const float on = reinterpret_cast<const float*>(
+                        a.short_data + (size_t)blk * BB + 4 * (size_t)D)[vloc & 31];
+                    const float sqr_x = on * on;
+                    const float Aa = sqr_x + P.sqr_y;
+                    const float Bb = a.ex_factor[vi] * P.y;
+                    const float Cc = (float)R * s_ipq[si] + ip;
+                    const double G = (double)Cc - ((double)R - 0.5) * (double)P.sumq;
+                    const double H = (double)Bb * G;
+                    const float exd = (float)((double)Aa - H);
+                    if (exd <= P.thr) {
+                        uint32_t slot = atomicAdd(&a.cand_cnt[P.q], 1u);
+                        if (slot < (uint32_t)a.cand_cap) {
+                            a.cand_d[(size_t)P.q * a.cand_cap + slot] = exd;
+                            a.cand_i[(size_t)P.q * a.cand_cap + slot] = a.ids[vi];
+                        }
+                        st_cand++;
+                    }
+                    st_surv++;
+                }
+            }
+            __syncthreads();
+        }
+        (void)M;
+    }
+    if (a.stats && lane == 0) {
+        atomicAdd(&a.stats[6], st_blocks);
+        atomicAdd(&a.stats[2], st_surv);
+        atomicAdd(&a.stats[4], st_cand);
+    }
+}
+
+// ---------------------------------------------------------------- merge
+// Per query: pool (k sorted, FLT_MAX-padded) + candidates -> k smallest by (dist, id).
+__global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
+                                                    uint32_t* __restrict__ pool_i, int k,
+                                                    uint32_t* __restrict__ cand_cnt,
+                                                    const float* __restrict__ cand_d,
+                                                    const uint32_t* __restrict__ cand_i,
+                                                    int cap, unsigned long long* stats) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t raw = cand_cnt[q];
+    const int nc = (int)min(raw, (uint32_t)cap);
+    if (nc == 0) return;
+    const int n = k + nc;
+    int P2 = 1;
+    while (P2 < n) P2 <<= 1;
+    float* sd = reinterpret_cast<float*>(smem);
+    uint32_t* si = reinterpret_cast<uint32_t*>(sd + P2);
+    for (int i = tid; i < P2; i += 256) {
+        float d = FLT_MAX;
+        uint32_t id = 0xFFFFFFFFu;
+        if (i < k) {
+            d = pool_d[(size_t)q * k + i];
+            id = d < FLT_MAX ? pool_i[(size_t)q * k + i] : 0xFFFFFFFFu;
+        } else if (i < n) {
+            d = cand_d[(size_t)q * cap + (i - k)];
+            id = cand_i[(size_t)q * cap + (i - k)];
+        }
+        sd[i] = d;
+        si[i] = id;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = tid; t < (P2 >> 1); t += 256) {
+                int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+                bool up = (lo & size) == 0;
+                float dl = sd[lo], dh = sd[hi];
+                uint32_t il = si[lo], ih = si[hi];
+                bool lt_hl = dh < dl || (dh == dl && ih < il);
+                bool lt_lh = dl < dh || (dl == dh && il < ih);
+                if (up ? lt_hl : lt_lh) {
+                    sd[lo] = dh;
+                    sd[hi] = dl;
+                    si[lo] = ih;
+                    si[hi] = il;
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = tid; i < k; i += 256) {
+        pool_d[(size_t)q * k + i] = sd[i];
+        pool_i[(size_t)q * k + i] = sd[i] < FLT_MAX ? si[i] : 0u;
+    }
+    if (tid == 0) {
+        if (raw > (uint32_t)cap && stats) atomicAdd(&stats[5], (unsigned long long)(raw - cap));
+        cand_cnt[q] = 0;
+    }
+}
+
+}  // namespace
+
+int bp_nchunk(int D) { return (D + 511) / 512; }
+
+hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
+                           hipStream_t s) {
+    if (nblocks == 0) return hipSuccess;
+    const int nchunk = bp_nchunk(D);
+    size_t total = nblocks * (size_t)nchunk * 32 * 64;
+    int blocks = (int)std::min<size_t>((total + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(relayout_kernel, dim3(blocks), dim3(256), 0, s, short_data, nblocks, D,
+                       nchunk, bp);
+    return hipGetLastError();
+}
+
+size_t batch_lds_bytes(int D) { return (size_t)bp_nchunk(D) * 8192 + 16 * (size_t)D + 4 * 1024; }
+
+hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1, int grid,
+                        hipStream_t s) {
+    const int nq = ba.nq, K = rb.K;
+    hipError_t e = launch_zero(rb.counts, (size_t)K, s);
+    if (e != hipSuccess) return e;
+    dim3 qg((nq + 255) / 256), qb(256);
+    hipLaunchKernelGGL(count_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq, r0, r1, K,
+                       rb.counts);
+    hipLaunchKernelGGL(scan_items_kernel, dim3(1), dim3(1024), 0, s, rb.counts, K, rb.cursor,
+                       (uint32_t*)ba.items, ba.n_items);
+    hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq, r0, r1, K,
+                       rb.cursor, (uint32_t*)ba.entries);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t lds = batch_lds_bytes(ba.D);
+    switch (ba.EX) {
+    case 2: hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid), dim3(kThreads), lds, s, ba); break;
+    case 3: hipLaunchKernelGGL(batch_scan_kernel<3>, dim3(grid), dim3(kThreads), lds, s, ba); break;
+    case 4: hipLaunchKernelGGL(batch_scan_kernel<4>, dim3(grid), dim3(kThreads), lds, s, ba); break;
+    case 6: hipLaunchKernelGGL(batch_scan_kernel<6>, dim3(grid), dim3(kThreads), lds, s, ba); break;
+    case 7: hipLaunchKernelGGL(batch_scan_kernel<7>, dim3(grid), dim3(kThreads), lds, s, ba); break;
+    case 8: hipLaunchKernelGGL(batch_scan_kernel<8>, dim3(grid), dim3(kThreads), lds, s, ba); break;
+    default: return hipErrorInvalidValue;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int P2 = 1;
+    while (P2 < ba.k + ba.cand_cap) P2 <<= 1;
+    hipLaunchKernelGGL(merge_kernel, dim3(nq), dim3(256), (size_t)P2 * 8, s, ba.pool_d,
+                       rb.pool_i, ba.k, ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.stats);
+    return hipGetLastError();
+}
+
+}  // namespace erq

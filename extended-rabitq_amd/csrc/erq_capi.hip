@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "erq.h"
+#include "erq_batch.h"
 #include "erq_internal.h"
 
 namespace erq {
@@ -73,7 +74,8 @@ void release(erq_index* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* ptrs[] = {h->short_data, h->long_code, h->ex_factor, h->ids, h->centroids,
-                    h->cnorm,      h->P,         h->csize,     h->bstart, h->vstart, h->ws};
+                    h->cnorm,      h->P,         h->csize,     h->bstart, h->vstart, h->ws,
+                    h->bp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -355,7 +357,7 @@ int erq_probe_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe
 
 static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                        const uint32_t* off, const uint32_t* pid, const float* pd, uint32_t* ids,
-                       float* dists, uint64_t* stats, hipStream_t s) {
+                       float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0) {
     ScanArgs a;
     a.q_rot = q_rot_dev;
     a.probe_off = off;
@@ -377,6 +379,7 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
     a.k = (int)k;
     a.nq = (int)nq;
     a.K = (int)h->K;
+    a.max_probes = max_probes;
     // HASearcher.hpp:56
     a.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
     hipError_t e = launch_scan(a, s);
@@ -448,6 +451,162 @@ int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, si
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "offsets");
     return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s);
+}
+
+static int ensure_bp(erq_index* h, hipStream_t s) {
+    if (h->bp) return ERQ_OK;
+    const size_t bytes = h->nblocks * (size_t)bp_nchunk((int)h->D) * 32 * 64;
+    hipError_t e = hipMalloc(&h->bp, bytes ? bytes : 64);
+    if (e != hipSuccess) {
+        h->bp = nullptr;
+        return hip_fail(e, "bit-plane codes alloc");
+    }
+    e = launch_relayout(h->short_data, h->nblocks, (int)h->D, h->bp, s);
+    if (e != hipSuccess) return hip_fail(e, "bit-plane relayout");
+    if (!h->num_cu) {
+        e = hipDeviceGetAttribute(&h->num_cu, hipDeviceAttributeMultiprocessorCount, h->device);
+        if (e != hipSuccess) return hip_fail(e, "device attribute");
+    }
+    return ERQ_OK;
+}
+
+static size_t batch_ws_bytes(size_t nq, size_t K, size_t max_nprobe, int cap) {
+    const size_t pairs = nq * max_nprobe;
+    return 6 * 256 + align_up(K * 4, 256) * 2 + align_up(8, 256) +
+           align_up((pairs / 8 + K + 8) * 8, 256) + align_up(pairs * 4 + 4, 256) +
+           align_up(nq * 4, 256) + align_up(nq * (size_t)cap * 4, 256) * 2;
+}
+
+static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                          const uint32_t* off, const uint32_t* pid, const float* pd,
+                          size_t max_nprobe, uint32_t* ids, float* dists, uint64_t* stats,
+                          hipStream_t s, char* ws_base, size_t ws_off) {
+    int rc = ensure_bp(h, s);
+    if (rc) return rc;
+    // round 0: the exact per-query scan of each query's nearest cluster
+    rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s, 1);
+    if (rc || max_nprobe <= 1) return rc;
+    const int cap = (int)std::max<size_t>(1024, 4096 - k);
+    const size_t pairs = nq * max_nprobe;
+    Ws ws{ws_base, ws_off};
+    uint32_t* counts = ws.take<uint32_t>(h->K);
+    uint32_t* cursor = ws.take<uint32_t>(h->K);
+    uint32_t* n_items = ws.take<uint32_t>(2);
+    uint32_t* items = ws.take<uint32_t>((pairs / 8 + h->K + 8) * 2);
+    uint32_t* entries = ws.take<uint32_t>(pairs + 1);
+    uint32_t* cand_cnt = ws.take<uint32_t>(nq);
+    float* cand_d = ws.take<float>(nq * (size_t)cap);
+    uint32_t* cand_i = ws.take<uint32_t>(nq * (size_t)cap);
+    hipError_t e = launch_zero(cand_cnt, nq, s);
+    if (e != hipSuccess) return hip_fail(e, "zero candidates");
+    BatchArgs ba;
+    ba.q_rot = q_rot_dev;
+    ba.probe_off = off;
+    ba.probe_sqr_y = pd;
+    ba.centroids = h->centroids;
+    ba.short_data = h->short_data;
+    ba.bp = h->bp;
+    ba.long_code = h->long_code;
+    ba.ex_factor = h->ex_factor;
+    ba.ids = h->ids;
+    ba.csize = h->csize;
+    ba.bstart = h->bstart;
+    ba.vstart = h->vstart;
+    ba.items = items;
+    ba.entries = entries;
+    ba.n_items = n_items;
+    ba.pool_d = dists;
+    ba.cand_cnt = cand_cnt;
+    ba.cand_d = cand_d;
+    ba.cand_i = cand_i;
+    ba.stats = reinterpret_cast<unsigned long long*>(stats);
+    ba.cand_cap = cap;
+    ba.D = (int)h->D;
+    ba.EX = (int)h->EX;
+    ba.k = (int)k;
+    ba.nq = (int)nq;
+    ba.nchunk = bp_nchunk((int)h->D);
+    ba.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
+    RoundBufs rb{pid, counts, cursor, ids, (int)h->K};
+    const int grid = std::max(1, h->num_cu) * 2;
+    for (size_t r0 = 1, r1 = 4; r0 < max_nprobe; r0 = r1, r1 *= 4) {
+        e = launch_round(ba, rb, (int)r0, (int)std::min(r1, max_nprobe), grid, s);
+        if (e != hipSuccess) return hip_fail(e, "batched round");
+    }
+    return ERQ_OK;
+}
+
+int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                         const uint32_t* probe_off_dev, const uint32_t* probe_ids_dev,
+                         const float* probe_sqr_y_dev, size_t max_nprobe, uint32_t* ids_dev,
+                         float* dists_dev, uint64_t* stats_dev, void* stream) {
+    if (!h || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev || !probe_sqr_y_dev ||
+                      !ids_dev || !dists_dev))) {
+        set_error("erq_scan_batched_dev: null argument (dists_dev is required)");
+        return ERQ_EINVAL;
+    }
+    if (k == 0 || k > 1024) {
+        set_error("erq_scan_batched_dev: k must be in [1, 1024]");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    if (nq * std::max<size_t>(max_nprobe, 1) >= (1u << 27)) {
+        set_error("erq_scan_batched_dev: nq * max_nprobe must stay below 2^27");
+        return ERQ_EUNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t s = pick(h, stream);
+    const int cap = (int)std::max<size_t>(1024, 4096 - k);
+    void* wsp;
+    int rc = workspace(h, batch_ws_bytes(nq, h->K, max_nprobe, cap), &wsp);
+    if (rc) return rc;
+    return batched_locked(h, q_rot_dev, nq, k, probe_off_dev, probe_ids_dev, probe_sqr_y_dev,
+                          max_nprobe, ids_dev, dists_dev, stats_dev, s, (char*)wsp, 0);
+}
+
+int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                           size_t nprobe, uint32_t* ids_dev, float* dists_dev,
+                           uint64_t* stats_dev, void* stream) {
+    if (!h || (nq && (!q_rot_dev || !ids_dev || !dists_dev))) {
+        set_error("erq_search_batched_dev: null argument (dists_dev is required)");
+        return ERQ_EINVAL;
+    }
+    if (k == 0 || k > 1024) {
+        set_error("erq_search_batched_dev: k must be in [1, 1024]");
+        return ERQ_EINVAL;
+    }
+    if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
+        set_error("erq_search_batched_dev: nprobe must be in [1, min(K, 2048)]");
+        return ERQ_EINVAL;
+    }
+    if (nq == 0) return ERQ_OK;
+    if (nq * nprobe >= (1u << 27)) {
+        set_error("erq_search_batched_dev: nq * nprobe must stay below 2^27");
+        return ERQ_EUNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t s = pick(h, stream);
+    const size_t K = h->K;
+    const int cap = (int)std::max<size_t>(1024, 4096 - k);
+    size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
+    size_t list_part = align_up(nq * nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) + 512;
+    size_t batch_part = batch_ws_bytes(nq, K, nprobe, cap);
+    void* wsp;
+    int rc = workspace(h, probe_part + list_part + batch_part, &wsp);
+    if (rc) return rc;
+    Ws ws{(char*)wsp};
+    ws.off = probe_part;
+    uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
+    float* pd = ws.take<float>(nq * nprobe);
+    uint32_t* off = ws.take<uint32_t>(nq + 1);
+    rc = probe_locked(h, q_rot_dev, nq, nprobe, pid, pd, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off, nq,
+                       (uint32_t)nprobe);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "offsets");
+    return batched_locked(h, q_rot_dev, nq, k, off, pid, pd, nprobe, ids_dev, dists_dev,
+                          stats_dev, s, (char*)wsp, probe_part + list_part);
 }
 
 int erq_search_batch(erq_index* h, const float* q_rot, size_t nq, size_t k, size_t nprobe,

@@ -27,6 +27,8 @@ struct erq_index {
     // grow-only workspace
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    uint8_t* bp = nullptr;  // bit-plane copy of the 1-bit codes (batched path, built on first use)
+    int num_cu = 0;
     std::mutex mu;
 };
 
@@ -53,6 +55,7 @@ struct ScanArgs {
     float* out_dists;
     unsigned long long* stats;
     int D, EX, k, nq, K;
+    int max_probes;  // > 0: scan at most this many leading probes per query
     float one_over_sqrtD;
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);

@@ -98,7 +98,9 @@ __global__ void __launch_bounds__(64) scan_kernel(ScanArgs a, int per_wave_bytes
     const int t4 = lane & 3, s16 = lane >> 2;
     const int g = lane >> 4, l16 = lane & 15;
 
-    const uint32_t p0 = a.probe_off[qi], p1 = a.probe_off[qi + 1];
+    const uint32_t p0 = a.probe_off[qi];
+    uint32_t p1 = a.probe_off[qi + 1];
+    if (a.max_probes > 0 && p1 - p0 > (uint32_t)a.max_probes) p1 = p0 + (uint32_t)a.max_probes;
     for (uint32_t p = p0; p < p1; ++p) {
         const uint32_t cid = a.probe_ids[p];
         if (cid >= (uint32_t)a.K) continue;  // not a cluster of this index

@@ -36,6 +36,10 @@ SIGNATURES = {
     "erq_scan_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                              c_vp]),
     "erq_search_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp, c_vp, c_vp]),
+    "erq_scan_batched_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_vp, c_vp, c_vp, c_size, c_vp,
+                                     c_vp, c_vp, c_vp]),
+    "erq_search_batched_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp, c_vp,
+                                       c_vp]),
     "erq_device_array": (c_vp, [c_vp, c_int]),
 }
 

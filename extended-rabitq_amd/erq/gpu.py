@@ -137,6 +137,25 @@ class IVF:
                                       _ptr(stats), _stream()))
         return ids, d
 
+    def scan_batched_dev(self, q_rot, k, probe_off, probe_ids, probe_d, max_nprobe, stats=None):
+        import torch
+        nq = q_rot.shape[0]
+        ids = torch.empty((nq, k), dtype=torch.int32, device=q_rot.device)
+        d = torch.empty((nq, k), dtype=torch.float32, device=q_rot.device)
+        check(_lib.lib().erq_scan_batched_dev(self._h, _ptr(q_rot), nq, k, _ptr(probe_off),
+                                              _ptr(probe_ids), _ptr(probe_d), max_nprobe,
+                                              _ptr(ids), _ptr(d), _ptr(stats), _stream()))
+        return ids, d
+
+    def search_batched_dev(self, q_rot, k, nprobe, stats=None):
+        import torch
+        nq = q_rot.shape[0]
+        ids = torch.empty((nq, k), dtype=torch.int32, device=q_rot.device)
+        d = torch.empty((nq, k), dtype=torch.float32, device=q_rot.device)
+        check(_lib.lib().erq_search_batched_dev(self._h, _ptr(q_rot), nq, k, nprobe, _ptr(ids),
+                                                _ptr(d), _ptr(stats), _stream()))
+        return ids, d
+
     def search_dev(self, q_rot, k, nprobe, stats=None):
         import torch
         nq = q_rot.shape[0]

@@ -99,6 +99,26 @@ int erq_scan_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
 int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
                    uint32_t* ids_dev, float* dists_dev, uint64_t* stats_dev, void* stream);
 
+/* Cluster-major batched form of erq_scan_dev for many queries with uneven probe lists:
+ * rank 0 of every query is scanned exactly as IVF::search does; ranks [1,4), [4,16),
+ * [16,64), ... are then processed in rounds, each round reading every probed cluster's
+ * codes once for all queries that probe it (FastScan sums on int8 MFMA) and pruning
+ * against each query's k-th distance at round start.  max_nprobe bounds every query's
+ * probe count.  Results are deterministic; with one probe per query they equal
+ * erq_scan_dev's.  dists_dev is required.  stats_dev (nullable) takes 8 counters:
+ * [0] blocks, [1] vectors, [3] probes of round 0; [2] survivors re-ranked (all rounds);
+ * [4] candidates kept, [5] candidates dropped on buffer overflow, [6] 16-vector tiles
+ * scanned in rounds >= 1. */
+int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                         const uint32_t* probe_off_dev, const uint32_t* probe_ids_dev,
+                         const float* probe_sqr_y_dev, size_t max_nprobe, uint32_t* ids_dev,
+                         float* dists_dev, uint64_t* stats_dev, void* stream);
+
+/* erq_probe_dev followed by erq_scan_batched_dev with a fixed nprobe. */
+int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                           size_t nprobe, uint32_t* ids_dev, float* dists_dev,
+                           uint64_t* stats_dev, void* stream);
+
 /* Device pointers of the resident index arrays (for tools that read the HBM layout).
  * which: 0 short blocks, 1 long codes, 2 ex factors, 3 ids, 4 rotated centroids,
  * 5 rotation P, 6 cluster sizes (u32), 7 first block per cluster (u32),

@@ -427,6 +427,100 @@ void orc_ivf_search_probes(const orc_index* idx, const float* q_rot, size_t k, s
     free(pd);
 }
 
+/* Round-synchronous batched search (this project's cluster-major schedule, checked
+ * against liberq's erq_scan_batched_dev): probe rank 0 exactly as IVF::search; ranks
+ * [1,4), [4,16), ... in rounds whose pruning threshold is the pool's k-th distance at
+ * round start; a vector is a candidate if its lower bound < threshold and its
+ * estimate <= threshold; candidates are merged into the pool by (distance, id). */
+static int dist_id_cmp(const void* a, const void* b) {
+    const cand *x = (const cand*)a, *y = (const cand*)b;
+    if (x->d < y->d) return -1;
+    if (x->d > y->d) return 1;
+    return (x->id > y->id) - (x->id < y->id);
+}
+
+void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
+                           const uint32_t* probe_ids, const float* probe_sqr_y, uint32_t* ids,
+                           float* dists) {
+    size_t D = idx->D;
+    float* unit_q = (float*)malloc(D * sizeof(float));
+    int16_t* qq = (int16_t*)malloc(D * sizeof(int16_t));
+    uint16_t* lut = (uint16_t*)malloc((D / 4) * 16 * sizeof(uint16_t));
+    uint32_t* pid = (uint32_t*)malloc((k + 1) * sizeof(uint32_t));
+    float* pd = (float*)malloc((k + 1) * sizeof(float));
+    orc_pool pool;
+    orc_pool_init(&pool, pid, pd, k);
+    float oos = orc_one_over_sqrtD(D);
+    if (nprobe > 0)
+        search_cluster(idx, probe_ids[0], q_rot, probe_sqr_y[0], oos, unit_q, qq, lut, &pool, NULL);
+    size_t ncap = 1024, nc = 0;
+    cand* cs = (cand*)malloc(ncap * sizeof(cand));
+    int R = 1 << idx->EX;
+    size_t block_bytes = 4 * D + 128, code_len = D * idx->EX / 8;
+    for (size_t r0 = 1, r1 = 4; r0 < nprobe; r0 = r1, r1 *= 4) {
+        float thr = orc_pool_distk(&pool);
+        nc = 0;
+        for (size_t i = 0; i < pool.size; i++) {
+            if (nc == ncap) cs = (cand*)realloc(cs, (ncap *= 2) * sizeof(cand));
+            cs[nc].id = pid[i];
+            cs[nc++].d = pd[i];
+        }
+        for (size_t r = r0; r < r1 && r < nprobe; r++) {
+            size_t cid = probe_ids[r];
+            float sqr_y = probe_sqr_y[r];
+            size_t n = (size_t)idx->cluster_sizes[cid];
+            const float* c = idx->centroids + cid * D;
+            float y = sqrtf(sqr_y);
+            float sumq = orc_normalize_query(unit_q, q_rot, c, y, D);
+            float delta;
+            orc_quantize_query(qq, unit_q, D, &delta);
+            int32_t shift = orc_pack_lut(lut, qq, D);
+            const uint8_t* block = idx->short_data + idx->block_start[cid] * block_bytes;
+            size_t v0 = idx->vec_start[cid];
+            float ipq[32];
+            for (size_t b = 0; b < (n + 31) / 32; b++, block += block_bytes) {
+                size_t np = n - b * 32 < 32 ? n - b * 32 : 32;
+                const float* onorm = (const float*)(block + 4 * D);
+                uint32_t mask = orc_fastscan_block(block, lut, onorm, sumq, y, delta, shift, oos,
+                                                   thr, D, ipq, NULL, NULL);
+                if (np < 32) mask &= (1u << np) - 1u;
+                while (mask) {
+                    int j = __builtin_ctz(mask);
+                    mask &= mask - 1;
+                    size_t vi = v0 + b * 32 + (size_t)j;
+                    float ip = orc_ip_ex(unit_q, idx->long_code + vi * code_len, D, (int)idx->EX);
+                    float a = onorm[j] * onorm[j] + sqr_y;
+                    float bb = idx->ex_factor[vi] * y;
+                    float cc = (float)R * ipq[j] + ip;
+                    double g = (double)cc - ((double)R - 0.5) * (double)sumq;
+                    float ex = (float)((double)a - (double)bb * g);
+                    if (ex <= thr) {
+                        if (nc == ncap) cs = (cand*)realloc(cs, (ncap *= 2) * sizeof(cand));
+                        cs[nc].id = idx->ids[vi];
+                        cs[nc++].d = ex;
+                    }
+                }
+            }
+        }
+        qsort(cs, nc, sizeof(cand), dist_id_cmp);
+        pool.size = nc < k ? nc : k;
+        for (size_t i = 0; i < pool.size; i++) {
+            pid[i] = cs[i].id;
+            pd[i] = cs[i].d;
+        }
+    }
+    for (size_t i = 0; i < k; i++) {
+        ids[i] = i < pool.size ? pid[i] : 0u;
+        if (dists) dists[i] = i < pool.size ? pd[i] : FLT_MAX;
+    }
+    free(cs);
+    free(unit_q);
+    free(qq);
+    free(lut);
+    free(pid);
+    free(pd);
+}
+
 void orc_ivf_search(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
                     uint32_t* ids, float* dists, orc_stats* st) {
     if (nprobe > idx->K) nprobe = idx->K;

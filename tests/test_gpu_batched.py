@@ -1,0 +1,126 @@
+"""GPU parity of the cluster-major batched search (erq_scan_batched_dev /
+erq_search_batched_dev) against the oracle's round-synchronous schedule, bit for bit;
+plus determinism and recall against the per-query path."""
+import numpy as np
+import pytest
+
+from conftest import build_cached, gpu_available, rotate_np
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+
+def _build(key, n, dim, ncl, bits, seed=1, nq=48):
+    from erq import data, index
+
+    def make():
+        x, cen = data.mixture_isotropic(n, dim, ncl, 0.3, seed=seed)
+        q, _ = data.mixture_isotropic(nq, dim, ncl, 0.3, seed=seed + 1, centres=cen)
+        c, assign = data.kmeans(x, ncl, 5, seed=seed + 3)
+        idx = index.build_index(x, c, assign, bits=bits, seed=seed + 2)
+        return idx, q, x
+    return build_cached(("batched",) + key, make)
+
+
+def _oracle_rounds(idx, qr, k, probe_lists):
+    from oracle import oracle as orc
+    oi = orc.OracleIndex(idx)
+    ids = np.empty((qr.shape[0], k), np.uint32)
+    ds = np.empty((qr.shape[0], k), np.float32)
+    for i in range(qr.shape[0]):
+        pid, pd = probe_lists[i]
+        ids[i], ds[i] = oi.search_rounds(qr[i], k, pid, pd)
+    return ids, ds
+
+
+def _probe_lists(idx, qr, nprobes):
+    from oracle import oracle as orc
+    oi = orc.OracleIndex(idx)
+    return [oi.centroid_topn(qr[i], int(nprobes[i])) for i in range(qr.shape[0])]
+
+
+def _gpu_batched(idx, qr, k, probe_lists, max_nprobe):
+    import torch
+    from erq.gpu import IVF
+    g = IVF.from_index(idx)
+    npq = np.array([len(p[0]) for p in probe_lists])
+    off = np.concatenate([[0], np.cumsum(npq)]).astype(np.uint32)
+    pid = np.concatenate([p[0] for p in probe_lists]).astype(np.uint32)
+    pd = np.concatenate([p[1] for p in probe_lists]).astype(np.float32)
+    t = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a).cuda()
+    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
+    ids, d = g.scan_batched_dev(torch.from_numpy(qr).cuda(), k, t(off), t(pid), t(pd),
+                                max_nprobe, stats=stats)
+    torch.cuda.synchronize()
+    out = ids.cpu().numpy().view(np.uint32), d.cpu().numpy(), stats.cpu().numpy()
+    g.close()
+    return out
+
+
+def _check(idx, q, k, nprobes, label):
+    qr = rotate_np(idx, q)
+    pl = _probe_lists(idx, qr, nprobes)
+    oid, od = _oracle_rounds(idx, qr, k, pl)
+    gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)))
+    print(f"[{label}] stats={st.tolist()}", flush=True)
+    assert st[5] == 0, "candidate buffer overflow"
+    np.testing.assert_array_equal(gid, oid, err_msg=f"{label}: ids")
+    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32), err_msg=f"{label}: dists")
+    return gid, gd
+
+
+@pytest.mark.parametrize("bits", [3, 4, 7, 9])
+def test_batched_matches_round_oracle_d128(bits):
+    idx, q, _ = _build(("d128", bits), 20000, 128, 64, bits)
+    _check(idx, q, 100, np.full(q.shape[0], 24), f"D128 B{bits}")
+
+
+def test_batched_uneven_probe_counts():
+    idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
+    rng = np.random.default_rng(5)
+    _check(idx, q, 50, rng.integers(1, 64, q.shape[0]), "uneven")
+
+
+def test_batched_d1024_b8():
+    idx, q, _ = _build(("d960", 8), 8000, 960, 48, 8, nq=24)
+    _check(idx, q, 100, np.full(q.shape[0], 20), "D960 B8")
+
+
+def test_batched_d1536_b4():
+    idx, q, _ = _build(("d1536", 4), 8000, 1536, 48, 4, nq=24)
+    _check(idx, q, 100, np.full(q.shape[0], 20), "D1536 B4")
+
+
+def test_batched_single_probe_equals_plain_and_is_deterministic():
+    import torch
+    from erq.gpu import IVF
+    idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
+    qr = torch.from_numpy(rotate_np(idx, q)).cuda()
+    g = IVF.from_index(idx)
+    pi, pdist = g.search_dev(qr, 100, 1)
+    bi, bd = g.search_batched_dev(qr, 100, 1)
+    b2i, b2d = g.search_batched_dev(qr, 100, 16)
+    b3i, b3d = g.search_batched_dev(qr, 100, 16)
+    torch.cuda.synchronize()
+    assert torch.equal(pi, bi) and torch.equal(pdist, bd)
+    assert torch.equal(b2i, b3i) and torch.equal(b2d, b3d)
+    g.close()
+
+
+def test_batched_recall_close_to_plain():
+    import torch
+    from erq import data
+    from erq.gpu import IVF
+    idx, q, x = _build(("d128", 4), 20000, 128, 64, 4)
+    gt, _ = data.exact_knn_np(x, q, 100)
+    qr = torch.from_numpy(rotate_np(idx, q)).cuda()
+    g = IVF.from_index(idx)
+    for nprobe in (4, 16):
+        pi, _ = g.search_dev(qr, 100, nprobe)
+        bi, _ = g.search_batched_dev(qr, 100, nprobe)
+        torch.cuda.synchronize()
+        rp = data.recall_at_k(pi.cpu().numpy(), gt, 100)
+        rb = data.recall_at_k(bi.cpu().numpy(), gt, 100)
+        print(f"nprobe={nprobe} recall plain={rp:.4f} batched={rb:.4f}")
+        assert rb >= rp - 0.005
+    g.close()

@@ -375,3 +375,28 @@ def test_search_recall_monotone_and_exhaustive(small_data):
     assert recalls[-1] > 0.97      # 8-bit ex codes, every cluster probed
     ids, d = oi.search(qr[0], 100, 64)
     assert np.all(np.diff(d) >= 0)
+
+
+def test_round_schedule_oracle(small_data):
+    """The round-synchronous schedule: one probe equals the per-query search; with more
+    probes the result is sorted by (distance, id) and its recall tracks the per-query one."""
+    from conftest import rotate_np
+    from erq import data, index
+    x, q, c, assign = small_data
+    idx = index.build_index(x, c, assign, bits=4, seed=3)
+    oi = orc.OracleIndex(idx)
+    qr = rotate_np(idx, q[:32])
+    gt, _ = data.exact_knn_np(x, q[:32], 100)
+    res_p, res_r = [], []
+    for i in range(32):
+        pid, pd = oi.centroid_topn(qr[i], 1)
+        a = oi.search_probes(qr[i], 100, pid, pd)
+        b = oi.search_rounds(qr[i], 100, pid, pd)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        pid, pd = oi.centroid_topn(qr[i], 16)
+        ids, d = oi.search_rounds(qr[i], 100, pid, pd)
+        assert all((d[j], ids[j]) <= (d[j + 1], ids[j + 1]) for j in range(99))
+        res_r.append(ids)
+        res_p.append(oi.search_probes(qr[i], 100, pid, pd)[0])
+    assert data.recall_at_k(res_r, gt, 100) >= data.recall_at_k(res_p, gt, 100) - 0.01
