@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--kmeans-iters", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--nprobes", type=str, default="8,16,24,32,48,64,96,128,192,256")
+    ap.add_argument("--sigma", type=float, default=0.3,
+                    help="within-component spread relative to the component spread")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="print the recall sweep of both paths and exit (no timing)")
     return ap.parse_args()
 
 
@@ -51,8 +55,9 @@ def build_workload(a, torch):
     from erq import data, index
     t0 = time.time()
     ncomp = a.clusters
-    x, (basis, centres) = data.mixture_powerlaw_torch(a.n, a.dim, ncomp, 0.3, seed=1, alpha=1.0)
-    q, _ = data.mixture_powerlaw_torch(a.nq, a.dim, ncomp, 0.3, seed=2, alpha=1.0,
+    x, (basis, centres) = data.mixture_powerlaw_torch(a.n, a.dim, ncomp, a.sigma, seed=1,
+                                                      alpha=1.0)
+    q, _ = data.mixture_powerlaw_torch(a.nq, a.dim, ncomp, a.sigma, seed=2, alpha=1.0,
                                        centres=centres, basis=basis)
     log(f"data generated N={a.n} D={a.dim} nq={a.nq} in {time.time() - t0:.1f}s")
     t0 = time.time()
@@ -174,6 +179,17 @@ def main():
         else:
             qps = qps_hi
         return dict(qps=qps, hi=hi, lo=lo, th=th, tl=tl, sweep=sweep)
+
+    if a.sweep_only:
+        for path in ("plain", "batched"):
+            for npb in [int(s) for s in a.nprobes.split(",")]:
+                if npb > idx.K:
+                    break
+                ids, _ = step(npb, path)
+                torch.cuda.synchronize()
+                rec = data.recall_at_k(ids.cpu().numpy().view(np.uint32), gt, a.k)
+                log(f"[{path}] sigma={a.sigma} nprobe={npb:4d} recall@{a.k}={rec:.4f}")
+        return
 
     plain = measure("plain")
     try:

@@ -391,35 +391,92 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
 }
 
 // ---------------------------------------------------------------- merge
-// Per query: pool (k sorted, FLT_MAX-padded) + candidates -> k smallest by (dist, id).
+// Per query: pool (k sorted, FLT_MAX-padded) + candidates -> the k smallest by (dist, id).
+// The k-th smallest distance key is found by an 8-bit radix select over both lists read
+// from global memory; entries up to it are gathered in LDS and bitonic-sorted.
+constexpr int kMergeBuf = 4096;
+
+__device__ __forceinline__ void merge_entry(int i, int k, int q, const float* pool_d,
+                                            const uint32_t* pool_i, const float* cand_d,
+                                            const uint32_t* cand_i, int cap, float& d,
+                                            uint32_t& id) {
+    if (i < k) {
+        d = pool_d[(size_t)q * k + i];
+        id = d < FLT_MAX ? pool_i[(size_t)q * k + i] : 0xFFFFFFFFu;
+    } else {
+        d = cand_d[(size_t)q * cap + (i - k)];
+        id = cand_i[(size_t)q * cap + (i - k)];
+    }
+}
+
 __global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
                                                     uint32_t* __restrict__ pool_i, int k,
                                                     uint32_t* __restrict__ cand_cnt,
                                                     const float* __restrict__ cand_d,
                                                     const uint32_t* __restrict__ cand_i,
                                                     int cap, unsigned long long* stats) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ float sd[kMergeBuf];
+    __shared__ uint32_t si[kMergeBuf];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
     const int q = blockIdx.x, tid = threadIdx.x;
     const uint32_t raw = cand_cnt[q];
     const int nc = (int)min(raw, (uint32_t)cap);
     if (nc == 0) return;
     const int n = k + nc;
-    int P2 = 1;
-    while (P2 < n) P2 <<= 1;
-    float* sd = reinterpret_cast<float*>(smem);
-    uint32_t* si = reinterpret_cast<uint32_t*>(sd + P2);
-    for (int i = tid; i < P2; i += 256) {
-        float d = FLT_MAX;
-        uint32_t id = 0xFFFFFFFFu;
-        if (i < k) {
-            d = pool_d[(size_t)q * k + i];
-            id = d < FLT_MAX ? pool_i[(size_t)q * k + i] : 0xFFFFFFFFu;
-        } else if (i < n) {
-            d = cand_d[(size_t)q * cap + (i - k)];
-            id = cand_i[(size_t)q * cap + (i - k)];
+    // radix select of the k-th smallest distance key
+    uint32_t prefix = 0, pmask = 0;
+    if (tid == 0) {
+        sh_rank = (uint32_t)min(k, n);
+        sh_cnt = 0;
+    }
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = tid; i < 256; i += 256) hist[i] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += 256) {
+            float d;
+            uint32_t id;
+            merge_entry(i, k, q, pool_d, pool_i, cand_d, cand_i, cap, d, id);
+            uint32_t kv = float_key(d);
+            if ((kv & pmask) == prefix) atomicAdd(&hist[(kv >> shift) & 255u], 1u);
         }
-        sd[i] = d;
-        si[i] = id;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t r = sh_rank, cum = 0, sel = 255;
+            for (int dg = 0; dg < 256; dg++) {
+                if (cum + hist[dg] >= r) {
+                    sel = (uint32_t)dg;
+                    break;
+                }
+                cum += hist[dg];
+            }
+            sh_sel = sel;
+            sh_rank = r - cum;
+        }
+        __syncthreads();
+        prefix |= sh_sel << shift;
+        pmask |= 255u << shift;
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += 256) {
+        float d;
+        uint32_t id;
+        merge_entry(i, k, q, pool_d, pool_i, cand_d, cand_i, cap, d, id);
+        if (float_key(d) <= prefix) {
+            uint32_t slot = atomicAdd(&sh_cnt, 1u);
+            if (slot < (uint32_t)kMergeBuf) {
+                sd[slot] = d;
+                si[slot] = id;
+            }
+        }
+    }
+    __syncthreads();
+    const int m = min((int)sh_cnt, kMergeBuf);
+    int P2 = 1;
+    while (P2 < max(m, k)) P2 <<= 1;
+    for (int i = m + tid; i < P2; i += 256) {
+        sd[i] = FLT_MAX;
+        si[i] = 0xFFFFFFFFu;
     }
     __syncthreads();
     for (int size = 2; size <= P2; size <<= 1)
@@ -500,9 +557,7 @@ hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    int P2 = 1;
-    while (P2 < ba.k + ba.cand_cap) P2 <<= 1;
-    hipLaunchKernelGGL(merge_kernel, dim3(nq), dim3(256), (size_t)P2 * 8, s, ba.pool_d,
+    hipLaunchKernelGGL(merge_kernel, dim3(nq), dim3(256), 0, s, ba.pool_d,
                        rb.pool_i, ba.k, ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.stats);
     return hipGetLastError();
 }

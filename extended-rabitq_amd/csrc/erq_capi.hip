@@ -52,6 +52,7 @@ namespace {
 
 constexpr size_t kMaxK = 8192;   // probe selection keeps all centroid keys in LDS
 constexpr size_t kMaxD = 4096;   // per-wave LDS slice of the scan kernel
+constexpr int kCandCap = 8192;   // batched rounds: candidates kept per query per round
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -94,7 +95,8 @@ struct Ws {
     }
 };
 
-hipStream_t pick(erq_index* h, void* s) { return s ? static_cast<hipStream_t>(s) : h->stream; }
+// _dev entry points run on the caller's stream exactly as given (0 is the null stream).
+hipStream_t pick(erq_index*, void* s) { return static_cast<hipStream_t>(s); }
 
 }  // namespace
 
@@ -406,6 +408,8 @@ int erq_scan_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                        dists_dev, stats_dev, pick(h, stream));
 }
 
+__global__ static void add_u32_to_u64(const uint32_t* src, uint64_t* dst) { *dst += *src; }
+
 __global__ static void fill_offsets(uint32_t* off, size_t nq, uint32_t nprobe) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i <= nq) off[i] = (uint32_t)(i * nprobe);
@@ -486,7 +490,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     // round 0: the exact per-query scan of each query's nearest cluster
     rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s, 1);
     if (rc || max_nprobe <= 1) return rc;
-    const int cap = (int)std::max<size_t>(1024, 4096 - k);
+    const int cap = kCandCap;
     const size_t pairs = nq * max_nprobe;
     Ws ws{ws_base, ws_off};
     uint32_t* counts = ws.take<uint32_t>(h->K);
@@ -556,7 +560,7 @@ int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t
     }
     std::lock_guard<std::mutex> lk(h->mu);
     hipStream_t s = pick(h, stream);
-    const int cap = (int)std::max<size_t>(1024, 4096 - k);
+    const int cap = kCandCap;
     void* wsp;
     int rc = workspace(h, batch_ws_bytes(nq, h->K, max_nprobe, cap), &wsp);
     if (rc) return rc;
@@ -587,7 +591,7 @@ int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size
     std::lock_guard<std::mutex> lk(h->mu);
     hipStream_t s = pick(h, stream);
     const size_t K = h->K;
-    const int cap = (int)std::max<size_t>(1024, 4096 - k);
+    const int cap = kCandCap;
     size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
     size_t list_part = align_up(nq * nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) + 512;
     size_t batch_part = batch_ws_bytes(nq, K, nprobe, cap);
@@ -609,6 +613,121 @@ int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size
                           stats_dev, s, (char*)wsp, probe_part + list_part);
 }
 
+static size_t learned_ws_bytes(size_t nq, size_t K, size_t max_np, size_t fb_np) {
+    return 8 * 256 + align_up(nq * K * 4, 256) + 2 * align_up(nq * fb_np * 4, 256) +
+           2 * align_up(nq * max_np * 4, 256) + align_up(nq * 4, 256);
+}
+
+static int learned_locked(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq,
+                          float mass, size_t min_np, size_t max_np, size_t fb_np, float min_conf,
+                          uint32_t* off, uint32_t* ids, float* sqr_y, uint32_t* n_fallback,
+                          hipStream_t s, char* base, size_t learned_off) {
+    Ws ws{base, learned_off};
+    float* prob = ws.take<float>(nq * h->K);
+    uint32_t* fb_ids = ws.take<uint32_t>(nq * fb_np);
+    float* fb_d = ws.take<float>(nq * fb_np);
+    uint32_t* pad_ids = ws.take<uint32_t>(nq * max_np);
+    float* pad_d = ws.take<float>(nq * max_np);
+    uint32_t* len = ws.take<uint32_t>(nq);
+    int rc = erq_mlp_forward_dev(m, q_rot_dev, nq, prob, s);
+    if (rc) return rc;
+    rc = probe_locked(h, q_rot_dev, nq, fb_np, fb_ids, fb_d, s);
+    if (rc) return rc;
+    hipError_t e = launch_adaptive_probe(prob, q_rot_dev, h->centroids, (int)nq, (int)h->K,
+                                         (int)h->D, mass, (int)min_np, (int)max_np, min_conf,
+                                         fb_ids, fb_d, (int)fb_np, pad_ids, pad_d, len, off, ids,
+                                         sqr_y, n_fallback, s);
+    if (e != hipSuccess) return hip_fail(e, "learned probe selection");
+    return ERQ_OK;
+}
+
+static int learned_args_ok(erq_index* h, erq_mlp* m, size_t min_np, size_t max_np, size_t fb_np,
+                           const char* who) {
+    if (erq_mlp_in_dim(m) != h->D || erq_mlp_out_dim(m) != h->K) {
+        set_error(std::string(who) + ": the model must map D inputs to K clusters");
+        return ERQ_EINVAL;
+    }
+    if (max_np == 0 || max_np > h->K || max_np > 2048 || fb_np == 0 || fb_np > max_np ||
+        min_np > max_np) {
+        set_error(std::string(who) + ": need 1 <= fallback <= max_nprobe <= min(K, 2048), "
+                  "min_nprobe <= max_nprobe");
+        return ERQ_EINVAL;
+    }
+    return ERQ_OK;
+}
+
+int erq_probe_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq, float mass,
+                          size_t min_nprobe, size_t max_nprobe, size_t fallback_nprobe,
+                          float min_conf, uint32_t* probe_off_dev, uint32_t* probe_ids_dev,
+                          float* probe_sqr_y_dev, uint32_t* n_fallback_dev, void* stream) {
+    if (!h || !m || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev || !probe_sqr_y_dev ||
+                            !n_fallback_dev))) {
+        set_error("erq_probe_learned_dev: null argument");
+        return ERQ_EINVAL;
+    }
+    int rc = learned_args_ok(h, m, min_nprobe, max_nprobe, fallback_nprobe, "erq_probe_learned_dev");
+    if (rc || nq == 0) return rc;
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t s = pick(h, stream);
+    size_t probe_part = align_up(nq * h->K * 4, 256) + align_up(nq * 4, 256) + 512;
+    void* wsp;
+    rc = workspace(h, probe_part + learned_ws_bytes(nq, h->K, max_nprobe, fallback_nprobe), &wsp);
+    if (rc) return rc;
+    return learned_locked(h, m, q_rot_dev, nq, mass, min_nprobe, max_nprobe, fallback_nprobe,
+                          min_conf, probe_off_dev, probe_ids_dev, probe_sqr_y_dev, n_fallback_dev,
+                          s, (char*)wsp, probe_part);
+}
+
+int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq, size_t k,
+                           float mass, size_t min_nprobe, size_t max_nprobe,
+                           size_t fallback_nprobe, float min_conf, uint32_t* ids_dev,
+                           float* dists_dev, uint64_t* stats_dev, void* stream) {
+    if (!h || !m || (nq && (!q_rot_dev || !ids_dev || !dists_dev))) {
+        set_error("erq_search_learned_dev: null argument (dists_dev is required)");
+        return ERQ_EINVAL;
+    }
+    if (k == 0 || k > 1024) {
+        set_error("erq_search_learned_dev: k must be in [1, 1024]");
+        return ERQ_EINVAL;
+    }
+    int rc = learned_args_ok(h, m, min_nprobe, max_nprobe, fallback_nprobe, "erq_search_learned_dev");
+    if (rc || nq == 0) return rc;
+    if (nq * max_nprobe >= (1u << 27)) {
+        set_error("erq_search_learned_dev: nq * max_nprobe must stay below 2^27");
+        return ERQ_EUNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t s = pick(h, stream);
+    const size_t K = h->K;
+    const int cap = kCandCap;
+    size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
+    size_t learned_part = learned_ws_bytes(nq, K, max_nprobe, fallback_nprobe);
+    size_t list_part = align_up(nq * max_nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) +
+                       align_up(8, 256) + 512;
+    size_t batch_part = batch_ws_bytes(nq, K, max_nprobe, cap);
+    void* wsp;
+    rc = workspace(h, probe_part + learned_part + list_part + batch_part, &wsp);
+    if (rc) return rc;
+    Ws ws{(char*)wsp, probe_part + learned_part};
+    uint32_t* off = ws.take<uint32_t>(nq + 1);
+    uint32_t* pid = ws.take<uint32_t>(nq * max_nprobe);
+    float* pd = ws.take<float>(nq * max_nprobe);
+    uint32_t* nfb = ws.take<uint32_t>(2);
+    hipError_t e = launch_zero(nfb, 2, s);
+    if (e != hipSuccess) return hip_fail(e, "zero");
+    rc = learned_locked(h, m, q_rot_dev, nq, mass, min_nprobe, max_nprobe, fallback_nprobe,
+                        min_conf, off, pid, pd, nfb, s, (char*)wsp, probe_part);
+    if (rc) return rc;
+    if (stats_dev) {
+        // counter 7: queries that took the fixed-nprobe fallback
+        hipLaunchKernelGGL(add_u32_to_u64, dim3(1), dim3(1), 0, s, nfb, stats_dev + 7);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "stats");
+    }
+    return batched_locked(h, q_rot_dev, nq, k, off, pid, pd, max_nprobe, ids_dev, dists_dev,
+                          stats_dev, s, (char*)wsp, probe_part + learned_part + list_part);
+}
+
 int erq_search_batch(erq_index* h, const float* q_rot, size_t nq, size_t k, size_t nprobe,
                      uint32_t* results, float* dists) {
     if (!h || (nq && (!q_rot || !results))) {
@@ -628,7 +747,7 @@ int erq_search_batch(erq_index* h, const float* q_rot, size_t nq, size_t k, size
         e = hipMemcpyAsync(dq, q_rot, nq * h->D * 4, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) rc = hip_fail(e, "copy queries");
     }
-    if (!rc) rc = erq_search_dev(h, dq, nq, k, nprobe, di, dd, nullptr, nullptr);
+    if (!rc) rc = erq_search_dev(h, dq, nq, k, nprobe, di, dd, nullptr, h->stream);
     if (!rc) {
         e = hipMemcpyAsync(results, di, nq * k * 4, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess && dists)
@@ -662,7 +781,7 @@ int erq_rotate(erq_index* h, const float* q, size_t nq, float* q_rot) {
         e = hipMemcpyAsync(dq, q, nq * h->DIM * 4, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) rc = hip_fail(e, "copy queries");
     }
-    if (!rc) rc = erq_rotate_dev(h, dq, nq, dr, nullptr);
+    if (!rc) rc = erq_rotate_dev(h, dq, nq, dr, h->stream);
     if (!rc) {
         e = hipMemcpyAsync(q_rot, dr, nq * h->D * 4, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

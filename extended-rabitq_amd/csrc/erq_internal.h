@@ -4,6 +4,8 @@
 #include <stdint.h>
 
 #include <mutex>
+
+#include "erq.h"
 #include <string>
 #include <vector>
 
@@ -73,5 +75,12 @@ hipError_t launch_pad_rows(const float* src, int rows, int cols, float* dst, int
 hipError_t launch_probe_select(const float* approx, const float* qnorm, const float* cnorm,
                                const float* q_rot, const float* centroids, int nq, int K, int D,
                                int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s);
+
+hipError_t launch_adaptive_probe(const float* prob, const float* q_rot, const float* centroids,
+                                 int nq, int K, int D, float mass, int min_np, int max_np,
+                                 float min_conf, const uint32_t* fb_ids, const float* fb_d,
+                                 int fb_np, uint32_t* pad_ids, float* pad_d, uint32_t* len,
+                                 uint32_t* off, uint32_t* csr_ids, float* csr_d,
+                                 uint32_t* n_fallback, hipStream_t s);
 
 }  // namespace erq

@@ -167,7 +167,196 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// Learned probe lists.  Per query: rank clusters by predicted probability p (descending,
+// ties by id), keep the shortest prefix whose mass reaches `mass` of sum(p) (at least
+// min_np, at most max_np), then order the kept clusters by their exact L2Sqr16 distance
+// (the order IVF::search scans in).  Low confidence -- sum(p) < min_conf, or the prefix
+// hits max_np before reaching the mass -- takes the fixed-nprobe list instead.
+__global__ void __launch_bounds__(kThreads)
+    adaptive_select_kernel(const float* __restrict__ prob, const float* __restrict__ q_rot,
+                           const float* __restrict__ centroids, int K, int D, float mass,
+                           int min_np, int max_np, float min_conf,
+                           const uint32_t* __restrict__ fb_ids, const float* __restrict__ fb_d,
+                           int fb_np, uint32_t* __restrict__ out_ids, float* __restrict__ out_d,
+                           uint32_t* __restrict__ out_len, uint32_t* __restrict__ n_fallback) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* keys = reinterpret_cast<uint32_t*>(smem);
+    float* cd = reinterpret_cast<float*>(smem + 4 * (size_t)K);
+    uint32_t* ci = reinterpret_cast<uint32_t*>(cd + kCapC);
+    float* qs = reinterpret_cast<float*>(ci + kCapC);
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
+    __shared__ float sh_sum[kThreads / 64];
+    __shared__ int sh_n, sh_fb;
+    const int tid = threadIdx.x, qi = blockIdx.x;
+    const int grp = tid >> 4, l = tid & 15;
+    const float* prow = prob + (size_t)qi * K;
+    float part = 0.f;
+    for (int c = tid; c < K; c += kThreads) {
+        float pc = prow[c];
+        part += pc;
+        keys[c] = float_key(-pc);
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if ((tid & 63) == 0) sh_sum[tid >> 6] = part;
+    if (tid == 0) sh_cnt = 0;
+    const float* q = q_rot + (size_t)qi * D;
+    for (int d = tid; d < D; d += kThreads) qs[d] = q[d];
+    __syncthreads();
+    float total = 0.f;
+    for (int w = 0; w < kThreads / 64; w++) total += sh_sum[w];
+    const int T = min(max_np, K);
+    const uint32_t U = radix_select(keys, K, (uint32_t)T, hist, &sh_sel, &sh_rank);
+    for (int c = tid; c < K; c += kThreads)
+        if (keys[c] <= U) {
+            uint32_t slot = atomicAdd(&sh_cnt, 1u);
+            if (slot < (uint32_t)kCapC) {
+                ci[slot] = (uint32_t)c;
+                cd[slot] = key_float(keys[c]);  // -p
+            }
+        }
+    __syncthreads();
+    const int cnt = min((int)sh_cnt, kCapC);
+    int P2 = 1;
+    while (P2 < cnt) P2 <<= 1;
+    for (int i = cnt + tid; i < P2; i += kThreads) {
+        cd[i] = FLT_MAX;
+        ci[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = tid; t < (P2 >> 1); t += kThreads) {
+                int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+                bool up = (lo & size) == 0;
+                float dl = cd[lo], dh = cd[hi];
+                uint32_t il = ci[lo], ih = ci[hi];
+                if (up ? cand_less(dh, ih, dl, il) : cand_less(dl, il, dh, ih)) {
+                    cd[lo] = dh;
+                    cd[hi] = dl;
+                    ci[lo] = ih;
+                    ci[hi] = il;
+                }
+            }
+            __syncthreads();
+        }
+    if (tid == 0) {
+        float acc = 0.f, goal = mass * total;
+        int n = 0;
+        while (n < T && acc < goal) acc += -cd[n++];
+        int fb = (total < min_conf) || (acc < goal);
+        sh_fb = fb;
+        sh_n = max(min(max(n, min_np), T), 1);
+    }
+    __syncthreads();
+    if (sh_fb) {
+        for (int i = tid; i < fb_np; i += kThreads) {
+            out_ids[(size_t)qi * max_np + i] = fb_ids[(size_t)qi * fb_np + i];
+            out_d[(size_t)qi * max_np + i] = fb_d[(size_t)qi * fb_np + i];
+        }
+        if (tid == 0) {
+            out_len[qi] = (uint32_t)fb_np;
+            atomicAdd(n_fallback, 1u);
+        }
+        return;
+    }
+    const int n = sh_n;
+    // exact distances of the kept clusters, then order them by (distance, id)
+    for (int base = 0; base < n; base += kThreads / 16) {
+        const int i = base + grp;
+        const bool act = i < n;
+        float pdist = l2sqr16_group(qs, centroids + (size_t)(act ? ci[i] : 0) * D, D, l, act);
+        if (act && l == 0) cd[i] = pdist;
+    }
+    __syncthreads();
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = n + tid; i < P; i += kThreads) {
+        cd[i] = FLT_MAX;
+        ci[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = tid; t < (P >> 1); t += kThreads) {
+                int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+                bool up = (lo & size) == 0;
+                float dl = cd[lo], dh = cd[hi];
+                uint32_t il = ci[lo], ih = ci[hi];
+                if (up ? cand_less(dh, ih, dl, il) : cand_less(dl, il, dh, ih)) {
+                    cd[lo] = dh;
+                    cd[hi] = dl;
+                    ci[lo] = ih;
+                    ci[hi] = il;
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = tid; i < n; i += kThreads) {
+        out_ids[(size_t)qi * max_np + i] = ci[i];
+        out_d[(size_t)qi * max_np + i] = cd[i];
+    }
+    if (tid == 0) out_len[qi] = (uint32_t)n;
+}
+
+// CSR offsets from per-query lengths (single workgroup, chunked inclusive scan).
+__global__ void __launch_bounds__(1024) lens_to_offsets_kernel(const uint32_t* __restrict__ len,
+                                                               int nq, uint32_t* __restrict__ off) {
+    __shared__ uint32_t sc[1024];
+    __shared__ uint32_t carry;
+    const int t = threadIdx.x;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nq; base += 1024) {
+        uint32_t v = base + t < nq ? len[base + t] : 0;
+        sc[t] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            uint32_t a = t >= o ? sc[t - o] : 0;
+            __syncthreads();
+            sc[t] += a;
+            __syncthreads();
+        }
+        if (base + t < nq) off[base + t] = carry + sc[t] - v;
+        __syncthreads();
+        if (t == 1023) carry += sc[1023];
+        __syncthreads();
+    }
+    if (t == 0) off[nq] = carry;
+}
+
+__global__ void compact_probes_kernel(const uint32_t* __restrict__ pid, const float* __restrict__ pd,
+                                      int stride, const uint32_t* __restrict__ off, int nq,
+                                      uint32_t* __restrict__ out_ids, float* __restrict__ out_d) {
+    const int q = blockIdx.x;
+    if (q >= nq) return;
+    const uint32_t a = off[q], n = off[q + 1] - a;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        out_ids[a + i] = pid[(size_t)q * stride + i];
+        out_d[a + i] = pd[(size_t)q * stride + i];
+    }
+}
+
 }  // namespace
+
+hipError_t launch_adaptive_probe(const float* prob, const float* q_rot, const float* centroids,
+                                 int nq, int K, int D, float mass, int min_np, int max_np,
+                                 float min_conf, const uint32_t* fb_ids, const float* fb_d,
+                                 int fb_np, uint32_t* pad_ids, float* pad_d, uint32_t* len,
+                                 uint32_t* off, uint32_t* csr_ids, float* csr_d,
+                                 uint32_t* n_fallback, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    if (max_np < 1 || max_np > kCapC || max_np > K || fb_np > max_np || min_np > max_np)
+        return hipErrorInvalidValue;
+    size_t lds = 4 * (size_t)K + 8 * (size_t)kCapC + 4 * (size_t)D;
+    hipLaunchKernelGGL(adaptive_select_kernel, dim3(nq), dim3(kThreads), lds, s, prob, q_rot,
+                       centroids, K, D, mass, min_np, max_np, min_conf, fb_ids, fb_d, fb_np,
+                       pad_ids, pad_d, len, n_fallback);
+    hipLaunchKernelGGL(lens_to_offsets_kernel, dim3(1), dim3(1024), 0, s, len, nq, off);
+    hipLaunchKernelGGL(compact_probes_kernel, dim3(nq), dim3(64), 0, s, pad_ids, pad_d, max_np,
+                       off, nq, csr_ids, csr_d);
+    return hipGetLastError();
+}
 
 size_t probe_select_lds(int K, int D) { return 4 * (size_t)K + 8 * (size_t)kCapC + 4 * (size_t)D; }
 

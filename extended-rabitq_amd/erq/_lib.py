@@ -40,6 +40,22 @@ SIGNATURES = {
                                      c_vp, c_vp, c_vp]),
     "erq_search_batched_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp, c_vp,
                                        c_vp]),
+    "erq_mlp_create": (c_int, [c_size, c_size, c_size, c_size, ctypes.c_uint64,
+                               ctypes.POINTER(c_vp)]),
+    "erq_mlp_free": (None, [c_vp]),
+    "erq_mlp_num_params": (c_size, [c_vp]),
+    "erq_mlp_in_dim": (c_size, [c_vp]),
+    "erq_mlp_out_dim": (c_size, [c_vp]),
+    "erq_mlp_set_params": (c_int, [c_vp, c_vp]),
+    "erq_mlp_get_params": (c_int, [c_vp, c_vp, c_vp]),
+    "erq_mlp_forward_dev": (c_int, [c_vp, c_vp, c_size, c_vp, c_vp]),
+    "erq_mlp_loss_grad_dev": (c_int, [c_vp, c_vp, c_vp, c_size, ctypes.c_float, c_vp, c_vp]),
+    "erq_mlp_adam_step_dev": (c_int, [c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, c_vp]),
+    "erq_probe_learned_dev": (c_int, [c_vp, c_vp, c_vp, c_size, ctypes.c_float, c_size, c_size,
+                                      c_size, ctypes.c_float, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "erq_search_learned_dev": (c_int, [c_vp, c_vp, c_vp, c_size, c_size, ctypes.c_float, c_size,
+                                       c_size, c_size, ctypes.c_float, c_vp, c_vp, c_vp, c_vp]),
     "erq_device_array": (c_vp, [c_vp, c_int]),
 }
 

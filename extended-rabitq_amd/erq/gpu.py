@@ -156,6 +156,33 @@ class IVF:
                                                 _ptr(d), _ptr(stats), _stream()))
         return ids, d
 
+    def probe_learned_dev(self, model, q_rot, mass, min_nprobe, max_nprobe, fallback_nprobe,
+                          min_conf):
+        """CSR probe lists chosen by the learned predictor (erq_probe_learned_dev)."""
+        import torch
+        nq = q_rot.shape[0]
+        dev = q_rot.device
+        off = torch.empty(nq + 1, dtype=torch.int32, device=dev)
+        pid = torch.empty(nq * max_nprobe, dtype=torch.int32, device=dev)
+        pd = torch.empty(nq * max_nprobe, dtype=torch.float32, device=dev)
+        nfb = torch.zeros(1, dtype=torch.int32, device=dev)
+        check(_lib.lib().erq_probe_learned_dev(self._h, model.handle, _ptr(q_rot), nq, mass,
+                                               min_nprobe, max_nprobe, fallback_nprobe, min_conf,
+                                               _ptr(off), _ptr(pid), _ptr(pd), _ptr(nfb),
+                                               _stream()))
+        return off, pid, pd, nfb
+
+    def search_learned_dev(self, model, q_rot, k, mass, min_nprobe, max_nprobe, fallback_nprobe,
+                           min_conf, stats=None):
+        import torch
+        nq = q_rot.shape[0]
+        ids = torch.empty((nq, k), dtype=torch.int32, device=q_rot.device)
+        d = torch.empty((nq, k), dtype=torch.float32, device=q_rot.device)
+        check(_lib.lib().erq_search_learned_dev(self._h, model.handle, _ptr(q_rot), nq, k, mass,
+                                                min_nprobe, max_nprobe, fallback_nprobe, min_conf,
+                                                _ptr(ids), _ptr(d), _ptr(stats), _stream()))
+        return ids, d
+
     def search_dev(self, q_rot, k, nprobe, stats=None):
         import torch
         nq = q_rot.shape[0]

@@ -6,9 +6,10 @@
  * queries, search.  Plain pointers and sizes only; every call returns ERQ_OK or an
  * error code, with a message in erq_last_error().  No exceptions cross the ABI.
  *
- * Pointer conventions: functions without a _dev suffix take host pointers; _dev
- * functions take device pointers and enqueue on `stream` (a hipStream_t; NULL means
- * the handle's own stream) and return without synchronising.
+ * Pointer conventions: functions without a _dev suffix take host pointers and run on the
+ * handle's own stream, synchronously; _dev functions take device pointers, enqueue on
+ * `stream` exactly as given (a hipStream_t; 0 is the null stream) and return without
+ * synchronising.
  * Thread safety: calls on one handle are serialised by a per-handle mutex.
  */
 #ifndef ERQ_H
@@ -118,6 +119,47 @@ int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t
 int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                            size_t nprobe, uint32_t* ids_dev, float* dists_dev,
                            uint64_t* stats_dev, void* stream);
+
+/* ---- learned probe predictor (no counterpart in the reference) ----
+ * MLP  D -> H -> H -> K with GELU, BF16 weights/activations, FP32 accumulation and FP32
+ * master weights; output p[c] = predicted probability that cluster c holds one of the
+ * query's top-k neighbours.  Parameters pack [W1 (H x D), b1, W2 (H x H), b2, W3 (K x H), b3]. */
+typedef struct erq_mlp erq_mlp;
+int erq_mlp_create(size_t in_dim, size_t hidden, size_t out_dim, size_t max_batch, uint64_t seed,
+                   erq_mlp** out);
+void erq_mlp_free(erq_mlp* m);
+size_t erq_mlp_num_params(const erq_mlp* m);
+size_t erq_mlp_in_dim(const erq_mlp* m);
+size_t erq_mlp_out_dim(const erq_mlp* m);
+int erq_mlp_set_params(erq_mlp* m, const float* host_params);
+/* Either output may be NULL; grads are those of the last erq_mlp_loss_grad_dev. */
+int erq_mlp_get_params(erq_mlp* m, float* host_params, float* host_grads);
+/* prob_dev = p for batch rows of x_dev (batch x in_dim), any batch size. */
+int erq_mlp_forward_dev(erq_mlp* m, const float* x_dev, size_t batch, float* prob_dev,
+                        void* stream);
+/* Forward + backward of the pos-weighted BCE (labels y_dev: batch x out_dim of 0/1, loss =
+ * batch mean of the per-query sum); gradients stay in the model; batch % 32 == 0. */
+int erq_mlp_loss_grad_dev(erq_mlp* m, const float* x_dev, const float* y_dev, size_t batch,
+                          float pos_weight, float* loss_dev, void* stream);
+int erq_mlp_adam_step_dev(erq_mlp* m, float lr, float beta1, float beta2, float eps,
+                          void* stream);
+
+/* Learned probe lists (CSR: probe_off nq+1, ids/sqr_y up to nq*max_nprobe): each query keeps
+ * its most probable clusters until their predicted mass reaches `mass` of the total (between
+ * min_nprobe and max_nprobe), ordered by exact centroid distance; a query whose total
+ * predicted mass is below min_conf, or whose mass does not reach the target within
+ * max_nprobe, takes the fixed list of its fallback_nprobe nearest clusters.  n_fallback_dev
+ * (one u32) is incremented per fallback query. */
+int erq_probe_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq, float mass,
+                          size_t min_nprobe, size_t max_nprobe, size_t fallback_nprobe,
+                          float min_conf, uint32_t* probe_off_dev, uint32_t* probe_ids_dev,
+                          float* probe_sqr_y_dev, uint32_t* n_fallback_dev, void* stream);
+
+/* erq_probe_learned_dev followed by erq_scan_batched_dev; stats counter [7] counts fallbacks. */
+int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq, size_t k,
+                           float mass, size_t min_nprobe, size_t max_nprobe,
+                           size_t fallback_nprobe, float min_conf, uint32_t* ids_dev,
+                           float* dists_dev, uint64_t* stats_dev, void* stream);
 
 /* Device pointers of the resident index arrays (for tools that read the HBM layout).
  * which: 0 short blocks, 1 long codes, 2 ex factors, 3 ids, 4 rotated centroids,
