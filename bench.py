@@ -43,9 +43,13 @@ def parse():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--kmeans-iters", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--nprobes", type=str, default="8,16,24,32,48,64,96,128,192,256")
-    ap.add_argument("--sigma", type=float, default=0.3,
+    ap.add_argument("--nprobes", type=str, default="8,16,32,64,96,128,192,256,384,512,768")
+    ap.add_argument("--sigma", type=float, default=1.5,
                     help="within-component spread relative to the component spread")
+    ap.add_argument("--train-queries", type=int, default=100000)
+    ap.add_argument("--label-nprobe", type=int, default=256)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--sweep-only", action="store_true",
                     help="print the recall sweep of both paths and exit (no timing)")
     return ap.parse_args()
@@ -75,7 +79,7 @@ def build_workload(a, torch):
                             P=P, matmul=matmul, threads=16, chunk=131072)
     log(f"HA index built (B={a.bits}) in {time.time() - t0:.1f}s; "
         f"short {idx.short_data.nbytes / 1e6:.0f} MB, long {idx.long_code.nbytes / 1e6:.0f} MB")
-    return x, q, idx
+    return x, q, idx, (basis, centres)
 
 
 def main():
@@ -88,7 +92,7 @@ def main():
     from erq import data
     from erq.gpu import IVF
     torch.cuda.set_device(0)
-    x, q, idx = build_workload(a, torch)
+    x, q, idx, mix = build_workload(a, torch)
     g = IVF.from_index(idx)
     D, EX = idx.D, idx.EX
 
@@ -197,6 +201,12 @@ def main():
     except Exception as e:  # reported, never hidden
         log(f"batched path failed: {e!r}")
         batched = None
+    learned = None
+    if not a.no_learned:
+        try:
+            learned = measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch)
+        except Exception as e:  # reported, never hidden
+            log(f"learned path failed: {e!r}")
     best = batched if batched is not None and batched["qps"] > plain["qps"] else plain
     qps = best["qps"]
     hi, lo, th = plain["hi"], plain["lo"], plain["th"]
@@ -208,8 +218,8 @@ def main():
 
     cpu = cpu_baseline(idx, q, a, hi[0], torch)
     out = {
-        "metric": "QPS at recall@100=0.95, D1536 N1M B4 (fixed nprobe)",
-        "value": round(qps, 1),
+        "metric": "QPS at recall@100=0.95, D1536 N1M B4: learned-probe vs fixed nprobe",
+        "value": round(learned["qps"] if learned else qps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -222,7 +232,15 @@ def main():
         "data": "synthetic power-law Gaussian mixture (seeded), unit-norm",
         "config": {"workload": "embedding-shaped IVF search", "N": a.n, "D": D, "clusters": idx.K,
                    "bits": a.bits, "nq": a.nq, "k": a.k,
-                   "path": "batched" if best is batched else "plain"},
+                   "path": "learned-probe batched" if learned else
+                   ("batched" if best is batched else "plain"),
+                   "sigma": a.sigma},
+        "fixed_nprobe_qps_at_target": round(qps, 1),
+        "learned": None if learned is None else {
+            "qps_at_target": round(learned["qps"], 1),
+            "vs_fixed": round(learned["qps"] / qps, 3),
+            "mass_bracket": learned["bracket"], "train": learned["train"],
+            "sweep": learned["sweep"]},
         "paths": {name: None if r is None else {
             "qps_at_target": round(r["qps"], 1),
             "nprobe_bracket": [r["lo"][0] if r["lo"] else None, r["hi"][0]],
@@ -241,6 +259,84 @@ def main():
     }
     print(json.dumps(out), flush=True)
     g.close()
+
+
+def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch):
+    """Train the probe predictor on labels from fixed-nprobe search over separate training
+    queries, then sweep the mass threshold and report QPS at the recall target."""
+    from erq import data
+    from erq.probe_model import ProbeMLP, cluster_of_ids
+    basis, centres = mix
+    t0 = time.time()
+    qt, _ = data.mixture_powerlaw_torch(a.train_queries, a.dim, a.clusters, a.sigma, seed=5,
+                                        alpha=1.0, centres=centres, basis=basis)
+    qt_r = g.rotate_dev(qt)
+    del qt
+    c_of = torch.from_numpy(cluster_of_ids(idx)).cuda()
+    lab = torch.empty((a.train_queries, a.k), dtype=torch.int32, device="cuda")
+    for s0 in range(0, a.train_queries, 20000):
+        ids, _ = g.search_batched_dev(qt_r[s0:s0 + 20000], a.k, min(a.label_nprobe, idx.K))
+        lab[s0:s0 + 20000] = c_of[ids.long()]
+    torch.cuda.synchronize()
+    t_lab = time.time() - t0
+    log(f"labels from fixed nprobe={a.label_nprobe} over {a.train_queries} queries in {t_lab:.1f}s")
+    t0 = time.time()
+    m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=1024, seed=7)
+    losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=1e-3, pos_weight=8.0, log=log)
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    del qt_r, lab
+    max_np = min(2048, idx.K, 4 * a.label_nprobe)
+
+    def lstep(mass, timed=False):
+        qr = g.rotate_dev(q)
+        return g.search_learned_dev(m, qr, a.k, mass, 4, max_np, min(a.label_nprobe, max_np),
+                                    0.5, stats=stats if timed else None)
+
+    sweep = []
+    for mass in (0.5, 0.7, 0.8, 0.9, 0.95, 0.97, 0.98, 0.99, 0.995, 0.998):
+        ids, _ = lstep(mass)
+        torch.cuda.synchronize()
+        rec = data.recall_at_k(ids.cpu().numpy().view(np.uint32), gt, a.k)
+        sweep.append((mass, rec))
+        log(f"[learned] mass={mass} recall@{a.k}={rec:.4f}")
+        if rec >= a.target:
+            break
+    above = [s for s in sweep if s[1] >= a.target]
+    below = [s for s in sweep if s[1] < a.target]
+    if not above:
+        raise RuntimeError("learned path did not reach the recall target")
+
+    def timed(mass):
+        for _ in range(a.warmup):
+            lstep(mass)
+        torch.cuda.synchronize()
+        stats.zero_()
+        t1 = time.perf_counter()
+        for _ in range(a.steps):
+            lstep(mass, timed=True)
+        torch.cuda.synchronize()
+        st = (stats.cpu().numpy() // a.steps).tolist()
+        return (time.perf_counter() - t1) * 1e3 / a.steps, st
+
+    hi = above[0]
+    ms_hi, st_hi = timed(hi[0])
+    qps = a.nq / (ms_hi / 1e3)
+    lo = below[-1] if below else None
+    if lo is not None:
+        ms_lo, _ = timed(lo[0])
+        qlo = a.nq / (ms_lo / 1e3)
+        f = (a.target - lo[1]) / (hi[1] - lo[1])
+        qps = qlo + f * (qps - qlo)
+    log(f"[learned] qps at target {qps:.0f} (mass {lo[0] if lo else None}..{hi[0]}), "
+        f"stats {st_hi}")
+    m.close()
+    return dict(qps=qps, bracket=[lo[0] if lo else None, hi[0]],
+                sweep=[[x, round(r, 4)] for x, r in sweep],
+                train={"queries": a.train_queries, "label_nprobe": a.label_nprobe,
+                       "epochs": a.epochs, "loss": [round(x, 3) for x in losses],
+                       "label_s": round(t_lab, 1), "train_s": round(t_train, 1),
+                       "fallbacks_at_hi": st_hi[7]})
 
 
 def cpu_baseline(idx, q, a, nprobe, torch):

@@ -357,6 +357,8 @@ int erq_probe_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe
                         pick(h, stream));
 }
 
+static int ensure_bp(erq_index* h, hipStream_t s);
+
 static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                        const uint32_t* off, const uint32_t* pid, const float* pd, uint32_t* ids,
                        float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0) {
@@ -382,11 +384,32 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
     a.nq = (int)nq;
     a.K = (int)h->K;
     a.max_probes = max_probes;
+    a.bp = nullptr;
+    a.nchunk = bp_nchunk((int)h->D);
+    if (h->plain_kernel != 1 && h->D <= 2048) {  // auto or MFMA: bit-plane integer path
+        int rc = ensure_bp(h, s);
+        if (rc) return rc;
+        a.bp = h->bp;
+    }
     // HASearcher.hpp:56
     a.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
     hipError_t e = launch_scan(a, s);
     if (e != hipSuccess) return hip_fail(e, "scan");
     return ERQ_OK;
+}
+
+int erq_set_option(erq_index* h, int option, long value) {
+    if (!h) {
+        set_error("erq_set_option: null handle");
+        return ERQ_EINVAL;
+    }
+    if (option == ERQ_OPT_PLAIN_KERNEL && value >= 0 && value <= 2) {
+        std::lock_guard<std::mutex> lk(h->mu);
+        h->plain_kernel = (int)value;
+        return ERQ_OK;
+    }
+    set_error("erq_set_option: unknown option or value");
+    return ERQ_EINVAL;
 }
 
 int erq_scan_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,

@@ -31,6 +31,7 @@ struct erq_index {
     size_t ws_bytes = 0;
     uint8_t* bp = nullptr;  // bit-plane copy of the 1-bit codes (batched path, built on first use)
     int num_cu = 0;
+    int plain_kernel = 0;   // ERQ_OPT_PLAIN_KERNEL
     std::mutex mu;
 };
 
@@ -58,6 +59,8 @@ struct ScanArgs {
     unsigned long long* stats;
     int D, EX, k, nq, K;
     int max_probes;  // > 0: scan at most this many leading probes per query
+    const uint8_t* bp;  // bit-plane codes (nullptr: LUT kernel)
+    int nchunk;         // 512-dim chunks of bp
     float one_over_sqrtD;
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);

@@ -295,9 +295,228 @@ __global__ void __launch_bounds__(64) scan_kernel(ScanArgs a, int per_wave_bytes
     }
 }
 
+// Same replay as scan_kernel, for D <= 2048 (no u16 wrap), with the FastScan integer
+// taken from the bit-plane codes on v_mfma_i32_16x16x64_i8: A rows 0/1 hold the query's
+// low/high int8 digits (q = 128*hi + lo), rows 2..15 are zero; B is bit-plane p of 16
+// vectors' code bytes (one v_and per dword); I = sum_p +-(acc_p >> p) equals the LUT sum
+// plus shift exactly.  Prep (normalize, quantize), lower bound, mask, survivor re-rank
+// and pool are the per-query kernel's.
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+template <int EX>
+__global__ void __launch_bounds__(64) scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int qi = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (qi >= a.nq) return;
+    const int D = a.D, k = a.k, nchunk = a.nchunk;
+    const int kcap = (k + 63) & ~63;
+    unsigned char* base = smem + (size_t)wid * per_wave_bytes;
+    float* uq = reinterpret_cast<float*>(base);
+    int8_t* A = reinterpret_cast<int8_t*>(base + 4 * (size_t)D);          // nchunk * 1 KB
+    int* sums = reinterpret_cast<int*>(base + 4 * (size_t)D + 1024 * (size_t)nchunk);
+    WaveLds w;
+    w.uq = uq;
+    w.lut = nullptr;
+    w.sums = sums;
+    w.pd = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(sums) + 128);
+    w.pid = reinterpret_cast<uint32_t*>(w.pd + kcap);
+    for (int i = lane; i < kcap; i += kWave) {
+        w.pd[i] = FLT_MAX;
+        w.pid[i] = 0;
+    }
+    int psize = 0;
+    wave_sync();
+
+    const float* q = a.q_rot + (size_t)qi * D;
+    const size_t BB = 4 * (size_t)D + 128;
+    const size_t CL = (size_t)D * EX / 8;
+    const int R = 1 << EX;
+    const float const_bound = 0.58f;  // FastScan.hpp:295
+    unsigned long long st_blocks = 0, st_vec = 0, st_surv = 0;
+    const int g = lane >> 4, l16 = lane & 15;
+
+    const uint32_t p0 = a.probe_off[qi];
+    uint32_t p1 = a.probe_off[qi + 1];
+    if (a.max_probes > 0 && p1 - p0 > (uint32_t)a.max_probes) p1 = p0 + (uint32_t)a.max_probes;
+    for (uint32_t p = p0; p < p1; ++p) {
+        const uint32_t cid = a.probe_ids[p];
+        if (cid >= (uint32_t)a.K) continue;
+        const float sqr_y = a.probe_sqr_y[p];
+        const float y = sqrtf(sqr_y);
+        const float* c = a.centroids + (size_t)cid * D;
+        float sumq;
+        if (y > 1e-5f) {
+            const float fac = (float)(1.0 / (double)y);
+            for (int d = lane; d < D; d += kWave) uq[d] = (q[d] - c[d]) * fac;
+            wave_sync();
+            float part = 0.f;
+            if (lane < 16)
+                for (int d = lane; d < D; d += 16) part = part + uq[d];
+            part = butterfly16(part);
+            sumq = __shfl(part, 0);
+        } else {
+            const float value = (float)(1.0 / (double)sqrtf((float)D));
+            for (int d = lane; d < D; d += kWave) uq[d] = value;
+            wave_sync();
+            sumq = (float)D * value;
+        }
+        float mx = 0.f, mn = 0.f;
+        for (int d = lane; d < D; d += kWave) {
+            float u = uq[d];
+            mx = fmaxf(mx, u);
+            mn = fminf(mn, u);
+        }
+        mx = wave_max(mx);
+        mn = wave_min(mn);
+        const float delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
+        // int8 digits in MFMA order: [chunk][plane p][lane group][row 0/1][16 bytes]
+        for (int d = lane; d < nchunk * 512; d += kWave) {
+            int qv = 0;
+            if (d < D) {
+                float tmp = uq[d] / delta;
+                double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
+                qv = (int)r;
+            }
+            const int ch = d >> 9, dd = d & 511;
+            const int gg = dd >> 7, jw = (dd >> 5) & 3, bb = (dd >> 3) & 3, pp = dd & 7;
+            int8_t* dst = A + ((((size_t)ch * 8 + pp) * 4 + gg) * 2) * 16 + 4 * jw + bb;
+            const int lo = qv & 127;
+            dst[0] = (int8_t)lo;
+            dst[16] = (int8_t)((qv - lo) >> 7);
+        }
+        wave_sync();
+
+        const int n = (int)a.csize[cid];
+        const int nb = (n + 31) >> 5;
+        const uint32_t blk0 = a.bstart[cid];
+        const uint32_t v0 = a.vstart[cid];
+        float distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+        const float c1 = (float)(0.5 * (double)sumq - (double)const_bound);
+        const float c2 = (-5.0f * y) * a.one_over_sqrtD;
+        const float qs = y * y;
+        for (int b = 0; b < nb; ++b) {
+            const uint32_t blk = blk0 + (uint32_t)b;
+            const int np = min(32, n - b * 32);
+            v4i acc0[8], acc1[8];
+            for (int pp = 0; pp < 8; pp++) acc0[pp] = acc1[pp] = v4i{0, 0, 0, 0};
+            for (int ch = 0; ch < nchunk; ch++) {
+                const uint8_t* src = a.bp + (((size_t)blk * nchunk + ch) * 32 + l16) * 64 + 16 * g;
+                const uint4 w0 = *reinterpret_cast<const uint4*>(src);
+                const uint4 w1 = *reinterpret_cast<const uint4*>(src + 16 * 64);
+                for (int pp = 0; pp < 8; pp++) {
+                    v4i av = v4i{0, 0, 0, 0};
+                    if (l16 < 2)
+                        av = *reinterpret_cast<const v4i*>(
+                            A + ((((size_t)ch * 8 + pp) * 4 + g) * 2 + l16) * 16);
+                    const uint32_t m = 0x01010101u << pp;
+                    const v4i b0 = v4i{(int)(w0.x & m), (int)(w0.y & m), (int)(w0.z & m),
+                                       (int)(w0.w & m)};
+                    const v4i b1 = v4i{(int)(w1.x & m), (int)(w1.y & m), (int)(w1.z & m),
+                                       (int)(w1.w & m)};
+                    acc0[pp] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b0, acc0[pp], 0, 0, 0);
+                    acc1[pp] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b1, acc1[pp], 0, 0, 0);
+                }
+            }
+            // rows 0 (low digits) and 1 (high digits) of column l16 live in lanes 0..15
+            if (lane < 16) {
+                int lo0 = -(acc0[7][0] >> 7), hi0 = -(acc0[7][1] >> 7);
+                int lo1 = -(acc1[7][0] >> 7), hi1 = -(acc1[7][1] >> 7);
+                for (int pp = 0; pp < 7; pp++) {
+                    lo0 += acc0[pp][0] >> pp;
+                    hi0 += acc0[pp][1] >> pp;
+                    lo1 += acc1[pp][0] >> pp;
+                    hi1 += acc1[pp][1] >> pp;
+                }
+                sums[lane] = 128 * hi0 + lo0;
+                sums[16 + lane] = 128 * hi1 + lo1;
+            }
+            wave_sync();
+            const float* onorm = reinterpret_cast<const float*>(a.short_data + (size_t)blk * BB +
+                                                                4 * (size_t)D);
+            bool cand = false;
+            float ipq_l = 0.f;
+            if (lane < 32) {
+                ipq_l = (float)sums[lane] * delta;
+                const float on = onorm[lane];
+                float t = ipq_l - c1;
+                t = t * c2;
+                t = t * on;
+                t = t + qs;
+                t = t + on * on;
+                cand = (lane < np) && (t < distk);
+            }
+            uint64_t mask = ballot(cand);
+            st_blocks++;
+            st_vec += np;
+            while (mask) {
+                uint64_t mm = mask;
+                for (int r = 0; r < g; r++) mm &= mm - 1;
+                const int j = mm ? (__ffsll((unsigned long long)mm) - 1) : -1;
+                const int jj = j < 0 ? 0 : j;
+                const uint32_t vi = v0 + (uint32_t)(b * 32 + jj);
+                float part = 0.f;
+                if (j >= 0) {
+                    const uint8_t* code = a.long_code + (size_t)vi * CL;
+                    for (int ch = 0; ch < (D >> 6); ch++) {
+                        float e[4];
+                        decode4<EX>(code + ch * 8 * EX, l16, e);
+                        for (int s4 = 0; s4 < 4; s4++)
+                            part = fmaf(uq[64 * ch + 16 * s4 + l16], e[s4], part);
+                    }
+                }
+                const float ip = butterfly16(part);
+                const float ipq_j = __shfl(ipq_l, jj);
+                float exd = 0.f;
+                uint32_t idv = 0;
+                if (j >= 0) {
+                    const float on = onorm[jj];
+                    const float sqr_x = on * on;
+                    const float Aa = sqr_x + sqr_y;
+                    const float Bb = a.ex_factor[vi] * y;
+                    const float Cc = (float)R * ipq_j + ip;
+                    const double G = (double)Cc - ((double)R - 0.5) * (double)sumq;
+                    const double H = (double)Bb * G;
+                    exd = (float)((double)Aa - H);
+                    idv = a.ids[vi];
+                }
+                const int cnt = min(4, __popcll(mask));
+                for (int gg = 0; gg < cnt; gg++) {
+                    const float dd = __shfl(exd, 16 * gg);
+                    const uint32_t ii = (uint32_t)__shfl((int)idv, 16 * gg);
+                    pool_insert(w, psize, k, lane, dd, ii);
+                }
+                for (int r = 0; r < cnt; r++) mask &= mask - 1;
+                st_surv += cnt;
+            }
+            distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+        }
+    }
+    uint32_t* oi = a.out_ids + (size_t)qi * k;
+    float* od = a.out_dists ? a.out_dists + (size_t)qi * k : nullptr;
+    for (int i = lane; i < k; i += kWave) {
+        oi[i] = i < psize ? w.pid[i] : 0u;
+        if (od) od[i] = i < psize ? w.pd[i] : FLT_MAX;
+    }
+    if (a.stats && lane == 0) {
+        atomicAdd(&a.stats[0], st_blocks);
+        atomicAdd(&a.stats[1], st_vec);
+        atomicAdd(&a.stats[2], st_surv);
+        atomicAdd(&a.stats[3], (unsigned long long)(p1 - p0));
+    }
+}
+
 template <int EX>
 hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
     const int waves = 1;
+    if (a.bp && a.D <= 2048) {
+        const size_t per_wave = 4 * (size_t)a.D + 1024 * (size_t)a.nchunk + 128 +
+                                (size_t)8 * ((a.k + 63) & ~63);
+        hipLaunchKernelGGL((scan_mfma_kernel<EX>), dim3(a.nq), dim3(64), per_wave, s, a,
+                           (int)per_wave);
+        return hipGetLastError();
+    }
     const size_t per_wave = (size_t)12 * a.D + 128 + (size_t)8 * ((a.k + 63) & ~63);
     const size_t lds = per_wave * waves;
     dim3 grid((a.nq + waves - 1) / waves), block(64 * waves);

@@ -63,6 +63,12 @@ class IVF:
         except Exception:
             pass
 
+    PLAIN_AUTO, PLAIN_LUT, PLAIN_MFMA = 0, 1, 2
+
+    def set_plain_kernel(self, mode):
+        """Integer kernel of the per-query scan (erq_set_option ERQ_OPT_PLAIN_KERNEL)."""
+        check(_lib.lib().erq_set_option(self._h, 1, mode))
+
     # ---- header (IVF.hpp:86-90) ----
     def padded_dim(self):
         return _lib.lib().erq_padded_dim(self._h)

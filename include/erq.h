@@ -33,6 +33,16 @@ enum {
 
 typedef struct erq_index erq_index;
 
+/* Options (erq_set_option). */
+enum {
+    /* Integer kernel of the per-query scan: 0 auto (int8 MFMA over the bit-plane copy of
+     * the codes when D <= 2048, else LUT), 1 LUT, 2 MFMA where D <= 2048.  Results are
+     * identical; only speed and the extra bit-plane copy (one more 1-bit code array in
+     * HBM) differ. */
+    ERQ_OPT_PLAIN_KERNEL = 1
+};
+int erq_set_option(erq_index* h, int option, long value);
+
 /* Message of the last error on the calling thread ("" if none). */
 const char* erq_last_error(void);
 

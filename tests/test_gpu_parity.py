@@ -43,12 +43,13 @@ def _oracle_run(idx, qr, k, nprobe):
     return ids, ds, pids, pds
 
 
-def _gpu_run(idx, qr, k, nprobe):
+def _gpu_run(idx, qr, k, nprobe, mode=0):
     from erq.gpu import IVF
     torch = _torch()
     g = IVF.from_index(idx)
+    g.set_plain_kernel(mode)
     q = torch.from_numpy(qr).cuda()
-    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
     ids, d = g.search_dev(q, k, nprobe, stats=stats)
     pid, pd = g.probe_dev(q, nprobe)
     torch.cuda.synchronize()
@@ -59,17 +60,20 @@ def _gpu_run(idx, qr, k, nprobe):
 
 
 def _assert_parity(idx, q, k, nprobe, label):
+    """Both integer kernels of the per-query scan (LUT, int8 MFMA) against the oracle."""
     qr = rotate_np(idx, q)
     oid, od, opid, opd = _oracle_run(idx, qr, k, nprobe)
-    gid, gd, gpid, gpd, st = _gpu_run(idx, qr, k, nprobe)
-    print(f"[{label}] k={k} nprobe={nprobe} stats={st.tolist()}", flush=True)
-    np.testing.assert_array_equal(gpid, opid, err_msg=f"{label}: probe ids")
-    np.testing.assert_array_equal(gpd.view(np.uint32), opd.view(np.uint32),
-                                  err_msg=f"{label}: probe distances")
-    np.testing.assert_array_equal(gid, oid, err_msg=f"{label}: result ids")
-    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32),
-                                  err_msg=f"{label}: result distances")
-    assert st[3] == q.shape[0] * nprobe
+    for mode, name in ((1, "lut"), (2, "mfma")):
+        gid, gd, gpid, gpd, st = _gpu_run(idx, qr, k, nprobe, mode)
+        lab = f"{label} [{name}]"
+        print(f"[{lab}] k={k} nprobe={nprobe} stats={st.tolist()}", flush=True)
+        np.testing.assert_array_equal(gpid, opid, err_msg=f"{lab}: probe ids")
+        np.testing.assert_array_equal(gpd.view(np.uint32), opd.view(np.uint32),
+                                      err_msg=f"{lab}: probe distances")
+        np.testing.assert_array_equal(gid, oid, err_msg=f"{lab}: result ids")
+        np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32),
+                                      err_msg=f"{lab}: result distances")
+        assert st[3] == q.shape[0] * nprobe
 
 
 @pytest.mark.parametrize("bits", [3, 4, 5, 7, 8, 9])
@@ -169,7 +173,9 @@ def test_rotation_gemm_bound():
     g = IVF.from_index(idx)
     rng = np.random.default_rng(11)
     qq = rng.standard_normal((300, 960)).astype(np.float32)
-    out = g.rotate_dev(torch.from_numpy(qq).cuda()).cpu().numpy()
+    out = g.rotate_dev(torch.from_numpy(qq).cuda())
+    torch.cuda.synchronize()
+    out = out.cpu().numpy()
     qp = np.zeros((300, idx.D))
     qp[:, :960] = qq
     ref = qp @ idx.P.astype(np.float64)
@@ -187,7 +193,9 @@ def test_centroid_distance_gemm():
     idx, q = _build(("d1536", 4), 6000, 1536, 32, 4, nq=24)
     qr = rotate_np(idx, q)
     g = IVF.from_index(idx)
-    a = g.centroid_dist_dev(torch.from_numpy(qr).cuda()).cpu().numpy()
+    a = g.centroid_dist_dev(torch.from_numpy(qr).cuda())
+    torch.cuda.synchronize()
+    a = a.cpu().numpy()
     C = idx.centroids.astype(np.float64)
     Q = qr.astype(np.float64)
     ref = (Q * Q).sum(1)[:, None] + (C * C).sum(1)[None, :] - 2 * Q @ C.T
