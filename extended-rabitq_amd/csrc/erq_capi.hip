@@ -386,6 +386,7 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
     a.max_probes = max_probes;
     a.bp = nullptr;
     a.nchunk = bp_nchunk((int)h->D);
+    a.waves_per_block = h->plain_waves;
     if (h->plain_kernel != 1 && h->D <= 2048) {  // auto or MFMA: bit-plane integer path
         int rc = ensure_bp(h, s);
         if (rc) return rc;
@@ -403,13 +404,31 @@ int erq_set_option(erq_index* h, int option, long value) {
         set_error("erq_set_option: null handle");
         return ERQ_EINVAL;
     }
+    std::lock_guard<std::mutex> lk(h->mu);
     if (option == ERQ_OPT_PLAIN_KERNEL && value >= 0 && value <= 2) {
-        std::lock_guard<std::mutex> lk(h->mu);
         h->plain_kernel = (int)value;
+        return ERQ_OK;
+    }
+    if (option == ERQ_OPT_PLAIN_WAVES && value >= 1 && value <= 4) {
+        h->plain_waves = (int)value;
+        return ERQ_OK;
+    }
+    if (option == ERQ_OPT_BATCH_WG_PER_CU && value >= 1 && value <= 8) {
+        h->batch_wg_per_cu = (int)value;
         return ERQ_OK;
     }
     set_error("erq_set_option: unknown option or value");
     return ERQ_EINVAL;
+}
+
+long erq_get_option(const erq_index* h, int option) {
+    if (!h) return -1;
+    switch (option) {
+    case ERQ_OPT_PLAIN_KERNEL: return h->plain_kernel;
+    case ERQ_OPT_PLAIN_WAVES: return h->plain_waves;
+    case ERQ_OPT_BATCH_WG_PER_CU: return h->batch_wg_per_cu;
+    default: return -1;
+    }
 }
 
 int erq_scan_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
@@ -555,7 +574,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     ba.nchunk = bp_nchunk((int)h->D);
     ba.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
     RoundBufs rb{pid, counts, cursor, ids, (int)h->K};
-    const int grid = std::max(1, h->num_cu) * 2;
+    const int grid = std::max(1, h->num_cu) * std::max(1, h->batch_wg_per_cu);
     for (size_t r0 = 1, r1 = 4; r0 < max_nprobe; r0 = r1, r1 *= 4) {
         e = launch_round(ba, rb, (int)r0, (int)std::min(r1, max_nprobe), grid, s);
         if (e != hipSuccess) return hip_fail(e, "batched round");
@@ -812,6 +831,72 @@ int erq_rotate(erq_index* h, const float* q, size_t nq, float* q_rot) {
     }
     if (dq) (void)hipFree(dq);
     if (dr) (void)hipFree(dr);
+    return rc;
+}
+
+// Times the result-invariant launch shapes on the caller's queries (HIP events on the
+// caller's stream) and keeps the fastest: waves (queries) per workgroup of the per-query
+// scan, and workgroups per CU of the persistent batched kernel.
+int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
+                 float* best_ms_out, void* stream) {
+    if (!h || !q_rot_dev || nq == 0) {
+        set_error("erq_autotune: null argument or empty batch");
+        return ERQ_EINVAL;
+    }
+    hipStream_t s = pick(h, stream);
+    uint32_t* ids = nullptr;
+    float* d = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&ids, nq * k * 4);
+    if (e == hipSuccess) e = hipMalloc(&d, nq * k * 4);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int rc = e == hipSuccess ? ERQ_OK : hip_fail(e, "erq_autotune setup");
+    auto time_it = [&](bool batched, float* ms) -> int {
+        int r = batched ? erq_search_batched_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s)
+                        : erq_search_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s);
+        if (r) return r;
+        hipError_t x = hipEventRecord(e0, s);
+        if (x == hipSuccess) {
+            r = batched ? erq_search_batched_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s)
+                        : erq_search_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s);
+            if (r) return r;
+            x = hipEventRecord(e1, s);
+        }
+        if (x == hipSuccess) x = hipEventSynchronize(e1);
+        if (x == hipSuccess) x = hipEventElapsedTime(ms, e0, e1);
+        return x == hipSuccess ? ERQ_OK : hip_fail(x, "erq_autotune timing");
+    };
+    float best_plain = 0, best_batch = 0;
+    int bw = h->plain_waves, bg = h->batch_wg_per_cu;
+    for (int wv = 1; wv <= 4 && !rc; wv *= 2) {
+        h->plain_waves = wv;
+        float ms;
+        rc = time_it(false, &ms);
+        if (!rc && (best_plain == 0 || ms < best_plain)) {
+            best_plain = ms;
+            bw = wv;
+        }
+    }
+    h->plain_waves = bw;
+    for (int wg = 1; wg <= 4 && !rc; wg++) {
+        h->batch_wg_per_cu = wg;
+        float ms;
+        rc = time_it(true, &ms);
+        if (!rc && (best_batch == 0 || ms < best_batch)) {
+            best_batch = ms;
+            bg = wg;
+        }
+    }
+    h->batch_wg_per_cu = bg;
+    if (best_ms_out) {
+        best_ms_out[0] = best_plain;
+        best_ms_out[1] = best_batch;
+    }
+    if (ids) (void)hipFree(ids);
+    if (d) (void)hipFree(d);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     return rc;
 }
 

@@ -32,6 +32,8 @@ struct erq_index {
     uint8_t* bp = nullptr;  // bit-plane copy of the 1-bit codes (batched path, built on first use)
     int num_cu = 0;
     int plain_kernel = 0;   // ERQ_OPT_PLAIN_KERNEL
+    int plain_waves = 1;    // ERQ_OPT_PLAIN_WAVES
+    int batch_wg_per_cu = 2;  // ERQ_OPT_BATCH_WG_PER_CU
     std::mutex mu;
 };
 
@@ -61,6 +63,7 @@ struct ScanArgs {
     int max_probes;  // > 0: scan at most this many leading probes per query
     const uint8_t* bp;  // bit-plane codes (nullptr: LUT kernel)
     int nchunk;         // 512-dim chunks of bp
+    int waves_per_block;  // queries (waves) per workgroup
     float one_over_sqrtD;
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);

@@ -67,7 +67,7 @@ __device__ __forceinline__ void pool_insert(const WaveLds& w, int& psize, int k,
 }
 
 template <int EX, bool WRAP>
-__global__ void __launch_bounds__(64) scan_kernel(ScanArgs a, int per_wave_bytes) {
+__global__ void __launch_bounds__(256) scan_kernel(ScanArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(64) scan_kernel(ScanArgs a, int per_wave_bytes
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 template <int EX>
-__global__ void __launch_bounds__(64) scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
+__global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -509,15 +509,18 @@ __global__ void __launch_bounds__(64) scan_mfma_kernel(ScanArgs a, int per_wave_
 
 template <int EX>
 hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
-    const int waves = 1;
+    int waves = a.waves_per_block > 0 ? a.waves_per_block : 1;
     if (a.bp && a.D <= 2048) {
         const size_t per_wave = 4 * (size_t)a.D + 1024 * (size_t)a.nchunk + 128 +
                                 (size_t)8 * ((a.k + 63) & ~63);
-        hipLaunchKernelGGL((scan_mfma_kernel<EX>), dim3(a.nq), dim3(64), per_wave, s, a,
+        while (waves > 1 && per_wave * waves > 65536) waves--;
+        dim3 grid((a.nq + waves - 1) / waves), block(64 * waves);
+        hipLaunchKernelGGL((scan_mfma_kernel<EX>), grid, block, per_wave * waves, s, a,
                            (int)per_wave);
         return hipGetLastError();
     }
     const size_t per_wave = (size_t)12 * a.D + 128 + (size_t)8 * ((a.k + 63) & ~63);
+    while (waves > 1 && per_wave * waves > 65536) waves--;
     const size_t lds = per_wave * waves;
     dim3 grid((a.nq + waves - 1) / waves), block(64 * waves);
     if (a.D > 2048)

@@ -23,6 +23,8 @@ SIGNATURES = {
                                  c_vp, c_vp, ctypes.POINTER(c_vp)]),
     "erq_index_free": (None, [c_vp]),
     "erq_set_option": (c_int, [c_vp, c_int, ctypes.c_long]),
+    "erq_get_option": (ctypes.c_long, [c_vp, c_int]),
+    "erq_autotune": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp]),
     "erq_padded_dim": (c_size, [c_vp]),
     "erq_num_clusters": (c_size, [c_vp]),
     "erq_dim": (c_size, [c_vp]),

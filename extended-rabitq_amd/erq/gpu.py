@@ -69,6 +69,19 @@ class IVF:
         """Integer kernel of the per-query scan (erq_set_option ERQ_OPT_PLAIN_KERNEL)."""
         check(_lib.lib().erq_set_option(self._h, 1, mode))
 
+    def option(self, opt):
+        return _lib.lib().erq_get_option(self._h, opt)
+
+    def set_option(self, opt, value):
+        check(_lib.lib().erq_set_option(self._h, opt, value))
+
+    def autotune(self, q_rot, k, nprobe):
+        """erq_autotune: returns (best per-query ms, best batched ms) and keeps the shapes."""
+        out = np.zeros(2, np.float32)
+        check(_lib.lib().erq_autotune(self._h, _ptr(q_rot), q_rot.shape[0], k, nprobe, _ptr(out),
+                                      _stream()))
+        return float(out[0]), float(out[1])
+
     # ---- header (IVF.hpp:86-90) ----
     def padded_dim(self):
         return _lib.lib().erq_padded_dim(self._h)

@@ -39,9 +39,20 @@ enum {
      * the codes when D <= 2048, else LUT), 1 LUT, 2 MFMA where D <= 2048.  Results are
      * identical; only speed and the extra bit-plane copy (one more 1-bit code array in
      * HBM) differ. */
-    ERQ_OPT_PLAIN_KERNEL = 1
+    ERQ_OPT_PLAIN_KERNEL = 1,
+    /* Queries (waves) per workgroup of the per-query scan, 1..4 (LDS permitting). */
+    ERQ_OPT_PLAIN_WAVES = 2,
+    /* Workgroups per CU of the persistent batched kernel, 1..8. */
+    ERQ_OPT_BATCH_WG_PER_CU = 3
 };
 int erq_set_option(erq_index* h, int option, long value);
+long erq_get_option(const erq_index* h, int option);
+
+/* Autotuner: times the result-invariant launch shapes (ERQ_OPT_PLAIN_WAVES,
+ * ERQ_OPT_BATCH_WG_PER_CU) on these queries with HIP events and keeps the fastest;
+ * best_ms_out (nullable, 2 floats) receives the best per-query and batched times. */
+int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
+                 float* best_ms_out, void* stream);
 
 /* Message of the last error on the calling thread ("" if none). */
 const char* erq_last_error(void);

@@ -124,3 +124,47 @@ def test_batched_recall_close_to_plain():
         print(f"nprobe={nprobe} recall plain={rp:.4f} batched={rb:.4f}")
         assert rb >= rp - 0.005
     g.close()
+
+
+def test_batched_search_in_hip_graph_replays_match_eager():
+    """The whole batched search (probe GEMM + selection + rounds + merges) captured in a
+    HIP graph; every replay, with eager work in between, equals the eager result."""
+    import torch
+    from erq.gpu import IVF
+    idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
+    qr = torch.from_numpy(rotate_np(idx, q)).cuda()
+    g = IVF.from_index(idx)
+    ei, ed = g.search_batched_dev(qr, 100, 24)        # eager; also sizes the workspace
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        gi, gd = g.search_batched_dev(qr, 100, 24)
+    for rep in range(4):
+        gi.fill_(-1)
+        gd.fill_(0)
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(gi, ei) and torch.equal(gd, ed), f"replay {rep}"
+        xi, xd = g.search_batched_dev(qr, 100, 24)    # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(xi, ei)
+    g.close()
+
+
+def test_autotuner_keeps_results():
+    import torch
+    from erq.gpu import IVF
+    idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
+    qr = torch.from_numpy(rotate_np(idx, q)).cuda()
+    g = IVF.from_index(idx)
+    pi, pd = g.search_dev(qr, 100, 16)
+    bi, bd = g.search_batched_dev(qr, 100, 16)
+    ms = g.autotune(qr, 100, 16)
+    print("autotune best ms", ms, "waves", g.option(2), "wg/cu", g.option(3))
+    assert 1 <= g.option(2) <= 4 and 1 <= g.option(3) <= 4
+    pi2, pd2 = g.search_dev(qr, 100, 16)
+    bi2, bd2 = g.search_batched_dev(qr, 100, 16)
+    torch.cuda.synchronize()
+    assert torch.equal(pi, pi2) and torch.equal(pd, pd2)
+    assert torch.equal(bi, bi2) and torch.equal(bd, bd2)
+    g.close()
