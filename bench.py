@@ -43,13 +43,16 @@ def parse():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--kmeans-iters", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--nprobes", type=str, default="8,16,32,64,96,128,192,256,384,512,768")
-    ap.add_argument("--sigma", type=float, default=1.5,
+    ap.add_argument("--nprobes", type=str, default="8,16,32,64,96,128,160,192,224,256,320,384,512")
+    ap.add_argument("--sigma", type=float, default=1.15,
                     help="within-component spread relative to the component spread")
     ap.add_argument("--train-queries", type=int, default=100000)
-    ap.add_argument("--label-nprobe", type=int, default=256)
+    ap.add_argument("--label-nprobe", type=int, default=0,
+                    help="nprobe of the label search (0: twice the fixed path's nprobe at target)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device copy first (HBM counter calibration)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="print the recall sweep of both paths and exit (no timing)")
     return ap.parse_args()
@@ -92,6 +95,13 @@ def main():
     from erq import data
     from erq.gpu import IVF
     torch.cuda.set_device(0)
+    if a.calib_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1073741824 bytes read, 1073741824 bytes written")
+        del src, dst
     x, q, idx, mix = build_workload(a, torch)
     g = IVF.from_index(idx)
     D, EX = idx.D, idx.EX
@@ -204,7 +214,8 @@ def main():
     learned = None
     if not a.no_learned:
         try:
-            learned = measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch)
+            learned = measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch,
+                                      fixed_np=(batched or plain)["hi"][0])
         except Exception as e:  # reported, never hidden
             log(f"learned path failed: {e!r}")
     best = batched if batched is not None and batched["qps"] > plain["qps"] else plain
@@ -261,7 +272,7 @@ def main():
     g.close()
 
 
-def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch):
+def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np):
     """Train the probe predictor on labels from fixed-nprobe search over separate training
     queries, then sweep the mass threshold and report QPS at the recall target."""
     from erq import data
@@ -273,24 +284,25 @@ def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch):
     qt_r = g.rotate_dev(qt)
     del qt
     c_of = torch.from_numpy(cluster_of_ids(idx)).cuda()
+    label_np = min(a.label_nprobe or 2 * fixed_np, idx.K, 2048)
     lab = torch.empty((a.train_queries, a.k), dtype=torch.int32, device="cuda")
     for s0 in range(0, a.train_queries, 20000):
-        ids, _ = g.search_batched_dev(qt_r[s0:s0 + 20000], a.k, min(a.label_nprobe, idx.K))
+        ids, _ = g.search_batched_dev(qt_r[s0:s0 + 20000], a.k, label_np)
         lab[s0:s0 + 20000] = c_of[ids.long()]
     torch.cuda.synchronize()
     t_lab = time.time() - t0
-    log(f"labels from fixed nprobe={a.label_nprobe} over {a.train_queries} queries in {t_lab:.1f}s")
+    log(f"labels from fixed nprobe={label_np} over {a.train_queries} queries in {t_lab:.1f}s")
     t0 = time.time()
     m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=1024, seed=7)
     losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=1e-3, pos_weight=8.0, log=log)
     torch.cuda.synchronize()
     t_train = time.time() - t0
     del qt_r, lab
-    max_np = min(2048, idx.K, 4 * a.label_nprobe)
+    max_np = label_np
 
     def lstep(mass, timed=False):
         qr = g.rotate_dev(q)
-        return g.search_learned_dev(m, qr, a.k, mass, 4, max_np, min(a.label_nprobe, max_np),
+        return g.search_learned_dev(m, qr, a.k, mass, 4, max_np, min(fixed_np, max_np),
                                     0.5, stats=stats if timed else None)
 
     sweep = []
@@ -333,7 +345,8 @@ def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch):
     m.close()
     return dict(qps=qps, bracket=[lo[0] if lo else None, hi[0]],
                 sweep=[[x, round(r, 4)] for x, r in sweep],
-                train={"queries": a.train_queries, "label_nprobe": a.label_nprobe,
+                train={"queries": a.train_queries, "label_nprobe": label_np,
+                       "fallback_nprobe": fixed_np,
                        "epochs": a.epochs, "loss": [round(x, 3) for x in losses],
                        "label_s": round(t_lab, 1), "train_s": round(t_train, 1),
                        "fallbacks_at_hi": st_hi[7]})
