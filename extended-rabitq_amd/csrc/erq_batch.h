@@ -20,7 +20,10 @@ struct BatchArgs {
     const uint32_t* vstart;
     const uint32_t* items;     // 2 x n_items: cluster, (first entry << 4) | count
     const uint32_t* entries;   // probe slots grouped by cluster
-    uint32_t* n_items;         // [0] items of this round, [1] work counter
+    uint32_t* n_items;         // [0] items of this round, [1] work counter, [2] entries
+    const uint32_t* probe_ids; // per slot: cluster id
+    int8_t* adig;              // per entry: 2 * Dp int8 digits in MFMA operand order
+    void* qparams;             // per entry: query parameters
     float* pool_d;             // nq x k, ascending (FLT_MAX padded)
     uint32_t* cand_cnt;        // nq
     float* cand_d;             // nq x cand_cap
@@ -41,6 +44,7 @@ struct RoundBufs {
 
 int bp_nchunk(int D);
 size_t batch_lds_bytes(int D);
+size_t prep_wave_bytes(int D);
 hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
                            hipStream_t s);
 hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s);

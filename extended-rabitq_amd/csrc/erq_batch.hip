@@ -134,6 +134,7 @@ __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __rest
     if (t == 0) {
         n_items[0] = carry_i;
         n_items[1] = 0;  // work counter for the persistent kernel
+        n_items[2] = carry_c;  // entries (query, cluster pairs) of the round
     }
 }
 
@@ -162,31 +163,106 @@ __device__ __forceinline__ int query_of_slot(const uint32_t* off, int nq, uint32
     return lo;
 }
 
-// ---------------------------------------------------------------- batched scan
+// ---------------------------------------------------------------- per-pair preparation
+// One wave per (query, cluster) pair of the round: the reference's query preparation
+// (normalize_query16 + high_acc_quantize16, space.hpp:695-778) and the int8 digits of the
+// quantized residual in the MFMA operand order, [chunk][plane p][lane group][digit][16 B]
+// (digit 0: q mod 128, digit 1: floor(q / 128)), plus the scalars the scan needs.
 struct QParam {
     float y, sqr_y, fac, sumq, delta, c1, c2, qs, thr;
-    int q, valid;
+    int q;
 };
 
+__global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_bytes) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t e = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (e >= a.n_items[2]) return;
+    const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512;
+    float* uq = reinterpret_cast<float*>(smem + (size_t)wid * per_wave_bytes);
+    int8_t* dig = reinterpret_cast<int8_t*>(uq + D);
+    const uint32_t slot = a.entries[e];
+    const int q = query_of_slot(a.probe_off, a.nq, slot);
+    const uint32_t cid = a.probe_ids[slot];
+    const float* qv = a.q_rot + (size_t)q * D;
+    const float* cen = a.centroids + (size_t)cid * D;
+    QParam P;
+    P.q = q;
+    P.sqr_y = a.probe_sqr_y[slot];
+    P.y = sqrtf(P.sqr_y);
+    float sumq;
+    if (P.y > 1e-5f) {
+        P.fac = (float)(1.0 / (double)P.y);
+        for (int d = lane; d < D; d += 64) uq[d] = (qv[d] - cen[d]) * P.fac;
+        wave_sync();
+        float part = 0.f;
+        if (lane < 16)
+            for (int d = lane; d < D; d += 16) part = part + uq[d];
+        part = butterfly16(part);
+        sumq = __shfl(part, 0);
+    } else {
+        P.fac = 0.f;
+        const float value = (float)(1.0 / (double)sqrtf((float)D));
+        for (int d = lane; d < D; d += 64) uq[d] = value;
+        wave_sync();
+        sumq = (float)D * value;
+    }
+    float mx = 0.f, mn = 0.f;
+    for (int d = lane; d < D; d += 64) {
+        float u = uq[d];
+        mx = fmaxf(mx, u);
+        mn = fminf(mn, u);
+    }
+    mx = wave_max(mx);
+    mn = wave_min(mn);
+    const float delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
+    for (int d = lane; d < Dp; d += 64) {
+        int qi = 0;
+        if (d < D) {
+            float tmp = uq[d] / delta;
+            double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
+            qi = (int)r;
+        }
+        const int c = d >> 9, dd = d & 511;
+        const int g = dd >> 7, jw = (dd >> 5) & 3, b = (dd >> 3) & 3, p = dd & 7;
+        int8_t* dst = dig + ((((size_t)c * 8 + p) * 4 + g) * 2) * 16 + 4 * jw + b;
+        const int lo = qi & 127;
+        dst[0] = (int8_t)lo;
+        dst[16] = (int8_t)((qi - lo) >> 7);
+    }
+    wave_sync();
+    uint4* gd = reinterpret_cast<uint4*>(a.adig + (size_t)e * 2 * Dp);
+    const uint4* ld = reinterpret_cast<const uint4*>(dig);
+    for (int i = lane; i < 2 * Dp / 16; i += 64) gd[i] = ld[i];
+    if (lane == 0) {
+        P.sumq = sumq;
+        P.delta = delta;
+        P.c1 = (float)(0.5 * (double)sumq - (double)0.58f);
+        P.c2 = (-5.0f * P.y) * a.one_over_sqrtD;
+        P.qs = P.y * P.y;
+        P.thr = a.pool_d[(size_t)q * a.k + a.k - 1];
+        reinterpret_cast<QParam*>(a.qparams)[e] = P;
+    }
+}
+
+// ---------------------------------------------------------------- batched scan
 template <int EX>
 __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int D = a.D, M = D >> 2, nchunk = a.nchunk;
-    // LDS: A digits [nchunk][8 p][4 g][16 m][16 B], per-wave unit_q scratch [4][D],
+    const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512;
+    // LDS: A operands [nchunk][8 p][4 g][16 rows][16 B] (row 2qq+d = digit d of query qq),
     // survivor list, per-query parameters.
     int8_t* A = reinterpret_cast<int8_t*>(smem);
-    float* uq_all = reinterpret_cast<float*>(smem + (size_t)nchunk * 8192);
-    uint32_t* surv = reinterpret_cast<uint32_t*>(uq_all + 4 * (size_t)D);
+    uint32_t* surv = reinterpret_cast<uint32_t*>(smem + (size_t)nchunk * 8192);
     __shared__ QParam qp[kGroup];
+    __shared__ int qvalid[kGroup];
     __shared__ uint32_t s_item, s_nsurv;
     __shared__ float s_ipq[4 * 16 * kGroup];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    float* uq = uq_all + (size_t)wid * D;
     const size_t BB = 4 * (size_t)D + 128;
     const size_t CL = (size_t)D * EX / 8;
     const int R = 1 << EX;
-    const float const_bound = 0.58f;
     unsigned long long st_blocks = 0, st_surv = 0, st_cand = 0;
 
     for (;;) {
@@ -197,85 +273,22 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         const uint32_t cid = a.items[2 * item];
         const uint32_t e0 = a.items[2 * item + 1] >> 4, elen = a.items[2 * item + 1] & 15u;
         const float* cen = a.centroids + (size_t)cid * D;
-
-        // ---- per-query preparation: 2 queries per wave ----
-        for (int qq = wid; qq < kGroup; qq += 4) {
-            QParam P;
-            P.valid = qq < (int)elen;
-            if (P.valid) {
-                const uint32_t slot = a.entries[e0 + qq];
-                const int q = query_of_slot(a.probe_off, a.nq, slot);
-                const float* qv = a.q_rot + (size_t)q * D;
-                P.q = q;
-                P.sqr_y = a.probe_sqr_y[slot];
-                P.y = sqrtf(P.sqr_y);
-                float sumq;
-                if (P.y > 1e-5f) {
-                    P.fac = (float)(1.0 / (double)P.y);
-                    for (int d = lane; d < D; d += 64) uq[d] = (qv[d] - cen[d]) * P.fac;
-                    wave_sync();
-                    float part = 0.f;
-                    if (lane < 16)
-                        for (int d = lane; d < D; d += 16) part = part + uq[d];
-                    part = butterfly16(part);
-                    sumq = __shfl(part, 0);
-                } else {
-                    P.fac = 0.f;
-                    const float value = (float)(1.0 / (double)sqrtf((float)D));
-                    for (int d = lane; d < D; d += 64) uq[d] = value;
-                    wave_sync();
-                    sumq = (float)D * value;
-                }
-                float mx = 0.f, mn = 0.f;
-                for (int d = lane; d < D; d += 64) {
-                    float u = uq[d];
-                    mx = fmaxf(mx, u);
-                    mn = fminf(mn, u);
-                }
-                mx = wave_max(mx);
-                mn = wave_min(mn);
-                const float delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
-                // digits into the MFMA A layout; rows 2qq (low 7 bits) and 2qq+1 (high)
-                for (int d = lane; d < nchunk * 512; d += 64) {
-                    int qi = 0;
-                    if (d < D) {
-                        float tmp = uq[d] / delta;
-                        double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
-                        qi = (int)r;
-                    }
-                    const int c = d >> 9, dd = d & 511;
-                    const int g = dd >> 7, jw = (dd >> 5) & 3, b = (dd >> 3) & 3, p = dd & 7;
-                    const int j = 4 * jw + b;
-                    const int lo = qi & 127, hi = (qi - lo) >> 7;
-                    int8_t* base = A + ((((size_t)c * 8 + p) * 4 + g) * 16) * 16 + j;
-                    base[(2 * qq) * 16] = (int8_t)lo;
-                    base[(2 * qq + 1) * 16] = (int8_t)hi;
-                }
-                P.sumq = sumq;
-                P.delta = delta;
-                P.c1 = (float)(0.5 * (double)sumq - (double)const_bound);
-                P.c2 = (-5.0f * P.y) * a.one_over_sqrtD;
-                P.qs = P.y * P.y;
-                const float* pd = a.pool_d + (size_t)q * a.k;
-                P.thr = pd[a.k - 1];
-            } else {
-                // idle rows: zero digits so the MFMA rows contribute nothing
-                for (int d = lane; d < nchunk * 512; d += 64) {
-                    const int c = d >> 9, dd = d & 511;
-                    const int g = dd >> 7, jw = (dd >> 5) & 3, b = (dd >> 3) & 3, p = dd & 7;
-                    int8_t* base = A + ((((size_t)c * 8 + p) * 4 + g) * 16) * 16 + 4 * jw + b;
-                    base[(2 * qq) * 16] = 0;
-                    base[(2 * qq + 1) * 16] = 0;
-                }
-                P.q = 0;
-                P.thr = -FLT_MAX;
-            }
-            if (lane == 0) qp[qq] = P;
-            wave_sync();
+        if (tid < kGroup) {
+            qvalid[tid] = tid < (int)elen;
+            if (tid < (int)elen) qp[tid] = reinterpret_cast<const QParam*>(a.qparams)[e0 + tid];
+        }
+        // stage the digits: 16-byte units (c, p, g, row); rows of missing queries are 0
+        const int units = nchunk * 8 * 4 * 16;
+        for (int u = tid; u < units; u += kThreads) {
+            const int row = u & 15, cpg = u >> 4;
+            const int qq = row >> 1, dgt = row & 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (qq < (int)elen)
+                v = reinterpret_cast<const uint4*>(a.adig + (size_t)(e0 + qq) * 2 * Dp)[cpg * 2 + dgt];
+            reinterpret_cast<uint4*>(A)[u] = v;
         }
         __syncthreads();
 
-        // ---- scan: 16-vector tiles, one tile per wave per pass ----
         const int n = (int)a.csize[cid];
         const int ntile = (n + 15) >> 4;
         const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
@@ -286,7 +299,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
             const int t = tb + wid;
             if (t < ntile) {
                 const uint32_t blk = b0 + (uint32_t)(t >> 1);
-                const int vib = 16 * (t & 1) + ln;  // vector within the 32-block
+                const int vib = 16 * (t & 1) + ln;
                 v4i acc[8];
                 for (int p = 0; p < 8; p++) acc[p] = v4i{0, 0, 0, 0};
                 for (int c = 0; c < nchunk; c++) {
@@ -301,18 +314,18 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                         acc[p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[p], 0, 0, 0);
                     }
                 }
-                // rows 4*lg + r of column ln: (query 2lg, planes 0/1), (query 2lg+1, 0/1)
                 int X[4];
                 for (int r = 0; r < 4; r++) {
-                    int s = -(acc[7][r] >> 7);
-                    for (int p = 0; p < 7; p++) s += acc[p][r] >> p;
-                    X[r] = s;
+                    int sacc = -(acc[7][r] >> 7);
+                    for (int p = 0; p < 7; p++) sacc += acc[p][r] >> p;
+                    X[r] = sacc;
                 }
-                const int vloc = 16 * t + ln;  // vector within the cluster
+                const int vloc = 16 * t + ln;
                 const float on = reinterpret_cast<const float*>(
                     a.short_data + (size_t)blk * BB + 4 * (size_t)D)[vib];
                 for (int h = 0; h < 2; h++) {
                     const int qq = 2 * lg + h;
+                    if (!qvalid[qq]) continue;
                     const QParam& P = qp[qq];
                     const int I = 128 * X[2 * h + 1] + X[2 * h];
                     const float ipq = (float)I * P.delta;
@@ -321,23 +334,22 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                     tl = tl * on;
                     tl = tl + P.qs;
                     tl = tl + on * on;
-                    if (P.valid && vloc < n && tl < P.thr) {
-                        uint32_t s = atomicAdd(&s_nsurv, 1u);
-                        surv[s] = ((uint32_t)qq << 16) | (uint32_t)vloc;
-                        s_ipq[s] = ipq;
+                    if (vloc < n && tl < P.thr) {
+                        uint32_t sidx = atomicAdd(&s_nsurv, 1u);
+                        surv[sidx] = ((uint32_t)qq << 16) | (uint32_t)vloc;
+                        s_ipq[sidx] = ipq;
                     }
                 }
                 if (lane == 0) st_blocks++;
             }
             __syncthreads();
 
-            // ---- re-rank survivors: 16 lanes each, reference-ordered FP32 chains ----
             const int ns = (int)s_nsurv;
             const int grp = tid >> 4, l16 = tid & 15;
             for (int sb = 0; sb < ns; sb += kThreads / 16) {
                 const int si = sb + grp;
                 const bool act = si < ns;
-                uint32_t sv = act ? surv[si] : 0u;
+                const uint32_t sv = act ? surv[si] : 0u;
                 const int qq = (int)(sv >> 16), vloc = (int)(sv & 0xFFFFu);
                 const QParam& P = qp[qq];
                 const uint32_t vi = v0 + (uint32_t)vloc;
@@ -369,10 +381,10 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                     const double H = (double)Bb * G;
                     const float exd = (float)((double)Aa - H);
                     if (exd <= P.thr) {
-                        uint32_t slot = atomicAdd(&a.cand_cnt[P.q], 1u);
-                        if (slot < (uint32_t)a.cand_cap) {
-                            a.cand_d[(size_t)P.q * a.cand_cap + slot] = exd;
-                            a.cand_i[(size_t)P.q * a.cand_cap + slot] = a.ids[vi];
+                        uint32_t cs = atomicAdd(&a.cand_cnt[P.q], 1u);
+                        if (cs < (uint32_t)a.cand_cap) {
+                            a.cand_d[(size_t)P.q * a.cand_cap + cs] = exd;
+                            a.cand_i[(size_t)P.q * a.cand_cap + cs] = a.ids[vi];
                         }
                         st_cand++;
                     }
@@ -381,7 +393,6 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
             }
             __syncthreads();
         }
-        (void)M;
     }
     if (a.stats && lane == 0) {
         atomicAdd(&a.stats[6], st_blocks);
@@ -522,7 +533,9 @@ hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uin
     return hipGetLastError();
 }
 
-size_t batch_lds_bytes(int D) { return (size_t)bp_nchunk(D) * 8192 + 16 * (size_t)D + 4 * 1024; }
+size_t batch_lds_bytes(int D) { return (size_t)bp_nchunk(D) * 8192 + 4 * 1024; }
+
+size_t prep_wave_bytes(int D) { return 4 * (size_t)D + 2 * (size_t)bp_nchunk(D) * 512; }
 
 hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
@@ -545,6 +558,15 @@ hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1
                        rb.cursor, (uint32_t*)ba.entries);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    {
+        const int wpb = 4;
+        const size_t pw = prep_wave_bytes(ba.D);
+        const size_t max_pairs = (size_t)nq * (size_t)(r1 - r0);
+        dim3 pg((unsigned)((max_pairs + wpb - 1) / wpb)), pb(64 * wpb);
+        hipLaunchKernelGGL(prep_kernel, pg, pb, pw * wpb, s, ba, (int)pw);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     const size_t lds = batch_lds_bytes(ba.D);
     switch (ba.EX) {
     case 2: hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid), dim3(kThreads), lds, s, ba); break;

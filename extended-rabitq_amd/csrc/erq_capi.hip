@@ -516,68 +516,97 @@ static int ensure_bp(erq_index* h, hipStream_t s) {
     return ERQ_OK;
 }
 
-static size_t batch_ws_bytes(size_t nq, size_t K, size_t max_nprobe, int cap) {
-    const size_t pairs = nq * max_nprobe;
-    return 6 * 256 + align_up(K * 4, 256) * 2 + align_up(8, 256) +
+// Round schedule of the batched search: [0,1) exact replay, then [1,4), [4,16), [16,64),
+// and from there rounds of at most 128 ranks.
+static size_t next_round(size_t r0) { return std::min(4 * r0, r0 + 128); }
+
+constexpr size_t kPairBudget = 1u << 20;  // (query, cluster) pairs prepared per round
+
+static size_t batch_chunk_nq(size_t nq, size_t max_nprobe) {
+    size_t width = 1;
+    for (size_t r0 = 1; r0 < max_nprobe; r0 = next_round(r0))
+        width = std::max(width, std::min(next_round(r0), max_nprobe) - r0);
+    return std::max<size_t>(1, std::min(nq, kPairBudget / width));
+}
+
+static size_t batch_ws_bytes(size_t nq, size_t K, size_t max_nprobe, int cap, size_t D) {
+    const size_t cq = batch_chunk_nq(nq, max_nprobe);
+    const size_t pairs = std::min(cq * std::max<size_t>(max_nprobe, 1), kPairBudget) + 8;
+    const size_t Dp = (size_t)bp_nchunk((int)D) * 512;
+    return 10 * 256 + align_up(K * 4, 256) * 2 + align_up(16, 256) +
            align_up((pairs / 8 + K + 8) * 8, 256) + align_up(pairs * 4 + 4, 256) +
-           align_up(nq * 4, 256) + align_up(nq * (size_t)cap * 4, 256) * 2;
+           align_up(cq * 4, 256) + align_up(cq * (size_t)cap * 4, 256) * 2 +
+           align_up(pairs * 2 * Dp, 256) + align_up(pairs * 48, 256);
 }
 
 static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                           const uint32_t* off, const uint32_t* pid, const float* pd,
                           size_t max_nprobe, uint32_t* ids, float* dists, uint64_t* stats,
                           hipStream_t s, char* ws_base, size_t ws_off) {
+    // D > 2048: the int8 MFMA integer would not reproduce the reference's u16 wrap, so the
+    // batched entry points run the exact per-query replay over every probe instead.
+    if (h->D > 2048) return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s);
     int rc = ensure_bp(h, s);
     if (rc) return rc;
     // round 0: the exact per-query scan of each query's nearest cluster
     rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s, 1);
     if (rc || max_nprobe <= 1) return rc;
     const int cap = kCandCap;
-    const size_t pairs = nq * max_nprobe;
+    const size_t cq = batch_chunk_nq(nq, max_nprobe);
+    const size_t pairs = std::min(cq * max_nprobe, kPairBudget) + 8;
+    const size_t Dp = (size_t)bp_nchunk((int)h->D) * 512;
     Ws ws{ws_base, ws_off};
     uint32_t* counts = ws.take<uint32_t>(h->K);
     uint32_t* cursor = ws.take<uint32_t>(h->K);
-    uint32_t* n_items = ws.take<uint32_t>(2);
+    uint32_t* n_items = ws.take<uint32_t>(4);
     uint32_t* items = ws.take<uint32_t>((pairs / 8 + h->K + 8) * 2);
     uint32_t* entries = ws.take<uint32_t>(pairs + 1);
-    uint32_t* cand_cnt = ws.take<uint32_t>(nq);
-    float* cand_d = ws.take<float>(nq * (size_t)cap);
-    uint32_t* cand_i = ws.take<uint32_t>(nq * (size_t)cap);
-    hipError_t e = launch_zero(cand_cnt, nq, s);
+    uint32_t* cand_cnt = ws.take<uint32_t>(cq);
+    float* cand_d = ws.take<float>(cq * (size_t)cap);
+    uint32_t* cand_i = ws.take<uint32_t>(cq * (size_t)cap);
+    int8_t* adig = ws.take<int8_t>(pairs * 2 * Dp);
+    void* qparams = ws.take<char>(pairs * 48);
+    hipError_t e = launch_zero(cand_cnt, cq, s);
     if (e != hipSuccess) return hip_fail(e, "zero candidates");
-    BatchArgs ba;
-    ba.q_rot = q_rot_dev;
-    ba.probe_off = off;
-    ba.probe_sqr_y = pd;
-    ba.centroids = h->centroids;
-    ba.short_data = h->short_data;
-    ba.bp = h->bp;
-    ba.long_code = h->long_code;
-    ba.ex_factor = h->ex_factor;
-    ba.ids = h->ids;
-    ba.csize = h->csize;
-    ba.bstart = h->bstart;
-    ba.vstart = h->vstart;
-    ba.items = items;
-    ba.entries = entries;
-    ba.n_items = n_items;
-    ba.pool_d = dists;
-    ba.cand_cnt = cand_cnt;
-    ba.cand_d = cand_d;
-    ba.cand_i = cand_i;
-    ba.stats = reinterpret_cast<unsigned long long*>(stats);
-    ba.cand_cap = cap;
-    ba.D = (int)h->D;
-    ba.EX = (int)h->EX;
-    ba.k = (int)k;
-    ba.nq = (int)nq;
-    ba.nchunk = bp_nchunk((int)h->D);
-    ba.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
-    RoundBufs rb{pid, counts, cursor, ids, (int)h->K};
     const int grid = std::max(1, h->num_cu) * std::max(1, h->batch_wg_per_cu);
-    for (size_t r0 = 1, r1 = 4; r0 < max_nprobe; r0 = r1, r1 *= 4) {
-        e = launch_round(ba, rb, (int)r0, (int)std::min(r1, max_nprobe), grid, s);
-        if (e != hipSuccess) return hip_fail(e, "batched round");
+    for (size_t q0 = 0; q0 < nq; q0 += cq) {
+        const size_t n = std::min(cq, nq - q0);
+        BatchArgs ba;
+        ba.q_rot = q_rot_dev + q0 * h->D;
+        ba.probe_off = off + q0;
+        ba.probe_sqr_y = pd;
+        ba.probe_ids = pid;
+        ba.centroids = h->centroids;
+        ba.short_data = h->short_data;
+        ba.bp = h->bp;
+        ba.long_code = h->long_code;
+        ba.ex_factor = h->ex_factor;
+        ba.ids = h->ids;
+        ba.csize = h->csize;
+        ba.bstart = h->bstart;
+        ba.vstart = h->vstart;
+        ba.items = items;
+        ba.entries = entries;
+        ba.n_items = n_items;
+        ba.adig = adig;
+        ba.qparams = qparams;
+        ba.pool_d = dists + q0 * k;
+        ba.cand_cnt = cand_cnt;
+        ba.cand_d = cand_d;
+        ba.cand_i = cand_i;
+        ba.stats = reinterpret_cast<unsigned long long*>(stats);
+        ba.cand_cap = cap;
+        ba.D = (int)h->D;
+        ba.EX = (int)h->EX;
+        ba.k = (int)k;
+        ba.nq = (int)n;
+        ba.nchunk = bp_nchunk((int)h->D);
+        ba.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
+        RoundBufs rb{pid, counts, cursor, ids + q0 * k, (int)h->K};
+        for (size_t r0 = 1; r0 < max_nprobe; r0 = next_round(r0)) {
+            e = launch_round(ba, rb, (int)r0, (int)std::min(next_round(r0), max_nprobe), grid, s);
+            if (e != hipSuccess) return hip_fail(e, "batched round");
+        }
     }
     return ERQ_OK;
 }
@@ -604,7 +633,7 @@ int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t
     hipStream_t s = pick(h, stream);
     const int cap = kCandCap;
     void* wsp;
-    int rc = workspace(h, batch_ws_bytes(nq, h->K, max_nprobe, cap), &wsp);
+    int rc = workspace(h, batch_ws_bytes(nq, h->K, max_nprobe, cap, h->D), &wsp);
     if (rc) return rc;
     return batched_locked(h, q_rot_dev, nq, k, probe_off_dev, probe_ids_dev, probe_sqr_y_dev,
                           max_nprobe, ids_dev, dists_dev, stats_dev, s, (char*)wsp, 0);
@@ -636,7 +665,7 @@ int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size
     const int cap = kCandCap;
     size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
     size_t list_part = align_up(nq * nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) + 512;
-    size_t batch_part = batch_ws_bytes(nq, K, nprobe, cap);
+    size_t batch_part = batch_ws_bytes(nq, K, nprobe, cap, h->D);
     void* wsp;
     int rc = workspace(h, probe_part + list_part + batch_part, &wsp);
     if (rc) return rc;
@@ -746,7 +775,7 @@ int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, siz
     size_t learned_part = learned_ws_bytes(nq, K, max_nprobe, fallback_nprobe);
     size_t list_part = align_up(nq * max_nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) +
                        align_up(8, 256) + 512;
-    size_t batch_part = batch_ws_bytes(nq, K, max_nprobe, cap);
+    size_t batch_part = batch_ws_bytes(nq, K, max_nprobe, cap, h->D);
     void* wsp;
     rc = workspace(h, probe_part + learned_part + list_part + batch_part, &wsp);
     if (rc) return rc;

@@ -429,7 +429,8 @@ void orc_ivf_search_probes(const orc_index* idx, const float* q_rot, size_t k, s
 
 /* Round-synchronous batched search (this project's cluster-major schedule, checked
  * against liberq's erq_scan_batched_dev): probe rank 0 exactly as IVF::search; ranks
- * [1,4), [4,16), ... in rounds whose pruning threshold is the pool's k-th distance at
+ * [1,4), [4,16), [16,64), then rounds 128 ranks wide; each round's pruning threshold is
+ * the pool's k-th distance at
  * round start; a vector is a candidate if its lower bound < threshold and its
  * estimate <= threshold; candidates are merged into the pool by (distance, id). */
 static int dist_id_cmp(const void* a, const void* b) {
@@ -457,7 +458,8 @@ void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, s
     cand* cs = (cand*)malloc(ncap * sizeof(cand));
     int R = 1 << idx->EX;
     size_t block_bytes = 4 * D + 128, code_len = D * idx->EX / 8;
-    for (size_t r0 = 1, r1 = 4; r0 < nprobe; r0 = r1, r1 *= 4) {
+    for (size_t r0 = 1, r1; r0 < nprobe; r0 = r1) {
+        r1 = 4 * r0 < r0 + 128 ? 4 * r0 : r0 + 128; /* [1,4) [4,16) [16,64) then 128 wide */
         float thr = orc_pool_distk(&pool);
         nc = 0;
         for (size_t i = 0; i < pool.size; i++) {
