@@ -72,9 +72,14 @@ class ProbeMLP:
         check(_lib.lib().erq_mlp_adam_step_dev(self._h, lr, beta1, beta2, eps, _stream()))
 
     def train(self, x, label_clusters, epochs=3, batch=512, lr=1e-3, pos_weight=4.0, seed=0,
-              log=None):
+              log=None, calibrate=True):
         """x: [n, D] CUDA float tensor; label_clusters: [n, k] CUDA int tensor of the cluster
-        of each of the query's top-k results (-1 = none).  Returns the per-epoch mean loss."""
+        of each of the query's top-k results (-1 = none).  Returns the per-epoch mean loss.
+
+        The positive-weighted loss is minimised by sigmoid(z) = w p / (w p + 1 - p), i.e.
+        z = ln w + logit p.  With calibrate, ln w is taken off the output bias afterwards so
+        that the network's outputs estimate p itself, which the probability-mass criterion
+        of erq_probe_learned_dev assumes."""
         import torch
         n = x.shape[0]
         g = torch.Generator(device=x.device)
@@ -92,7 +97,17 @@ class ProbeMLP:
             losses.append(float(tot) / max(cnt, 1))
             if log:
                 log(f"probe MLP epoch {ep}: loss {losses[-1]:.4f}")
+        if calibrate and pos_weight != 1.0:
+            self.shift_output_bias(-float(np.log(pos_weight)))
         return losses
+
+    def shift_output_bias(self, delta):
+        """Adds delta to every output logit (the last out_dim parameters are the output bias)."""
+        import torch
+        torch.cuda.synchronize()   # training steps run on the caller's stream
+        theta = self.get_params()
+        theta[-self.out_dim:] += np.float32(delta)
+        self.set_params(theta)
 
 
 def labels_dense(label_clusters, K):

@@ -68,3 +68,25 @@ def test_bf16_gradient_tracks_fp64_gradient():
     assert abs(L - mo.loss(theta, x, y, din, hid, dout, 3.0)) < 0.02 * abs(L)
     cos = g @ g64 / np.linalg.norm(g) / np.linalg.norm(g64)
     assert cos > 0.999
+
+
+def test_pos_weighted_bce_optimum_is_shifted_logit():
+    """For a constant logit z and label rate p, the pos-weighted BCE
+    -w p log s(z) - (1-p) log(1-s(z)) is minimised at z = ln w + logit p; the learned
+    probe predictor takes ln w off its output bias on that basis."""
+    din, hid, dout = 4, 4, 1
+    n = hid * din + hid + hid * hid + hid + dout * hid + dout
+    x = np.zeros((10, din))
+    y = np.zeros((10, dout))
+    y[:3] = 1.0                                    # p = 0.3
+    w = 5.0
+    zs = np.linspace(-3, 4, 70001)
+    theta = np.zeros(n)
+    best = None
+    for z in zs[::100]:
+        theta[-1] = z
+        L = mo.loss(theta, x, y, din, hid, dout, w)
+        if best is None or L < best[0]:
+            best = (L, z)
+    expect = math.log(w) + math.log(0.3 / 0.7)
+    assert abs(best[1] - expect) < 0.011, (best, expect)
