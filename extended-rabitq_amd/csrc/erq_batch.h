@@ -18,9 +18,9 @@ struct BatchArgs {
     const uint32_t* csize;
     const uint32_t* bstart;
     const uint32_t* vstart;
-    const uint32_t* items;     // 2 x n_items: cluster, (first entry << 4) | count
+    const uint32_t* items;     // 3 x n_items: cluster, (first entry << 4) | count, first tile
     const uint32_t* entries;   // probe slots grouped by cluster
-    uint32_t* n_items;         // [0] items of this round, [1] work counter, [2] entries
+    uint32_t* n_items;         // [0] items of this round, [2] entries
     const uint32_t* probe_ids; // per slot: cluster id
     int8_t* adig;              // per entry: 2 * Dp int8 digits in MFMA operand order
     void* qparams;             // per entry: query parameters
@@ -28,6 +28,10 @@ struct BatchArgs {
     uint32_t* cand_cnt;        // nq
     float* cand_d;             // nq x cand_cap
     uint32_t* cand_i;          // nq x cand_cap
+    uint4* surv;               // survivor records: entry, cluster, vector, FastScan product,
+                               // one slice of surv_cap / grid records per scan workgroup
+    uint32_t* surv_count;      // per scan workgroup: records in its slice
+    size_t surv_cap;
     unsigned long long* stats; // 8 counters (see erq.h)
     int cand_cap;
     int D, EX, k, nq, nchunk;
@@ -45,6 +49,8 @@ struct RoundBufs {
 int bp_nchunk(int D);
 size_t batch_lds_bytes(int D);
 size_t prep_wave_bytes(int D);
+// Upper bound on the work items of one round of at most `pairs` (query, cluster) pairs.
+size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize);
 hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
                            hipStream_t s);
 hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s);

@@ -34,6 +34,8 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kGroup = 8;      // queries per work item (16 MFMA rows = 8 x 2 planes)
 constexpr int kThreads = 256;  // 4 waves
+constexpr int kItemTiles = 16; // 16-vector tiles per work item (4 per wave)
+constexpr int kQSurv = kItemTiles * 16;  // survivors of one query in one work item, at most
 
 // ---------------------------------------------------------------- relayout
 // One thread per output byte: (block, chunk, vector, byte B).
@@ -92,37 +94,52 @@ __global__ void count_kernel(const uint32_t* __restrict__ off, const uint32_t* _
 }
 
 // Single workgroup: exclusive scans of counts and of items per cluster, then items.
+// A work item is (cluster, up to kGroup consecutive entries, up to kItemTiles tiles of the
+// cluster), three words: cluster, (first entry << 4) | entry count, first tile.
 __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __restrict__ counts,
+                                                          const uint32_t* __restrict__ csize,
                                                           int K, uint32_t* __restrict__ cursor,
                                                           uint32_t* __restrict__ items,
                                                           uint32_t* __restrict__ n_items) {
-    __shared__ uint32_t s_cnt[1024], s_it[1024];
+    __shared__ uint32_t s_cnt[1024], s_it[1024], s_ns[1024];
     __shared__ uint32_t carry_c, carry_i;
     const int t = threadIdx.x;
     if (t == 0) carry_c = carry_i = 0;
     __syncthreads();
     for (int base = 0; base < K; base += 1024) {
-        int c = base + t;
-        uint32_t cnt = c < K ? counts[c] : 0;
-        uint32_t it = (cnt + kGroup - 1) / kGroup;
+        const int c = base + t;
+        const uint32_t cnt = c < K ? counts[c] : 0;
+        const uint32_t ntile = c < K ? (csize[c] + 15) / 16 : 0;
+        const uint32_t nsplit = (ntile + kItemTiles - 1) / kItemTiles;  // 0: empty cluster
+        const uint32_t it = (cnt + kGroup - 1) / kGroup * nsplit;
         s_cnt[t] = cnt;
         s_it[t] = it;
+        s_ns[t] = nsplit;
         __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+        for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scans
             uint32_t a = t >= o ? s_cnt[t - o] : 0, b = t >= o ? s_it[t - o] : 0;
             __syncthreads();
             s_cnt[t] += a;
             s_it[t] += b;
             __syncthreads();
         }
-        uint32_t ex_c = carry_c + s_cnt[t] - cnt, ex_i = carry_i + s_it[t] - it;
-        if (c < K) {
-            cursor[c] = ex_c;
-            for (uint32_t j = 0; j < it; j++) {
-                uint32_t len = min((uint32_t)kGroup, cnt - j * kGroup);
-                items[2 * (ex_i + j)] = (uint32_t)c;
-                items[2 * (ex_i + j) + 1] = ((ex_c + j * kGroup) << 4) | len;
+        if (c < K) cursor[c] = carry_c + s_cnt[t] - cnt;
+        // items of this chunk of clusters, one thread per item
+        const uint32_t nit = s_it[1023];
+        for (uint32_t i = t; i < nit; i += 1024) {
+            int lo = 0, hi = 1023;  // first cluster slot whose inclusive item count exceeds i
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_it[mid] > i) hi = mid; else lo = mid + 1;
             }
+            const uint32_t before_it = lo ? s_it[lo - 1] : 0, before_c = lo ? s_cnt[lo - 1] : 0;
+            const uint32_t ccnt = s_cnt[lo] - before_c, ns = s_ns[lo];
+            const uint32_t j = i - before_it, grp = j / ns, sp = j % ns;
+            const uint32_t w = carry_i + i;
+            items[3 * w] = (uint32_t)(base + lo);
+            items[3 * w + 1] = ((carry_c + before_c + grp * kGroup) << 4) |
+                               min((uint32_t)kGroup, ccnt - grp * kGroup);
+            items[3 * w + 2] = sp * kItemTiles;
         }
         __syncthreads();
         if (t == 1023) {
@@ -133,7 +150,6 @@ __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __rest
     }
     if (t == 0) {
         n_items[0] = carry_i;
-        n_items[1] = 0;  // work counter for the persistent kernel
         n_items[2] = carry_c;  // entries (query, cluster pairs) of the round
     }
 }
@@ -176,7 +192,11 @@ struct QParam {
 __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint32_t e = blockIdx.x * (blockDim.x >> 6) + wid;
+    // XCD-aware: workgroup b runs on XCD b % 8; entries are grouped by cluster, so giving
+    // each XCD a contiguous eighth of them keeps a centroid's reads in one L2.
+    uint32_t blk = blockIdx.x;
+    if ((gridDim.x & 7) == 0) blk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t e = blk * (blockDim.x >> 6) + wid;
     if (e >= a.n_items[2]) return;
     const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512;
     float* uq = reinterpret_cast<float*>(smem + (size_t)wid * per_wave_bytes);
@@ -193,11 +213,14 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
     float sumq;
     if (P.y > 1e-5f) {
         P.fac = (float)(1.0 / (double)P.y);
+#pragma unroll 4
         for (int d = lane; d < D; d += 64) uq[d] = (qv[d] - cen[d]) * P.fac;
         wave_sync();
         float part = 0.f;
-        if (lane < 16)
+        if (lane < 16) {
+#pragma unroll 16
             for (int d = lane; d < D; d += 16) part = part + uq[d];
+        }
         part = butterfly16(part);
         sumq = __shfl(part, 0);
     } else {
@@ -208,6 +231,7 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
         sumq = (float)D * value;
     }
     float mx = 0.f, mn = 0.f;
+#pragma unroll 4
     for (int d = lane; d < D; d += 64) {
         float u = uq[d];
         mx = fmaxf(mx, u);
@@ -216,6 +240,7 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
     mx = wave_max(mx);
     mn = wave_min(mn);
     const float delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
+#pragma unroll 4
     for (int d = lane; d < Dp; d += 64) {
         int qi = 0;
         if (d < D) {
@@ -246,40 +271,117 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
 }
 
 // ---------------------------------------------------------------- batched scan
-template <int EX>
+// One workgroup per work item.  Each wave loads the codes of its (up to 4) tiles of the
+// item while the item's digits are staged in LDS, runs the bit-plane MFMAs, and appends
+// every vector whose lower bound is below a query's threshold to that query's LDS list.
+// The lists are then written to the round's survivor list (one record per (query,
+// vector): entry, cluster, vector, scaled FastScan product) for reest_kernel.  If that list
+// is full, the workgroup re-estimates its survivors itself (same formulas, same result).
+
+// The EX-bit re-estimate of one survivor by a 16-lane group, in the reference's order:
+// 16 lane-strided FMA chains over u * code, reduced by the xor butterfly, then the distance
+// in double as C promotes it (HASearcher.hpp:183-231).  u(d) comes from `ures(d)`.
+template <int EX, typename U>
+__device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P, const U& ures,
+                                            const uint8_t* code, int l16, bool act, float ipq,
+                                            float on, uint32_t vi) {
+    float part = 0.f;
+    if (act) {
+#pragma unroll 4
+        for (int ch = 0; ch < (a.D >> 6); ch++) {
+            float e[4];
+            decode4<EX>(code + ch * 8 * EX, l16, e);
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) part = fmaf(ures(64 * ch + 16 * s4 + l16), e[s4], part);
+        }
+    }
+    const float ip = butterfly16(part);
+    if (!act) return FLT_MAX;
+    const float R = (float)(1 << EX);
+    const float sqr_x = on * on;
+    const float Aa = sqr_x + P.sqr_y;
+    const float Bb = a.ex_factor[vi] * P.y;
+    const float Cc = R * ipq + ip;
+    const double G = (double)Cc - ((double)R - 0.5) * (double)P.sumq;
+    const double H = (double)Bb * G;
+    return (float)((double)Aa - H);
+}
+
+__device__ __forceinline__ void add_candidate(const BatchArgs& a, int q, float exd, uint32_t id) {
+    const uint32_t cs = atomicAdd(&a.cand_cnt[q], 1u);
+    if (cs < (uint32_t)a.cand_cap) {
+        a.cand_d[(size_t)q * a.cand_cap + cs] = exd;
+        a.cand_i[(size_t)q * a.cand_cap + cs] = id;
+    }
+}
+
+template <int NCH>
+__device__ __forceinline__ void load_tile_codes(const uint8_t* bp, uint32_t blk, int vib, int lg,
+                                                uint4 (&w)[NCH]) {
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+        w[c] = *reinterpret_cast<const uint4*>(bp + (((size_t)blk * NCH + c) * 32 + vib) * 64 +
+                                               16 * lg);
+}
+
+template <int EX, int NCH>
 __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512;
-    // LDS: A operands [nchunk][8 p][4 g][16 rows][16 B] (row 2qq+d = digit d of query qq),
-    // survivor list, per-query parameters.
+    constexpr int Dp = NCH * 512;
+    constexpr int kWaveTiles = kItemTiles / 4;
+    const int D = a.D;
+    // LDS: A operands [NCH][8 p][4 g][16 rows][16 B] (row 2qq+d = digit d of query qq),
+    // then per-query survivor lists (vector, scaled FastScan product).  When the
+    // workgroup re-estimates survivors itself, the A region holds each wave's residual
+    // (NCH * 8192 >= 4 waves * D floats).
     int8_t* A = reinterpret_cast<int8_t*>(smem);
-    uint32_t* surv = reinterpret_cast<uint32_t*>(smem + (size_t)nchunk * 8192);
+    uint32_t* s_vec = reinterpret_cast<uint32_t*>(smem + (size_t)NCH * 8192);
+    float* s_ipq = reinterpret_cast<float*>(s_vec + kGroup * kQSurv);
     __shared__ QParam qp[kGroup];
-    __shared__ int qvalid[kGroup];
-    __shared__ uint32_t s_item, s_nsurv;
-    __shared__ float s_ipq[4 * 16 * kGroup];
+    __shared__ uint32_t s_cnt[kGroup];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int ln = lane & 15, lg = lane >> 4;
     const size_t BB = 4 * (size_t)D + 128;
     const size_t CL = (size_t)D * EX / 8;
-    const int R = 1 << EX;
     unsigned long long st_blocks = 0, st_surv = 0, st_cand = 0;
+    // this workgroup's slice of the survivor list
+    const size_t slice_cap = a.surv_cap / gridDim.x;
+    uint4* slice = a.surv + (size_t)blockIdx.x * slice_cap;
+    size_t used = 0;
+    const uint32_t nitems = a.n_items[0];
 
-    for (;;) {
-        if (tid == 0) s_item = atomicAdd(&a.n_items[1], 1u);
-        __syncthreads();
-        const uint32_t item = s_item;
-        if (item >= a.n_items[0]) break;
-        const uint32_t cid = a.items[2 * item];
-        const uint32_t e0 = a.items[2 * item + 1] >> 4, elen = a.items[2 * item + 1] & 15u;
-        const float* cen = a.centroids + (size_t)cid * D;
+    // XCD-aware order: workgroup b runs on XCD b % 8, so each XCD takes a contiguous eighth
+    // of the items (clusters in order) and the items of one cluster share that XCD's L2.
+    uint32_t it_lo = 0, it_hi = nitems, it_step = gridDim.x, it_first = blockIdx.x;
+    if ((gridDim.x & 7) == 0) {
+        const uint32_t x = blockIdx.x & 7;
+        it_lo = (uint32_t)((uint64_t)nitems * x / 8);
+        it_hi = (uint32_t)((uint64_t)nitems * (x + 1) / 8);
+        it_step = gridDim.x >> 3;
+        it_first = it_lo + (blockIdx.x >> 3);
+    }
+    for (uint32_t item = it_first; item < it_hi; item += it_step) {
+        __syncthreads();  // the previous item's LDS is no longer read
+        const uint32_t cid = a.items[3 * item];
+        const uint32_t e0 = a.items[3 * item + 1] >> 4, elen = a.items[3 * item + 1] & 15u;
+        const int tfirst = (int)a.items[3 * item + 2];
+        const int n = (int)a.csize[cid];
+        const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
+        const int tend = min((n + 15) >> 4, tfirst + kItemTiles);
+        // this wave's tiles: tfirst + wid + 4j, j < kWaveTiles; codes loaded before staging
+        uint4 w[kWaveTiles][NCH];
+#pragma unroll
+        for (int j = 0; j < kWaveTiles; j++) {
+            const int t = tfirst + wid + 4 * j;
+            if (t < tend) load_tile_codes<NCH>(a.bp, b0 + (uint32_t)(t >> 1), 16 * (t & 1) + ln, lg, w[j]);
+        }
         if (tid < kGroup) {
-            qvalid[tid] = tid < (int)elen;
+            s_cnt[tid] = 0;
             if (tid < (int)elen) qp[tid] = reinterpret_cast<const QParam*>(a.qparams)[e0 + tid];
         }
         // stage the digits: 16-byte units (c, p, g, row); rows of missing queries are 0
-        const int units = nchunk * 8 * 4 * 16;
-        for (int u = tid; u < units; u += kThreads) {
+        for (int u = tid; u < NCH * 512; u += kThreads) {
             const int row = u & 15, cpg = u >> 4;
             const int qq = row >> 1, dgt = row & 1;
             uint4 v = make_uint4(0, 0, 0, 0);
@@ -289,113 +391,210 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         }
         __syncthreads();
 
-        const int n = (int)a.csize[cid];
-        const int ntile = (n + 15) >> 4;
-        const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
-        const int ln = lane & 15, lg = lane >> 4;
-        for (int tb = 0; tb < ntile; tb += 4) {
-            if (tid == 0) s_nsurv = 0;
-            __syncthreads();
-            const int t = tb + wid;
-            if (t < ntile) {
-                const uint32_t blk = b0 + (uint32_t)(t >> 1);
-                const int vib = 16 * (t & 1) + ln;
-                v4i acc[8];
-                for (int p = 0; p < 8; p++) acc[p] = v4i{0, 0, 0, 0};
-                for (int c = 0; c < nchunk; c++) {
-                    const uint4 w = *reinterpret_cast<const uint4*>(
-                        a.bp + (((size_t)blk * nchunk + c) * 32 + vib) * 64 + 16 * lg);
-                    for (int p = 0; p < 8; p++) {
-                        const uint32_t m = 0x01010101u << p;
-                        v4i bv = v4i{(int)(w.x & m), (int)(w.y & m), (int)(w.z & m),
-                                     (int)(w.w & m)};
-                        const v4i av = *reinterpret_cast<const v4i*>(
-                            A + ((((size_t)c * 8 + p) * 4 + lg) * 16 + ln) * 16);
-                        acc[p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[p], 0, 0, 0);
-                    }
+#pragma unroll
+        for (int j = 0; j < kWaveTiles; j++) {
+            const int t = tfirst + wid + 4 * j;
+            if (t >= tend) break;
+            const uint32_t blk = b0 + (uint32_t)(t >> 1);
+            const int vib = 16 * (t & 1) + ln;
+            v4i acc[8];
+#pragma unroll
+            for (int p = 0; p < 8; p++) acc[p] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+#pragma unroll
+                for (int p = 0; p < 8; p++) {
+                    const uint32_t m = 0x01010101u << p;
+                    v4i bv = v4i{(int)(w[j][c].x & m), (int)(w[j][c].y & m), (int)(w[j][c].z & m),
+                                 (int)(w[j][c].w & m)};
+                    const v4i av = *reinterpret_cast<const v4i*>(
+                        A + (((c * 8 + p) * 4 + lg) * 16 + ln) * 16);
+                    acc[p] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[p], 0, 0, 0);
                 }
-                int X[4];
-                for (int r = 0; r < 4; r++) {
-                    int sacc = -(acc[7][r] >> 7);
-                    for (int p = 0; p < 7; p++) sacc += acc[p][r] >> p;
-                    X[r] = sacc;
-                }
-                const int vloc = 16 * t + ln;
-                const float on = reinterpret_cast<const float*>(
-                    a.short_data + (size_t)blk * BB + 4 * (size_t)D)[vib];
-                for (int h = 0; h < 2; h++) {
-                    const int qq = 2 * lg + h;
-                    if (!qvalid[qq]) continue;
-                    const QParam& P = qp[qq];
-                    const int I = 128 * X[2 * h + 1] + X[2 * h];
-                    const float ipq = (float)I * P.delta;
-                    float tl = ipq - P.c1;
-                    tl = tl * P.c2;
-                    tl = tl * on;
-                    tl = tl + P.qs;
-                    tl = tl + on * on;
-                    if (vloc < n && tl < P.thr) {
-                        uint32_t sidx = atomicAdd(&s_nsurv, 1u);
-                        surv[sidx] = ((uint32_t)qq << 16) | (uint32_t)vloc;
-                        s_ipq[sidx] = ipq;
-                    }
-                }
-                if (lane == 0) st_blocks++;
             }
-            __syncthreads();
-
-            const int ns = (int)s_nsurv;
-            const int grp = tid >> 4, l16 = tid & 15;
-            for (int sb = 0; sb < ns; sb += kThreads / 16) {
-                const int si = sb + grp;
-                const bool act = si < ns;
-                const uint32_t sv = act ? surv[si] : 0u;
-                const int qq = (int)(sv >> 16), vloc = (int)(sv & 0xFFFFu);
+            int X[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int sacc = -(acc[7][r] >> 7);
+#pragma unroll
+                for (int p = 0; p < 7; p++) sacc += acc[p][r] >> p;
+                X[r] = sacc;
+            }
+            const int vloc = 16 * t + ln;
+            const float on = reinterpret_cast<const float*>(
+                a.short_data + (size_t)blk * BB + 4 * (size_t)D)[vib];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int qq = 2 * lg + h;
+                if (qq >= (int)elen) continue;
                 const QParam& P = qp[qq];
-                const uint32_t vi = v0 + (uint32_t)vloc;
-                float part = 0.f;
-                if (act) {
-                    const float* qv = a.q_rot + (size_t)P.q * D;
-                    const uint8_t* code = a.long_code + (size_t)vi * CL;
-                    const float value = (float)(1.0 / (double)sqrtf((float)D));
-                    for (int ch = 0; ch < (D >> 6); ch++) {
-                        float e[4];
-                        decode4<EX>(code + ch * 8 * EX, l16, e);
-                        for (int s4 = 0; s4 < 4; s4++) {
-                            const int d = 64 * ch + 16 * s4 + l16;
-                            const float u = P.y > 1e-5f ? (qv[d] - cen[d]) * P.fac : value;
-                            part = fmaf(u, e[s4], part);
-                        }
-                    }
+                const int I = 128 * X[2 * h + 1] + X[2 * h];
+                const float ipq = (float)I * P.delta;
+                float tl = ipq - P.c1;
+                tl = tl * P.c2;
+                tl = tl * on;
+                tl = tl + P.qs;
+                tl = tl + on * on;
+                if (vloc < n && tl < P.thr) {
+                    const uint32_t sidx = atomicAdd(&s_cnt[qq], 1u);
+                    s_vec[qq * kQSurv + sidx] = (uint32_t)vloc;
+                    s_ipq[qq * kQSurv + sidx] = ipq;
                 }
-                const float ip = butterfly16(part);
-                if (act && l16 == 0) {
-                    const uint32_t blk = b0 + (uint32_t)(vloc >> 5);
-                    const float on = reinterpret_cast<const float*>(
-                        a.short_data + (size_t)blk * BB + 4 * (size_t)D)[vloc & 31];
-                    const float sqr_x = on * on;
-                    const float Aa = sqr_x + P.sqr_y;
-                    const float Bb = a.ex_factor[vi] * P.y;
-                    const float Cc = (float)R * s_ipq[si] + ip;
-                    const double G = (double)Cc - ((double)R - 0.5) * (double)P.sumq;
-                    const double H = (double)Bb * G;
-                    const float exd = (float)((double)Aa - H);
+            }
+            if (lane == 0) st_blocks++;
+        }
+        __syncthreads();
+
+        int qoff[kGroup + 1];
+        qoff[0] = 0;
+#pragma unroll
+        for (int qq = 0; qq < kGroup; qq++)
+            qoff[qq + 1] = qoff[qq] + (qq < (int)elen ? (int)s_cnt[qq] : 0);
+        const int tot = qoff[kGroup];
+        if (tot == 0) continue;  // uniform: every thread read the same counts
+        if (used + (size_t)tot <= slice_cap) {
+            for (int i = tid; i < tot; i += kThreads) {
+                int qq = 0;
+#pragma unroll
+                for (int j = 1; j < kGroup; j++) qq += i >= qoff[j] ? 1 : 0;
+                const int si = qq * kQSurv + (i - qoff[qq]);
+                slice[used + i] = make_uint4(e0 + (uint32_t)qq, cid, s_vec[si],
+                                             __float_as_uint(s_ipq[si]));
+            }
+            used += (size_t)tot;
+            continue;
+        }
+        // the slice is full: re-estimate this item's survivors here
+        __syncthreads();  // every wave is done with the digits in A
+        const float value = (float)(1.0 / (double)sqrtf((float)D));
+        float* uw = reinterpret_cast<float*>(A) + (size_t)wid * D;
+        const float* cen = a.centroids + (size_t)cid * D;
+        int cstart[kGroup + 1];
+        cstart[0] = 0;
+#pragma unroll
+        for (int qq = 0; qq < kGroup; qq++)
+            cstart[qq + 1] = cstart[qq] + ((qoff[qq + 1] - qoff[qq] + 15) >> 4);
+        int staged = -1;
+        for (int ci = wid; ci < cstart[kGroup]; ci += 4) {
+            int qq = 0;
+#pragma unroll
+            for (int j = 1; j < kGroup; j++) qq += ci >= cstart[j] ? 1 : 0;
+            const QParam& P = qp[qq];
+            if (qq != staged) {
+                wave_sync();
+                const float* qv = a.q_rot + (size_t)P.q * D;
+                const bool use_res = P.y > 1e-5f;
+#pragma unroll 4
+                for (int d = lane; d < D; d += 64) uw[d] = use_res ? (qv[d] - cen[d]) * P.fac : value;
+                wave_sync();
+                staged = qq;
+            }
+            const int cnt = qoff[qq + 1] - qoff[qq];
+            const int cb = (ci - cstart[qq]) * 16;
+            for (int it = 0; it < 4 && cb + 4 * it < cnt; it++) {
+                const int si = cb + 4 * it + lg;
+                const bool act = si < cnt;
+                const int vloc = act ? (int)s_vec[qq * kQSurv + si] : 0;
+                const uint32_t vi = v0 + (uint32_t)vloc;
+                const float on = reinterpret_cast<const float*>(
+                    a.short_data + (size_t)(b0 + (uint32_t)(vloc >> 5)) * BB + 4 * (size_t)D)[vloc & 31];
+                const float exd = reestimate<EX>(
+                    a, P, [&](int d) { return uw[d]; }, a.long_code + (size_t)vi * CL, ln, act,
+                    act ? s_ipq[qq * kQSurv + si] : 0.f, on, vi);
+                if (act && ln == 0) {
                     if (exd <= P.thr) {
-                        uint32_t cs = atomicAdd(&a.cand_cnt[P.q], 1u);
-                        if (cs < (uint32_t)a.cand_cap) {
-                            a.cand_d[(size_t)P.q * a.cand_cap + cs] = exd;
-                            a.cand_i[(size_t)P.q * a.cand_cap + cs] = a.ids[vi];
-                        }
+                        add_candidate(a, P.q, exd, a.ids[vi]);
                         st_cand++;
                     }
                     st_surv++;
                 }
             }
-            __syncthreads();
+        }
+    }
+    if (tid == 0) a.surv_count[blockIdx.x] = (uint32_t)used;
+    if (a.stats && lane == 0) {
+        atomicAdd(&a.stats[6], st_blocks);
+        atomicAdd(&a.stats[2], st_surv);
+        atomicAdd(&a.stats[4], st_cand);
+    }
+}
+
+// The round's survivor list, one record per 16-lane group (four per wave), u(d) read
+// straight from the rotated query and the centroid.
+// The round's survivor list, slice by slice: kReestWG workgroups per scan workgroup's
+// slice, each wave taking a contiguous 1/(4 kReestWG) of it, four records (one per 16-lane
+// group) at a time.  Records of one (query, cluster) pair are contiguous in a slice; when
+// the four records of a step share their pair, the residual u is staged once in the
+// wave's LDS slice and kept for the next step, otherwise each group forms u from the
+// rotated query and the centroid as it goes.
+constexpr int kReestWG = 4;
+
+template <int EX>
+__global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int D = a.D;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int l16 = lane & 15, lg = lane >> 4;
+    float* uw = reinterpret_cast<float*>(smem) + (size_t)wid * D;
+    const int g = blockIdx.x / kReestWG;
+    const size_t cnt = a.surv_count[g];
+    const uint4* slice = a.surv + (size_t)g * (a.surv_cap / scan_grid);
+    const size_t nw = (size_t)kReestWG * 4;
+    const size_t wv = (size_t)(blockIdx.x % kReestWG) * 4 + wid;
+    const size_t per = ((cnt + nw - 1) / nw + 3) & ~size_t(3);
+    const size_t r_begin = min(cnt, wv * per), r_end = min(cnt, r_begin + per);
+    const size_t CL = (size_t)D * EX / 8;
+    const size_t BB = 4 * (size_t)D + 128;
+    const float value = (float)(1.0 / (double)sqrtf((float)D));
+    unsigned long long st_surv = 0, st_cand = 0;
+    uint32_t staged = 0xFFFFFFFFu;
+    for (size_t r0 = r_begin; r0 < r_end; r0 += 4) {  // wave-uniform
+        const size_t r = r0 + lg;
+        const bool act = r < r_end;
+        const uint4 rec = act ? slice[r] : make_uint4(0, 0, 0, 0);
+        const QParam P = act ? reinterpret_cast<const QParam*>(a.qparams)[rec.x] : QParam{};
+        const uint32_t cid = rec.y, vloc = rec.z;
+        const uint32_t vi = act ? a.vstart[cid] + vloc : 0;
+        const float* qv = a.q_rot + (size_t)P.q * D;
+        const float* cen = a.centroids + (size_t)cid * D;
+        const bool use_res = P.y > 1e-5f;
+        const float on = act ? reinterpret_cast<const float*>(
+                                   a.short_data + (size_t)(a.bstart[cid] + (vloc >> 5)) * BB +
+                                   4 * (size_t)D)[vloc & 31]
+                             : 0.f;
+        const uint32_t e_first = __shfl(rec.x, 0);  // group 0 is always active
+        const bool same = __all(!act || rec.x == e_first);
+        float exd;
+        if (same) {
+            if (e_first != staged) {
+                // every lane stages with group 0's pair (inactive groups hold no record)
+                const QParam P0 = reinterpret_cast<const QParam*>(a.qparams)[e_first];
+                const float* q0 = a.q_rot + (size_t)P0.q * D;
+                const float* c0 = a.centroids + (size_t)__shfl(cid, 0) * D;
+                const bool res0 = P0.y > 1e-5f;
+                wave_sync();
+#pragma unroll 4
+                for (int d = lane; d < D; d += 64) uw[d] = res0 ? (q0[d] - c0[d]) * P0.fac : value;
+                wave_sync();
+                staged = e_first;
+            }
+            exd = reestimate<EX>(a, P, [&](int d) { return uw[d]; },
+                                 a.long_code + (size_t)vi * CL, l16, act, __uint_as_float(rec.w),
+                                 on, vi);
+        } else {
+            exd = reestimate<EX>(
+                a, P, [&](int d) { return use_res ? (qv[d] - cen[d]) * P.fac : value; },
+                a.long_code + (size_t)vi * CL, l16, act, __uint_as_float(rec.w), on, vi);
+        }
+        if (act && l16 == 0) {
+            if (exd <= P.thr) {
+                add_candidate(a, P.q, exd, a.ids[vi]);
+                st_cand++;
+            }
+            st_surv++;
         }
     }
     if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[6], st_blocks);
         atomicAdd(&a.stats[2], st_surv);
         atomicAdd(&a.stats[4], st_cand);
     }
@@ -533,7 +732,13 @@ hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uin
     return hipGetLastError();
 }
 
-size_t batch_lds_bytes(int D) { return (size_t)bp_nchunk(D) * 8192 + 4 * 1024; }
+size_t batch_lds_bytes(int D) { return (size_t)bp_nchunk(D) * 8192 + (size_t)kGroup * kQSurv * 8; }
+
+size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize) {
+    const size_t tiles_per_item = (size_t)kItemTiles * 16;
+    const size_t max_split = (max_csize + tiles_per_item - 1) / tiles_per_item;
+    return (pairs / kGroup + 1) * max_split + N / tiles_per_item + K + 8;
+}
 
 size_t prep_wave_bytes(int D) { return 4 * (size_t)D + 2 * (size_t)bp_nchunk(D) * 512; }
 
@@ -541,6 +746,56 @@ hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+template <int EX, int NCH>
+static hipError_t launch_batch_scan_nch(const BatchArgs& ba, int grid, size_t lds, hipStream_t s) {
+    if (lds + 1024 > 65536) {  // dynamic plus static LDS beyond the 64 KB default
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&batch_scan_kernel<EX, NCH>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((batch_scan_kernel<EX, NCH>), dim3(grid), dim3(kThreads), lds, s, ba);
+    return hipGetLastError();
+}
+
+template <int EX>
+static hipError_t launch_batch_scan_ex(const BatchArgs& ba, int grid, size_t lds, hipStream_t s) {
+    switch (ba.nchunk) {
+    case 1: return launch_batch_scan_nch<EX, 1>(ba, grid, lds, s);
+    case 2: return launch_batch_scan_nch<EX, 2>(ba, grid, lds, s);
+    case 3: return launch_batch_scan_nch<EX, 3>(ba, grid, lds, s);
+    case 4: return launch_batch_scan_nch<EX, 4>(ba, grid, lds, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+static hipError_t launch_batch_scan(const BatchArgs& ba, int grid, size_t lds, hipStream_t s) {
+    switch (ba.EX) {
+    case 2: return launch_batch_scan_ex<2>(ba, grid, lds, s);
+    case 3: return launch_batch_scan_ex<3>(ba, grid, lds, s);
+    case 4: return launch_batch_scan_ex<4>(ba, grid, lds, s);
+    case 6: return launch_batch_scan_ex<6>(ba, grid, lds, s);
+    case 7: return launch_batch_scan_ex<7>(ba, grid, lds, s);
+    case 8: return launch_batch_scan_ex<8>(ba, grid, lds, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+static hipError_t launch_reest(const BatchArgs& ba, int scan_grid, hipStream_t s) {
+    const dim3 grid(scan_grid * kReestWG);
+    const size_t lds = 4 * (size_t)ba.D * 4;  // one residual per wave
+    switch (ba.EX) {
+    case 2: hipLaunchKernelGGL(reest_kernel<2>, grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 3: hipLaunchKernelGGL(reest_kernel<3>, grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 4: hipLaunchKernelGGL(reest_kernel<4>, grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 6: hipLaunchKernelGGL(reest_kernel<6>, grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 7: hipLaunchKernelGGL(reest_kernel<7>, grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 8: hipLaunchKernelGGL(reest_kernel<8>, grid, dim3(256), lds, s, ba, scan_grid); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -552,8 +807,8 @@ hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1
     dim3 qg((nq + 255) / 256), qb(256);
     hipLaunchKernelGGL(count_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq, r0, r1, K,
                        rb.counts);
-    hipLaunchKernelGGL(scan_items_kernel, dim3(1), dim3(1024), 0, s, rb.counts, K, rb.cursor,
-                       (uint32_t*)ba.items, ba.n_items);
+    hipLaunchKernelGGL(scan_items_kernel, dim3(1), dim3(1024), 0, s, rb.counts, ba.csize, K,
+                       rb.cursor, (uint32_t*)ba.items, ba.n_items);
     hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq, r0, r1, K,
                        rb.cursor, (uint32_t*)ba.entries);
     e = hipGetLastError();
@@ -568,15 +823,10 @@ hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1
         if (e != hipSuccess) return e;
     }
     const size_t lds = batch_lds_bytes(ba.D);
-    switch (ba.EX) {
-    case 2: hipLaunchKernelGGL(batch_scan_kernel<2>, dim3(grid), dim3(kThreads), lds, s, ba); break;
-    case 3: hipLaunchKernelGGL(batch_scan_kernel<3>, dim3(grid), dim3(kThreads), lds, s, ba); break;
-    case 4: hipLaunchKernelGGL(batch_scan_kernel<4>, dim3(grid), dim3(kThreads), lds, s, ba); break;
-    case 6: hipLaunchKernelGGL(batch_scan_kernel<6>, dim3(grid), dim3(kThreads), lds, s, ba); break;
-    case 7: hipLaunchKernelGGL(batch_scan_kernel<7>, dim3(grid), dim3(kThreads), lds, s, ba); break;
-    case 8: hipLaunchKernelGGL(batch_scan_kernel<8>, dim3(grid), dim3(kThreads), lds, s, ba); break;
-    default: return hipErrorInvalidValue;
-    }
+    e = launch_batch_scan(ba, grid, lds, s);
+    if (e != hipSuccess) return e;
+    e = launch_reest(ba, grid, s);
+    if (e != hipSuccess) return e;
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(merge_kernel, dim3(nq), dim3(256), 0, s, ba.pool_d,

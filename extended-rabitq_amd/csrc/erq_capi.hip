@@ -417,6 +417,10 @@ int erq_set_option(erq_index* h, int option, long value) {
         h->batch_wg_per_cu = (int)value;
         return ERQ_OK;
     }
+    if (option == ERQ_OPT_BATCH_SURVIVOR_CAP && value >= 0 && value <= (1L << 30)) {
+        h->batch_surv_cap = (size_t)value;
+        return ERQ_OK;
+    }
     set_error("erq_set_option: unknown option or value");
     return ERQ_EINVAL;
 }
@@ -427,6 +431,7 @@ long erq_get_option(const erq_index* h, int option) {
     case ERQ_OPT_PLAIN_KERNEL: return h->plain_kernel;
     case ERQ_OPT_PLAIN_WAVES: return h->plain_waves;
     case ERQ_OPT_BATCH_WG_PER_CU: return h->batch_wg_per_cu;
+    case ERQ_OPT_BATCH_SURVIVOR_CAP: return (long)h->batch_surv_cap;
     default: return -1;
     }
 }
@@ -521,6 +526,7 @@ static int ensure_bp(erq_index* h, hipStream_t s) {
 static size_t next_round(size_t r0) { return std::min(4 * r0, r0 + 128); }
 
 constexpr size_t kPairBudget = 1u << 20;  // (query, cluster) pairs prepared per round
+constexpr size_t kMaxBatchGrid = 4096;    // workgroups of the persistent batched kernel
 
 static size_t batch_chunk_nq(size_t nq, size_t max_nprobe) {
     size_t width = 1;
@@ -529,12 +535,21 @@ static size_t batch_chunk_nq(size_t nq, size_t max_nprobe) {
     return std::max<size_t>(1, std::min(nq, kPairBudget / width));
 }
 
-static size_t batch_ws_bytes(size_t nq, size_t K, size_t max_nprobe, int cap, size_t D) {
+static size_t max_cluster_size(const erq_index* h) {
+    uint64_t m = 0;
+    for (uint64_t v : h->sizes) m = std::max(m, v);
+    return (size_t)m;
+}
+
+static size_t batch_ws_bytes(const erq_index* h, size_t nq, size_t max_nprobe, int cap) {
+    const size_t K = h->K, D = h->D;
     const size_t cq = batch_chunk_nq(nq, max_nprobe);
     const size_t pairs = std::min(cq * std::max<size_t>(max_nprobe, 1), kPairBudget) + 8;
     const size_t Dp = (size_t)bp_nchunk((int)D) * 512;
-    return 10 * 256 + align_up(K * 4, 256) * 2 + align_up(16, 256) +
-           align_up((pairs / 8 + K + 8) * 8, 256) + align_up(pairs * 4 + 4, 256) +
+    const size_t nitems = batch_items_bound(pairs, K, h->N, max_cluster_size(h));
+    return 12 * 256 + align_up(K * 4, 256) * 2 + align_up(16, 256) +
+           align_up((size_t)h->batch_surv_cap * 16, 256) + align_up(kMaxBatchGrid * 4, 256) +
+           align_up(nitems * 12, 256) + align_up(pairs * 4 + 4, 256) +
            align_up(cq * 4, 256) + align_up(cq * (size_t)cap * 4, 256) * 2 +
            align_up(pairs * 2 * Dp, 256) + align_up(pairs * 48, 256);
 }
@@ -559,16 +574,19 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     uint32_t* counts = ws.take<uint32_t>(h->K);
     uint32_t* cursor = ws.take<uint32_t>(h->K);
     uint32_t* n_items = ws.take<uint32_t>(4);
-    uint32_t* items = ws.take<uint32_t>((pairs / 8 + h->K + 8) * 2);
+    uint32_t* items = ws.take<uint32_t>(batch_items_bound(pairs, h->K, h->N, max_cluster_size(h)) * 3);
     uint32_t* entries = ws.take<uint32_t>(pairs + 1);
     uint32_t* cand_cnt = ws.take<uint32_t>(cq);
     float* cand_d = ws.take<float>(cq * (size_t)cap);
     uint32_t* cand_i = ws.take<uint32_t>(cq * (size_t)cap);
     int8_t* adig = ws.take<int8_t>(pairs * 2 * Dp);
     void* qparams = ws.take<char>(pairs * 48);
+    uint4* surv = ws.take<uint4>(h->batch_surv_cap);
+    uint32_t* surv_count = ws.take<uint32_t>(kMaxBatchGrid);
     hipError_t e = launch_zero(cand_cnt, cq, s);
     if (e != hipSuccess) return hip_fail(e, "zero candidates");
-    const int grid = std::max(1, h->num_cu) * std::max(1, h->batch_wg_per_cu);
+    const int grid = (int)std::min<size_t>(
+        kMaxBatchGrid, (size_t)std::max(1, h->num_cu) * std::max(1, h->batch_wg_per_cu));
     for (size_t q0 = 0; q0 < nq; q0 += cq) {
         const size_t n = std::min(cq, nq - q0);
         BatchArgs ba;
@@ -594,6 +612,9 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.cand_cnt = cand_cnt;
         ba.cand_d = cand_d;
         ba.cand_i = cand_i;
+        ba.surv = surv;
+        ba.surv_count = surv_count;
+        ba.surv_cap = h->batch_surv_cap;
         ba.stats = reinterpret_cast<unsigned long long*>(stats);
         ba.cand_cap = cap;
         ba.D = (int)h->D;
@@ -633,7 +654,7 @@ int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t
     hipStream_t s = pick(h, stream);
     const int cap = kCandCap;
     void* wsp;
-    int rc = workspace(h, batch_ws_bytes(nq, h->K, max_nprobe, cap, h->D), &wsp);
+    int rc = workspace(h, batch_ws_bytes(h, nq, max_nprobe, cap), &wsp);
     if (rc) return rc;
     return batched_locked(h, q_rot_dev, nq, k, probe_off_dev, probe_ids_dev, probe_sqr_y_dev,
                           max_nprobe, ids_dev, dists_dev, stats_dev, s, (char*)wsp, 0);
@@ -665,7 +686,7 @@ int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size
     const int cap = kCandCap;
     size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
     size_t list_part = align_up(nq * nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) + 512;
-    size_t batch_part = batch_ws_bytes(nq, K, nprobe, cap, h->D);
+    size_t batch_part = batch_ws_bytes(h, nq, nprobe, cap);
     void* wsp;
     int rc = workspace(h, probe_part + list_part + batch_part, &wsp);
     if (rc) return rc;
@@ -775,7 +796,7 @@ int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, siz
     size_t learned_part = learned_ws_bytes(nq, K, max_nprobe, fallback_nprobe);
     size_t list_part = align_up(nq * max_nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) +
                        align_up(8, 256) + 512;
-    size_t batch_part = batch_ws_bytes(nq, K, max_nprobe, cap, h->D);
+    size_t batch_part = batch_ws_bytes(h, nq, max_nprobe, cap);
     void* wsp;
     rc = workspace(h, probe_part + learned_part + list_part + batch_part, &wsp);
     if (rc) return rc;

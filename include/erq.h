@@ -43,7 +43,11 @@ enum {
     /* Queries (waves) per workgroup of the per-query scan, 1..4 (LDS permitting). */
     ERQ_OPT_PLAIN_WAVES = 2,
     /* Workgroups per CU of the persistent batched kernel, 1..8. */
-    ERQ_OPT_BATCH_WG_PER_CU = 3
+    ERQ_OPT_BATCH_WG_PER_CU = 3,
+    /* Records (16 bytes each) of the batched search's survivor list per round, 0..2^30
+     * (default 2^24).  Survivors beyond it are re-estimated inside the scan kernel
+     * instead of by the separate re-estimation kernel; results are identical. */
+    ERQ_OPT_BATCH_SURVIVOR_CAP = 4
 };
 int erq_set_option(erq_index* h, int option, long value);
 long erq_get_option(const erq_index* h, int option);

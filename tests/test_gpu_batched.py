@@ -39,10 +39,12 @@ def _probe_lists(idx, qr, nprobes):
     return [oi.centroid_topn(qr[i], int(nprobes[i])) for i in range(qr.shape[0])]
 
 
-def _gpu_batched(idx, qr, k, probe_lists, max_nprobe):
+def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None):
     import torch
     from erq.gpu import IVF
     g = IVF.from_index(idx)
+    if survivor_cap is not None:
+        g.set_option(4, survivor_cap)   # ERQ_OPT_BATCH_SURVIVOR_CAP
     npq = np.array([len(p[0]) for p in probe_lists])
     off = np.concatenate([[0], np.cumsum(npq)]).astype(np.uint32)
     pid = np.concatenate([p[0] for p in probe_lists]).astype(np.uint32)
@@ -57,11 +59,11 @@ def _gpu_batched(idx, qr, k, probe_lists, max_nprobe):
     return out
 
 
-def _check(idx, q, k, nprobes, label):
+def _check(idx, q, k, nprobes, label, survivor_cap=None):
     qr = rotate_np(idx, q)
     pl = _probe_lists(idx, qr, nprobes)
     oid, od = _oracle_rounds(idx, qr, k, pl)
-    gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)))
+    gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)), survivor_cap)
     print(f"[{label}] stats={st.tolist()}", flush=True)
     assert st[5] == 0, "candidate buffer overflow"
     np.testing.assert_array_equal(gid, oid, err_msg=f"{label}: ids")
@@ -75,6 +77,14 @@ def test_batched_matches_round_oracle_d128(bits):
     _check(idx, q, 100, np.full(q.shape[0], 24), f"D128 B{bits}")
 
 
+@pytest.mark.parametrize("cap", [0, 1000])
+def test_batched_survivor_list_overflow(cap):
+    """With the survivor list capped (none, or part of a round's survivors), workgroups
+    re-estimate the overflow themselves; the result stays bit-exact."""
+    idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
+    _check(idx, q, 100, np.full(q.shape[0], 24), f"cap {cap}", survivor_cap=cap)
+
+
 def test_batched_uneven_probe_counts():
     idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
     rng = np.random.default_rng(5)
@@ -86,9 +96,10 @@ def test_batched_d1024_b8():
     _check(idx, q, 100, np.full(q.shape[0], 20), "D960 B8")
 
 
-def test_batched_d1536_b4():
+@pytest.mark.parametrize("cap", [None, 0])
+def test_batched_d1536_b4(cap):
     idx, q, _ = _build(("d1536", 4), 8000, 1536, 48, 4, nq=24)
-    _check(idx, q, 100, np.full(q.shape[0], 20), "D1536 B4")
+    _check(idx, q, 100, np.full(q.shape[0], 20), f"D1536 B4 cap {cap}", survivor_cap=cap)
 
 
 def test_batched_single_probe_equals_plain_and_is_deterministic():
