@@ -705,9 +705,9 @@ int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size
                           stats_dev, s, (char*)wsp, probe_part + list_part);
 }
 
-static size_t learned_ws_bytes(size_t nq, size_t K, size_t max_np, size_t fb_np) {
-    return 8 * 256 + align_up(nq * K * 4, 256) + 2 * align_up(nq * fb_np * 4, 256) +
-           2 * align_up(nq * max_np * 4, 256) + align_up(nq * 4, 256);
+static size_t learned_ws_bytes(size_t nq, size_t K, size_t max_np) {
+    return 8 * 256 + align_up(nq * K * 4, 256) + 2 * align_up(nq * max_np * 4, 256) +
+           align_up(nq * 4, 256);
 }
 
 static int learned_locked(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq,
@@ -716,19 +716,15 @@ static int learned_locked(erq_index* h, erq_mlp* m, const float* q_rot_dev, size
                           hipStream_t s, char* base, size_t learned_off) {
     Ws ws{base, learned_off};
     float* prob = ws.take<float>(nq * h->K);
-    uint32_t* fb_ids = ws.take<uint32_t>(nq * fb_np);
-    float* fb_d = ws.take<float>(nq * fb_np);
     uint32_t* pad_ids = ws.take<uint32_t>(nq * max_np);
     float* pad_d = ws.take<float>(nq * max_np);
     uint32_t* len = ws.take<uint32_t>(nq);
     int rc = erq_mlp_forward_dev(m, q_rot_dev, nq, prob, s);
     if (rc) return rc;
-    rc = probe_locked(h, q_rot_dev, nq, fb_np, fb_ids, fb_d, s);
-    if (rc) return rc;
     hipError_t e = launch_adaptive_probe(prob, q_rot_dev, h->centroids, (int)nq, (int)h->K,
                                          (int)h->D, mass, (int)min_np, (int)max_np, min_conf,
-                                         fb_ids, fb_d, (int)fb_np, pad_ids, pad_d, len, off, ids,
-                                         sqr_y, n_fallback, s);
+                                         (int)fb_np, pad_ids, pad_d, len, off, ids, sqr_y,
+                                         n_fallback, s);
     if (e != hipSuccess) return hip_fail(e, "learned probe selection");
     return ERQ_OK;
 }
@@ -761,9 +757,9 @@ int erq_probe_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, size
     if (rc || nq == 0) return rc;
     std::lock_guard<std::mutex> lk(h->mu);
     hipStream_t s = pick(h, stream);
-    size_t probe_part = align_up(nq * h->K * 4, 256) + align_up(nq * 4, 256) + 512;
+    const size_t probe_part = 0;  // fallback lists are formed by the selection kernel itself
     void* wsp;
-    rc = workspace(h, probe_part + learned_ws_bytes(nq, h->K, max_nprobe, fallback_nprobe), &wsp);
+    rc = workspace(h, probe_part + learned_ws_bytes(nq, h->K, max_nprobe), &wsp);
     if (rc) return rc;
     return learned_locked(h, m, q_rot_dev, nq, mass, min_nprobe, max_nprobe, fallback_nprobe,
                           min_conf, probe_off_dev, probe_ids_dev, probe_sqr_y_dev, n_fallback_dev,
@@ -792,8 +788,8 @@ int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, siz
     hipStream_t s = pick(h, stream);
     const size_t K = h->K;
     const int cap = kCandCap;
-    size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
-    size_t learned_part = learned_ws_bytes(nq, K, max_nprobe, fallback_nprobe);
+    const size_t probe_part = 0;  // fallback lists are formed by the selection kernel itself
+    size_t learned_part = learned_ws_bytes(nq, K, max_nprobe);
     size_t list_part = align_up(nq * max_nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) +
                        align_up(8, 256) + 512;
     size_t batch_part = batch_ws_bytes(h, nq, max_nprobe, cap);

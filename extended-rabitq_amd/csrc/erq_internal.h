@@ -85,9 +85,8 @@ hipError_t launch_probe_select(const float* approx, const float* qnorm, const fl
 
 hipError_t launch_adaptive_probe(const float* prob, const float* q_rot, const float* centroids,
                                  int nq, int K, int D, float mass, int min_np, int max_np,
-                                 float min_conf, const uint32_t* fb_ids, const float* fb_d,
-                                 int fb_np, uint32_t* pad_ids, float* pad_d, uint32_t* len,
-                                 uint32_t* off, uint32_t* csr_ids, float* csr_d,
+                                 float min_conf, int fb_np, uint32_t* pad_ids, float* pad_d,
+                                 uint32_t* len, uint32_t* off, uint32_t* csr_ids, float* csr_d,
                                  uint32_t* n_fallback, hipStream_t s);
 
 }  // namespace erq

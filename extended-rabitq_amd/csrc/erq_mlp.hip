@@ -27,6 +27,7 @@ struct erq_mlp {
     float* adam_v = nullptr;
     __bf16* wb = nullptr;     // BF16 copies of W1, W2, W3 (no biases)
     long step = 0;
+    float xscale = 1.f;       // inputs enter layer 1 as bf16(xscale * x)
     // activations (max_batch rows)
     __bf16* xb = nullptr;     // B x in
     float* z1 = nullptr;      // B x hid (pre-activation)
@@ -163,10 +164,11 @@ hipError_t gemm(const __bf16* A, int lda, const __bf16* B, int ldb, int M, int N
     return hipGetLastError();
 }
 
-__global__ void f32_to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, size_t n) {
+__global__ void f32_to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, size_t n,
+                                   float scale) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
-        y[i] = (__bf16)x[i];
+        y[i] = (__bf16)(scale * x[i]);
 }
 
 __global__ void sigmoid_kernel(const float* __restrict__ z, float* __restrict__ p, size_t n) {
@@ -252,11 +254,11 @@ hipError_t refresh_bf16(erq_mlp* m, hipStream_t s) {
     Layout L = layout(m);
     size_t n1 = (size_t)m->hid * m->in, n2 = (size_t)m->hid * m->hid, n3 = (size_t)m->out * m->hid;
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(grid_for(n1)), dim3(256), 0, s, m->master + L.w1,
-                       m->wb, n1);
+                       m->wb, n1, 1.f);
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(grid_for(n2)), dim3(256), 0, s, m->master + L.w2,
-                       m->wb + n1, n2);
+                       m->wb + n1, n2, 1.f);
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(grid_for(n3)), dim3(256), 0, s, m->master + L.w3,
-                       m->wb + n1 + n2, n3);
+                       m->wb + n1 + n2, n3, 1.f);
     return hipGetLastError();
 }
 
@@ -266,7 +268,7 @@ hipError_t forward(erq_mlp* m, const float* x, int B, hipStream_t s) {
     const size_t n1 = (size_t)m->hid * m->in, n2 = (size_t)m->hid * m->hid;
     const __bf16 *W1 = m->wb, *W2 = m->wb + n1, *W3 = m->wb + n1 + n2;
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(grid_for((size_t)B * m->in)), dim3(256), 0, s, x,
-                       m->xb, (size_t)B * m->in);
+                       m->xb, (size_t)B * m->in, m->xscale);
     hipError_t e = gemm<0, 1, EPI_BIAS_GELU>(m->xb, m->in, W1, m->in, B, m->hid, m->in, m->z1,
                                              m->hid, m->h1, m->master + L.b1, s);
     if (e == hipSuccess)
@@ -416,6 +418,18 @@ int erq_mlp_create(size_t in_dim, size_t hidden, size_t out_dim, size_t max_batc
 }
 
 void erq_mlp_free(erq_mlp* m) { mlp_release(m); }
+
+int erq_mlp_set_input_scale(erq_mlp* m, float scale) {
+    if (!m || !(scale > 0.f) || !std::isfinite(scale)) {
+        set_error("erq_mlp_set_input_scale: null model or scale not positive and finite");
+        return ERQ_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->xscale = scale;
+    return ERQ_OK;
+}
+
+float erq_mlp_input_scale(const erq_mlp* m) { return m ? m->xscale : 0.f; }
 
 size_t erq_mlp_num_params(const erq_mlp* m) { return m ? m->nparams : 0; }
 size_t erq_mlp_in_dim(const erq_mlp* m) { return m ? (size_t)m->in : 0; }

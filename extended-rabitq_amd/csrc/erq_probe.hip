@@ -24,7 +24,9 @@ __device__ __forceinline__ bool cand_less(float da, uint32_t ia, float db, uint3
     return da < db || (da == db && ia < ib);
 }
 
-// Block-wide radix select: the key of rank `rank` (1-based) among keys[0..n).
+// Block-wide radix select: the key of rank `rank` (1-based) among keys[0..n).  Per 8-bit
+// digit: an LDS histogram, then wave 0 finds the digit holding the rank (each lane sums
+// four bins, a shuffle prefix scan over the lanes).
 __device__ uint32_t radix_select(const uint32_t* keys, int n, uint32_t rank, uint32_t* hist,
                                  uint32_t* sh_sel, uint32_t* sh_rank) {
     const int tid = threadIdx.x;
@@ -38,17 +40,31 @@ __device__ uint32_t radix_select(const uint32_t* keys, int n, uint32_t rank, uin
             if ((kv & pmask) == prefix) atomicAdd(&hist[(kv >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t r = *sh_rank, cum = 0, sel = 255;
-            for (int dgt = 0; dgt < 256; dgt++) {
-                if (cum + hist[dgt] >= r) {
-                    sel = (uint32_t)dgt;
-                    break;
-                }
-                cum += hist[dgt];
+        if (tid < 64) {
+            const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2],
+                           h3 = hist[4 * tid + 3];
+            const uint32_t own = h0 + h1 + h2 + h3;
+            uint32_t incl = own;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o);
+                if (tid >= o) incl += v;
             }
-            *sh_sel = sel;
-            *sh_rank = r - cum;
+            const uint32_t r = *sh_rank, excl = incl - own;
+            // the lane whose bins span rank r (the last lane if r exceeds the count)
+            const bool mine = (excl < r && r <= incl) || (tid == 63 && r > incl);
+            if (mine) {
+                uint32_t cum = excl, sel = 4 * tid + 3;
+                const uint32_t hs[4] = {h0, h1, h2, h3};
+                for (int j = 0; j < 4; j++) {
+                    if (cum + hs[j] >= r) {
+                        sel = 4 * tid + j;
+                        break;
+                    }
+                    cum += hs[j];
+                }
+                *sh_sel = sel;
+                *sh_rank = r - cum;
+            }
         }
         __syncthreads();
         prefix |= *sh_sel << shift;
@@ -62,11 +78,13 @@ __device__ uint32_t radix_select(const uint32_t* keys, int n, uint32_t rank, uin
 __device__ __forceinline__ float l2sqr16_group(const float* qs, const float* c, int D, int l,
                                                bool active) {
     float p = 0.f;
-    if (active)
+    if (active) {
+#pragma unroll 8
         for (int d = l; d < D; d += 16) {
             float t = qs[d] - c[d];
             p = fmaf(t, t, p);
         }
+    }
     return butterfly16(p);
 }
 
@@ -168,114 +186,14 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Learned probe lists.  Per query: rank clusters by predicted probability p (descending,
-// ties by id), keep the shortest prefix whose mass reaches `mass` of sum(p) (at least
-// min_np, at most max_np), then order the kept clusters by their exact L2Sqr16 distance
-// (the order IVF::search scans in).  Low confidence -- sum(p) < min_conf, or the prefix
-// hits max_np before reaching the mass -- takes the fixed-nprobe list instead.
-__global__ void __launch_bounds__(kThreads)
-    adaptive_select_kernel(const float* __restrict__ prob, const float* __restrict__ q_rot,
-                           const float* __restrict__ centroids, int K, int D, float mass,
-                           int min_np, int max_np, float min_conf,
-                           const uint32_t* __restrict__ fb_ids, const float* __restrict__ fb_d,
-                           int fb_np, uint32_t* __restrict__ out_ids, float* __restrict__ out_d,
-                           uint32_t* __restrict__ out_len, uint32_t* __restrict__ n_fallback) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    uint32_t* keys = reinterpret_cast<uint32_t*>(smem);
-    float* cd = reinterpret_cast<float*>(smem + 4 * (size_t)K);
-    uint32_t* ci = reinterpret_cast<uint32_t*>(cd + kCapC);
-    float* qs = reinterpret_cast<float*>(ci + kCapC);
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
-    __shared__ float sh_sum[kThreads / 64];
-    __shared__ int sh_n, sh_fb;
-    const int tid = threadIdx.x, qi = blockIdx.x;
-    const int grp = tid >> 4, l = tid & 15;
-    const float* prow = prob + (size_t)qi * K;
-    float part = 0.f;
-    for (int c = tid; c < K; c += kThreads) {
-        float pc = prow[c];
-        part += pc;
-        keys[c] = float_key(-pc);
-    }
-    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-    if ((tid & 63) == 0) sh_sum[tid >> 6] = part;
-    if (tid == 0) sh_cnt = 0;
-    const float* q = q_rot + (size_t)qi * D;
-    for (int d = tid; d < D; d += kThreads) qs[d] = q[d];
-    __syncthreads();
-    float total = 0.f;
-    for (int w = 0; w < kThreads / 64; w++) total += sh_sum[w];
-    const int T = min(max_np, K);
-    const uint32_t U = radix_select(keys, K, (uint32_t)T, hist, &sh_sel, &sh_rank);
-    for (int c = tid; c < K; c += kThreads)
-        if (keys[c] <= U) {
-            uint32_t slot = atomicAdd(&sh_cnt, 1u);
-            if (slot < (uint32_t)kCapC) {
-                ci[slot] = (uint32_t)c;
-                cd[slot] = key_float(keys[c]);  // -p
-            }
-        }
-    __syncthreads();
-    const int cnt = min((int)sh_cnt, kCapC);
-    int P2 = 1;
-    while (P2 < cnt) P2 <<= 1;
-    for (int i = cnt + tid; i < P2; i += kThreads) {
-        cd[i] = FLT_MAX;
-        ci[i] = 0xFFFFFFFFu;
-    }
-    __syncthreads();
-    for (int size = 2; size <= P2; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = tid; t < (P2 >> 1); t += kThreads) {
-                int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
-                bool up = (lo & size) == 0;
-                float dl = cd[lo], dh = cd[hi];
-                uint32_t il = ci[lo], ih = ci[hi];
-                if (up ? cand_less(dh, ih, dl, il) : cand_less(dl, il, dh, ih)) {
-                    cd[lo] = dh;
-                    cd[hi] = dl;
-                    ci[lo] = ih;
-                    ci[hi] = il;
-                }
-            }
-            __syncthreads();
-        }
-    if (tid == 0) {
-        float acc = 0.f, goal = mass * total;
-        int n = 0;
-        while (n < T && acc < goal) acc += -cd[n++];
-        int fb = (total < min_conf) || (acc < goal);
-        sh_fb = fb;
-        sh_n = max(min(max(n, min_np), T), 1);
-    }
-    __syncthreads();
-    if (sh_fb) {
-        for (int i = tid; i < fb_np; i += kThreads) {
-            out_ids[(size_t)qi * max_np + i] = fb_ids[(size_t)qi * fb_np + i];
-            out_d[(size_t)qi * max_np + i] = fb_d[(size_t)qi * fb_np + i];
-        }
-        if (tid == 0) {
-            out_len[qi] = (uint32_t)fb_np;
-            atomicAdd(n_fallback, 1u);
-        }
-        return;
-    }
-    const int n = sh_n;
-    // exact distances of the kept clusters, then order them by (distance, id)
-    for (int base = 0; base < n; base += kThreads / 16) {
-        const int i = base + grp;
-        const bool act = i < n;
-        float pdist = l2sqr16_group(qs, centroids + (size_t)(act ? ci[i] : 0) * D, D, l, act);
-        if (act && l == 0) cd[i] = pdist;
-    }
-    __syncthreads();
-    int P = 1;
-    while (P < n) P <<= 1;
-    for (int i = n + tid; i < P; i += kThreads) {
-        cd[i] = FLT_MAX;
-        ci[i] = 0xFFFFFFFFu;
-    }
-    __syncthreads();
+// ties by id) and keep the shortest prefix whose mass reaches `mass` of the mass of the
+// max_np most probable clusters (at least min_np, at most max_np), then order the kept
+// clusters by their exact L2Sqr16 distance (the order IVF::search scans in).  A query
+// whose total p is below min_conf takes the fixed list instead: the fb_np nearest clusters
+// by exact L2Sqr16 (ties by id), computed here over all K -- the list probe_select_kernel
+// returns.
+__device__ void bitonic_by_key(float* cd, uint32_t* ci, int P) {
+    const int tid = threadIdx.x;
     for (int size = 2; size <= P; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             for (int t = tid; t < (P >> 1); t += kThreads) {
@@ -292,6 +210,134 @@ __global__ void __launch_bounds__(kThreads)
             }
             __syncthreads();
         }
+}
+
+// keys <= U into (cd, ci) (capped at kCapC), padded to a power of two and sorted by
+// (value, id); returns the count kept
+__device__ int gather_sorted(const uint32_t* keys, int K, uint32_t U, float* cd, uint32_t* ci,
+                             uint32_t* sh_cnt) {
+    const int tid = threadIdx.x;
+    if (tid == 0) *sh_cnt = 0;
+    __syncthreads();
+    for (int c = tid; c < K; c += kThreads)
+        if (keys[c] <= U) {
+            uint32_t slot = atomicAdd(sh_cnt, 1u);
+            if (slot < (uint32_t)kCapC) {
+                ci[slot] = (uint32_t)c;
+                cd[slot] = key_float(keys[c]);
+            }
+        }
+    __syncthreads();
+    const int cnt = min((int)*sh_cnt, kCapC);
+    int P2 = 1;
+    while (P2 < cnt) P2 <<= 1;
+    for (int i = cnt + tid; i < P2; i += kThreads) {
+        cd[i] = FLT_MAX;
+        ci[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    bitonic_by_key(cd, ci, P2);
+    return cnt;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    adaptive_select_kernel(const float* __restrict__ prob, const float* __restrict__ q_rot,
+                           const float* __restrict__ centroids, int K, int D, float mass,
+                           int min_np, int max_np, float min_conf, int fb_np,
+                           uint32_t* __restrict__ out_ids, float* __restrict__ out_d,
+                           uint32_t* __restrict__ out_len, uint32_t* __restrict__ n_fallback) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* keys = reinterpret_cast<uint32_t*>(smem);
+    float* cd = reinterpret_cast<float*>(smem + 4 * (size_t)K);
+    uint32_t* ci = reinterpret_cast<uint32_t*>(cd + kCapC);
+    float* qs = reinterpret_cast<float*>(ci + kCapC);
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_sel, sh_rank, sh_cnt, sh_n;
+    __shared__ float sh_sum[kThreads / 64], sh_part[kThreads];
+    const int tid = threadIdx.x, qi = blockIdx.x;
+    const int grp = tid >> 4, l = tid & 15;
+    const float* prow = prob + (size_t)qi * K;
+    float part = 0.f;
+    for (int c = tid; c < K; c += kThreads) {
+        float pc = prow[c];
+        part += pc;
+        keys[c] = float_key(-pc);
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if ((tid & 63) == 0) sh_sum[tid >> 6] = part;
+    const float* q = q_rot + (size_t)qi * D;
+    for (int d = tid; d < D; d += kThreads) qs[d] = q[d];
+    __syncthreads();
+    float total = 0.f;
+    for (int w = 0; w < kThreads / 64; w++) total += sh_sum[w];
+
+    if (total < min_conf) {  // uniform
+        for (int base = 0; base < K; base += kThreads / 16) {
+            const int c = base + grp;
+            const bool act = c < K;
+            const float dist = l2sqr16_group(qs, centroids + (size_t)(act ? c : 0) * D, D, l, act);
+            if (act && l == 0) keys[c] = float_key(dist);
+        }
+        __syncthreads();
+        const uint32_t U = radix_select(keys, K, (uint32_t)fb_np, hist, &sh_sel, &sh_rank);
+        gather_sorted(keys, K, U, cd, ci, &sh_cnt);
+        for (int i = tid; i < fb_np; i += kThreads) {
+            out_ids[(size_t)qi * max_np + i] = ci[i];
+            out_d[(size_t)qi * max_np + i] = cd[i];
+        }
+        if (tid == 0) {
+            out_len[qi] = (uint32_t)fb_np;
+            atomicAdd(n_fallback, 1u);
+        }
+        return;
+    }
+
+    const int T = min(max_np, K);
+    const uint32_t U = radix_select(keys, K, (uint32_t)T, hist, &sh_sel, &sh_rank);
+    const int cnt = gather_sorted(keys, K, U, cd, ci, &sh_cnt);  // cd = -p, most probable first
+    // n = shortest prefix of the sorted list whose mass reaches mass * (mass of the first L);
+    // prefix sums blocked by thread (thread t sums entries [t*per, (t+1)*per))
+    const int L = min(T, cnt);
+    const int per = (L + kThreads - 1) / kThreads;
+    float own = 0.f;
+    for (int i = tid * per; i < min(L, (tid + 1) * per); i++) own += -cd[i];
+    sh_part[tid] = own;
+    if (tid == 0) sh_n = (uint32_t)L;
+    __syncthreads();
+    for (int o = 1; o < kThreads; o <<= 1) {  // inclusive scan of the thread sums
+        const float v = tid >= o ? sh_part[tid - o] : 0.f;
+        __syncthreads();
+        sh_part[tid] += v;
+        __syncthreads();
+    }
+    const float goal = mass * sh_part[kThreads - 1];
+    float acc = tid ? sh_part[tid - 1] : 0.f;
+    for (int i = tid * per; i < min(L, (tid + 1) * per); i++) {
+        acc += -cd[i];
+        if (acc >= goal) {
+            atomicMin(&sh_n, (uint32_t)(i + 1));
+            break;
+        }
+    }
+    __syncthreads();
+    const int n = max(min(max((int)sh_n, min_np), T), 1);
+    __syncthreads();
+    // exact distances of the kept clusters, then order them by (distance, id)
+    for (int base = 0; base < n; base += kThreads / 16) {
+        const int i = base + grp;
+        const bool act = i < n;
+        float pdist = l2sqr16_group(qs, centroids + (size_t)(act ? ci[i] : 0) * D, D, l, act);
+        if (act && l == 0) cd[i] = pdist;
+    }
+    __syncthreads();
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = n + tid; i < P; i += kThreads) {
+        cd[i] = FLT_MAX;
+        ci[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    bitonic_by_key(cd, ci, P);
     for (int i = tid; i < n; i += kThreads) {
         out_ids[(size_t)qi * max_np + i] = ci[i];
         out_d[(size_t)qi * max_np + i] = cd[i];
@@ -341,17 +387,14 @@ __global__ void compact_probes_kernel(const uint32_t* __restrict__ pid, const fl
 
 hipError_t launch_adaptive_probe(const float* prob, const float* q_rot, const float* centroids,
                                  int nq, int K, int D, float mass, int min_np, int max_np,
-                                 float min_conf, const uint32_t* fb_ids, const float* fb_d,
-                                 int fb_np, uint32_t* pad_ids, float* pad_d, uint32_t* len,
-                                 uint32_t* off, uint32_t* csr_ids, float* csr_d,
+                                 float min_conf, int fb_np, uint32_t* pad_ids, float* pad_d,
+                                 uint32_t* len, uint32_t* off, uint32_t* csr_ids, float* csr_d,
                                  uint32_t* n_fallback, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    if (max_np < 1 || max_np > kCapC || max_np > K || fb_np > max_np || min_np > max_np)
-        return hipErrorInvalidValue;
-    size_t lds = 4 * (size_t)K + 8 * (size_t)kCapC + 4 * (size_t)D;
+    const size_t lds = 4 * (size_t)K + 8 * (size_t)kCapC + 4 * (size_t)D;
     hipLaunchKernelGGL(adaptive_select_kernel, dim3(nq), dim3(kThreads), lds, s, prob, q_rot,
-                       centroids, K, D, mass, min_np, max_np, min_conf, fb_ids, fb_d, fb_np,
-                       pad_ids, pad_d, len, n_fallback);
+                       centroids, K, D, mass, min_np, max_np, min_conf, fb_np, pad_ids, pad_d, len,
+                       n_fallback);
     hipLaunchKernelGGL(lens_to_offsets_kernel, dim3(1), dim3(1024), 0, s, len, nq, off);
     hipLaunchKernelGGL(compact_probes_kernel, dim3(nq), dim3(64), 0, s, pad_ids, pad_d, max_np,
                        off, nq, csr_ids, csr_d);

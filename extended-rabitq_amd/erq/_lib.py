@@ -50,6 +50,8 @@ SIGNATURES = {
     "erq_mlp_in_dim": (c_size, [c_vp]),
     "erq_mlp_out_dim": (c_size, [c_vp]),
     "erq_mlp_set_params": (c_int, [c_vp, c_vp]),
+    "erq_mlp_set_input_scale": (c_int, [c_vp, ctypes.c_float]),
+    "erq_mlp_input_scale": (ctypes.c_float, [c_vp]),
     "erq_mlp_get_params": (c_int, [c_vp, c_vp, c_vp]),
     "erq_mlp_forward_dev": (c_int, [c_vp, c_vp, c_size, c_vp, c_vp]),
     "erq_mlp_loss_grad_dev": (c_int, [c_vp, c_vp, c_vp, c_size, ctypes.c_float, c_vp, c_vp]),

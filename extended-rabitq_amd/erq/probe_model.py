@@ -14,11 +14,13 @@ from .gpu import _ptr, _stream
 
 
 class ProbeMLP:
-    def __init__(self, in_dim, out_dim, hidden=512, max_batch=1024, seed=0):
+    def __init__(self, in_dim, out_dim, hidden=512, max_batch=1024, seed=0, input_scale=1.0):
         h = ctypes.c_void_p()
         check(_lib.lib().erq_mlp_create(in_dim, hidden, out_dim, max_batch, seed,
                                         ctypes.byref(h)))
         self._h = h
+        if input_scale != 1.0:
+            check(_lib.lib().erq_mlp_set_input_scale(h, input_scale))
         self.in_dim, self.out_dim, self.hidden, self.max_batch = in_dim, out_dim, hidden, max_batch
 
     def close(self):

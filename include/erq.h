@@ -158,6 +158,10 @@ size_t erq_mlp_num_params(const erq_mlp* m);
 size_t erq_mlp_in_dim(const erq_mlp* m);
 size_t erq_mlp_out_dim(const erq_mlp* m);
 int erq_mlp_set_params(erq_mlp* m, const float* host_params);
+/* Layer 1 reads bf16(scale * x) (default scale 1): with scale = sqrt(D), rotated unit-norm
+ * queries enter with unit variance per coordinate. */
+int erq_mlp_set_input_scale(erq_mlp* m, float scale);
+float erq_mlp_input_scale(const erq_mlp* m);
 /* Either output may be NULL; grads are those of the last erq_mlp_loss_grad_dev. */
 int erq_mlp_get_params(erq_mlp* m, float* host_params, float* host_grads);
 /* prob_dev = p for batch rows of x_dev (batch x in_dim), any batch size. */
@@ -171,10 +175,10 @@ int erq_mlp_adam_step_dev(erq_mlp* m, float lr, float beta1, float beta2, float 
                           void* stream);
 
 /* Learned probe lists (CSR: probe_off nq+1, ids/sqr_y up to nq*max_nprobe): each query keeps
- * its most probable clusters until their predicted mass reaches `mass` of the total (between
- * min_nprobe and max_nprobe), ordered by exact centroid distance; a query whose total
- * predicted mass is below min_conf, or whose mass does not reach the target within
- * max_nprobe, takes the fixed list of its fallback_nprobe nearest clusters.  n_fallback_dev
+ * its most probable clusters until their predicted probability mass reaches `mass` of the
+ * mass of its max_nprobe most probable clusters (at least min_nprobe of them), ordered by
+ * exact centroid distance; a query whose total predicted mass over all clusters is below
+ * min_conf takes the fixed list of its fallback_nprobe nearest clusters.  n_fallback_dev
  * (one u32) is incremented per fallback query. */
 int erq_probe_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq, float mass,
                           size_t min_nprobe, size_t max_nprobe, size_t fallback_nprobe,

@@ -1,6 +1,7 @@
 """FP64 reference of the probe-predictor MLP (test infrastructure only).
 
-p = sigmoid(W3 gelu(W2 gelu(W1 x + b1) + b2) + b3), weights out-major ([out x in]),
+p = sigmoid(W3 gelu(W2 gelu(W1 (s x) + b1) + b2) + b3), weights out-major ([out x in]),
+s the input scale (1 unless given),
 GELU with erf, loss = mean over the batch of sum over clusters of the pos-weighted BCE
     l = w*y*softplus(-z) + (1-y)*softplus(z).
 Parameters are packed as [W1, b1, W2, b2, W3, b3] (the layout liberq's erq_mlp uses).
@@ -35,9 +36,9 @@ def softplus(z):
     return np.maximum(z, 0) + np.log1p(np.exp(-np.abs(z)))
 
 
-def forward(theta, x, din, hid, dout):
+def forward(theta, x, din, hid, dout, xs=1.0):
     W1, b1, W2, b2, W3, b3 = unpack(np.asarray(theta, np.float64), din, hid, dout)
-    x = np.asarray(x, np.float64)
+    x = np.asarray(x, np.float64) * xs
     z1 = x @ W1.T + b1
     h1 = gelu(z1)
     z2 = h1 @ W2.T + b2
@@ -46,15 +47,15 @@ def forward(theta, x, din, hid, dout):
     return dict(x=x, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3)
 
 
-def loss(theta, x, y, din, hid, dout, w):
-    f = forward(theta, x, din, hid, dout)
+def loss(theta, x, y, din, hid, dout, w, xs=1.0):
+    f = forward(theta, x, din, hid, dout, xs)
     z, y = f["z3"], np.asarray(y, np.float64)
     return float((w * y * softplus(-z) + (1 - y) * softplus(z)).sum() / x.shape[0])
 
 
-def grad(theta, x, y, din, hid, dout, w):
+def grad(theta, x, y, din, hid, dout, w, xs=1.0):
     W1, b1, W2, b2, W3, b3 = unpack(np.asarray(theta, np.float64), din, hid, dout)
-    f = forward(theta, x, din, hid, dout)
+    f = forward(theta, x, din, hid, dout, xs)
     B = x.shape[0]
     p = 1.0 / (1.0 + np.exp(-f["z3"]))
     y = np.asarray(y, np.float64)
@@ -78,13 +79,14 @@ def round_bf16(a):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def grad_bf16(theta, x, y, din, hid, dout, w):
-    """Loss and gradient with the GPU's precision steps: BF16 weights, inputs, activations
-    and backward GEMM operands; FP32 pre-activations; everything else in FP64."""
+def grad_bf16(theta, x, y, din, hid, dout, w, xs=1.0):
+    """Loss and gradient with the GPU's precision steps: BF16 weights, inputs (scaled in
+    FP32 first), activations and backward GEMM operands; FP32 pre-activations; everything
+    else in FP64."""
     W1, b1, W2, b2, W3, b3 = unpack(np.asarray(theta, np.float64), din, hid, dout)
     W1b, W2b, W3b = round_bf16(W1), round_bf16(W2), round_bf16(W3)
     f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
-    xb = round_bf16(x)
+    xb = round_bf16(np.float32(xs) * np.asarray(x, np.float32))
     z1 = f32(xb @ W1b.T + b1)
     h1 = round_bf16(gelu(z1))
     z2 = f32(h1 @ W2b.T + b2)

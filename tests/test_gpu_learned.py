@@ -19,27 +19,30 @@ def _mlp_case(din=64, hid=64, dout=40, B=64, seed=0):
     return theta, x, y
 
 
-def test_mlp_forward_and_gradients_match_oracle():
+@pytest.mark.parametrize("xs", [1.0, 3.7])
+def test_mlp_forward_and_gradients_match_oracle(xs):
     import torch
     from erq.probe_model import ProbeMLP
     from oracle import mlp_oracle as mo
     din, hid, dout, B = 64, 64, 40, 64
     theta, x, y = _mlp_case(din, hid, dout, B)
-    m = ProbeMLP(din, dout, hidden=hid, max_batch=B, seed=1)
+    if xs != 1.0:
+        x = x / xs   # keep the pre-activations at the unscaled case's size
+    m = ProbeMLP(din, dout, hidden=hid, max_batch=B, seed=1, input_scale=xs)
     m.set_params(theta)
     xt, yt = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
     p = m.forward_dev(xt)
     loss = m.loss_grad_dev(xt, yt, 3.0)
     torch.cuda.synchronize()
     g = m.get_grads()
-    L_ref, g_ref, p_ref = mo.grad_bf16(theta.astype(np.float64), x, y, din, hid, dout, 3.0)
+    L_ref, g_ref, p_ref = mo.grad_bf16(theta.astype(np.float64), x, y, din, hid, dout, 3.0, xs)
     # same precision steps: only FP32 accumulation order differs
     np.testing.assert_allclose(p.cpu().numpy(), p_ref, rtol=2e-3, atol=1e-5)
     assert abs(float(loss[0]) - L_ref) <= 1e-3 * abs(L_ref)
     scale = np.abs(g_ref).max()
     assert np.abs(g - g_ref).max() <= 2e-2 * scale, np.abs(g - g_ref).max() / scale
     # against the exact FP64 network: the BF16 path keeps the gradient's direction
-    g64 = mo.grad(theta.astype(np.float64), x, y, din, hid, dout, 3.0)
+    g64 = mo.grad(theta.astype(np.float64), x, y, din, hid, dout, 3.0, xs)
     assert g @ g64 / np.linalg.norm(g) / np.linalg.norm(g64) > 0.995
     m.close()
 

@@ -90,3 +90,17 @@ def test_pos_weighted_bce_optimum_is_shifted_logit():
             best = (L, z)
     expect = math.log(w) + math.log(0.3 / 0.7)
     assert abs(best[1] - expect) < 0.011, (best, expect)
+
+
+def test_input_scale_equals_scaled_first_layer():
+    """MLP(s x) is the network with W1 scaled by s; its W1 gradient is s times that one's."""
+    theta, x, y, din, hid, dout = _setup(seed=6)
+    xs = 2.75
+    t2 = theta.copy()
+    t2[:hid * din] *= xs
+    assert math.isclose(mo.loss(theta, x, y, din, hid, dout, 3.0, xs),
+                        mo.loss(t2, x, y, din, hid, dout, 3.0), rel_tol=1e-12)
+    g = mo.grad(theta, x, y, din, hid, dout, 3.0, xs)
+    g2 = mo.grad(t2, x, y, din, hid, dout, 3.0)
+    np.testing.assert_allclose(g[:hid * din], xs * g2[:hid * din], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(g[hid * din:], g2[hid * din:], rtol=1e-9, atol=1e-12)
