@@ -46,10 +46,10 @@ def parse():
     ap.add_argument("--nprobes", type=str, default="8,16,32,64,96,128,160,192,224,256,320,384,512")
     ap.add_argument("--sigma", type=float, default=1.15,
                     help="within-component spread relative to the component spread")
-    ap.add_argument("--train-queries", type=int, default=100000)
+    ap.add_argument("--train-queries", type=int, default=200000)
     ap.add_argument("--label-nprobe", type=int, default=0,
-                    help="nprobe of the label search (0: twice the fixed path's nprobe at target)")
-    ap.add_argument("--epochs", type=int, default=4)
+                    help="nprobe of the label search (0: 256) and the learned lists' maximum")
+    ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device copy first (HBM counter calibration)")
@@ -284,7 +284,7 @@ def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np)
     qt_r = g.rotate_dev(qt)
     del qt
     c_of = torch.from_numpy(cluster_of_ids(idx)).cuda()
-    label_np = min(a.label_nprobe or 2 * fixed_np, idx.K, 2048)
+    label_np = min(a.label_nprobe or 256, idx.K, 2048)
     lab = torch.empty((a.train_queries, a.k), dtype=torch.int32, device="cuda")
     for s0 in range(0, a.train_queries, 20000):
         ids, _ = g.search_batched_dev(qt_r[s0:s0 + 20000], a.k, label_np)
@@ -293,8 +293,10 @@ def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np)
     t_lab = time.time() - t0
     log(f"labels from fixed nprobe={label_np} over {a.train_queries} queries in {t_lab:.1f}s")
     t0 = time.time()
-    m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=1024, seed=7)
-    losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=1e-3, pos_weight=8.0, log=log)
+    # inputs scaled to unit variance per coordinate (rotated unit-norm queries)
+    m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=1024, seed=7,
+                 input_scale=float(np.sqrt(idx.D)))
+    losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=1e-3, pos_weight=4.0, log=log)
     torch.cuda.synchronize()
     t_train = time.time() - t0
     del qt_r, lab
@@ -306,7 +308,7 @@ def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np)
                                     0.5, stats=stats if timed else None)
 
     sweep = []
-    for mass in (0.5, 0.7, 0.8, 0.9, 0.95, 0.97, 0.98, 0.99, 0.995, 0.998):
+    for mass in (0.9, 0.95, 0.97, 0.975, 0.98, 0.985, 0.99, 0.995, 0.998):
         ids, _ = lstep(mass)
         torch.cuda.synchronize()
         rec = data.recall_at_k(ids.cpu().numpy().view(np.uint32), gt, a.k)
