@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device copy first (HBM counter calibration)")
+    ap.add_argument("--profile-plain", type=int, default=0, metavar="NPROBE",
+                    help="only time the plain path (rotate, probe, scan) at this nprobe and "
+                         "print its roofline: the command the kernel profiles are taken with")
     ap.add_argument("--sweep-only", action="store_true",
                     help="print the recall sweep of both paths and exit (no timing)")
     return ap.parse_args()
@@ -105,6 +108,11 @@ def main():
     x, q, idx, mix = build_workload(a, torch)
     g = IVF.from_index(idx)
     D, EX = idx.D, idx.EX
+    if a.profile_plain:
+        del x
+        profile_plain(a, g, q, idx, torch)
+        g.close()
+        return
 
     t0 = time.time()
     gt, _ = data.exact_knn_torch(x, q, a.k)
@@ -224,8 +232,7 @@ def main():
     ms_per_step = a.nq / qps * 1e3
 
     # roofline of the plain path's dominant kernel (the scan) at its upper bracketing nprobe
-    scan_bytes = th["stats"][0] * (4 * D + 128) + th["stats"][2] * (D * EX // 8 + 8)
-    achieved = scan_bytes / (th["scan_ms"] / 1e3) / 1e9
+    roofline = scan_roofline(th["stats"], th["scan_ms"], D, EX)
 
     cpu = cpu_baseline(idx, q, a, hi[0], torch)
     out = {
@@ -262,14 +269,58 @@ def main():
             "stats_at_hi": r["th"]["stats"],
             "sweep": [[n, round(x, 4)] for n, x in r["sweep"]]}
             for name, r in (("plain", plain), ("batched", batched))},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "scan_kernel",
-                     "algorithmic_bytes": int(scan_bytes)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     g.close()
+
+
+def scan_roofline(stats, scan_ms, D, EX):
+    """Algorithmic bytes of the per-query scan (every probed block's 4D code bytes and 32
+    norms, plus each survivor's EX-bit code and two factors) over its measured time."""
+    scan_bytes = stats[0] * (4 * D + 128) + stats[2] * (D * EX // 8 + 8)
+    achieved = scan_bytes / (scan_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "scan_mfma_kernel", "algorithmic_bytes": int(scan_bytes)}
+
+
+def profile_plain(a, g, q, idx, torch):
+    """The plain path alone at one nprobe: W untimed and K timed steps, HIP events around
+    each stage on the stream the kernels run on."""
+    nprobe = a.profile_plain
+    stream = torch.cuda.current_stream()
+    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
+    off = torch.arange(0, (a.nq + 1) * nprobe, nprobe, dtype=torch.int32, device="cuda")
+
+    def one(st):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record(stream)
+        qr = g.rotate_dev(q)
+        ev[1].record(stream)
+        pid, pd = g.probe_dev(qr, nprobe)
+        ev[2].record(stream)
+        g.scan_dev(qr, a.k, off, pid, pd, stats=st)
+        ev[3].record(stream)
+        return ev
+
+    for _ in range(a.warmup):
+        one(None)
+    torch.cuda.synchronize()
+    evs = [one(stats) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    ms = [sum(e[i].elapsed_time(e[i + 1]) for e in evs) / a.steps for i in range(3)]
+    st = (stats.cpu().numpy() // a.steps).tolist()
+    D, EX = idx.D, idx.EX
+    flop_rot = 2.0 * a.nq * D * D
+    flop_cen = 2.0 * a.nq * idx.K * D
+    print(json.dumps({"profile": "plain", "nprobe": nprobe, "nq": a.nq, "steps": a.steps,
+                      "rotate_ms": round(ms[0], 3), "probe_ms": round(ms[1], 3),
+                      "scan_ms": round(ms[2], 3), "stats": st,
+                      "rotation_tflops": round(flop_rot / ms[0] / 1e9, 1),
+                      "roofline": scan_roofline(st, ms[2], D, EX),
+                      "centroid_gemm_flop": flop_cen}), flush=True)
 
 
 def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np):

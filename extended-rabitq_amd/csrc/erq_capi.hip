@@ -74,9 +74,9 @@ int upload(T** dst, const T* src, size_t count, const char* what) {
 void release(erq_index* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* ptrs[] = {h->short_data, h->long_code, h->ex_factor, h->ids, h->centroids,
-                    h->cnorm,      h->P,         h->csize,     h->bstart, h->vstart, h->ws,
-                    h->bp};
+    void* ptrs[] = {h->short_data, h->long_code, h->ex_factor, h->ids,    h->centroids,
+                    h->cnorm,      h->P,         h->csize,     h->bstart, h->vstart,
+                    h->ws,         h->bp,        h->order};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -394,6 +394,28 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
     }
     // HASearcher.hpp:56
     a.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
+    a.qorder = nullptr;
+    if (h->plain_order && nq > 1) {
+        // queries grouped by first probed cluster: the result of each query is unchanged
+        const size_t need = nq + h->K + 1;
+        if (need > h->order_cap) {
+            if (h->order) {
+                hipError_t e = hipDeviceSynchronize();  // earlier calls may still read it
+                if (e != hipSuccess) return hip_fail(e, "order sync");
+                (void)hipFree(h->order);
+                h->order = nullptr;
+                h->order_cap = 0;
+            }
+            const size_t want = need + need / 4;
+            hipError_t e = hipMalloc(&h->order, want * 4);
+            if (e != hipSuccess) return hip_fail(e, "order alloc");
+            h->order_cap = want;
+        }
+        hipError_t e = launch_query_order(off, pid, (int)nq, (int)h->K, h->order,
+                                          h->order + h->K + 1, s);
+        if (e != hipSuccess) return hip_fail(e, "query order");
+        a.qorder = h->order + h->K + 1;
+    }
     hipError_t e = launch_scan(a, s);
     if (e != hipSuccess) return hip_fail(e, "scan");
     return ERQ_OK;
@@ -417,6 +439,10 @@ int erq_set_option(erq_index* h, int option, long value) {
         h->batch_wg_per_cu = (int)value;
         return ERQ_OK;
     }
+    if (option == ERQ_OPT_PLAIN_ORDER && (value == 0 || value == 1)) {
+        h->plain_order = (int)value;
+        return ERQ_OK;
+    }
     if (option == ERQ_OPT_BATCH_SURVIVOR_CAP && value >= 0 && value <= (1L << 30)) {
         h->batch_surv_cap = (size_t)value;
         return ERQ_OK;
@@ -432,6 +458,7 @@ long erq_get_option(const erq_index* h, int option) {
     case ERQ_OPT_PLAIN_WAVES: return h->plain_waves;
     case ERQ_OPT_BATCH_WG_PER_CU: return h->batch_wg_per_cu;
     case ERQ_OPT_BATCH_SURVIVOR_CAP: return (long)h->batch_surv_cap;
+    case ERQ_OPT_PLAIN_ORDER: return h->plain_order;
     default: return -1;
     }
 }

@@ -30,6 +30,9 @@ struct erq_index {
     void* ws = nullptr;
     size_t ws_bytes = 0;
     uint8_t* bp = nullptr;  // bit-plane copy of the 1-bit codes (batched path, built on first use)
+    uint32_t* order = nullptr;  // per-query scan order scratch (counts, cursor, order)
+    size_t order_cap = 0;       // u32 entries of `order`
+    int plain_order = 1;        // ERQ_OPT_PLAIN_ORDER
     int num_cu = 0;
     int plain_kernel = 0;   // ERQ_OPT_PLAIN_KERNEL
     int plain_waves = 1;    // ERQ_OPT_PLAIN_WAVES
@@ -66,8 +69,12 @@ struct ScanArgs {
     int nchunk;         // 512-dim chunks of bp
     int waves_per_block;  // queries (waves) per workgroup
     float one_over_sqrtD;
+    const uint32_t* qorder;  // nullable: the order queries are taken in (a permutation of nq)
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
+// qorder[nq] = queries grouped by their first probed cluster; scratch holds K + 1 u32.
+hipError_t launch_query_order(const uint32_t* probe_off, const uint32_t* probe_ids, int nq, int K,
+                              uint32_t* scratch, uint32_t* qorder, hipStream_t s);
 
 // C[m][n] = sum_k A[m][k] * B(k, n); B row-major [K x N] (b_trans = 0) or stored as
 // B^T row-major [N x K] (b_trans = 1).  epi = 0: store; epi = 1: rn[m] + cn[n] - 2*acc.

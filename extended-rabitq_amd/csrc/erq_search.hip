@@ -16,6 +16,70 @@ namespace erq {
 
 namespace {
 
+// The query a wave scans.  With an order (queries grouped by first probed cluster), the
+// workgroups on one XCD (workgroup b runs on XCD b % 8) take a contiguous stretch of it,
+// so concurrently scanned queries share probed clusters in that XCD's L2.  -1: none.
+__device__ __forceinline__ int query_of_wave(const ScanArgs& a, int wid) {
+    const int waves = blockDim.x >> 6;
+    if (!a.qorder) {
+        const int qi = blockIdx.x * waves + wid;
+        return qi < a.nq ? qi : -1;
+    }
+    const int nwg = gridDim.x, x = blockIdx.x & 7;
+    int start = 0;  // slots of the XCDs before x
+    for (int y = 0; y < x; y++) start += (nwg - y + 7) / 8;
+    const int slot = (start + (blockIdx.x >> 3)) * waves + wid;
+    return slot < a.nq ? (int)a.qorder[slot] : -1;
+}
+
+__device__ __forceinline__ uint32_t first_probe(const uint32_t* off, const uint32_t* pid, int q,
+                                                int K) {
+    const uint32_t s = off[q], e = off[q + 1];
+    return s < e ? min(pid[s], (uint32_t)K) : (uint32_t)K;
+}
+
+__global__ void order_zero_kernel(uint32_t* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void order_count_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pid,
+                                   int nq, int K, uint32_t* __restrict__ counts) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq) atomicAdd(&counts[first_probe(off, pid, q, K)], 1u);
+}
+
+// Single workgroup: counts[0..n) -> exclusive prefix sums, in place.
+__global__ void __launch_bounds__(1024) order_scan_kernel(uint32_t* counts, int n) {
+    __shared__ uint32_t sc[1024];
+    __shared__ uint32_t carry;
+    const int t = threadIdx.x;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const uint32_t v = base + t < n ? counts[base + t] : 0;
+        sc[t] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const uint32_t u = t >= o ? sc[t - o] : 0;
+            __syncthreads();
+            sc[t] += u;
+            __syncthreads();
+        }
+        if (base + t < n) counts[base + t] = carry + sc[t] - v;
+        __syncthreads();
+        if (t == 1023) carry += sc[1023];
+        __syncthreads();
+    }
+}
+
+__global__ void order_scatter_kernel(const uint32_t* __restrict__ off,
+                                     const uint32_t* __restrict__ pid, int nq, int K,
+                                     uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq) order[atomicAdd(&cursor[first_probe(off, pid, q, K)], 1u)] = (uint32_t)q;
+}
+
+
 struct WaveLds {
     float* uq;
     uint16_t* lut;
@@ -71,8 +135,8 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs a, int per_wave_byte
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    const int qi = blockIdx.x * (blockDim.x >> 6) + wid;
-    if (qi >= a.nq) return;
+    const int qi = query_of_wave(a, wid);
+    if (qi < 0) return;
     const int D = a.D, M = D >> 2, k = a.k;
     const int kcap = (k + 63) & ~63;
     unsigned char* base = smem + (size_t)wid * per_wave_bytes;
@@ -303,19 +367,23 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs a, int per_wave_byte
 // and pool are the per-query kernel's.
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-template <int EX>
+template <int EX, int NCH>
 __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    const int qi = blockIdx.x * (blockDim.x >> 6) + wid;
-    if (qi >= a.nq) return;
-    const int D = a.D, k = a.k, nchunk = a.nchunk;
+    const int qi = query_of_wave(a, wid);
+    if (qi < 0) return;
+    const int D = a.D, k = a.k;
+    constexpr int nchunk = NCH;
     const int kcap = (k + 63) & ~63;
     unsigned char* base = smem + (size_t)wid * per_wave_bytes;
     float* uq = reinterpret_cast<float*>(base);
     int8_t* A = reinterpret_cast<int8_t*>(base + 4 * (size_t)D);          // nchunk * 1 KB
-    int* sums = reinterpret_cast<int*>(base + 4 * (size_t)D + 1024 * (size_t)nchunk);
+    // nchunk * 1 KB of zeros laid out like A: lanes holding MFMA rows 2..15 read their
+    // (zero) A operand from here at the same offsets, so every lane loads unconditionally
+    int8_t* Z = A + 1024 * (size_t)nchunk;
+    int* sums = reinterpret_cast<int*>(base + 4 * (size_t)D + 2048 * (size_t)nchunk);
     WaveLds w;
     w.uq = uq;
     w.lut = nullptr;
@@ -326,6 +394,8 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
         w.pd[i] = FLT_MAX;
         w.pid[i] = 0;
     }
+    for (int i = lane; i < nchunk * 64; i += kWave)
+        reinterpret_cast<uint4*>(Z)[i] = make_uint4(0, 0, 0, 0);
     int psize = 0;
     wave_sync();
 
@@ -336,6 +406,8 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
     const float const_bound = 0.58f;  // FastScan.hpp:295
     unsigned long long st_blocks = 0, st_vec = 0, st_surv = 0;
     const int g = lane >> 4, l16 = lane & 15;
+    // A operand of this lane at (chunk, plane) = abase + (chunk * 8 + plane) * 128
+    const int8_t* abase = l16 < 2 ? A + (g * 2 + l16) * 16 : Z + g * 32;
 
     const uint32_t p0 = a.probe_off[qi];
     uint32_t p1 = a.probe_off[qi + 1];
@@ -396,50 +468,72 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
         const float c1 = (float)(0.5 * (double)sumq - (double)const_bound);
         const float c2 = (-5.0f * y) * a.one_over_sqrtD;
         const float qs = y * y;
+        // codes and norms of block b + 1 are loaded while block b is on the MFMA
+        uint4 wn0[NCH], wn1[NCH];
+        float on_n = 0.f;
+        auto load_block = [&](uint32_t blk) {
+#pragma unroll
+            for (int ch = 0; ch < NCH; ch++) {
+                const uint8_t* src = a.bp + (((size_t)blk * NCH + ch) * 32 + l16) * 64 + 16 * g;
+                wn0[ch] = *reinterpret_cast<const uint4*>(src);
+                wn1[ch] = *reinterpret_cast<const uint4*>(src + 16 * 64);
+            }
+            on_n = lane < 32 ? reinterpret_cast<const float*>(a.short_data + (size_t)blk * BB +
+                                                              4 * (size_t)D)[lane]
+                             : 0.f;
+        };
+        if (nb > 0) load_block(blk0);
         for (int b = 0; b < nb; ++b) {
             const uint32_t blk = blk0 + (uint32_t)b;
             const int np = min(32, n - b * 32);
-            v4i acc0[8], acc1[8];
-            for (int pp = 0; pp < 8; pp++) acc0[pp] = acc1[pp] = v4i{0, 0, 0, 0};
-            for (int ch = 0; ch < nchunk; ch++) {
-                const uint8_t* src = a.bp + (((size_t)blk * nchunk + ch) * 32 + l16) * 64 + 16 * g;
-                const uint4 w0 = *reinterpret_cast<const uint4*>(src);
-                const uint4 w1 = *reinterpret_cast<const uint4*>(src + 16 * 64);
-                for (int pp = 0; pp < 8; pp++) {
-                    v4i av = v4i{0, 0, 0, 0};
-                    if (l16 < 2)
-                        av = *reinterpret_cast<const v4i*>(
-                            A + ((((size_t)ch * 8 + pp) * 4 + g) * 2 + l16) * 16);
-                    const uint32_t m = 0x01010101u << pp;
-                    const v4i b0 = v4i{(int)(w0.x & m), (int)(w0.y & m), (int)(w0.z & m),
-                                       (int)(w0.w & m)};
-                    const v4i b1 = v4i{(int)(w1.x & m), (int)(w1.y & m), (int)(w1.z & m),
-                                       (int)(w1.w & m)};
-                    acc0[pp] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b0, acc0[pp], 0, 0, 0);
-                    acc1[pp] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b1, acc1[pp], 0, 0, 0);
+            uint4 wc0[NCH], wc1[NCH];
+#pragma unroll
+            for (int ch = 0; ch < NCH; ch++) {
+                wc0[ch] = wn0[ch];
+                wc1[ch] = wn1[ch];
+            }
+            const float on_l = on_n;  // lane < 32: norm of vector `lane` of this block
+            if (b + 1 < nb) load_block(blk + 1);
+            // plane by plane: the NCH-chunk MFMA chain of plane pp for both 16-vector tiles,
+            // then folded into the running sums (plane 7 carries the sign: 0x80 is -128)
+            int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+#pragma unroll 2
+            for (int pp = 0; pp < 8; pp++) {
+                v4i acc0 = v4i{0, 0, 0, 0}, acc1 = v4i{0, 0, 0, 0};
+                const uint32_t m = 0x01010101u << pp;
+#pragma unroll
+                for (int ch = 0; ch < NCH; ch++) {
+                    const v4i av = *reinterpret_cast<const v4i*>(abase + (ch * 8 + pp) * 128);
+                    const v4i b0 = v4i{(int)(wc0[ch].x & m), (int)(wc0[ch].y & m),
+                                       (int)(wc0[ch].z & m), (int)(wc0[ch].w & m)};
+                    const v4i b1 = v4i{(int)(wc1[ch].x & m), (int)(wc1[ch].y & m),
+                                       (int)(wc1[ch].z & m), (int)(wc1[ch].w & m)};
+                    acc0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b0, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b1, acc1, 0, 0, 0);
+                }
+                if (pp < 7) {
+                    lo0 += acc0[0] >> pp;
+                    hi0 += acc0[1] >> pp;
+                    lo1 += acc1[0] >> pp;
+                    hi1 += acc1[1] >> pp;
+                } else {
+                    lo0 -= acc0[0] >> 7;
+                    hi0 -= acc0[1] >> 7;
+                    lo1 -= acc1[0] >> 7;
+                    hi1 -= acc1[1] >> 7;
                 }
             }
             // rows 0 (low digits) and 1 (high digits) of column l16 live in lanes 0..15
             if (lane < 16) {
-                int lo0 = -(acc0[7][0] >> 7), hi0 = -(acc0[7][1] >> 7);
-                int lo1 = -(acc1[7][0] >> 7), hi1 = -(acc1[7][1] >> 7);
-                for (int pp = 0; pp < 7; pp++) {
-                    lo0 += acc0[pp][0] >> pp;
-                    hi0 += acc0[pp][1] >> pp;
-                    lo1 += acc1[pp][0] >> pp;
-                    hi1 += acc1[pp][1] >> pp;
-                }
                 sums[lane] = 128 * hi0 + lo0;
                 sums[16 + lane] = 128 * hi1 + lo1;
             }
             wave_sync();
-            const float* onorm = reinterpret_cast<const float*>(a.short_data + (size_t)blk * BB +
-                                                                4 * (size_t)D);
             bool cand = false;
             float ipq_l = 0.f;
             if (lane < 32) {
                 ipq_l = (float)sums[lane] * delta;
-                const float on = onorm[lane];
+                const float on = on_l;
                 float t = ipq_l - c1;
                 t = t * c2;
                 t = t * on;
@@ -468,10 +562,11 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
                 }
                 const float ip = butterfly16(part);
                 const float ipq_j = __shfl(ipq_l, jj);
+                const float on_j = __shfl(on_l, jj);
                 float exd = 0.f;
                 uint32_t idv = 0;
                 if (j >= 0) {
-                    const float on = onorm[jj];
+                    const float on = on_j;
                     const float sqr_x = on * on;
                     const float Aa = sqr_x + sqr_y;
                     const float Bb = a.ex_factor[vi] * y;
@@ -511,12 +606,17 @@ template <int EX>
 hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
     int waves = a.waves_per_block > 0 ? a.waves_per_block : 1;
     if (a.bp && a.D <= 2048) {
-        const size_t per_wave = 4 * (size_t)a.D + 1024 * (size_t)a.nchunk + 128 +
+        const size_t per_wave = 4 * (size_t)a.D + 2048 * (size_t)a.nchunk + 128 +
                                 (size_t)8 * ((a.k + 63) & ~63);
         while (waves > 1 && per_wave * waves > 65536) waves--;
         dim3 grid((a.nq + waves - 1) / waves), block(64 * waves);
-        hipLaunchKernelGGL((scan_mfma_kernel<EX>), grid, block, per_wave * waves, s, a,
-                           (int)per_wave);
+        switch (a.nchunk) {
+        case 1: hipLaunchKernelGGL((scan_mfma_kernel<EX, 1>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
+        case 2: hipLaunchKernelGGL((scan_mfma_kernel<EX, 2>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
+        case 3: hipLaunchKernelGGL((scan_mfma_kernel<EX, 3>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
+        case 4: hipLaunchKernelGGL((scan_mfma_kernel<EX, 4>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
+        default: return hipErrorInvalidValue;
+        }
         return hipGetLastError();
     }
     const size_t per_wave = (size_t)12 * a.D + 128 + (size_t)8 * ((a.k + 63) & ~63);
@@ -531,6 +631,19 @@ hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
 }
 
 }  // namespace
+
+hipError_t launch_query_order(const uint32_t* probe_off, const uint32_t* probe_ids, int nq, int K,
+                              uint32_t* scratch, uint32_t* qorder, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    const int nb = K + 1;
+    hipLaunchKernelGGL(order_zero_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, scratch, nb);
+    hipLaunchKernelGGL(order_count_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, probe_off,
+                       probe_ids, nq, K, scratch);
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, s, scratch, nb);
+    hipLaunchKernelGGL(order_scatter_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, probe_off,
+                       probe_ids, nq, K, scratch, qorder);
+    return hipGetLastError();
+}
 
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s) {
     switch (a.EX) {

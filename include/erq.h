@@ -47,7 +47,11 @@ enum {
     /* Records (16 bytes each) of the batched search's survivor list per round, 0..2^30
      * (default 2^24).  Survivors beyond it are re-estimated inside the scan kernel
      * instead of by the separate re-estimation kernel; results are identical. */
-    ERQ_OPT_BATCH_SURVIVOR_CAP = 4
+    ERQ_OPT_BATCH_SURVIVOR_CAP = 4,
+    /* Order in which the per-query scan takes queries: 1 (default) grouped by first probed
+     * cluster, so queries scanned at the same time share clusters in L2; 0 as given.
+     * Results are identical. */
+    ERQ_OPT_PLAIN_ORDER = 5
 };
 int erq_set_option(erq_index* h, int option, long value);
 long erq_get_option(const erq_index* h, int option);

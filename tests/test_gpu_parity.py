@@ -202,3 +202,23 @@ def test_centroid_distance_gemm():
     scale = (Q * Q).sum(1)[:, None] + (C * C).sum(1)[None, :]
     assert np.all(np.abs(a - ref) <= (4 * idx.D + 64) * 2.0 ** -24 * scale)
     g.close()
+
+
+def test_query_order_does_not_change_results():
+    """ERQ_OPT_PLAIN_ORDER (queries taken grouped by first probed cluster) only reorders
+    work: ids, distances and counters equal those of the order as given."""
+    from erq.gpu import IVF
+    torch = _torch()
+    idx, q = _build(("d128", 4), 20000, 128, 64, 4)
+    qr = torch.from_numpy(rotate_np(idx, q)).cuda()
+    g = IVF.from_index(idx)
+    out = []
+    for order in (0, 1):
+        g.set_option(5, order)
+        st = torch.zeros(8, dtype=torch.int64, device="cuda")
+        ids, d = g.search_dev(qr, 100, 12, stats=st)
+        torch.cuda.synchronize()
+        out.append((ids.cpu(), d.cpu(), st.cpu()))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    assert torch.equal(out[0][2], out[1][2])
+    g.close()
