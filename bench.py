@@ -13,6 +13,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -56,6 +57,9 @@ def parse():
     ap.add_argument("--profile-plain", type=int, default=0, metavar="NPROBE",
                     help="only time the plain path (rotate, probe, scan) at this nprobe and "
                          "print its roofline: the command the kernel profiles are taken with")
+    ap.add_argument("--profile-learned", type=float, default=0.0, metavar="MASS",
+                    help="train the probe predictor, then only time the learned batched search "
+                         "at this mass (the command its kernel profile is taken with)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="print the recall sweep of both paths and exit (no timing)")
     return ap.parse_args()
@@ -111,6 +115,11 @@ def main():
     if a.profile_plain:
         del x
         profile_plain(a, g, q, idx, torch)
+        g.close()
+        return
+    if a.profile_learned:
+        del x
+        profile_learned(a, g, q, idx, mix, torch)
         g.close()
         return
 
@@ -233,6 +242,7 @@ def main():
 
     # roofline of the plain path's dominant kernel (the scan) at its upper bracketing nprobe
     roofline = scan_roofline(th["stats"], th["scan_ms"], D, EX)
+    roofline["traffic"], roofline["traffic_source"] = profiled_traffic(a, hi[0])
 
     cpu = cpu_baseline(idx, q, a, hi[0], torch)
     out = {
@@ -286,6 +296,21 @@ def scan_roofline(stats, scan_ms, D, EX):
             "kernel": "scan_mfma_kernel", "algorithmic_bytes": int(scan_bytes)}
 
 
+def profiled_traffic(a, nprobe):
+    """HBM bytes per scan launch from the committed rocprofv3 counter summary of this exact
+    workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None if there is none."""
+    if a.nq != 20000 or a.n % 1_000_000:
+        return None, None
+    name = (f"scan_mfma_kernel_plain_d{a.dim}_n{a.n // 1_000_000}m_b{a.bits}_"
+            f"nprobe{nprobe}.md")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    try:
+        m = re.search(r"traffic_bytes_per_launch: (\d+)", open(path).read())
+    except OSError:
+        return None, None
+    return (int(m.group(1)), "profiles/" + name) if m else (None, None)
+
+
 def profile_plain(a, g, q, idx, torch):
     """The plain path alone at one nprobe: W untimed and K timed steps, HIP events around
     each stage on the stream the kernels run on."""
@@ -323,9 +348,9 @@ def profile_plain(a, g, q, idx, torch):
                       "centroid_gemm_flop": flop_cen}), flush=True)
 
 
-def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np):
-    """Train the probe predictor on labels from fixed-nprobe search over separate training
-    queries, then sweep the mass threshold and report QPS at the recall target."""
+def train_probe_model(a, g, idx, mix, torch):
+    """Labels from the fixed-nprobe batched search over separate training queries, then the
+    probe MLP trained on them.  Returns (model, label_nprobe, losses, label_s, train_s)."""
     from erq import data
     from erq.probe_model import ProbeMLP, cluster_of_ids
     basis, centres = mix
@@ -349,8 +374,40 @@ def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np)
                  input_scale=float(np.sqrt(idx.D)))
     losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=1e-3, pos_weight=4.0, log=log)
     torch.cuda.synchronize()
-    t_train = time.time() - t0
-    del qt_r, lab
+    return m, label_np, losses, t_lab, time.time() - t0
+
+
+def profile_learned(a, g, q, idx, mix, torch):
+    m, label_np, _, _, _ = train_probe_model(a, g, idx, mix, torch)
+    stream = torch.cuda.current_stream()
+    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
+
+    def one(st):
+        return g.search_learned_dev(m, g.rotate_dev(q), a.k, a.profile_learned, 4, label_np,
+                                    192, 0.5, stats=st)
+
+    for _ in range(a.warmup):
+        one(None)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(a.steps):
+        one(stats)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    print(json.dumps({"profile": "learned", "mass": a.profile_learned, "nq": a.nq,
+                      "steps": a.steps, "ms_per_search": round(ms, 3),
+                      "qps": round(a.nq / ms * 1e3, 1),
+                      "stats": (stats.cpu().numpy() // a.steps).tolist()}), flush=True)
+    m.close()
+
+
+def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np):
+    """Train the probe predictor on labels from fixed-nprobe search over separate training
+    queries, then sweep the mass threshold and report QPS at the recall target."""
+    from erq import data
+    m, label_np, losses, t_lab, t_train = train_probe_model(a, g, idx, mix, torch)
     max_np = label_np
 
     def lstep(mass, timed=False):
