@@ -198,6 +198,7 @@ def main():
                         probe_ms=t_probe / a.steps, scan_ms=t_scan / a.steps, stats=st)
 
         th = timed(hi[0])
+        th.update(scan_volume(g, q, idx, hi[0], th, torch))
         log(f"[{path}] timed nprobe={hi[0]}: {th}")
         qps_hi = a.nq / (th["wall_ms"] / 1e3)
         tl = None
@@ -235,6 +236,11 @@ def main():
                                       fixed_np=(batched or plain)["hi"][0])
         except Exception as e:  # reported, never hidden
             log(f"learned path failed: {e!r}")
+    uneven = None
+    try:
+        uneven = measure_uneven(a, g, q, gt, torch)
+    except Exception as e:  # reported, never hidden
+        log(f"uneven-nprobe leg failed: {e!r}")
     best = batched if batched is not None and batched["qps"] > plain["qps"] else plain
     qps = best["qps"]
     hi, lo, th = plain["hi"], plain["lo"], plain["th"]
@@ -243,6 +249,16 @@ def main():
     # roofline of the plain path's dominant kernel (the scan) at its upper bracketing nprobe
     roofline = scan_roofline(th["stats"], th["scan_ms"], D, EX)
     roofline["traffic"], roofline["traffic_source"] = profiled_traffic(a, hi[0])
+
+    # the drop-in's host-pointer entry (erq_search_batch): rotated queries from host memory,
+    # results back to host memory, device buffers allocated per call -- PCIe-inclusive
+    qr_host = g.rotate_dev(q).cpu().numpy()
+    g.search_batch(qr_host, a.k, hi[0])
+    t0 = time.perf_counter()
+    g.search_batch(qr_host, a.k, hi[0])
+    host_qps = a.nq / (time.perf_counter() - t0)
+    log(f"[host] erq_search_batch at nprobe {hi[0]}: {host_qps:.0f} queries/s")
+    del qr_host
 
     cpu = cpu_baseline(idx, q, a, hi[0], torch)
     out = {
@@ -264,6 +280,8 @@ def main():
                    ("batched" if best is batched else "plain"),
                    "sigma": a.sigma},
         "fixed_nprobe_qps_at_target": round(qps, 1),
+        "host_pointer_plain_qps": round(host_qps, 1),
+        "uneven_nprobe_graph": uneven,
         "learned": None if learned is None else {
             "qps_at_target": round(learned["qps"], 1),
             "vs_fixed": round(learned["qps"] / qps, 3),
@@ -277,6 +295,8 @@ def main():
                                 "probe": round(r["th"]["probe_ms"], 3),
                                 "scan": round(r["th"]["scan_ms"], 3)},
             "stats_at_hi": r["th"]["stats"],
+            "scan_algorithmic_gbs_at_hi": r["th"].get("algorithmic_gbs"),
+            "scan_unique_bytes_at_hi": r["th"].get("unique_bytes"),
             "sweep": [[n, round(x, 4)] for n, x in r["sweep"]]}
             for name, r in (("plain", plain), ("batched", batched))},
         "roofline": roofline,
@@ -294,6 +314,68 @@ def scan_roofline(stats, scan_ms, D, EX):
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": "scan_mfma_kernel", "algorithmic_bytes": int(scan_bytes)}
+
+
+def scan_volume(g, q, idx, nprobe, th, torch):
+    """Algorithmic scan bytes of one step at this nprobe (every (query, cluster) pair's
+    blocks plus each survivor's EX-bit code and factors), the same bytes counted once per
+    distinct probed cluster, and the algorithmic rate over the measured scan time."""
+    D, EX = idx.D, idx.EX
+    pid, _ = g.probe_dev(g.rotate_dev(q), nprobe)
+    nblk = torch.from_numpy(((np.diff(idx.vec_start()) + 31) // 32).astype(np.int64)).cuda()
+    p = pid.long().reshape(-1)
+    blocks = int(nblk[p].sum())
+    unique_blocks = int(nblk[torch.unique(p)].sum())
+    alg = blocks * (4 * D + 128) + th["stats"][2] * (D * EX // 8 + 8)
+    return {"pair_blocks": blocks, "algorithmic_bytes": alg,
+            "algorithmic_gbs": round(alg / (th["scan_ms"] / 1e3) / 1e9, 1),
+            "unique_bytes": unique_blocks * (4 * D + 128)}
+
+
+def measure_uneven(a, g, q, gt, torch):
+    """Per-query nprobe drawn uniformly from 8..256 (seed 6), the whole search (rotation,
+    probe lists, batched scan) captured once in a HIP graph and replayed; launch shapes from
+    the autotuner."""
+    from erq import data
+    rng = np.random.default_rng(6)
+    npq = torch.from_numpy(rng.integers(8, 257, a.nq).astype(np.int64)).cuda()
+    qr0 = g.rotate_dev(q)
+    tuned = g.autotune(qr0, a.k, 128)
+    cols = torch.arange(256, device="cuda")
+    off = torch.zeros(a.nq + 1, dtype=torch.int64, device="cuda")
+    off[1:] = torch.cumsum(npq, 0)
+    off32 = off.to(torch.int32)
+    keep = torch.nonzero((cols[None, :] < npq[:, None]).reshape(-1)).reshape(-1)
+
+    def run():  # capturable: fixed shapes, no host synchronisation
+        qr = g.rotate_dev(q)
+        pid, pd = g.probe_dev(qr, 256)
+        return g.scan_batched_dev(qr, a.k, off32, pid.reshape(-1).index_select(0, keep),
+                                  pd.reshape(-1).index_select(0, keep), 256)
+
+    ids, _ = run()  # eager: sizes the workspace and the boolean-mask outputs
+    torch.cuda.synchronize()
+    rec = data.recall_at_k(ids.cpu().numpy().view(np.uint32), gt, a.k)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        gids, _ = run()
+    graph.replay()
+    torch.cuda.synchronize()
+    same = bool(torch.equal(gids, ids))
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(a.steps):
+        graph.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    log(f"[uneven] nprobe U[8,256] recall@{a.k}={rec:.4f} graph replay {ms:.2f} ms "
+        f"({a.nq / ms * 1e3:.0f} q/s), graph == eager: {same}")
+    return {"qps": round(a.nq / ms * 1e3, 1), "recall": round(rec, 4), "ms": round(ms, 3),
+            "mean_nprobe": round(float(npq.float().mean()), 1), "graph_equals_eager": same,
+            "autotune_ms": [round(x, 3) for x in tuned],
+            "plain_waves": g.option(2), "batch_wg_per_cu": g.option(3)}
 
 
 def profiled_traffic(a, nprobe):
@@ -481,9 +563,19 @@ def cpu_baseline(idx, q, a, nprobe, torch):
         n += 1
     dt = time.perf_counter() - t0
     log(f"cpu baseline: {n} queries in {dt:.1f}s (1 thread)")
-    return {"value": round(n / dt, 2), "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"{n} queries of the same batch, nprobe={nprobe}, k={a.k}, single thread, "
-                      f"scalar C oracle"}
+    out = {"value": round(n / dt, 2), "unit": "queries/s", "cores": 1, "kind": "port",
+           "sample": f"{n} queries of the same batch, nprobe={nprobe}, k={a.k}, single thread, "
+                     f"scalar C oracle"}
+    # all host cores (OpenMP over queries) on a bounded sample sized from the 1-thread rate
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 10**6)
+    m = int(min(qr.shape[0], max(threads, n / dt * threads * 2.0)))
+    t0 = time.perf_counter()
+    oi.search_batch(qr[:m], a.k, nprobe, threads=threads)
+    dt = time.perf_counter() - t0
+    log(f"cpu baseline: {m} queries in {dt:.1f}s ({threads} threads)")
+    out["all_cores"] = {"value": round(m / dt, 2), "cores": threads,
+                        "sample": f"{m} queries, OpenMP over queries"}
+    return out
 
 
 if __name__ == "__main__":

@@ -189,6 +189,10 @@ struct QParam {
     int q;
 };
 
+// u[d] lives at uq[swz(d)]: one 8-float pad per 128 floats, so that the quantization loop's
+// lanes (whose dims differ by multiples of 128) hit different LDS banks
+__device__ __forceinline__ int swz(int d) { return d + 8 * (d >> 7); }
+
 __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -200,7 +204,6 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
     if (e >= a.n_items[2]) return;
     const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512;
     float* uq = reinterpret_cast<float*>(smem + (size_t)wid * per_wave_bytes);
-    int8_t* dig = reinterpret_cast<int8_t*>(uq + D);
     const uint32_t slot = a.entries[e];
     const int q = query_of_slot(a.probe_off, a.nq, slot);
     const uint32_t cid = a.probe_ids[slot];
@@ -214,51 +217,61 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
     if (P.y > 1e-5f) {
         P.fac = (float)(1.0 / (double)P.y);
 #pragma unroll 4
-        for (int d = lane; d < D; d += 64) uq[d] = (qv[d] - cen[d]) * P.fac;
+        for (int d = lane; d < D; d += 64) uq[swz(d)] = (qv[d] - cen[d]) * P.fac;
         wave_sync();
         float part = 0.f;
         if (lane < 16) {
 #pragma unroll 16
-            for (int d = lane; d < D; d += 16) part = part + uq[d];
+            for (int d = lane; d < D; d += 16) part = part + uq[swz(d)];
         }
         part = butterfly16(part);
         sumq = __shfl(part, 0);
     } else {
         P.fac = 0.f;
         const float value = (float)(1.0 / (double)sqrtf((float)D));
-        for (int d = lane; d < D; d += 64) uq[d] = value;
+        for (int d = lane; d < D; d += 64) uq[swz(d)] = value;
         wave_sync();
         sumq = (float)D * value;
     }
     float mx = 0.f, mn = 0.f;
 #pragma unroll 4
     for (int d = lane; d < D; d += 64) {
-        float u = uq[d];
+        float u = uq[swz(d)];
         mx = fmaxf(mx, u);
         mn = fminf(mn, u);
     }
     mx = wave_max(mx);
     mn = wave_min(mn);
     const float delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
-#pragma unroll 4
-    for (int d = lane; d < Dp; d += 64) {
-        int qi = 0;
-        if (d < D) {
-            float tmp = uq[d] / delta;
-            double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
-            qi = (int)r;
-        }
-        const int c = d >> 9, dd = d & 511;
-        const int g = dd >> 7, jw = (dd >> 5) & 3, b = (dd >> 3) & 3, p = dd & 7;
-        int8_t* dst = dig + ((((size_t)c * 8 + p) * 4 + g) * 2) * 16 + 4 * jw + b;
-        const int lo = qi & 127;
-        dst[0] = (int8_t)lo;
-        dst[16] = (int8_t)((qi - lo) >> 7);
-    }
-    wave_sync();
+    // One 16-byte unit (chunk c, plane p, lane group g) per lane and step: dims
+    // 512c + 128g + 32jw + 8b + p at byte 4jw + b, low digits then high digits, written
+    // straight to the entry's operand rows in HBM.
     uint4* gd = reinterpret_cast<uint4*>(a.adig + (size_t)e * 2 * Dp);
-    const uint4* ld = reinterpret_cast<const uint4*>(dig);
-    for (int i = lane; i < 2 * Dp / 16; i += 64) gd[i] = ld[i];
+    for (int un = lane; un < nchunk * 32; un += 64) {
+        const int c = un >> 5, p = (un >> 2) & 7, g = un & 3;
+        uint32_t lo[4], hi[4];
+#pragma unroll
+        for (int jw = 0; jw < 4; jw++) {
+            uint32_t lw = 0, hw = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int d = 512 * c + 128 * g + 32 * jw + 8 * b + p;
+                int qi = 0;
+                if (d < D) {  // high_acc_quantize16, space.hpp:695-711
+                    float tmp = uq[swz(d)] / delta;
+                    double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
+                    qi = (int)r;
+                }
+                const int l7 = qi & 127;
+                lw |= (uint32_t)l7 << (8 * b);
+                hw |= ((uint32_t)((qi - l7) >> 7) & 0xFFu) << (8 * b);
+            }
+            lo[jw] = lw;
+            hi[jw] = hw;
+        }
+        gd[2 * un] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+        gd[2 * un + 1] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    }
     if (lane == 0) {
         P.sumq = sumq;
         P.delta = delta;
@@ -512,8 +525,9 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         }
     }
     if (tid == 0) a.surv_count[blockIdx.x] = (uint32_t)used;
-    if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[6], st_blocks);
+    // survivors and candidates were counted by the first lane of each 16-lane group
+    if (a.stats && ln == 0) {
+        if (lane == 0) atomicAdd(&a.stats[6], st_blocks);
         atomicAdd(&a.stats[2], st_surv);
         atomicAdd(&a.stats[4], st_cand);
     }
@@ -594,7 +608,7 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             st_surv++;
         }
     }
-    if (a.stats && lane == 0) {
+    if (a.stats && l16 == 0) {  // counted by the first lane of each 16-lane group
         atomicAdd(&a.stats[2], st_surv);
         atomicAdd(&a.stats[4], st_cand);
     }
@@ -651,17 +665,29 @@ __global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
             if ((kv & pmask) == prefix) atomicAdd(&hist[(kv >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t r = sh_rank, cum = 0, sel = 255;
-            for (int dg = 0; dg < 256; dg++) {
-                if (cum + hist[dg] >= r) {
-                    sel = (uint32_t)dg;
-                    break;
-                }
-                cum += hist[dg];
+        if (tid < 64) {  // the digit holding rank sh_rank: four bins per lane, shuffle scan
+            const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2],
+                           h3 = hist[4 * tid + 3];
+            const uint32_t own = h0 + h1 + h2 + h3;
+            uint32_t incl = own;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o);
+                if (tid >= o) incl += v;
             }
-            sh_sel = sel;
-            sh_rank = r - cum;
+            const uint32_t r = sh_rank, excl = incl - own;
+            if ((excl < r && r <= incl) || (tid == 63 && r > incl)) {
+                uint32_t cum = excl, sel = 4 * tid + 3;
+                const uint32_t hs[4] = {h0, h1, h2, h3};
+                for (int j = 0; j < 4; j++) {
+                    if (cum + hs[j] >= r) {
+                        sel = 4 * tid + j;
+                        break;
+                    }
+                    cum += hs[j];
+                }
+                sh_sel = sel;
+                sh_rank = r - cum;
+            }
         }
         __syncthreads();
         prefix |= sh_sel << shift;
@@ -740,7 +766,7 @@ size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize) {
     return (pairs / kGroup + 1) * max_split + N / tiles_per_item + K + 8;
 }
 
-size_t prep_wave_bytes(int D) { return 4 * (size_t)D + 2 * (size_t)bp_nchunk(D) * 512; }
+size_t prep_wave_bytes(int D) { return 4 * ((size_t)D + 8 * ((size_t)D / 128 + 1)); }
 
 hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;

@@ -40,6 +40,7 @@ struct erq_mlp {
     __bf16* dz1 = nullptr;
     float* tmp = nullptr;     // B x hid fp32 scratch
     float* loss = nullptr;    // 1
+    float* part = nullptr;    // partial sums of the loss and of the bias gradients
     std::mutex mu;
 };
 
@@ -192,8 +193,27 @@ __global__ void bce_kernel(const float* __restrict__ z, const float* __restrict_
         part += w * yi * sp_neg + (1.f - yi) * sp_pos;
         dz[i] = (__bf16)((w * yi * (p - 1.f) + (1.f - yi) * p) * inv_batch);
     }
+    // one partial per workgroup (summed in a fixed order by sum_partials_kernel)
+    __shared__ float wsum[4];
     for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, part * inv_batch);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) loss[blockIdx.x] = ((wsum[0] + wsum[1]) + (wsum[2] + wsum[3])) * inv_batch;
+}
+
+// out[0] = sum of in[0..n) in a fixed order (single workgroup of 256)
+__global__ void __launch_bounds__(256) sum_partials_kernel(const float* __restrict__ in, int n,
+                                                           float* __restrict__ out) {
+    __shared__ float sh[256];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n; i += 256) s += in[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 // dz = dh * gelu'(z), dh fp32 [B x H]
@@ -205,13 +225,37 @@ __global__ void gelu_back_kernel(const float* __restrict__ dh, const float* __re
 }
 
 // db[n] = sum_b dz[b][n]
-__global__ void colsum_kernel(const __bf16* __restrict__ dz, int B, int N,
-                              float* __restrict__ db) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+// Column sums of dz [B x N] in two passes with a fixed order: workgroup (column tile of 64,
+// row slice rs of kColSlices) -> part[rs][n]; then db[n] = sum over rs.
+constexpr int kColSlices = 16;
+
+__global__ void __launch_bounds__(256) colsum_partial_kernel(const __bf16* __restrict__ dz, int B,
+                                                             int N, float* __restrict__ part) {
+    __shared__ float sh[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + lane, rs = blockIdx.y;
+    const int r0 = (int)((long)B * rs / kColSlices), r1 = (int)((long)B * (rs + 1) / kColSlices);
+    float s = 0.f;
+    if (n < N)
+        for (int b = r0 + wv; b < r1; b += 4) s += (float)dz[(size_t)b * N + n];
+    sh[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && n < N) part[(size_t)rs * N + n] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
+}
+
+__global__ void colsum_final_kernel(const float* __restrict__ part, int N, float* __restrict__ db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     float s = 0.f;
-    for (int b = 0; b < B; b++) s += (float)dz[(size_t)b * N + n];
+    for (int rs = 0; rs < kColSlices; rs++) s += part[(size_t)rs * N + n];
     db[n] = s;
+}
+
+hipError_t colsum(const __bf16* dz, int B, int N, float* part, float* db, hipStream_t s) {
+    hipLaunchKernelGGL(colsum_partial_kernel, dim3((N + 63) / 64, kColSlices), dim3(256), 0, s, dz,
+                       B, N, part);
+    hipLaunchKernelGGL(colsum_final_kernel, dim3((N + 255) / 256), dim3(256), 0, s, part, N, db);
+    return hipGetLastError();
 }
 
 __global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
@@ -287,15 +331,16 @@ hipError_t backward(erq_mlp* m, const float* y, int B, float pos_weight, hipStre
     const __bf16 *W2 = m->wb + n1, *W3 = m->wb + n1 + n2;
     const size_t no = (size_t)B * m->out, nh = (size_t)B * m->hid;
     hipError_t e = hipSuccess;
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, m->loss, (size_t)1);
-    hipLaunchKernelGGL(bce_kernel, dim3(grid_for(no)), dim3(256), 0, s, m->logit, y, m->dlog, no,
-                       pos_weight, 1.0f / B, m->loss);
+    const int gb = grid_for(no);
+    hipLaunchKernelGGL(bce_kernel, dim3(gb), dim3(256), 0, s, m->logit, y, m->dlog, no,
+                       pos_weight, 1.0f / B, m->part);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, m->part, gb, m->loss);
     // layer 3: dW3[out x hid] = dlog^T h2 ; db3 ; dh2 = dlog W3
     e = gemm<1, 0, EPI_F32>(m->dlog, m->out, m->h2, m->hid, m->out, m->hid, B,
                             m->grad + L.w3, m->hid, nullptr, nullptr, s);
     if (e) return e;
-    hipLaunchKernelGGL(colsum_kernel, dim3((m->out + 255) / 256), dim3(256), 0, s, m->dlog, B,
-                       m->out, m->grad + L.b3);
+    e = colsum(m->dlog, B, m->out, m->part, m->grad + L.b3, s);
+    if (e) return e;
     e = gemm<0, 0, EPI_F32>(m->dlog, m->out, W3, m->hid, B, m->hid, m->out, m->tmp, m->hid,
                             nullptr, nullptr, s);
     if (e) return e;
@@ -305,8 +350,8 @@ hipError_t backward(erq_mlp* m, const float* y, int B, float pos_weight, hipStre
     e = gemm<1, 0, EPI_F32>(m->dz2, m->hid, m->h1, m->hid, m->hid, m->hid, B, m->grad + L.w2,
                             m->hid, nullptr, nullptr, s);
     if (e) return e;
-    hipLaunchKernelGGL(colsum_kernel, dim3((m->hid + 255) / 256), dim3(256), 0, s, m->dz2, B,
-                       m->hid, m->grad + L.b2);
+    e = colsum(m->dz2, B, m->hid, m->part, m->grad + L.b2, s);
+    if (e) return e;
     e = gemm<0, 0, EPI_F32>(m->dz2, m->hid, W2, m->hid, B, m->hid, m->hid, m->tmp, m->hid,
                             nullptr, nullptr, s);
     if (e) return e;
@@ -316,8 +361,8 @@ hipError_t backward(erq_mlp* m, const float* y, int B, float pos_weight, hipStre
     e = gemm<1, 0, EPI_F32>(m->dz1, m->hid, m->xb, m->in, m->hid, m->in, B, m->grad + L.w1,
                             m->in, nullptr, nullptr, s);
     if (e) return e;
-    hipLaunchKernelGGL(colsum_kernel, dim3((m->hid + 255) / 256), dim3(256), 0, s, m->dz1, B,
-                       m->hid, m->grad + L.b1);
+    e = colsum(m->dz1, B, m->hid, m->part, m->grad + L.b1, s);
+    if (e) return e;
     return hipGetLastError();
 }
 
@@ -332,7 +377,8 @@ void mlp_release(erq_mlp* m) {
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     void* ptrs[] = {m->master, m->grad, m->adam_m, m->adam_v, m->wb, m->xb, m->z1, m->h1,
-                    m->z2, m->h2, m->logit, m->dlog, m->dz2, m->dz1, m->tmp, m->loss};
+                    m->z2, m->h2, m->logit, m->dlog, m->dz2, m->dz1, m->tmp, m->loss,
+                    m->part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -389,6 +435,8 @@ int erq_mlp_create(size_t in_dim, size_t hidden, size_t out_dim, size_t max_batc
     if (e == hipSuccess) e = alloc(&m->dz1, B * hidden);
     if (e == hipSuccess) e = alloc(&m->tmp, B * hidden);
     if (e == hipSuccess) e = alloc(&m->loss, 1);
+    if (e == hipSuccess)
+        e = alloc(&m->part, std::max<size_t>(8192, (size_t)kColSlices * std::max(out_dim, hidden)));
     if (e != hipSuccess) {
         mlp_release(m);
         return hip_fail(e, "erq_mlp_create");

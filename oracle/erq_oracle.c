@@ -442,7 +442,7 @@ static int dist_id_cmp(const void* a, const void* b) {
 
 void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
                            const uint32_t* probe_ids, const float* probe_sqr_y, uint32_t* ids,
-                           float* dists) {
+                           float* dists, orc_stats* st) {
     size_t D = idx->D;
     float* unit_q = (float*)malloc(D * sizeof(float));
     int16_t* qq = (int16_t*)malloc(D * sizeof(int16_t));
@@ -453,7 +453,7 @@ void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, s
     orc_pool_init(&pool, pid, pd, k);
     float oos = orc_one_over_sqrtD(D);
     if (nprobe > 0)
-        search_cluster(idx, probe_ids[0], q_rot, probe_sqr_y[0], oos, unit_q, qq, lut, &pool, NULL);
+        search_cluster(idx, probe_ids[0], q_rot, probe_sqr_y[0], oos, unit_q, qq, lut, &pool, st);
     size_t ncap = 1024, nc = 0;
     cand* cs = (cand*)malloc(ncap * sizeof(cand));
     int R = 1 << idx->EX;
@@ -486,6 +486,7 @@ void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, s
                 uint32_t mask = orc_fastscan_block(block, lut, onorm, sumq, y, delta, shift, oos,
                                                    thr, D, ipq, NULL, NULL);
                 if (np < 32) mask &= (1u << np) - 1u;
+                if (st) st->survivors += (uint64_t)__builtin_popcount(mask);
                 while (mask) {
                     int j = __builtin_ctz(mask);
                     mask &= mask - 1;
@@ -497,6 +498,7 @@ void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, s
                     double g = (double)cc - ((double)R - 0.5) * (double)sumq;
                     float ex = (float)((double)a - (double)bb * g);
                     if (ex <= thr) {
+                        if (st) st->inserts++;
                         if (nc == ncap) cs = (cand*)realloc(cs, (ncap *= 2) * sizeof(cand));
                         cs[nc].id = idx->ids[vi];
                         cs[nc++].d = ex;

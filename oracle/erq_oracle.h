@@ -75,10 +75,12 @@ void orc_ivf_search(const orc_index* idx, const float* q_rot, size_t k, size_t n
 void orc_ivf_search_probes(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
                            const uint32_t* probe_ids, const float* probe_sqr_y,
                            uint32_t* ids, float* dists, orc_stats* st);
-/* Round-synchronous batched schedule (see erq_oracle.c). */
+/* Round-synchronous batched schedule (see erq_oracle.c).  st (nullable): blocks, vectors
+ * and survivors of rank 0 plus the survivors of the later rounds; `inserts` counts the
+ * later rounds' candidates (estimated distance <= the round threshold). */
 void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
                            const uint32_t* probe_ids, const float* probe_sqr_y, uint32_t* ids,
-                           float* dists);
+                           float* dists, orc_stats* st);
 /* Batch over nq queries with OpenMP (threads<=0: all). */
 void orc_ivf_search_batch(const orc_index* idx, const float* q_rot, size_t nq, size_t k,
                           const uint32_t* nprobe_per_q, uint32_t* ids, float* dists, int threads);

@@ -69,7 +69,7 @@ def lib():
                                             c_vp, c_vp, c_vp, ctypes.POINTER(OrcStats)]
         L.orc_ivf_search_rounds.restype = None
         L.orc_ivf_search_rounds.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size, c_vp,
-                                            c_vp, c_vp, c_vp]
+                                            c_vp, c_vp, c_vp, ctypes.POINTER(OrcStats)]
         L.orc_ivf_search_batch.restype = None
         L.orc_ivf_search_batch.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size, c_vp,
                                            c_vp, c_vp, ctypes.c_int]
@@ -232,14 +232,17 @@ class OracleIndex:
                                     _p(ids), _p(d), None)
         return ids, d
 
-    def search_rounds(self, q_rot, k, probe_ids, probe_d):
+    def search_rounds(self, q_rot, k, probe_ids, probe_d, stats=False):
         q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)
         pid = np.ascontiguousarray(probe_ids, np.uint32)
         pd = np.ascontiguousarray(probe_d, np.float32)
         ids = np.empty(k, np.uint32)
         d = np.empty(k, np.float32)
+        st = OrcStats()
         lib().orc_ivf_search_rounds(ctypes.byref(self.view), _p(q), k, pid.size, _p(pid), _p(pd),
-                                    _p(ids), _p(d))
+                                    _p(ids), _p(d), ctypes.byref(st))
+        if stats:
+            return ids, d, (st.blocks, st.survivors, st.inserts)
         return ids, d
 
     def search_batch(self, q_rot, k, nprobe, threads=0):

@@ -27,10 +27,12 @@ def _oracle_rounds(idx, qr, k, probe_lists):
     oi = orc.OracleIndex(idx)
     ids = np.empty((qr.shape[0], k), np.uint32)
     ds = np.empty((qr.shape[0], k), np.float32)
+    tot = np.zeros(3, np.int64)  # rank-0 blocks, survivors, later rounds' candidates
     for i in range(qr.shape[0]):
         pid, pd = probe_lists[i]
-        ids[i], ds[i] = oi.search_rounds(qr[i], k, pid, pd)
-    return ids, ds
+        ids[i], ds[i], st = oi.search_rounds(qr[i], k, pid, pd, stats=True)
+        tot += np.asarray(st, np.int64)
+    return ids, ds, tot
 
 
 def _probe_lists(idx, qr, nprobes):
@@ -62,10 +64,13 @@ def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None):
 def _check(idx, q, k, nprobes, label, survivor_cap=None):
     qr = rotate_np(idx, q)
     pl = _probe_lists(idx, qr, nprobes)
-    oid, od = _oracle_rounds(idx, qr, k, pl)
+    oid, od, ost = _oracle_rounds(idx, qr, k, pl)
     gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)), survivor_cap)
-    print(f"[{label}] stats={st.tolist()}", flush=True)
+    print(f"[{label}] stats={st.tolist()} oracle blocks/survivors/candidates={ost.tolist()}",
+          flush=True)
     assert st[5] == 0, "candidate buffer overflow"
+    # the work counters match the schedule's: rank-0 blocks, survivors, round candidates
+    assert (st[0], st[2], st[4]) == tuple(int(v) for v in ost)
     np.testing.assert_array_equal(gid, oid, err_msg=f"{label}: ids")
     np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32), err_msg=f"{label}: dists")
     return gid, gd
