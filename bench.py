@@ -60,6 +60,10 @@ def parse():
     ap.add_argument("--profile-learned", type=float, default=0.0, metavar="MASS",
                     help="train the probe predictor, then only time the learned batched search "
                          "at this mass (the command its kernel profile is taken with)")
+    ap.add_argument("--save-mlp", type=str, default="",
+                    help="save the trained probe predictor to this .npz file")
+    ap.add_argument("--load-mlp", type=str, default="",
+                    help="load the probe predictor from this .npz file instead of training it")
     ap.add_argument("--sweep-only", action="store_true",
                     help="print the recall sweep of both paths and exit (no timing)")
     return ap.parse_args()
@@ -436,6 +440,10 @@ def train_probe_model(a, g, idx, mix, torch):
     from erq import data
     from erq.probe_model import ProbeMLP, cluster_of_ids
     basis, centres = mix
+    if a.load_mlp:
+        m = ProbeMLP.load(a.load_mlp)
+        log(f"probe MLP loaded from {a.load_mlp}")
+        return m, min(a.label_nprobe or 256, idx.K, 2048), [], 0.0, 0.0
     t0 = time.time()
     qt, _ = data.mixture_powerlaw_torch(a.train_queries, a.dim, a.clusters, a.sigma, seed=5,
                                         alpha=1.0, centres=centres, basis=basis)
@@ -456,6 +464,8 @@ def train_probe_model(a, g, idx, mix, torch):
                  input_scale=float(np.sqrt(idx.D)))
     losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=1e-3, pos_weight=4.0, log=log)
     torch.cuda.synchronize()
+    if a.save_mlp:
+        m.save(a.save_mlp)
     return m, label_np, losses, t_lab, time.time() - t0
 
 

@@ -20,7 +20,8 @@ struct BatchArgs {
     const uint32_t* vstart;
     const uint32_t* items;     // 3 x n_items: cluster, (first entry << 4) | count, first tile
     const uint32_t* entries;   // probe slots grouped by cluster
-    uint32_t* n_items;         // [0] items of this round, [2] entries
+    uint32_t* n_items;         // [0] items of this round, [2] entries, [3] queued items
+    uint32_t* ovf;             // items queued for the inline pass (slice full)
     const uint32_t* probe_ids; // per slot: cluster id
     int8_t* adig;              // per entry: 2 * Dp int8 digits in MFMA operand order
     void* qparams;             // per entry: query parameters

@@ -151,6 +151,7 @@ __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __rest
     if (t == 0) {
         n_items[0] = carry_i;
         n_items[2] = carry_c;  // entries (query, cluster pairs) of the round
+        n_items[3] = 0;        // items queued for the inline pass
     }
 }
 
@@ -337,7 +338,11 @@ __device__ __forceinline__ void load_tile_codes(const uint8_t* bp, uint32_t blk,
                                                16 * lg);
 }
 
-template <int EX, int NCH>
+// INLINE = false: the main pass over the round's items; an item whose survivors do not fit
+// in the workgroup's slice is queued for the INLINE = true pass instead.  INLINE = true:
+// the queued items again, with their survivors re-estimated in the workgroup (the same
+// formulas), so every item's survivors are handled exactly once either way.
+template <int EX, int NCH, bool INLINE>
 __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int Dp = NCH * 512;
@@ -367,15 +372,18 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     // XCD-aware order: workgroup b runs on XCD b % 8, so each XCD takes a contiguous eighth
     // of the items (clusters in order) and the items of one cluster share that XCD's L2.
     uint32_t it_lo = 0, it_hi = nitems, it_step = gridDim.x, it_first = blockIdx.x;
-    if ((gridDim.x & 7) == 0) {
+    if (INLINE) {
+        it_hi = a.n_items[3];  // queued items
+    } else if ((gridDim.x & 7) == 0) {
         const uint32_t x = blockIdx.x & 7;
         it_lo = (uint32_t)((uint64_t)nitems * x / 8);
         it_hi = (uint32_t)((uint64_t)nitems * (x + 1) / 8);
         it_step = gridDim.x >> 3;
         it_first = it_lo + (blockIdx.x >> 3);
     }
-    for (uint32_t item = it_first; item < it_hi; item += it_step) {
+    for (uint32_t it = it_first; it < it_hi; it += it_step) {
         __syncthreads();  // the previous item's LDS is no longer read
+        const uint32_t item = INLINE ? a.ovf[it] : it;
         const uint32_t cid = a.items[3 * item];
         const uint32_t e0 = a.items[3 * item + 1] >> 4, elen = a.items[3 * item + 1] & 15u;
         const int tfirst = (int)a.items[3 * item + 2];
@@ -454,7 +462,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                     s_ipq[qq * kQSurv + sidx] = ipq;
                 }
             }
-            if (lane == 0) st_blocks++;
+            if (lane == 0 && !INLINE) st_blocks++;
         }
         __syncthreads();
 
@@ -465,19 +473,23 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
             qoff[qq + 1] = qoff[qq] + (qq < (int)elen ? (int)s_cnt[qq] : 0);
         const int tot = qoff[kGroup];
         if (tot == 0) continue;  // uniform: every thread read the same counts
-        if (used + (size_t)tot <= slice_cap) {
-            for (int i = tid; i < tot; i += kThreads) {
-                int qq = 0;
+        if (!INLINE) {
+            if (used + (size_t)tot <= slice_cap) {
+                for (int i = tid; i < tot; i += kThreads) {
+                    int qq = 0;
 #pragma unroll
-                for (int j = 1; j < kGroup; j++) qq += i >= qoff[j] ? 1 : 0;
-                const int si = qq * kQSurv + (i - qoff[qq]);
-                slice[used + i] = make_uint4(e0 + (uint32_t)qq, cid, s_vec[si],
-                                             __float_as_uint(s_ipq[si]));
+                    for (int j = 1; j < kGroup; j++) qq += i >= qoff[j] ? 1 : 0;
+                    const int si = qq * kQSurv + (i - qoff[qq]);
+                    slice[used + i] = make_uint4(e0 + (uint32_t)qq, cid, s_vec[si],
+                                                 __float_as_uint(s_ipq[si]));
+                }
+                used += (size_t)tot;
+            } else if (tid == 0) {  // the slice is full: queue the item for the inline pass
+                a.ovf[atomicAdd(&a.n_items[3], 1u)] = item;
             }
-            used += (size_t)tot;
             continue;
         }
-        // the slice is full: re-estimate this item's survivors here
+        // inline pass: re-estimate this item's survivors here
         __syncthreads();  // every wave is done with the digits in A
         const float value = (float)(1.0 / (double)sqrtf((float)D));
         float* uw = reinterpret_cast<float*>(A) + (size_t)wid * D;
@@ -524,39 +536,62 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
             }
         }
     }
-    if (tid == 0) a.surv_count[blockIdx.x] = (uint32_t)used;
-    // survivors and candidates were counted by the first lane of each 16-lane group
-    if (a.stats && ln == 0) {
-        if (lane == 0) atomicAdd(&a.stats[6], st_blocks);
-        atomicAdd(&a.stats[2], st_surv);
-        atomicAdd(&a.stats[4], st_cand);
+    if (!INLINE && tid == 0) a.surv_count[blockIdx.x] = (uint32_t)used;
+    if (a.stats) {
+        wave_stat_add(&a.stats[6], st_blocks);
+        wave_stat_add(&a.stats[2], st_surv);
+        wave_stat_add(&a.stats[4], st_cand);
     }
 }
 
 // The round's survivor list, one record per 16-lane group (four per wave), u(d) read
 // straight from the rotated query and the centroid.
-// The round's survivor list, slice by slice: kReestWG workgroups per scan workgroup's
-// slice, each wave taking a contiguous 1/(4 kReestWG) of it, four records (one per 16-lane
-// group) at a time.  Records of one (query, cluster) pair are contiguous in a slice; when
-// the four records of a step share their pair, the residual u is staged once in the
-// wave's LDS slice and kept for the next step, otherwise each group forms u from the
-// rotated query and the centroid as it goes.
-constexpr int kReestWG = 4;
-
+// The round's survivor list, all slices as one index space (exclusive prefix of the slice
+// counts in LDS), each wave taking an equal contiguous share, four records (one per 16-lane
+// group) at a time.  Records of one (query, cluster) pair are contiguous; when the four
+// records of a step share their pair, the residual u is staged once in the wave's LDS
+// slice and kept for the next step, otherwise each group forms u from the rotated query
+// and the centroid as it goes.
+constexpr int kReestWG = 4;  // re-estimation workgroups per scan workgroup
 template <int EX>
 __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int D = a.D;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
     float* uw = reinterpret_cast<float*>(smem) + (size_t)wid * D;
-    const int g = blockIdx.x / kReestWG;
-    const size_t cnt = a.surv_count[g];
-    const uint4* slice = a.surv + (size_t)g * (a.surv_cap / scan_grid);
-    const size_t nw = (size_t)kReestWG * 4;
-    const size_t wv = (size_t)(blockIdx.x % kReestWG) * 4 + wid;
-    const size_t per = ((cnt + nw - 1) / nw + 3) & ~size_t(3);
-    const size_t r_begin = min(cnt, wv * per), r_end = min(cnt, r_begin + per);
+    uint32_t* pre = reinterpret_cast<uint32_t*>(smem + 4 * 4 * (size_t)D);  // scan_grid + 1
+    __shared__ uint32_t wsum[4];
+    // XCD-aware: workgroup b runs on XCD b % 8, like scan workgroup g on XCD g % 8; the
+    // workgroups of one XCD share the slices of that XCD's scan workgroups (clusters of one
+    // contiguous range, whose centroid rows then stay in that XCD's L2).  ns slices
+    // x, x + xs, x + 2 xs, ...; their exclusive count prefix goes to pre[0..ns].
+    const bool xcd = (gridDim.x & 7) == 0 && (scan_grid & 7) == 0;
+    const int x = xcd ? (int)(blockIdx.x & 7) : 0, xs = xcd ? 8 : 1;
+    const int ns = scan_grid / xs;
+    const int per_t = (ns + 255) / 256;
+    uint32_t own = 0;
+    for (int i = tid * per_t; i < min(ns, (tid + 1) * per_t); i++) own += a.surv_count[x + xs * i];
+    uint32_t incl = own;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t base = incl - own;
+    for (int w = 0; w < wid; w++) base += wsum[w];
+    for (int i = tid * per_t; i < min(ns, (tid + 1) * per_t); i++) {
+        pre[i] = base;
+        base += a.surv_count[x + xs * i];
+    }
+    if (tid == 255) pre[ns] = base;
+    __syncthreads();
+    const size_t total = pre[ns];
+    const size_t slice_cap = a.surv_cap / scan_grid;
+    const size_t nw = (size_t)(gridDim.x / xs) * 4;
+    const size_t wv = (size_t)(blockIdx.x / xs) * 4 + wid;
+    const size_t r_begin = total * wv / nw, r_end = total * (wv + 1) / nw;
     const size_t CL = (size_t)D * EX / 8;
     const size_t BB = 4 * (size_t)D + 128;
     const float value = (float)(1.0 / (double)sqrtf((float)D));
@@ -565,7 +600,15 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
     for (size_t r0 = r_begin; r0 < r_end; r0 += 4) {  // wave-uniform
         const size_t r = r0 + lg;
         const bool act = r < r_end;
-        const uint4 rec = act ? slice[r] : make_uint4(0, 0, 0, 0);
+        uint4 rec = make_uint4(0, 0, 0, 0);
+        if (act) {  // slice of record r: the last i with pre[i] <= r
+            int lo = 0, hi = ns - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pre[mid] <= (uint32_t)r) lo = mid; else hi = mid - 1;
+            }
+            rec = a.surv[(size_t)(x + xs * lo) * slice_cap + (r - pre[lo])];
+        }
         const QParam P = act ? reinterpret_cast<const QParam*>(a.qparams)[rec.x] : QParam{};
         const uint32_t cid = rec.y, vloc = rec.z;
         const uint32_t vi = act ? a.vstart[cid] + vloc : 0;
@@ -608,9 +651,9 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             st_surv++;
         }
     }
-    if (a.stats && l16 == 0) {  // counted by the first lane of each 16-lane group
-        atomicAdd(&a.stats[2], st_surv);
-        atomicAdd(&a.stats[4], st_cand);
+    if (a.stats) {
+        wave_stat_add(&a.stats[2], st_surv);
+        wave_stat_add(&a.stats[4], st_cand);
     }
 }
 
@@ -779,11 +822,16 @@ template <int EX, int NCH>
 static hipError_t launch_batch_scan_nch(const BatchArgs& ba, int grid, size_t lds, hipStream_t s) {
     if (lds + 1024 > 65536) {  // dynamic plus static LDS beyond the 64 KB default
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&batch_scan_kernel<EX, NCH>),
+            reinterpret_cast<const void*>(&batch_scan_kernel<EX, NCH, false>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_scan_kernel<EX, NCH, true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((batch_scan_kernel<EX, NCH>), dim3(grid), dim3(kThreads), lds, s, ba);
+    hipLaunchKernelGGL((batch_scan_kernel<EX, NCH, false>), dim3(grid), dim3(kThreads), lds, s, ba);
+    // the queued items (none unless a workgroup's survivor slice filled up)
+    hipLaunchKernelGGL((batch_scan_kernel<EX, NCH, true>), dim3(grid), dim3(kThreads), lds, s, ba);
     return hipGetLastError();
 }
 
@@ -812,7 +860,7 @@ static hipError_t launch_batch_scan(const BatchArgs& ba, int grid, size_t lds, h
 
 static hipError_t launch_reest(const BatchArgs& ba, int scan_grid, hipStream_t s) {
     const dim3 grid(scan_grid * kReestWG);
-    const size_t lds = 4 * (size_t)ba.D * 4;  // one residual per wave
+    const size_t lds = 4 * (size_t)ba.D * 4 + 4 * ((size_t)scan_grid + 1);  // residuals, prefix
     switch (ba.EX) {
     case 2: hipLaunchKernelGGL(reest_kernel<2>, grid, dim3(256), lds, s, ba, scan_grid); break;
     case 3: hipLaunchKernelGGL(reest_kernel<3>, grid, dim3(256), lds, s, ba, scan_grid); break;

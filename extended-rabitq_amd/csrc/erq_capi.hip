@@ -574,9 +574,9 @@ static size_t batch_ws_bytes(const erq_index* h, size_t nq, size_t max_nprobe, i
     const size_t pairs = std::min(cq * std::max<size_t>(max_nprobe, 1), kPairBudget) + 8;
     const size_t Dp = (size_t)bp_nchunk((int)D) * 512;
     const size_t nitems = batch_items_bound(pairs, K, h->N, max_cluster_size(h));
-    return 12 * 256 + align_up(K * 4, 256) * 2 + align_up(16, 256) +
+    return 13 * 256 + align_up(K * 4, 256) * 2 + align_up(16, 256) +
            align_up((size_t)h->batch_surv_cap * 16, 256) + align_up(kMaxBatchGrid * 4, 256) +
-           align_up(nitems * 12, 256) + align_up(pairs * 4 + 4, 256) +
+           align_up(nitems * 12, 256) + align_up(nitems * 4, 256) + align_up(pairs * 4 + 4, 256) +
            align_up(cq * 4, 256) + align_up(cq * (size_t)cap * 4, 256) * 2 +
            align_up(pairs * 2 * Dp, 256) + align_up(pairs * 48, 256);
 }
@@ -601,7 +601,9 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     uint32_t* counts = ws.take<uint32_t>(h->K);
     uint32_t* cursor = ws.take<uint32_t>(h->K);
     uint32_t* n_items = ws.take<uint32_t>(4);
-    uint32_t* items = ws.take<uint32_t>(batch_items_bound(pairs, h->K, h->N, max_cluster_size(h)) * 3);
+    const size_t nitems_max = batch_items_bound(pairs, h->K, h->N, max_cluster_size(h));
+    uint32_t* items = ws.take<uint32_t>(nitems_max * 3);
+    uint32_t* ovf = ws.take<uint32_t>(nitems_max);
     uint32_t* entries = ws.take<uint32_t>(pairs + 1);
     uint32_t* cand_cnt = ws.take<uint32_t>(cq);
     float* cand_d = ws.take<float>(cq * (size_t)cap);
@@ -631,6 +633,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.bstart = h->bstart;
         ba.vstart = h->vstart;
         ba.items = items;
+        ba.ovf = ovf;
         ba.entries = entries;
         ba.n_items = n_items;
         ba.adig = adig;

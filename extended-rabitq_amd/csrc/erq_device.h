@@ -43,6 +43,21 @@ __device__ __forceinline__ int wave_sum_i(int v) {
     return v;
 }
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned lo = __shfl_xor((unsigned)v, o), hi = __shfl_xor((unsigned)(v >> 32), o);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+
+// Adds this wave's counts (summed over its lanes) to a stats counter; lane 0 only, and
+// only when nonzero, so that idle workgroups do not contend on the counters.
+__device__ __forceinline__ void wave_stat_add(unsigned long long* stat, unsigned long long v) {
+    v = wave_sum_u64(v);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(stat, v);
+}
+
 // Byte position (0..15) inside a codebook's 16 bytes that holds vector v%16 in the
 // FastScan block (reference pack_codes.hpp:44-65, perm0 = {0,8,1,9,...,7,15}).
 __device__ __forceinline__ int byte_of_vec(int v) {

@@ -103,6 +103,24 @@ class ProbeMLP:
             self.shift_output_bias(-float(np.log(pos_weight)))
         return losses
 
+    def save(self, path):
+        """Weights and shape to an .npz file (plain arrays; loads with allow_pickle=False)."""
+        np.savez(path, params=self.get_params(),
+                 shape=np.array([self.in_dim, self.hidden, self.out_dim], np.int64),
+                 input_scale=np.array([self.input_scale()], np.float32))
+
+    @classmethod
+    def load(cls, path, max_batch=1024):
+        with np.load(path, allow_pickle=False) as f:
+            din, hid, dout = (int(v) for v in f["shape"])
+            m = cls(din, dout, hidden=hid, max_batch=max_batch,
+                    input_scale=float(f["input_scale"][0]))
+            m.set_params(f["params"])
+        return m
+
+    def input_scale(self):
+        return float(_lib.lib().erq_mlp_input_scale(self._h))
+
     def shift_output_bias(self, delta):
         """Adds delta to every output logit (the last out_dim parameters are the output bias)."""
         import torch

@@ -45,7 +45,7 @@ enum {
     /* Workgroups per CU of the persistent batched kernel, 1..8. */
     ERQ_OPT_BATCH_WG_PER_CU = 3,
     /* Records (16 bytes each) of the batched search's survivor list per round, 0..2^30
-     * (default 2^24).  Survivors beyond it are re-estimated inside the scan kernel
+     * (default 2^26).  Survivors beyond it are re-estimated inside the scan kernel
      * instead of by the separate re-estimation kernel; results are identical. */
     ERQ_OPT_BATCH_SURVIVOR_CAP = 4,
     /* Order in which the per-query scan takes queries: 1 (default) grouped by first probed

@@ -134,3 +134,19 @@ def test_learned_probe_search_end_to_end():
     assert torch.equal(fb_i, bi) and torch.equal(fb_d, bd)
     m.close()
     g.close()
+
+
+def test_probe_model_save_load_round_trip(tmp_path):
+    import torch
+    from erq.probe_model import ProbeMLP
+    theta, x, _ = _mlp_case(64, 64, 40, 32, seed=9)
+    m = ProbeMLP(64, 40, hidden=64, max_batch=32, seed=1, input_scale=2.5)
+    m.set_params(theta)
+    path = str(tmp_path / "probe.npz")
+    m.save(path)
+    m2 = ProbeMLP.load(path, max_batch=32)
+    assert (m2.in_dim, m2.out_dim, m2.hidden) == (64, 40, 64) and m2.input_scale() == 2.5
+    xt = torch.from_numpy(x).cuda()
+    assert torch.equal(m.forward_dev(xt), m2.forward_dev(xt))
+    m.close()
+    m2.close()
