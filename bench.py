@@ -462,7 +462,8 @@ def train_probe_model(a, g, idx, mix, torch):
     # inputs scaled to unit variance per coordinate (rotated unit-norm queries)
     m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=1024, seed=7,
                  input_scale=float(np.sqrt(idx.D)))
-    losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=1e-3, pos_weight=4.0, log=log)
+    losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=2e-3, pos_weight=4.0, log=log,
+                     schedule="cosine")
     torch.cuda.synchronize()
     if a.save_mlp:
         m.save(a.save_mlp)

@@ -1,8 +1,8 @@
 // erq_batch.hip -- cluster-major batched ExRaBitQ search on gfx950.
 //
 // Many queries with uneven probe lists are searched together.  Probe ranks are split into
-// rounds ([0,1) [1,4) [4,16) [16,64) ...).  Round 0 is the exact per-query scan
-// (erq_search.hip).  Every later round:
+// rounds ([0,4) [4,16) [16,64) [64,192), then 128 wide).  Round 0 is the exact per-query
+// scan (erq_search.hip).  Every later round:
 //   1. buckets the round's (query, rank) pairs by cluster (count / scan / scatter),
 //   2. runs one persistent kernel over (cluster, <= 8 queries) work items, so each
 //      cluster's codes are read once per round for all the queries that probe it,

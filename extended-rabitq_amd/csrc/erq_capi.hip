@@ -548,8 +548,9 @@ static int ensure_bp(erq_index* h, hipStream_t s) {
     return ERQ_OK;
 }
 
-// Round schedule of the batched search: [0,1) exact replay, then [1,4), [4,16), [16,64),
-// and from there rounds of at most 128 ranks.
+// Round schedule of the batched search: ranks [0, kExactRanks) by the exact per-query
+// replay, then [4,16), [16,64), [64,192) and from there rounds of at most 128 ranks.
+constexpr size_t kExactRanks = 4;
 static size_t next_round(size_t r0) { return std::min(4 * r0, r0 + 128); }
 
 constexpr size_t kPairBudget = 1u << 20;  // (query, cluster) pairs prepared per round
@@ -557,7 +558,7 @@ constexpr size_t kMaxBatchGrid = 4096;    // workgroups of the persistent batche
 
 static size_t batch_chunk_nq(size_t nq, size_t max_nprobe) {
     size_t width = 1;
-    for (size_t r0 = 1; r0 < max_nprobe; r0 = next_round(r0))
+    for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0))
         width = std::max(width, std::min(next_round(r0), max_nprobe) - r0);
     return std::max<size_t>(1, std::min(nq, kPairBudget / width));
 }
@@ -590,9 +591,9 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     if (h->D > 2048) return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s);
     int rc = ensure_bp(h, s);
     if (rc) return rc;
-    // round 0: the exact per-query scan of each query's nearest cluster
-    rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s, 1);
-    if (rc || max_nprobe <= 1) return rc;
+    // round 0: the exact per-query scan of each query's kExactRanks nearest clusters
+    rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s, (int)kExactRanks);
+    if (rc || max_nprobe <= kExactRanks) return rc;
     const int cap = kCandCap;
     const size_t cq = batch_chunk_nq(nq, max_nprobe);
     const size_t pairs = std::min(cq * max_nprobe, kPairBudget) + 8;
@@ -654,7 +655,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.nchunk = bp_nchunk((int)h->D);
         ba.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
         RoundBufs rb{pid, counts, cursor, ids + q0 * k, (int)h->K};
-        for (size_t r0 = 1; r0 < max_nprobe; r0 = next_round(r0)) {
+        for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0)) {
             e = launch_round(ba, rb, (int)r0, (int)std::min(next_round(r0), max_nprobe), grid, s);
             if (e != hipSuccess) return hip_fail(e, "batched round");
         }

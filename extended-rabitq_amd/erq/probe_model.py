@@ -74,27 +74,34 @@ class ProbeMLP:
         check(_lib.lib().erq_mlp_adam_step_dev(self._h, lr, beta1, beta2, eps, _stream()))
 
     def train(self, x, label_clusters, epochs=3, batch=512, lr=1e-3, pos_weight=4.0, seed=0,
-              log=None, calibrate=True):
+              log=None, calibrate=True, schedule="constant", soft=False):
         """x: [n, D] CUDA float tensor; label_clusters: [n, k] CUDA int tensor of the cluster
         of each of the query's top-k results (-1 = none).  Returns the per-epoch mean loss.
 
         The positive-weighted loss is minimised by sigmoid(z) = w p / (w p + 1 - p), i.e.
         z = ln w + logit p.  With calibrate, ln w is taken off the output bias afterwards so
         that the network's outputs estimate p itself, which the probability-mass criterion
-        of erq_probe_learned_dev assumes."""
+        of erq_probe_learned_dev assumes.  schedule "cosine" decays the learning rate from lr
+        to 0 over the run (half a cosine); "constant" keeps lr.  soft: the target of a cluster
+        is the share of the query's top-k it holds (instead of 1 if it holds any), so the
+        outputs rank clusters by how many neighbours they hold."""
         import torch
         n = x.shape[0]
         g = torch.Generator(device=x.device)
         g.manual_seed(seed)
         losses = []
+        steps = max(1, epochs * (n // batch))
+        step = 0
         for ep in range(epochs):
             perm = torch.randperm(n, generator=g, device=x.device)
             tot, cnt = torch.zeros((), device=x.device), 0
             for s in range(0, n - batch + 1, batch):
                 sel = perm[s:s + batch]
-                y = labels_dense(label_clusters[sel], self.out_dim)
+                y = labels_dense(label_clusters[sel], self.out_dim, soft=soft)
                 tot = tot + self.loss_grad_dev(x[sel], y, pos_weight)[0]
-                self.adam_step(lr)
+                lr_t = lr if schedule == "constant" else 0.5 * lr * (1 + np.cos(np.pi * step / steps))
+                self.adam_step(float(lr_t))
+                step += 1
                 cnt += 1
             losses.append(float(tot) / max(cnt, 1))
             if log:
@@ -130,14 +137,18 @@ class ProbeMLP:
         self.set_params(theta)
 
 
-def labels_dense(label_clusters, K):
-    """0/1 [n, K] float labels from per-query cluster lists (-1 entries ignored)."""
+def labels_dense(label_clusters, K, soft=False):
+    """[n, K] float labels from per-query cluster lists (-1 entries ignored): 1 where the
+    cluster holds any listed neighbour, or with soft, the share of the list it holds."""
     import torch
     n = label_clusters.shape[0]
     y = torch.zeros((n, K + 1), dtype=torch.float32, device=label_clusters.device)
     lc = label_clusters.long().clone()
     lc[lc < 0] = K
-    y.scatter_(1, lc, 1.0)
+    if soft:
+        y.scatter_add_(1, lc, torch.full(lc.shape, 1.0 / lc.shape[1], device=lc.device))
+    else:
+        y.scatter_(1, lc, 1.0)
     return y[:, :K].contiguous()
 
 

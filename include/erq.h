@@ -130,13 +130,13 @@ int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, si
                    uint32_t* ids_dev, float* dists_dev, uint64_t* stats_dev, void* stream);
 
 /* Cluster-major batched form of erq_scan_dev for many queries with uneven probe lists:
- * rank 0 of every query is scanned exactly as IVF::search does; ranks [1,4), [4,16),
- * [16,64) and then 128 ranks at a time are processed in rounds, each round reading every
- * probed cluster's
- * codes once for all queries that probe it (FastScan sums on int8 MFMA) and pruning
- * against each query's k-th distance at round start.  max_nprobe bounds every query's
- * probe count.  Results are deterministic; with one probe per query they equal
- * erq_scan_dev's.  For D > 2048 the exact per-query replay runs instead.  dists_dev is required.  stats_dev (nullable) takes 8 counters:
+ * ranks [0,4) of every query are scanned exactly as IVF::search does; ranks [4,16),
+ * [16,64), [64,192) and then 128 ranks at a time are processed in rounds, each round
+ * reading every probed cluster's codes once for all queries that probe it (FastScan sums
+ * on int8 MFMA) and pruning against each query's k-th distance at round start.
+ * max_nprobe bounds every query's probe count.  Results are deterministic; with at most 4
+ * probes per query they equal erq_scan_dev's.  For D > 2048 the exact per-query replay runs
+ * instead.  dists_dev is required.  stats_dev (nullable) takes 8 counters:
  * [0] blocks, [1] vectors, [3] probes of round 0; [2] survivors re-ranked (all rounds);
  * [4] candidates kept, [5] candidates dropped on buffer overflow, [6] 16-vector tiles
  * scanned in rounds >= 1. */

@@ -428,11 +428,11 @@ void orc_ivf_search_probes(const orc_index* idx, const float* q_rot, size_t k, s
 }
 
 /* Round-synchronous batched search (this project's cluster-major schedule, checked
- * against liberq's erq_scan_batched_dev): probe rank 0 exactly as IVF::search; ranks
- * [1,4), [4,16), [16,64), then rounds 128 ranks wide; each round's pruning threshold is
- * the pool's k-th distance at
- * round start; a vector is a candidate if its lower bound < threshold and its
- * estimate <= threshold; candidates are merged into the pool by (distance, id). */
+ * against liberq's erq_scan_batched_dev): probe ranks [0,4) exactly as IVF::search; ranks
+ * [4,16), [16,64), [64,192), then rounds 128 ranks wide; each round's pruning threshold is
+ * the pool's k-th distance at round start; a vector is a candidate if its lower bound <
+ * threshold and its estimate <= threshold; candidates are merged into the pool by
+ * (distance, id). */
 static int dist_id_cmp(const void* a, const void* b) {
     const cand *x = (const cand*)a, *y = (const cand*)b;
     if (x->d < y->d) return -1;
@@ -452,14 +452,16 @@ void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, s
     orc_pool pool;
     orc_pool_init(&pool, pid, pd, k);
     float oos = orc_one_over_sqrtD(D);
-    if (nprobe > 0)
-        search_cluster(idx, probe_ids[0], q_rot, probe_sqr_y[0], oos, unit_q, qq, lut, &pool, st);
+    /* ranks [0, 4): the exact replay (IVF::search) */
+    size_t exact = nprobe < 4 ? nprobe : 4;
+    for (size_t r = 0; r < exact; r++)
+        search_cluster(idx, probe_ids[r], q_rot, probe_sqr_y[r], oos, unit_q, qq, lut, &pool, st);
     size_t ncap = 1024, nc = 0;
     cand* cs = (cand*)malloc(ncap * sizeof(cand));
     int R = 1 << idx->EX;
     size_t block_bytes = 4 * D + 128, code_len = D * idx->EX / 8;
-    for (size_t r0 = 1, r1; r0 < nprobe; r0 = r1) {
-        r1 = 4 * r0 < r0 + 128 ? 4 * r0 : r0 + 128; /* [1,4) [4,16) [16,64) then 128 wide */
+    for (size_t r0 = 4, r1; r0 < nprobe; r0 = r1) {
+        r1 = 4 * r0 < r0 + 128 ? 4 * r0 : r0 + 128; /* [4,16) [16,64) [64,192) then 128 wide */
         float thr = orc_pool_distk(&pool);
         nc = 0;
         for (size_t i = 0; i < pool.size; i++) {

@@ -107,7 +107,7 @@ def test_batched_d1536_b4(cap):
     _check(idx, q, 100, np.full(q.shape[0], 20), f"D1536 B4 cap {cap}", survivor_cap=cap)
 
 
-def test_batched_single_probe_equals_plain_and_is_deterministic():
+def test_batched_exact_ranks_equal_plain_and_is_deterministic():
     import torch
     from erq.gpu import IVF
     idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
@@ -115,10 +115,13 @@ def test_batched_single_probe_equals_plain_and_is_deterministic():
     g = IVF.from_index(idx)
     pi, pdist = g.search_dev(qr, 100, 1)
     bi, bd = g.search_batched_dev(qr, 100, 1)
+    p4i, p4d = g.search_dev(qr, 100, 4)       # ranks [0,4) are the exact replay
+    b4i, b4d = g.search_batched_dev(qr, 100, 4)
     b2i, b2d = g.search_batched_dev(qr, 100, 16)
     b3i, b3d = g.search_batched_dev(qr, 100, 16)
     torch.cuda.synchronize()
     assert torch.equal(pi, bi) and torch.equal(pdist, bd)
+    assert torch.equal(p4i, b4i) and torch.equal(p4d, b4d)
     assert torch.equal(b2i, b3i) and torch.equal(b2d, b3d)
     g.close()
 
