@@ -13,6 +13,8 @@ struct BatchArgs {
     const uint8_t* short_data; // HA blocks (factors are read from here)
     const uint8_t* bp;         // bit-plane copy of the 1-bit codes
     const uint8_t* long_code;
+    const uint32_t* exl;       // lane-major EX-bit codes, exl_w dwords per vector lane
+    int exl_w;
     const float* ex_factor;
     const uint32_t* ids;
     const uint32_t* csize;
@@ -55,6 +57,9 @@ size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize);
 hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
                            hipStream_t s);
 hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s);
+// exl (N x 16 x exl_words(D, EX) u32, erq_device.h) from long_code.
+hipError_t launch_exl_relayout(const uint8_t* long_code, size_t N, int D, int EX, uint32_t* exl,
+                               hipStream_t s);
 hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1, int grid,
                         hipStream_t s);
 

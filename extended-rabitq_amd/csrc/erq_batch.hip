@@ -67,6 +67,38 @@ __global__ void relayout_kernel(const uint8_t* __restrict__ short_data, size_t n
     }
 }
 
+// exl (lane-major EX-bit codes, erq_device.h) from the reference's compact layout: one
+// thread per (vector, lane), values decoded with decode4 and packed in increasing i.
+template <int EX>
+__global__ void exl_relayout_kernel(const uint8_t* __restrict__ long_code, size_t N, int D, int W,
+                                    uint32_t* __restrict__ exl) {
+    constexpr int vpd = 32 / EX;
+    const size_t CL = (size_t)D * EX / 8;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < N * 16;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const size_t v = t >> 4;
+        const int l = (int)(t & 15);
+        const uint8_t* code = long_code + v * CL;
+        uint32_t* out = exl + (v * 16 + l) * (size_t)W;
+        uint32_t cur = 0;
+        int w = 0;
+        for (int ch = 0; ch < D / 64; ch++) {
+            float e[4];
+            decode4<EX>(code + ch * 8 * EX, l, e);
+            for (int s4 = 0; s4 < 4; s4++) {
+                const int i = 4 * ch + s4;
+                if (i / vpd != w) {
+                    out[w++] = cur;
+                    cur = 0;
+                }
+                cur |= (uint32_t)e[s4] << ((i % vpd) * EX);
+            }
+        }
+        out[w++] = cur;
+        for (; w < W; w++) out[w] = 0;
+    }
+}
+
 // ---------------------------------------------------------------- work lists
 __global__ void zero_u32_kernel(uint32_t* p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -293,22 +325,14 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
 // is full, the workgroup re-estimates its survivors itself (same formulas, same result).
 
 // The EX-bit re-estimate of one survivor by a 16-lane group, in the reference's order:
-// 16 lane-strided FMA chains over u * code, reduced by the xor butterfly, then the distance
-// in double as C promotes it (HASearcher.hpp:183-231).  u(d) comes from `ures(d)`.
+// 16 lane-strided FMA chains over u * code (ex_chain on the lane-major codes), reduced by
+// the xor butterfly, then the distance in double as C promotes it (HASearcher.hpp:183-231).
+// uf(i) gives u for the lane's dim 16 i + l.
 template <int EX, typename U>
-__device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P, const U& ures,
-                                            const uint8_t* code, int l16, bool act, float ipq,
+__device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P, const U& uf,
+                                            const uint32_t* lanecode, bool act, float ipq,
                                             float on, uint32_t vi) {
-    float part = 0.f;
-    if (act) {
-#pragma unroll 4
-        for (int ch = 0; ch < (a.D >> 6); ch++) {
-            float e[4];
-            decode4<EX>(code + ch * 8 * EX, l16, e);
-#pragma unroll
-            for (int s4 = 0; s4 < 4; s4++) part = fmaf(ures(64 * ch + 16 * s4 + l16), e[s4], part);
-        }
-    }
+    const float part = act ? ex_chain<EX>(uf, lanecode, a.D >> 4, a.exl_w) : 0.f;
     const float ip = butterfly16(part);
     if (!act) return FLT_MAX;
     const float R = (float)(1 << EX);
@@ -353,7 +377,10 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     // workgroup re-estimates survivors itself, the A region holds each wave's residual
     // (NCH * 8192 >= 4 waves * D floats).
     int8_t* A = reinterpret_cast<int8_t*>(smem);
-    uint32_t* s_vec = reinterpret_cast<uint32_t*>(smem + (size_t)NCH * 8192);
+    // the A region also holds the inline pass's residuals: 4 waves x 16 x S floats
+    const int S = ul_stride(D);
+    const size_t abytes = max((size_t)NCH * 8192, (size_t)256 * S);
+    uint32_t* s_vec = reinterpret_cast<uint32_t*>(smem + abytes);
     float* s_ipq = reinterpret_cast<float*>(s_vec + kGroup * kQSurv);
     __shared__ QParam qp[kGroup];
     __shared__ uint32_t s_cnt[kGroup];
@@ -492,7 +519,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         // inline pass: re-estimate this item's survivors here
         __syncthreads();  // every wave is done with the digits in A
         const float value = (float)(1.0 / (double)sqrtf((float)D));
-        float* uw = reinterpret_cast<float*>(A) + (size_t)wid * D;
+        float* uw = reinterpret_cast<float*>(A) + (size_t)wid * 16 * S;
         const float* cen = a.centroids + (size_t)cid * D;
         int cstart[kGroup + 1];
         cstart[0] = 0;
@@ -510,7 +537,8 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                 const float* qv = a.q_rot + (size_t)P.q * D;
                 const bool use_res = P.y > 1e-5f;
 #pragma unroll 4
-                for (int d = lane; d < D; d += 64) uw[d] = use_res ? (qv[d] - cen[d]) * P.fac : value;
+                for (int d = lane; d < D; d += 64)
+                    uw[ul_index(d, S)] = use_res ? (qv[d] - cen[d]) * P.fac : value;
                 wave_sync();
                 staged = qq;
             }
@@ -524,7 +552,8 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                 const float on = reinterpret_cast<const float*>(
                     a.short_data + (size_t)(b0 + (uint32_t)(vloc >> 5)) * BB + 4 * (size_t)D)[vloc & 31];
                 const float exd = reestimate<EX>(
-                    a, P, [&](int d) { return uw[d]; }, a.long_code + (size_t)vi * CL, ln, act,
+                    a, P, [&](int i) { return uw[ln * S + i]; },
+                    a.exl + ((size_t)vi * 16 + ln) * a.exl_w, act,
                     act ? s_ipq[qq * kQSurv + si] : 0.f, on, vi);
                 if (act && ln == 0) {
                     if (exd <= P.thr) {
@@ -559,8 +588,9 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
     const int D = a.D;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
-    float* uw = reinterpret_cast<float*>(smem) + (size_t)wid * D;
-    uint32_t* pre = reinterpret_cast<uint32_t*>(smem + 4 * 4 * (size_t)D);  // scan_grid + 1
+    const int S = ul_stride(D);  // lane-major residual rows (erq_device.h)
+    float* uw = reinterpret_cast<float*>(smem) + (size_t)wid * 16 * S;
+    uint32_t* pre = reinterpret_cast<uint32_t*>(smem + (size_t)256 * S);  // scan_grid + 1
     __shared__ uint32_t wsum[4];
     // XCD-aware: workgroup b runs on XCD b % 8, like scan workgroup g on XCD g % 8; the
     // workgroups of one XCD share the slices of that XCD's scan workgroups (clusters of one
@@ -631,17 +661,22 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
                 const bool res0 = P0.y > 1e-5f;
                 wave_sync();
 #pragma unroll 4
-                for (int d = lane; d < D; d += 64) uw[d] = res0 ? (q0[d] - c0[d]) * P0.fac : value;
+                for (int d = lane; d < D; d += 64)
+                    uw[ul_index(d, S)] = res0 ? (q0[d] - c0[d]) * P0.fac : value;
                 wave_sync();
                 staged = e_first;
             }
-            exd = reestimate<EX>(a, P, [&](int d) { return uw[d]; },
-                                 a.long_code + (size_t)vi * CL, l16, act, __uint_as_float(rec.w),
-                                 on, vi);
+            exd = reestimate<EX>(a, P, [&](int i) { return uw[l16 * S + i]; },
+                                 a.exl + ((size_t)vi * 16 + l16) * a.exl_w, act,
+                                 __uint_as_float(rec.w), on, vi);
         } else {
             exd = reestimate<EX>(
-                a, P, [&](int d) { return use_res ? (qv[d] - cen[d]) * P.fac : value; },
-                a.long_code + (size_t)vi * CL, l16, act, __uint_as_float(rec.w), on, vi);
+                a, P,
+                [&](int i) {
+                    const int d = 16 * i + l16;
+                    return use_res ? (qv[d] - cen[d]) * P.fac : value;
+                },
+                a.exl + ((size_t)vi * 16 + l16) * a.exl_w, act, __uint_as_float(rec.w), on, vi);
         }
         if (act && l16 == 0) {
             if (exd <= P.thr) {
@@ -801,7 +836,10 @@ hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uin
     return hipGetLastError();
 }
 
-size_t batch_lds_bytes(int D) { return (size_t)bp_nchunk(D) * 8192 + (size_t)kGroup * kQSurv * 8; }
+size_t batch_lds_bytes(int D) {
+    return std::max((size_t)bp_nchunk(D) * 8192, (size_t)256 * ul_stride(D)) +
+           (size_t)kGroup * kQSurv * 8;
+}
 
 size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize) {
     const size_t tiles_per_item = (size_t)kItemTiles * 16;
@@ -810,6 +848,23 @@ size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize) {
 }
 
 size_t prep_wave_bytes(int D) { return 4 * ((size_t)D + 8 * ((size_t)D / 128 + 1)); }
+
+hipError_t launch_exl_relayout(const uint8_t* long_code, size_t N, int D, int EX, uint32_t* exl,
+                               hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    const int W = exl_words(D, EX);
+    const int blocks = (int)std::min<size_t>((N * 16 + 255) / 256, 1 << 16);
+    switch (EX) {
+    case 2: hipLaunchKernelGGL(exl_relayout_kernel<2>, dim3(blocks), dim3(256), 0, s, long_code, N, D, W, exl); break;
+    case 3: hipLaunchKernelGGL(exl_relayout_kernel<3>, dim3(blocks), dim3(256), 0, s, long_code, N, D, W, exl); break;
+    case 4: hipLaunchKernelGGL(exl_relayout_kernel<4>, dim3(blocks), dim3(256), 0, s, long_code, N, D, W, exl); break;
+    case 6: hipLaunchKernelGGL(exl_relayout_kernel<6>, dim3(blocks), dim3(256), 0, s, long_code, N, D, W, exl); break;
+    case 7: hipLaunchKernelGGL(exl_relayout_kernel<7>, dim3(blocks), dim3(256), 0, s, long_code, N, D, W, exl); break;
+    case 8: hipLaunchKernelGGL(exl_relayout_kernel<8>, dim3(blocks), dim3(256), 0, s, long_code, N, D, W, exl); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
@@ -860,7 +915,7 @@ static hipError_t launch_batch_scan(const BatchArgs& ba, int grid, size_t lds, h
 
 static hipError_t launch_reest(const BatchArgs& ba, int scan_grid, hipStream_t s) {
     const dim3 grid(scan_grid * kReestWG);
-    const size_t lds = 4 * (size_t)ba.D * 4 + 4 * ((size_t)scan_grid + 1);  // residuals, prefix
+    const size_t lds = (size_t)256 * ul_stride(ba.D) + 4 * ((size_t)scan_grid + 1);  // residuals, prefix
     switch (ba.EX) {
     case 2: hipLaunchKernelGGL(reest_kernel<2>, grid, dim3(256), lds, s, ba, scan_grid); break;
     case 3: hipLaunchKernelGGL(reest_kernel<3>, grid, dim3(256), lds, s, ba, scan_grid); break;

@@ -12,6 +12,7 @@
 
 #include "erq.h"
 #include "erq_batch.h"
+#include "erq_device.h"
 #include "erq_internal.h"
 
 namespace erq {
@@ -76,7 +77,7 @@ void release(erq_index* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* ptrs[] = {h->short_data, h->long_code, h->ex_factor, h->ids,    h->centroids,
                     h->cnorm,      h->P,         h->csize,     h->bstart, h->vstart,
-                    h->ws,         h->bp,        h->order};
+                    h->ws,         h->bp,        h->order, h->exl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -392,6 +393,8 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
         if (rc) return rc;
         a.bp = h->bp;
     }
+    a.exl = h->exl;
+    a.exl_w = exl_words((int)h->D, (int)h->EX);
     // HASearcher.hpp:56
     a.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
     a.qorder = nullptr;
@@ -531,8 +534,14 @@ int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, si
     return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s);
 }
 
+// The bit-plane copy of the 1-bit codes and the lane-major copy of the EX-bit codes,
+// built on the stream on first use of a kernel that reads them.
 static int ensure_bp(erq_index* h, hipStream_t s) {
-    if (h->bp) return ERQ_OK;
+    if (h->bp && h->exl) return ERQ_OK;
+    if (h->bp) {
+        (void)hipFree(h->bp);
+        h->bp = nullptr;
+    }
     const size_t bytes = h->nblocks * (size_t)bp_nchunk((int)h->D) * 32 * 64;
     hipError_t e = hipMalloc(&h->bp, bytes ? bytes : 64);
     if (e != hipSuccess) {
@@ -541,6 +550,14 @@ static int ensure_bp(erq_index* h, hipStream_t s) {
     }
     e = launch_relayout(h->short_data, h->nblocks, (int)h->D, h->bp, s);
     if (e != hipSuccess) return hip_fail(e, "bit-plane relayout");
+    const size_t exl_bytes = h->N * 16 * (size_t)exl_words((int)h->D, (int)h->EX) * 4;
+    e = hipMalloc(&h->exl, exl_bytes ? exl_bytes : 64);
+    if (e != hipSuccess) {
+        h->exl = nullptr;
+        return hip_fail(e, "lane-major ex-code alloc");
+    }
+    e = launch_exl_relayout(h->long_code, h->N, (int)h->D, (int)h->EX, h->exl, s);
+    if (e != hipSuccess) return hip_fail(e, "ex-code relayout");
     if (!h->num_cu) {
         e = hipDeviceGetAttribute(&h->num_cu, hipDeviceAttributeMultiprocessorCount, h->device);
         if (e != hipSuccess) return hip_fail(e, "device attribute");
@@ -628,6 +645,8 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.short_data = h->short_data;
         ba.bp = h->bp;
         ba.long_code = h->long_code;
+        ba.exl = h->exl;
+        ba.exl_w = exl_words((int)h->D, (int)h->EX);
         ba.ex_factor = h->ex_factor;
         ba.ids = h->ids;
         ba.csize = h->csize;

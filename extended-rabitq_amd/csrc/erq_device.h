@@ -104,6 +104,45 @@ __device__ __forceinline__ void decode4(const uint8_t* __restrict__ c, int l, fl
     }
 }
 
+// Lane-major copy of the EX-bit codes ("exl"), for the 16-lane re-estimate: lane l of a
+// vector's group handles dims d = 16 i + l (i = 0 .. D/16 - 1), in increasing i, which is
+// the order of the reference's lane-strided FMA chains.  Per vector 16 x W dwords, lane l's
+// W dwords contiguous (16-byte aligned); value i sits in dword i / vpd at bit (i % vpd) * EX,
+// vpd = floor(32 / EX).
+__host__ __device__ constexpr int exl_vpd(int ex) { return 32 / ex; }
+__host__ __device__ inline int exl_words(int D, int ex) {
+    const int vpd = exl_vpd(ex), w = (D / 16 + vpd - 1) / vpd;
+    return (w + 3) & ~3;
+}
+
+// The residual u in LDS, lane-major like exl: u[16 i + l] at ul[l * ul_stride(D) + i].  The
+// stride (a multiple of 4, = 4 m with m odd) spreads the 16 lanes' rows over distinct banks.
+__host__ __device__ inline int ul_stride(int D) { return D / 16 + ((D / 64) % 2 == 0 ? 4 : 8); }
+__device__ __forceinline__ int ul_index(int d, int S) { return (d & 15) * S + (d >> 4); }
+
+// One lane's share of <u, e> for the re-estimate: part = fma(u_i, e_i, part) over
+// i = 0 .. nvals-1 in order; uf(i) gives u for dim 16 i + l, code = the lane's W dwords.
+template <int EX, typename UF>
+__device__ __forceinline__ float ex_chain(const UF& uf, const uint32_t* __restrict__ code,
+                                          int nvals, int W) {
+    constexpr int vpd = 32 / EX;
+    constexpr uint32_t mask = (1u << EX) - 1u;
+    float part = 0.f;
+    const uint4* cw = reinterpret_cast<const uint4*>(code);
+    for (int w4 = 0; w4 < W / 4; w4++) {
+        const uint4 c4 = cw[w4];
+        const uint32_t cs[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int i0 = (4 * w4 + j) * vpd;
+#pragma unroll
+            for (int t = 0; t < vpd; t++)
+                if (i0 + t < nvals) part = fmaf(uf(i0 + t), (float)((cs[j] >> (t * EX)) & mask), part);
+        }
+    }
+    return part;
+}
+
 // Float key with the same order as the float (for radix selection / sorting).
 __device__ __forceinline__ uint32_t float_key(float f) {
     uint32_t u = __float_as_uint(f);

@@ -30,6 +30,7 @@ struct erq_index {
     void* ws = nullptr;
     size_t ws_bytes = 0;
     uint8_t* bp = nullptr;  // bit-plane copy of the 1-bit codes (batched path, built on first use)
+    uint32_t* exl = nullptr;    // lane-major EX-bit codes (with bp, built on first use)
     uint32_t* order = nullptr;  // per-query scan order scratch (counts, cursor, order)
     size_t order_cap = 0;       // u32 entries of `order`
     int plain_order = 1;        // ERQ_OPT_PLAIN_ORDER
@@ -66,6 +67,8 @@ struct ScanArgs {
     int D, EX, k, nq, K;
     int max_probes;  // > 0: scan at most this many leading probes per query
     const uint8_t* bp;  // bit-plane codes (nullptr: LUT kernel)
+    const uint32_t* exl;  // lane-major EX-bit codes (with bp)
+    int exl_w;
     int nchunk;         // 512-dim chunks of bp
     int waves_per_block;  // queries (waves) per workgroup
     float one_over_sqrtD;

@@ -378,12 +378,15 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
     constexpr int nchunk = NCH;
     const int kcap = (k + 63) & ~63;
     unsigned char* base = smem + (size_t)wid * per_wave_bytes;
+    // the residual u, lane-major (u[16 i + l] at uq[l * S + i]) for the re-estimate's lanes
+    const int S = ul_stride(D);
     float* uq = reinterpret_cast<float*>(base);
-    int8_t* A = reinterpret_cast<int8_t*>(base + 4 * (size_t)D);          // nchunk * 1 KB
+    const size_t ubytes = 4 * 16 * (size_t)S;
+    int8_t* A = reinterpret_cast<int8_t*>(base + ubytes);                 // nchunk * 1 KB
     // nchunk * 1 KB of zeros laid out like A: lanes holding MFMA rows 2..15 read their
     // (zero) A operand from here at the same offsets, so every lane loads unconditionally
     int8_t* Z = A + 1024 * (size_t)nchunk;
-    int* sums = reinterpret_cast<int*>(base + 4 * (size_t)D + 2048 * (size_t)nchunk);
+    int* sums = reinterpret_cast<int*>(base + ubytes + 2048 * (size_t)nchunk);
     WaveLds w;
     w.uq = uq;
     w.lut = nullptr;
@@ -421,22 +424,22 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
         float sumq;
         if (y > 1e-5f) {
             const float fac = (float)(1.0 / (double)y);
-            for (int d = lane; d < D; d += kWave) uq[d] = (q[d] - c[d]) * fac;
+            for (int d = lane; d < D; d += kWave) uq[ul_index(d, S)] = (q[d] - c[d]) * fac;
             wave_sync();
             float part = 0.f;
             if (lane < 16)
-                for (int d = lane; d < D; d += 16) part = part + uq[d];
+                for (int i = 0; i < (D >> 4); i++) part = part + uq[lane * S + i];
             part = butterfly16(part);
             sumq = __shfl(part, 0);
         } else {
             const float value = (float)(1.0 / (double)sqrtf((float)D));
-            for (int d = lane; d < D; d += kWave) uq[d] = value;
+            for (int d = lane; d < D; d += kWave) uq[ul_index(d, S)] = value;
             wave_sync();
             sumq = (float)D * value;
         }
         float mx = 0.f, mn = 0.f;
         for (int d = lane; d < D; d += kWave) {
-            float u = uq[d];
+            float u = uq[ul_index(d, S)];
             mx = fmaxf(mx, u);
             mn = fminf(mn, u);
         }
@@ -447,7 +450,7 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
         for (int d = lane; d < nchunk * 512; d += kWave) {
             int qv = 0;
             if (d < D) {
-                float tmp = uq[d] / delta;
+                float tmp = uq[ul_index(d, S)] / delta;
                 double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
                 qv = (int)r;
             }
@@ -552,13 +555,9 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
                 const uint32_t vi = v0 + (uint32_t)(b * 32 + jj);
                 float part = 0.f;
                 if (j >= 0) {
-                    const uint8_t* code = a.long_code + (size_t)vi * CL;
-                    for (int ch = 0; ch < (D >> 6); ch++) {
-                        float e[4];
-                        decode4<EX>(code + ch * 8 * EX, l16, e);
-                        for (int s4 = 0; s4 < 4; s4++)
-                            part = fmaf(uq[64 * ch + 16 * s4 + l16], e[s4], part);
-                    }
+                    const float* ur = uq + l16 * S;
+                    part = ex_chain<EX>([&](int i) { return ur[i]; },
+                                        a.exl + ((size_t)vi * 16 + l16) * a.exl_w, D >> 4, a.exl_w);
                 }
                 const float ip = butterfly16(part);
                 const float ipq_j = __shfl(ipq_l, jj);
@@ -606,7 +605,7 @@ template <int EX>
 hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
     int waves = a.waves_per_block > 0 ? a.waves_per_block : 1;
     if (a.bp && a.D <= 2048) {
-        const size_t per_wave = 4 * (size_t)a.D + 2048 * (size_t)a.nchunk + 128 +
+        const size_t per_wave = 4 * 16 * (size_t)ul_stride(a.D) + 2048 * (size_t)a.nchunk + 128 +
                                 (size_t)8 * ((a.k + 63) & ~63);
         while (waves > 1 && per_wave * waves > 65536) waves--;
         dim3 grid((a.nq + waves - 1) / waves), block(64 * waves);
