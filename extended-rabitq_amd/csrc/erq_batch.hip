@@ -222,10 +222,6 @@ struct QParam {
     int q;
 };
 
-// u[d] lives at uq[swz(d)]: one 8-float pad per 128 floats, so that the quantization loop's
-// lanes (whose dims differ by multiples of 128) hit different LDS banks
-__device__ __forceinline__ int swz(int d) { return d + 8 * (d >> 7); }
-
 __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -235,76 +231,21 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
     if ((gridDim.x & 7) == 0) blk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const uint32_t e = blk * (blockDim.x >> 6) + wid;
     if (e >= a.n_items[2]) return;
-    const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512;
+    const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512, S = ul_stride(D);
     float* uq = reinterpret_cast<float*>(smem + (size_t)wid * per_wave_bytes);
     const uint32_t slot = a.entries[e];
     const int q = query_of_slot(a.probe_off, a.nq, slot);
     const uint32_t cid = a.probe_ids[slot];
-    const float* qv = a.q_rot + (size_t)q * D;
-    const float* cen = a.centroids + (size_t)cid * D;
     QParam P;
     P.q = q;
     P.sqr_y = a.probe_sqr_y[slot];
     P.y = sqrtf(P.sqr_y);
-    float sumq;
-    if (P.y > 1e-5f) {
-        P.fac = (float)(1.0 / (double)P.y);
-#pragma unroll 4
-        for (int d = lane; d < D; d += 64) uq[swz(d)] = (qv[d] - cen[d]) * P.fac;
-        wave_sync();
-        float part = 0.f;
-        if (lane < 16) {
-#pragma unroll 16
-            for (int d = lane; d < D; d += 16) part = part + uq[swz(d)];
-        }
-        part = butterfly16(part);
-        sumq = __shfl(part, 0);
-    } else {
-        P.fac = 0.f;
-        const float value = (float)(1.0 / (double)sqrtf((float)D));
-        for (int d = lane; d < D; d += 64) uq[swz(d)] = value;
-        wave_sync();
-        sumq = (float)D * value;
-    }
-    float mx = 0.f, mn = 0.f;
-#pragma unroll 4
-    for (int d = lane; d < D; d += 64) {
-        float u = uq[swz(d)];
-        mx = fmaxf(mx, u);
-        mn = fminf(mn, u);
-    }
-    mx = wave_max(mx);
-    mn = wave_min(mn);
-    const float delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
-    // One 16-byte unit (chunk c, plane p, lane group g) per lane and step: dims
-    // 512c + 128g + 32jw + 8b + p at byte 4jw + b, low digits then high digits, written
-    // straight to the entry's operand rows in HBM.
-    uint4* gd = reinterpret_cast<uint4*>(a.adig + (size_t)e * 2 * Dp);
-    for (int un = lane; un < nchunk * 32; un += 64) {
-        const int c = un >> 5, p = (un >> 2) & 7, g = un & 3;
-        uint32_t lo[4], hi[4];
-#pragma unroll
-        for (int jw = 0; jw < 4; jw++) {
-            uint32_t lw = 0, hw = 0;
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int d = 512 * c + 128 * g + 32 * jw + 8 * b + p;
-                int qi = 0;
-                if (d < D) {  // high_acc_quantize16, space.hpp:695-711
-                    float tmp = uq[swz(d)] / delta;
-                    double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
-                    qi = (int)r;
-                }
-                const int l7 = qi & 127;
-                lw |= (uint32_t)l7 << (8 * b);
-                hw |= ((uint32_t)((qi - l7) >> 7) & 0xFFu) << (8 * b);
-            }
-            lo[jw] = lw;
-            hi[jw] = hw;
-        }
-        gd[2 * un] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
-        gd[2 * un + 1] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
-    }
+    float sumq, delta;
+    prepare_residual(a.q_rot + (size_t)q * D, a.centroids + (size_t)cid * D, P.y, D, S, uq, lane,
+                     P.fac, sumq, delta);
+    // the digits go straight to the entry's operand rows in HBM
+    quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta,
+                    reinterpret_cast<uint32_t*>(a.adig + (size_t)e * 2 * Dp), lane);
     if (lane == 0) {
         P.sumq = sumq;
         P.delta = delta;
@@ -847,7 +788,7 @@ size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize) {
     return (pairs / kGroup + 1) * max_split + N / tiles_per_item + K + 8;
 }
 
-size_t prep_wave_bytes(int D) { return 4 * ((size_t)D + 8 * ((size_t)D / 128 + 1)); }
+size_t prep_wave_bytes(int D) { return 4 * 16 * (size_t)ul_stride(D); }
 
 hipError_t launch_exl_relayout(const uint8_t* long_code, size_t N, int D, int EX, uint32_t* exl,
                                hipStream_t s) {

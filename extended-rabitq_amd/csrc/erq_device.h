@@ -120,6 +120,103 @@ __host__ __device__ inline int exl_words(int D, int ex) {
 __host__ __device__ inline int ul_stride(int D) { return D / 16 + ((D / 64) % 2 == 0 ? 4 : 8); }
 __device__ __forceinline__ int ul_index(int d, int S) { return (d & 15) * S + (d >> 4); }
 
+// high_acc_quantize16 (space.hpp:695-711) of one value: the IEEE float quotient
+// tmp = u / delta, rounded half away from zero in double as the reference writes it.
+// rd = 1.0 / (double)delta.  The double product u * rd is within 2^-52 (relative) of
+// u / delta, and a quotient of two floats never lies closer than 2^-49 (relative) to a
+// float rounding boundary without being the float itself, so rounding the product to
+// float gives exactly the IEEE float division.
+__device__ __forceinline__ int quantize_digit(float u, double rd) {
+    const float tmp = (float)((double)u * rd);
+    const double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
+    return (int)r;
+}
+
+// The reference's per-(query, cluster) preparation (normalize_query16 and the digit scale
+// of high_acc_quantize16, space.hpp:695-778) by one wave: u = (q - c) * (1 / y) stored
+// lane-major in uq (stride S), sumq = the sum of u in _mm512 lane order (16 lane-strided
+// chains, then butterfly16), delta = max |u| / 8191.  D is a multiple of 64; q and c are
+// 16-byte aligned.
+__device__ __forceinline__ void prepare_residual(const float* __restrict__ q,
+                                                 const float* __restrict__ c, float y, int D,
+                                                 int S, float* uq, int lane, float& fac,
+                                                 float& sumq, float& delta) {
+    float mx = 0.f, mn = 0.f;
+    if (y > 1e-5f) {
+        fac = (float)(1.0 / (double)y);
+        for (int d = 4 * lane; d < D; d += 256) {
+            const float4 qa = *reinterpret_cast<const float4*>(q + d);
+            const float4 ca = *reinterpret_cast<const float4*>(c + d);
+            const float u0 = (qa.x - ca.x) * fac, u1 = (qa.y - ca.y) * fac;
+            const float u2 = (qa.z - ca.z) * fac, u3 = (qa.w - ca.w) * fac;
+            mx = fmaxf(fmaxf(mx, u0), fmaxf(u1, fmaxf(u2, u3)));
+            mn = fminf(fminf(mn, u0), fminf(u1, fminf(u2, u3)));
+            uq[ul_index(d, S)] = u0;
+            uq[ul_index(d + 1, S)] = u1;
+            uq[ul_index(d + 2, S)] = u2;
+            uq[ul_index(d + 3, S)] = u3;
+        }
+        wave_sync();
+        float part = 0.f;
+        if (lane < 16) {
+            const float4* row = reinterpret_cast<const float4*>(uq + lane * S);
+            for (int i = 0; i < (D >> 6); i++) {
+                const float4 v = row[i];
+                part = part + v.x;
+                part = part + v.y;
+                part = part + v.z;
+                part = part + v.w;
+            }
+        }
+        part = butterfly16(part);
+        sumq = __shfl(part, 0);
+    } else {
+        fac = 0.f;
+        const float value = (float)(1.0 / (double)sqrtf((float)D));
+        for (int d = lane; d < D; d += kWave) uq[ul_index(d, S)] = value;
+        mx = value;
+        wave_sync();
+        sumq = (float)D * value;
+    }
+    mx = wave_max(mx);
+    mn = wave_min(mn);
+    delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
+}
+
+// The int8 MFMA digits of the quantized residual (digit 0: q mod 128, digit 1: floor(q /
+// 128)) in operand order: 16-byte units (chunk c, plane p, lane group g) at unit index
+// 32 c + 4 p + g, low-digit row then high-digit row; byte 4 jw + b of a row is dim
+// 512 c + 128 g + 32 jw + 8 b + p.  One task = (c, g, jw, plane pair 2 pp, 2 pp + 1): the
+// 8 dims 512 c + 128 g + 32 jw + 8 b + 2 pp + {0, 1}; task t of a wave has jw = t & 3,
+// pp = (t >> 2) & 3, g = (t >> 4) & 3, c = t >> 6 (lane-major u: conflict-free LDS reads
+// within 16 lanes).  Dims >= D (a whole 32-dim block, D being a multiple of 64) get 0.
+// dst is the pair's 2 * nchunk * 512 bytes as dwords.
+__device__ __forceinline__ void quantize_digits(const float* uq, int S, int D, int nchunk,
+                                                double rd, uint32_t* dst, int lane) {
+    for (int t = lane; t < nchunk * 64; t += kWave) {
+        const int jw = t & 3, pp = (t >> 2) & 3, g = (t >> 4) & 3, c = t >> 6;
+        const int base = 512 * c + 128 * g + 32 * jw + 2 * pp;
+        uint32_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+        if (base < D) {
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int d = base + 8 * b;
+                const int q0 = quantize_digit(uq[ul_index(d, S)], rd);
+                const int q1 = quantize_digit(uq[ul_index(d + 1, S)], rd);
+                lo0 |= (uint32_t)(q0 & 127) << (8 * b);
+                hi0 |= ((uint32_t)(q0 >> 7) & 0xFFu) << (8 * b);
+                lo1 |= (uint32_t)(q1 & 127) << (8 * b);
+                hi1 |= ((uint32_t)(q1 >> 7) & 0xFFu) << (8 * b);
+            }
+        }
+        const int un = 32 * c + 8 * pp + g;  // plane 2 pp; plane 2 pp + 1 is un + 4
+        dst[8 * un + jw] = lo0;
+        dst[8 * un + 4 + jw] = hi0;
+        dst[8 * un + 32 + jw] = lo1;
+        dst[8 * un + 36 + jw] = hi1;
+    }
+}
+
 // One lane's share of <u, e> for the re-estimate: part = fma(u_i, e_i, part) over
 // i = 0 .. nvals-1 in order; uf(i) gives u for dim 16 i + l, code = the lane's W dwords.
 template <int EX, typename UF>

@@ -201,14 +201,11 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs a, int per_wave_byte
         mn = wave_min(mn);
         const float vmax = fmaxf(fabsf(mn), fabsf(mx));
         const float delta = vmax / 8191.0f;
+        const double rdelta = 1.0 / (double)delta;
         int shift_part = 0;
         for (int m = lane; m < M; m += kWave) {
             int qv[4];
-            for (int t = 0; t < 4; t++) {
-                float tmp = w.uq[4 * m + t] / delta;
-                double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
-                qv[t] = (int)r;
-            }
+            for (int t = 0; t < 4; t++) qv[t] = quantize_digit(w.uq[4 * m + t], rdelta);
             // min over the 16 subset sums = sum of the negative terms
             const int vmin = min(0, qv[0]) + min(0, qv[1]) + min(0, qv[2]) + min(0, qv[3]);
             uint32_t wv[8];
@@ -421,46 +418,10 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
         const float sqr_y = a.probe_sqr_y[p];
         const float y = sqrtf(sqr_y);
         const float* c = a.centroids + (size_t)cid * D;
-        float sumq;
-        if (y > 1e-5f) {
-            const float fac = (float)(1.0 / (double)y);
-            for (int d = lane; d < D; d += kWave) uq[ul_index(d, S)] = (q[d] - c[d]) * fac;
-            wave_sync();
-            float part = 0.f;
-            if (lane < 16)
-                for (int i = 0; i < (D >> 4); i++) part = part + uq[lane * S + i];
-            part = butterfly16(part);
-            sumq = __shfl(part, 0);
-        } else {
-            const float value = (float)(1.0 / (double)sqrtf((float)D));
-            for (int d = lane; d < D; d += kWave) uq[ul_index(d, S)] = value;
-            wave_sync();
-            sumq = (float)D * value;
-        }
-        float mx = 0.f, mn = 0.f;
-        for (int d = lane; d < D; d += kWave) {
-            float u = uq[ul_index(d, S)];
-            mx = fmaxf(mx, u);
-            mn = fminf(mn, u);
-        }
-        mx = wave_max(mx);
-        mn = wave_min(mn);
-        const float delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
-        // int8 digits in MFMA order: [chunk][plane p][lane group][row 0/1][16 bytes]
-        for (int d = lane; d < nchunk * 512; d += kWave) {
-            int qv = 0;
-            if (d < D) {
-                float tmp = uq[ul_index(d, S)] / delta;
-                double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
-                qv = (int)r;
-            }
-            const int ch = d >> 9, dd = d & 511;
-            const int gg = dd >> 7, jw = (dd >> 5) & 3, bb = (dd >> 3) & 3, pp = dd & 7;
-            int8_t* dst = A + ((((size_t)ch * 8 + pp) * 4 + gg) * 2) * 16 + 4 * jw + bb;
-            const int lo = qv & 127;
-            dst[0] = (int8_t)lo;
-            dst[16] = (int8_t)((qv - lo) >> 7);
-        }
+        float fac, sumq, delta;
+        prepare_residual(q, c, y, D, S, uq, lane, fac, sumq, delta);
+        quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta, reinterpret_cast<uint32_t*>(A),
+                        lane);
         wave_sync();
 
         const int n = (int)a.csize[cid];
