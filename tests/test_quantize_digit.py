@@ -1,0 +1,58 @@
+"""The digit quantization of the kernels takes the IEEE float quotient u / delta of
+high_acc_quantize16 (space.hpp:695-711) as float((double)u * (1.0 / (double)delta)).
+
+The claim: the double product is within 2^-52 (relative) of the exact quotient, and a
+quotient of two floats is never within 2^-49 (relative) of a float rounding boundary
+unless it is exactly representable, so both round to the same float.  Checked here
+against numpy's IEEE float32 division on random residual scales and on quotients placed
+a few ulps either side of every kind of half-integer, where the digit flips.
+"""
+import numpy as np
+
+
+def _digits(tmp):
+    r = tmp.astype(np.float64) + 0.5 - (tmp < 0).astype(np.float64)
+    return np.trunc(r).astype(np.int64)
+
+
+def _check(u, delta):
+    ref = (u / delta).astype(np.float32)  # IEEE float32 division
+    rd = 1.0 / delta.astype(np.float64)
+    got = (u.astype(np.float64) * rd).astype(np.float32)
+    assert ref.dtype == np.float32
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(_digits(got), _digits(ref))
+
+
+def test_random_residuals():
+    rng = np.random.default_rng(11)
+    n = 4_000_000
+    vmax = (0.02 + rng.random(n)).astype(np.float32)
+    delta = (vmax / np.float32(8191.0)).astype(np.float32)
+    u = ((rng.random(n) * 2 - 1).astype(np.float32) * vmax).astype(np.float32)
+    _check(u, delta)
+
+
+def test_near_half_integers():
+    rng = np.random.default_rng(12)
+    n = 400_000
+    vmax = (0.02 + rng.random(n)).astype(np.float32)
+    delta = (vmax / np.float32(8191.0)).astype(np.float32)
+    k = rng.integers(0, 8191, n)
+    u0 = ((k + 0.5) * delta.astype(np.float64)).astype(np.float32)
+    for step in range(-3, 4):
+        u = u0.copy()
+        for _ in range(abs(step)):
+            u = np.nextafter(u, np.float32(np.inf if step > 0 else -np.inf)).astype(np.float32)
+        _check(u, delta)
+        _check(-u, delta)
+
+
+def test_extremes():
+    # delta of the constant residual (|q - c| <= 1e-5) and the full digit range
+    for D in (64, 128, 1536, 3072):
+        value = np.float32(1.0 / np.sqrt(np.float32(D)))
+        delta = np.full(3, value / np.float32(8191.0), np.float32)
+        u = np.array([value, -value, 0.0], np.float32)
+        _check(u, delta)
+        assert list(_digits((u / delta).astype(np.float32))) == [8191, -8191, 0]
