@@ -514,14 +514,11 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     }
 }
 
-// The round's survivor list, one record per 16-lane group (four per wave), u(d) read
-// straight from the rotated query and the centroid.
 // The round's survivor list, all slices as one index space (exclusive prefix of the slice
-// counts in LDS), each wave taking an equal contiguous share, four records (one per 16-lane
-// group) at a time.  Records of one (query, cluster) pair are contiguous; when the four
-// records of a step share their pair, the residual u is staged once in the wave's LDS
-// slice and kept for the next step, otherwise each group forms u from the rotated query
-// and the centroid as it goes.
+// counts in LDS), each wave taking an equal contiguous share.  Records of one (query,
+// cluster) pair come in runs; the wave walks its share run by run: it stages the pair's
+// residual u once in its LDS slice (lane-major, from 16-byte loads of the rotated query and
+// the centroid) and re-estimates the run's records four at a time, one per 16-lane group.
 constexpr int kReestWG = 4;  // re-estimation workgroups per scan workgroup
 template <int EX>
 __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) {
@@ -563,62 +560,62 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
     const size_t nw = (size_t)(gridDim.x / xs) * 4;
     const size_t wv = (size_t)(blockIdx.x / xs) * 4 + wid;
     const size_t r_begin = total * wv / nw, r_end = total * (wv + 1) / nw;
-    const size_t CL = (size_t)D * EX / 8;
     const size_t BB = 4 * (size_t)D + 128;
     const float value = (float)(1.0 / (double)sqrtf((float)D));
     unsigned long long st_surv = 0, st_cand = 0;
     uint32_t staged = 0xFFFFFFFFu;
-    for (size_t r0 = r_begin; r0 < r_end; r0 += 4) {  // wave-uniform
-        const size_t r = r0 + lg;
-        const bool act = r < r_end;
-        uint4 rec = make_uint4(0, 0, 0, 0);
-        if (act) {  // slice of record r: the last i with pre[i] <= r
-            int lo = 0, hi = ns - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (pre[mid] <= (uint32_t)r) lo = mid; else hi = mid - 1;
+    QParam P{};
+    int sl = -1;  // slice of this group's record (records only move forward)
+    for (size_t r = r_begin; r < r_end;) {  // wave-uniform
+        const size_t rr = r + lg;
+        const bool inr = rr < r_end;
+        uint4 rec = make_uint4(0xFFFFFFFFu, 0, 0, 0);
+        if (inr) {
+            if (sl < 0) {  // the last i with pre[i] <= rr
+                int lo = 0, hi = ns - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (pre[mid] <= (uint32_t)rr) lo = mid; else hi = mid - 1;
+                }
+                sl = lo;
             }
-            rec = a.surv[(size_t)(x + xs * lo) * slice_cap + (r - pre[lo])];
+            while (pre[sl + 1] <= (uint32_t)rr) sl++;
+            rec = a.surv[(size_t)(x + xs * sl) * slice_cap + (rr - pre[sl])];
         }
-        const QParam P = act ? reinterpret_cast<const QParam*>(a.qparams)[rec.x] : QParam{};
+        // the run of group 0's pair: the leading groups whose record has its entry
+        const uint32_t e0 = __shfl(rec.x, 0);  // group 0 is always in range
+        const uint64_t same = __ballot(inr && rec.x == e0);
+        const int m = same == ~0ull ? 4 : __builtin_ctzll(~same) >> 4;
+        const bool act = lg < m;
+        if (e0 != staged) {
+            P = reinterpret_cast<const QParam*>(a.qparams)[e0];
+            const float* qv = a.q_rot + (size_t)P.q * D;
+            const float* cen = a.centroids + (size_t)__shfl(rec.y, 0) * D;
+            wave_sync();
+            if (P.y > 1e-5f) {
+                for (int d = 4 * lane; d < D; d += 256) {
+                    const float4 qa = *reinterpret_cast<const float4*>(qv + d);
+                    const float4 ca = *reinterpret_cast<const float4*>(cen + d);
+                    uw[ul_index(d, S)] = (qa.x - ca.x) * P.fac;
+                    uw[ul_index(d + 1, S)] = (qa.y - ca.y) * P.fac;
+                    uw[ul_index(d + 2, S)] = (qa.z - ca.z) * P.fac;
+                    uw[ul_index(d + 3, S)] = (qa.w - ca.w) * P.fac;
+                }
+            } else {
+                for (int d = lane; d < D; d += 64) uw[ul_index(d, S)] = value;
+            }
+            wave_sync();
+            staged = e0;
+        }
         const uint32_t cid = rec.y, vloc = rec.z;
         const uint32_t vi = act ? a.vstart[cid] + vloc : 0;
-        const float* qv = a.q_rot + (size_t)P.q * D;
-        const float* cen = a.centroids + (size_t)cid * D;
-        const bool use_res = P.y > 1e-5f;
         const float on = act ? reinterpret_cast<const float*>(
                                    a.short_data + (size_t)(a.bstart[cid] + (vloc >> 5)) * BB +
                                    4 * (size_t)D)[vloc & 31]
                              : 0.f;
-        const uint32_t e_first = __shfl(rec.x, 0);  // group 0 is always active
-        const bool same = __all(!act || rec.x == e_first);
-        float exd;
-        if (same) {
-            if (e_first != staged) {
-                // every lane stages with group 0's pair (inactive groups hold no record)
-                const QParam P0 = reinterpret_cast<const QParam*>(a.qparams)[e_first];
-                const float* q0 = a.q_rot + (size_t)P0.q * D;
-                const float* c0 = a.centroids + (size_t)__shfl(cid, 0) * D;
-                const bool res0 = P0.y > 1e-5f;
-                wave_sync();
-#pragma unroll 4
-                for (int d = lane; d < D; d += 64)
-                    uw[ul_index(d, S)] = res0 ? (q0[d] - c0[d]) * P0.fac : value;
-                wave_sync();
-                staged = e_first;
-            }
-            exd = reestimate<EX>(a, P, [&](int i) { return uw[l16 * S + i]; },
-                                 a.exl + ((size_t)vi * 16 + l16) * a.exl_w, act,
-                                 __uint_as_float(rec.w), on, vi);
-        } else {
-            exd = reestimate<EX>(
-                a, P,
-                [&](int i) {
-                    const int d = 16 * i + l16;
-                    return use_res ? (qv[d] - cen[d]) * P.fac : value;
-                },
-                a.exl + ((size_t)vi * 16 + l16) * a.exl_w, act, __uint_as_float(rec.w), on, vi);
-        }
+        const float exd = reestimate<EX>(a, P, [&](int i) { return uw[l16 * S + i]; },
+                                         a.exl + ((size_t)vi * 16 + l16) * a.exl_w, act,
+                                         __uint_as_float(rec.w), on, vi);
         if (act && l16 == 0) {
             if (exd <= P.thr) {
                 add_candidate(a, P.q, exd, a.ids[vi]);
@@ -626,6 +623,7 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             }
             st_surv++;
         }
+        r += m;
     }
     if (a.stats) {
         wave_stat_add(&a.stats[2], st_surv);

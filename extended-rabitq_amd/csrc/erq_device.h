@@ -226,15 +226,24 @@ __device__ __forceinline__ float ex_chain(const UF& uf, const uint32_t* __restri
     constexpr uint32_t mask = (1u << EX) - 1u;
     float part = 0.f;
     const uint4* cw = reinterpret_cast<const uint4*>(code);
-    for (int w4 = 0; w4 < W / 4; w4++) {
-        const uint4 c4 = cw[w4];
-        const uint32_t cs[4] = {c4.x, c4.y, c4.z, c4.w};
+    const int n4 = W / 4;
+    for (int b4 = 0; b4 < n4; b4 += 4) {
+        uint4 c4[4];  // up to four 16-byte loads in flight before the chain uses them
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int i0 = (4 * w4 + j) * vpd;
+        for (int j4 = 0; j4 < 4; j4++)
+            if (b4 + j4 < n4) c4[j4] = cw[b4 + j4];
 #pragma unroll
-            for (int t = 0; t < vpd; t++)
-                if (i0 + t < nvals) part = fmaf(uf(i0 + t), (float)((cs[j] >> (t * EX)) & mask), part);
+        for (int j4 = 0; j4 < 4; j4++) {
+            if (b4 + j4 >= n4) break;
+            const uint32_t cs[4] = {c4[j4].x, c4[j4].y, c4[j4].z, c4[j4].w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int i0 = (4 * (b4 + j4) + j) * vpd;
+#pragma unroll
+                for (int t = 0; t < vpd; t++)
+                    if (i0 + t < nvals)
+                        part = fmaf(uf(i0 + t), (float)((cs[j] >> (t * EX)) & mask), part);
+            }
         }
     }
     return part;
