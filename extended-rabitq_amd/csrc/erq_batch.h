@@ -21,7 +21,7 @@ struct BatchArgs {
     const uint32_t* bstart;
     const uint32_t* vstart;
     const uint32_t* items;     // 3 x n_items: cluster, (first entry << 4) | count, first tile
-    const uint32_t* entries;   // probe slots grouped by cluster
+    const uint32_t* entries;   // (probe slot, query) per entry, grouped by cluster
     uint32_t* n_items;         // [0] items of this round, [2] entries, [3] queued items
     uint32_t* ovf;             // items queued for the inline pass (slice full)
     const uint32_t* probe_ids; // per slot: cluster id

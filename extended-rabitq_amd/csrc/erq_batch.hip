@@ -198,18 +198,8 @@ __global__ void scatter_kernel(const uint32_t* __restrict__ off, const uint32_t*
         uint32_t c = pid[p];
         if (c >= (uint32_t)K) continue;
         uint32_t pos = atomicAdd(&cursor[c], 1u);
-        entries[pos] = p;  // the probe slot; its query is found by the consumer
+        reinterpret_cast<uint2*>(entries)[pos] = make_uint2(p, (uint32_t)q);
     }
-}
-
-// query of a probe slot: binary search in the CSR offsets
-__device__ __forceinline__ int query_of_slot(const uint32_t* off, int nq, uint32_t p) {
-    int lo = 0, hi = nq;  // off[lo] <= p < off[hi]
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (off[mid] <= p) lo = mid; else hi = mid;
-    }
-    return lo;
 }
 
 // ---------------------------------------------------------------- per-pair preparation
@@ -233,8 +223,9 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
     if (e >= a.n_items[2]) return;
     const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512, S = ul_stride(D);
     float* uq = reinterpret_cast<float*>(smem + (size_t)wid * per_wave_bytes);
-    const uint32_t slot = a.entries[e];
-    const int q = query_of_slot(a.probe_off, a.nq, slot);
+    const uint2 ent = reinterpret_cast<const uint2*>(a.entries)[e];
+    const uint32_t slot = ent.x;
+    const int q = (int)ent.y;
     const uint32_t cid = a.probe_ids[slot];
     QParam P;
     P.q = q;

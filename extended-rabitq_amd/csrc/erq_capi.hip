@@ -450,6 +450,10 @@ int erq_set_option(erq_index* h, int option, long value) {
         h->batch_surv_cap = (size_t)value;
         return ERQ_OK;
     }
+    if (option == ERQ_OPT_BATCH_PAIR_BUDGET && value >= 1024 && value <= (1L << 26)) {
+        h->batch_pair_budget = (size_t)value;
+        return ERQ_OK;
+    }
     set_error("erq_set_option: unknown option or value");
     return ERQ_EINVAL;
 }
@@ -462,6 +466,7 @@ long erq_get_option(const erq_index* h, int option) {
     case ERQ_OPT_BATCH_WG_PER_CU: return h->batch_wg_per_cu;
     case ERQ_OPT_BATCH_SURVIVOR_CAP: return (long)h->batch_surv_cap;
     case ERQ_OPT_PLAIN_ORDER: return h->plain_order;
+    case ERQ_OPT_BATCH_PAIR_BUDGET: return (long)h->batch_pair_budget;
     default: return -1;
     }
 }
@@ -570,14 +575,13 @@ static int ensure_bp(erq_index* h, hipStream_t s) {
 constexpr size_t kExactRanks = 4;
 static size_t next_round(size_t r0) { return std::min(4 * r0, r0 + 128); }
 
-constexpr size_t kPairBudget = 1u << 20;  // (query, cluster) pairs prepared per round
 constexpr size_t kMaxBatchGrid = 4096;    // workgroups of the persistent batched kernel
 
-static size_t batch_chunk_nq(size_t nq, size_t max_nprobe) {
+static size_t batch_chunk_nq(size_t nq, size_t max_nprobe, size_t budget) {
     size_t width = 1;
     for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0))
         width = std::max(width, std::min(next_round(r0), max_nprobe) - r0);
-    return std::max<size_t>(1, std::min(nq, kPairBudget / width));
+    return std::max<size_t>(1, std::min(nq, budget / width));
 }
 
 static size_t max_cluster_size(const erq_index* h) {
@@ -588,13 +592,13 @@ static size_t max_cluster_size(const erq_index* h) {
 
 static size_t batch_ws_bytes(const erq_index* h, size_t nq, size_t max_nprobe, int cap) {
     const size_t K = h->K, D = h->D;
-    const size_t cq = batch_chunk_nq(nq, max_nprobe);
-    const size_t pairs = std::min(cq * std::max<size_t>(max_nprobe, 1), kPairBudget) + 8;
+    const size_t cq = batch_chunk_nq(nq, max_nprobe, h->batch_pair_budget);
+    const size_t pairs = std::min(cq * std::max<size_t>(max_nprobe, 1), h->batch_pair_budget) + 8;
     const size_t Dp = (size_t)bp_nchunk((int)D) * 512;
     const size_t nitems = batch_items_bound(pairs, K, h->N, max_cluster_size(h));
     return 13 * 256 + align_up(K * 4, 256) * 2 + align_up(16, 256) +
            align_up((size_t)h->batch_surv_cap * 16, 256) + align_up(kMaxBatchGrid * 4, 256) +
-           align_up(nitems * 12, 256) + align_up(nitems * 4, 256) + align_up(pairs * 4 + 4, 256) +
+           align_up(nitems * 12, 256) + align_up(nitems * 4, 256) + align_up(pairs * 8 + 8, 256) +
            align_up(cq * 4, 256) + align_up(cq * (size_t)cap * 4, 256) * 2 +
            align_up(pairs * 2 * Dp, 256) + align_up(pairs * 48, 256);
 }
@@ -612,8 +616,8 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s, (int)kExactRanks);
     if (rc || max_nprobe <= kExactRanks) return rc;
     const int cap = kCandCap;
-    const size_t cq = batch_chunk_nq(nq, max_nprobe);
-    const size_t pairs = std::min(cq * max_nprobe, kPairBudget) + 8;
+    const size_t cq = batch_chunk_nq(nq, max_nprobe, h->batch_pair_budget);
+    const size_t pairs = std::min(cq * max_nprobe, h->batch_pair_budget) + 8;
     const size_t Dp = (size_t)bp_nchunk((int)h->D) * 512;
     Ws ws{ws_base, ws_off};
     uint32_t* counts = ws.take<uint32_t>(h->K);
@@ -622,7 +626,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     const size_t nitems_max = batch_items_bound(pairs, h->K, h->N, max_cluster_size(h));
     uint32_t* items = ws.take<uint32_t>(nitems_max * 3);
     uint32_t* ovf = ws.take<uint32_t>(nitems_max);
-    uint32_t* entries = ws.take<uint32_t>(pairs + 1);
+    uint32_t* entries = ws.take<uint32_t>(2 * pairs + 2);
     uint32_t* cand_cnt = ws.take<uint32_t>(cq);
     float* cand_d = ws.take<float>(cq * (size_t)cap);
     uint32_t* cand_i = ws.take<uint32_t>(cq * (size_t)cap);

@@ -39,6 +39,7 @@ struct erq_index {
     int plain_waves = 1;    // ERQ_OPT_PLAIN_WAVES
     int batch_wg_per_cu = 3;  // ERQ_OPT_BATCH_WG_PER_CU
     size_t batch_surv_cap = size_t(1) << 26;  // ERQ_OPT_BATCH_SURVIVOR_CAP
+    size_t batch_pair_budget = size_t(1) << 22;  // ERQ_OPT_BATCH_PAIR_BUDGET
     std::mutex mu;
 };
 

@@ -51,7 +51,11 @@ enum {
     /* Order in which the per-query scan takes queries: 1 (default) grouped by first probed
      * cluster, so queries scanned at the same time share clusters in L2; 0 as given.
      * Results are identical. */
-    ERQ_OPT_PLAIN_ORDER = 5
+    ERQ_OPT_PLAIN_ORDER = 5,
+    /* (query, cluster) pairs the batched search prepares per round, 1024..2^26 (default
+     * 2^22; 3 KB of digits each at D = 1536).  Queries are taken in chunks whose widest
+     * round fits it.  Results are identical; only workspace size and speed differ. */
+    ERQ_OPT_BATCH_PAIR_BUDGET = 6
 };
 int erq_set_option(erq_index* h, int option, long value);
 long erq_get_option(const erq_index* h, int option);

@@ -41,12 +41,15 @@ def _probe_lists(idx, qr, nprobes):
     return [oi.centroid_topn(qr[i], int(nprobes[i])) for i in range(qr.shape[0])]
 
 
-def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None):
+def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None, pair_budget=None):
     import torch
     from erq.gpu import IVF
     g = IVF.from_index(idx)
     if survivor_cap is not None:
         g.set_option(4, survivor_cap)   # ERQ_OPT_BATCH_SURVIVOR_CAP
+    if pair_budget is not None:
+        g.set_option(6, pair_budget)    # ERQ_OPT_BATCH_PAIR_BUDGET
+        assert g.option(6) == pair_budget
     npq = np.array([len(p[0]) for p in probe_lists])
     off = np.concatenate([[0], np.cumsum(npq)]).astype(np.uint32)
     pid = np.concatenate([p[0] for p in probe_lists]).astype(np.uint32)
@@ -61,11 +64,11 @@ def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None):
     return out
 
 
-def _check(idx, q, k, nprobes, label, survivor_cap=None):
+def _check(idx, q, k, nprobes, label, survivor_cap=None, pair_budget=None):
     qr = rotate_np(idx, q)
     pl = _probe_lists(idx, qr, nprobes)
     oid, od, ost = _oracle_rounds(idx, qr, k, pl)
-    gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)), survivor_cap)
+    gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)), survivor_cap, pair_budget)
     print(f"[{label}] stats={st.tolist()} oracle blocks/survivors/candidates={ost.tolist()}",
           flush=True)
     assert st[5] == 0, "candidate buffer overflow"
@@ -88,6 +91,13 @@ def test_batched_survivor_list_overflow(cap):
     re-estimate the overflow themselves; the result stays bit-exact."""
     idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
     _check(idx, q, 100, np.full(q.shape[0], 24), f"cap {cap}", survivor_cap=cap)
+
+
+def test_batched_query_chunks():
+    """A pair budget of 1024 with rounds up to 48 wide takes the 48 queries in chunks of
+    21; each query's rounds are its own, so the result is the same."""
+    idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
+    _check(idx, q, 100, np.full(q.shape[0], 64), "chunks", pair_budget=1024)
 
 
 def test_batched_uneven_probe_counts():
