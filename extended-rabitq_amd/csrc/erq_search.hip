@@ -86,6 +86,8 @@ struct WaveLds {
     int* sums;
     float* pd;
     uint32_t* pid;
+    float* nd;      // a block's re-estimated survivors (MFMA kernel), up to 32
+    uint32_t* nid;
 };
 
 __device__ __forceinline__ size_t wave_lds_bytes(int D, int k) {
@@ -130,6 +132,66 @@ __device__ __forceinline__ void pool_insert(const WaveLds& w, int& psize, int k,
     psize += (psize < k);
 }
 
+// ns successive pool_insert calls (entries nd[t], nid[t], t = 0 .. ns-1 in insertion
+// order, ns <= 32) as one merge.  pool_insert keeps the pool as the first min(k, size)
+// entries of all inserted ones in the order (distance ascending; of equal distances the
+// later-inserted first): a new entry lands after the pool entries strictly below it, and a
+// full pool drops its last entry.  So new entry t ends at (pool entries with a distance
+// below its own) + (new entries before it in that order), and pool entry i at
+// i + (new entries with a distance <= its own).  nd is overwritten.
+__device__ __forceinline__ void pool_insert_bulk(const WaveLds& w, int& psize, int k, int lane,
+                                                 float* nd, const uint32_t* nid, int ns) {
+    float dt = 0.f;
+    uint32_t it = 0;
+    int rt = 0, at = k;
+    if (lane < ns) {
+        dt = nd[lane];
+        it = nid[lane];
+        for (int u = 0; u < ns; u++) {
+            const float du = nd[u];
+            rt += (du < dt || (du == dt && u > lane)) ? 1 : 0;
+        }
+        int lo = 0, hi = psize;  // pool entries with pd < dt
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (w.pd[mid] < dt) lo = mid + 1; else hi = mid;
+        }
+        at = lo + rt;
+    }
+    wave_sync();
+    if (lane < ns) nd[rt] = dt;  // the new distances, ascending
+    wave_sync();
+    // pool entries move up, highest 64 first; each lane reads before any lane writes
+    for (int base = (psize - 1) & ~(kWave - 1); psize > 0 && base >= 0; base -= kWave) {
+        const int i = base + lane;
+        float vd = 0.f;
+        uint32_t vid = 0;
+        int dst = k;
+        if (i < psize) {
+            vd = w.pd[i];
+            vid = w.pid[i];
+            int lo = 0, hi = ns;  // new entries with a distance <= vd
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (nd[mid] <= vd) lo = mid + 1; else hi = mid;
+            }
+            dst = i + lo;
+        }
+        wave_sync();
+        if (dst < k) {
+            w.pd[dst] = vd;
+            w.pid[dst] = vid;
+        }
+        wave_sync();
+    }
+    if (at < k) {
+        w.pd[at] = dt;
+        w.pid[at] = it;
+    }
+    wave_sync();
+    psize = min(k, psize + ns);
+}
+
 template <int EX, bool WRAP>
 __global__ void __launch_bounds__(256) scan_kernel(ScanArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -143,6 +205,8 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs a, int per_wave_byte
     WaveLds w;
     w.uq = reinterpret_cast<float*>(base);
     w.lut = reinterpret_cast<uint16_t*>(base + 4 * D);
+    w.nd = nullptr;
+    w.nid = nullptr;
     w.sums = reinterpret_cast<int*>(base + 12 * D);
     w.pd = reinterpret_cast<float*>(base + 12 * D + 128);
     w.pid = reinterpret_cast<uint32_t*>(w.pd + kcap);
@@ -390,6 +454,8 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
     w.sums = sums;
     w.pd = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(sums) + 128);
     w.pid = reinterpret_cast<uint32_t*>(w.pd + kcap);
+    w.nd = reinterpret_cast<float*>(w.pid + kcap);
+    w.nid = reinterpret_cast<uint32_t*>(w.nd + 32);
     for (int i = lane; i < kcap; i += kWave) {
         w.pd[i] = FLT_MAX;
         w.pid[i] = 0;
@@ -508,14 +574,21 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
             uint64_t mask = ballot(cand);
             st_blocks++;
             st_vec += np;
-            while (mask) {
+            // the block's survivors, four at a time (group g takes the g-th remaining set
+            // bit), re-estimated into the new-entry list in bit order, then inserted at once
+            const int ns = __popcll(mask);
+            for (int s0 = 0; s0 < ns; s0 += 4) {
                 uint64_t mm = mask;
                 for (int r = 0; r < g; r++) mm &= mm - 1;
                 const int j = mm ? (__ffsll((unsigned long long)mm) - 1) : -1;
                 const int jj = j < 0 ? 0 : j;
                 const uint32_t vi = v0 + (uint32_t)(b * 32 + jj);
+                float exf = 0.f;
+                uint32_t idv = 0;
                 float part = 0.f;
                 if (j >= 0) {
+                    exf = a.ex_factor[vi];
+                    idv = a.ids[vi];
                     const float* ur = uq + l16 * S;
                     part = ex_chain<EX>([&](int i) { return ur[i]; },
                                         a.exl + ((size_t)vi * 16 + l16) * a.exl_w, D >> 4, a.exl_w);
@@ -523,28 +596,24 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
                 const float ip = butterfly16(part);
                 const float ipq_j = __shfl(ipq_l, jj);
                 const float on_j = __shfl(on_l, jj);
-                float exd = 0.f;
-                uint32_t idv = 0;
-                if (j >= 0) {
+                if (j >= 0 && l16 == 0) {
                     const float on = on_j;
                     const float sqr_x = on * on;
                     const float Aa = sqr_x + sqr_y;
-                    const float Bb = a.ex_factor[vi] * y;
+                    const float Bb = exf * y;
                     const float Cc = (float)R * ipq_j + ip;
                     const double G = (double)Cc - ((double)R - 0.5) * (double)sumq;
                     const double H = (double)Bb * G;
-                    exd = (float)((double)Aa - H);
-                    idv = a.ids[vi];
+                    w.nd[s0 + g] = (float)((double)Aa - H);
+                    w.nid[s0 + g] = idv;
                 }
-                const int cnt = min(4, __popcll(mask));
-                for (int gg = 0; gg < cnt; gg++) {
-                    const float dd = __shfl(exd, 16 * gg);
-                    const uint32_t ii = (uint32_t)__shfl((int)idv, 16 * gg);
-                    pool_insert(w, psize, k, lane, dd, ii);
-                }
-                for (int r = 0; r < cnt; r++) mask &= mask - 1;
-                st_surv += cnt;
+                for (int r = 0; r < 4; r++) mask &= mask - 1;
             }
+            if (ns) {
+                wave_sync();
+                pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
+            }
+            st_surv += ns;
             distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
         }
     }
@@ -567,7 +636,7 @@ hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
     int waves = a.waves_per_block > 0 ? a.waves_per_block : 1;
     if (a.bp && a.D <= 2048) {
         const size_t per_wave = 4 * 16 * (size_t)ul_stride(a.D) + 2048 * (size_t)a.nchunk + 128 +
-                                (size_t)8 * ((a.k + 63) & ~63);
+                                (size_t)8 * ((a.k + 63) & ~63) + 256;
         while (waves > 1 && per_wave * waves > 65536) waves--;
         dim3 grid((a.nq + waves - 1) / waves), block(64 * waves);
         switch (a.nchunk) {
