@@ -654,7 +654,20 @@ __global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
     const int q = blockIdx.x, tid = threadIdx.x;
     const uint32_t raw = cand_cnt[q];
     const int nc = (int)min(raw, (uint32_t)cap);
-    if (nc == 0) return;
+    if (nc == 0) {
+        // no candidates: the pool (sorted by distance) is already in (distance, id) order
+        // unless two entries share a distance (a pool from the per-query replay puts the
+        // later-inserted of equal distances first); then it is re-sorted like any merge
+        if (tid == 0) sh_cnt = 0;
+        __syncthreads();
+        for (int i = tid; i + 1 < k; i += 256) {
+            const float a = pool_d[(size_t)q * k + i];
+            if (a < FLT_MAX && a == pool_d[(size_t)q * k + i + 1]) sh_cnt = 1;
+        }
+        __syncthreads();
+        if (sh_cnt == 0) return;
+        __syncthreads();
+    }
     const int n = k + nc;
     // radix select of the k-th smallest distance key
     uint32_t prefix = 0, pmask = 0;

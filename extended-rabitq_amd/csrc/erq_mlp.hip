@@ -41,6 +41,12 @@ struct erq_mlp {
     float* tmp = nullptr;     // B x hid fp32 scratch
     float* loss = nullptr;    // 1
     float* part = nullptr;    // partial sums of the loss and of the bias gradients
+    // inference activations for batches above max_batch (inf_rows rows, made on first use)
+    size_t inf_rows = 0;
+    __bf16* ixb = nullptr;
+    float* iz = nullptr;      // pre-activations of both hidden layers (not kept)
+    __bf16* ih1 = nullptr;
+    __bf16* ih2 = nullptr;
     std::mutex mu;
 };
 
@@ -51,6 +57,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int GBM = 64, GBN = 64, GBK = 32, GPAD = 8;
+constexpr size_t kInferRows = 32768;  // rows per inference chunk above max_batch
 
 enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_F32 = 2, EPI_ACC_F32 = 3 };
 
@@ -176,6 +183,12 @@ __global__ void sigmoid_kernel(const float* __restrict__ z, float* __restrict__ 
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = 1.0f / (1.0f + expf(-z[i]));
+}
+
+__global__ void sigmoid_inplace_kernel(float* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 1.0f / (1.0f + expf(-p[i]));
 }
 
 // BCE with logits, positive weight w: l = w*y*softplus(-z) + (1-y)*softplus(z);
@@ -306,22 +319,37 @@ hipError_t refresh_bf16(erq_mlp* m, hipStream_t s) {
     return hipGetLastError();
 }
 
-// forward on batch B (B <= max_batch) from fp32 inputs; leaves z1,h1,z2,h2,logit
-hipError_t forward(erq_mlp* m, const float* x, int B, hipStream_t s) {
+// forward on B rows from fp32 inputs into the given activation buffers (B rows each);
+// logits (B x out, fp32) into logit
+struct Acts {
+    __bf16* xb;
+    float* z1;
+    __bf16* h1;
+    float* z2;
+    __bf16* h2;
+    float* logit;
+};
+
+hipError_t forward_into(erq_mlp* m, const Acts& A, const float* x, int B, hipStream_t s) {
     Layout L = layout(m);
     const size_t n1 = (size_t)m->hid * m->in, n2 = (size_t)m->hid * m->hid;
     const __bf16 *W1 = m->wb, *W2 = m->wb + n1, *W3 = m->wb + n1 + n2;
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(grid_for((size_t)B * m->in)), dim3(256), 0, s, x,
-                       m->xb, (size_t)B * m->in, m->xscale);
-    hipError_t e = gemm<0, 1, EPI_BIAS_GELU>(m->xb, m->in, W1, m->in, B, m->hid, m->in, m->z1,
-                                             m->hid, m->h1, m->master + L.b1, s);
+                       A.xb, (size_t)B * m->in, m->xscale);
+    hipError_t e = gemm<0, 1, EPI_BIAS_GELU>(A.xb, m->in, W1, m->in, B, m->hid, m->in, A.z1,
+                                             m->hid, A.h1, m->master + L.b1, s);
     if (e == hipSuccess)
-        e = gemm<0, 1, EPI_BIAS_GELU>(m->h1, m->hid, W2, m->hid, B, m->hid, m->hid, m->z2, m->hid,
-                                      m->h2, m->master + L.b2, s);
+        e = gemm<0, 1, EPI_BIAS_GELU>(A.h1, m->hid, W2, m->hid, B, m->hid, m->hid, A.z2, m->hid,
+                                      A.h2, m->master + L.b2, s);
     if (e == hipSuccess)
-        e = gemm<0, 1, EPI_BIAS_F32>(m->h2, m->hid, W3, m->hid, B, m->out, m->hid, m->logit,
+        e = gemm<0, 1, EPI_BIAS_F32>(A.h2, m->hid, W3, m->hid, B, m->out, m->hid, A.logit,
                                      m->out, nullptr, m->master + L.b3, s);
     return e;
+}
+
+// forward on batch B (B <= max_batch) from fp32 inputs; leaves z1,h1,z2,h2,logit
+hipError_t forward(erq_mlp* m, const float* x, int B, hipStream_t s) {
+    return forward_into(m, Acts{m->xb, m->z1, m->h1, m->z2, m->h2, m->logit}, x, B, s);
 }
 
 // backward after forward with labels y (fp32 0/1): grads into m->grad, loss into m->loss
@@ -378,7 +406,7 @@ void mlp_release(erq_mlp* m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     void* ptrs[] = {m->master, m->grad, m->adam_m, m->adam_v, m->wb, m->xb, m->z1, m->h1,
                     m->z2, m->h2, m->logit, m->dlog, m->dz2, m->dz1, m->tmp, m->loss,
-                    m->part};
+                    m->part, m->ixb, m->iz, m->ih1, m->ih2};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -517,13 +545,42 @@ int erq_mlp_forward_dev(erq_mlp* m, const float* x_dev, size_t batch, float* pro
     }
     std::lock_guard<std::mutex> lk(m->mu);
     hipStream_t s = mlp_stream(m, stream);
-    for (size_t b0 = 0; b0 < batch; b0 += m->max_batch) {
-        int B = (int)std::min<size_t>(m->max_batch, batch - b0);
-        hipError_t e = forward(m, x_dev + b0 * m->in, B, s);
+    if (batch <= (size_t)m->max_batch) {
+        if (batch == 0) return ERQ_OK;
+        hipError_t e = forward(m, x_dev, (int)batch, s);
         if (e != hipSuccess) return hip_fail(e, "mlp forward");
-        size_t n = (size_t)B * m->out;
-        hipLaunchKernelGGL(sigmoid_kernel, dim3(grid_for(n)), dim3(256), 0, s, m->logit,
-                           prob_dev + b0 * m->out, n);
+        const size_t n = batch * m->out;
+        hipLaunchKernelGGL(sigmoid_kernel, dim3(grid_for(n)), dim3(256), 0, s, m->logit, prob_dev,
+                           n);
+        e = hipGetLastError();
+        return e == hipSuccess ? ERQ_OK : hip_fail(e, "mlp sigmoid");
+    }
+    // large batches: chunks of up to kInferRows rows through the inference buffers, the
+    // logits written straight into prob_dev and turned into probabilities there
+    const size_t rows = std::min<size_t>(batch, kInferRows);
+    if (m->inf_rows < rows) {
+        void* old[] = {m->ixb, m->iz, m->ih1, m->ih2};
+        for (void* p : old)
+            if (p) (void)hipFree(p);
+        m->ixb = nullptr;
+        m->iz = nullptr;
+        m->ih1 = m->ih2 = nullptr;
+        m->inf_rows = 0;
+        hipError_t e = alloc(&m->ixb, rows * m->in);
+        if (e == hipSuccess) e = alloc(&m->iz, rows * m->hid);
+        if (e == hipSuccess) e = alloc(&m->ih1, rows * m->hid);
+        if (e == hipSuccess) e = alloc(&m->ih2, rows * m->hid);
+        if (e != hipSuccess) return hip_fail(e, "mlp inference buffers");
+        m->inf_rows = rows;
+    }
+    for (size_t b0 = 0; b0 < batch; b0 += rows) {
+        const int B = (int)std::min<size_t>(rows, batch - b0);
+        float* pr = prob_dev + b0 * m->out;
+        hipError_t e = forward_into(m, Acts{m->ixb, m->iz, m->ih1, m->iz, m->ih2, pr},
+                                    x_dev + b0 * m->in, B, s);
+        if (e != hipSuccess) return hip_fail(e, "mlp forward");
+        const size_t n = (size_t)B * m->out;
+        hipLaunchKernelGGL(sigmoid_inplace_kernel, dim3(grid_for(n)), dim3(256), 0, s, pr, n);
         e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "mlp sigmoid");
     }

@@ -100,6 +100,22 @@ def test_batched_query_chunks():
     _check(idx, q, 100, np.full(q.shape[0], 64), "chunks", pair_budget=1024)
 
 
+def test_batched_duplicate_vectors():
+    """Every vector 24 times over (ties everywhere), rounds past the exact ranks."""
+    from erq import data, index
+
+    def make():
+        base, cen = data.mixture_isotropic(300, 128, 8, 0.3, seed=31)
+        x = np.repeat(base, 24, axis=0)
+        x = x[np.random.default_rng(32).permutation(x.shape[0])]
+        c, assign = data.kmeans(x, 8, 4, seed=33)
+        idx = index.build_index(x, c, assign, bits=4, seed=34)
+        q, _ = data.mixture_isotropic(24, 128, 8, 0.3, seed=35, centres=cen)
+        return idx, q, x
+    idx, q, _ = build_cached(("batched", "dup"), make)
+    _check(idx, q, 100, np.full(q.shape[0], 8), "duplicates")
+
+
 def test_batched_uneven_probe_counts():
     idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
     rng = np.random.default_rng(5)

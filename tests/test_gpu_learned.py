@@ -59,6 +59,11 @@ def test_mlp_non_multiple_batch_forward_and_chunking():
     _, _, p_ref = mo.grad_bf16(theta.astype(np.float64), x, np.zeros((100, dout)), din, hid,
                                dout, 1.0)
     np.testing.assert_allclose(p, p_ref, rtol=2e-3, atol=1e-5)
+    # batches above max_batch run through the inference buffers; every row is computed
+    # as in a batch of at most max_batch rows
+    xt = torch.from_numpy(x).cuda()
+    parts = [m.forward_dev(xt[i:i + 32]) for i in range(0, 100, 32)]
+    assert torch.equal(torch.cat(parts), torch.from_numpy(p).cuda())
     m.close()
 
 

@@ -152,6 +152,20 @@ def test_tiny_and_ragged_clusters():
     _assert_parity(idx, q, 300, len(sizes), "ragged k>N")
 
 
+def test_duplicate_vectors_tie_order():
+    """Every vector 24 times over: equal codes give equal distances, so the pool's order
+    among ties (a new entry before equal older ones) decides which ids are kept."""
+    from erq import data, index
+    base, cen = data.mixture_isotropic(300, 128, 6, 0.3, seed=21)
+    x = np.repeat(base, 24, axis=0)
+    perm = np.random.default_rng(22).permutation(x.shape[0])
+    x = x[perm]
+    c, assign = data.kmeans(x, 6, 4, seed=23)
+    idx = index.build_index(x, c, assign, bits=4, seed=24)
+    q, _ = data.mixture_isotropic(24, 128, 6, 0.3, seed=25, centres=cen)
+    _assert_parity(idx, q, 100, 3, "duplicates")
+
+
 def test_host_pointer_search_matches_batch():
     """erq_search (IVF::search drop-in, host pointers) equals the device batch path."""
     from erq.gpu import IVF
