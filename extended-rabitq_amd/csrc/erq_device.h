@@ -249,6 +249,49 @@ __device__ __forceinline__ float ex_chain(const UF& uf, const uint32_t* __restri
     return part;
 }
 
+// ex_chain for two vectors at once (codes ca, cb; cb unused when !has_b): their loads are
+// issued together and each u value feeds both chains, each chain in the order of ex_chain.
+template <int EX, typename UF>
+__device__ __forceinline__ void ex_chain2(const UF& uf, const uint32_t* __restrict__ ca,
+                                          const uint32_t* __restrict__ cb, bool has_b,
+                                          int nvals, int W, float& pa, float& pb) {
+    constexpr int vpd = 32 / EX;
+    constexpr uint32_t mask = (1u << EX) - 1u;
+    pa = 0.f;
+    pb = 0.f;
+    const uint4* wa = reinterpret_cast<const uint4*>(ca);
+    const uint4* wb = reinterpret_cast<const uint4*>(cb);
+    const int n4 = W / 4;
+    for (int b4 = 0; b4 < n4; b4 += 4) {
+        uint4 xa[4], xb[4];
+#pragma unroll
+        for (int j4 = 0; j4 < 4; j4++) {
+            if (b4 + j4 < n4) {
+                xa[j4] = wa[b4 + j4];
+                xb[j4] = has_b ? wb[b4 + j4] : make_uint4(0, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j4 = 0; j4 < 4; j4++) {
+            if (b4 + j4 >= n4) break;
+            const uint32_t sa[4] = {xa[j4].x, xa[j4].y, xa[j4].z, xa[j4].w};
+            const uint32_t sb[4] = {xb[j4].x, xb[j4].y, xb[j4].z, xb[j4].w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int i0 = (4 * (b4 + j4) + j) * vpd;
+#pragma unroll
+                for (int t = 0; t < vpd; t++) {
+                    if (i0 + t < nvals) {
+                        const float u = uf(i0 + t);
+                        pa = fmaf(u, (float)((sa[j] >> (t * EX)) & mask), pa);
+                        pb = fmaf(u, (float)((sb[j] >> (t * EX)) & mask), pb);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Float key with the same order as the float (for radix selection / sorting).
 __device__ __forceinline__ uint32_t float_key(float f) {
     uint32_t u = __float_as_uint(f);

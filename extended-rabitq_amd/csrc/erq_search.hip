@@ -428,8 +428,24 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs a, int per_wave_byte
 // and pool are the per-query kernel's.
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+// The re-estimated distance of a survivor in the reference's operation order
+// (HASearcher.hpp:215-220, the double promotion as C performs it): on = |o| of the vector,
+// Bb = ex_factor * y, ipq = scaled FastScan product, ip = <u, ex code>.
+template <int EX>
+__device__ __forceinline__ float ex_distance(float on, float sqr_y, float Bb, float ipq, float ip,
+                                             float sumq) {
+    constexpr int R = 1 << EX;
+    const float sqr_x = on * on;
+    const float Aa = sqr_x + sqr_y;
+    const float Cc = (float)R * ipq + ip;
+    const double G = (double)Cc - ((double)R - 0.5) * (double)sumq;
+    const double H = (double)Bb * G;
+    return (float)((double)Aa - H);
+}
+
 template <int EX, int NCH>
-__global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -574,40 +590,50 @@ __global__ void __launch_bounds__(256) scan_mfma_kernel(ScanArgs a, int per_wave
             uint64_t mask = ballot(cand);
             st_blocks++;
             st_vec += np;
-            // the block's survivors, four at a time (group g takes the g-th remaining set
-            // bit), re-estimated into the new-entry list in bit order, then inserted at once
+            // the block's survivors, eight at a time (group g takes the g-th and (g + 4)-th
+            // remaining set bits), re-estimated into the new-entry list in bit order, then
+            // inserted at once
             const int ns = __popcll(mask);
-            for (int s0 = 0; s0 < ns; s0 += 4) {
+            for (int s0 = 0; s0 < ns; s0 += 8) {
                 uint64_t mm = mask;
                 for (int r = 0; r < g; r++) mm &= mm - 1;
                 const int j = mm ? (__ffsll((unsigned long long)mm) - 1) : -1;
-                const int jj = j < 0 ? 0 : j;
+                for (int r = 0; r < 4; r++) mm &= mm - 1;
+                const int j2 = mm ? (__ffsll((unsigned long long)mm) - 1) : -1;
+                const int jj = j < 0 ? 0 : j, jj2 = j2 < 0 ? 0 : j2;
                 const uint32_t vi = v0 + (uint32_t)(b * 32 + jj);
-                float exf = 0.f;
-                uint32_t idv = 0;
-                float part = 0.f;
+                const uint32_t vi2 = v0 + (uint32_t)(b * 32 + jj2);
+                float exf = 0.f, exf2 = 0.f;
+                uint32_t idv = 0, idv2 = 0;
+                float part = 0.f, part2 = 0.f;
                 if (j >= 0) {
                     exf = a.ex_factor[vi];
                     idv = a.ids[vi];
+                    if (j2 >= 0) {
+                        exf2 = a.ex_factor[vi2];
+                        idv2 = a.ids[vi2];
+                    }
                     const float* ur = uq + l16 * S;
-                    part = ex_chain<EX>([&](int i) { return ur[i]; },
-                                        a.exl + ((size_t)vi * 16 + l16) * a.exl_w, D >> 4, a.exl_w);
+                    ex_chain2<EX>([&](int i) { return ur[i]; },
+                                  a.exl + ((size_t)vi * 16 + l16) * a.exl_w,
+                                  a.exl + ((size_t)vi2 * 16 + l16) * a.exl_w, j2 >= 0, D >> 4,
+                                  a.exl_w, part, part2);
                 }
-                const float ip = butterfly16(part);
-                const float ipq_j = __shfl(ipq_l, jj);
-                const float on_j = __shfl(on_l, jj);
-                if (j >= 0 && l16 == 0) {
-                    const float on = on_j;
-                    const float sqr_x = on * on;
-                    const float Aa = sqr_x + sqr_y;
-                    const float Bb = exf * y;
-                    const float Cc = (float)R * ipq_j + ip;
-                    const double G = (double)Cc - ((double)R - 0.5) * (double)sumq;
-                    const double H = (double)Bb * G;
-                    w.nd[s0 + g] = (float)((double)Aa - H);
-                    w.nid[s0 + g] = idv;
+                const float ip = butterfly16(part), ip2 = butterfly16(part2);
+                const float ipq_j = __shfl(ipq_l, jj), ipq_j2 = __shfl(ipq_l, jj2);
+                const float on_j = __shfl(on_l, jj), on_j2 = __shfl(on_l, jj2);
+                if (l16 == 0) {
+                    if (j >= 0) {
+                        w.nd[s0 + g] = ex_distance<EX>(on_j, sqr_y, exf * y, ipq_j, ip, sumq);
+                        w.nid[s0 + g] = idv;
+                    }
+                    if (j2 >= 0) {
+                        w.nd[s0 + 4 + g] =
+                            ex_distance<EX>(on_j2, sqr_y, exf2 * y, ipq_j2, ip2, sumq);
+                        w.nid[s0 + 4 + g] = idv2;
+                    }
                 }
-                for (int r = 0; r < 4; r++) mask &= mask - 1;
+                for (int r = 0; r < 8; r++) mask &= mask - 1;
             }
             if (ns) {
                 wave_sync();
