@@ -20,9 +20,9 @@ struct BatchArgs {
     const uint32_t* csize;
     const uint32_t* bstart;
     const uint32_t* vstart;
-    const uint32_t* items;     // 3 x n_items: cluster, (first entry << 4) | count, first tile
+    const uint32_t* items;     // 3 words per item: cluster, (first entry << 4) | count, first tile
     const uint32_t* entries;   // (probe slot, query) per entry, grouped by cluster
-    uint32_t* n_items;         // [0] items of this round, [2] entries, [3] queued items
+    uint32_t* n_items;         // [0] items of this round, [1] its first item, [2] entries, [3] queued
     uint32_t* ovf;             // items queued for the inline pass (slice full)
     const uint32_t* probe_ids; // per slot: cluster id
     int8_t* adig;              // per entry: 2 * Dp int8 digits in MFMA operand order
@@ -60,7 +60,18 @@ hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s);
 // exl (N x 16 x exl_words(D, EX) u32, erq_device.h) from long_code.
 hipError_t launch_exl_relayout(const uint8_t* long_code, size_t N, int D, int EX, uint32_t* exl,
                                hipStream_t s);
-hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1, int grid,
-                        hipStream_t s);
+constexpr int kFirstRoundRank = 4;  // ranks below it are the exact per-query replay's
+constexpr int kMaxRounds = 32;
+// The work lists of every round past the exact ranks (ranks [4, r_end), R rounds:
+// [4,16), [16,64), [64,192), then 128 wide) and the per-entry preparation (digits and
+// parameters, no threshold), which need only the probe lists.  ba.items / entries / adig /
+// qparams cover all rounds; ba.n_items = rinfo, 4 (R + 1) words (round r at 4 r: items,
+// first item, entries, queued items; totals at 4 R); rb.counts / cursor hold R x K bins.
+// prep_grid: workgroups (4 waves each, a multiple of 8) of the preparation kernel.
+hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int r_end,
+                           int prep_grid, hipStream_t s);
+// One round's scan, re-estimate and merge (ba.n_items = rinfo + 4 r), with each query's
+// threshold read from pool_d (its k-th distance when the round starts).
+hipError_t launch_round_scan(const BatchArgs& ba, const RoundBufs& rb, int grid, hipStream_t s);
 
 }  // namespace erq

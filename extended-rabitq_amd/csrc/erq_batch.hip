@@ -106,60 +106,98 @@ __global__ void zero_u32_kernel(uint32_t* p, size_t n) {
         p[i] = 0;
 }
 
-__device__ __forceinline__ void round_range(const uint32_t* off, int q, int r0, int r1,
-                                            uint32_t& a, uint32_t& b) {
-    uint32_t s = off[q], e = off[q + 1];
-    a = min(e, s + (uint32_t)r0);
-    b = min(e, s + (uint32_t)r1);
+// The round of a probe rank past the exact ones: [4,16), [16,64), [64,192), then 128 wide
+// (the host's round schedule).
+__device__ __forceinline__ uint32_t round_of_rank(uint32_t rank) {
+    if (rank < 16) return 0;
+    if (rank < 64) return 1;
+    if (rank < 192) return 2;
+    return 3 + (rank - 192) / 128;
 }
 
+// Entries of all rounds at once, binned by (round, cluster): bin = round * K + cluster.
 __global__ void count_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pid,
-                             int nq, int r0, int r1, int K, uint32_t* __restrict__ counts) {
+                             int nq, int r_first, int r_end, int K,
+                             uint32_t* __restrict__ counts) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
-    uint32_t a, b;
-    round_range(off, q, r0, r1, a, b);
-    for (uint32_t p = a; p < b; p++) {
-        uint32_t c = pid[p];
-        if (c < (uint32_t)K) atomicAdd(&counts[c], 1u);
+    const uint32_t s = off[q], e = min(off[q + 1], s + (uint32_t)r_end);
+    for (uint32_t p = s + (uint32_t)r_first; p < e; p++) {
+        const uint32_t c = pid[p];
+        if (c < (uint32_t)K) atomicAdd(&counts[round_of_rank(p - s) * K + c], 1u);
     }
 }
 
-// Single workgroup: exclusive scans of counts and of items per cluster, then items.
+// Single workgroup: exclusive scans of the bin counts and of items per bin, then the items.
 // A work item is (cluster, up to kGroup consecutive entries, up to kItemTiles tiles of the
-// cluster), three words: cluster, (first entry << 4) | entry count, first tile.
+// cluster), three words: cluster, (first entry << 4) | entry count, first tile.  Items and
+// entries are in (round, cluster) order; rinfo[4r + 0..3] = the round's items, first item,
+// entries, items queued for the inline pass (0); rinfo[4R + 0], [4R + 2] = the totals.
 __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __restrict__ counts,
                                                           const uint32_t* __restrict__ csize,
-                                                          int K, uint32_t* __restrict__ cursor,
+                                                          int K, int R,
+                                                          uint32_t* __restrict__ cursor,
                                                           uint32_t* __restrict__ items,
-                                                          uint32_t* __restrict__ n_items) {
+                                                          uint32_t* __restrict__ rinfo) {
     __shared__ uint32_t s_cnt[1024], s_it[1024], s_ns[1024];
     __shared__ uint32_t carry_c, carry_i;
-    const int t = threadIdx.x;
+    __shared__ uint32_t r_item[kMaxRounds + 1], r_ent[kMaxRounds + 1];
+    __shared__ uint32_t w_c[16], w_i[16];
+    const int t = threadIdx.x, nb = K * R, lane = t & 63, wv = t >> 6;
     if (t == 0) carry_c = carry_i = 0;
     __syncthreads();
-    for (int base = 0; base < K; base += 1024) {
-        const int c = base + t;
-        const uint32_t cnt = c < K ? counts[c] : 0;
-        const uint32_t ntile = c < K ? (csize[c] + 15) / 16 : 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int bin = base + t;
+        const int c = bin % K;
+        const uint32_t cnt = bin < nb ? counts[bin] : 0;
+        const uint32_t ntile = bin < nb ? (csize[c] + 15) / 16 : 0;
         const uint32_t nsplit = (ntile + kItemTiles - 1) / kItemTiles;  // 0: empty cluster
         const uint32_t it = (cnt + kGroup - 1) / kGroup * nsplit;
-        s_cnt[t] = cnt;
-        s_it[t] = it;
+        // inclusive scans: within each wave by shuffles, then over the 16 wave totals
+        uint32_t ic = cnt, ii = it;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t a = __shfl_up(ic, o), b = __shfl_up(ii, o);
+            if (lane >= o) {
+                ic += a;
+                ii += b;
+            }
+        }
+        if (lane == 63) {
+            w_c[wv] = ic;
+            w_i[wv] = ii;
+        }
         s_ns[t] = nsplit;
         __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scans
-            uint32_t a = t >= o ? s_cnt[t - o] : 0, b = t >= o ? s_it[t - o] : 0;
-            __syncthreads();
-            s_cnt[t] += a;
-            s_it[t] += b;
-            __syncthreads();
+        if (wv == 0) {
+            uint32_t vc = lane < 16 ? w_c[lane] : 0, vi = lane < 16 ? w_i[lane] : 0;
+            const uint32_t oc = vc, oi = vi;
+            for (int o = 1; o < 16; o <<= 1) {
+                const uint32_t a = __shfl_up(vc, o), b = __shfl_up(vi, o);
+                if (lane >= o) {
+                    vc += a;
+                    vi += b;
+                }
+            }
+            if (lane < 16) {  // exclusive prefix of the wave totals
+                w_c[lane] = vc - oc;
+                w_i[lane] = vi - oi;
+            }
         }
-        if (c < K) cursor[c] = carry_c + s_cnt[t] - cnt;
-        // items of this chunk of clusters, one thread per item
+        __syncthreads();
+        s_cnt[t] = ic + w_c[wv];
+        s_it[t] = ii + w_i[wv];
+        __syncthreads();
+        if (bin < nb) {
+            cursor[bin] = carry_c + s_cnt[t] - cnt;
+            if (c == 0) {  // the first bin of round bin / K
+                r_item[bin / K] = carry_i + s_it[t] - it;
+                r_ent[bin / K] = carry_c + s_cnt[t] - cnt;
+            }
+        }
+        // items of this chunk of bins, one thread per item
         const uint32_t nit = s_it[1023];
         for (uint32_t i = t; i < nit; i += 1024) {
-            int lo = 0, hi = 1023;  // first cluster slot whose inclusive item count exceeds i
+            int lo = 0, hi = 1023;  // first bin slot whose inclusive item count exceeds i
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
                 if (s_it[mid] > i) hi = mid; else lo = mid + 1;
@@ -168,7 +206,7 @@ __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __rest
             const uint32_t ccnt = s_cnt[lo] - before_c, ns = s_ns[lo];
             const uint32_t j = i - before_it, grp = j / ns, sp = j % ns;
             const uint32_t w = carry_i + i;
-            items[3 * w] = (uint32_t)(base + lo);
+            items[3 * w] = (uint32_t)((base + lo) % K);
             items[3 * w + 1] = ((carry_c + before_c + grp * kGroup) << 4) |
                                min((uint32_t)kGroup, ccnt - grp * kGroup);
             items[3 * w + 2] = sp * kItemTiles;
@@ -181,23 +219,28 @@ __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __rest
         __syncthreads();
     }
     if (t == 0) {
-        n_items[0] = carry_i;
-        n_items[2] = carry_c;  // entries (query, cluster pairs) of the round
-        n_items[3] = 0;        // items queued for the inline pass
+        r_item[R] = carry_i;
+        r_ent[R] = carry_c;
+    }
+    __syncthreads();
+    if (t <= R) {
+        rinfo[4 * t + 0] = t < R ? r_item[t + 1] - r_item[t] : carry_i;
+        rinfo[4 * t + 1] = t < R ? r_item[t] : 0;
+        rinfo[4 * t + 2] = t < R ? r_ent[t + 1] - r_ent[t] : carry_c;
+        rinfo[4 * t + 3] = 0;
     }
 }
 
 __global__ void scatter_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pid,
-                               int nq, int r0, int r1, int K, uint32_t* __restrict__ cursor,
-                               uint32_t* __restrict__ entries) {
+                               int nq, int r_first, int r_end, int K,
+                               uint32_t* __restrict__ cursor, uint32_t* __restrict__ entries) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
-    uint32_t a, b;
-    round_range(off, q, r0, r1, a, b);
-    for (uint32_t p = a; p < b; p++) {
-        uint32_t c = pid[p];
+    const uint32_t s = off[q], e = min(off[q + 1], s + (uint32_t)r_end);
+    for (uint32_t p = s + (uint32_t)r_first; p < e; p++) {
+        const uint32_t c = pid[p];
         if (c >= (uint32_t)K) continue;
-        uint32_t pos = atomicAdd(&cursor[c], 1u);
+        const uint32_t pos = atomicAdd(&cursor[round_of_rank(p - s) * K + c], 1u);
         reinterpret_cast<uint2*>(entries)[pos] = make_uint2(p, (uint32_t)q);
     }
 }
@@ -215,36 +258,38 @@ struct QParam {
 __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    // XCD-aware: workgroup b runs on XCD b % 8; entries are grouped by cluster, so giving
-    // each XCD a contiguous eighth of them keeps a centroid's reads in one L2.
-    uint32_t blk = blockIdx.x;
-    if ((gridDim.x & 7) == 0) blk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-    const uint32_t e = blk * (blockDim.x >> 6) + wid;
-    if (e >= a.n_items[2]) return;
     const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512, S = ul_stride(D);
     float* uq = reinterpret_cast<float*>(smem + (size_t)wid * per_wave_bytes);
-    const uint2 ent = reinterpret_cast<const uint2*>(a.entries)[e];
-    const uint32_t slot = ent.x;
-    const int q = (int)ent.y;
-    const uint32_t cid = a.probe_ids[slot];
-    QParam P;
-    P.q = q;
-    P.sqr_y = a.probe_sqr_y[slot];
-    P.y = sqrtf(P.sqr_y);
-    float sumq, delta;
-    prepare_residual(a.q_rot + (size_t)q * D, a.centroids + (size_t)cid * D, P.y, D, S, uq, lane,
-                     P.fac, sumq, delta);
-    // the digits go straight to the entry's operand rows in HBM
-    quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta,
-                    reinterpret_cast<uint32_t*>(a.adig + (size_t)e * 2 * Dp), lane);
-    if (lane == 0) {
-        P.sumq = sumq;
-        P.delta = delta;
-        P.c1 = (float)(0.5 * (double)sumq - (double)0.58f);
-        P.c2 = (-5.0f * P.y) * a.one_over_sqrtD;
-        P.qs = P.y * P.y;
-        P.thr = a.pool_d[(size_t)q * a.k + a.k - 1];
-        reinterpret_cast<QParam*>(a.qparams)[e] = P;
+    // A fixed grid (a multiple of 8 workgroups) walks the entries.  XCD-aware: workgroup b
+    // runs on XCD b % 8; entries are grouped by cluster, so giving each XCD a contiguous
+    // eighth of them keeps a centroid's reads in one L2.
+    const uint32_t n = a.n_items[2];
+    const uint32_t x = blockIdx.x & 7, nw = (gridDim.x >> 3) * 4;
+    const uint32_t lo = (uint32_t)((uint64_t)n * x / 8), hi = (uint32_t)((uint64_t)n * (x + 1) / 8);
+    for (uint32_t e = lo + (blockIdx.x >> 3) * 4 + wid; e < hi; e += nw) {
+        const uint2 ent = reinterpret_cast<const uint2*>(a.entries)[e];
+        const uint32_t slot = ent.x;
+        const int q = (int)ent.y;
+        const uint32_t cid = a.probe_ids[slot];
+        QParam P;
+        P.q = q;
+        P.sqr_y = a.probe_sqr_y[slot];
+        P.y = sqrtf(P.sqr_y);
+        float sumq, delta;
+        prepare_residual(a.q_rot + (size_t)q * D, a.centroids + (size_t)cid * D, P.y, D, S,
+                         uq, lane, P.fac, sumq, delta);
+        // the digits go straight to the entry's operand rows in HBM
+        quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta,
+                        reinterpret_cast<uint32_t*>(a.adig + (size_t)e * 2 * Dp), lane);
+        if (lane == 0) {
+            P.sumq = sumq;
+            P.delta = delta;
+            P.c1 = (float)(0.5 * (double)sumq - (double)0.58f);
+            P.c2 = (-5.0f * P.y) * a.one_over_sqrtD;
+            P.qs = P.y * P.y;
+            P.thr = 0.f;  // the round's threshold is read when the round runs
+            reinterpret_cast<QParam*>(a.qparams)[e] = P;
+        }
     }
 }
 
@@ -342,7 +387,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     }
     for (uint32_t it = it_first; it < it_hi; it += it_step) {
         __syncthreads();  // the previous item's LDS is no longer read
-        const uint32_t item = INLINE ? a.ovf[it] : it;
+        const uint32_t item = INLINE ? a.ovf[it] : a.n_items[1] + it;  // global index
         const uint32_t cid = a.items[3 * item];
         const uint32_t e0 = a.items[3 * item + 1] >> 4, elen = a.items[3 * item + 1] & 15u;
         const int tfirst = (int)a.items[3 * item + 2];
@@ -358,7 +403,11 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         }
         if (tid < kGroup) {
             s_cnt[tid] = 0;
-            if (tid < (int)elen) qp[tid] = reinterpret_cast<const QParam*>(a.qparams)[e0 + tid];
+            if (tid < (int)elen) {
+                QParam P = reinterpret_cast<const QParam*>(a.qparams)[e0 + tid];
+                P.thr = a.pool_d[(size_t)P.q * a.k + a.k - 1];  // k-th distance at round start
+                qp[tid] = P;
+            }
         }
         // stage the digits: 16-byte units (c, p, g, row); rows of missing queries are 0
         for (int u = tid; u < NCH * 512; u += kThreads) {
@@ -580,6 +629,7 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
         const bool act = lg < m;
         if (e0 != staged) {
             P = reinterpret_cast<const QParam*>(a.qparams)[e0];
+            P.thr = a.pool_d[(size_t)P.q * a.k + a.k - 1];
             const float* qv = a.q_rot + (size_t)P.q * D;
             const float* cen = a.centroids + (size_t)__shfl(rec.y, 0) * D;
             wave_sync();
@@ -871,38 +921,42 @@ static hipError_t launch_reest(const BatchArgs& ba, int scan_grid, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_round(const BatchArgs& ba, const RoundBufs& rb, int r0, int r1, int grid,
-                        hipStream_t s) {
+hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int r_end,
+                           int prep_grid, hipStream_t s) {
+    if (prep_grid < 8 || prep_grid % 8) return hipErrorInvalidValue;
     const int nq = ba.nq, K = rb.K;
-    hipError_t e = launch_zero(rb.counts, (size_t)K, s);
+    if (R < 1 || R > kMaxRounds) return hipErrorInvalidValue;
+    hipError_t e = launch_zero(rb.counts, (size_t)K * R, s);
     if (e != hipSuccess) return e;
     dim3 qg((nq + 255) / 256), qb(256);
-    hipLaunchKernelGGL(count_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq, r0, r1, K,
-                       rb.counts);
-    hipLaunchKernelGGL(scan_items_kernel, dim3(1), dim3(1024), 0, s, rb.counts, ba.csize, K,
+    hipLaunchKernelGGL(count_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
+                       (int)kFirstRoundRank, r_end, K, rb.counts);
+    hipLaunchKernelGGL(scan_items_kernel, dim3(1), dim3(1024), 0, s, rb.counts, ba.csize, K, R,
                        rb.cursor, (uint32_t*)ba.items, ba.n_items);
-    hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq, r0, r1, K,
-                       rb.cursor, (uint32_t*)ba.entries);
+    hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
+                       (int)kFirstRoundRank, r_end, K, rb.cursor, (uint32_t*)ba.entries);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    {
-        const int wpb = 4;
-        const size_t pw = prep_wave_bytes(ba.D);
-        const size_t max_pairs = (size_t)nq * (size_t)(r1 - r0);
-        dim3 pg((unsigned)((max_pairs + wpb - 1) / wpb)), pb(64 * wpb);
-        hipLaunchKernelGGL(prep_kernel, pg, pb, pw * wpb, s, ba, (int)pw);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
+    // every entry of every round at once: ba.n_items points at the totals (rinfo + 4 R)
+    BatchArgs all = ba;
+    all.n_items = ba.n_items + 4 * R;
+    const int wpb = 4;
+    const size_t pw = prep_wave_bytes(ba.D);
+    hipLaunchKernelGGL(prep_kernel, dim3((unsigned)prep_grid), dim3(64 * wpb), pw * wpb, s, all,
+                       (int)pw);
+    return hipGetLastError();
+}
+
+hipError_t launch_round_scan(const BatchArgs& ba, const RoundBufs& rb, int grid, hipStream_t s) {
     const size_t lds = batch_lds_bytes(ba.D);
-    e = launch_batch_scan(ba, grid, lds, s);
+    hipError_t e = launch_batch_scan(ba, grid, lds, s);
     if (e != hipSuccess) return e;
     e = launch_reest(ba, grid, s);
     if (e != hipSuccess) return e;
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(merge_kernel, dim3(nq), dim3(256), 0, s, ba.pool_d,
-                       rb.pool_i, ba.k, ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.stats);
+    hipLaunchKernelGGL(merge_kernel, dim3(ba.nq), dim3(256), 0, s, ba.pool_d, rb.pool_i, ba.k,
+                       ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.stats);
     return hipGetLastError();
 }
 

@@ -80,6 +80,12 @@ void release(erq_index* h) {
                     h->ws,         h->bp,        h->order, h->exl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (h->aux) {
+        (void)hipStreamSynchronize(h->aux);
+        (void)hipStreamDestroy(h->aux);
+    }
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -572,17 +578,10 @@ static int ensure_bp(erq_index* h, hipStream_t s) {
 
 // Round schedule of the batched search: ranks [0, kExactRanks) by the exact per-query
 // replay, then [4,16), [16,64), [64,192) and from there rounds of at most 128 ranks.
-constexpr size_t kExactRanks = 4;
+constexpr size_t kExactRanks = kFirstRoundRank;
 static size_t next_round(size_t r0) { return std::min(4 * r0, r0 + 128); }
 
 constexpr size_t kMaxBatchGrid = 4096;    // workgroups of the persistent batched kernel
-
-static size_t batch_chunk_nq(size_t nq, size_t max_nprobe, size_t budget) {
-    size_t width = 1;
-    for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0))
-        width = std::max(width, std::min(next_round(r0), max_nprobe) - r0);
-    return std::max<size_t>(1, std::min(nq, budget / width));
-}
 
 static size_t max_cluster_size(const erq_index* h) {
     uint64_t m = 0;
@@ -590,17 +589,69 @@ static size_t max_cluster_size(const erq_index* h) {
     return (size_t)m;
 }
 
+// Queries per chunk: every round of a chunk is prepared ahead, so the chunk's pairs over all
+// ranks past the exact ones must fit the pair budget.
+static size_t batch_chunk_nq(size_t nq, size_t max_nprobe, size_t budget) {
+    const size_t width = std::max<size_t>(1, max_nprobe - kExactRanks);
+    return std::max<size_t>(1, std::min(nq, budget / width));
+}
+
+// The batched search's workspace, in take order (sizing with a null base gives the bytes).
+struct BatchBufs {
+    size_t cq = 0, R = 0, r_end = 0;
+    uint32_t *counts, *cursor, *rinfo, *items, *ovf, *entries, *cand_cnt, *cand_i, *surv_count;
+    float* cand_d;
+    int8_t* adig;
+    char* qparams;
+    uint4* surv;
+};
+
+static void batch_layout(const erq_index* h, size_t nq, size_t max_nprobe, int cap, Ws& ws,
+                         BatchBufs& b) {
+    const size_t K = h->K, Dp = (size_t)bp_nchunk((int)h->D) * 512;
+    const size_t maxc = max_cluster_size(h);
+    b.cq = batch_chunk_nq(nq, max_nprobe, h->batch_pair_budget);
+    b.R = 0;
+    size_t pairs = 0, items = 0, max_items = 0;
+    for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0)) {
+        const size_t r1 = std::min(next_round(r0), max_nprobe), pr = b.cq * (r1 - r0) + 8;
+        const size_t it = batch_items_bound(pr, K, h->N, maxc);
+        pairs += pr;
+        items += it;
+        max_items = std::max(max_items, it);
+        b.R++;
+        b.r_end = r1;
+    }
+    b.counts = ws.take<uint32_t>(b.R * K);
+    b.cursor = ws.take<uint32_t>(b.R * K);
+    b.rinfo = ws.take<uint32_t>(4 * (b.R + 1));
+    b.items = ws.take<uint32_t>(3 * items);
+    b.ovf = ws.take<uint32_t>(max_items);
+    b.entries = ws.take<uint32_t>(2 * pairs);
+    b.cand_cnt = ws.take<uint32_t>(b.cq);
+    b.cand_d = ws.take<float>(b.cq * (size_t)cap);
+    b.cand_i = ws.take<uint32_t>(b.cq * (size_t)cap);
+    b.adig = ws.take<int8_t>(pairs * 2 * Dp);
+    b.qparams = ws.take<char>(pairs * 48);
+    b.surv = ws.take<uint4>(h->batch_surv_cap);
+    b.surv_count = ws.take<uint32_t>(kMaxBatchGrid);
+}
+
+static int ensure_aux(erq_index* h) {
+    hipError_t e = hipSuccess;
+    if (!h->aux) e = hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking);
+    if (e == hipSuccess && !h->ev_fork)
+        e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess && !h->ev_join)
+        e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);
+    return e == hipSuccess ? ERQ_OK : hip_fail(e, "batched search stream");
+}
+
 static size_t batch_ws_bytes(const erq_index* h, size_t nq, size_t max_nprobe, int cap) {
-    const size_t K = h->K, D = h->D;
-    const size_t cq = batch_chunk_nq(nq, max_nprobe, h->batch_pair_budget);
-    const size_t pairs = std::min(cq * std::max<size_t>(max_nprobe, 1), h->batch_pair_budget) + 8;
-    const size_t Dp = (size_t)bp_nchunk((int)D) * 512;
-    const size_t nitems = batch_items_bound(pairs, K, h->N, max_cluster_size(h));
-    return 13 * 256 + align_up(K * 4, 256) * 2 + align_up(16, 256) +
-           align_up((size_t)h->batch_surv_cap * 16, 256) + align_up(kMaxBatchGrid * 4, 256) +
-           align_up(nitems * 12, 256) + align_up(nitems * 4, 256) + align_up(pairs * 8 + 8, 256) +
-           align_up(cq * 4, 256) + align_up(cq * (size_t)cap * 4, 256) * 2 +
-           align_up(pairs * 2 * Dp, 256) + align_up(pairs * 48, 256);
+    Ws ws{nullptr};
+    BatchBufs b;
+    batch_layout(h, nq, max_nprobe, cap, ws, b);
+    return ws.off + 256;
 }
 
 static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
@@ -612,35 +663,27 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     if (h->D > 2048) return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s);
     int rc = ensure_bp(h, s);
     if (rc) return rc;
-    // round 0: the exact per-query scan of each query's kExactRanks nearest clusters
-    rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s, (int)kExactRanks);
-    if (rc || max_nprobe <= kExactRanks) return rc;
+    // ranks [0, kExactRanks): the exact per-query scan of each query's nearest clusters
+    if (max_nprobe <= kExactRanks)
+        return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s,
+                           (int)kExactRanks);
     const int cap = kCandCap;
-    const size_t cq = batch_chunk_nq(nq, max_nprobe, h->batch_pair_budget);
-    const size_t pairs = std::min(cq * max_nprobe, h->batch_pair_budget) + 8;
-    const size_t Dp = (size_t)bp_nchunk((int)h->D) * 512;
     Ws ws{ws_base, ws_off};
-    uint32_t* counts = ws.take<uint32_t>(h->K);
-    uint32_t* cursor = ws.take<uint32_t>(h->K);
-    uint32_t* n_items = ws.take<uint32_t>(4);
-    const size_t nitems_max = batch_items_bound(pairs, h->K, h->N, max_cluster_size(h));
-    uint32_t* items = ws.take<uint32_t>(nitems_max * 3);
-    uint32_t* ovf = ws.take<uint32_t>(nitems_max);
-    uint32_t* entries = ws.take<uint32_t>(2 * pairs + 2);
-    uint32_t* cand_cnt = ws.take<uint32_t>(cq);
-    float* cand_d = ws.take<float>(cq * (size_t)cap);
-    uint32_t* cand_i = ws.take<uint32_t>(cq * (size_t)cap);
-    int8_t* adig = ws.take<int8_t>(pairs * 2 * Dp);
-    void* qparams = ws.take<char>(pairs * 48);
-    uint4* surv = ws.take<uint4>(h->batch_surv_cap);
-    uint32_t* surv_count = ws.take<uint32_t>(kMaxBatchGrid);
-    hipError_t e = launch_zero(cand_cnt, cq, s);
-    if (e != hipSuccess) return hip_fail(e, "zero candidates");
+    BatchBufs b;
+    batch_layout(h, nq, max_nprobe, cap, ws, b);
+    if (b.R > (size_t)kMaxRounds) {
+        set_error("batched search: too many rounds");
+        return ERQ_EUNSUPPORTED;
+    }
+    rc = ensure_aux(h);
+    if (rc) return rc;
+    const size_t cq = b.cq;
     const int grid = (int)std::min<size_t>(
         kMaxBatchGrid, (size_t)std::max(1, h->num_cu) * std::max(1, h->batch_wg_per_cu));
-    for (size_t q0 = 0; q0 < nq; q0 += cq) {
-        const size_t n = std::min(cq, nq - q0);
-        BatchArgs ba;
+    const int prep_grid = 6 * ((std::max(1, h->num_cu) + 7) / 8 * 8);  // 6 fit per CU (LDS)
+    BatchArgs ba;
+    RoundBufs rb;
+    auto setup = [&](size_t q0, size_t n) {
         ba.q_rot = q_rot_dev + q0 * h->D;
         ba.probe_off = off + q0;
         ba.probe_sqr_y = pd;
@@ -656,18 +699,18 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.csize = h->csize;
         ba.bstart = h->bstart;
         ba.vstart = h->vstart;
-        ba.items = items;
-        ba.ovf = ovf;
-        ba.entries = entries;
-        ba.n_items = n_items;
-        ba.adig = adig;
-        ba.qparams = qparams;
+        ba.items = b.items;
+        ba.ovf = b.ovf;
+        ba.entries = b.entries;
+        ba.n_items = b.rinfo;
+        ba.adig = b.adig;
+        ba.qparams = b.qparams;
         ba.pool_d = dists + q0 * k;
-        ba.cand_cnt = cand_cnt;
-        ba.cand_d = cand_d;
-        ba.cand_i = cand_i;
-        ba.surv = surv;
-        ba.surv_count = surv_count;
+        ba.cand_cnt = b.cand_cnt;
+        ba.cand_d = b.cand_d;
+        ba.cand_i = b.cand_i;
+        ba.surv = b.surv;
+        ba.surv_count = b.surv_count;
         ba.surv_cap = h->batch_surv_cap;
         ba.stats = reinterpret_cast<unsigned long long*>(stats);
         ba.cand_cap = cap;
@@ -677,11 +720,42 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.nq = (int)n;
         ba.nchunk = bp_nchunk((int)h->D);
         ba.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
-        RoundBufs rb{pid, counts, cursor, ids + q0 * k, (int)h->K};
-        for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0)) {
-            e = launch_round(ba, rb, (int)r0, (int)std::min(next_round(r0), max_nprobe), grid, s);
-            if (e != hipSuccess) return hip_fail(e, "batched round");
+        rb = RoundBufs{pid, b.counts, b.cursor, ids + q0 * k, (int)h->K};
+    };
+    auto rounds = [&]() -> hipError_t {
+        for (size_t r = 0; r < b.R; r++) {
+            BatchArgs br = ba;
+            br.n_items = b.rinfo + 4 * r;
+            hipError_t e = launch_round_scan(br, rb, grid, s);
+            if (e != hipSuccess) return e;
         }
+        return hipSuccess;
+    };
+    // the first chunk's rounds are all prepared on the second stream while the exact ranks
+    // of every query run on the caller's; joined before the first round's scan
+    setup(0, std::min(cq, nq));
+    hipError_t e = hipEventRecord(h->ev_fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->aux, h->ev_fork, 0);
+    if (e == hipSuccess) e = launch_prep_all(ba, rb, (int)b.R, (int)b.r_end, prep_grid, h->aux);
+    hipError_t ej = hipEventRecord(h->ev_join, h->aux);
+    if (e != hipSuccess || ej != hipSuccess) {
+        (void)hipStreamWaitEvent(s, h->ev_join, 0);
+        return hip_fail(e != hipSuccess ? e : ej, "batched round preparation");
+    }
+    rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s, (int)kExactRanks);
+    e = hipStreamWaitEvent(s, h->ev_join, 0);  // joined even if the exact ranks failed
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "batched search join");
+    for (size_t q0 = 0; q0 < nq; q0 += cq) {
+        const size_t n = std::min(cq, nq - q0);
+        if (q0) {
+            setup(q0, n);
+            e = launch_prep_all(ba, rb, (int)b.R, (int)b.r_end, prep_grid, s);
+            if (e != hipSuccess) return hip_fail(e, "batched round preparation");
+        }
+        e = launch_zero(b.cand_cnt, n, s);
+        if (e == hipSuccess) e = rounds();
+        if (e != hipSuccess) return hip_fail(e, "batched round");
     }
     return ERQ_OK;
 }

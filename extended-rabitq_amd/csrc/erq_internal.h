@@ -39,7 +39,11 @@ struct erq_index {
     int plain_waves = 1;    // ERQ_OPT_PLAIN_WAVES
     int batch_wg_per_cu = 3;  // ERQ_OPT_BATCH_WG_PER_CU
     size_t batch_surv_cap = size_t(1) << 26;  // ERQ_OPT_BATCH_SURVIVOR_CAP
-    size_t batch_pair_budget = size_t(1) << 22;  // ERQ_OPT_BATCH_PAIR_BUDGET
+    size_t batch_pair_budget = size_t(1) << 23;  // ERQ_OPT_BATCH_PAIR_BUDGET
+    // second stream of the batched search: the rounds' work lists and preparation run on it
+    // while the exact ranks run on the caller's stream (fork and join by events)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::mutex mu;
 };
 

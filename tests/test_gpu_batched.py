@@ -100,6 +100,12 @@ def test_batched_query_chunks():
     _check(idx, q, 100, np.full(q.shape[0], 64), "chunks", pair_budget=1024)
 
 
+def test_batched_rounds_past_rank_192():
+    """256 probes: rounds [4,16), [16,64), [64,192) and [192,256), all prepared at once."""
+    idx, q, _ = _build(("d128", "k256"), 20000, 128, 256, 4, nq=24)
+    _check(idx, q, 100, np.full(q.shape[0], 256), "rank 192+")
+
+
 def test_batched_duplicate_vectors():
     """Every vector 24 times over (ties everywhere), rounds past the exact ranks."""
     from erq import data, index
