@@ -253,6 +253,7 @@ __global__ void scatter_kernel(const uint32_t* __restrict__ off, const uint32_t*
 struct QParam {
     float y, sqr_y, fac, sumq, delta, c1, c2, qs, thr;
     int q;
+    uint32_t cid;  // the pair's cluster
 };
 
 __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_bytes) {
@@ -273,6 +274,7 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
         const uint32_t cid = a.probe_ids[slot];
         QParam P;
         P.q = q;
+        P.cid = cid;
         P.sqr_y = a.probe_sqr_y[slot];
         P.y = sqrtf(P.sqr_y);
         float sumq, delta;
@@ -298,7 +300,7 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
 // item while the item's digits are staged in LDS, runs the bit-plane MFMAs, and appends
 // every vector whose lower bound is below a query's threshold to that query's LDS list.
 // The lists are then written to the round's survivor list (one record per (query,
-// vector): entry, cluster, vector, scaled FastScan product) for reest_kernel.  If that list
+// vector): entry, vector, scaled FastScan product, |o - c|) for reest_kernel.  If that list
 // is full, the workgroup re-estimates its survivors itself (same formulas, same result).
 
 // The EX-bit re-estimate of one survivor by a 16-lane group, in the reference's order:
@@ -359,6 +361,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     const size_t abytes = max((size_t)NCH * 8192, (size_t)256 * S);
     uint32_t* s_vec = reinterpret_cast<uint32_t*>(smem + abytes);
     float* s_ipq = reinterpret_cast<float*>(s_vec + kGroup * kQSurv);
+    float* s_on = s_ipq + kGroup * kQSurv;
     __shared__ QParam qp[kGroup];
     __shared__ uint32_t s_cnt[kGroup];
 
@@ -468,6 +471,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                     const uint32_t sidx = atomicAdd(&s_cnt[qq], 1u);
                     s_vec[qq * kQSurv + sidx] = (uint32_t)vloc;
                     s_ipq[qq * kQSurv + sidx] = ipq;
+                    s_on[qq * kQSurv + sidx] = on;
                 }
             }
             if (lane == 0 && !INLINE) st_blocks++;
@@ -488,8 +492,9 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
 #pragma unroll
                     for (int j = 1; j < kGroup; j++) qq += i >= qoff[j] ? 1 : 0;
                     const int si = qq * kQSurv + (i - qoff[qq]);
-                    slice[used + i] = make_uint4(e0 + (uint32_t)qq, cid, s_vec[si],
-                                                 __float_as_uint(s_ipq[si]));
+                    slice[used + i] = make_uint4(e0 + (uint32_t)qq, v0 + s_vec[si],
+                                                 __float_as_uint(s_ipq[si]),
+                                                 __float_as_uint(s_on[si]));
                 }
                 used += (size_t)tot;
             } else if (tid == 0) {  // the slice is full: queue the item for the inline pass
@@ -530,8 +535,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                 const bool act = si < cnt;
                 const int vloc = act ? (int)s_vec[qq * kQSurv + si] : 0;
                 const uint32_t vi = v0 + (uint32_t)vloc;
-                const float on = reinterpret_cast<const float*>(
-                    a.short_data + (size_t)(b0 + (uint32_t)(vloc >> 5)) * BB + 4 * (size_t)D)[vloc & 31];
+                const float on = act ? s_on[qq * kQSurv + si] : 0.f;
                 const float exd = reestimate<EX>(
                     a, P, [&](int i) { return uw[ln * S + i]; },
                     a.exl + ((size_t)vi * 16 + ln) * a.exl_w, act,
@@ -600,7 +604,6 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
     const size_t nw = (size_t)(gridDim.x / xs) * 4;
     const size_t wv = (size_t)(blockIdx.x / xs) * 4 + wid;
     const size_t r_begin = total * wv / nw, r_end = total * (wv + 1) / nw;
-    const size_t BB = 4 * (size_t)D + 128;
     const float value = (float)(1.0 / (double)sqrtf((float)D));
     unsigned long long st_surv = 0, st_cand = 0;
     uint32_t staged = 0xFFFFFFFFu;
@@ -631,7 +634,7 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             P = reinterpret_cast<const QParam*>(a.qparams)[e0];
             P.thr = a.pool_d[(size_t)P.q * a.k + a.k - 1];
             const float* qv = a.q_rot + (size_t)P.q * D;
-            const float* cen = a.centroids + (size_t)__shfl(rec.y, 0) * D;
+            const float* cen = a.centroids + (size_t)P.cid * D;
             wave_sync();
             if (P.y > 1e-5f) {
                 for (int d = 4 * lane; d < D; d += 256) {
@@ -648,15 +651,11 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             wave_sync();
             staged = e0;
         }
-        const uint32_t cid = rec.y, vloc = rec.z;
-        const uint32_t vi = act ? a.vstart[cid] + vloc : 0;
-        const float on = act ? reinterpret_cast<const float*>(
-                                   a.short_data + (size_t)(a.bstart[cid] + (vloc >> 5)) * BB +
-                                   4 * (size_t)D)[vloc & 31]
-                             : 0.f;
+        const uint32_t vi = act ? rec.y : 0;
+        const float on = act ? __uint_as_float(rec.w) : 0.f;
         const float exd = reestimate<EX>(a, P, [&](int i) { return uw[l16 * S + i]; },
                                          a.exl + ((size_t)vi * 16 + l16) * a.exl_w, act,
-                                         __uint_as_float(rec.w), on, vi);
+                                         __uint_as_float(rec.z), on, vi);
         if (act && l16 == 0) {
             if (exd <= P.thr) {
                 add_candidate(a, P.q, exd, a.ids[vi]);
@@ -831,7 +830,7 @@ hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uin
 
 size_t batch_lds_bytes(int D) {
     return std::max((size_t)bp_nchunk(D) * 8192, (size_t)256 * ul_stride(D)) +
-           (size_t)kGroup * kQSurv * 8;
+           (size_t)kGroup * kQSurv * 12;
 }
 
 size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize) {
