@@ -23,7 +23,7 @@ struct BatchArgs {
     const uint32_t* items;     // 3 words per item: cluster, (first entry << 4) | count, first tile
     const uint32_t* entries;   // (probe slot, query) per entry, grouped by cluster
     uint32_t* n_items;         // [0] items of this round, [1] its first item, [2] entries, [3] queued
-    uint32_t* ovf;             // items queued for the inline pass (slice full)
+    uint32_t* ovf;             // queued item halves (2 x item + half) for the inline pass
     const uint32_t* probe_ids; // per slot: cluster id
     int8_t* adig;              // per entry: 2 * Dp int8 digits in MFMA operand order
     void* qparams;             // per entry: query parameters
@@ -50,7 +50,8 @@ struct RoundBufs {
 };
 
 int bp_nchunk(int D);
-size_t batch_lds_bytes(int D);
+size_t batch_lds_bytes(int D);         // main scan pass
+size_t batch_inline_lds_bytes(int D);  // inline pass
 size_t prep_wave_bytes(int D);
 // Upper bound on the work items of one round of at most `pairs` (query, cluster) pairs.
 size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize);

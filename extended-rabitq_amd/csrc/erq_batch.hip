@@ -32,10 +32,14 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int kGroup = 8;      // queries per work item (16 MFMA rows = 8 x 2 planes)
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kItemGroup = 16; // entries per work item (32 rows of the main pass's MFMA)
+constexpr int kGroup = 8;      // entries per inline pass over an item half (16 MFMA rows)
 constexpr int kThreads = 256;  // 4 waves
-constexpr int kItemTiles = 16; // 16-vector tiles per work item (4 per wave)
+constexpr int kItemTiles = 16; // 16-vector tiles (8 blocks of 32) per work item
 constexpr int kQSurv = kItemTiles * 16;  // survivors of one query in one work item, at most
+constexpr int kMainSurv = 128; // of them kept in LDS by the main pass (more: inline pass)
 
 // ---------------------------------------------------------------- relayout
 // One thread per output byte: (block, chunk, vector, byte B).
@@ -129,8 +133,8 @@ __global__ void count_kernel(const uint32_t* __restrict__ off, const uint32_t* _
 }
 
 // Single workgroup: exclusive scans of the bin counts and of items per bin, then the items.
-// A work item is (cluster, up to kGroup consecutive entries, up to kItemTiles tiles of the
-// cluster), three words: cluster, (first entry << 4) | entry count, first tile.  Items and
+// A work item is (cluster, up to kItemGroup consecutive entries, up to kItemTiles tiles of
+// the cluster), three words: cluster, (first entry << 5) | entry count, first tile.  Items and
 // entries are in (round, cluster) order; rinfo[4r + 0..3] = the round's items, first item,
 // entries, items queued for the inline pass (0); rinfo[4R + 0], [4R + 2] = the totals.
 __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __restrict__ counts,
@@ -152,7 +156,7 @@ __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __rest
         const uint32_t cnt = bin < nb ? counts[bin] : 0;
         const uint32_t ntile = bin < nb ? (csize[c] + 15) / 16 : 0;
         const uint32_t nsplit = (ntile + kItemTiles - 1) / kItemTiles;  // 0: empty cluster
-        const uint32_t it = (cnt + kGroup - 1) / kGroup * nsplit;
+        const uint32_t it = (cnt + kItemGroup - 1) / kItemGroup * nsplit;
         // inclusive scans: within each wave by shuffles, then over the 16 wave totals
         uint32_t ic = cnt, ii = it;
         for (int o = 1; o < 64; o <<= 1) {
@@ -207,8 +211,8 @@ __global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __rest
             const uint32_t j = i - before_it, grp = j / ns, sp = j % ns;
             const uint32_t w = carry_i + i;
             items[3 * w] = (uint32_t)((base + lo) % K);
-            items[3 * w + 1] = ((carry_c + before_c + grp * kGroup) << 4) |
-                               min((uint32_t)kGroup, ccnt - grp * kGroup);
+            items[3 * w + 1] = ((carry_c + before_c + grp * kItemGroup) << 5) |
+                               min((uint32_t)kItemGroup, ccnt - grp * kItemGroup);
             items[3 * w + 2] = sp * kItemTiles;
         }
         __syncthreads();
@@ -341,12 +345,186 @@ __device__ __forceinline__ void load_tile_codes(const uint8_t* bp, uint32_t blk,
                                                16 * lg);
 }
 
-// INLINE = false: the main pass over the round's items; an item whose survivors do not fit
-// in the workgroup's slice is queued for the INLINE = true pass instead.  INLINE = true:
-// the queued items again, with their survivors re-estimated in the workgroup (the same
-// formulas), so every item's survivors are handled exactly once either way.
-template <int EX, int NCH, bool INLINE>
+// The main pass: one workgroup per work item (cluster, up to 16 entries, up to 8 blocks of
+// 32 vectors; each wave takes blocks w and w + 4).  The item's digits are staged in LDS as
+// the 32-row A operand of v_mfma_i32_32x32x32_i8 (row 2 qq + d = digit d of entry qq);
+// B is bit-plane p of a block's code bytes as 0/1 bytes ((byte >> p) & 1), so all planes
+// and chunks accumulate into one 32 x 32 integer tile: lane (h, n) ends with the sums of
+// vector n for rows 8 b + 4 h + i.  Every vector whose lower bound is below its entry's
+// threshold goes to that entry's LDS list; the lists are written to the round's survivor
+// list (one record per (entry, vector): entry, vector, scaled FastScan product, |o - c|)
+// for reest_kernel.  An item with a list beyond kMainSurv, or whose records do not fit in
+// the workgroup's slice, is queued (both halves) for batch_inline_kernel instead.
+template <int EX, int NCH>
 __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int Dp = NCH * 512;
+    const int D = a.D;
+    int8_t* A = reinterpret_cast<int8_t*>(smem);  // [NCH][8 p][4 g][32 rows][16 B]
+    uint32_t* s_vec = reinterpret_cast<uint32_t*>(smem + (size_t)NCH * 16384);
+    float* s_ipq = reinterpret_cast<float*>(s_vec + kItemGroup * kMainSurv);
+    float* s_on = s_ipq + kItemGroup * kMainSurv;
+    __shared__ QParam qp[kItemGroup];
+    __shared__ uint32_t s_cnt[kItemGroup];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int h = lane >> 5, n32 = lane & 31;
+    const size_t BB = 4 * (size_t)D + 128;
+    unsigned long long st_tiles = 0;
+    // this workgroup's slice of the survivor list
+    const size_t slice_cap = a.surv_cap / gridDim.x;
+    uint4* slice = a.surv + (size_t)blockIdx.x * slice_cap;
+    size_t used = 0;
+    const uint32_t nitems = a.n_items[0], ifirst = a.n_items[1];
+
+    // XCD-aware order: workgroup b runs on XCD b % 8, so each XCD takes a contiguous eighth
+    // of the items (clusters in order) and the items of one cluster share that XCD's L2.
+    uint32_t it_lo = 0, it_hi = nitems, it_step = gridDim.x, it_first = blockIdx.x;
+    if ((gridDim.x & 7) == 0) {
+        const uint32_t x = blockIdx.x & 7;
+        it_lo = (uint32_t)((uint64_t)nitems * x / 8);
+        it_hi = (uint32_t)((uint64_t)nitems * (x + 1) / 8);
+        it_step = gridDim.x >> 3;
+        it_first = it_lo + (blockIdx.x >> 3);
+    }
+    for (uint32_t it = it_first; it < it_hi; it += it_step) {
+        __syncthreads();  // the previous item's LDS is no longer read
+        const uint32_t item = ifirst + it;  // global index
+        const uint32_t cid = a.items[3 * item];
+        const uint32_t e0 = a.items[3 * item + 1] >> 5, elen = a.items[3 * item + 1] & 31u;
+        const int bfirst = (int)a.items[3 * item + 2] >> 1;
+        const int n = (int)a.csize[cid];
+        const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
+        const int bend = min((n + 31) >> 5, bfirst + kItemTiles / 2);
+        // this wave's blocks bfirst + wid + 4 j: 16 code bytes per (chunk, K half) per lane
+        uint4 w[2][NCH][2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int bl = bfirst + wid + 4 * j;
+            if (bl < bend) {
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+                    const uint8_t* src =
+                        a.bp + (((size_t)(b0 + (uint32_t)bl) * NCH + c) * 32 + n32) * 64 + 16 * h;
+                    w[j][c][0] = *reinterpret_cast<const uint4*>(src);
+                    w[j][c][1] = *reinterpret_cast<const uint4*>(src + 32);
+                }
+            }
+        }
+        if (tid < kItemGroup) {
+            s_cnt[tid] = 0;
+            if (tid < (int)elen) {
+                QParam P = reinterpret_cast<const QParam*>(a.qparams)[e0 + tid];
+                P.thr = a.pool_d[(size_t)P.q * a.k + a.k - 1];  // k-th distance at round start
+                qp[tid] = P;
+            }
+        }
+        // stage the digits: 16-byte units (c, p, g) x 32 rows; rows of missing entries are 0
+        for (int u = tid; u < NCH * 1024; u += kThreads) {
+            const int row = u & 31, cpg = u >> 5;
+            const int qq = row >> 1, dgt = row & 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (qq < (int)elen)
+                v = reinterpret_cast<const uint4*>(a.adig + (size_t)(e0 + qq) * 2 * Dp)[cpg * 2 + dgt];
+            reinterpret_cast<uint4*>(A)[u] = v;
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int bl = bfirst + wid + 4 * j;
+            if (bl >= bend) break;
+            // four independent accumulation chains (planes p, p + 4 share one), summed
+            // after: integer addition, so the order does not matter
+            v16i acc4[4] = {};
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+#pragma unroll
+                for (int sk = 0; sk < 2; sk++) {
+                    const uint4 cw = w[j][c][sk];
+#pragma unroll
+                    for (int p = 0; p < 8; p++) {
+                        const v4i bv = v4i{(int)((cw.x >> p) & 0x01010101u),
+                                           (int)((cw.y >> p) & 0x01010101u),
+                                           (int)((cw.z >> p) & 0x01010101u),
+                                           (int)((cw.w >> p) & 0x01010101u)};
+                        const v4i av = *reinterpret_cast<const v4i*>(
+                            A + ((((c * 8 + p) * 4 + 2 * sk + h) * 32) + n32) * 16);
+                        acc4[p & 3] =
+                            __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc4[p & 3], 0, 0, 0);
+                    }
+                }
+            }
+            const v16i acc = (acc4[0] + acc4[1]) + (acc4[2] + acc4[3]);
+            const int vloc = 32 * bl + n32;
+            const float on = reinterpret_cast<const float*>(
+                a.short_data + (size_t)(b0 + (uint32_t)bl) * BB + 4 * (size_t)D)[n32];
+            if (lane == 0) st_tiles += (n - 32 * bl > 16) ? 2 : 1;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+#pragma unroll
+                for (int i2 = 0; i2 < 2; i2++) {
+                    const int qq = 4 * b + 2 * h + i2;
+                    if (qq >= (int)elen) continue;
+                    const QParam& P = qp[qq];
+                    const int I = 128 * acc[4 * b + 2 * i2 + 1] + acc[4 * b + 2 * i2];
+                    const float ipq = (float)I * P.delta;
+                    float tl = ipq - P.c1;
+                    tl = tl * P.c2;
+                    tl = tl * on;
+                    tl = tl + P.qs;
+                    tl = tl + on * on;
+                    if (vloc < n && tl < P.thr) {
+                        const uint32_t sidx = atomicAdd(&s_cnt[qq], 1u);
+                        if (sidx < (uint32_t)kMainSurv) {
+                            s_vec[qq * kMainSurv + sidx] = (uint32_t)vloc;
+                            s_ipq[qq * kMainSurv + sidx] = ipq;
+                            s_on[qq * kMainSurv + sidx] = on;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        int qoff[kItemGroup + 1];
+        qoff[0] = 0;
+        bool full = false;
+#pragma unroll
+        for (int qq = 0; qq < kItemGroup; qq++) {
+            const int c = qq < (int)elen ? (int)s_cnt[qq] : 0;
+            full |= c > kMainSurv;
+            qoff[qq + 1] = qoff[qq] + c;
+        }
+        const int tot = qoff[kItemGroup];
+        if (tot == 0) continue;  // uniform: every thread read the same counts
+        if (!full && used + (size_t)tot <= slice_cap) {
+            for (int i = tid; i < tot; i += kThreads) {
+                int qq = 0;
+#pragma unroll
+                for (int jq = 1; jq < kItemGroup; jq++) qq += i >= qoff[jq] ? 1 : 0;
+                const int si = qq * kMainSurv + (i - qoff[qq]);
+                slice[used + i] = make_uint4(e0 + (uint32_t)qq, v0 + s_vec[si],
+                                             __float_as_uint(s_ipq[si]),
+                                             __float_as_uint(s_on[si]));
+            }
+            used += (size_t)tot;
+        } else if (tid == 0) {  // lists or slice full: both halves go to the inline pass
+            const uint32_t at = atomicAdd(&a.n_items[3], 2u);
+            a.ovf[at] = 2 * item;
+            a.ovf[at + 1] = 2 * item + 1;
+        }
+    }
+    if (tid == 0) a.surv_count[blockIdx.x] = (uint32_t)used;
+    if (a.stats) wave_stat_add(&a.stats[6], st_tiles);
+}
+
+// The inline pass: the halves (up to kGroup entries) of the items the main pass queued
+// (survivors beyond its LDS lists or its slice), scanned again with the 16 x 16 x 64 MFMA
+// and their survivors re-estimated in the workgroup (the same formulas), so every item's
+// survivors are handled exactly once either way.  Queue entry = 2 x item + half.
+template <int EX, int NCH>
+__global__ void __launch_bounds__(kThreads) batch_inline_kernel(BatchArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int Dp = NCH * 512;
     constexpr int kWaveTiles = kItemTiles / 4;
@@ -369,30 +547,16 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     const int ln = lane & 15, lg = lane >> 4;
     const size_t BB = 4 * (size_t)D + 128;
     const size_t CL = (size_t)D * EX / 8;
-    unsigned long long st_blocks = 0, st_surv = 0, st_cand = 0;
-    // this workgroup's slice of the survivor list
-    const size_t slice_cap = a.surv_cap / gridDim.x;
-    uint4* slice = a.surv + (size_t)blockIdx.x * slice_cap;
-    size_t used = 0;
-    const uint32_t nitems = a.n_items[0];
-
-    // XCD-aware order: workgroup b runs on XCD b % 8, so each XCD takes a contiguous eighth
-    // of the items (clusters in order) and the items of one cluster share that XCD's L2.
-    uint32_t it_lo = 0, it_hi = nitems, it_step = gridDim.x, it_first = blockIdx.x;
-    if (INLINE) {
-        it_hi = a.n_items[3];  // queued items
-    } else if ((gridDim.x & 7) == 0) {
-        const uint32_t x = blockIdx.x & 7;
-        it_lo = (uint32_t)((uint64_t)nitems * x / 8);
-        it_hi = (uint32_t)((uint64_t)nitems * (x + 1) / 8);
-        it_step = gridDim.x >> 3;
-        it_first = it_lo + (blockIdx.x >> 3);
-    }
-    for (uint32_t it = it_first; it < it_hi; it += it_step) {
+    unsigned long long st_surv = 0, st_cand = 0;
+    const uint32_t nq_items = a.n_items[3];  // queued item halves
+    for (uint32_t it = blockIdx.x; it < nq_items; it += gridDim.x) {
         __syncthreads();  // the previous item's LDS is no longer read
-        const uint32_t item = INLINE ? a.ovf[it] : a.n_items[1] + it;  // global index
+        const uint32_t qe = a.ovf[it], item = qe >> 1, half = qe & 1u;  // global item index
         const uint32_t cid = a.items[3 * item];
-        const uint32_t e0 = a.items[3 * item + 1] >> 4, elen = a.items[3 * item + 1] & 15u;
+        const uint32_t ecnt = a.items[3 * item + 1] & 31u;
+        if (ecnt <= kGroup * half) continue;  // uniform: no entries in this half
+        const uint32_t e0 = (a.items[3 * item + 1] >> 5) + kGroup * half;
+        const uint32_t elen = min((uint32_t)kGroup, ecnt - kGroup * half);
         const int tfirst = (int)a.items[3 * item + 2];
         const int n = (int)a.csize[cid];
         const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
@@ -474,7 +638,6 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                     s_on[qq * kQSurv + sidx] = on;
                 }
             }
-            if (lane == 0 && !INLINE) st_blocks++;
         }
         __syncthreads();
 
@@ -485,23 +648,6 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
             qoff[qq + 1] = qoff[qq] + (qq < (int)elen ? (int)s_cnt[qq] : 0);
         const int tot = qoff[kGroup];
         if (tot == 0) continue;  // uniform: every thread read the same counts
-        if (!INLINE) {
-            if (used + (size_t)tot <= slice_cap) {
-                for (int i = tid; i < tot; i += kThreads) {
-                    int qq = 0;
-#pragma unroll
-                    for (int j = 1; j < kGroup; j++) qq += i >= qoff[j] ? 1 : 0;
-                    const int si = qq * kQSurv + (i - qoff[qq]);
-                    slice[used + i] = make_uint4(e0 + (uint32_t)qq, v0 + s_vec[si],
-                                                 __float_as_uint(s_ipq[si]),
-                                                 __float_as_uint(s_on[si]));
-                }
-                used += (size_t)tot;
-            } else if (tid == 0) {  // the slice is full: queue the item for the inline pass
-                a.ovf[atomicAdd(&a.n_items[3], 1u)] = item;
-            }
-            continue;
-        }
         // inline pass: re-estimate this item's survivors here
         __syncthreads();  // every wave is done with the digits in A
         const float value = (float)(1.0 / (double)sqrtf((float)D));
@@ -550,9 +696,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
             }
         }
     }
-    if (!INLINE && tid == 0) a.surv_count[blockIdx.x] = (uint32_t)used;
     if (a.stats) {
-        wave_stat_add(&a.stats[6], st_blocks);
         wave_stat_add(&a.stats[2], st_surv);
         wave_stat_add(&a.stats[4], st_cand);
     }
@@ -828,7 +972,10 @@ hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uin
     return hipGetLastError();
 }
 
-size_t batch_lds_bytes(int D) {
+size_t batch_lds_bytes(int D) {  // main pass
+    return (size_t)bp_nchunk(D) * 16384 + (size_t)kItemGroup * kMainSurv * 12;
+}
+size_t batch_inline_lds_bytes(int D) {
     return std::max((size_t)bp_nchunk(D) * 8192, (size_t)256 * ul_stride(D)) +
            (size_t)kGroup * kQSurv * 12;
 }
@@ -836,7 +983,7 @@ size_t batch_lds_bytes(int D) {
 size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize) {
     const size_t tiles_per_item = (size_t)kItemTiles * 16;
     const size_t max_split = (max_csize + tiles_per_item - 1) / tiles_per_item;
-    return (pairs / kGroup + 1) * max_split + N / tiles_per_item + K + 8;
+    return (pairs / kItemGroup + 1) * max_split + N / tiles_per_item + K + 8;
 }
 
 size_t prep_wave_bytes(int D) { return 4 * 16 * (size_t)ul_stride(D); }
@@ -866,41 +1013,41 @@ hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s) {
 }
 
 template <int EX, int NCH>
-static hipError_t launch_batch_scan_nch(const BatchArgs& ba, int grid, size_t lds, hipStream_t s) {
-    if (lds + 1024 > 65536) {  // dynamic plus static LDS beyond the 64 KB default
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&batch_scan_kernel<EX, NCH, false>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_scan_kernel<EX, NCH, true>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((batch_scan_kernel<EX, NCH, false>), dim3(grid), dim3(kThreads), lds, s, ba);
-    // the queued items (none unless a workgroup's survivor slice filled up)
-    hipLaunchKernelGGL((batch_scan_kernel<EX, NCH, true>), dim3(grid), dim3(kThreads), lds, s, ba);
+static hipError_t launch_batch_scan_nch(const BatchArgs& ba, int grid, hipStream_t s) {
+    const size_t lds = batch_lds_bytes(ba.D), ilds = batch_inline_lds_bytes(ba.D);
+    hipError_t e = hipSuccess;
+    if (lds + 2048 > 65536)  // dynamic plus static LDS beyond the 64 KB default
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_scan_kernel<EX, NCH>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess && ilds + 2048 > 65536)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&batch_inline_kernel<EX, NCH>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)ilds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((batch_scan_kernel<EX, NCH>), dim3(grid), dim3(kThreads), lds, s, ba);
+    // the queued item halves (none unless a list or a workgroup's slice filled up)
+    hipLaunchKernelGGL((batch_inline_kernel<EX, NCH>), dim3(grid), dim3(kThreads), ilds, s, ba);
     return hipGetLastError();
 }
 
 template <int EX>
-static hipError_t launch_batch_scan_ex(const BatchArgs& ba, int grid, size_t lds, hipStream_t s) {
+static hipError_t launch_batch_scan_ex(const BatchArgs& ba, int grid, hipStream_t s) {
     switch (ba.nchunk) {
-    case 1: return launch_batch_scan_nch<EX, 1>(ba, grid, lds, s);
-    case 2: return launch_batch_scan_nch<EX, 2>(ba, grid, lds, s);
-    case 3: return launch_batch_scan_nch<EX, 3>(ba, grid, lds, s);
-    case 4: return launch_batch_scan_nch<EX, 4>(ba, grid, lds, s);
+    case 1: return launch_batch_scan_nch<EX, 1>(ba, grid, s);
+    case 2: return launch_batch_scan_nch<EX, 2>(ba, grid, s);
+    case 3: return launch_batch_scan_nch<EX, 3>(ba, grid, s);
+    case 4: return launch_batch_scan_nch<EX, 4>(ba, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-static hipError_t launch_batch_scan(const BatchArgs& ba, int grid, size_t lds, hipStream_t s) {
+static hipError_t launch_batch_scan(const BatchArgs& ba, int grid, hipStream_t s) {
     switch (ba.EX) {
-    case 2: return launch_batch_scan_ex<2>(ba, grid, lds, s);
-    case 3: return launch_batch_scan_ex<3>(ba, grid, lds, s);
-    case 4: return launch_batch_scan_ex<4>(ba, grid, lds, s);
-    case 6: return launch_batch_scan_ex<6>(ba, grid, lds, s);
-    case 7: return launch_batch_scan_ex<7>(ba, grid, lds, s);
-    case 8: return launch_batch_scan_ex<8>(ba, grid, lds, s);
+    case 2: return launch_batch_scan_ex<2>(ba, grid, s);
+    case 3: return launch_batch_scan_ex<3>(ba, grid, s);
+    case 4: return launch_batch_scan_ex<4>(ba, grid, s);
+    case 6: return launch_batch_scan_ex<6>(ba, grid, s);
+    case 7: return launch_batch_scan_ex<7>(ba, grid, s);
+    case 8: return launch_batch_scan_ex<8>(ba, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -947,8 +1094,7 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
 }
 
 hipError_t launch_round_scan(const BatchArgs& ba, const RoundBufs& rb, int grid, hipStream_t s) {
-    const size_t lds = batch_lds_bytes(ba.D);
-    hipError_t e = launch_batch_scan(ba, grid, lds, s);
+    hipError_t e = launch_batch_scan(ba, grid, s);
     if (e != hipSuccess) return e;
     e = launch_reest(ba, grid, s);
     if (e != hipSuccess) return e;

@@ -626,7 +626,7 @@ static void batch_layout(const erq_index* h, size_t nq, size_t max_nprobe, int c
     b.cursor = ws.take<uint32_t>(b.R * K);
     b.rinfo = ws.take<uint32_t>(4 * (b.R + 1));
     b.items = ws.take<uint32_t>(3 * items);
-    b.ovf = ws.take<uint32_t>(max_items);
+    b.ovf = ws.take<uint32_t>(2 * max_items);
     b.entries = ws.take<uint32_t>(2 * pairs);
     b.cand_cnt = ws.take<uint32_t>(b.cq);
     b.cand_d = ws.take<float>(b.cq * (size_t)cap);

@@ -37,7 +37,7 @@ struct erq_index {
     int num_cu = 0;
     int plain_kernel = 0;   // ERQ_OPT_PLAIN_KERNEL
     int plain_waves = 1;    // ERQ_OPT_PLAIN_WAVES
-    int batch_wg_per_cu = 3;  // ERQ_OPT_BATCH_WG_PER_CU
+    int batch_wg_per_cu = 2;  // ERQ_OPT_BATCH_WG_PER_CU
     size_t batch_surv_cap = size_t(1) << 26;  // ERQ_OPT_BATCH_SURVIVOR_CAP
     size_t batch_pair_budget = size_t(1) << 23;  // ERQ_OPT_BATCH_PAIR_BUDGET
     // second stream of the batched search: the rounds' work lists and preparation run on it
