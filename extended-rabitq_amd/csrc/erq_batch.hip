@@ -387,6 +387,22 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         it_step = gridDim.x >> 3;
         it_first = it_lo + (blockIdx.x >> 3);
     }
+    // The item's digits (16-byte units (c, p, g) x 32 rows; rows of missing entries are 0)
+    // pass through registers: the next item's are loaded while this one computes.
+    constexpr int kStage = NCH * 1024 / kThreads;
+    uint4 dreg[kStage];
+    auto load_digits = [&](uint32_t item) {
+        const uint32_t f = a.items[3 * item + 1], de0 = f >> 5, delen = f & 31u;
+#pragma unroll
+        for (int i = 0; i < kStage; i++) {
+            const int u = tid + kThreads * i, row = u & 31, cpg = u >> 5;
+            const int qq = row >> 1, dgt = row & 1;
+            dreg[i] = qq < (int)delen ? reinterpret_cast<const uint4*>(
+                                            a.adig + (size_t)(de0 + qq) * 2 * Dp)[cpg * 2 + dgt]
+                                      : make_uint4(0, 0, 0, 0);
+        }
+    };
+    if (it_first < it_hi) load_digits(ifirst + it_first);
     for (uint32_t it = it_first; it < it_hi; it += it_step) {
         __syncthreads();  // the previous item's LDS is no longer read
         const uint32_t item = ifirst + it;  // global index
@@ -419,24 +435,18 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                 qp[tid] = P;
             }
         }
-        // stage the digits: 16-byte units (c, p, g) x 32 rows; rows of missing entries are 0
-        for (int u = tid; u < NCH * 1024; u += kThreads) {
-            const int row = u & 31, cpg = u >> 5;
-            const int qq = row >> 1, dgt = row & 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (qq < (int)elen)
-                v = reinterpret_cast<const uint4*>(a.adig + (size_t)(e0 + qq) * 2 * Dp)[cpg * 2 + dgt];
-            reinterpret_cast<uint4*>(A)[u] = v;
-        }
+#pragma unroll
+        for (int i = 0; i < kStage; i++) reinterpret_cast<uint4*>(A)[tid + kThreads * i] = dreg[i];
         __syncthreads();
+        if (it + it_step < it_hi) load_digits(item + it_step);
 
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int bl = bfirst + wid + 4 * j;
             if (bl >= bend) break;
-            // four independent accumulation chains (planes p, p + 4 share one), summed
-            // after: integer addition, so the order does not matter
-            v16i acc4[4] = {};
+            // two independent accumulation chains (even and odd planes), summed after:
+            // integer addition, so the order does not matter
+            v16i acc4[2] = {};
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
 #pragma unroll
@@ -450,12 +460,12 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                                            (int)((cw.w >> p) & 0x01010101u)};
                         const v4i av = *reinterpret_cast<const v4i*>(
                             A + ((((c * 8 + p) * 4 + 2 * sk + h) * 32) + n32) * 16);
-                        acc4[p & 3] =
-                            __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc4[p & 3], 0, 0, 0);
+                        acc4[p & 1] =
+                            __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc4[p & 1], 0, 0, 0);
                     }
                 }
             }
-            const v16i acc = (acc4[0] + acc4[1]) + (acc4[2] + acc4[3]);
+            const v16i acc = acc4[0] + acc4[1];
             const int vloc = 32 * bl + n32;
             const float on = reinterpret_cast<const float*>(
                 a.short_data + (size_t)(b0 + (uint32_t)bl) * BB + 4 * (size_t)D)[n32];
