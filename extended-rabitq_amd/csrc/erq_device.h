@@ -126,10 +126,20 @@ __device__ __forceinline__ int ul_index(int d, int S) { return (d & 15) * S + (d
 // u / delta, and a quotient of two floats never lies closer than 2^-49 (relative) to a
 // float rounding boundary without being the float itself, so rounding the product to
 // float gives exactly the IEEE float division.
+// The reference's tmp + 0.5 - [tmp < 0] is taken as one addition of +-0.5: for |tmp| >=
+// 2^-29 both forms are exact in double, below it both truncate to 0 (every float with
+// |tmp| <= 8192.5 checked).
 __device__ __forceinline__ int quantize_digit(float u, double rd) {
     const float tmp = (float)((double)u * rd);
-    const double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
+    const double r = (double)tmp + (tmp < 0.0f ? -0.5 : 0.5);
     return (int)r;
+}
+
+// Byte 0 of each of a, b, c, d as the bytes 0..3 of one word.
+__device__ __forceinline__ uint32_t pack_low_bytes(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    const uint32_t ab = __builtin_amdgcn_perm(b, a, 0x0c0c0400u);
+    const uint32_t cd = __builtin_amdgcn_perm(d, c, 0x0c0c0400u);
+    return __builtin_amdgcn_perm(cd, ab, 0x05040100u);
 }
 
 // The reference's per-(query, cluster) preparation (normalize_query16 and the digit scale
@@ -198,16 +208,18 @@ __device__ __forceinline__ void quantize_digits(const float* uq, int S, int D, i
         const int base = 512 * c + 128 * g + 32 * jw + 2 * pp;
         uint32_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
         if (base < D) {
+            int q0[4], q1[4];
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const int d = base + 8 * b;
-                const int q0 = quantize_digit(uq[ul_index(d, S)], rd);
-                const int q1 = quantize_digit(uq[ul_index(d + 1, S)], rd);
-                lo0 |= (uint32_t)(q0 & 127) << (8 * b);
-                hi0 |= ((uint32_t)(q0 >> 7) & 0xFFu) << (8 * b);
-                lo1 |= (uint32_t)(q1 & 127) << (8 * b);
-                hi1 |= ((uint32_t)(q1 >> 7) & 0xFFu) << (8 * b);
+                q0[b] = quantize_digit(uq[ul_index(d, S)], rd);
+                q1[b] = quantize_digit(uq[ul_index(d + 1, S)], rd);
             }
+            // digit 0 = q & 127 (byte b of the word), digit 1 = floor(q / 128) (its low byte)
+            lo0 = pack_low_bytes(q0[0], q0[1], q0[2], q0[3]) & 0x7F7F7F7Fu;
+            lo1 = pack_low_bytes(q1[0], q1[1], q1[2], q1[3]) & 0x7F7F7F7Fu;
+            hi0 = pack_low_bytes(q0[0] >> 7, q0[1] >> 7, q0[2] >> 7, q0[3] >> 7);
+            hi1 = pack_low_bytes(q1[0] >> 7, q1[1] >> 7, q1[2] >> 7, q1[3] >> 7);
         }
         const int un = 32 * c + 8 * pp + g;  // plane 2 pp; plane 2 pp + 1 is un + 4
         dst[8 * un + jw] = lo0;

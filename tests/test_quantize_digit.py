@@ -15,6 +15,27 @@ def _digits(tmp):
     return np.trunc(r).astype(np.int64)
 
 
+def _digits_one_add(tmp):
+    # the kernels' form: one addition of +-0.5
+    r = tmp.astype(np.float64) + np.where(tmp < 0, -0.5, 0.5)
+    return np.trunc(r).astype(np.int64)
+
+
+def test_single_addition_rounding():
+    rng = np.random.default_rng(13)
+    parts = [(rng.random(2_000_000) * 16384 - 8192).astype(np.float32),
+             (np.arange(-8192, 8193) + 0.5).astype(np.float32),
+             np.arange(-8192, 8193).astype(np.float32)]
+    for e in range(-149, 14):  # every binade, both signs, values next to half-integers
+        x = np.float32(2.0) ** e * (1 + rng.random(2000)).astype(np.float32)
+        parts += [x, -x, np.nextafter(x, np.float32(0)), np.nextafter(x, np.float32(np.inf))]
+    h = (np.arange(-8192, 8192) + 0.5).astype(np.float32)
+    parts += [np.nextafter(h, np.float32(-np.inf)), np.nextafter(h, np.float32(np.inf))]
+    t = np.concatenate(parts).astype(np.float32)
+    t = t[np.abs(t) <= 8192.5]
+    np.testing.assert_array_equal(_digits_one_add(t), _digits(t))
+
+
 def _check(u, delta):
     ref = (u / delta).astype(np.float32)  # IEEE float32 division
     rd = 1.0 / delta.astype(np.float64)
