@@ -42,19 +42,21 @@ enum {
     ERQ_OPT_PLAIN_KERNEL = 1,
     /* Queries (waves) per workgroup of the per-query scan, 1..4 (LDS permitting). */
     ERQ_OPT_PLAIN_WAVES = 2,
-    /* Workgroups per CU of the persistent batched kernel, 1..8. */
+    /* Workgroups per CU of the persistent batched kernel, 1..8 (default 2: its LDS holds
+     * two per CU at D <= 1536). */
     ERQ_OPT_BATCH_WG_PER_CU = 3,
     /* Records (16 bytes each) of the batched search's survivor list per round, 0..2^30
-     * (default 2^26).  Survivors beyond it are re-estimated inside the scan kernel
+     * (default 2^26).  Survivors beyond it are re-estimated by an inline pass of the scan
      * instead of by the separate re-estimation kernel; results are identical. */
     ERQ_OPT_BATCH_SURVIVOR_CAP = 4,
     /* Order in which the per-query scan takes queries: 1 (default) grouped by first probed
      * cluster, so queries scanned at the same time share clusters in L2; 0 as given.
      * Results are identical. */
     ERQ_OPT_PLAIN_ORDER = 5,
-    /* (query, cluster) pairs the batched search prepares per round, 1024..2^26 (default
-     * 2^22; 3 KB of digits each at D = 1536).  Queries are taken in chunks whose widest
-     * round fits it.  Results are identical; only workspace size and speed differ. */
+    /* (query, cluster) pairs the batched search prepares ahead, 1024..2^26 (default 2^23;
+     * 3 KB of digits each at D = 1536).  Every round of a chunk of queries is prepared at
+     * once, so a chunk's pairs over all ranks past the exact ones must fit it.  Results
+     * are identical; only workspace size and speed differ. */
     ERQ_OPT_BATCH_PAIR_BUDGET = 6
 };
 int erq_set_option(erq_index* h, int option, long value);
