@@ -43,8 +43,10 @@ struct BatchArgs {
 
 struct RoundBufs {
     const uint32_t* probe_ids;
-    uint32_t* counts;  // K
-    uint32_t* cursor;  // K
+    uint32_t* counts;  // bins ((round, cluster): R x K)
+    uint32_t* cursor;  // bins
+    uint32_t* local;   // 2 per bin: entries and items before it in its block of 256 bins
+    uint32_t* btot;    // 2 per block of 256 bins
     uint32_t* pool_i;  // nq x k
     int K;
 };

@@ -132,106 +132,125 @@ __global__ void count_kernel(const uint32_t* __restrict__ off, const uint32_t* _
     }
 }
 
-// Single workgroup: exclusive scans of the bin counts and of items per bin, then the items.
-// A work item is (cluster, up to kItemGroup consecutive entries, up to kItemTiles tiles of
-// the cluster), three words: cluster, (first entry << 5) | entry count, first tile.  Items and
-// entries are in (round, cluster) order; rinfo[4r + 0..3] = the round's items, first item,
-// entries, items queued for the inline pass (0); rinfo[4R + 0], [4R + 2] = the totals.
-__global__ void __launch_bounds__(1024) scan_items_kernel(const uint32_t* __restrict__ counts,
-                                                          const uint32_t* __restrict__ csize,
-                                                          int K, int R,
-                                                          uint32_t* __restrict__ cursor,
-                                                          uint32_t* __restrict__ items,
-                                                          uint32_t* __restrict__ rinfo) {
-    __shared__ uint32_t s_cnt[1024], s_it[1024], s_ns[1024];
-    __shared__ uint32_t carry_c, carry_i;
-    __shared__ uint32_t r_item[kMaxRounds + 1], r_ent[kMaxRounds + 1];
-    __shared__ uint32_t w_c[16], w_i[16];
-    const int t = threadIdx.x, nb = K * R, lane = t & 63, wv = t >> 6;
-    if (t == 0) carry_c = carry_i = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        const int bin = base + t;
-        const int c = bin % K;
-        const uint32_t cnt = bin < nb ? counts[bin] : 0;
-        const uint32_t ntile = bin < nb ? (csize[c] + 15) / 16 : 0;
-        const uint32_t nsplit = (ntile + kItemTiles - 1) / kItemTiles;  // 0: empty cluster
-        const uint32_t it = (cnt + kItemGroup - 1) / kItemGroup * nsplit;
-        // inclusive scans: within each wave by shuffles, then over the 16 wave totals
-        uint32_t ic = cnt, ii = it;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t a = __shfl_up(ic, o), b = __shfl_up(ii, o);
-            if (lane >= o) {
-                ic += a;
-                ii += b;
-            }
+// Work items from the bin counts, by workgroups of 256 (small enough to start beside other
+// kernels).  A work item is (cluster, up to kItemGroup consecutive entries, up to
+// kItemTiles tiles of the cluster), three words: cluster, (first entry << 5) | entry count,
+// first tile.  Items and entries are in (round, cluster) order; rinfo[4r + 0..3] = the
+// round's items, first item, entries, items queued for the inline pass (0); rinfo[4R + 0],
+// [4R + 2] = the totals.
+__device__ __forceinline__ uint32_t bin_items(const uint32_t* csize, int K, int bin, uint32_t cnt,
+                                              uint32_t& nsplit) {
+    const uint32_t ntile = (csize[bin % K] + 15) / 16;
+    nsplit = (ntile + kItemTiles - 1) / kItemTiles;  // 0: empty cluster
+    return (cnt + kItemGroup - 1) / kItemGroup * nsplit;
+}
+
+// Inclusive scan of (a, b) over the 256 threads of a workgroup.
+__device__ __forceinline__ void block_scan2(uint32_t& a, uint32_t& b, uint32_t* wa, uint32_t* wb) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = __shfl_up(a, o), y = __shfl_up(b, o);
+        if (lane >= o) {
+            a += x;
+            b += y;
         }
-        if (lane == 63) {
-            w_c[wv] = ic;
-            w_i[wv] = ii;
-        }
-        s_ns[t] = nsplit;
-        __syncthreads();
-        if (wv == 0) {
-            uint32_t vc = lane < 16 ? w_c[lane] : 0, vi = lane < 16 ? w_i[lane] : 0;
-            const uint32_t oc = vc, oi = vi;
-            for (int o = 1; o < 16; o <<= 1) {
-                const uint32_t a = __shfl_up(vc, o), b = __shfl_up(vi, o);
-                if (lane >= o) {
-                    vc += a;
-                    vi += b;
-                }
-            }
-            if (lane < 16) {  // exclusive prefix of the wave totals
-                w_c[lane] = vc - oc;
-                w_i[lane] = vi - oi;
-            }
-        }
-        __syncthreads();
-        s_cnt[t] = ic + w_c[wv];
-        s_it[t] = ii + w_i[wv];
-        __syncthreads();
-        if (bin < nb) {
-            cursor[bin] = carry_c + s_cnt[t] - cnt;
-            if (c == 0) {  // the first bin of round bin / K
-                r_item[bin / K] = carry_i + s_it[t] - it;
-                r_ent[bin / K] = carry_c + s_cnt[t] - cnt;
-            }
-        }
-        // items of this chunk of bins, one thread per item
-        const uint32_t nit = s_it[1023];
-        for (uint32_t i = t; i < nit; i += 1024) {
-            int lo = 0, hi = 1023;  // first bin slot whose inclusive item count exceeds i
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_it[mid] > i) hi = mid; else lo = mid + 1;
-            }
-            const uint32_t before_it = lo ? s_it[lo - 1] : 0, before_c = lo ? s_cnt[lo - 1] : 0;
-            const uint32_t ccnt = s_cnt[lo] - before_c, ns = s_ns[lo];
-            const uint32_t j = i - before_it, grp = j / ns, sp = j % ns;
-            const uint32_t w = carry_i + i;
-            items[3 * w] = (uint32_t)((base + lo) % K);
-            items[3 * w + 1] = ((carry_c + before_c + grp * kItemGroup) << 5) |
-                               min((uint32_t)kItemGroup, ccnt - grp * kItemGroup);
-            items[3 * w + 2] = sp * kItemTiles;
-        }
-        __syncthreads();
-        if (t == 1023) {
-            carry_c += s_cnt[1023];
-            carry_i += s_it[1023];
-        }
-        __syncthreads();
     }
-    if (t == 0) {
-        r_item[R] = carry_i;
-        r_ent[R] = carry_c;
+    if (lane == 63) {
+        wa[wv] = a;
+        wb[wv] = b;
+    }
+    __syncthreads();
+    for (int w = 0; w < wv; w++) {
+        a += wa[w];
+        b += wb[w];
+    }
+}
+
+// Per bin: entries and items before it within its workgroup of 256 bins; per workgroup
+// the totals.
+__global__ void __launch_bounds__(256) bin_scan_kernel(const uint32_t* __restrict__ counts,
+                                                      const uint32_t* __restrict__ csize, int K,
+                                                      int nb, uint32_t* __restrict__ local,
+                                                      uint32_t* __restrict__ btot) {
+    __shared__ uint32_t wa[4], wb[4];
+    const int bin = blockIdx.x * 256 + threadIdx.x;
+    uint32_t ns, cnt = bin < nb ? counts[bin] : 0;
+    uint32_t it = bin < nb ? bin_items(csize, K, bin, cnt, ns) : 0;
+    uint32_t ic = cnt, ii = it;
+    block_scan2(ic, ii, wa, wb);
+    if (bin < nb) {
+        local[2 * bin] = ic - cnt;
+        local[2 * bin + 1] = ii - it;
+    }
+    if (threadIdx.x == 255) {
+        btot[2 * blockIdx.x] = ic;
+        btot[2 * blockIdx.x + 1] = ii;
+    }
+}
+
+// One workgroup: exclusive prefix of the workgroup totals (in place) and the round table.
+__global__ void __launch_bounds__(256) bin_total_kernel(uint32_t* __restrict__ btot, int nblk,
+                                                       const uint32_t* __restrict__ local, int K,
+                                                       int R, uint32_t* __restrict__ rinfo) {
+    __shared__ uint32_t wa[4], wb[4];
+    __shared__ uint32_t s_item[kMaxRounds + 1], s_ent[kMaxRounds + 1];
+    const int t = threadIdx.x, per = (nblk + 255) / 256;
+    const int i_lo = min(nblk, t * per), i_hi = min(nblk, (t + 1) * per);
+    uint32_t oc = 0, oi = 0;
+    for (int i = i_lo; i < i_hi; i++) {
+        oc += btot[2 * i];
+        oi += btot[2 * i + 1];
+    }
+    uint32_t ic = oc, ii = oi;
+    block_scan2(ic, ii, wa, wb);
+    uint32_t c = ic - oc, it = ii - oi;  // exclusive prefix of this thread's first total
+    for (int i = i_lo; i < i_hi; i++) {
+        const uint32_t a = btot[2 * i], b = btot[2 * i + 1];
+        btot[2 * i] = c;
+        btot[2 * i + 1] = it;
+        c += a;
+        it += b;
+    }
+    if (t == 255) {  // the totals
+        s_ent[R] = ic;
+        s_item[R] = ii;
+    }
+    __syncthreads();  // every workgroup offset is in btot
+    if (t < R) {  // round t starts at bin t K: its workgroup's offset + the bin's local prefix
+        const int bin = t * K, blk = bin / 256;
+        s_ent[t] = btot[2 * blk] + local[2 * bin];
+        s_item[t] = btot[2 * blk + 1] + local[2 * bin + 1];
     }
     __syncthreads();
     if (t <= R) {
-        rinfo[4 * t + 0] = t < R ? r_item[t + 1] - r_item[t] : carry_i;
-        rinfo[4 * t + 1] = t < R ? r_item[t] : 0;
-        rinfo[4 * t + 2] = t < R ? r_ent[t + 1] - r_ent[t] : carry_c;
+        rinfo[4 * t + 0] = t < R ? s_item[t + 1] - s_item[t] : s_item[R];
+        rinfo[4 * t + 1] = t < R ? s_item[t] : 0;
+        rinfo[4 * t + 2] = t < R ? s_ent[t + 1] - s_ent[t] : s_ent[R];
         rinfo[4 * t + 3] = 0;
+    }
+}
+
+// Per bin: the cursor of its entries and its items.
+__global__ void __launch_bounds__(256) bin_items_kernel(const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ csize, int K,
+                                                       int nb, const uint32_t* __restrict__ local,
+                                                       const uint32_t* __restrict__ btot,
+                                                       uint32_t* __restrict__ cursor,
+                                                       uint32_t* __restrict__ items) {
+    const int bin = blockIdx.x * 256 + threadIdx.x;
+    if (bin >= nb) return;
+    const uint32_t cnt = counts[bin];
+    uint32_t ns;
+    const uint32_t nit = bin_items(csize, K, bin, cnt, ns);
+    const uint32_t c0 = btot[2 * blockIdx.x] + local[2 * bin];
+    const uint32_t i0 = btot[2 * blockIdx.x + 1] + local[2 * bin + 1];
+    cursor[bin] = c0;
+    for (uint32_t j = 0; j < nit; j++) {
+        const uint32_t grp = j / ns, sp = j % ns, w = i0 + j;
+        items[3 * w] = (uint32_t)(bin % K);
+        items[3 * w + 1] = ((c0 + grp * kItemGroup) << 5) |
+                           min((uint32_t)kItemGroup, cnt - grp * kItemGroup);
+        items[3 * w + 2] = sp * kItemTiles;
     }
 }
 
@@ -1087,8 +1106,13 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
     dim3 qg((nq + 255) / 256), qb(256);
     hipLaunchKernelGGL(count_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
                        (int)kFirstRoundRank, r_end, K, rb.counts);
-    hipLaunchKernelGGL(scan_items_kernel, dim3(1), dim3(1024), 0, s, rb.counts, ba.csize, K, R,
-                       rb.cursor, (uint32_t*)ba.items, ba.n_items);
+    const int nb = K * R, nblk = (nb + 255) / 256;
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(nblk), dim3(256), 0, s, rb.counts, ba.csize, K, nb,
+                       rb.local, rb.btot);
+    hipLaunchKernelGGL(bin_total_kernel, dim3(1), dim3(256), 0, s, rb.btot, nblk, rb.local, K, R,
+                       ba.n_items);
+    hipLaunchKernelGGL(bin_items_kernel, dim3(nblk), dim3(256), 0, s, rb.counts, ba.csize, K, nb,
+                       rb.local, rb.btot, rb.cursor, (uint32_t*)ba.items);
     hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
                        (int)kFirstRoundRank, r_end, K, rb.cursor, (uint32_t*)ba.entries);
     e = hipGetLastError();
