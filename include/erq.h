@@ -9,7 +9,9 @@
  * Pointer conventions: functions without a _dev suffix take host pointers and run on the
  * handle's own stream, synchronously; _dev functions take device pointers, enqueue on
  * `stream` exactly as given (a hipStream_t; 0 is the null stream) and return without
- * synchronising.
+ * synchronising.  The batched searches fork part of their work onto a second stream of the
+ * handle and join it back into `stream` before returning (events, so a HIP graph capture
+ * of `stream` takes both).
  * Thread safety: calls on one handle are serialised by a per-handle mutex.
  */
 #ifndef ERQ_H
