@@ -37,6 +37,7 @@ struct BatchArgs {
     size_t surv_cap;
     unsigned long long* stats; // 8 counters (see erq.h)
     int cand_cap;
+    int num_cu;                // compute units (sizes the re-estimate's grid)
     int D, EX, k, nq, nchunk;
     float one_over_sqrtD;
 };

@@ -281,32 +281,50 @@ struct QParam {
 
 __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ float s_sum[4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int D = a.D, nchunk = a.nchunk, Dp = nchunk * 512, S = ul_stride(D);
     float* uq = reinterpret_cast<float*>(smem + (size_t)wid * per_wave_bytes);
-    // A fixed grid (a multiple of 8 workgroups) walks the entries.  XCD-aware: workgroup b
-    // runs on XCD b % 8; entries are grouped by cluster, so giving each XCD a contiguous
-    // eighth of them keeps a centroid's reads in one L2.
+    // A fixed grid (a multiple of 8 workgroups) walks the entries, four at a time per
+    // workgroup (one per wave).  XCD-aware: workgroup b runs on XCD b % 8; entries are
+    // grouped by cluster, so giving each XCD a contiguous eighth of them keeps a centroid's
+    // reads in one L2.
     const uint32_t n = a.n_items[2];
     const uint32_t x = blockIdx.x & 7, nw = (gridDim.x >> 3) * 4;
     const uint32_t lo = (uint32_t)((uint64_t)n * x / 8), hi = (uint32_t)((uint64_t)n * (x + 1) / 8);
-    for (uint32_t e = lo + (blockIdx.x >> 3) * 4 + wid; e < hi; e += nw) {
-        const uint2 ent = reinterpret_cast<const uint2*>(a.entries)[e];
-        const uint32_t slot = ent.x;
-        const int q = (int)ent.y;
-        const uint32_t cid = a.probe_ids[slot];
+    uint32_t step = 0;
+    for (uint32_t base = lo + (blockIdx.x >> 3) * 4; base < hi; base += nw, step++) {  // uniform
+        const uint32_t e = base + wid;
+        const bool valid = e < hi;
         QParam P;
-        P.q = q;
-        P.cid = cid;
-        P.sqr_y = a.probe_sqr_y[slot];
-        P.y = sqrtf(P.sqr_y);
-        float sumq, delta;
-        prepare_residual(a.q_rot + (size_t)q * D, a.centroids + (size_t)cid * D, P.y, D, S,
-                         uq, lane, P.fac, sumq, delta);
+        float delta = 0.f;
+        bool res = true;
+        if (valid) {
+            const uint2 ent = reinterpret_cast<const uint2*>(a.entries)[e];
+            const uint32_t slot = ent.x, cid = a.probe_ids[slot];
+            P.q = (int)ent.y;
+            P.cid = cid;
+            P.sqr_y = a.probe_sqr_y[slot];
+            P.y = sqrtf(P.sqr_y);
+            res = residual_delta(a.q_rot + (size_t)P.q * D, a.centroids + (size_t)cid * D, P.y,
+                                 D, S, uq, lane, P.fac, delta);
+        }
+        __syncthreads();
+        // the four residual sums by one wave (16 lanes per entry; the wave rotates), while
+        // every wave quantizes its own entry
+        if (wid == (int)(step & 3)) {
+            const float* ur =
+                reinterpret_cast<const float*>(smem + (size_t)(lane >> 4) * per_wave_bytes);
+            const float part = residual_sum16(ur, S, D, lane & 15);
+            if ((lane & 15) == 0) s_sum[lane >> 4] = part;
+        }
         // the digits go straight to the entry's operand rows in HBM
-        quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta,
-                        reinterpret_cast<uint32_t*>(a.adig + (size_t)e * 2 * Dp), lane);
-        if (lane == 0) {
+        if (valid)
+            quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta,
+                            reinterpret_cast<uint32_t*>(a.adig + (size_t)e * 2 * Dp), lane);
+        __syncthreads();
+        if (valid && lane == 0) {
+            const float sumq = res ? s_sum[wid] : (float)D * (float)(1.0 / (double)sqrtf((float)D));
             P.sumq = sumq;
             P.delta = delta;
             P.c1 = (float)(0.5 * (double)sumq - (double)0.58f);
@@ -1082,8 +1100,12 @@ static hipError_t launch_batch_scan(const BatchArgs& ba, int grid, hipStream_t s
 }
 
 static hipError_t launch_reest(const BatchArgs& ba, int scan_grid, hipStream_t s) {
-    const dim3 grid(scan_grid * kReestWG);
     const size_t lds = (size_t)256 * ul_stride(ba.D) + 4 * ((size_t)scan_grid + 1);  // residuals, prefix
+    // as many workgroups as stay resident together (LDS-bound), a multiple of 8: each takes
+    // a fixed share of the records, so a second partial wave of workgroups would be a tail
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, 163840 / (lds + 1024)));
+    const dim3 grid(ba.num_cu > 0 ? (unsigned)(per_cu * ((ba.num_cu + 7) / 8 * 8))
+                                  : (unsigned)(scan_grid * kReestWG));
     switch (ba.EX) {
     case 2: hipLaunchKernelGGL(reest_kernel<2>, grid, dim3(256), lds, s, ba, scan_grid); break;
     case 3: hipLaunchKernelGGL(reest_kernel<3>, grid, dim3(256), lds, s, ba, scan_grid); break;

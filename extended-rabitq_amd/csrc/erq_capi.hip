@@ -717,6 +717,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.surv_cap = h->batch_surv_cap;
         ba.stats = reinterpret_cast<unsigned long long*>(stats);
         ba.cand_cap = cap;
+        ba.num_cu = h->num_cu;
         ba.D = (int)h->D;
         ba.EX = (int)h->EX;
         ba.k = (int)k;

@@ -143,16 +143,16 @@ __device__ __forceinline__ uint32_t pack_low_bytes(uint32_t a, uint32_t b, uint3
 }
 
 // The reference's per-(query, cluster) preparation (normalize_query16 and the digit scale
-// of high_acc_quantize16, space.hpp:695-778) by one wave: u = (q - c) * (1 / y) stored
-// lane-major in uq (stride S), sumq = the sum of u in _mm512 lane order (16 lane-strided
-// chains, then butterfly16), delta = max |u| / 8191.  D is a multiple of 64; q and c are
-// 16-byte aligned.
-__device__ __forceinline__ void prepare_residual(const float* __restrict__ q,
-                                                 const float* __restrict__ c, float y, int D,
-                                                 int S, float* uq, int lane, float& fac,
-                                                 float& sumq, float& delta) {
+// of high_acc_quantize16, space.hpp:695-778) by one wave, in two parts.
+// residual_delta: u = (q - c) * (1 / y) stored lane-major in uq (stride S), or u = 1/sqrt(D)
+// when y <= 1e-5 (then returns false), and delta = max |u| / 8191.  D is a multiple of 64;
+// q and c are 16-byte aligned.
+__device__ __forceinline__ bool residual_delta(const float* __restrict__ q,
+                                               const float* __restrict__ c, float y, int D, int S,
+                                               float* uq, int lane, float& fac, float& delta) {
     float mx = 0.f, mn = 0.f;
-    if (y > 1e-5f) {
+    const bool res = y > 1e-5f;
+    if (res) {
         fac = (float)(1.0 / (double)y);
         for (int d = 4 * lane; d < D; d += 256) {
             const float4 qa = *reinterpret_cast<const float4*>(q + d);
@@ -166,31 +166,47 @@ __device__ __forceinline__ void prepare_residual(const float* __restrict__ q,
             uq[ul_index(d + 2, S)] = u2;
             uq[ul_index(d + 3, S)] = u3;
         }
-        wave_sync();
-        float part = 0.f;
-        if (lane < 16) {
-            const float4* row = reinterpret_cast<const float4*>(uq + lane * S);
-            for (int i = 0; i < (D >> 6); i++) {
-                const float4 v = row[i];
-                part = part + v.x;
-                part = part + v.y;
-                part = part + v.z;
-                part = part + v.w;
-            }
-        }
-        part = butterfly16(part);
-        sumq = __shfl(part, 0);
     } else {
         fac = 0.f;
         const float value = (float)(1.0 / (double)sqrtf((float)D));
         for (int d = lane; d < D; d += kWave) uq[ul_index(d, S)] = value;
         mx = value;
-        wave_sync();
-        sumq = (float)D * value;
     }
     mx = wave_max(mx);
     mn = wave_min(mn);
     delta = fmaxf(fabsf(mn), fabsf(mx)) / 8191.0f;
+    return res;
+}
+
+// sumq: the sum of u in _mm512 lane order (16 lane-strided chains over the lane-major rows,
+// then butterfly16), by the 16 lanes of a group (l = lane & 15); every lane of the group
+// gets it.  When y <= 1e-5 the reference takes D / sqrt(D) instead (prepare_residual).
+__device__ __forceinline__ float residual_sum16(const float* uq, int S, int D, int l) {
+    const float4* row = reinterpret_cast<const float4*>(uq + l * S);
+    float part = 0.f;
+    for (int i = 0; i < (D >> 6); i++) {
+        const float4 v = row[i];
+        part = part + v.x;
+        part = part + v.y;
+        part = part + v.z;
+        part = part + v.w;
+    }
+    return butterfly16(part);
+}
+
+// Both parts by one wave (lanes 0..15 run the sum).
+__device__ __forceinline__ void prepare_residual(const float* __restrict__ q,
+                                                 const float* __restrict__ c, float y, int D,
+                                                 int S, float* uq, int lane, float& fac,
+                                                 float& sumq, float& delta) {
+    const bool res = residual_delta(q, c, y, D, S, uq, lane, fac, delta);
+    wave_sync();
+    if (res) {
+        const float part = residual_sum16(uq, S, D, lane & 15);
+        sumq = __shfl(part, 0);
+    } else {
+        sumq = (float)D * (float)(1.0 / (double)sqrtf((float)D));
+    }
 }
 
 // The int8 MFMA digits of the quantized residual (digit 0: q mod 128, digit 1: floor(q /
