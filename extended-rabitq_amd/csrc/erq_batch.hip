@@ -2,23 +2,26 @@
 //
 // Many queries with uneven probe lists are searched together.  Probe ranks are split into
 // rounds ([0,4) [4,16) [16,64) [64,192), then 128 wide).  Round 0 is the exact per-query
-// scan (erq_search.hip).  Every later round:
-//   1. buckets the round's (query, rank) pairs by cluster (count / scan / scatter),
-//   2. runs one persistent kernel over (cluster, <= 8 queries) work items, so each
-//      cluster's codes are read once per round for all the queries that probe it,
-//   3. merges each query's new candidates into its top-k pool.
+// scan (erq_search.hip).  The later rounds:
+//   1. bucket every round's (query, rank) pairs by (round, cluster) at once (count / bin
+//      scans / scatter) and prepare every pair's quantized residual (prep_kernel),
+//   2. per round, run one persistent kernel over (cluster, <= 16 entries) work items, so
+//      each cluster's codes are read once per round for all the queries that probe it,
+//      then re-estimate the survivors (reest_kernel),
+//   3. merge each query's new candidates into its top-k pool.
 // Within a round the pruning threshold of a query is fixed (its k-th distance at round
 // start), so the candidate set -- and the merged result -- does not depend on scheduling.
 //
 // The FastScan integer I = <x_b, q> (q: the 14-bit quantized query residual of
-// high_acc_quantize16, space.hpp:695-711) is computed with v_mfma_i32_16x16x64_i8:
+// high_acc_quantize16, space.hpp:695-711; q = 128 hi + lo, lo in [0,127]) on int8 MFMA:
 //   * at load, the 1-bit codes are re-laid out per 512-dim chunk as 64 bytes per vector,
 //     byte B bit p = dim 512c + 8B + p ("bit-plane" layout);
-//   * a lane holds 16 code bytes of one vector; bit-plane p of them, masked in place
-//     (one v_and per dword, values 0 or 2^p), is the B operand of MFMA p;
-//   * the A operand holds, for 8 queries x 2 planes (q = 128*hi + lo, lo in [0,127]),
-//     the int8 digits of the dims that bit-plane p of those bytes covers;
-//   * I = sum_p (acc_p >> p) (p = 7 enters with the sign flipped: 0x80 is -128).
+//   * main pass, v_mfma_i32_32x32x32_i8: B = 16 code bytes of one vector per lane taken as
+//     bit-plane p ((byte >> p) & 1), A = the digits (16 entries x 2 rows) of the dims that
+//     plane covers; all planes and chunks accumulate into one integer tile;
+//   * inline pass, v_mfma_i32_16x16x64_i8 (8 entries x 2 rows): B = the bytes masked in
+//     place (one v_and per dword, values 0 or 2^p), I = sum_p (acc_p >> p), plane 7 with
+//     the sign flipped (0x80 is -128).
 // The integer equals the reference's S + shift whenever its u16 lanes do not wrap
 // (always for D <= 2048).  Lower bounds and EX-bit re-estimates use the same formulas,
 // in the same order, as the per-query kernel.
