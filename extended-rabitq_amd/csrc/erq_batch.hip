@@ -801,13 +801,12 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
     const float value = (float)(1.0 / (double)sqrtf((float)D));
     unsigned long long st_surv = 0, st_cand = 0;
     uint32_t staged = 0xFFFFFFFFu;
-    QParam P{};
+    QParam P{}, Pn{};
+    uint32_t pn_e = 0xFFFFFFFFu;  // entry whose parameters are in Pn
     int sl = -1;  // slice of this group's record (records only move forward)
-    for (size_t r = r_begin; r < r_end;) {  // wave-uniform
-        const size_t rr = r + lg;
-        const bool inr = rr < r_end;
-        uint4 rec = make_uint4(0xFFFFFFFFu, 0, 0, 0);
-        if (inr) {
+    auto load_rec = [&](size_t rr) {
+        uint4 v = make_uint4(0xFFFFFFFFu, 0, 0, 0);
+        if (rr < r_end) {
             if (sl < 0) {  // the last i with pre[i] <= rr
                 int lo = 0, hi = ns - 1;
                 while (lo < hi) {
@@ -817,15 +816,25 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
                 sl = lo;
             }
             while (pre[sl + 1] <= (uint32_t)rr) sl++;
-            rec = a.surv[(size_t)(x + xs * sl) * slice_cap + (rr - pre[sl])];
+            v = a.surv[(size_t)(x + xs * sl) * slice_cap + (rr - pre[sl])];
         }
+        return v;
+    };
+    // Each step's records, and the next run's parameters when that run starts inside this
+    // step, are loaded one step ahead.
+    uint4 rec = load_rec(r_begin + lg);
+    for (size_t r = r_begin; r < r_end;) {  // wave-uniform
+        const bool inr = r + lg < r_end;
         // the run of group 0's pair: the leading groups whose record has its entry
         const uint32_t e0 = __shfl(rec.x, 0);  // group 0 is always in range
         const uint64_t same = __ballot(inr && rec.x == e0);
         const int m = same == ~0ull ? 4 : __builtin_ctzll(~same) >> 4;
         const bool act = lg < m;
+        const uint4 rec_next = load_rec(r + m + lg);
+        const uint32_t e_next = m < 4 ? __shfl(rec.x, 16 * m) : 0xFFFFFFFFu;
         if (e0 != staged) {
-            P = reinterpret_cast<const QParam*>(a.qparams)[e0];
+            if (e0 != pn_e) P = reinterpret_cast<const QParam*>(a.qparams)[e0];
+            else P = Pn;
             P.thr = a.pool_d[(size_t)P.q * a.k + a.k - 1];
             const float* qv = a.q_rot + (size_t)P.q * D;
             const float* cen = a.centroids + (size_t)P.cid * D;
@@ -845,6 +854,10 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             wave_sync();
             staged = e0;
         }
+        if (e_next != 0xFFFFFFFFu) {  // the next run's first record is in this step
+            Pn = reinterpret_cast<const QParam*>(a.qparams)[e_next];
+            pn_e = e_next;
+        }
         const uint32_t vi = act ? rec.y : 0;
         const float on = act ? __uint_as_float(rec.w) : 0.f;
         const float exd = reestimate<EX>(a, P, [&](int i) { return uw[l16 * S + i]; },
@@ -857,6 +870,7 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             }
             st_surv++;
         }
+        rec = rec_next;
         r += m;
     }
     if (a.stats) {
