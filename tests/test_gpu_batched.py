@@ -133,6 +133,12 @@ def test_batched_d1024_b8():
     _check(idx, q, 100, np.full(q.shape[0], 20), "D960 B8")
 
 
+def test_batched_d2048_b3():
+    """Four 512-dim chunks (the largest MFMA shape: 88 KB of LDS per workgroup)."""
+    idx, q, _ = _build(("d2048", 3), 4000, 2048, 24, 3, nq=16)
+    _check(idx, q, 50, np.full(q.shape[0], 16), "D2048 B3")
+
+
 @pytest.mark.parametrize("cap", [None, 0])
 def test_batched_d1536_b4(cap):
     idx, q, _ = _build(("d1536", 4), 8000, 1536, 48, 4, nq=24)
