@@ -127,10 +127,19 @@ def test_learned_probe_search_end_to_end():
     assert r_learned >= r_fixed - 0.01
     # probe lists are ordered by exact centroid distance and hold distinct clusters
     pid_h, pd_h = pid.cpu().numpy(), pd.cpu().numpy()
+    from oracle import oracle as orc
+    oi = orc.OracleIndex(idx)
+    qte_np = rotate_np(idx, qte)
     for q in range(0, 256, 37):
         seg = slice(offs[q], offs[q + 1])
         assert np.all(np.diff(pd_h[seg]) >= 0)
         assert len(set(pid_h[seg].tolist())) == offs[q + 1] - offs[q]
+        # the distances are the oracle's L2Sqr16, bit for bit
+        all_i, all_d = oi.centroid_topn(qte_np[q], idx.K)
+        want = np.empty(idx.K, np.float32)
+        want[all_i] = all_d
+        np.testing.assert_array_equal(pd_h[seg].view(np.uint32),
+                                      want[pid_h[seg]].view(np.uint32))
     # no confidence -> every query falls back to the fixed list = batched fixed search
     fb_i, fb_d = g.search_learned_dev(m, qte_r, 100, mass=0.95, min_nprobe=2, max_nprobe=48,
                                       fallback_nprobe=16, min_conf=1e30)
