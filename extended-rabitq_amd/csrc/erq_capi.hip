@@ -366,6 +366,12 @@ int erq_probe_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe
 
 static int ensure_bp(erq_index* h, hipStream_t s);
 
+static size_t max_cluster_size(const erq_index* h) {
+    uint64_t m = 0;
+    for (uint64_t v : h->sizes) m = std::max(m, v);
+    return (size_t)m;
+}
+
 static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                        const uint32_t* off, const uint32_t* pid, const float* pd, uint32_t* ids,
                        float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0) {
@@ -405,7 +411,7 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
     a.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
     a.qorder = nullptr;
     if (h->plain_order && nq > 1) {
-        // queries grouped by first probed cluster: the result of each query is unchanged
+        // queries by scan cost, heaviest first: the result of each query is unchanged
         const size_t need = nq + h->K + 1;
         if (need > h->order_cap) {
             if (h->order) {
@@ -420,8 +426,15 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
             if (e != hipSuccess) return hip_fail(e, "order alloc");
             h->order_cap = want;
         }
-        hipError_t e = launch_query_order(off, pid, (int)nq, (int)h->K, h->order,
-                                          h->order + h->K + 1, s);
+        // cost bins: a query scans at most every block (max_probes > 0: at most that many
+        // of the largest clusters); shift so that the bound lands below K
+        size_t bound = h->nblocks;
+        if (max_probes > 0)
+            bound = std::min(bound, (size_t)max_probes * ((max_cluster_size(h) + 31) / 32));
+        int shift = 0;
+        while ((bound >> shift) >= h->K) shift++;
+        hipError_t e = launch_query_order(off, pid, h->csize, (int)nq, (int)h->K, max_probes,
+                                          shift, h->order, h->order + h->K + 1, s);
         if (e != hipSuccess) return hip_fail(e, "query order");
         a.qorder = h->order + h->K + 1;
     }
@@ -583,11 +596,6 @@ static size_t next_round(size_t r0) { return std::min(4 * r0, r0 + 128); }
 
 constexpr size_t kMaxBatchGrid = 4096;    // workgroups of the persistent batched kernel
 
-static size_t max_cluster_size(const erq_index* h) {
-    uint64_t m = 0;
-    for (uint64_t v : h->sizes) m = std::max(m, v);
-    return (size_t)m;
-}
 
 // Queries per chunk: every round of a chunk is prepared ahead, so the chunk's pairs over all
 // ranks past the exact ones must fit the pair budget.

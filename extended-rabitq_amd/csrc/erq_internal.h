@@ -80,8 +80,11 @@ struct ScanArgs {
     const uint32_t* qorder;  // nullable: the order queries are taken in (a permutation of nq)
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
-// qorder[nq] = queries grouped by their first probed cluster; scratch holds K + 1 u32.
-hipError_t launch_query_order(const uint32_t* probe_off, const uint32_t* probe_ids, int nq, int K,
+// qorder[nq] = the queries by scan cost, heaviest first (cost: the 32-vector blocks of the
+// probed clusters, the first max_probes when > 0, shifted right by `shift`, K bins);
+// scratch holds K + 1 u32.
+hipError_t launch_query_order(const uint32_t* probe_off, const uint32_t* probe_ids,
+                              const uint32_t* csize, int nq, int K, int max_probes, int shift,
                               uint32_t* scratch, uint32_t* qorder, hipStream_t s);
 
 // C[m][n] = sum_k A[m][k] * B(k, n); B row-major [K x N] (b_trans = 0) or stored as

@@ -16,26 +16,31 @@ namespace erq {
 
 namespace {
 
-// The query a wave scans.  With an order (queries grouped by first probed cluster), the
-// workgroups on one XCD (workgroup b runs on XCD b % 8) take a contiguous stretch of it,
-// so concurrently scanned queries share probed clusters in that XCD's L2.  -1: none.
+// The query a wave scans: slot blockIdx.x * waves + wid of the order when there is one
+// (queries by scan cost, heaviest first: workgroups start in index order, so the light
+// queries fill the last, partial generation of waves), else the query of that index.
+// -1: none.
 __device__ __forceinline__ int query_of_wave(const ScanArgs& a, int wid) {
-    const int waves = blockDim.x >> 6;
-    if (!a.qorder) {
-        const int qi = blockIdx.x * waves + wid;
-        return qi < a.nq ? qi : -1;
-    }
-    const int nwg = gridDim.x, x = blockIdx.x & 7;
-    int start = 0;  // slots of the XCDs before x
-    for (int y = 0; y < x; y++) start += (nwg - y + 7) / 8;
-    const int slot = (start + (blockIdx.x >> 3)) * waves + wid;
-    return slot < a.nq ? (int)a.qorder[slot] : -1;
+    const int slot = blockIdx.x * (blockDim.x >> 6) + wid;
+    if (slot >= a.nq) return -1;
+    return a.qorder ? (int)a.qorder[slot] : slot;
 }
 
-__device__ __forceinline__ uint32_t first_probe(const uint32_t* off, const uint32_t* pid, int q,
-                                                int K) {
-    const uint32_t s = off[q], e = off[q + 1];
-    return s < e ? min(pid[s], (uint32_t)K) : (uint32_t)K;
+// The order's bin of query q: its scan cost -- the 32-vector blocks of its probed clusters
+// (the first max_probes of them when max_probes > 0) -- shifted right by `shift`, taken
+// from K - 1 so that the heaviest queries come first.
+__device__ __forceinline__ uint32_t order_bin(const uint32_t* off, const uint32_t* pid,
+                                              const uint32_t* csize, int q, int K, int max_probes,
+                                              int shift) {
+    const uint32_t s = off[q];
+    uint32_t e = off[q + 1];
+    if (max_probes > 0) e = min(e, s + (uint32_t)max_probes);
+    uint32_t blocks = 0;
+    for (uint32_t p = s; p < e; p++) {
+        const uint32_t c = pid[p];
+        if (c < (uint32_t)K) blocks += (csize[c] + 31) >> 5;
+    }
+    return (uint32_t)K - 1 - min(blocks >> shift, (uint32_t)K - 1);
 }
 
 __global__ void order_zero_kernel(uint32_t* p, int n) {
@@ -43,9 +48,10 @@ __global__ void order_zero_kernel(uint32_t* p, int n) {
 }
 
 __global__ void order_count_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pid,
-                                   int nq, int K, uint32_t* __restrict__ counts) {
+                                   const uint32_t* __restrict__ csize, int nq, int K,
+                                   int max_probes, int shift, uint32_t* __restrict__ counts) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nq) atomicAdd(&counts[first_probe(off, pid, q, K)], 1u);
+    if (q < nq) atomicAdd(&counts[order_bin(off, pid, csize, q, K, max_probes, shift)], 1u);
 }
 
 // Single workgroup: counts[0..n) -> exclusive prefix sums, in place.
@@ -73,10 +79,14 @@ __global__ void __launch_bounds__(1024) order_scan_kernel(uint32_t* counts, int 
 }
 
 __global__ void order_scatter_kernel(const uint32_t* __restrict__ off,
-                                     const uint32_t* __restrict__ pid, int nq, int K,
-                                     uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+                                     const uint32_t* __restrict__ pid,
+                                     const uint32_t* __restrict__ csize, int nq, int K,
+                                     int max_probes, int shift, uint32_t* __restrict__ cursor,
+                                     uint32_t* __restrict__ order) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nq) order[atomicAdd(&cursor[first_probe(off, pid, q, K)], 1u)] = (uint32_t)q;
+    if (q < nq)
+        order[atomicAdd(&cursor[order_bin(off, pid, csize, q, K, max_probes, shift)], 1u)] =
+            (uint32_t)q;
 }
 
 
@@ -687,16 +697,17 @@ hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
 
 }  // namespace
 
-hipError_t launch_query_order(const uint32_t* probe_off, const uint32_t* probe_ids, int nq, int K,
+hipError_t launch_query_order(const uint32_t* probe_off, const uint32_t* probe_ids,
+                              const uint32_t* csize, int nq, int K, int max_probes, int shift,
                               uint32_t* scratch, uint32_t* qorder, hipStream_t s) {
     if (nq == 0) return hipSuccess;
     const int nb = K + 1;
     hipLaunchKernelGGL(order_zero_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, scratch, nb);
     hipLaunchKernelGGL(order_count_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, probe_off,
-                       probe_ids, nq, K, scratch);
+                       probe_ids, csize, nq, K, max_probes, shift, scratch);
     hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, s, scratch, nb);
     hipLaunchKernelGGL(order_scatter_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, probe_off,
-                       probe_ids, nq, K, scratch, qorder);
+                       probe_ids, csize, nq, K, max_probes, shift, scratch, qorder);
     return hipGetLastError();
 }
 

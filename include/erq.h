@@ -51,9 +51,9 @@ enum {
      * (default 2^26).  Survivors beyond it are re-estimated by an inline pass of the scan
      * instead of by the separate re-estimation kernel; results are identical. */
     ERQ_OPT_BATCH_SURVIVOR_CAP = 4,
-    /* Order in which the per-query scan takes queries: 1 (default) grouped by first probed
-     * cluster, so queries scanned at the same time share clusters in L2; 0 as given.
-     * Results are identical. */
+    /* Order in which the per-query scan takes queries: 1 (default) by scan cost (blocks of
+     * the probed clusters), heaviest first, so that light queries fill the last, partial
+     * generation of waves; 0 as given.  Results are identical. */
     ERQ_OPT_PLAIN_ORDER = 5,
     /* (query, cluster) pairs the batched search prepares ahead, 1024..2^26 (default 2^23;
      * 3 KB of digits each at D = 1536).  Every round of a chunk of queries is prepared at

@@ -219,7 +219,7 @@ def test_centroid_distance_gemm():
 
 
 def test_query_order_does_not_change_results():
-    """ERQ_OPT_PLAIN_ORDER (queries taken grouped by first probed cluster) only reorders
+    """ERQ_OPT_PLAIN_ORDER (queries taken by scan cost, heaviest first) only reorders
     work: ids, distances and counters equal those of the order as given."""
     from erq.gpu import IVF
     torch = _torch()
