@@ -126,13 +126,12 @@ __device__ __forceinline__ int ul_index(int d, int S) { return (d & 15) * S + (d
 // u / delta, and a quotient of two floats never lies closer than 2^-49 (relative) to a
 // float rounding boundary without being the float itself, so rounding the product to
 // float gives exactly the IEEE float division.
-// The reference's tmp + 0.5 - [tmp < 0] is taken as one addition of +-0.5: for |tmp| >=
-// 2^-29 both forms are exact in double, below it both truncate to 0 (every float with
-// |tmp| <= 8192.5 checked).
+// The reference's (int)(tmp + 0.5 - [tmp < 0]), evaluated in double, rounds half away from
+// zero: it equals roundf(tmp) for every float |tmp| <= 8192.5 (|tmp| <= 8191 here; checked
+// for every float in that range), and roundf needs no double arithmetic.
 __device__ __forceinline__ int quantize_digit(float u, double rd) {
     const float tmp = (float)((double)u * rd);
-    const double r = (double)tmp + (tmp < 0.0f ? -0.5 : 0.5);
-    return (int)r;
+    return (int)roundf(tmp);
 }
 
 // Byte 0 of each of a, b, c, d as the bytes 0..3 of one word.

@@ -5,8 +5,13 @@ The claim: the double product is within 2^-52 (relative) of the exact quotient, 
 quotient of two floats is never within 2^-49 (relative) of a float rounding boundary
 unless it is exactly representable, so both round to the same float.  Checked here
 against numpy's IEEE float32 division on random residual scales and on quotients placed
-a few ulps either side of every kind of half-integer, where the digit flips.
+a few ulps either side of every kind of half-integer, where the digit flips.  The kernels
+then round the quotient with roundf; that it equals the reference's rounding in double is
+checked for every float in range, through a numpy form of roundf that is itself checked
+against the C library's.
 """
+import ctypes
+import ctypes.util
 import numpy as np
 
 
@@ -77,3 +82,37 @@ def test_extremes():
         u = np.array([value, -value, 0.0], np.float32)
         _check(u, delta)
         assert list(_digits((u / delta).astype(np.float32))) == [8191, -8191, 0]
+
+
+def test_roundf_equals_double_rounding_exhaustive():
+    # the kernels round the quotient with roundf (half away from zero); the reference adds
+    # 0.5 - [tmp < 0] in double and truncates.  Every non-negative float up to 8192.5;
+    # both forms are odd functions of tmp, so the negative side mirrors it (sampled below).
+    lim = int(np.float32(8192.5).view(np.uint32))
+    step = 1 << 26
+    for s in range(0, lim + 1, step):
+        t = np.arange(s, min(lim + 1, s + step), dtype=np.uint32).view(np.float32)
+        r = np.trunc(t)
+        got = (r + (np.abs(t - r) >= np.float32(0.5)).astype(np.float32)).astype(np.int64)
+        np.testing.assert_array_equal(got, _digits(t))
+    rng = np.random.default_rng(14)
+    t = (rng.random(1_000_000) * -8192.5).astype(np.float32)
+    h = -(np.arange(0, 8192) + 0.5).astype(np.float32)
+    t = np.concatenate([t, h, np.nextafter(h, np.float32(0)), np.nextafter(h, np.float32(-np.inf))])
+    r = np.trunc(t)
+    got = (r - (np.abs(t - r) >= np.float32(0.5)).astype(np.float32)).astype(np.int64)
+    np.testing.assert_array_equal(got, _digits(t))
+
+
+def test_numpy_roundf_is_libm_roundf():
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.roundf.restype = ctypes.c_float
+    libm.roundf.argtypes = [ctypes.c_float]
+    rng = np.random.default_rng(15)
+    h = (np.arange(-8192, 8192) + 0.5).astype(np.float32)
+    t = np.concatenate([(rng.random(4000) * 16385 - 8192.5).astype(np.float32), h,
+                        np.nextafter(h, np.float32(np.inf)), np.nextafter(h, np.float32(-np.inf))])
+    r = np.trunc(t)
+    mine = r + np.where(np.abs(t - r) >= np.float32(0.5), np.sign(t), np.float32(0))
+    want = np.array([libm.roundf(float(v)) for v in t], np.float32)
+    np.testing.assert_array_equal(mine, want)
