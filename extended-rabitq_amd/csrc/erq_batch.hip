@@ -309,8 +309,9 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
             P.cid = cid;
             P.sqr_y = a.probe_sqr_y[slot];
             P.y = sqrtf(P.sqr_y);
-            res = residual_delta(a.q_rot + (size_t)P.q * D, a.centroids + (size_t)cid * D, P.y,
-                                 D, S, uq, lane, P.fac, delta);
+            // three slices in flight: the entry's rows arrive in two waits, not six (D = 1536)
+            res = residual_delta<3>(a.q_rot + (size_t)P.q * D, a.centroids + (size_t)cid * D,
+                                    P.y, D, S, uq, lane, P.fac, delta);
         }
         __syncthreads();
         // the four residual sums by one wave (16 lanes per entry; the wave rotates), while

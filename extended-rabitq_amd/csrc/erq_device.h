@@ -145,7 +145,10 @@ __device__ __forceinline__ uint32_t pack_low_bytes(uint32_t a, uint32_t b, uint3
 // of high_acc_quantize16, space.hpp:695-778) by one wave, in two parts.
 // residual_delta: u = (q - c) * (1 / y) stored lane-major in uq (stride S), or u = 1/sqrt(D)
 // when y <= 1e-5 (then returns false), and delta = max |u| / 8191.  D is a multiple of 64;
-// q and c are 16-byte aligned.
+// q and c are 16-byte aligned.  NL: 256-dim slices of q and c loaded before any is used
+// (loads past D are clamped into the row and dropped; max and min do not depend on the
+// order the slices are taken in).
+template <int NL = 1>
 __device__ __forceinline__ bool residual_delta(const float* __restrict__ q,
                                                const float* __restrict__ c, float y, int D, int S,
                                                float* uq, int lane, float& fac, float& delta) {
@@ -153,17 +156,28 @@ __device__ __forceinline__ bool residual_delta(const float* __restrict__ q,
     const bool res = y > 1e-5f;
     if (res) {
         fac = (float)(1.0 / (double)y);
-        for (int d = 4 * lane; d < D; d += 256) {
-            const float4 qa = *reinterpret_cast<const float4*>(q + d);
-            const float4 ca = *reinterpret_cast<const float4*>(c + d);
-            const float u0 = (qa.x - ca.x) * fac, u1 = (qa.y - ca.y) * fac;
-            const float u2 = (qa.z - ca.z) * fac, u3 = (qa.w - ca.w) * fac;
-            mx = fmaxf(fmaxf(mx, u0), fmaxf(u1, fmaxf(u2, u3)));
-            mn = fminf(fminf(mn, u0), fminf(u1, fminf(u2, u3)));
-            uq[ul_index(d, S)] = u0;
-            uq[ul_index(d + 1, S)] = u1;
-            uq[ul_index(d + 2, S)] = u2;
-            uq[ul_index(d + 3, S)] = u3;
+        for (int d0 = 4 * lane; d0 < D; d0 += 256 * NL) {
+            float4 qa[NL], ca[NL];
+#pragma unroll
+            for (int j = 0; j < NL; j++) {
+                const int d = min(d0 + 256 * j, D - 4);
+                qa[j] = *reinterpret_cast<const float4*>(q + d);
+                ca[j] = *reinterpret_cast<const float4*>(c + d);
+            }
+#pragma unroll
+            for (int j = 0; j < NL; j++) {
+                const int d = d0 + 256 * j;
+                if (d < D) {
+                    const float u0 = (qa[j].x - ca[j].x) * fac, u1 = (qa[j].y - ca[j].y) * fac;
+                    const float u2 = (qa[j].z - ca[j].z) * fac, u3 = (qa[j].w - ca[j].w) * fac;
+                    mx = fmaxf(fmaxf(mx, u0), fmaxf(u1, fmaxf(u2, u3)));
+                    mn = fminf(fminf(mn, u0), fminf(u1, fminf(u2, u3)));
+                    uq[ul_index(d, S)] = u0;
+                    uq[ul_index(d + 1, S)] = u1;
+                    uq[ul_index(d + 2, S)] = u2;
+                    uq[ul_index(d + 3, S)] = u3;
+                }
+            }
         }
     } else {
         fac = 0.f;
