@@ -123,13 +123,15 @@ __device__ __forceinline__ uint32_t round_of_rank(uint32_t rank) {
 }
 
 // Entries of all rounds at once, binned by (round, cluster): bin = round * K + cluster.
+// One wave per query (workgroups of 256 = 4 queries); the lanes take its probes, so the
+// atomics of one query are in flight together.
 __global__ void count_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pid,
                              int nq, int r_first, int r_end, int K,
                              uint32_t* __restrict__ counts) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (q >= nq) return;
     const uint32_t s = off[q], e = min(off[q + 1], s + (uint32_t)r_end);
-    for (uint32_t p = s + (uint32_t)r_first; p < e; p++) {
+    for (uint32_t p = s + (uint32_t)r_first + lane; p < e; p += 64) {
         const uint32_t c = pid[p];
         if (c < (uint32_t)K) atomicAdd(&counts[round_of_rank(p - s) * K + c], 1u);
     }
@@ -257,13 +259,14 @@ __global__ void __launch_bounds__(256) bin_items_kernel(const uint32_t* __restri
     }
 }
 
+// One wave per query, as count_kernel.
 __global__ void scatter_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pid,
                                int nq, int r_first, int r_end, int K,
                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ entries) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (q >= nq) return;
     const uint32_t s = off[q], e = min(off[q + 1], s + (uint32_t)r_end);
-    for (uint32_t p = s + (uint32_t)r_first; p < e; p++) {
+    for (uint32_t p = s + (uint32_t)r_first + lane; p < e; p += 64) {
         const uint32_t c = pid[p];
         if (c >= (uint32_t)K) continue;
         const uint32_t pos = atomicAdd(&cursor[round_of_rank(p - s) * K + c], 1u);
@@ -1143,7 +1146,7 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
     if (R < 1 || R > kMaxRounds) return hipErrorInvalidValue;
     hipError_t e = launch_zero(rb.counts, (size_t)K * R, s);
     if (e != hipSuccess) return e;
-    dim3 qg((nq + 255) / 256), qb(256);
+    dim3 qg((nq + 3) / 4), qb(256);  // a wave per query
     hipLaunchKernelGGL(count_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
                        (int)kFirstRoundRank, r_end, K, rb.counts);
     const int nb = K * R, nblk = (nb + 255) / 256;
