@@ -99,7 +99,7 @@ def build_workload(a, torch):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the search runs on one GPU (no multi-GPU path); --gpus must be 1"}))
         sys.exit(2)
     import torch
 

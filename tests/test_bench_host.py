@@ -1,0 +1,49 @@
+"""bench.py's host-side contract, on the CPU: the one-GPU refusal, the roofline arithmetic and
+the traffic figure it reads from the committed counter profile."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench(monkeypatch, argv=()):
+    sys.path.insert(0, REPO)
+    import bench
+    monkeypatch.setattr(sys, "argv", ["bench.py", *argv])
+    return bench, bench.parse()
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=REPO)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in line["error"]
+
+
+def test_scan_roofline_arithmetic(monkeypatch):
+    bench, _ = _bench(monkeypatch)
+    # blocks x (4D + 128) + survivors x (D*EX/8 + 8), over the scan time, against 8 TB/s
+    stats = [68441459, 2131345010, 11944812, 3840000, 0, 0, 0, 0]
+    r = bench.scan_roofline(stats, 73.598, 1536, 3)
+    assert r["algorithmic_bytes"] == 68441459 * 6272 + 11944812 * 584 == 436240601056
+    assert r["achieved"] == pytest.approx(436240601056 / 0.073598 / 1e9, abs=0.1)
+    assert r["frac"] == pytest.approx(r["achieved"] / 8000.0, abs=1e-4)
+    assert (r["bound"], r["unit"], r["peak"]) == ("hbm", "GB/s", 8000.0)
+
+
+def test_profiled_traffic_reads_committed_profile(monkeypatch):
+    bench, a = _bench(monkeypatch)
+    traffic, src = bench.profiled_traffic(a, 192)
+    assert src == "profiles/scan_mfma_kernel_plain_d1536_n1m_b4_nprobe192.md"
+    text = open(os.path.join(REPO, src)).read()
+    assert f"traffic_bytes_per_launch: {traffic}" in text
+    # FETCH_SIZE (KiB, x2 copy calibration) + WRITE_SIZE (KiB, x1) of the same launch
+    assert traffic > 436240601056  # at least the algorithmic bytes
+    # another workload has no committed profile
+    monkeypatch.setattr(a, "nq", 1000)
+    assert bench.profiled_traffic(a, 192) == (None, None)
