@@ -11,7 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
-#include <queue>
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -28,11 +28,35 @@ inline int byte_of_vec(int v) {
     return w < 8 ? 2 * w : 2 * (w - 8) + 1;
 }
 
+// Per-thread buffers of the rescale search.
+struct Scratch {
+    struct Ev {
+        double t;
+        uint32_t i;
+    };
+    std::vector<int> cur;
+    std::vector<Ev> ev, sorted, late;
+    std::vector<uint32_t> cnt;
+    explicit Scratch(size_t D) : cur(D) {}
+};
+
+inline bool ev_less(const Scratch::Ev& a, const Scratch::Ev& b) {
+    return a.t < b.t || (a.t == b.t && a.i < b.i);
+}
+
 // Critical-point search for the rescale factor t maximising <o_bar(t)+1/2, o>/||o_bar(t)+1/2||
 // over o_bar(t)_i = floor(t*o_i + eps) clipped to 2^EX - 1; o holds |u| (all >= 0).
 // Writes the codes and returns 1/<o_bar + 1/2, o> (1.0 if not finite).
-float quantize_magnitudes(const float* o, size_t D, int ex_bits, uint8_t* code,
-                          std::vector<int>& cur, std::vector<std::pair<double, size_t>>& heap_buf) {
+//
+// DataQuantizer::fast_quantize pops its events -- t = (v+1)/o_i, the next breakpoint of
+// dimension i -- from a min-heap of (t, i), pushing a dimension's next breakpoint when it
+// pops the previous one (while below t_end).  A dimension's breakpoints increase with v, so
+// the pops are every event it ever holds in ascending (t, i) order.  Here those events are
+// listed up front and put in that order directly: a counting sort on a bucket of t that is
+// monotone in t, then insertion sort by (t, i) (each bucket holds a few), with the first
+// events at or past t_end (small magnitudes only) sorted on their own and taken last.  The
+// sweep then makes the same updates in the same order as the heap.
+float quantize_magnitudes(const float* o, size_t D, int ex_bits, uint8_t* code, Scratch& s) {
     const double eps = 1e-5;
     const int n_enum = 10;
     const int top = (1 << ex_bits) - 1;
@@ -51,34 +75,66 @@ float quantize_magnitudes(const float* o, size_t D, int ex_bits, uint8_t* code,
 
     double sqr_den = D * 0.25;
     double num = 0;
+    s.ev.clear();
+    s.late.clear();
     for (size_t i = 0; i < D; i++) {
-        cur[i] = int(t_start * o[i] + eps);
-        sqr_den += cur[i] * cur[i] + cur[i];
-        num += (cur[i] + 0.5) * o[i];
+        const int c = int(t_start * o[i] + eps);
+        s.cur[i] = c;
+        sqr_den += c * c + c;
+        num += (c + 0.5) * o[i];
+        const double oi = o[i];
+        const double t0 = (double)(c + 1) / oi;  // pushed whatever its value
+        if (t0 < t_end)
+            s.ev.push_back({t0, (uint32_t)i});
+        else
+            s.late.push_back({t0, (uint32_t)i});
+        for (int v = c + 1; v < top; v++) {  // pushed while below t_end
+            const double tn = (double)(v + 1) / oi;
+            if (!(tn < t_end)) break;
+            s.ev.push_back({tn, (uint32_t)i});
+        }
     }
-    using Ev = std::pair<double, size_t>;
-    heap_buf.clear();
-    for (size_t i = 0; i < D; i++) heap_buf.emplace_back((double)(cur[i] + 1) / o[i], i);
-    std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> events(std::greater<Ev>(),
-                                                                       std::move(heap_buf));
+    const size_t n = s.ev.size();
+    s.sorted.resize(n);
+    if (n > 0) {
+        double t_lo = t_end;
+        for (const auto& e : s.ev) t_lo = std::min(t_lo, e.t);
+        const size_t B = 4 * n + 1;
+        const double scale = (double)B / (t_end - t_lo);
+        auto bucket = [&](double t) {
+            const size_t b = (size_t)((t - t_lo) * scale);
+            return b < B ? b : B - 1;
+        };
+        s.cnt.assign(B + 1, 0);
+        for (const auto& e : s.ev) s.cnt[bucket(e.t) + 1]++;
+        for (size_t b = 0; b < B; b++) s.cnt[b + 1] += s.cnt[b];
+        for (const auto& e : s.ev) s.sorted[s.cnt[bucket(e.t)]++] = e;
+        for (size_t k = 1; k < n; k++) {
+            const Scratch::Ev x = s.sorted[k];
+            size_t j = k;
+            while (j > 0 && ev_less(x, s.sorted[j - 1])) {
+                s.sorted[j] = s.sorted[j - 1];
+                j--;
+            }
+            s.sorted[j] = x;
+        }
+    }
+    std::sort(s.late.begin(), s.late.end(), ev_less);
+
     double best_ip = 0, best_t = 0;
-    while (!events.empty()) {
-        Ev e = events.top();
-        events.pop();
-        size_t i = e.second;
-        int v = ++cur[i];
+    auto pop = [&](const Scratch::Ev& e) {
+        const int v = ++s.cur[e.i];
         sqr_den += 2 * v;
-        num += o[i];
-        double ip = num / std::sqrt(sqr_den);
+        num += o[e.i];
+        const double ip = num / std::sqrt(sqr_den);
         if (ip > best_ip) {
             best_ip = ip;
-            best_t = e.first;
+            best_t = e.t;
         }
-        if (v < top) {
-            double tn = (double)(v + 1) / o[i];
-            if (tn < t_end) events.emplace(tn, i);
-        }
-    }
+    };
+    for (const auto& e : s.sorted) pop(e);
+    for (const auto& e : s.late) pop(e);
+
     double numer = 0;
     for (size_t i = 0; i < D; i++) {
         int c = int(best_t * o[i] + eps);
@@ -143,6 +199,13 @@ void compact_code(const uint8_t* raw, size_t D, int ex_bits, uint8_t* out) {
 
 extern "C" {
 
+// The rescale search of one vector: o = |u| (D values) -> codes in [0, 2^EX - 1]; returns
+// 1/<o_bar + 1/2, o> (1.0 if not finite).  The per-vector step of erqb_quantize_cluster.
+float erqb_quantize_magnitudes(const float* o, size_t D, int ex_bits, uint8_t* code) {
+    Scratch s(D);
+    return quantize_magnitudes(o, D, ex_bits, code, s);
+}
+
 // Quantize one cluster: r[n x D] rotated residuals (cluster-local order), onorm[n].
 // blocks: ceil(n/32) * (4D + 128) bytes; long_code: n * D*EX/8; xipnorm: n.
 int erqb_quantize_cluster(const float* r, const float* onorm, size_t n, size_t D, int ex_bits,
@@ -160,9 +223,7 @@ int erqb_quantize_cluster(const float* r, const float* onorm, size_t n, size_t D
 
     std::vector<float> absu(D);
     std::vector<uint8_t> raw(D);
-    std::vector<int> cur(D);
-    std::vector<std::pair<double, size_t>> heap_buf;
-    heap_buf.reserve(D);
+    Scratch scratch(D);
     for (size_t v = 0; v < n * D; v++)
         if (!std::isfinite(r[v])) return -2;
     for (size_t v = 0; v < n; v++)
@@ -188,7 +249,7 @@ int erqb_quantize_cluster(const float* r, const float* onorm, size_t n, size_t D
         reinterpret_cast<float*>(blk + 4 * D)[vb] = onorm[v];
 
         // EX-bit code on |u|, flipped where the sign bit is 0 (Quantizer.hpp:470-488).
-        float ipn = quantize_magnitudes(absu.data(), D, ex_bits, raw.data(), cur, heap_buf);
+        float ipn = quantize_magnitudes(absu.data(), D, ex_bits, raw.data(), scratch);
         xipnorm[v] = ipn * 2 * onorm[v];
         for (size_t d = 0; d < D; d++)
             if (!(x[d] > 0)) raw[d] = (uint8_t)((~raw[d]) & mask);

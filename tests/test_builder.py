@@ -146,3 +146,69 @@ def test_zero_residual_vector_is_quantized():
     r[2, 5] = np.nan
     assert _lib.builder().erqb_quantize_cluster(p(r), p(onorm), n, D, ex, p(blocks), p(longc),
                                                 p(xip)) == -2
+
+
+def _fast_quantize_heap(o, ex):
+    """DataQuantizer::fast_quantize (Quantizer.hpp:138-219) as written: a min-heap of
+    (t, dimension) events, in double precision like the C++."""
+    import heapq
+    import math
+    eps, n_enum, top = 1e-5, 10, (1 << ex) - 1
+    o = [float(v) for v in o]
+    D = len(o)
+    max_o = max(o)
+    t_start = float(top // 3) / max_o
+    t_end = float(top + n_enum) / max_o
+    cur = [int(t_start * v + eps) for v in o]
+    sqr_den = D * 0.25
+    num = 0.0
+    for c, v in zip(cur, o):
+        sqr_den += c * c + c
+        num += (c + 0.5) * v
+    heap = [((c + 1) / v if v > 0 else math.inf, i) for i, (c, v) in enumerate(zip(cur, o))]
+    heapq.heapify(heap)
+    best_ip, best_t = 0.0, 0.0
+    while heap:
+        t, i = heapq.heappop(heap)
+        cur[i] += 1
+        v = cur[i]
+        sqr_den += 2 * v
+        num += o[i]
+        ip = num / math.sqrt(sqr_den)
+        if ip > best_ip:
+            best_ip, best_t = ip, t
+        if v < top and o[i] > 0:
+            tn = (v + 1) / o[i]
+            if tn < t_end:
+                heapq.heappush(heap, (tn, i))
+    codes = [min(int(best_t * v + eps), top) for v in o]
+    numer = 0.0
+    for c, v in zip(codes, o):
+        numer += (c + 0.5) * v
+    ipn = np.float32(1.0 / numer)
+    return np.array(codes, np.uint8), ipn if np.isfinite(ipn) else np.float32(1.0)
+
+
+@pytest.mark.parametrize("ex", [2, 3, 4, 6, 7, 8])
+def test_rescale_search_follows_the_heap_order(ex):
+    """The builder lists the events and orders them instead of keeping a heap; codes and
+    factor equal the heap form bit for bit, with zero magnitudes and many equal breakpoints
+    (magnitudes on a coarse grid, so (v+1)/o ties across dimensions)."""
+    import ctypes
+    from erq import _lib
+    rng = np.random.default_rng(40 + ex)
+    D = 128
+    cases = [np.abs(rng.standard_normal(D)).astype(np.float32) for _ in range(3)]
+    grid = (rng.integers(0, 9, D) / 8.0).astype(np.float32)
+    grid[rng.integers(0, D, 5)] = 0.0
+    cases.append(grid)
+    cases.append(np.full(D, 0.25, np.float32))
+    for o in cases:
+        o = o / np.float32(np.linalg.norm(o.astype(np.float64)))
+        o = o.astype(np.float32)
+        code = np.zeros(D, np.uint8)
+        ipn = _lib.builder().erqb_quantize_magnitudes(
+            o.ctypes.data_as(ctypes.c_void_p), D, ex, code.ctypes.data_as(ctypes.c_void_p))
+        want_code, want_ipn = _fast_quantize_heap(o, ex)
+        np.testing.assert_array_equal(code, want_code)
+        assert np.float32(ipn) == want_ipn
