@@ -4,23 +4,26 @@
   lib/liberq_builder.so  host index quantizer (g++ + OpenMP)
 
 Run directly (``python extended-rabitq_amd/build.py``) or through
-``__graft_entry__.build()``.  Every call recompiles.
+``__graft_entry__.build()``.  Every call recompiles.  The HIP sources are compiled to
+objects in parallel (each translation unit's kernels are self-contained), then linked.
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "lib")
+OBJ = os.path.join(PKG, "_build")
 INCLUDE = os.path.join(REPO, "include")
 
 HIP_SOURCES = ["erq_capi.hip", "erq_search.hip", "erq_gemm.hip", "erq_probe.hip", "erq_batch.hip",
                "erq_mlp.hip"]
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument", "-munsafe-fp-atomics",
     "-ffp-contract=off",
@@ -30,6 +33,10 @@ HIPCC_FLAGS = [
 def _run(cmd):
     print("+", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+
+
+def _hipcc():
+    return os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
 
 
 def build_builder():
@@ -43,10 +50,20 @@ def build_builder():
 
 def build_hip():
     os.makedirs(LIB, exist_ok=True)
+    os.makedirs(OBJ, exist_ok=True)
     out = os.path.join(LIB, "liberq.so")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    hipcc = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
-    _run([hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-I", CSRC, "-o", out + ".tmp"] + srcs)
+    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs = [os.path.join(OBJ, s.replace(".hip", ".o")) for s in srcs]
+
+    def compile_one(pair):
+        src, obj = pair
+        _run([_hipcc()] + HIPCC_FLAGS + ["-I", INCLUDE, "-I", CSRC, "-c", "-o", obj,
+                                         os.path.join(CSRC, src)])
+
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "0") or 0) or os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(compile_one, zip(srcs, objs)))
+    _run([_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
     os.replace(out + ".tmp", out)
     return out
 

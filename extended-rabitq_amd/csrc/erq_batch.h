@@ -36,6 +36,8 @@ struct BatchArgs {
     uint32_t* surv_count;      // per scan workgroup: records in its slice
     size_t surv_cap;
     unsigned long long* stats; // 8 counters (see erq.h)
+    uint32_t* redo;            // the round's queries whose candidates overflowed: [0] count,
+                               // then the queries (rescanned by the round-mode scan)
     int cand_cap;
     int num_cu;                // compute units (sizes the re-estimate's grid)
     int D, EX, k, nq, nchunk;
@@ -52,6 +54,8 @@ struct RoundBufs {
     int K;
 };
 
+// 512-dim chunks of the bit-plane codes: ceil(D / 512), rounded up to 6 or 8 above 4 (the
+// kernels are built for 1, 2, 3, 4, 6 and 8 chunks; the padding chunks' codes and digits are 0).
 int bp_nchunk(int D);
 size_t batch_lds_bytes(int D);         // main scan pass
 size_t batch_inline_lds_bytes(int D);  // inline pass

@@ -372,6 +372,28 @@ __device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P,
     return (float)((double)Aa - H);
 }
 
+// Above D = 2048 (nchunk > 4): a survivor found on the exact FastScan integer I gets the
+// reference's integer (lo_wraps, on the pair's digits qv) and the lower-bound test again,
+// against the threshold in P.  Every lane of the 16-lane group calls it (inactive groups with
+// any valid vector).  Returns whether the vector survives; ipq = its scaled product.
+__device__ __forceinline__ bool wrap_retest(const BatchArgs& a, const QParam& P,
+                                            const int16_t* qv, int vloc, int I, float on, int l,
+                                            float& ipq) {
+    const uint32_t blk = a.bstart[P.cid] + (uint32_t)(vloc >> 5);
+    const int vb = vloc & 31;
+    const int wr = lo_wraps(qv, a.nchunk, l, [&](int c) {
+        return *reinterpret_cast<const uint32_t*>(
+            a.bp + (((size_t)blk * a.nchunk + c) * 32 + vb) * 64 + 4 * l);
+    });
+    ipq = (float)(I - 65536 * wr) * P.delta;
+    float tl = ipq - P.c1;
+    tl = tl * P.c2;
+    tl = tl * on;
+    tl = tl + P.qs;
+    tl = tl + on * on;
+    return tl < P.thr;
+}
+
 __device__ __forceinline__ void add_candidate(const BatchArgs& a, int q, float exd, uint32_t id) {
     const uint32_t cs = atomicAdd(&a.cand_cnt[q], 1u);
     if (cs < (uint32_t)a.cand_cap) {
@@ -396,18 +418,21 @@ __device__ __forceinline__ void load_tile_codes(const uint8_t* bp, uint32_t blk,
 // and chunks accumulate into one 32 x 32 integer tile: lane (h, n) ends with the sums of
 // vector n for rows 8 b + 4 h + i.  Every vector whose lower bound is below its entry's
 // threshold goes to that entry's LDS list; the lists are written to the round's survivor
-// list (one record per (entry, vector): entry, vector, scaled FastScan product, |o - c|)
+// list (one record per (entry, vector): entry, vector, exact FastScan integer, |o - c|)
 // for reest_kernel.  An item with a list beyond kMainSurv, or whose records do not fit in
-// the workgroup's slice, is queued (both halves) for batch_inline_kernel instead.
+// the workgroup's slice, is queued (both halves) for batch_inline_kernel instead.  Up to
+// 4 chunks the next item's digits and both blocks' codes are held in registers ahead of
+// use; above (D > 2048) they are loaded when needed.
 template <int EX, int NCH>
 __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int Dp = NCH * 512;
+    constexpr bool kAhead = NCH <= 4;
     const int D = a.D;
     int8_t* A = reinterpret_cast<int8_t*>(smem);  // [NCH][8 p][4 g][32 rows][16 B]
     uint32_t* s_vec = reinterpret_cast<uint32_t*>(smem + (size_t)NCH * 16384);
-    float* s_ipq = reinterpret_cast<float*>(s_vec + kItemGroup * kMainSurv);
-    float* s_on = s_ipq + kItemGroup * kMainSurv;
+    int* s_int = reinterpret_cast<int*>(s_vec + kItemGroup * kMainSurv);
+    float* s_on = reinterpret_cast<float*>(s_int + kItemGroup * kMainSurv);
     __shared__ QParam qp[kItemGroup];
     __shared__ uint32_t s_cnt[kItemGroup];
 
@@ -434,16 +459,20 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     // The item's digits (16-byte units (c, p, g) x 32 rows; rows of missing entries are 0)
     // pass through registers: the next item's are loaded while this one computes.
     constexpr int kStage = NCH * 1024 / kThreads;
-    uint4 dreg[kStage];
-    auto load_digits = [&](uint32_t item) {
+    constexpr int kHeld = kAhead ? kStage : 1;
+    uint4 dreg[kHeld];
+    auto digit_unit = [&](uint32_t item, int i) {
         const uint32_t f = a.items[3 * item + 1], de0 = f >> 5, delen = f & 31u;
+        const int u = tid + kThreads * i, row = u & 31, cpg = u >> 5;
+        const int qq = row >> 1, dgt = row & 1;
+        return qq < (int)delen ? reinterpret_cast<const uint4*>(
+                                     a.adig + (size_t)(de0 + qq) * 2 * Dp)[cpg * 2 + dgt]
+                               : make_uint4(0, 0, 0, 0);
+    };
+    auto load_digits = [&](uint32_t item) {
+        if constexpr (kAhead) {
 #pragma unroll
-        for (int i = 0; i < kStage; i++) {
-            const int u = tid + kThreads * i, row = u & 31, cpg = u >> 5;
-            const int qq = row >> 1, dgt = row & 1;
-            dreg[i] = qq < (int)delen ? reinterpret_cast<const uint4*>(
-                                            a.adig + (size_t)(de0 + qq) * 2 * Dp)[cpg * 2 + dgt]
-                                      : make_uint4(0, 0, 0, 0);
+            for (int i = 0; i < kStage; i++) dreg[i] = digit_unit(item, i);
         }
     };
     if (it_first < it_hi) load_digits(ifirst + it_first);
@@ -456,19 +485,23 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         const int n = (int)a.csize[cid];
         const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
         const int bend = min((n + 31) >> 5, bfirst + kItemTiles / 2);
-        // this wave's blocks bfirst + wid + 4 j: 16 code bytes per (chunk, K half) per lane
-        uint4 w[2][NCH][2];
+        // block bl's codes: 16 bytes per (chunk, K half) per lane
+        auto load_codes = [&](int bl, uint4 (&wc)[NCH][2]) {
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int bl = bfirst + wid + 4 * j;
-            if (bl < bend) {
+            for (int c = 0; c < NCH; c++) {
+                const uint8_t* src =
+                    a.bp + (((size_t)(b0 + (uint32_t)bl) * NCH + c) * 32 + n32) * 64 + 16 * h;
+                wc[c][0] = *reinterpret_cast<const uint4*>(src);
+                wc[c][1] = *reinterpret_cast<const uint4*>(src + 32);
+            }
+        };
+        // this wave's blocks bfirst + wid + 4 j
+        uint4 w[kAhead ? 2 : 1][NCH][2];
+        if constexpr (kAhead) {
 #pragma unroll
-                for (int c = 0; c < NCH; c++) {
-                    const uint8_t* src =
-                        a.bp + (((size_t)(b0 + (uint32_t)bl) * NCH + c) * 32 + n32) * 64 + 16 * h;
-                    w[j][c][0] = *reinterpret_cast<const uint4*>(src);
-                    w[j][c][1] = *reinterpret_cast<const uint4*>(src + 32);
-                }
+            for (int j = 0; j < 2; j++) {
+                const int bl = bfirst + wid + 4 * j;
+                if (bl < bend) load_codes(bl, w[j]);
             }
         }
         if (tid < kItemGroup) {
@@ -479,8 +512,14 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                 qp[tid] = P;
             }
         }
+        if constexpr (kAhead) {
 #pragma unroll
-        for (int i = 0; i < kStage; i++) reinterpret_cast<uint4*>(A)[tid + kThreads * i] = dreg[i];
+            for (int i = 0; i < kStage; i++)
+                reinterpret_cast<uint4*>(A)[tid + kThreads * i] = dreg[i];
+        } else {
+            for (int i = 0; i < kStage; i++)
+                reinterpret_cast<uint4*>(A)[tid + kThreads * i] = digit_unit(item, i);
+        }
         __syncthreads();
         if (it + it_step < it_hi) load_digits(item + it_step);
 
@@ -488,6 +527,8 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         for (int j = 0; j < 2; j++) {
             const int bl = bfirst + wid + 4 * j;
             if (bl >= bend) break;
+            uint4 (&wj)[NCH][2] = w[kAhead ? j : 0];
+            if constexpr (!kAhead) load_codes(bl, wj);
             // two independent accumulation chains (even and odd planes), summed after:
             // integer addition, so the order does not matter
             v16i acc4[2] = {};
@@ -495,7 +536,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
             for (int c = 0; c < NCH; c++) {
 #pragma unroll
                 for (int sk = 0; sk < 2; sk++) {
-                    const uint4 cw = w[j][c][sk];
+                    const uint4 cw = wj[c][sk];
 #pragma unroll
                     for (int p = 0; p < 8; p++) {
                         const v4i bv = v4i{(int)((cw.x >> p) & 0x01010101u),
@@ -532,7 +573,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                         const uint32_t sidx = atomicAdd(&s_cnt[qq], 1u);
                         if (sidx < (uint32_t)kMainSurv) {
                             s_vec[qq * kMainSurv + sidx] = (uint32_t)vloc;
-                            s_ipq[qq * kMainSurv + sidx] = ipq;
+                            s_int[qq * kMainSurv + sidx] = I;
                             s_on[qq * kMainSurv + sidx] = on;
                         }
                     }
@@ -559,8 +600,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                 for (int jq = 1; jq < kItemGroup; jq++) qq += i >= qoff[jq] ? 1 : 0;
                 const int si = qq * kMainSurv + (i - qoff[qq]);
                 slice[used + i] = make_uint4(e0 + (uint32_t)qq, v0 + s_vec[si],
-                                             __float_as_uint(s_ipq[si]),
-                                             __float_as_uint(s_on[si]));
+                                             (uint32_t)s_int[si], __float_as_uint(s_on[si]));
             }
             used += (size_t)tot;
         } else if (tid == 0) {  // lists or slice full: both halves go to the inline pass
@@ -588,12 +628,15 @@ __global__ void __launch_bounds__(kThreads) batch_inline_kernel(BatchArgs a) {
     // workgroup re-estimates survivors itself, the A region holds each wave's residual
     // (NCH * 8192 >= 4 waves * D floats).
     int8_t* A = reinterpret_cast<int8_t*>(smem);
-    // the A region also holds the inline pass's residuals: 4 waves x 16 x S floats
+    // the A region also holds the inline pass's residuals: 4 waves x 16 x S floats; above
+    // 4 chunks each wave's digits for the wrap correction follow (Dp int16 per wave)
+    constexpr bool kWrap = NCH > 4;
     const int S = ul_stride(D);
     const size_t abytes = max((size_t)NCH * 8192, (size_t)256 * S);
-    uint32_t* s_vec = reinterpret_cast<uint32_t*>(smem + abytes);
-    float* s_ipq = reinterpret_cast<float*>(s_vec + kGroup * kQSurv);
-    float* s_on = s_ipq + kGroup * kQSurv;
+    int16_t* qvw = reinterpret_cast<int16_t*>(smem + abytes) + (size_t)(threadIdx.x >> 6) * Dp;
+    uint32_t* s_vec = reinterpret_cast<uint32_t*>(smem + abytes + (kWrap ? 8 * (size_t)Dp : 0));
+    int* s_int = reinterpret_cast<int*>(s_vec + kGroup * kQSurv);
+    float* s_on = reinterpret_cast<float*>(s_int + kGroup * kQSurv);
     __shared__ QParam qp[kGroup];
     __shared__ uint32_t s_cnt[kGroup];
 
@@ -688,7 +731,7 @@ __global__ void __launch_bounds__(kThreads) batch_inline_kernel(BatchArgs a) {
                 if (vloc < n && tl < P.thr) {
                     const uint32_t sidx = atomicAdd(&s_cnt[qq], 1u);
                     s_vec[qq * kQSurv + sidx] = (uint32_t)vloc;
-                    s_ipq[qq * kQSurv + sidx] = ipq;
+                    s_int[qq * kQSurv + sidx] = I;
                     s_on[qq * kQSurv + sidx] = on;
                 }
             }
@@ -726,20 +769,26 @@ __global__ void __launch_bounds__(kThreads) batch_inline_kernel(BatchArgs a) {
                 for (int d = lane; d < D; d += 64)
                     uw[ul_index(d, S)] = use_res ? (qv[d] - cen[d]) * P.fac : value;
                 wave_sync();
+                if constexpr (kWrap) {
+                    store_qv(uw, S, D, NCH, 1.0 / (double)P.delta, qvw, lane);
+                    wave_sync();
+                }
                 staged = qq;
             }
             const int cnt = qoff[qq + 1] - qoff[qq];
             const int cb = (ci - cstart[qq]) * 16;
             for (int it = 0; it < 4 && cb + 4 * it < cnt; it++) {
                 const int si = cb + 4 * it + lg;
-                const bool act = si < cnt;
+                bool act = si < cnt;
                 const int vloc = act ? (int)s_vec[qq * kQSurv + si] : 0;
                 const uint32_t vi = v0 + (uint32_t)vloc;
                 const float on = act ? s_on[qq * kQSurv + si] : 0.f;
+                const int I = act ? s_int[qq * kQSurv + si] : 0;
+                float ipq = (float)I * P.delta;
+                if constexpr (kWrap) act = wrap_retest(a, P, qvw, vloc, I, on, ln, ipq) && act;
                 const float exd = reestimate<EX>(
                     a, P, [&](int i) { return uw[ln * S + i]; },
-                    a.exl + ((size_t)vi * 16 + ln) * a.exl_w, act,
-                    act ? s_ipq[qq * kQSurv + si] : 0.f, on, vi);
+                    a.exl + ((size_t)vi * 16 + ln) * a.exl_w, act, ipq, on, vi);
                 if (act && ln == 0) {
                     if (exd <= P.thr) {
                         add_candidate(a, P.q, exd, a.ids[vi]);
@@ -770,7 +819,11 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
     const int l16 = lane & 15, lg = lane >> 4;
     const int S = ul_stride(D);  // lane-major residual rows (erq_device.h)
     float* uw = reinterpret_cast<float*>(smem) + (size_t)wid * 16 * S;
-    uint32_t* pre = reinterpret_cast<uint32_t*>(smem + (size_t)256 * S);  // scan_grid + 1
+    // above 4 chunks: each wave's digits of the staged pair, for the wrap correction
+    const bool wrap = a.nchunk > 4;
+    const size_t qv_bytes = wrap ? (size_t)a.nchunk * 1024 : 0;
+    int16_t* qvw = reinterpret_cast<int16_t*>(smem + (size_t)256 * S + (size_t)wid * qv_bytes);
+    uint32_t* pre = reinterpret_cast<uint32_t*>(smem + (size_t)256 * S + 4 * qv_bytes);
     __shared__ uint32_t wsum[4];
     // XCD-aware: workgroup b runs on XCD b % 8, like scan workgroup g on XCD g % 8; the
     // workgroups of one XCD share the slices of that XCD's scan workgroups (clusters of one
@@ -856,18 +909,27 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
                 for (int d = lane; d < D; d += 64) uw[ul_index(d, S)] = value;
             }
             wave_sync();
+            if (wrap) {
+                store_qv(uw, S, D, a.nchunk, 1.0 / (double)P.delta, qvw, lane);
+                wave_sync();
+            }
             staged = e0;
         }
         if (e_next != 0xFFFFFFFFu) {  // the next run's first record is in this step
             Pn = reinterpret_cast<const QParam*>(a.qparams)[e_next];
             pn_e = e_next;
         }
-        const uint32_t vi = act ? rec.y : 0;
+        const uint32_t vi = act ? rec.y : a.vstart[P.cid];
         const float on = act ? __uint_as_float(rec.w) : 0.f;
+        const int I = act ? (int)rec.z : 0;
+        float ipq = (float)I * P.delta;
+        bool keep = act;
+        if (wrap)  // wave-uniform
+            keep = wrap_retest(a, P, qvw, (int)(vi - a.vstart[P.cid]), I, on, l16, ipq) && act;
         const float exd = reestimate<EX>(a, P, [&](int i) { return uw[l16 * S + i]; },
-                                         a.exl + ((size_t)vi * 16 + l16) * a.exl_w, act,
-                                         __uint_as_float(rec.z), on, vi);
-        if (act && l16 == 0) {
+                                         a.exl + ((size_t)vi * 16 + l16) * a.exl_w, keep, ipq,
+                                         on, vi);
+        if (keep && l16 == 0) {
             if (exd <= P.thr) {
                 add_candidate(a, P.q, exd, a.ids[vi]);
                 st_cand++;
@@ -902,19 +964,35 @@ __device__ __forceinline__ void merge_entry(int i, int k, int q, const float* po
     }
 }
 
+// A query whose candidates did not all fit (more than `cap`, or more than kMergeBuf entries
+// at or below the k-th key) keeps its pool unchanged and goes to the round's redo list: the
+// round-mode scan then rescans the round for it with an LDS pool, exactly.
+__device__ __forceinline__ void merge_redo(int q, uint32_t* cand_cnt, uint32_t* redo,
+                                           unsigned long long* stats) {
+    const uint32_t at = atomicAdd(&redo[0], 1u);
+    redo[1 + at] = (uint32_t)q;
+    cand_cnt[q] = 0;
+    if (stats) atomicAdd(&stats[5], 1ull);
+}
+
 __global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
                                                     uint32_t* __restrict__ pool_i, int k,
                                                     uint32_t* __restrict__ cand_cnt,
                                                     const float* __restrict__ cand_d,
                                                     const uint32_t* __restrict__ cand_i,
-                                                    int cap, unsigned long long* stats) {
+                                                    int cap, uint32_t* __restrict__ redo,
+                                                    unsigned long long* stats) {
     __shared__ float sd[kMergeBuf];
     __shared__ uint32_t si[kMergeBuf];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
     const int q = blockIdx.x, tid = threadIdx.x;
     const uint32_t raw = cand_cnt[q];
-    const int nc = (int)min(raw, (uint32_t)cap);
+    if (raw > (uint32_t)cap) {  // uniform
+        if (tid == 0) merge_redo(q, cand_cnt, redo, stats);
+        return;
+    }
+    const int nc = (int)raw;
     if (nc == 0) {
         // no candidates: the pool (sorted by distance) is already in (distance, id) order
         // unless two entries share a distance (a pool from the per-query replay puts the
@@ -989,7 +1067,11 @@ __global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
         }
     }
     __syncthreads();
-    const int m = min((int)sh_cnt, kMergeBuf);
+    if (sh_cnt > (uint32_t)kMergeBuf) {  // uniform: ties at the k-th key beyond the buffer
+        if (tid == 0) merge_redo(q, cand_cnt, redo, stats);
+        return;
+    }
+    const int m = (int)sh_cnt;
     int P2 = 1;
     while (P2 < max(m, k)) P2 <<= 1;
     for (int i = m + tid; i < P2; i += 256) {
@@ -1019,15 +1101,15 @@ __global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
         pool_d[(size_t)q * k + i] = sd[i];
         pool_i[(size_t)q * k + i] = sd[i] < FLT_MAX ? si[i] : 0u;
     }
-    if (tid == 0) {
-        if (raw > (uint32_t)cap && stats) atomicAdd(&stats[5], (unsigned long long)(raw - cap));
-        cand_cnt[q] = 0;
-    }
+    if (tid == 0) cand_cnt[q] = 0;
 }
 
 }  // namespace
 
-int bp_nchunk(int D) { return (D + 511) / 512; }
+int bp_nchunk(int D) {
+    const int c = (D + 511) / 512;
+    return c == 5 ? 6 : (c == 7 ? 8 : c);
+}
 
 hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
                            hipStream_t s) {
@@ -1044,7 +1126,8 @@ size_t batch_lds_bytes(int D) {  // main pass
     return (size_t)bp_nchunk(D) * 16384 + (size_t)kItemGroup * kMainSurv * 12;
 }
 size_t batch_inline_lds_bytes(int D) {
-    return std::max((size_t)bp_nchunk(D) * 8192, (size_t)256 * ul_stride(D)) +
+    const size_t nch = (size_t)bp_nchunk(D);
+    return std::max(nch * 8192, (size_t)256 * ul_stride(D)) + (nch > 4 ? 8 * nch * 512 : 0) +
            (size_t)kGroup * kQSurv * 12;
 }
 
@@ -1080,6 +1163,12 @@ hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename K>
+static hipError_t set_lds(K kernel, size_t bytes) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 template <int EX, int NCH>
 static hipError_t launch_batch_scan_nch(const BatchArgs& ba, int grid, hipStream_t s) {
     const size_t lds = batch_lds_bytes(ba.D), ilds = batch_inline_lds_bytes(ba.D);
@@ -1104,6 +1193,8 @@ static hipError_t launch_batch_scan_ex(const BatchArgs& ba, int grid, hipStream_
     case 2: return launch_batch_scan_nch<EX, 2>(ba, grid, s);
     case 3: return launch_batch_scan_nch<EX, 3>(ba, grid, s);
     case 4: return launch_batch_scan_nch<EX, 4>(ba, grid, s);
+    case 6: return launch_batch_scan_nch<EX, 6>(ba, grid, s);
+    case 8: return launch_batch_scan_nch<EX, 8>(ba, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1121,9 +1212,25 @@ static hipError_t launch_batch_scan(const BatchArgs& ba, int grid, hipStream_t s
 }
 
 static hipError_t launch_reest(const BatchArgs& ba, int scan_grid, hipStream_t s) {
-    const size_t lds = (size_t)256 * ul_stride(ba.D) + 4 * ((size_t)scan_grid + 1);  // residuals, prefix
+    // residuals, digits (wrap correction above 4 chunks), slice prefix
+    const size_t lds = (size_t)256 * ul_stride(ba.D) +
+                       (ba.nchunk > 4 ? 4 * (size_t)ba.nchunk * 1024 : 0) +
+                       4 * ((size_t)scan_grid + 1);
     // as many workgroups as stay resident together (LDS-bound), a multiple of 8: each takes
     // a fixed share of the records, so a second partial wave of workgroups would be a tail
+    if (lds > 65536) {  // above the 64 KB default (large D): raise the kernel's limit
+        hipError_t e = hipSuccess;
+        switch (ba.EX) {
+        case 2: e = set_lds(reest_kernel<2>, lds); break;
+        case 3: e = set_lds(reest_kernel<3>, lds); break;
+        case 4: e = set_lds(reest_kernel<4>, lds); break;
+        case 6: e = set_lds(reest_kernel<6>, lds); break;
+        case 7: e = set_lds(reest_kernel<7>, lds); break;
+        case 8: e = set_lds(reest_kernel<8>, lds); break;
+        default: return hipErrorInvalidValue;
+        }
+        if (e != hipSuccess) return e;
+    }
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, 163840 / (lds + 1024)));
     const dim3 grid(ba.num_cu > 0 ? (unsigned)(per_cu * ((ba.num_cu + 7) / 8 * 8))
                                   : (unsigned)(scan_grid * kReestWG));
@@ -1165,6 +1272,10 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
     all.n_items = ba.n_items + 4 * R;
     const int wpb = 4;
     const size_t pw = prep_wave_bytes(ba.D);
+    if (pw * wpb > 65536) {
+        e = set_lds(prep_kernel, pw * wpb);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(prep_kernel, dim3((unsigned)prep_grid), dim3(64 * wpb), pw * wpb, s, all,
                        (int)pw);
     return hipGetLastError();
@@ -1178,7 +1289,7 @@ hipError_t launch_round_scan(const BatchArgs& ba, const RoundBufs& rb, int grid,
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(merge_kernel, dim3(ba.nq), dim3(256), 0, s, ba.pool_d, rb.pool_i, ba.k,
-                       ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.stats);
+                       ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.redo, ba.stats);
     return hipGetLastError();
 }
 

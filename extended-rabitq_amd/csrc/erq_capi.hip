@@ -2,7 +2,13 @@
 //
 // Owns the resident index (the HA arrays of IVF::load, IVF.hpp:254-293, copied to HBM
 // unchanged), a per-handle stream and a grow-only workspace, and sequences the search
-// stages: rotation GEMM -> centroid GEMM -> probe selection -> per-query scan.
+// stages: rotation GEMM -> centroid GEMM -> probe selection -> per-query scan (or the
+// cluster-major rounds).
+//
+// Every call on a handle holds its mutex, and is ordered after the previous call's device
+// work when the two run on different streams (an event recorded when each call returns), so
+// the workspace and the other shared buffers are never in use by two calls at once.  No
+// exception crosses the ABI: bodies run through guarded().
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -52,12 +58,60 @@ using namespace erq;
 namespace {
 
 constexpr size_t kMaxK = 8192;   // probe selection keeps all centroid keys in LDS
-constexpr size_t kMaxD = 4096;   // per-wave LDS slice of the scan kernel
-constexpr int kCandCap = 8192;   // batched rounds: candidates kept per query per round
+constexpr size_t kMaxD = 4096;   // bit-plane kernels: at most 8 chunks of 512 dims
+constexpr int kCandCap = 8192;   // batched rounds: candidates kept per query per round (more:
+                                 // the query's round is redone by the round-mode scan)
+// Digits prepared ahead by the batched search (2 * Dp bytes per (query, cluster) pair): the
+// pair budget is capped so that they stay within 24 GiB.
+constexpr size_t kDigitBytes = size_t(3) << 33;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 bool ex_ok(size_t ex) { return ex == 2 || ex == 3 || ex == 4 || ex == 6 || ex == 7 || ex == 8; }
+
+// Runs an entry point's body; a C++ exception becomes an error code (a failed host
+// allocation is ERQ_ENOMEM).
+template <typename F>
+int guarded(F&& f) noexcept {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return ERQ_ENOMEM;
+    } catch (const std::exception& ex) {
+        set_error(std::string("internal error: ") + ex.what());
+        return ERQ_EINVAL;
+    } catch (...) {
+        set_error("internal error");
+        return ERQ_EINVAL;
+    }
+}
+
+bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// One call on a handle: holds the handle's mutex; the outermost call on a thread waits for
+// the previous call's work when it ran on another stream, and records its own end.  Under
+// HIP graph capture neither is done: the graph's launch orders the captured work.
+struct Call {
+    erq_index* h;
+    hipStream_t s;
+    std::unique_lock<std::recursive_mutex> lk;
+    bool outer;
+    Call(erq_index* h_, hipStream_t s_) : h(h_), s(s_), lk(h_->mu), outer(h_->depth++ == 0) {
+        if (outer && h->last_valid && h->last_stream != s && !capturing(s))
+            (void)hipStreamWaitEvent(s, h->ev_last, 0);
+    }
+    ~Call() {
+        if (outer && !capturing(s) && hipEventRecord(h->ev_last, s) == hipSuccess) {
+            h->last_stream = s;
+            h->last_valid = true;
+        }
+        h->depth--;
+    }
+};
 
 template <typename T>
 int upload(T** dst, const T* src, size_t count, const char* what) {
@@ -75,9 +129,11 @@ int upload(T** dst, const T* src, size_t count, const char* what) {
 void release(erq_index* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->last_valid) (void)hipEventSynchronize(h->ev_last);
     void* ptrs[] = {h->short_data, h->long_code, h->ex_factor, h->ids,    h->centroids,
                     h->cnorm,      h->P,         h->csize,     h->bstart, h->vstart,
-                    h->ws,         h->bp,        h->order, h->exl};
+                    h->ws,         h->bp,        h->order,     h->exl,    h->hq,
+                    h->hids,       h->hdists,    h->hnp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->aux) {
@@ -86,6 +142,7 @@ void release(erq_index* h) {
     }
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->ev_last) (void)hipEventDestroy(h->ev_last);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -105,6 +162,31 @@ struct Ws {
 // _dev entry points run on the caller's stream exactly as given (0 is the null stream).
 hipStream_t pick(erq_index*, void* s) { return static_cast<hipStream_t>(s); }
 
+// Device buffers of the host-pointer calls: nq rotated queries, nq x k results, nq + 1
+// probe offsets; grown (never shrunk) on demand, so repeated calls allocate nothing.
+int host_bufs(erq_index* h, size_t nq, size_t k) {
+    if (nq <= h->hq_cap && k <= h->hk_cap) return ERQ_OK;
+    const size_t cq = std::max(nq, h->hq_cap), ck = std::max(k, h->hk_cap);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(e, "host buffers sync");
+    void* old[] = {h->hq, h->hids, h->hdists, h->hnp};
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    h->hq = nullptr;
+    h->hids = nullptr;
+    h->hdists = nullptr;
+    h->hnp = nullptr;
+    h->hq_cap = h->hk_cap = 0;
+    e = hipMalloc(&h->hq, cq * h->D * 4);
+    if (e == hipSuccess) e = hipMalloc(&h->hids, cq * ck * 4);
+    if (e == hipSuccess) e = hipMalloc(&h->hdists, cq * ck * 4);
+    if (e == hipSuccess) e = hipMalloc(&h->hnp, (cq + 1) * 4);
+    if (e != hipSuccess) return hip_fail(e, "host-call buffers");
+    h->hq_cap = cq;
+    h->hk_cap = ck;
+    return ERQ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -115,144 +197,187 @@ int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint6
                      const float* P, const float* centroids, const uint8_t* short_data,
                      const uint8_t* long_code, const float* ex_factor, const uint32_t* ids,
                      erq_index** out) {
-    if (!out || !cluster_sizes || !centroids || !short_data || !long_code || !ex_factor || !ids) {
-        set_error("erq_index_create: null argument");
-        return ERQ_EINVAL;
-    }
-    *out = nullptr;
-    if (!ex_ok(EX_BITS)) {
-        set_error("erq_index_create: EX_BITS must be one of 2,3,4,6,7,8");
-        return ERQ_EFORMAT;
-    }
-    if (K == 0 || DIM == 0) {
-        set_error("erq_index_create: empty index");
-        return ERQ_EFORMAT;
-    }
-    const size_t D = align_up(DIM, 64);
-    if (K > kMaxK || D > kMaxD) {
-        set_error("erq_index_create: K > 8192 or padded dim > 4096 is not supported");
-        return ERQ_EUNSUPPORTED;
-    }
-    if (N > 0xFFFFFFFFull) {
-        set_error("erq_index_create: N exceeds 32-bit ids");
-        return ERQ_EUNSUPPORTED;
-    }
-    size_t total = 0, nblocks = 0;
-    std::vector<uint32_t> cs(K), bs(K + 1), vs(K + 1);
-    for (size_t k = 0; k < K; k++) {
-        cs[k] = (uint32_t)cluster_sizes[k];
-        bs[k] = (uint32_t)nblocks;
-        vs[k] = (uint32_t)total;
-        total += cluster_sizes[k];
-        nblocks += (cluster_sizes[k] + 31) / 32;
-    }
-    bs[K] = (uint32_t)nblocks;
-    vs[K] = (uint32_t)total;
-    if (total != N) {
-        set_error("erq_index_create: cluster sizes do not sum to N");
-        return ERQ_EFORMAT;
-    }
-    erq_index* h = new erq_index();
-    hipError_t e = hipGetDevice(&h->device);
-    if (e != hipSuccess) {
-        delete h;
-        return hip_fail(e, "hipGetDevice");
-    }
-    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete h;
-        return hip_fail(e, "hipStreamCreate");
-    }
-    h->N = N;
-    h->DIM = DIM;
-    h->D = D;
-    h->K = K;
-    h->EX = EX_BITS;
-    h->nblocks = nblocks;
-    h->sizes.assign(cluster_sizes, cluster_sizes + K);
-    const size_t BB = 4 * D + 128;
-    int rc;
-    if ((rc = upload(&h->short_data, short_data, nblocks * BB, "upload short data")) ||
-        (rc = upload(&h->long_code, long_code, N * D * EX_BITS / 8, "upload long codes")) ||
-        (rc = upload(&h->ex_factor, ex_factor, N, "upload ex factors")) ||
-        (rc = upload(&h->ids, ids, N, "upload ids")) ||
-        (rc = upload(&h->centroids, centroids, K * D, "upload centroids")) ||
-        (rc = upload(&h->csize, cs.data(), K, "upload sizes")) ||
-        (rc = upload(&h->bstart, bs.data(), K + 1, "upload block starts")) ||
-        (rc = upload(&h->vstart, vs.data(), K + 1, "upload vector starts"))) {
-        release(h);
-        return rc;
-    }
-    if (P && (rc = upload(&h->P, P, D * D, "upload rotation"))) {
-        release(h);
-        return rc;
-    }
-    e = hipMalloc(&h->cnorm, K * sizeof(float));
-    if (e != hipSuccess) {
-        release(h);
-        return hip_fail(e, "alloc centroid norms");
-    }
-    e = launch_row_sqnorm(h->centroids, (int)K, (int)D, (int)D, h->cnorm, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) {
-        release(h);
-        return hip_fail(e, "centroid norms");
-    }
-    *out = h;
-    return ERQ_OK;
+    return guarded([&]() -> int {
+        if (!out || !cluster_sizes || !centroids || !short_data || !long_code || !ex_factor ||
+            !ids) {
+            set_error("erq_index_create: null argument");
+            return ERQ_EINVAL;
+        }
+        *out = nullptr;
+        if (!ex_ok(EX_BITS)) {
+            set_error("erq_index_create: EX_BITS must be one of 2,3,4,6,7,8");
+            return ERQ_EFORMAT;
+        }
+        if (K == 0 || DIM == 0) {
+            set_error("erq_index_create: empty index");
+            return ERQ_EFORMAT;
+        }
+        const size_t D = align_up(DIM, 64);
+        if (K > kMaxK || D > kMaxD) {
+            set_error("erq_index_create: K > 8192 or padded dim > 4096 is not supported");
+            return ERQ_EUNSUPPORTED;
+        }
+        if (N > 0xFFFFFFFFull) {
+            set_error("erq_index_create: N exceeds 32-bit ids");
+            return ERQ_EUNSUPPORTED;
+        }
+        size_t total = 0, nblocks = 0;
+        std::vector<uint32_t> cs(K), bs(K + 1), vs(K + 1);
+        for (size_t k = 0; k < K; k++) {
+            if (cluster_sizes[k] > N) {
+                set_error("erq_index_create: a cluster is larger than N");
+                return ERQ_EFORMAT;
+            }
+            cs[k] = (uint32_t)cluster_sizes[k];
+            bs[k] = (uint32_t)nblocks;
+            vs[k] = (uint32_t)total;
+            total += cluster_sizes[k];
+            nblocks += (cluster_sizes[k] + 31) / 32;
+        }
+        bs[K] = (uint32_t)nblocks;
+        vs[K] = (uint32_t)total;
+        if (total != N) {
+            set_error("erq_index_create: cluster sizes do not sum to N");
+            return ERQ_EFORMAT;
+        }
+        erq_index* h = new erq_index();
+        hipError_t e = hipGetDevice(&h->device);
+        if (e != hipSuccess) {
+            delete h;
+            return hip_fail(e, "hipGetDevice");
+        }
+        e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_last, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            release(h);
+            return hip_fail(e, "hipStreamCreate");
+        }
+        h->N = N;
+        h->DIM = DIM;
+        h->D = D;
+        h->K = K;
+        h->EX = EX_BITS;
+        h->nblocks = nblocks;
+        h->sizes.assign(cluster_sizes, cluster_sizes + K);
+        const size_t BB = 4 * D + 128;
+        int rc;
+        if ((rc = upload(&h->short_data, short_data, nblocks * BB, "upload short data")) ||
+            (rc = upload(&h->long_code, long_code, N * D * EX_BITS / 8, "upload long codes")) ||
+            (rc = upload(&h->ex_factor, ex_factor, N, "upload ex factors")) ||
+            (rc = upload(&h->ids, ids, N, "upload ids")) ||
+            (rc = upload(&h->centroids, centroids, K * D, "upload centroids")) ||
+            (rc = upload(&h->csize, cs.data(), K, "upload sizes")) ||
+            (rc = upload(&h->bstart, bs.data(), K + 1, "upload block starts")) ||
+            (rc = upload(&h->vstart, vs.data(), K + 1, "upload vector starts"))) {
+            release(h);
+            return rc;
+        }
+        if (P && (rc = upload(&h->P, P, D * D, "upload rotation"))) {
+            release(h);
+            return rc;
+        }
+        e = hipMalloc(&h->cnorm, K * sizeof(float));
+        if (e != hipSuccess) {
+            release(h);
+            return hip_fail(e, "alloc centroid norms");
+        }
+        e = launch_row_sqnorm(h->centroids, (int)K, (int)D, (int)D, h->cnorm, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) {
+            release(h);
+            return hip_fail(e, "centroid norms");
+        }
+        *out = h;
+        return ERQ_OK;
+    });
 }
 
 int erq_index_load(const char* path, erq_index** out) {
-    if (!path || !out) {
-        set_error("erq_index_load: null argument");
-        return ERQ_EINVAL;
-    }
-    *out = nullptr;
-    std::ifstream in(path, std::ios::binary);
-    if (!in) {
-        set_error(std::string("erq_index_load: cannot open ") + path);
-        return ERQ_EIO;
-    }
-    uint64_t hdr[4];
-    if (!in.read(reinterpret_cast<char*>(hdr), sizeof(hdr))) {
-        set_error("erq_index_load: truncated header");
-        return ERQ_EIO;
-    }
-    const size_t N = hdr[0], DIM = hdr[1], K = hdr[2], EX = hdr[3];
-    if (!ex_ok(EX) || K == 0 || DIM == 0 || K > (1u << 24) || DIM > (1u << 16)) {
-        set_error("erq_index_load: bad header");
-        return ERQ_EFORMAT;
-    }
-    const size_t D = align_up(DIM, 64);
-    std::vector<uint64_t> sizes(K);
-    if (!in.read(reinterpret_cast<char*>(sizes.data()), K * 8)) {
-        set_error("erq_index_load: truncated cluster sizes");
-        return ERQ_EIO;
-    }
-    size_t total = 0, nblocks = 0;
-    for (auto s : sizes) {
-        total += s;
-        nblocks += (s + 31) / 32;
-    }
-    if (total != N) {
-        set_error("erq_index_load: cluster sizes do not sum to N");
-        return ERQ_EFORMAT;
-    }
-    std::vector<float> P(D * D), C(K * D), ef(N);
-    std::vector<uint8_t> sd(nblocks * (4 * D + 128)), lc(N * D * EX / 8);
-    std::vector<uint32_t> ids(N);
-    bool ok = in.read(reinterpret_cast<char*>(P.data()), P.size() * 4) &&
-              in.read(reinterpret_cast<char*>(C.data()), C.size() * 4) &&
-              in.read(reinterpret_cast<char*>(sd.data()), sd.size()) &&
-              in.read(reinterpret_cast<char*>(lc.data()), lc.size()) &&
-              in.read(reinterpret_cast<char*>(ef.data()), ef.size() * 4) &&
-              in.read(reinterpret_cast<char*>(ids.data()), ids.size() * 4);
-    if (!ok) {
-        set_error("erq_index_load: truncated index body");
-        return ERQ_EIO;
-    }
-    return erq_index_create(N, DIM, K, EX, sizes.data(), P.data(), C.data(), sd.data(), lc.data(),
-                            ef.data(), ids.data(), out);
+    return guarded([&]() -> int {
+        if (!path || !out) {
+            set_error("erq_index_load: null argument");
+            return ERQ_EINVAL;
+        }
+        *out = nullptr;
+        std::ifstream in(path, std::ios::binary);
+        if (!in) {
+            set_error(std::string("erq_index_load: cannot open ") + path);
+            return ERQ_EIO;
+        }
+        in.seekg(0, std::ios::end);
+        const std::streamoff fend = in.tellg();
+        in.seekg(0, std::ios::beg);
+        if (fend < 0) {
+            set_error("erq_index_load: cannot size the file");
+            return ERQ_EIO;
+        }
+        const uint64_t fsize = (uint64_t)fend;
+        uint64_t hdr[4];
+        if (!in.read(reinterpret_cast<char*>(hdr), sizeof(hdr))) {
+            set_error("erq_index_load: truncated header");
+            return ERQ_EIO;
+        }
+        const uint64_t N = hdr[0], DIM = hdr[1], K = hdr[2], EX = hdr[3];
+        if (!ex_ok(EX) || K == 0 || DIM == 0 || K > (1u << 24) || DIM > (1u << 16) ||
+            N > 0xFFFFFFFFull) {
+            set_error("erq_index_load: bad header");
+            return ERQ_EFORMAT;
+        }
+        // the header fixes every section's size: check the file holds them all before any
+        // buffer is sized from it
+        if (fsize < 32 + 8 * K) {
+            set_error("erq_index_load: truncated cluster sizes");
+            return ERQ_EIO;
+        }
+        const uint64_t D = align_up(DIM, 64);
+        std::vector<uint64_t> sizes(K);
+        if (!in.read(reinterpret_cast<char*>(sizes.data()), K * 8)) {
+            set_error("erq_index_load: truncated cluster sizes");
+            return ERQ_EIO;
+        }
+        uint64_t total = 0, nblocks = 0;
+        for (auto s : sizes) {
+            if (s > N) {
+                set_error("erq_index_load: cluster sizes do not sum to N");
+                return ERQ_EFORMAT;
+            }
+            total += s;
+            nblocks += (s + 31) / 32;
+        }
+        if (total != N) {
+            set_error("erq_index_load: cluster sizes do not sum to N");
+            return ERQ_EFORMAT;
+        }
+        const uint64_t body = 4 * D * D + 4 * K * D + nblocks * (4 * D + 128) + N * D * EX / 8 +
+                              4 * N + 4 * N;
+        if (fsize < 32 + 8 * K + body) {
+            set_error("erq_index_load: truncated index body");
+            return ERQ_EIO;
+        }
+        if (fsize > 32 + 8 * K + body) {
+            set_error("erq_index_load: bytes after the index body");
+            return ERQ_EFORMAT;
+        }
+        if (K > kMaxK || D > kMaxD) {
+            set_error("erq_index_load: K > 8192 or padded dim > 4096 is not supported");
+            return ERQ_EUNSUPPORTED;
+        }
+        std::vector<float> P(D * D), C(K * D), ef(N);
+        std::vector<uint8_t> sd(nblocks * (4 * D + 128)), lc(N * D * EX / 8);
+        std::vector<uint32_t> ids(N);
+        bool ok = in.read(reinterpret_cast<char*>(P.data()), P.size() * 4) &&
+                  in.read(reinterpret_cast<char*>(C.data()), C.size() * 4) &&
+                  in.read(reinterpret_cast<char*>(sd.data()), sd.size()) &&
+                  in.read(reinterpret_cast<char*>(lc.data()), lc.size()) &&
+                  in.read(reinterpret_cast<char*>(ef.data()), ef.size() * 4) &&
+                  in.read(reinterpret_cast<char*>(ids.data()), ids.size() * 4);
+        if (!ok) {
+            set_error("erq_index_load: truncated index body");
+            return ERQ_EIO;
+        }
+        return erq_index_create(N, DIM, K, EX, sizes.data(), P.data(), C.data(), sd.data(),
+                                lc.data(), ef.data(), ids.data(), out);
+    });
 }
 
 void erq_index_free(erq_index* h) { release(h); }
@@ -280,59 +405,68 @@ const void* erq_device_array(const erq_index* h, int which) {
 }
 
 int erq_rotate_dev(erq_index* h, const float* q_dev, size_t nq, float* q_rot_dev, void* stream) {
-    if (!h || (!q_dev && nq) || (!q_rot_dev && nq)) {
-        set_error("erq_rotate_dev: null argument");
-        return ERQ_EINVAL;
-    }
-    if (!h->P) {
-        set_error("erq_rotate_dev: index has no rotation matrix");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    std::lock_guard<std::mutex> lk(h->mu);
-    hipStream_t s = pick(h, stream);
-    const float* A = q_dev;
-    if (h->DIM != h->D) {
-        void* ws;
-        int rc = workspace(h, nq * h->D * sizeof(float), &ws);
-        if (rc) return rc;
-        hipError_t e = launch_pad_rows(q_dev, (int)nq, (int)h->DIM, (float*)ws, (int)h->D, s);
-        if (e != hipSuccess) return hip_fail(e, "pad queries");
-        A = (const float*)ws;
-    }
-    hipError_t e = launch_gemm(A, (int)h->D, h->P, (int)h->D, 0, q_rot_dev, (int)h->D, (int)nq,
-                               (int)h->D, (int)h->D, 0, nullptr, nullptr, s);
-    if (e != hipSuccess) return hip_fail(e, "rotation gemm");
-    return ERQ_OK;
+    return guarded([&]() -> int {
+        if (!h || (!q_dev && nq) || (!q_rot_dev && nq)) {
+            set_error("erq_rotate_dev: null argument");
+            return ERQ_EINVAL;
+        }
+        if (!h->P) {
+            set_error("erq_rotate_dev: index has no rotation matrix");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        const float* A = q_dev;
+        if (h->DIM != h->D) {
+            void* ws;
+            int rc = workspace(h, nq * h->D * sizeof(float), &ws);
+            if (rc) return rc;
+            hipError_t e =
+                launch_pad_rows(q_dev, (int)nq, (int)h->DIM, (float*)ws, (int)h->D, s);
+            if (e != hipSuccess) return hip_fail(e, "pad queries");
+            A = (const float*)ws;
+        }
+        hipError_t e = launch_gemm(A, (int)h->D, h->P, (int)h->D, 0, q_rot_dev, (int)h->D,
+                                   (int)nq, (int)h->D, (int)h->D, 0, nullptr, nullptr, s);
+        if (e != hipSuccess) return hip_fail(e, "rotation gemm");
+        return ERQ_OK;
+    });
 }
 
 int erq_centroid_dist_dev(erq_index* h, const float* q_rot_dev, size_t nq, float* out_dev,
                           void* stream) {
-    if (!h || (nq && (!q_rot_dev || !out_dev))) {
-        set_error("erq_centroid_dist_dev: null argument");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    std::lock_guard<std::mutex> lk(h->mu);
-    hipStream_t s = pick(h, stream);
-    void* wsp;
-    int rc = workspace(h, align_up(nq * 4, 256), &wsp);
-    if (rc) return rc;
-    float* qn = (float*)wsp;
-    hipError_t e = launch_row_sqnorm(q_rot_dev, (int)nq, (int)h->D, (int)h->D, qn, s);
-    if (e == hipSuccess)
-        e = launch_gemm(q_rot_dev, (int)h->D, h->centroids, (int)h->D, 1, out_dev, (int)h->K,
-                        (int)nq, (int)h->K, (int)h->D, 1, qn, h->cnorm, s);
-    if (e != hipSuccess) return hip_fail(e, "centroid gemm");
-    return ERQ_OK;
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot_dev || !out_dev))) {
+            set_error("erq_centroid_dist_dev: null argument");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        void* wsp;
+        int rc = workspace(h, align_up(nq * 4, 256), &wsp);
+        if (rc) return rc;
+        float* qn = (float*)wsp;
+        hipError_t e = launch_row_sqnorm(q_rot_dev, (int)nq, (int)h->D, (int)h->D, qn, s);
+        if (e == hipSuccess)
+            e = launch_gemm(q_rot_dev, (int)h->D, h->centroids, (int)h->D, 1, out_dev,
+                            (int)h->K, (int)nq, (int)h->K, (int)h->D, 1, qn, h->cnorm, s);
+        if (e != hipSuccess) return hip_fail(e, "centroid gemm");
+        return ERQ_OK;
+    });
+}
+
+// Workspace bytes of probe_locked (approximate distances and query norms), at offset 0.
+static size_t probe_ws_bytes(size_t nq, size_t K) {
+    return align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
 }
 
 static int probe_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
                         uint32_t* pid, float* pd, hipStream_t s) {
     const size_t K = h->K, D = h->D;
     void* wsp;
-    size_t need = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
-    int rc = workspace(h, need, &wsp);
+    int rc = workspace(h, probe_ws_bytes(nq, K), &wsp);
     if (rc) return rc;
     Ws ws{(char*)wsp};
     float* approx = ws.take<float>(nq * K);
@@ -342,26 +476,28 @@ static int probe_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t 
         e = launch_gemm(q_rot_dev, (int)D, h->centroids, (int)D, 1, approx, (int)K, (int)nq,
                         (int)K, (int)D, 1, qn, h->cnorm, s);
     if (e != hipSuccess) return hip_fail(e, "centroid gemm");
-    e = launch_probe_select(approx, qn, h->cnorm, q_rot_dev, h->centroids, (int)nq, (int)K, (int)D,
-                            (int)nprobe, pid, pd, s);
+    e = launch_probe_select(approx, qn, h->cnorm, q_rot_dev, h->centroids, (int)nq, (int)K,
+                            (int)D, (int)nprobe, pid, pd, s);
     if (e != hipSuccess) return hip_fail(e, "probe selection");
     return ERQ_OK;
 }
 
 int erq_probe_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
                   uint32_t* probe_ids_dev, float* probe_sqr_y_dev, void* stream) {
-    if (!h || (nq && (!q_rot_dev || !probe_ids_dev || !probe_sqr_y_dev))) {
-        set_error("erq_probe_dev: null argument");
-        return ERQ_EINVAL;
-    }
-    if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
-        set_error("erq_probe_dev: nprobe must be in [1, min(K, 2048)]");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    std::lock_guard<std::mutex> lk(h->mu);
-    return probe_locked(h, q_rot_dev, nq, nprobe, probe_ids_dev, probe_sqr_y_dev,
-                        pick(h, stream));
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot_dev || !probe_ids_dev || !probe_sqr_y_dev))) {
+            set_error("erq_probe_dev: null argument");
+            return ERQ_EINVAL;
+        }
+        if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
+            set_error("erq_probe_dev: nprobe must be in [1, min(K, 2048)]");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        return probe_locked(h, q_rot_dev, nq, nprobe, probe_ids_dev, probe_sqr_y_dev, s);
+    });
 }
 
 static int ensure_bp(erq_index* h, hipStream_t s);
@@ -372,9 +508,10 @@ static size_t max_cluster_size(const erq_index* h) {
     return (size_t)m;
 }
 
-static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
-                       const uint32_t* off, const uint32_t* pid, const float* pd, uint32_t* ids,
-                       float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0) {
+// The per-query scan's arguments (bp / exl set when the MFMA kernel runs).
+static ScanArgs scan_args(const erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                          const uint32_t* off, const uint32_t* pid, const float* pd,
+                          uint32_t* ids, float* dists, uint64_t* stats, int max_probes) {
     ScanArgs a;
     a.q_rot = q_rot_dev;
     a.probe_off = off;
@@ -398,18 +535,28 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
     a.K = (int)h->K;
     a.max_probes = max_probes;
     a.bp = nullptr;
-    a.nchunk = bp_nchunk((int)h->D);
-    a.waves_per_block = h->plain_waves;
-    if (h->plain_kernel != 1 && h->D <= 2048) {  // auto or MFMA: bit-plane integer path
-        int rc = ensure_bp(h, s);
-        if (rc) return rc;
-        a.bp = h->bp;
-    }
     a.exl = h->exl;
     a.exl_w = exl_words((int)h->D, (int)h->EX);
+    a.nchunk = bp_nchunk((int)h->D);
+    a.waves_per_block = h->plain_waves;
     // HASearcher.hpp:56
     a.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
     a.qorder = nullptr;
+    a.qlist = nullptr;
+    a.r0 = a.r1 = 0;
+    return a;
+}
+
+static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                       const uint32_t* off, const uint32_t* pid, const float* pd, uint32_t* ids,
+                       float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0) {
+    ScanArgs a = scan_args(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, max_probes);
+    if (h->plain_kernel != 1) {  // auto or MFMA: bit-plane integer path
+        int rc = ensure_bp(h, s);
+        if (rc) return rc;
+        a.bp = h->bp;
+        a.exl = h->exl;
+    }
     if (h->plain_order && nq > 1) {
         // queries by scan cost, heaviest first: the result of each query is unchanged
         const size_t need = nq + h->K + 1;
@@ -444,37 +591,39 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
 }
 
 int erq_set_option(erq_index* h, int option, long value) {
-    if (!h) {
-        set_error("erq_set_option: null handle");
+    return guarded([&]() -> int {
+        if (!h) {
+            set_error("erq_set_option: null handle");
+            return ERQ_EINVAL;
+        }
+        std::lock_guard<std::recursive_mutex> lk(h->mu);
+        if (option == ERQ_OPT_PLAIN_KERNEL && value >= 0 && value <= 2) {
+            h->plain_kernel = (int)value;
+            return ERQ_OK;
+        }
+        if (option == ERQ_OPT_PLAIN_WAVES && value >= 1 && value <= 4) {
+            h->plain_waves = (int)value;
+            return ERQ_OK;
+        }
+        if (option == ERQ_OPT_BATCH_WG_PER_CU && value >= 1 && value <= 8) {
+            h->batch_wg_per_cu = (int)value;
+            return ERQ_OK;
+        }
+        if (option == ERQ_OPT_PLAIN_ORDER && (value == 0 || value == 1)) {
+            h->plain_order = (int)value;
+            return ERQ_OK;
+        }
+        if (option == ERQ_OPT_BATCH_SURVIVOR_CAP && value >= 0 && value <= (1L << 30)) {
+            h->batch_surv_cap = (size_t)value;
+            return ERQ_OK;
+        }
+        if (option == ERQ_OPT_BATCH_PAIR_BUDGET && value >= 1024 && value <= (1L << 26)) {
+            h->batch_pair_budget = (size_t)value;
+            return ERQ_OK;
+        }
+        set_error("erq_set_option: unknown option or value");
         return ERQ_EINVAL;
-    }
-    std::lock_guard<std::mutex> lk(h->mu);
-    if (option == ERQ_OPT_PLAIN_KERNEL && value >= 0 && value <= 2) {
-        h->plain_kernel = (int)value;
-        return ERQ_OK;
-    }
-    if (option == ERQ_OPT_PLAIN_WAVES && value >= 1 && value <= 4) {
-        h->plain_waves = (int)value;
-        return ERQ_OK;
-    }
-    if (option == ERQ_OPT_BATCH_WG_PER_CU && value >= 1 && value <= 8) {
-        h->batch_wg_per_cu = (int)value;
-        return ERQ_OK;
-    }
-    if (option == ERQ_OPT_PLAIN_ORDER && (value == 0 || value == 1)) {
-        h->plain_order = (int)value;
-        return ERQ_OK;
-    }
-    if (option == ERQ_OPT_BATCH_SURVIVOR_CAP && value >= 0 && value <= (1L << 30)) {
-        h->batch_surv_cap = (size_t)value;
-        return ERQ_OK;
-    }
-    if (option == ERQ_OPT_BATCH_PAIR_BUDGET && value >= 1024 && value <= (1L << 26)) {
-        h->batch_pair_budget = (size_t)value;
-        return ERQ_OK;
-    }
-    set_error("erq_set_option: unknown option or value");
-    return ERQ_EINVAL;
+    });
 }
 
 long erq_get_option(const erq_index* h, int option) {
@@ -494,19 +643,22 @@ int erq_scan_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                  const uint32_t* probe_off_dev, const uint32_t* probe_ids_dev,
                  const float* probe_sqr_y_dev, uint32_t* ids_dev, float* dists_dev,
                  uint64_t* stats_dev, void* stream) {
-    if (!h || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev || !probe_sqr_y_dev ||
-                      !ids_dev))) {
-        set_error("erq_scan_dev: null argument");
-        return ERQ_EINVAL;
-    }
-    if (k == 0 || k > 1024) {
-        set_error("erq_scan_dev: k must be in [1, 1024]");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    std::lock_guard<std::mutex> lk(h->mu);
-    return scan_locked(h, q_rot_dev, nq, k, probe_off_dev, probe_ids_dev, probe_sqr_y_dev, ids_dev,
-                       dists_dev, stats_dev, pick(h, stream));
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev || !probe_sqr_y_dev ||
+                          !ids_dev))) {
+            set_error("erq_scan_dev: null argument");
+            return ERQ_EINVAL;
+        }
+        if (k == 0 || k > 1024) {
+            set_error("erq_scan_dev: k must be in [1, 1024]");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        return scan_locked(h, q_rot_dev, nq, k, probe_off_dev, probe_ids_dev, probe_sqr_y_dev,
+                           ids_dev, dists_dev, stats_dev, s);
+    });
 }
 
 __global__ static void add_u32_to_u64(const uint32_t* src, uint64_t* dst) { *dst += *src; }
@@ -516,46 +668,51 @@ __global__ static void fill_offsets(uint32_t* off, size_t nq, uint32_t nprobe) {
     if (i <= nq) off[i] = (uint32_t)(i * nprobe);
 }
 
+// Workspace bytes of a probe list of nq queries x nprobe (ids, distances, offsets).
+static size_t list_ws_bytes(size_t nq, size_t nprobe) {
+    return align_up(nq * nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) + 512;
+}
+
 int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
                    uint32_t* ids_dev, float* dists_dev, uint64_t* stats_dev, void* stream) {
-    if (!h || (nq && (!q_rot_dev || !ids_dev))) {
-        set_error("erq_search_dev: null argument");
-        return ERQ_EINVAL;
-    }
-    if (k == 0 || k > 1024) {
-        set_error("erq_search_dev: k must be in [1, 1024]");
-        return ERQ_EINVAL;
-    }
-    if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
-        set_error("erq_search_dev: nprobe must be in [1, min(K, 2048)]");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    if (nq * nprobe >= 0xFFFFFFFFull) {
-        set_error("erq_search_dev: nq * nprobe exceeds 32-bit probe offsets");
-        return ERQ_EUNSUPPORTED;
-    }
-    std::lock_guard<std::mutex> lk(h->mu);
-    hipStream_t s = pick(h, stream);
-    // probe lists live in the second part of the workspace; size the whole thing first
-    const size_t K = h->K;
-    size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
-    size_t list_part = align_up(nq * nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) + 512;
-    void* wsp;
-    int rc = workspace(h, probe_part + list_part, &wsp);
-    if (rc) return rc;
-    Ws ws{(char*)wsp};
-    ws.off = probe_part;
-    uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
-    float* pd = ws.take<float>(nq * nprobe);
-    uint32_t* off = ws.take<uint32_t>(nq + 1);
-    rc = probe_locked(h, q_rot_dev, nq, nprobe, pid, pd, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off, nq,
-                       (uint32_t)nprobe);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "offsets");
-    return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s);
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot_dev || !ids_dev))) {
+            set_error("erq_search_dev: null argument");
+            return ERQ_EINVAL;
+        }
+        if (k == 0 || k > 1024) {
+            set_error("erq_search_dev: k must be in [1, 1024]");
+            return ERQ_EINVAL;
+        }
+        if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
+            set_error("erq_search_dev: nprobe must be in [1, min(K, 2048)]");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        if (nq * nprobe >= 0xFFFFFFFFull) {
+            set_error("erq_search_dev: nq * nprobe exceeds 32-bit probe offsets");
+            return ERQ_EUNSUPPORTED;
+        }
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        // probe lists live in the second part of the workspace; size the whole thing first
+        const size_t probe_part = probe_ws_bytes(nq, h->K);
+        void* wsp;
+        int rc = workspace(h, probe_part + list_ws_bytes(nq, nprobe), &wsp);
+        if (rc) return rc;
+        Ws ws{(char*)wsp};
+        ws.off = probe_part;
+        uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
+        float* pd = ws.take<float>(nq * nprobe);
+        uint32_t* off = ws.take<uint32_t>(nq + 1);
+        rc = probe_locked(h, q_rot_dev, nq, nprobe, pid, pd, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off,
+                           nq, (uint32_t)nprobe);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "offsets");
+        return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s);
+    });
 }
 
 // The bit-plane copy of the 1-bit codes and the lane-major copy of the EX-bit codes,
@@ -596,6 +753,11 @@ static size_t next_round(size_t r0) { return std::min(4 * r0, r0 + 128); }
 
 constexpr size_t kMaxBatchGrid = 4096;    // workgroups of the persistent batched kernel
 
+// (query, cluster) pairs prepared ahead: the option, capped so the digits fit kDigitBytes.
+static size_t pair_budget(const erq_index* h) {
+    const size_t Dp = (size_t)bp_nchunk((int)h->D) * 512;
+    return std::max<size_t>(1024, std::min(h->batch_pair_budget, kDigitBytes / (2 * Dp)));
+}
 
 // Queries per chunk: every round of a chunk is prepared ahead, so the chunk's pairs over all
 // ranks past the exact ones must fit the pair budget.
@@ -608,7 +770,7 @@ static size_t batch_chunk_nq(size_t nq, size_t max_nprobe, size_t budget) {
 struct BatchBufs {
     size_t cq = 0, R = 0, r_end = 0;
     uint32_t *counts, *cursor, *local, *btot, *rinfo, *items, *ovf, *entries, *cand_cnt, *cand_i,
-        *surv_count;
+        *surv_count, *redo;
     float* cand_d;
     int8_t* adig;
     char* qparams;
@@ -619,7 +781,7 @@ static void batch_layout(const erq_index* h, size_t nq, size_t max_nprobe, int c
                          BatchBufs& b) {
     const size_t K = h->K, Dp = (size_t)bp_nchunk((int)h->D) * 512;
     const size_t maxc = max_cluster_size(h);
-    b.cq = batch_chunk_nq(nq, max_nprobe, h->batch_pair_budget);
+    b.cq = batch_chunk_nq(nq, max_nprobe, pair_budget(h));
     b.R = 0;
     size_t pairs = 0, items = 0, max_items = 0;
     for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0)) {
@@ -642,6 +804,7 @@ static void batch_layout(const erq_index* h, size_t nq, size_t max_nprobe, int c
     b.cand_cnt = ws.take<uint32_t>(b.cq);
     b.cand_d = ws.take<float>(b.cq * (size_t)cap);
     b.cand_i = ws.take<uint32_t>(b.cq * (size_t)cap);
+    b.redo = ws.take<uint32_t>(b.R * (b.cq + 1));
     b.adig = ws.take<int8_t>(pairs * 2 * Dp);
     b.qparams = ws.take<char>(pairs * 48);
     b.surv = ws.take<uint4>(h->batch_surv_cap);
@@ -659,6 +822,7 @@ static int ensure_aux(erq_index* h) {
 }
 
 static size_t batch_ws_bytes(const erq_index* h, size_t nq, size_t max_nprobe, int cap) {
+    if (max_nprobe <= kExactRanks) return 256;  // the exact replay alone
     Ws ws{nullptr};
     BatchBufs b;
     batch_layout(h, nq, max_nprobe, cap, ws, b);
@@ -669,9 +833,6 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
                           const uint32_t* off, const uint32_t* pid, const float* pd,
                           size_t max_nprobe, uint32_t* ids, float* dists, uint64_t* stats,
                           hipStream_t s, char* ws_base, size_t ws_off) {
-    // D > 2048: the int8 MFMA integer would not reproduce the reference's u16 wrap, so the
-    // batched entry points run the exact per-query replay over every probe instead.
-    if (h->D > 2048) return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s);
     int rc = ensure_bp(h, s);
     if (rc) return rc;
     // ranks [0, kExactRanks): the exact per-query scan of each query's nearest clusters
@@ -692,9 +853,17 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     const int grid = (int)std::min<size_t>(
         kMaxBatchGrid, (size_t)std::max(1, h->num_cu) * std::max(1, h->batch_wg_per_cu));
     const int prep_grid = 6 * ((std::max(1, h->num_cu) + 7) / 8 * 8);  // 6 fit per CU (LDS)
+    std::vector<size_t> rlo, rhi;  // each round's probe ranks
+    for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0)) {
+        rlo.push_back(r0);
+        rhi.push_back(std::min(next_round(r0), max_nprobe));
+    }
     BatchArgs ba;
     RoundBufs rb;
+    size_t cur_q0 = 0, cur_n = 0;
     auto setup = [&](size_t q0, size_t n) {
+        cur_q0 = q0;
+        cur_n = n;
         ba.q_rot = q_rot_dev + q0 * h->D;
         ba.probe_off = off + q0;
         ba.probe_sqr_y = pd;
@@ -724,6 +893,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.surv_count = b.surv_count;
         ba.surv_cap = h->batch_surv_cap;
         ba.stats = reinterpret_cast<unsigned long long*>(stats);
+        ba.redo = b.redo;
         ba.cand_cap = cap;
         ba.num_cu = h->num_cu;
         ba.D = (int)h->D;
@@ -735,10 +905,22 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         rb = RoundBufs{pid, b.counts, b.cursor, b.local, b.btot, ids + q0 * k, (int)h->K};
     };
     auto rounds = [&]() -> hipError_t {
+        // the round-mode rescan of the queries whose candidates overflowed (none, normally:
+        // its waves read an empty list and exit)
+        ScanArgs sa = scan_args(h, q_rot_dev + cur_q0 * h->D, cur_n, k, off + cur_q0, pid, pd,
+                                ids + cur_q0 * k, dists + cur_q0 * k, nullptr, 0);
+        sa.bp = h->bp;
+        sa.exl = h->exl;
         for (size_t r = 0; r < b.R; r++) {
             BatchArgs br = ba;
             br.n_items = b.rinfo + 4 * r;
+            br.redo = b.redo + r * (cq + 1);
             hipError_t e = launch_round_scan(br, rb, grid, s);
+            if (e != hipSuccess) return e;
+            sa.qlist = br.redo;
+            sa.r0 = (int)rlo[r];
+            sa.r1 = (int)rhi[r];
+            e = launch_scan_round(sa, s);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -766,6 +948,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
             if (e != hipSuccess) return hip_fail(e, "batched round preparation");
         }
         e = launch_zero(b.cand_cnt, n, s);
+        if (e == hipSuccess) e = launch_zero(b.redo, b.R * (cq + 1), s);
         if (e == hipSuccess) e = rounds();
         if (e != hipSuccess) return hip_fail(e, "batched round");
     }
@@ -776,73 +959,74 @@ int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t
                          const uint32_t* probe_off_dev, const uint32_t* probe_ids_dev,
                          const float* probe_sqr_y_dev, size_t max_nprobe, uint32_t* ids_dev,
                          float* dists_dev, uint64_t* stats_dev, void* stream) {
-    if (!h || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev || !probe_sqr_y_dev ||
-                      !ids_dev || !dists_dev))) {
-        set_error("erq_scan_batched_dev: null argument (dists_dev is required)");
-        return ERQ_EINVAL;
-    }
-    if (k == 0 || k > 1024) {
-        set_error("erq_scan_batched_dev: k must be in [1, 1024]");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    if (nq * std::max<size_t>(max_nprobe, 1) >= (1u << 27)) {
-        set_error("erq_scan_batched_dev: nq * max_nprobe must stay below 2^27");
-        return ERQ_EUNSUPPORTED;
-    }
-    std::lock_guard<std::mutex> lk(h->mu);
-    hipStream_t s = pick(h, stream);
-    const int cap = kCandCap;
-    void* wsp;
-    int rc = workspace(h, batch_ws_bytes(h, nq, max_nprobe, cap), &wsp);
-    if (rc) return rc;
-    return batched_locked(h, q_rot_dev, nq, k, probe_off_dev, probe_ids_dev, probe_sqr_y_dev,
-                          max_nprobe, ids_dev, dists_dev, stats_dev, s, (char*)wsp, 0);
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev || !probe_sqr_y_dev ||
+                          !ids_dev || !dists_dev))) {
+            set_error("erq_scan_batched_dev: null argument (dists_dev is required)");
+            return ERQ_EINVAL;
+        }
+        if (k == 0 || k > 1024) {
+            set_error("erq_scan_batched_dev: k must be in [1, 1024]");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        if (nq * std::max<size_t>(max_nprobe, 1) >= (1u << 27)) {
+            set_error("erq_scan_batched_dev: nq * max_nprobe must stay below 2^27");
+            return ERQ_EUNSUPPORTED;
+        }
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        void* wsp;
+        int rc = workspace(h, batch_ws_bytes(h, nq, max_nprobe, kCandCap), &wsp);
+        if (rc) return rc;
+        return batched_locked(h, q_rot_dev, nq, k, probe_off_dev, probe_ids_dev, probe_sqr_y_dev,
+                              max_nprobe, ids_dev, dists_dev, stats_dev, s, (char*)wsp, 0);
+    });
 }
 
 int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                            size_t nprobe, uint32_t* ids_dev, float* dists_dev,
                            uint64_t* stats_dev, void* stream) {
-    if (!h || (nq && (!q_rot_dev || !ids_dev || !dists_dev))) {
-        set_error("erq_search_batched_dev: null argument (dists_dev is required)");
-        return ERQ_EINVAL;
-    }
-    if (k == 0 || k > 1024) {
-        set_error("erq_search_batched_dev: k must be in [1, 1024]");
-        return ERQ_EINVAL;
-    }
-    if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
-        set_error("erq_search_batched_dev: nprobe must be in [1, min(K, 2048)]");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    if (nq * nprobe >= (1u << 27)) {
-        set_error("erq_search_batched_dev: nq * nprobe must stay below 2^27");
-        return ERQ_EUNSUPPORTED;
-    }
-    std::lock_guard<std::mutex> lk(h->mu);
-    hipStream_t s = pick(h, stream);
-    const size_t K = h->K;
-    const int cap = kCandCap;
-    size_t probe_part = align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
-    size_t list_part = align_up(nq * nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) + 512;
-    size_t batch_part = batch_ws_bytes(h, nq, nprobe, cap);
-    void* wsp;
-    int rc = workspace(h, probe_part + list_part + batch_part, &wsp);
-    if (rc) return rc;
-    Ws ws{(char*)wsp};
-    ws.off = probe_part;
-    uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
-    float* pd = ws.take<float>(nq * nprobe);
-    uint32_t* off = ws.take<uint32_t>(nq + 1);
-    rc = probe_locked(h, q_rot_dev, nq, nprobe, pid, pd, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off, nq,
-                       (uint32_t)nprobe);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "offsets");
-    return batched_locked(h, q_rot_dev, nq, k, off, pid, pd, nprobe, ids_dev, dists_dev,
-                          stats_dev, s, (char*)wsp, probe_part + list_part);
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot_dev || !ids_dev || !dists_dev))) {
+            set_error("erq_search_batched_dev: null argument (dists_dev is required)");
+            return ERQ_EINVAL;
+        }
+        if (k == 0 || k > 1024) {
+            set_error("erq_search_batched_dev: k must be in [1, 1024]");
+            return ERQ_EINVAL;
+        }
+        if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
+            set_error("erq_search_batched_dev: nprobe must be in [1, min(K, 2048)]");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        if (nq * nprobe >= (1u << 27)) {
+            set_error("erq_search_batched_dev: nq * nprobe must stay below 2^27");
+            return ERQ_EUNSUPPORTED;
+        }
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        const size_t probe_part = probe_ws_bytes(nq, h->K);
+        const size_t list_part = list_ws_bytes(nq, nprobe);
+        const size_t batch_part = batch_ws_bytes(h, nq, nprobe, kCandCap);
+        void* wsp;
+        int rc = workspace(h, probe_part + list_part + batch_part, &wsp);
+        if (rc) return rc;
+        Ws ws{(char*)wsp};
+        ws.off = probe_part;
+        uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
+        float* pd = ws.take<float>(nq * nprobe);
+        uint32_t* off = ws.take<uint32_t>(nq + 1);
+        rc = probe_locked(h, q_rot_dev, nq, nprobe, pid, pd, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off,
+                           nq, (uint32_t)nprobe);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "offsets");
+        return batched_locked(h, q_rot_dev, nq, k, off, pid, pd, nprobe, ids_dev, dists_dev,
+                              stats_dev, s, (char*)wsp, probe_part + list_part);
+    });
 }
 
 static size_t learned_ws_bytes(size_t nq, size_t K, size_t max_np) {
@@ -888,105 +1072,177 @@ int erq_probe_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, size
                           size_t min_nprobe, size_t max_nprobe, size_t fallback_nprobe,
                           float min_conf, uint32_t* probe_off_dev, uint32_t* probe_ids_dev,
                           float* probe_sqr_y_dev, uint32_t* n_fallback_dev, void* stream) {
-    if (!h || !m || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev || !probe_sqr_y_dev ||
-                            !n_fallback_dev))) {
-        set_error("erq_probe_learned_dev: null argument");
-        return ERQ_EINVAL;
-    }
-    int rc = learned_args_ok(h, m, min_nprobe, max_nprobe, fallback_nprobe, "erq_probe_learned_dev");
-    if (rc || nq == 0) return rc;
-    std::lock_guard<std::mutex> lk(h->mu);
-    hipStream_t s = pick(h, stream);
-    const size_t probe_part = 0;  // fallback lists are formed by the selection kernel itself
-    void* wsp;
-    rc = workspace(h, probe_part + learned_ws_bytes(nq, h->K, max_nprobe), &wsp);
-    if (rc) return rc;
-    return learned_locked(h, m, q_rot_dev, nq, mass, min_nprobe, max_nprobe, fallback_nprobe,
-                          min_conf, probe_off_dev, probe_ids_dev, probe_sqr_y_dev, n_fallback_dev,
-                          s, (char*)wsp, probe_part);
+    return guarded([&]() -> int {
+        if (!h || !m || (nq && (!q_rot_dev || !probe_off_dev || !probe_ids_dev ||
+                                !probe_sqr_y_dev || !n_fallback_dev))) {
+            set_error("erq_probe_learned_dev: null argument");
+            return ERQ_EINVAL;
+        }
+        int rc = learned_args_ok(h, m, min_nprobe, max_nprobe, fallback_nprobe,
+                                 "erq_probe_learned_dev");
+        if (rc || nq == 0) return rc;
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        void* wsp;
+        rc = workspace(h, learned_ws_bytes(nq, h->K, max_nprobe), &wsp);
+        if (rc) return rc;
+        return learned_locked(h, m, q_rot_dev, nq, mass, min_nprobe, max_nprobe,
+                              fallback_nprobe, min_conf, probe_off_dev, probe_ids_dev,
+                              probe_sqr_y_dev, n_fallback_dev, s, (char*)wsp, 0);
+    });
 }
 
 int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq, size_t k,
                            float mass, size_t min_nprobe, size_t max_nprobe,
                            size_t fallback_nprobe, float min_conf, uint32_t* ids_dev,
                            float* dists_dev, uint64_t* stats_dev, void* stream) {
-    if (!h || !m || (nq && (!q_rot_dev || !ids_dev || !dists_dev))) {
-        set_error("erq_search_learned_dev: null argument (dists_dev is required)");
-        return ERQ_EINVAL;
-    }
-    if (k == 0 || k > 1024) {
-        set_error("erq_search_learned_dev: k must be in [1, 1024]");
-        return ERQ_EINVAL;
-    }
-    int rc = learned_args_ok(h, m, min_nprobe, max_nprobe, fallback_nprobe, "erq_search_learned_dev");
-    if (rc || nq == 0) return rc;
-    if (nq * max_nprobe >= (1u << 27)) {
-        set_error("erq_search_learned_dev: nq * max_nprobe must stay below 2^27");
-        return ERQ_EUNSUPPORTED;
-    }
-    std::lock_guard<std::mutex> lk(h->mu);
-    hipStream_t s = pick(h, stream);
-    const size_t K = h->K;
-    const int cap = kCandCap;
-    const size_t probe_part = 0;  // fallback lists are formed by the selection kernel itself
-    size_t learned_part = learned_ws_bytes(nq, K, max_nprobe);
-    size_t list_part = align_up(nq * max_nprobe * 4, 256) * 2 + align_up((nq + 1) * 4, 256) +
-                       align_up(8, 256) + 512;
-    size_t batch_part = batch_ws_bytes(h, nq, max_nprobe, cap);
-    void* wsp;
-    rc = workspace(h, probe_part + learned_part + list_part + batch_part, &wsp);
+    return guarded([&]() -> int {
+        if (!h || !m || (nq && (!q_rot_dev || !ids_dev || !dists_dev))) {
+            set_error("erq_search_learned_dev: null argument (dists_dev is required)");
+            return ERQ_EINVAL;
+        }
+        if (k == 0 || k > 1024) {
+            set_error("erq_search_learned_dev: k must be in [1, 1024]");
+            return ERQ_EINVAL;
+        }
+        int rc = learned_args_ok(h, m, min_nprobe, max_nprobe, fallback_nprobe,
+                                 "erq_search_learned_dev");
+        if (rc || nq == 0) return rc;
+        if (nq * max_nprobe >= (1u << 27)) {
+            set_error("erq_search_learned_dev: nq * max_nprobe must stay below 2^27");
+            return ERQ_EUNSUPPORTED;
+        }
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        const size_t learned_part = learned_ws_bytes(nq, h->K, max_nprobe);
+        const size_t list_part = list_ws_bytes(nq, max_nprobe) + align_up(8, 256);
+        const size_t batch_part = batch_ws_bytes(h, nq, max_nprobe, kCandCap);
+        void* wsp;
+        rc = workspace(h, learned_part + list_part + batch_part, &wsp);
+        if (rc) return rc;
+        Ws ws{(char*)wsp, learned_part};
+        uint32_t* off = ws.take<uint32_t>(nq + 1);
+        uint32_t* pid = ws.take<uint32_t>(nq * max_nprobe);
+        float* pd = ws.take<float>(nq * max_nprobe);
+        uint32_t* nfb = ws.take<uint32_t>(2);
+        hipError_t e = launch_zero(nfb, 2, s);
+        if (e != hipSuccess) return hip_fail(e, "zero");
+        rc = learned_locked(h, m, q_rot_dev, nq, mass, min_nprobe, max_nprobe, fallback_nprobe,
+                            min_conf, off, pid, pd, nfb, s, (char*)wsp, 0);
+        if (rc) return rc;
+        if (stats_dev) {
+            // counter 7: queries that took the fixed-nprobe fallback
+            hipLaunchKernelGGL(add_u32_to_u64, dim3(1), dim3(1), 0, s, nfb, stats_dev + 7);
+            e = hipGetLastError();
+            if (e != hipSuccess) return hip_fail(e, "stats");
+        }
+        return batched_locked(h, q_rot_dev, nq, k, off, pid, pd, max_nprobe, ids_dev, dists_dev,
+                              stats_dev, s, (char*)wsp, learned_part + list_part);
+    });
+}
+
+// Host-pointer search of nq rotated queries through the per-handle device buffers; probe
+// counts per query (np_host, nullable: every query takes `nprobe`).
+static int host_search(erq_index* h, const float* q_rot, size_t nq, size_t k, size_t nprobe,
+                       const uint32_t* np_host, unsigned flags, uint32_t* results, float* dists) {
+    hipStream_t s = h->stream;
+    Call call(h, s);
+    int rc = host_bufs(h, nq, k);
     if (rc) return rc;
-    Ws ws{(char*)wsp, probe_part + learned_part};
-    uint32_t* off = ws.take<uint32_t>(nq + 1);
-    uint32_t* pid = ws.take<uint32_t>(nq * max_nprobe);
-    float* pd = ws.take<float>(nq * max_nprobe);
-    uint32_t* nfb = ws.take<uint32_t>(2);
-    hipError_t e = launch_zero(nfb, 2, s);
-    if (e != hipSuccess) return hip_fail(e, "zero");
-    rc = learned_locked(h, m, q_rot_dev, nq, mass, min_nprobe, max_nprobe, fallback_nprobe,
-                        min_conf, off, pid, pd, nfb, s, (char*)wsp, probe_part);
-    if (rc) return rc;
-    if (stats_dev) {
-        // counter 7: queries that took the fixed-nprobe fallback
-        hipLaunchKernelGGL(add_u32_to_u64, dim3(1), dim3(1), 0, s, nfb, stats_dev + 7);
-        e = hipGetLastError();
-        if (e != hipSuccess) return hip_fail(e, "stats");
+    hipError_t e = hipMemcpyAsync(h->hq, q_rot, nq * h->D * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "copy queries");
+    const bool batched = (flags & ERQ_SEARCH_BATCHED) != 0;
+    if (!np_host) {
+        rc = batched ? erq_search_batched_dev(h, h->hq, nq, k, nprobe, h->hids, h->hdists, nullptr, s)
+                     : erq_search_dev(h, h->hq, nq, k, nprobe, h->hids, h->hdists, nullptr, s);
+    } else {
+        // CSR offsets of the per-query probe counts on the host; the probe lists are the
+        // max-count lists of every query, cut to its own count
+        std::vector<uint32_t> offs(nq + 1);
+        size_t max_np = 0, tot = 0;
+        for (size_t i = 0; i < nq; i++) {
+            offs[i] = (uint32_t)tot;
+            tot += np_host[i];
+            max_np = std::max<size_t>(max_np, np_host[i]);
+        }
+        offs[nq] = (uint32_t)tot;
+        e = hipMemcpyAsync(h->hnp, offs.data(), (nq + 1) * 4, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "copy probe offsets");
+        const size_t probe_part = probe_ws_bytes(nq, h->K);
+        const size_t pad_part = list_ws_bytes(nq, max_np);
+        const size_t csr_part = list_ws_bytes(tot, 1);
+        const size_t batch_part = batched ? batch_ws_bytes(h, nq, max_np, kCandCap) : 0;
+        void* wsp;
+        rc = workspace(h, probe_part + pad_part + csr_part + batch_part, &wsp);
+        if (rc) return rc;
+        Ws ws{(char*)wsp, probe_part};
+        uint32_t* pad_id = ws.take<uint32_t>(nq * max_np);
+        float* pad_d = ws.take<float>(nq * max_np);
+        ws.off = probe_part + pad_part;
+        uint32_t* cid = ws.take<uint32_t>(tot);
+        float* cd = ws.take<float>(tot);
+        rc = probe_locked(h, h->hq, nq, max_np, pad_id, pad_d, s);
+        if (rc) return rc;
+        e = launch_compact_probes(pad_id, pad_d, (int)max_np, h->hnp, (int)nq, cid, cd, s);
+        if (e != hipSuccess) return hip_fail(e, "probe lists");
+        rc = batched ? batched_locked(h, h->hq, nq, k, h->hnp, cid, cd, max_np, h->hids,
+                                      h->hdists, nullptr, s, (char*)wsp,
+                                      probe_part + pad_part + csr_part)
+                     : scan_locked(h, h->hq, nq, k, h->hnp, cid, cd, h->hids, h->hdists, nullptr,
+                                   s);
     }
-    return batched_locked(h, q_rot_dev, nq, k, off, pid, pd, max_nprobe, ids_dev, dists_dev,
-                          stats_dev, s, (char*)wsp, probe_part + learned_part + list_part);
+    if (rc) return rc;
+    e = hipMemcpyAsync(results, h->hids, nq * k * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && dists)
+        e = hipMemcpyAsync(dists, h->hdists, nq * k * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "copy results");
+    return ERQ_OK;
 }
 
 int erq_search_batch(erq_index* h, const float* q_rot, size_t nq, size_t k, size_t nprobe,
                      uint32_t* results, float* dists) {
-    if (!h || (nq && (!q_rot || !results))) {
-        set_error("erq_search_batch: null argument");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    float* dq = nullptr;
-    uint32_t* di = nullptr;
-    float* dd = nullptr;
-    hipError_t e = hipMalloc(&dq, nq * h->D * 4);
-    if (e == hipSuccess) e = hipMalloc(&di, nq * k * 4);
-    if (e == hipSuccess) e = hipMalloc(&dd, nq * k * 4);
-    int rc = ERQ_OK;
-    if (e != hipSuccess) rc = hip_fail(e, "erq_search_batch alloc");
-    if (!rc) {
-        e = hipMemcpyAsync(dq, q_rot, nq * h->D * 4, hipMemcpyHostToDevice, h->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "copy queries");
-    }
-    if (!rc) rc = erq_search_dev(h, dq, nq, k, nprobe, di, dd, nullptr, h->stream);
-    if (!rc) {
-        e = hipMemcpyAsync(results, di, nq * k * 4, hipMemcpyDeviceToHost, h->stream);
-        if (e == hipSuccess && dists)
-            e = hipMemcpyAsync(dists, dd, nq * k * 4, hipMemcpyDeviceToHost, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "copy results");
-    }
-    if (dq) (void)hipFree(dq);
-    if (di) (void)hipFree(di);
-    if (dd) (void)hipFree(dd);
-    return rc;
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot || !results))) {
+            set_error("erq_search_batch: null argument");
+            return ERQ_EINVAL;
+        }
+        if (k == 0 || k > 1024 || nprobe == 0 || nprobe > h->K || nprobe > 2048) {
+            set_error("erq_search_batch: need k in [1, 1024], nprobe in [1, min(K, 2048)]");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        return host_search(h, q_rot, nq, k, nprobe, nullptr, 0, results, dists);
+    });
+}
+
+int erq_search_batch_nprobe(erq_index* h, const float* q_rot, size_t nq, size_t k,
+                            const uint32_t* nprobe_per_q, uint32_t* results, float* dists,
+                            unsigned flags) {
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot || !results || !nprobe_per_q))) {
+            set_error("erq_search_batch_nprobe: null argument");
+            return ERQ_EINVAL;
+        }
+        if (k == 0 || k > 1024 || (flags & ~(unsigned)ERQ_SEARCH_BATCHED)) {
+            set_error("erq_search_batch_nprobe: k must be in [1, 1024], flags ERQ_SEARCH_BATCHED");
+            return ERQ_EINVAL;
+        }
+        size_t tot = 0;
+        for (size_t i = 0; i < nq; i++) {
+            if (nprobe_per_q[i] == 0 || nprobe_per_q[i] > h->K || nprobe_per_q[i] > 2048) {
+                set_error("erq_search_batch_nprobe: every nprobe must be in [1, min(K, 2048)]");
+                return ERQ_EINVAL;
+            }
+            tot += nprobe_per_q[i];
+        }
+        if (nq == 0) return ERQ_OK;
+        if (tot >= (1u << 27)) {
+            set_error("erq_search_batch_nprobe: the probes of all queries must stay below 2^27");
+            return ERQ_EUNSUPPORTED;
+        }
+        return host_search(h, q_rot, nq, k, 0, nprobe_per_q, flags, results, dists);
+    });
 }
 
 int erq_search(erq_index* h, const float* q_rot, size_t k, size_t nprobe, uint32_t* results,
@@ -995,95 +1251,104 @@ int erq_search(erq_index* h, const float* q_rot, size_t k, size_t nprobe, uint32
 }
 
 int erq_rotate(erq_index* h, const float* q, size_t nq, float* q_rot) {
-    if (!h || (nq && (!q || !q_rot))) {
-        set_error("erq_rotate: null argument");
-        return ERQ_EINVAL;
-    }
-    if (nq == 0) return ERQ_OK;
-    float *dq = nullptr, *dr = nullptr;
-    hipError_t e = hipMalloc(&dq, nq * h->DIM * 4);
-    if (e == hipSuccess) e = hipMalloc(&dr, nq * h->D * 4);
-    int rc = ERQ_OK;
-    if (e != hipSuccess) rc = hip_fail(e, "erq_rotate alloc");
-    if (!rc) {
-        e = hipMemcpyAsync(dq, q, nq * h->DIM * 4, hipMemcpyHostToDevice, h->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "copy queries");
-    }
-    if (!rc) rc = erq_rotate_dev(h, dq, nq, dr, h->stream);
-    if (!rc) {
-        e = hipMemcpyAsync(q_rot, dr, nq * h->D * 4, hipMemcpyDeviceToHost, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "copy rotated queries");
-    }
-    if (dq) (void)hipFree(dq);
-    if (dr) (void)hipFree(dr);
-    return rc;
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q || !q_rot))) {
+            set_error("erq_rotate: null argument");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        hipStream_t s = h->stream;
+        Call call(h, s);
+        // the padded input rows and the output share the host-call query buffer's role: the
+        // input goes to the workspace tail, the output to hq
+        int rc = host_bufs(h, nq, 1);
+        if (rc) return rc;
+        void* wsp;
+        const size_t pad_part = align_up(nq * h->D * 4, 256);  // erq_rotate_dev pads here
+        rc = workspace(h, pad_part + nq * h->DIM * 4, &wsp);
+        if (rc) return rc;
+        float* dq = reinterpret_cast<float*>((char*)wsp + pad_part);
+        hipError_t e = hipMemcpyAsync(dq, q, nq * h->DIM * 4, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "copy queries");
+        rc = erq_rotate_dev(h, dq, nq, h->hq, s);
+        if (rc) return rc;
+        e = hipMemcpyAsync(q_rot, h->hq, nq * h->D * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "copy rotated queries");
+        return ERQ_OK;
+    });
 }
 
 // Times the result-invariant launch shapes on the caller's queries (HIP events on the
 // caller's stream) and keeps the fastest: waves (queries) per workgroup of the per-query
-// scan, and workgroups per CU of the persistent batched kernel.
+// scan, and workgroups per CU of the persistent batched kernel.  Holds the handle for the
+// whole sweep, so no other call sees an intermediate setting.
 int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
                  float* best_ms_out, void* stream) {
-    if (!h || !q_rot_dev || nq == 0) {
-        set_error("erq_autotune: null argument or empty batch");
-        return ERQ_EINVAL;
-    }
-    hipStream_t s = pick(h, stream);
-    uint32_t* ids = nullptr;
-    float* d = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipMalloc(&ids, nq * k * 4);
-    if (e == hipSuccess) e = hipMalloc(&d, nq * k * 4);
-    if (e == hipSuccess) e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    int rc = e == hipSuccess ? ERQ_OK : hip_fail(e, "erq_autotune setup");
-    auto time_it = [&](bool batched, float* ms) -> int {
-        int r = batched ? erq_search_batched_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s)
-                        : erq_search_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s);
-        if (r) return r;
-        hipError_t x = hipEventRecord(e0, s);
-        if (x == hipSuccess) {
-            r = batched ? erq_search_batched_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s)
+    return guarded([&]() -> int {
+        if (!h || !q_rot_dev || nq == 0) {
+            set_error("erq_autotune: null argument or empty batch");
+            return ERQ_EINVAL;
+        }
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        uint32_t* ids = nullptr;
+        float* d = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipMalloc(&ids, nq * k * 4);
+        if (e == hipSuccess) e = hipMalloc(&d, nq * k * 4);
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        int rc = e == hipSuccess ? ERQ_OK : hip_fail(e, "erq_autotune setup");
+        auto time_it = [&](bool batched, float* ms) -> int {
+            int r = batched
+                        ? erq_search_batched_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s)
                         : erq_search_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s);
             if (r) return r;
-            x = hipEventRecord(e1, s);
+            hipError_t x = hipEventRecord(e0, s);
+            if (x == hipSuccess) {
+                r = batched
+                        ? erq_search_batched_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s)
+                        : erq_search_dev(h, q_rot_dev, nq, k, nprobe, ids, d, nullptr, s);
+                if (r) return r;
+                x = hipEventRecord(e1, s);
+            }
+            if (x == hipSuccess) x = hipEventSynchronize(e1);
+            if (x == hipSuccess) x = hipEventElapsedTime(ms, e0, e1);
+            return x == hipSuccess ? ERQ_OK : hip_fail(x, "erq_autotune timing");
+        };
+        float best_plain = 0, best_batch = 0;
+        int bw = h->plain_waves, bg = h->batch_wg_per_cu;
+        for (int wv = 1; wv <= 4 && !rc; wv *= 2) {
+            h->plain_waves = wv;
+            float ms;
+            rc = time_it(false, &ms);
+            if (!rc && (best_plain == 0 || ms < best_plain)) {
+                best_plain = ms;
+                bw = wv;
+            }
         }
-        if (x == hipSuccess) x = hipEventSynchronize(e1);
-        if (x == hipSuccess) x = hipEventElapsedTime(ms, e0, e1);
-        return x == hipSuccess ? ERQ_OK : hip_fail(x, "erq_autotune timing");
-    };
-    float best_plain = 0, best_batch = 0;
-    int bw = h->plain_waves, bg = h->batch_wg_per_cu;
-    for (int wv = 1; wv <= 4 && !rc; wv *= 2) {
-        h->plain_waves = wv;
-        float ms;
-        rc = time_it(false, &ms);
-        if (!rc && (best_plain == 0 || ms < best_plain)) {
-            best_plain = ms;
-            bw = wv;
+        h->plain_waves = bw;
+        for (int wg = 1; wg <= 4 && !rc; wg++) {
+            h->batch_wg_per_cu = wg;
+            float ms;
+            rc = time_it(true, &ms);
+            if (!rc && (best_batch == 0 || ms < best_batch)) {
+                best_batch = ms;
+                bg = wg;
+            }
         }
-    }
-    h->plain_waves = bw;
-    for (int wg = 1; wg <= 4 && !rc; wg++) {
-        h->batch_wg_per_cu = wg;
-        float ms;
-        rc = time_it(true, &ms);
-        if (!rc && (best_batch == 0 || ms < best_batch)) {
-            best_batch = ms;
-            bg = wg;
+        h->batch_wg_per_cu = bg;
+        if (best_ms_out) {
+            best_ms_out[0] = best_plain;
+            best_ms_out[1] = best_batch;
         }
-    }
-    h->batch_wg_per_cu = bg;
-    if (best_ms_out) {
-        best_ms_out[0] = best_plain;
-        best_ms_out[1] = best_batch;
-    }
-    if (ids) (void)hipFree(ids);
-    if (d) (void)hipFree(d);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    return rc;
+        if (ids) (void)hipFree(ids);
+        if (d) (void)hipFree(d);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        return rc;
+    });
 }
 
 }  // extern "C"

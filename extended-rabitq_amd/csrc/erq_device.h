@@ -229,15 +229,18 @@ __device__ __forceinline__ void prepare_residual(const float* __restrict__ q,
 // 8 dims 512 c + 128 g + 32 jw + 8 b + 2 pp + {0, 1}; task t of a wave has jw = t & 3,
 // pp = (t >> 2) & 3, g = (t >> 4) & 3, c = t >> 6 (lane-major u: conflict-free LDS reads
 // within 16 lanes).  Dims >= D (a whole 32-dim block, D being a multiple of 64) get 0.
-// dst is the pair's 2 * nchunk * 512 bytes as dwords.
+// dst is the pair's 2 * nchunk * 512 bytes as dwords.  qv (nullable) also receives the
+// digits q themselves as int16 in the bit-plane order of the codes (dim 512 c + 8 B + p at
+// qv[(64 c + B) * 8 + p]), which the wrap correction reads.
 __device__ __forceinline__ void quantize_digits(const float* uq, int S, int D, int nchunk,
-                                                double rd, uint32_t* dst, int lane) {
+                                                double rd, uint32_t* dst, int lane,
+                                                int16_t* qv = nullptr) {
     for (int t = lane; t < nchunk * 64; t += kWave) {
         const int jw = t & 3, pp = (t >> 2) & 3, g = (t >> 4) & 3, c = t >> 6;
         const int base = 512 * c + 128 * g + 32 * jw + 2 * pp;
         uint32_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+        int q0[4] = {0, 0, 0, 0}, q1[4] = {0, 0, 0, 0};
         if (base < D) {
-            int q0[4], q1[4];
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const int d = base + 8 * b;
@@ -255,6 +258,78 @@ __device__ __forceinline__ void quantize_digits(const float* uq, int S, int D, i
         dst[8 * un + 4 + jw] = hi0;
         dst[8 * un + 32 + jw] = lo1;
         dst[8 * un + 36 + jw] = hi1;
+        if (qv) {
+            // byte B = 16 g + 4 jw + b, planes 2 pp and 2 pp + 1: one dword per b
+            uint32_t* qd = reinterpret_cast<uint32_t*>(qv);
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                qd[(64 * c + 16 * g + 4 * jw + b) * 4 + pp] =
+                    ((uint32_t)q0[b] & 0xFFFFu) | ((uint32_t)q1[b] << 16);
+        }
+    }
+}
+
+// ---- the reference's u16 wraparound above D = 2048 (FastScan.hpp:237-290) ----
+// The reference adds the low and the high bytes of the shifted LUT entries
+// L'_m[code] = sum_{k<4} bit(4m+k) q_{4m+k} - sum_{k<4} min(0, q_{4m+k})
+//            = sum_{k<4} (bit(4m+k) ? max(q_{4m+k}, 0) : max(-q_{4m+k}, 0))
+// in u16 lanes, separately for even and odd codebooks, and takes each of the four half-sums
+// mod 2^16.  A high byte is at most 127 (L' <= 4 * 8191), so the high half-sums stay below
+// 2^16 for D <= 4096; a low half-sum reaches 2^16 only above D = 2048 and stays below 2^17.
+// So the reference's integer is the exact inner product minus 65536 per low half-sum that
+// reached 2^16, and that integer is never larger than the exact one: a vector the lower
+// bound rejects on the exact integer is rejected by the reference too.
+// lo_wraps returns the number of wrapped low half-sums (0, 1 or 2) of one vector for the
+// 16 lanes of a group (l = lane & 15; every lane of the group gets the result).  code(c)
+// returns the lane's dword of chunk c of the vector's bit-plane codes (bytes 4l .. 4l+3,
+// byte B bit p = dim 512 c + 8 B + p), qv the digits as quantize_digits stores them.  In a
+// byte, bits 0..3 belong to an even codebook, bits 4..7 to the next (odd) one.
+template <typename CF>
+__device__ __forceinline__ int lo_wraps(const int16_t* __restrict__ qv, int nchunk, int l,
+                                        const CF& code) {
+    int se = 0, so = 0;
+    for (int c = 0; c < nchunk; c++) {
+        const uint32_t dw = code(c);
+        const uint4* qq = reinterpret_cast<const uint4*>(qv + (size_t)(64 * c + 4 * l) * 8);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const uint4 w = qq[b];  // dims 8 B .. 8 B + 7 of byte B = 4 l + b, two per dword
+            const uint32_t x = dw >> (8 * b);
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+            int e = 0, o = 0;
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const int qa = (int)(int16_t)(ws[h] & 0xFFFFu), qb = (int)(int16_t)(ws[h] >> 16);
+                const int va = max(((x >> (2 * h)) & 1u) ? qa : -qa, 0);
+                const int vb = max(((x >> (2 * h + 1)) & 1u) ? qb : -qb, 0);
+                if (h < 2) e += va + vb; else o += va + vb;
+            }
+            se += e & 255;
+            so += o & 255;
+        }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+        se += __shfl_xor(se, off);
+        so += __shfl_xor(so, off);
+    }
+    return (se >> 16) + (so >> 16);
+}
+
+// The digits q alone (no MFMA operands), in quantize_digits' qv order, from the lane-major
+// residual: dims 512 c + 8 B + 2 h + {0, 1} as one dword per task.
+__device__ __forceinline__ void store_qv(const float* uq, int S, int D, int nchunk, double rd,
+                                         int16_t* qv, int lane) {
+    uint32_t* qd = reinterpret_cast<uint32_t*>(qv);
+    for (int t = lane; t < nchunk * 256; t += kWave) {
+        const int d = 2 * t;  // = 512 c + 8 B + 2 h
+        uint32_t v = 0;
+        if (d < D) {
+            const int q0 = quantize_digit(uq[ul_index(d, S)], rd);
+            const int q1 = quantize_digit(uq[ul_index(d + 1, S)], rd);
+            v = ((uint32_t)q0 & 0xFFFFu) | ((uint32_t)q1 << 16);
+        }
+        qd[t] = v;
     }
 }
 

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <new>
 
 #include "erq.h"
 #include <string>
@@ -44,7 +45,21 @@ struct erq_index {
     // while the exact ranks run on the caller's stream (fork and join by events)
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    std::mutex mu;
+    // ordering between calls on different streams: every call records ev_last on its stream
+    // when it returns, and the next call on another stream waits for it first (the buffers
+    // above are shared by all calls on the handle)
+    hipEvent_t ev_last = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool last_valid = false;
+    // device buffers of the host-pointer calls (erq_search, erq_search_batch*), grow-only
+    float* hq = nullptr;         // hq_cap x D rotated queries
+    uint32_t* hids = nullptr;    // hq_cap x hk
+    float* hdists = nullptr;     // hq_cap x hk
+    uint32_t* hnp = nullptr;     // hq_cap + 1 (per-query probe counts, then CSR offsets)
+    size_t hq_cap = 0, hk_cap = 0;
+    int depth = 0;  // nesting of entry points on this thread (only the outermost orders)
+    // recursive: the host-pointer calls hold it while they call the device-pointer ones
+    std::recursive_mutex mu;
 };
 
 namespace erq {
@@ -78,8 +93,18 @@ struct ScanArgs {
     int waves_per_block;  // queries (waves) per workgroup
     float one_over_sqrtD;
     const uint32_t* qorder;  // nullable: the order queries are taken in (a permutation of nq)
+    // Round mode (the batched search's redo of a round whose candidates overflowed): the
+    // queries qlist[1 .. qlist[0]] rescan probe ranks [r0, r1) against the k-th distance of
+    // the pool in out_ids / out_dists (fixed for the round), and merge every candidate whose
+    // estimate is at most that distance into the pool by (distance, id).
+    const uint32_t* qlist;
+    int r0, r1;
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
+// The round-mode rescan (a.qlist set): a fixed grid of waves that take the listed queries.
+hipError_t launch_scan_round(const ScanArgs& a, hipStream_t s);
+// Bytes of LDS the round-mode kernel needs per wave (D, k), and whether it fits.
+size_t scan_round_wave_bytes(int D, int k);
 // qorder[nq] = the queries by scan cost, heaviest first (cost: the 32-vector blocks of the
 // probed clusters, the first max_probes when > 0, shifted right by `shift`, K bins);
 // scratch holds K + 1 u32.
@@ -100,6 +125,12 @@ hipError_t launch_pad_rows(const float* src, int rows, int cols, float* dst, int
 hipError_t launch_probe_select(const float* approx, const float* qnorm, const float* cnorm,
                                const float* q_rot, const float* centroids, int nq, int K, int D,
                                int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s);
+
+// CSR probe lists (ids and distances) from padded per-query rows of `stride` entries: query q
+// keeps the first off[q + 1] - off[q] of its row.
+hipError_t launch_compact_probes(const uint32_t* pad_ids, const float* pad_d, int stride,
+                                 const uint32_t* off, int nq, uint32_t* csr_ids, float* csr_d,
+                                 hipStream_t s);
 
 hipError_t launch_adaptive_probe(const float* prob, const float* q_rot, const float* centroids,
                                  int nq, int K, int D, float mass, int min_np, int max_np,

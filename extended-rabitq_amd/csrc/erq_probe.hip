@@ -401,6 +401,15 @@ hipError_t launch_adaptive_probe(const float* prob, const float* q_rot, const fl
     return hipGetLastError();
 }
 
+hipError_t launch_compact_probes(const uint32_t* pad_ids, const float* pad_d, int stride,
+                                 const uint32_t* off, int nq, uint32_t* csr_ids, float* csr_d,
+                                 hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_probes_kernel, dim3(nq), dim3(64), 0, s, pad_ids, pad_d, stride,
+                       off, nq, csr_ids, csr_d);
+    return hipGetLastError();
+}
+
 size_t probe_select_lds(int K, int D) { return 4 * (size_t)K + 8 * (size_t)kCapC + 4 * (size_t)D; }
 
 hipError_t launch_probe_select(const float* approx, const float* qnorm, const float* cnorm,

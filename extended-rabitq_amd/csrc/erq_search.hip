@@ -9,6 +9,7 @@
 //   lut[M][16]     u16 shifted subset sums           pack_high_acc_LUT, HASearcher.hpp:128-185
 // then scans 32-vector blocks straight from the HA layout in HBM (4D code bytes +
 // 32 float factors per block) and re-ranks survivors from their EX-bit long codes.
+#include "erq_batch.h"
 #include "erq_device.h"
 #include "erq_internal.h"
 
@@ -430,12 +431,14 @@ __global__ void __launch_bounds__(256) scan_kernel(ScanArgs a, int per_wave_byte
     }
 }
 
-// Same replay as scan_kernel, for D <= 2048 (no u16 wrap), with the FastScan integer
-// taken from the bit-plane codes on v_mfma_i32_16x16x64_i8: A rows 0/1 hold the query's
-// low/high int8 digits (q = 128*hi + lo), rows 2..15 are zero; B is bit-plane p of 16
-// vectors' code bytes (one v_and per dword); I = sum_p +-(acc_p >> p) equals the LUT sum
-// plus shift exactly.  Prep (normalize, quantize), lower bound, mask, survivor re-rank
-// and pool are the per-query kernel's.
+// The same replay with the FastScan integer taken from the bit-plane codes on
+// v_mfma_i32_16x16x64_i8: A rows 0/1 hold the query's low/high int8 digits (q = 128*hi + lo),
+// rows 2..15 are zero; B is bit-plane p of 16 vectors' code bytes (one v_and per dword);
+// I = sum_p +-(acc_p >> p) is the exact inner product (LUT sum plus shift).  Above D = 2048
+// the reference's u16 lanes can wrap: the vectors whose lower bound passes on the exact
+// integer (a superset of the reference's, see lo_wraps) get the reference's integer from
+// their low-byte half-sums before the mask is final.  Prep (normalize, quantize), lower
+// bound, mask, survivor re-rank and pool are the LUT kernel's.
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 // The re-estimated distance of a survivor in the reference's operation order
@@ -453,18 +456,85 @@ __device__ __forceinline__ float ex_distance(float on, float sqr_y, float Bb, fl
     return (float)((double)Aa - H);
 }
 
-template <int EX, int NCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    const int qi = query_of_wave(a, wid);
-    if (qi < 0) return;
+// (distance, id) order of the batched rounds' merge.
+__device__ __forceinline__ bool did_less(float da, uint32_t ia, float db, uint32_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+// ns new entries (nd, nid, ns <= 32, ids distinct and absent from the pool) merged into a
+// pool sorted by (distance, id), keeping its k smallest: new entry t lands after the pool
+// entries and the new entries below it, pool entry i moves up by the new entries below it.
+// nd / nid are overwritten (sorted).
+__device__ __forceinline__ void pool_merge_did(const WaveLds& w, int& psize, int k, int lane,
+                                               float* nd, uint32_t* nid, int ns) {
+    float dt = 0.f;
+    uint32_t it = 0;
+    int rt = 0, at = k;
+    if (lane < ns) {
+        dt = nd[lane];
+        it = nid[lane];
+        for (int u = 0; u < ns; u++) rt += did_less(nd[u], nid[u], dt, it) ? 1 : 0;
+        int lo = 0, hi = psize;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (did_less(w.pd[mid], w.pid[mid], dt, it)) lo = mid + 1; else hi = mid;
+        }
+        at = lo + rt;
+    }
+    wave_sync();
+    if (lane < ns) {
+        nd[rt] = dt;
+        nid[rt] = it;
+    }
+    wave_sync();
+    for (int base = (psize - 1) & ~(kWave - 1); psize > 0 && base >= 0; base -= kWave) {
+        const int i = base + lane;
+        float vd = 0.f;
+        uint32_t vid = 0;
+        int dst = k;
+        if (i < psize) {
+            vd = w.pd[i];
+            vid = w.pid[i];
+            int lo = 0, hi = ns;  // new entries below (vd, vid)
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (did_less(nd[mid], nid[mid], vd, vid)) lo = mid + 1; else hi = mid;
+            }
+            dst = i + lo;
+        }
+        wave_sync();
+        if (dst < k) {
+            w.pd[dst] = vd;
+            w.pid[dst] = vid;
+        }
+        wave_sync();
+    }
+    if (at < k) {
+        w.pd[at] = dt;
+        w.pid[at] = it;
+    }
+    wave_sync();
+    psize = min(k, psize + ns);
+}
+
+enum { kModeReplay = 0, kModeRound = 1 };
+
+// LDS of one wave of scan_mfma_kernel: the lane-major residual, the A operand and its zero
+// twin, the digits for the wrap correction (NCH > 4), the block's integer sums, the pool and
+// a block's new entries.
+__host__ __device__ inline size_t mfma_wave_bytes(int D, int nchunk, int k) {
+    const size_t kcap = (size_t)((k + 63) & ~63);
+    return 4 * 16 * (size_t)ul_stride(D) + 2048 * (size_t)nchunk +
+           (nchunk > 4 ? 1024 * (size_t)nchunk : 0) + 128 + 8 * kcap + 256;
+}
+
+template <int EX, int NCH, int MODE>
+__device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsigned char* base,
+                                                int lane) {
     const int D = a.D, k = a.k;
     constexpr int nchunk = NCH;
+    constexpr bool kWrap = NCH > 4;  // D > 2048
     const int kcap = (k + 63) & ~63;
-    unsigned char* base = smem + (size_t)wid * per_wave_bytes;
     // the residual u, lane-major (u[16 i + l] at uq[l * S + i]) for the re-estimate's lanes
     const int S = ul_stride(D);
     float* uq = reinterpret_cast<float*>(base);
@@ -473,7 +543,9 @@ scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
     // nchunk * 1 KB of zeros laid out like A: lanes holding MFMA rows 2..15 read their
     // (zero) A operand from here at the same offsets, so every lane loads unconditionally
     int8_t* Z = A + 1024 * (size_t)nchunk;
-    int* sums = reinterpret_cast<int*>(base + ubytes + 2048 * (size_t)nchunk);
+    int16_t* qv = reinterpret_cast<int16_t*>(base + ubytes + 2048 * (size_t)nchunk);
+    int* sums = reinterpret_cast<int*>(base + ubytes + 2048 * (size_t)nchunk +
+                                       (kWrap ? 1024 * (size_t)nchunk : 0));
     WaveLds w;
     w.uq = uq;
     w.lut = nullptr;
@@ -482,28 +554,81 @@ scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
     w.pid = reinterpret_cast<uint32_t*>(w.pd + kcap);
     w.nd = reinterpret_cast<float*>(w.pid + kcap);
     w.nid = reinterpret_cast<uint32_t*>(w.nd + 32);
-    for (int i = lane; i < kcap; i += kWave) {
-        w.pd[i] = FLT_MAX;
-        w.pid[i] = 0;
-    }
     for (int i = lane; i < nchunk * 64; i += kWave)
         reinterpret_cast<uint4*>(Z)[i] = make_uint4(0, 0, 0, 0);
     int psize = 0;
-    wave_sync();
+    float thr = FLT_MAX;  // round mode: the pool's k-th distance at round start
+    if (MODE == kModeReplay) {
+        for (int i = lane; i < kcap; i += kWave) {
+            w.pd[i] = FLT_MAX;
+            w.pid[i] = 0;
+        }
+        wave_sync();
+    } else {
+        // the pool at round start (sorted by distance, FLT_MAX-padded); equal distances are
+        // put in id order, as the round's merge orders its pool
+        const float* gd = a.out_dists + (size_t)qi * k;
+        const uint32_t* gi = a.out_ids + (size_t)qi * k;
+        int cnt = 0;
+        for (int i = lane; i < kcap; i += kWave) {
+            const float d = i < k ? gd[i] : FLT_MAX;
+            w.pd[i] = d;
+            w.pid[i] = d < FLT_MAX ? gi[i] : 0xFFFFFFFFu;
+            cnt += d < FLT_MAX ? 1 : 0;
+        }
+        psize = wave_sum_i(cnt);
+        wave_sync();
+        constexpr int kMaxOwn = 16;  // k <= 1024
+        float od[kMaxOwn];
+        uint32_t oid[kMaxOwn];
+        int opos[kMaxOwn];
+#pragma unroll
+        for (int r = 0; r < kMaxOwn; r++) {
+            const int i = lane + kWave * r;
+            opos[r] = -1;
+            if (i < psize) {
+                const float d = w.pd[i];
+                const uint32_t id = w.pid[i];
+                int lo = 0, hi = psize;  // first entry with distance d
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (w.pd[mid] < d) lo = mid + 1; else hi = mid;
+                }
+                int pos = lo;
+                for (int j = lo; j < psize && w.pd[j] == d; j++) pos += w.pid[j] < id ? 1 : 0;
+                od[r] = d;
+                oid[r] = id;
+                opos[r] = pos;
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < kMaxOwn; r++)
+            if (opos[r] >= 0) {
+                w.pd[opos[r]] = od[r];
+                w.pid[opos[r]] = oid[r];
+            }
+        wave_sync();
+        thr = psize == k ? w.pd[k - 1] : FLT_MAX;
+    }
 
     const float* q = a.q_rot + (size_t)qi * D;
     const size_t BB = 4 * (size_t)D + 128;
-    const size_t CL = (size_t)D * EX / 8;
-    const int R = 1 << EX;
     const float const_bound = 0.58f;  // FastScan.hpp:295
     unsigned long long st_blocks = 0, st_vec = 0, st_surv = 0;
     const int g = lane >> 4, l16 = lane & 15;
     // A operand of this lane at (chunk, plane) = abase + (chunk * 8 + plane) * 128
     const int8_t* abase = l16 < 2 ? A + (g * 2 + l16) * 16 : Z + g * 32;
 
-    const uint32_t p0 = a.probe_off[qi];
+    uint32_t p0 = a.probe_off[qi];
     uint32_t p1 = a.probe_off[qi + 1];
-    if (a.max_probes > 0 && p1 - p0 > (uint32_t)a.max_probes) p1 = p0 + (uint32_t)a.max_probes;
+    if (MODE == kModeReplay) {
+        if (a.max_probes > 0 && p1 - p0 > (uint32_t)a.max_probes) p1 = p0 + (uint32_t)a.max_probes;
+    } else {
+        p1 = min(p1, p0 + (uint32_t)a.r1);
+        p0 = p0 + (uint32_t)a.r0;
+    }
+    const uint32_t np_total = p1 > p0 ? p1 - p0 : 0;
     for (uint32_t p = p0; p < p1; ++p) {
         const uint32_t cid = a.probe_ids[p];
         if (cid >= (uint32_t)a.K) continue;
@@ -513,14 +638,14 @@ scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
         float fac, sumq, delta;
         prepare_residual(q, c, y, D, S, uq, lane, fac, sumq, delta);
         quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta, reinterpret_cast<uint32_t*>(A),
-                        lane);
+                        lane, kWrap ? qv : nullptr);
         wave_sync();
 
         const int n = (int)a.csize[cid];
         const int nb = (n + 31) >> 5;
         const uint32_t blk0 = a.bstart[cid];
         const uint32_t v0 = a.vstart[cid];
-        float distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+        float distk = MODE == kModeReplay ? ((psize == k) ? w.pd[k - 1] : FLT_MAX) : thr;
         const float c1 = (float)(0.5 * (double)sumq - (double)const_bound);
         const float c2 = (-5.0f * y) * a.one_over_sqrtD;
         const float qs = y * y;
@@ -587,23 +712,49 @@ scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
             wave_sync();
             bool cand = false;
             float ipq_l = 0.f;
-            if (lane < 32) {
-                ipq_l = (float)sums[lane] * delta;
-                const float on = on_l;
-                float t = ipq_l - c1;
-                t = t * c2;
-                t = t * on;
-                t = t + qs;
-                t = t + on * on;
-                cand = (lane < np) && (t < distk);
-            }
+            auto lower_test = [&]() {
+                cand = false;
+                if (lane < 32) {
+                    ipq_l = (float)sums[lane] * delta;
+                    const float on = on_l;
+                    float t = ipq_l - c1;
+                    t = t * c2;
+                    t = t * on;
+                    t = t + qs;
+                    t = t + on * on;
+                    cand = (lane < np) && (t < distk);
+                }
+            };
+            lower_test();
             uint64_t mask = ballot(cand);
+            if constexpr (kWrap) {
+                // the reference's integer for the vectors that pass on the exact one (four at
+                // a time, one per 16-lane group), then the lower bound again
+                if (mask) {
+                    for (uint64_t mw = mask; mw;) {
+                        uint64_t mm = mw;
+                        for (int r = 0; r < g; r++) mm &= mm - 1;
+                        const int j = mm ? (__ffsll((unsigned long long)mm) - 1) : -1;
+                        const int jj = j < 0 ? 0 : j;
+                        const int wr = lo_wraps(qv, NCH, l16, [&](int ch) {
+                            return *reinterpret_cast<const uint32_t*>(
+                                a.bp + (((size_t)blk * NCH + ch) * 32 + jj) * 64 + 4 * l16);
+                        });
+                        if (j >= 0 && l16 == 0 && wr) sums[j] -= 65536 * wr;
+                        for (int r = 0; r < 4; r++) mw &= mw - 1;
+                    }
+                    wave_sync();
+                    lower_test();
+                    mask = ballot(cand);
+                }
+            }
             st_blocks++;
             st_vec += np;
             // the block's survivors, eight at a time (group g takes the g-th and (g + 4)-th
             // remaining set bits), re-estimated into the new-entry list in bit order, then
             // inserted at once
             const int ns = __popcll(mask);
+            int nn = 0;  // round mode: new entries (estimate <= threshold)
             for (int s0 = 0; s0 < ns; s0 += 8) {
                 uint64_t mm = mask;
                 for (int r = 0; r < g; r++) mm &= mm - 1;
@@ -632,25 +783,55 @@ scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
                 const float ip = butterfly16(part), ip2 = butterfly16(part2);
                 const float ipq_j = __shfl(ipq_l, jj), ipq_j2 = __shfl(ipq_l, jj2);
                 const float on_j = __shfl(on_l, jj), on_j2 = __shfl(on_l, jj2);
-                if (l16 == 0) {
-                    if (j >= 0) {
-                        w.nd[s0 + g] = ex_distance<EX>(on_j, sqr_y, exf * y, ipq_j, ip, sumq);
-                        w.nid[s0 + g] = idv;
+                if (MODE == kModeReplay) {
+                    if (l16 == 0) {
+                        if (j >= 0) {
+                            w.nd[s0 + g] = ex_distance<EX>(on_j, sqr_y, exf * y, ipq_j, ip, sumq);
+                            w.nid[s0 + g] = idv;
+                        }
+                        if (j2 >= 0) {
+                            w.nd[s0 + 4 + g] =
+                                ex_distance<EX>(on_j2, sqr_y, exf2 * y, ipq_j2, ip2, sumq);
+                            w.nid[s0 + 4 + g] = idv2;
+                        }
                     }
-                    if (j2 >= 0) {
-                        w.nd[s0 + 4 + g] =
-                            ex_distance<EX>(on_j2, sqr_y, exf2 * y, ipq_j2, ip2, sumq);
-                        w.nid[s0 + 4 + g] = idv2;
+                } else {
+                    // candidates: estimate <= the round's threshold, kept in bit order
+                    const float e1 = ex_distance<EX>(on_j, sqr_y, exf * y, ipq_j, ip, sumq);
+                    const float e2 = ex_distance<EX>(on_j2, sqr_y, exf2 * y, ipq_j2, ip2, sumq);
+                    const bool k1 = j >= 0 && e1 <= thr, k2 = j2 >= 0 && e2 <= thr;
+                    const uint64_t b1 = ballot(k1 && l16 == 0), b2 = ballot(k2 && l16 == 0);
+                    const int c1n = __popcll(b1), c2n = __popcll(b2);
+                    if (l16 == 0) {
+                        // group g's slot: after the groups before it (first survivors, then
+                        // the second ones), in bit order of the survivors
+                        const uint64_t below = (1ull << (16 * g)) - 1;
+                        if (k1) {
+                            const int at = nn + __popcll(b1 & below);
+                            w.nd[at] = e1;
+                            w.nid[at] = idv;
+                        }
+                        if (k2) {
+                            const int at = nn + c1n + __popcll(b2 & below);
+                            w.nd[at] = e2;
+                            w.nid[at] = idv2;
+                        }
                     }
+                    nn += c1n + c2n;
                 }
                 for (int r = 0; r < 8; r++) mask &= mask - 1;
             }
-            if (ns) {
+            if (MODE == kModeReplay) {
+                if (ns) {
+                    wave_sync();
+                    pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
+                }
+                st_surv += ns;
+                distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+            } else if (nn) {
                 wave_sync();
-                pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
+                pool_merge_did(w, psize, k, lane, w.nd, w.nid, nn);
             }
-            st_surv += ns;
-            distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
         }
     }
     uint32_t* oi = a.out_ids + (size_t)qi * k;
@@ -659,30 +840,58 @@ scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
         oi[i] = i < psize ? w.pid[i] : 0u;
         if (od) od[i] = i < psize ? w.pd[i] : FLT_MAX;
     }
-    if (a.stats && lane == 0) {
+    if (MODE == kModeReplay && a.stats && lane == 0) {
         atomicAdd(&a.stats[0], st_blocks);
         atomicAdd(&a.stats[1], st_vec);
         atomicAdd(&a.stats[2], st_surv);
-        atomicAdd(&a.stats[3], (unsigned long long)(p1 - p0));
+        atomicAdd(&a.stats[3], (unsigned long long)np_total);
     }
+}
+
+template <int EX, int NCH, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NCH > 4 ? 2 : 3)))
+scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    unsigned char* base = smem + (size_t)wid * per_wave_bytes;
+    if (MODE == kModeReplay) {
+        const int qi = query_of_wave(a, wid);
+        if (qi < 0) return;
+        scan_query_mfma<EX, NCH, MODE>(a, qi, base, lane);
+    } else {
+        const uint32_t nl = a.qlist[0];
+        const uint32_t wpb = blockDim.x >> 6;
+        for (uint32_t i = blockIdx.x * wpb + (uint32_t)wid; i < nl; i += gridDim.x * wpb) {
+            scan_query_mfma<EX, NCH, MODE>(a, (int)a.qlist[1 + i], base, lane);
+            wave_sync();
+        }
+    }
+}
+
+template <int EX, int MODE>
+hipError_t launch_mfma(const ScanArgs& a, dim3 grid, dim3 block, size_t lds, int per_wave,
+                       hipStream_t s) {
+    switch (a.nchunk) {
+    case 1: hipLaunchKernelGGL((scan_mfma_kernel<EX, 1, MODE>), grid, block, lds, s, a, per_wave); break;
+    case 2: hipLaunchKernelGGL((scan_mfma_kernel<EX, 2, MODE>), grid, block, lds, s, a, per_wave); break;
+    case 3: hipLaunchKernelGGL((scan_mfma_kernel<EX, 3, MODE>), grid, block, lds, s, a, per_wave); break;
+    case 4: hipLaunchKernelGGL((scan_mfma_kernel<EX, 4, MODE>), grid, block, lds, s, a, per_wave); break;
+    case 6: hipLaunchKernelGGL((scan_mfma_kernel<EX, 6, MODE>), grid, block, lds, s, a, per_wave); break;
+    case 8: hipLaunchKernelGGL((scan_mfma_kernel<EX, 8, MODE>), grid, block, lds, s, a, per_wave); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 template <int EX>
 hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
     int waves = a.waves_per_block > 0 ? a.waves_per_block : 1;
-    if (a.bp && a.D <= 2048) {
-        const size_t per_wave = 4 * 16 * (size_t)ul_stride(a.D) + 2048 * (size_t)a.nchunk + 128 +
-                                (size_t)8 * ((a.k + 63) & ~63) + 256;
+    if (a.bp) {
+        const size_t per_wave = mfma_wave_bytes(a.D, a.nchunk, a.k);
         while (waves > 1 && per_wave * waves > 65536) waves--;
         dim3 grid((a.nq + waves - 1) / waves), block(64 * waves);
-        switch (a.nchunk) {
-        case 1: hipLaunchKernelGGL((scan_mfma_kernel<EX, 1>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
-        case 2: hipLaunchKernelGGL((scan_mfma_kernel<EX, 2>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
-        case 3: hipLaunchKernelGGL((scan_mfma_kernel<EX, 3>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
-        case 4: hipLaunchKernelGGL((scan_mfma_kernel<EX, 4>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
-        default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
+        return launch_mfma<EX, kModeReplay>(a, grid, block, per_wave * waves, (int)per_wave, s);
     }
     const size_t per_wave = (size_t)12 * a.D + 128 + (size_t)8 * ((a.k + 63) & ~63);
     while (waves > 1 && per_wave * waves > 65536) waves--;
@@ -693,6 +902,16 @@ hipError_t launch_ex(const ScanArgs& a, hipStream_t s) {
     else
         hipLaunchKernelGGL((scan_kernel<EX, false>), grid, block, lds, s, a, (int)per_wave);
     return hipGetLastError();
+}
+
+template <int EX>
+hipError_t launch_round_ex(const ScanArgs& a, hipStream_t s) {
+    const size_t per_wave = mfma_wave_bytes(a.D, a.nchunk, a.k);
+    const int waves = per_wave * 2 <= 65536 ? 2 : 1;
+    // a fixed grid: the listed queries are few (rounds whose candidates overflowed), and the
+    // list's length is only known on the device
+    dim3 grid(256), block(64 * waves);
+    return launch_mfma<EX, kModeRound>(a, grid, block, per_wave * waves, (int)per_wave, s);
 }
 
 }  // namespace
@@ -709,6 +928,21 @@ hipError_t launch_query_order(const uint32_t* probe_off, const uint32_t* probe_i
     hipLaunchKernelGGL(order_scatter_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, probe_off,
                        probe_ids, csize, nq, K, max_probes, shift, scratch, qorder);
     return hipGetLastError();
+}
+
+size_t scan_round_wave_bytes(int D, int k) { return mfma_wave_bytes(D, bp_nchunk(D), k); }
+
+hipError_t launch_scan_round(const ScanArgs& a, hipStream_t s) {
+    if (!a.bp || !a.qlist) return hipErrorInvalidValue;
+    switch (a.EX) {
+    case 2: return launch_round_ex<2>(a, s);
+    case 3: return launch_round_ex<3>(a, s);
+    case 4: return launch_round_ex<4>(a, s);
+    case 6: return launch_round_ex<6>(a, s);
+    case 7: return launch_round_ex<7>(a, s);
+    case 8: return launch_round_ex<8>(a, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s) {

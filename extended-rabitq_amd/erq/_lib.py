@@ -33,6 +33,8 @@ SIGNATURES = {
     "erq_rotate": (c_int, [c_vp, c_vp, c_size, c_vp]),
     "erq_search": (c_int, [c_vp, c_vp, c_size, c_size, c_vp, c_vp]),
     "erq_search_batch": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp]),
+    "erq_search_batch_nprobe": (c_int, [c_vp, c_vp, c_size, c_size, c_vp, c_vp, c_vp,
+                                        ctypes.c_uint]),
     "erq_rotate_dev": (c_int, [c_vp, c_vp, c_size, c_vp, c_vp]),
     "erq_centroid_dist_dev": (c_int, [c_vp, c_vp, c_size, c_vp, c_vp]),
     "erq_probe_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_vp, c_vp, c_vp]),

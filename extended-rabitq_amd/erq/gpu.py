@@ -122,6 +122,20 @@ class IVF:
         check(_lib.lib().erq_search_batch(self._h, _ptr(q), nq, k, nprobe, _ptr(ids), _ptr(d)))
         return ids, d
 
+    def search_batch_nprobe(self, q_rot, k, nprobe_per_q, batched=False):
+        """IVF::search over many queries, each with its own nprobe (host arrays); batched
+        selects the cluster-major rounds instead of the exact per-query replay."""
+        q = np.ascontiguousarray(q_rot, dtype=np.float32)
+        npq = np.ascontiguousarray(nprobe_per_q, dtype=np.uint32)
+        nq = q.shape[0]
+        if npq.shape != (nq,):
+            raise ValueError("one nprobe per query")
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        check(_lib.lib().erq_search_batch_nprobe(self._h, _ptr(q), nq, k, _ptr(npq), _ptr(ids),
+                                                 _ptr(d), 1 if batched else 0))
+        return ids, d
+
     # ---- device path (torch CUDA tensors; enqueued on torch's current stream) ----
     def rotate_dev(self, q):
         import torch

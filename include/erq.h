@@ -12,7 +12,11 @@
  * synchronising.  The batched searches fork part of their work onto a second stream of the
  * handle and join it back into `stream` before returning (events, so a HIP graph capture
  * of `stream` takes both).
- * Thread safety: calls on one handle are serialised by a per-handle mutex.
+ * Thread safety: calls on one handle are serialised by a per-handle mutex, and a call on one
+ * stream is ordered after the device work of the previous call on the handle when that ran
+ * on another stream (an event each call records when it returns).  Calls captured into a HIP
+ * graph are not ordered that way: the graph's launch orders them, and a replay must not
+ * overlap another call on the same handle.
  */
 #ifndef ERQ_H
 #define ERQ_H
@@ -37,10 +41,10 @@ typedef struct erq_index erq_index;
 
 /* Options (erq_set_option). */
 enum {
-    /* Integer kernel of the per-query scan: 0 auto (int8 MFMA over the bit-plane copy of
-     * the codes when D <= 2048, else LUT), 1 LUT, 2 MFMA where D <= 2048.  Results are
-     * identical; only speed and the extra bit-plane copy (one more 1-bit code array in
-     * HBM) differ. */
+    /* Integer kernel of the per-query scan: 0 auto (= 2), 1 LUT, 2 int8 MFMA over the
+     * bit-plane copy of the codes (above D = 2048 with the reference's u16 wraparound
+     * reproduced for every vector the lower bound passes).  Results are identical; only
+     * speed and the extra bit-plane copy (one more 1-bit code array in HBM) differ. */
     ERQ_OPT_PLAIN_KERNEL = 1,
     /* Queries (waves) per workgroup of the per-query scan, 1..4 (LDS permitting). */
     ERQ_OPT_PLAIN_WAVES = 2,
@@ -104,9 +108,19 @@ int erq_rotate(erq_index* h, const float* q, size_t nq, float* q_rot);
 int erq_search(erq_index* h, const float* q_rot, size_t k, size_t nprobe, uint32_t* results,
                float* dists);
 
-/* IVF::search over nq queries (host pointers), fixed nprobe. */
+/* IVF::search over nq queries (host pointers), fixed nprobe.  erq_search and this call
+ * keep their device buffers in the handle (grown on demand, never per call). */
 int erq_search_batch(erq_index* h, const float* q_rot, size_t nq, size_t k, size_t nprobe,
                      uint32_t* results, float* dists);
+
+/* IVF::search over nq queries (host pointers) with each query's own nprobe
+ * (nprobe_per_q[i] in [1, min(K, 2048)]): flags 0 runs the exact per-query replay (each
+ * query's result equals IVF::search with its nprobe), ERQ_SEARCH_BATCHED the cluster-major
+ * rounds of erq_scan_batched_dev. */
+enum { ERQ_SEARCH_BATCHED = 1 };
+int erq_search_batch_nprobe(erq_index* h, const float* q_rot, size_t nq, size_t k,
+                            const uint32_t* nprobe_per_q, uint32_t* results, float* dists,
+                            unsigned flags);
 
 /* ---- device-pointer stages of the same path (enqueue on `stream`) ---- */
 
@@ -143,11 +157,12 @@ int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, si
  * reading every probed cluster's codes once for all queries that probe it (FastScan sums
  * on int8 MFMA) and pruning against each query's k-th distance at round start.
  * max_nprobe bounds every query's probe count.  Results are deterministic; with at most 4
- * probes per query they equal erq_scan_dev's.  For D > 2048 the exact per-query replay runs
- * instead.  dists_dev is required.  stats_dev (nullable) takes 8 counters:
- * [0] blocks, [1] vectors, [3] probes of round 0; [2] survivors re-ranked (all rounds);
- * [4] candidates kept, [5] candidates dropped on buffer overflow, [6] 16-vector tiles
- * scanned in rounds >= 1. */
+ * probes per query they equal erq_scan_dev's.  No candidate is ever dropped: a query whose
+ * round produced more candidates than the per-query buffer holds has that round rescanned
+ * alone with an exact per-query pool.  dists_dev is required.  stats_dev (nullable) takes 8
+ * counters: [0] blocks, [1] vectors, [3] probes of round 0; [2] survivors re-ranked (all
+ * rounds); [4] candidates kept, [5] (query, round) rescans after a candidate overflow, [6]
+ * 16-vector tiles scanned in rounds >= 1. */
 int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                          const uint32_t* probe_off_dev, const uint32_t* probe_ids_dev,
                          const float* probe_sqr_y_dev, size_t max_nprobe, uint32_t* ids_dev,

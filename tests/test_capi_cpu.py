@@ -88,3 +88,39 @@ def test_null_arguments_rejected():
     assert L.erq_index_load(None, None) == 1
     assert L.erq_search(None, None, 10, 1, None, None) == 1
     assert L.erq_padded_dim(None) == 0
+
+
+def test_huge_header_is_rejected_before_allocating(tmp_path):
+    """A header whose sections would need ~10^13 bytes, with cluster sizes that do sum to N:
+    the file-size check returns ERQ_EIO before any buffer is sized from the header (an
+    allocation that size would throw across the C ABI)."""
+    p = tmp_path / "h.index"
+    N = (1 << 32) - 1
+    with open(p, "wb") as f:
+        np.array([N, 4096, 2, 8], dtype="<u8").tofile(f)
+        np.array([N - 7, 7], dtype="<u8").tofile(f)
+        f.write(b"\0" * 4096)
+    rc, msg = _load_rc(str(p))
+    assert rc == 2 and "truncated" in msg
+
+
+def test_trailing_bytes_are_eformat(small_data, tmp_path):
+    from erq import index
+    x, q, c, assign = small_data
+    idx = index.build_index(x[:2000], c, assign[:2000], bits=4, seed=3)
+    p = tmp_path / "t.index"
+    idx.save(str(p))
+    with open(p, "ab") as f:
+        f.write(b"\0" * 8)
+    rc, msg = _load_rc(str(p))
+    assert rc == 3 and "after the index body" in msg
+
+
+def test_batch_nprobe_rejects_bad_counts():
+    import ctypes
+    from erq import _lib
+    L = _lib.lib()
+    npq = np.array([4, 0], np.uint32)
+    # a null handle is rejected before anything else is read
+    assert L.erq_search_batch_nprobe(None, None, 2, 10, npq.ctypes.data_as(ctypes.c_void_p),
+                                     None, None, 0) == 1

@@ -64,14 +64,16 @@ def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None, pair_bu
     return out
 
 
-def _check(idx, q, k, nprobes, label, survivor_cap=None, pair_budget=None):
+def _check(idx, q, k, nprobes, label, survivor_cap=None, pair_budget=None, redo=0):
     qr = rotate_np(idx, q)
     pl = _probe_lists(idx, qr, nprobes)
     oid, od, ost = _oracle_rounds(idx, qr, k, pl)
+    print(f"[{label}] k={k} nq={qr.shape[0]} max_nprobe={int(max(nprobes))}", flush=True)
     gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)), survivor_cap, pair_budget)
     print(f"[{label}] stats={st.tolist()} oracle blocks/survivors/candidates={ost.tolist()}",
           flush=True)
-    assert st[5] == 0, "candidate buffer overflow"
+    # [5]: (query, round) rescans after a candidate overflow
+    assert st[5] == redo, f"rescans {st[5]}, expected {redo}"
     # the work counters match the schedule's: rank-0 blocks, survivors, round candidates
     assert (st[0], st[2], st[4]) == tuple(int(v) for v in ost)
     np.testing.assert_array_equal(gid, oid, err_msg=f"{label}: ids")
@@ -224,4 +226,69 @@ def test_autotuner_keeps_results():
     torch.cuda.synchronize()
     assert torch.equal(pi, pi2) and torch.equal(pd, pd2)
     assert torch.equal(bi, bi2) and torch.equal(bd, bd2)
+    g.close()
+
+
+def _skewed_index(small, large, n_small, n_large, dup_large=False, seed=31):
+    """Clusters at growing distance along one axis (cluster c centred at 3c e_0), the first
+    `small` of n_small vectors, the next `large` of n_large; with dup_large every vector of a
+    large cluster is the same vector.  Queries sit at the first centre, so they probe the
+    clusters in order."""
+    from erq import index
+    rng = np.random.default_rng(seed)
+    dim = 128
+    K = small + large
+    cen = np.zeros((K, dim), np.float32)
+    cen[:, 0] = 3.0 * np.arange(K)
+    xs, lab = [], []
+    for c in range(K):
+        n = n_small if c < small else n_large
+        if dup_large and c >= small:
+            one = cen[c] + 0.3 * rng.standard_normal(dim).astype(np.float32)
+            xs.append(np.repeat(one[None], n, axis=0))
+        else:
+            xs.append(cen[c] + 0.3 * rng.standard_normal((n, dim)).astype(np.float32))
+        lab += [c] * n
+    x = np.concatenate(xs)
+    idx = index.build_index(x, cen, np.asarray(lab), bits=4, seed=seed + 1)
+    q = cen[0] + 0.3 * rng.standard_normal((6, dim)).astype(np.float32)
+    return idx, q
+
+
+def test_batched_candidate_overflow_is_rescanned():
+    """k = 1024 and 4 small nearest clusters (200 vectors): the pool is not full after the
+    exact ranks, so every vector of the next 12 clusters (12 x 800 > 8192) is a candidate of
+    round [4,16); each query's round is rescanned exactly, and the later round runs normally."""
+    idx, q = _skewed_index(4, 16, 50, 800)
+    _check(idx, q, 1024, np.full(q.shape[0], 20), "candidate overflow", redo=q.shape[0])
+
+
+def test_batched_ties_beyond_merge_buffer_are_rescanned():
+    """5000 copies of one vector in the fifth cluster: its 5000 equal distances sit at the
+    k-th key (k = 1024), more than the merge keeps; the rescan orders them by id."""
+    idx, q = _skewed_index(4, 1, 50, 5000, dup_large=True)
+    _check(idx, q, 1024, np.full(q.shape[0], 5), "tied candidates", redo=q.shape[0])
+
+
+def test_calls_on_different_streams_are_ordered():
+    """A batched search enqueued on a side stream, then at once a host-pointer search on the
+    handle's own stream (which reuses the same workspace): the second call waits for the
+    first call's kernels, so the side-stream result equals the one computed alone."""
+    import torch
+    from erq.gpu import IVF
+    idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4, nq=48)
+    qr = rotate_np(idx, q)
+    qd = torch.from_numpy(np.tile(qr, (40, 1))).cuda()     # enough work to still be running
+    g = IVF.from_index(idx)
+    ref_ids, ref_d = g.search_batched_dev(qd, 100, 48)
+    torch.cuda.synchronize()
+    ref_h, _ = g.search_batch(qr[:8], 100, 48)
+    side = torch.cuda.Stream()
+    for _ in range(3):
+        with torch.cuda.stream(side):
+            ids, d = g.search_batched_dev(qd, 100, 48)
+        hids, _ = g.search_batch(qr[:8], 100, 48)           # no synchronisation in between
+        torch.cuda.synchronize()
+        assert torch.equal(ids, ref_ids) and torch.equal(d, ref_d)
+        np.testing.assert_array_equal(hids, ref_h)
     g.close()

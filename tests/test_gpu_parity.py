@@ -236,3 +236,33 @@ def test_query_order_does_not_change_results():
     assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
     assert torch.equal(out[0][2], out[1][2])
     g.close()
+
+
+def test_host_batch_with_per_query_nprobe():
+    """erq_search_batch_nprobe (host pointers, each query its own nprobe): the exact replay
+    equals IVF::search per query; the batched flag equals the oracle's round schedule."""
+    from erq.gpu import IVF
+    from oracle import oracle as orc
+    idx, q = _build(("d128", 4), 20000, 128, 64, 4)
+    qr = rotate_np(idx, q)
+    rng = np.random.default_rng(17)
+    npq = rng.integers(1, 40, qr.shape[0]).astype(np.uint32)
+    oi = orc.OracleIndex(idx)
+    g = IVF.from_index(idx)
+    print("[per-query nprobe] exact replay", flush=True)
+    ids, d = g.search_batch_nprobe(qr, 50, npq)
+    for i in range(qr.shape[0]):
+        oid, od = oi.search(qr[i], 50, int(npq[i]))
+        np.testing.assert_array_equal(ids[i], oid, err_msg=f"query {i}")
+        np.testing.assert_array_equal(d[i].view(np.uint32), od.view(np.uint32))
+    print("[per-query nprobe] batched rounds", flush=True)
+    bids, bd = g.search_batch_nprobe(qr, 50, npq, batched=True)
+    for i in range(qr.shape[0]):
+        pid, pd = oi.centroid_topn(qr[i], int(npq[i]))
+        oid, od = oi.search_rounds(qr[i], 50, pid, pd)
+        np.testing.assert_array_equal(bids[i], oid, err_msg=f"query {i}")
+        np.testing.assert_array_equal(bd[i].view(np.uint32), od.view(np.uint32))
+    # repeated calls reuse the handle's device buffers: same answers
+    ids2, _ = g.search_batch_nprobe(qr, 50, npq)
+    np.testing.assert_array_equal(ids2, ids)
+    g.close()

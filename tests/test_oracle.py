@@ -63,6 +63,50 @@ def test_normalize_query_unit_and_sum():
     assert np.all(u0 == v) and s0 == np.float32(D) * v
 
 
+def test_normalize_query_sum_order_known_answer():
+    # normalize_query16 (space.hpp:749-778) adds u into 16 lane accumulators (lane l takes
+    # dims l, l+16, l+32, ... in order), then _mm512_reduce_add_ps.  With q - c = u exactly
+    # (c = 0, norm = 1 so fac = 1): lane 0 holds 2^24, +1, +1, -2^24 (dims 0, 16, 32, 48):
+    # 2^24 + 1 rounds back to 2^24 twice, so the lane ends at 0; lane 8 holds 3 (dim 8);
+    # the tree gives 3.  A sequential sum over dims meets 3 while at 2^24 (2^24 + 3 -> 2^24 + 4),
+    # keeps 2^24 + 4 through both +1 (ties to even), ends at 4.
+    D = 64
+    u = np.zeros(D, np.float32)
+    u[0], u[16], u[32], u[48], u[8] = 2.0 ** 24, 1.0, 1.0, -(2.0 ** 24), 3.0
+    unit, sumq = orc.normalize_query(u, np.zeros(D, np.float32), 1.0)
+    np.testing.assert_array_equal(unit, u)
+    assert sumq == 3.0
+    seq = np.float32(0)
+    for v in u:
+        seq = np.float32(seq + v)
+    assert seq == 4.0                                   # the order the known answer rules out
+
+
+@pytest.mark.parametrize("ex", [2, 3, 4, 6, 7, 8])
+def test_ip_ex_lane_order_and_fma_known_answers(ex):
+    # IP*_fxu* (space.hpp:109-497): per 16-dim group an FMA into 16 lane accumulators, then
+    # _mm512_reduce_add_ps.  Codes 0..3 fit every EX.
+    D = 64
+    # (a) lane order: lane 0 gets 2^24*1, 1*1, 1*1, -2^24*1 -> 0 (each +1 rounds away);
+    #     lane 8 gets 1*3 -> 3; total 3.  Summed over dims in order: 2^24 + 3 -> 2^24 + 4,
+    #     +1, +1 stay (ties to even), -2^24 -> 4.
+    u = np.zeros(D, np.float32)
+    e = np.zeros(D, np.uint8)
+    u[0], u[16], u[32], u[48], u[8] = 2.0 ** 24, 1.0, 1.0, -(2.0 ** 24), 1.0
+    e[0] = e[16] = e[32] = e[48] = 1
+    e[8] = 3
+    assert orc.ip_ex(u, orc.ex_encode(e, ex), ex) == 3.0
+    # (b) fused multiply-add: lane 1 gets (-1)*3 then (1 + 2^-23)*3.  fma(1 + 2^-23, 3, -3)
+    #     is exactly 3 * 2^-23; a rounded product (3 + 2^-21) minus 3 would give 2^-21.
+    u = np.zeros(D, np.float32)
+    e = np.zeros(D, np.uint8)
+    u[1], u[17] = -1.0, np.float32(1.0 + 2.0 ** -23)
+    e[1] = e[17] = 3
+    got = orc.ip_ex(u, orc.ex_encode(e, ex), ex)
+    assert got == np.float32(3 * 2.0 ** -23)
+    assert got != np.float32(2.0 ** -21)
+
+
 def test_quantize_query_rounding_and_range():
     D = 512
     u = RNG.standard_normal(D).astype(np.float32)
