@@ -67,6 +67,28 @@ constexpr size_t kDigitBytes = size_t(3) << 33;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Scratch of erq_search_dev_noalloc: one array per device, held until the process ends.  In
+// host code `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress, resolved once per device when the first index is created there.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// With `device` current: the device address of its scratch (resolved on the first call).
+void resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void* ptr = nullptr;
+    g_scratch_err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? ptr : nullptr;
+    g_scratch_done[device] = true;
+}
+
 bool ex_ok(size_t ex) { return ex == 2 || ex == 3 || ex == 4 || ex == 6 || ex == 7 || ex == 8; }
 
 // Runs an entry point's body; a C++ exception becomes an error code (a failed host
@@ -246,6 +268,7 @@ int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint6
             delete h;
             return hip_fail(e, "hipGetDevice");
         }
+        resolve_scratch(h->device);
         e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_last, hipEventDisableTiming);
         if (e != hipSuccess) {
@@ -462,13 +485,11 @@ static size_t probe_ws_bytes(size_t nq, size_t K) {
     return align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
 }
 
-static int probe_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
-                        uint32_t* pid, float* pd, hipStream_t s) {
+// Probe selection with its scratch (probe_ws_bytes) at `base`.
+static int probe_into(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
+                      uint32_t* pid, float* pd, hipStream_t s, char* base) {
     const size_t K = h->K, D = h->D;
-    void* wsp;
-    int rc = workspace(h, probe_ws_bytes(nq, K), &wsp);
-    if (rc) return rc;
-    Ws ws{(char*)wsp};
+    Ws ws{base};
     float* approx = ws.take<float>(nq * K);
     float* qn = ws.take<float>(nq);
     hipError_t e = launch_row_sqnorm(q_rot_dev, (int)nq, (int)D, (int)D, qn, s);
@@ -480,6 +501,14 @@ static int probe_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t 
                             (int)D, (int)nprobe, pid, pd, s);
     if (e != hipSuccess) return hip_fail(e, "probe selection");
     return ERQ_OK;
+}
+
+static int probe_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
+                        uint32_t* pid, float* pd, hipStream_t s) {
+    void* wsp;
+    int rc = workspace(h, probe_ws_bytes(nq, h->K), &wsp);
+    if (rc) return rc;
+    return probe_into(h, q_rot_dev, nq, nprobe, pid, pd, s, (char*)wsp);
 }
 
 int erq_probe_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
@@ -547,20 +576,29 @@ static ScanArgs scan_args(const erq_index* h, const float* q_rot_dev, size_t nq,
     return a;
 }
 
+// The per-query scan.  order_buf (nullable, nq + K + 1 u32): scratch of the query order
+// given by the caller instead of the handle's; with it nothing is allocated, and the
+// bit-plane kernel runs only if its copies of the codes already exist (else the LUT kernel,
+// with identical results).
 static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                        const uint32_t* off, const uint32_t* pid, const float* pd, uint32_t* ids,
-                       float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0) {
+                       float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0,
+                       uint32_t* order_buf = nullptr) {
     ScanArgs a = scan_args(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, max_probes);
     if (h->plain_kernel != 1) {  // auto or MFMA: bit-plane integer path
-        int rc = ensure_bp(h, s);
-        if (rc) return rc;
-        a.bp = h->bp;
-        a.exl = h->exl;
+        if (!order_buf) {
+            int rc = ensure_bp(h, s);
+            if (rc) return rc;
+        }
+        if (h->bp && h->exl) {
+            a.bp = h->bp;
+            a.exl = h->exl;
+        }
     }
     if (h->plain_order && nq > 1) {
         // queries by scan cost, heaviest first: the result of each query is unchanged
         const size_t need = nq + h->K + 1;
-        if (need > h->order_cap) {
+        if (!order_buf && need > h->order_cap) {
             if (h->order) {
                 hipError_t e = hipDeviceSynchronize();  // earlier calls may still read it
                 if (e != hipSuccess) return hip_fail(e, "order sync");
@@ -580,10 +618,11 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
             bound = std::min(bound, (size_t)max_probes * ((max_cluster_size(h) + 31) / 32));
         int shift = 0;
         while ((bound >> shift) >= h->K) shift++;
+        uint32_t* ob = order_buf ? order_buf : h->order;
         hipError_t e = launch_query_order(off, pid, h->csize, (int)nq, (int)h->K, max_probes,
-                                          shift, h->order, h->order + h->K + 1, s);
+                                          shift, ob, ob + h->K + 1, s);
         if (e != hipSuccess) return hip_fail(e, "query order");
-        a.qorder = h->order + h->K + 1;
+        a.qorder = ob + h->K + 1;
     }
     hipError_t e = launch_scan(a, s);
     if (e != hipSuccess) return hip_fail(e, "scan");
@@ -714,6 +753,72 @@ int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, si
         return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s);
     });
 }
+
+// erq_search_dev with every intermediate in the device's scratch array: no allocation.
+int erq_search_dev_noalloc(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                           size_t nprobe, uint32_t* ids_dev, float* dists_dev,
+                           uint64_t* stats_dev, void* stream) {
+    return guarded([&]() -> int {
+        if (!h || (nq && (!q_rot_dev || !ids_dev))) {
+            set_error("erq_search_dev_noalloc: null argument");
+            return ERQ_EINVAL;
+        }
+        if (k == 0 || k > 1024) {
+            set_error("erq_search_dev_noalloc: k must be in [1, 1024]");
+            return ERQ_EINVAL;
+        }
+        if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
+            set_error("erq_search_dev_noalloc: nprobe must be in [1, min(K, 2048)]");
+            return ERQ_EINVAL;
+        }
+        if (nq == 0) return ERQ_OK;
+        const size_t need = erq_search_noalloc_bytes(h, nq, nprobe);
+        if (need > kScratchBytes) {
+            set_error("erq_search_dev_noalloc: " + std::to_string(nq) + " queries need " +
+                      std::to_string(need) + " bytes of scratch, more than the " +
+                      std::to_string(kScratchBytes) + " it has; search fewer at a time");
+            return ERQ_EUNSUPPORTED;
+        }
+        if (h->device < 0 || h->device >= kMaxDevices || !g_scratch_ptr[h->device]) {
+            set_error(std::string("erq_search_dev_noalloc: no scratch on this device: ") +
+                      (h->device >= 0 && h->device < kMaxDevices
+                           ? hipGetErrorString(g_scratch_err[h->device])
+                           : "device index out of range"));
+            return ERQ_EHIP;
+        }
+        hipStream_t s = pick(h, stream);
+        Call call(h, s);
+        // layout: probe selection's approximate distances and norms, the probe lists and
+        // offsets, the query-order bins
+        char* base = static_cast<char*>(g_scratch_ptr[h->device]);
+        const size_t probe_part = probe_ws_bytes(nq, h->K);
+        Ws ws{base, probe_part};
+        uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
+        float* pd = ws.take<float>(nq * nprobe);
+        uint32_t* off = ws.take<uint32_t>(nq + 1);
+        uint32_t* order = ws.take<uint32_t>(nq + h->K + 1);
+        int rc = probe_into(h, q_rot_dev, nq, nprobe, pid, pd, s, base);
+        if (rc) return rc;
+        hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off,
+                           nq, (uint32_t)nprobe);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "offsets");
+        return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s, 0,
+                           order);
+    });
+}
+
+size_t erq_search_noalloc_bytes(const erq_index* h, size_t nq, size_t nprobe) {
+    if (!h) return 0;
+    Ws ws{nullptr, probe_ws_bytes(nq, h->K)};
+    ws.take<uint32_t>(nq * nprobe);
+    ws.take<float>(nq * nprobe);
+    ws.take<uint32_t>(nq + 1);
+    ws.take<uint32_t>(nq + h->K + 1);
+    return ws.off;
+}
+
+size_t erq_search_noalloc_capacity(void) { return kScratchBytes; }
 
 // The bit-plane copy of the 1-bit codes and the lane-major copy of the EX-bit codes,
 // built on the stream on first use of a kernel that reads them.

@@ -41,6 +41,10 @@ SIGNATURES = {
     "erq_scan_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                              c_vp]),
     "erq_search_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp, c_vp, c_vp]),
+    "erq_search_dev_noalloc": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp, c_vp,
+                                       c_vp]),
+    "erq_search_noalloc_bytes": (c_size, [c_vp, c_size, c_size]),
+    "erq_search_noalloc_capacity": (c_size, []),
     "erq_scan_batched_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_vp, c_vp, c_vp, c_size, c_vp,
                                      c_vp, c_vp, c_vp]),
     "erq_search_batched_dev": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp, c_vp,

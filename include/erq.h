@@ -151,6 +151,23 @@ int erq_scan_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
 int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
                    uint32_t* ids_dev, float* dists_dev, uint64_t* stats_dev, void* stream);
 
+/* erq_search_dev without any device allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Every intermediate (approximate centroid distances, probe lists,
+ * query-order bins) lives in a fixed per-device scratch array of
+ * erq_search_noalloc_capacity() bytes (64 MiB), held from the library's first index on that
+ * device until the process ends; a call whose need (erq_search_noalloc_bytes: about
+ * nq * (4 K + 8 nprobe) bytes) exceeds it fails with ERQ_EUNSUPPORTED.  All calls on one
+ * device share that array: two such calls that overlap in time on one GPU (different
+ * handles, or streams not ordered by the caller) race.  The int8-MFMA scan runs if the
+ * handle already holds its copies of the codes (built by an earlier allocating search),
+ * else the LUT scan; results are identical to erq_search_dev's either way. */
+int erq_search_dev_noalloc(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
+                           size_t nprobe, uint32_t* ids_dev, float* dists_dev,
+                           uint64_t* stats_dev, void* stream);
+size_t erq_search_noalloc_bytes(const erq_index* h, size_t nq, size_t nprobe);
+size_t erq_search_noalloc_capacity(void);
+
 /* Cluster-major batched form of erq_scan_dev for many queries with uneven probe lists:
  * ranks [0,4) of every query are scanned exactly as IVF::search does; ranks [4,16),
  * [16,64), [64,192) and then 128 ranks at a time are processed in rounds, each round
