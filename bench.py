@@ -1,18 +1,23 @@
-"""Benchmark: QPS at recall@100 = 0.95 on the embedding-shaped workload
-(N = 1M, D = 1536 unit-norm power-law mixture, 4096 IVF clusters, B = 4, K = 100,
-20,000 queries per batch), plus the scan kernel's HBM roofline and a CPU baseline.
+"""Benchmark: QPS at recall@100 = 0.95 on the embedding-shaped workload (N = 1M, D = 1536
+unit-norm power-law mixture, 4096 IVF clusters, B = 4, K = 100, 20,000 queries per batch):
+the learned-probe batched search (the headline), against fixed-nprobe search, plus the
+dominant kernel's roofline and a CPU baseline.
 
-A "step" is one pass of the search over the whole query batch, inputs resident in HBM:
-rotation (MFMA GEMM) -> centroid contraction + exact top-nprobe -> per-query scan.
-Protocol (reference src/test_search.cpp:83-150): sweep nprobe, recall against exact
-ground truth (GPU FP32, FP64 spot check), QPS at recall 0.95 by linear interpolation
-between the two bracketing nprobes; each bracketing nprobe is timed over --steps steps.
+A "step" is one search of the whole 20,000-query batch with the inputs resident in HBM:
+rotation (MFMA GEMM) -> probe lists (MLP + learned selection, or centroid contraction +
+exact top-nprobe) -> scan.  The headline `value` is the learned search at the smallest
+probability mass whose recall@100 reaches 0.95 (found by a sweep and a bisection of the
+mass, untimed), timed over exactly --steps steps after --warmup steps; `ms_per_step` is that
+same measured time per step.  Fixed-nprobe paths follow the reference's protocol
+(src/test_search.cpp:83-150): nprobe sweep, QPS at the target by linear interpolation
+between the bracketing nprobes.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
 import os
+import platform
 import re
 import sys
 import time
@@ -24,17 +29,18 @@ sys.path.insert(0, os.path.join(REPO, "extended-rabitq_amd"))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+TARGETS = (0.90, 0.95, 0.99)
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def parse():
+def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=1536)
     ap.add_argument("--clusters", type=int, default=4096)
@@ -44,7 +50,8 @@ def parse():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--kmeans-iters", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--nprobes", type=str, default="8,16,32,64,96,128,160,192,224,256,320,384,512")
+    ap.add_argument("--nprobes", type=str,
+                    default="8,16,32,48,64,96,128,160,192,224,256,320,384,512,768,1024")
     ap.add_argument("--sigma", type=float, default=1.15,
                     help="within-component spread relative to the component spread")
     ap.add_argument("--train-queries", type=int, default=200000)
@@ -52,21 +59,22 @@ def parse():
                     help="nprobe of the label search (0: 256) and the learned lists' maximum")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device copy first (HBM counter calibration)")
     ap.add_argument("--profile-plain", type=int, default=0, metavar="NPROBE",
                     help="only time the plain path (rotate, probe, scan) at this nprobe and "
                          "print its roofline: the command the kernel profiles are taken with")
     ap.add_argument("--profile-learned", type=float, default=0.0, metavar="MASS",
-                    help="train the probe predictor, then only time the learned batched search "
-                         "at this mass (the command its kernel profile is taken with)")
+                    help="train (or --load-mlp) the probe predictor, then only time the learned "
+                         "batched search at this mass and print its stages")
     ap.add_argument("--save-mlp", type=str, default="",
                     help="save the trained probe predictor to this .npz file")
     ap.add_argument("--load-mlp", type=str, default="",
                     help="load the probe predictor from this .npz file instead of training it")
     ap.add_argument("--sweep-only", action="store_true",
-                    help="print the recall sweep of both paths and exit (no timing)")
-    return ap.parse_args()
+                    help="print the recall sweep of both fixed paths and exit (no timing)")
+    return ap.parse_args(argv)
 
 
 def build_workload(a, torch):
@@ -96,10 +104,399 @@ def build_workload(a, torch):
     return x, q, idx, (basis, centres)
 
 
-def main():
-    a = parse()
+def host_info():
+    """CPU model, logical CPUs, physical cores and SMT state of this host (Linux /proc, /sys)."""
+    model, logical, cores = platform.processor() or "unknown", os.cpu_count() or 0, set()
+    try:
+        phys = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                cores.add((phys, v))
+    except OSError:
+        pass
+    smt = None
+    try:
+        smt = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": logical, "physical_cores": len(cores) or None,
+            "smt": smt, "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def interpolate(sweep, target):
+    """(lo, hi) bracketing points of a (knob, recall, ...) sweep and the fraction of the way
+    from lo to hi at which the recall equals target (the reference's linear interpolation)."""
+    above = [s for s in sweep if s[1] >= target]
+    below = [s for s in sweep if s[1] < target]
+    if not above:
+        return None
+    hi = min(above, key=lambda s: s[0])
+    lo = max([s for s in below if s[0] < hi[0]], key=lambda s: s[0], default=None)
+    f = 1.0 if lo is None else (target - lo[1]) / (hi[1] - lo[1])
+    return lo, hi, f
+
+
+class Runner:
+    """The workload on the GPU: the index handle, queries, ground truth and timing helpers."""
+
+    def __init__(self, a, g, idx, q, gt, torch):
+        self.a, self.g, self.idx, self.q, self.gt, self.torch = a, g, idx, q, gt, torch
+        self.stream = torch.cuda.current_stream()
+
+    def recall(self, ids):
+        from erq import data
+        return data.recall_at_k(ids.cpu().numpy().view(np.uint32), self.gt, self.a.k)
+
+    def timed(self, fn, steps=None, warmup=None):
+        """W untimed and K timed calls of fn, synchronized on both sides of the timed
+        region; returns ms per call (host wall clock around the K calls)."""
+        torch = self.torch
+        steps = self.a.steps if steps is None else steps
+        warmup = self.a.warmup if warmup is None else warmup
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    def staged(self, fn, steps):
+        """Per-stage ms per call from the library's HIP events (erq_timing_begin/end)."""
+        self.torch.cuda.synchronize()
+        self.g.timing_begin(4096)
+        for _ in range(steps):
+            fn()
+        self.torch.cuda.synchronize()
+        st = self.g.timing_end()
+        return {k: round(v[0] / steps, 4) for k, v in st.items() if v[1]}
+
+
+def fixed_path(r, path):
+    """The reference protocol on one fixed-nprobe path: nprobe sweep (recall against exact
+    ground truth), each bracketing nprobe timed; QPS interpolated at each recall target."""
+    a, g, q = r.a, r.g, r.q
+    torch = r.torch
+
+    def step(npb, stats=None):
+        qr = g.rotate_dev(q)
+        if path == "plain":
+            return g.search_dev(qr, a.k, npb, stats=stats)
+        return g.search_batched_dev(qr, a.k, npb, stats=stats)
+
+    sweep = []
+    for npb in [int(s) for s in a.nprobes.split(",")]:
+        if npb > min(r.idx.K, 2048):
+            break
+        ids, _ = step(npb)
+        torch.cuda.synchronize()
+        rec = r.recall(ids)
+        sweep.append([npb, rec, None])
+        log(f"[{path}] nprobe={npb:4d} recall@{a.k}={rec:.4f}")
+        if rec >= max(TARGETS) or rec >= 0.9995:
+            break
+    out = {"sweep": [[n, round(x, 4)] for n, x, _ in sweep], "targets": {}}
+    timed = {}
+
+    def ms_at(npb):
+        if npb not in timed:
+            timed[npb] = r.timed(lambda: step(npb))
+            log(f"[{path}] nprobe={npb} timed: {timed[npb]:.3f} ms per {a.nq} queries")
+        return timed[npb]
+
+    for tgt in TARGETS:
+        br = interpolate(sweep, tgt)
+        if br is None:
+            out["targets"][str(tgt)] = None
+            continue
+        lo, hi, f = br
+        qhi = a.nq / (ms_at(hi[0]) / 1e3)
+        qlo = a.nq / (ms_at(lo[0]) / 1e3) if lo else qhi
+        out["targets"][str(tgt)] = {"qps": round(qlo + f * (qhi - qlo), 1),
+                                    "nprobe_bracket": [lo[0] if lo else None, hi[0]],
+                                    "recall_bracket": [round(lo[1], 4) if lo else None,
+                                                       round(hi[1], 4)]}
+    out["timed_ms"] = {str(k): round(v, 3) for k, v in timed.items()}
+    return out
+
+
+def train_probe_model(a, g, idx, mix, torch):
+    """Labels from the fixed-nprobe batched search over separate training queries, then the
+    probe MLP trained on them.  Returns (model, label_nprobe, losses, label_s, train_s)."""
+    from erq import data
+    from erq.probe_model import ProbeMLP, cluster_of_ids
+    basis, centres = mix
+    if a.load_mlp:
+        m = ProbeMLP.load(a.load_mlp)
+        log(f"probe MLP loaded from {a.load_mlp}")
+        return m, min(a.label_nprobe or 256, idx.K, 2048), [], 0.0, 0.0
+    t0 = time.time()
+    qt, _ = data.mixture_powerlaw_torch(a.train_queries, a.dim, a.clusters, a.sigma, seed=5,
+                                        alpha=1.0, centres=centres, basis=basis)
+    qt_r = g.rotate_dev(qt)
+    del qt
+    c_of = torch.from_numpy(cluster_of_ids(idx)).cuda()
+    label_np = min(a.label_nprobe or 256, idx.K, 2048)
+    lab = torch.empty((a.train_queries, a.k), dtype=torch.int32, device="cuda")
+    for s0 in range(0, a.train_queries, 20000):
+        ids, _ = g.search_batched_dev(qt_r[s0:s0 + 20000], a.k, label_np)
+        lab[s0:s0 + 20000] = c_of[ids.long()]
+    torch.cuda.synchronize()
+    t_lab = time.time() - t0
+    log(f"labels from fixed nprobe={label_np} over {a.train_queries} queries in {t_lab:.1f}s")
+    t0 = time.time()
+    # inputs scaled to unit variance per coordinate (rotated unit-norm queries)
+    m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=1024, seed=7,
+                 input_scale=float(np.sqrt(idx.D)))
+    losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=2e-3, pos_weight=4.0, log=log,
+                     schedule="cosine")
+    torch.cuda.synchronize()
+    if a.save_mlp:
+        m.save(a.save_mlp)
+    return m, label_np, losses, t_lab, time.time() - t0
+
+
+def learned_path(r, m, max_np, fixed_np):
+    """The learned search: recall against the probability mass, and for each target the
+    smallest mass reaching it (sweep, then bisection), timed there."""
+    a, g, q = r.a, r.g, r.q
+    torch = r.torch
+
+    def step(mass, stats=None):
+        return g.search_learned_dev(m, g.rotate_dev(q), a.k, mass, 4, max_np, fixed_np, 0.5,
+                                    stats=stats)
+
+    cache = {}
+
+    def rec_at(mass):
+        mass = round(mass, 6)
+        if mass not in cache:
+            ids, _ = step(mass)
+            torch.cuda.synchronize()
+            cache[mass] = r.recall(ids)
+            log(f"[learned] mass={mass} recall@{a.k}={cache[mass]:.4f}")
+        return cache[mass]
+
+    grid = (0.5, 0.7, 0.8, 0.85, 0.9, 0.93, 0.95, 0.96, 0.97, 0.975, 0.98, 0.985, 0.99, 0.993,
+            0.995, 0.997, 0.998, 0.999, 1.0)
+    for mass in grid:
+        if rec_at(mass) >= max(TARGETS):
+            break
+    out = {"sweep": [[k, round(v, 4)] for k, v in sorted(cache.items())], "targets": {}}
+    for tgt in TARGETS:
+        ok = [mm for mm, rr in cache.items() if rr >= tgt]
+        if not ok:
+            out["targets"][str(tgt)] = None
+            continue
+        hi = min(ok)
+        lo = max([mm for mm, rr in cache.items() if rr < tgt and mm < hi], default=None)
+        for _ in range(4 if lo is not None else 0):   # bisection: the smallest mass reaching it
+            mid = round(0.5 * (lo + hi), 6)
+            if rec_at(mid) >= tgt:
+                hi = mid
+            else:
+                lo = mid
+        ms = r.timed(lambda: step(hi))
+        off, _, _, nfb = g.probe_learned_dev(m, g.rotate_dev(q), hi, 4, max_np, fixed_np, 0.5)
+        mean_np = float(np.diff(off.cpu().numpy()).mean())
+        out["targets"][str(tgt)] = {"qps": round(a.nq / ms * 1e3, 1), "ms": round(ms, 3),
+                                    "mass": hi, "recall": round(cache[round(hi, 6)], 4),
+                                    "mean_nprobe": round(mean_np, 2),
+                                    "fallback_queries": int(nfb[0])}
+        log(f"[learned] target {tgt}: mass {hi} recall {cache[round(hi, 6)]:.4f} "
+            f"{ms:.3f} ms ({a.nq / ms * 1e3:.0f} q/s), mean nprobe {mean_np:.1f}")
+    out["step"] = step
+    return out
+
+
+def stage_bytes(r, m, mass, max_np, fixed_np):
+    """Algorithmic bytes of one learned search, per stage, from its probe lists: the exact
+    ranks' replay reads each probed block (4D + 128 B) and each survivor's ex-code and two
+    factors (D EX/8 + 8 B); the batched scan reads each work item's blocks once for its up to
+    16 queries (ceil(queries / 16) x blocks x (4D + 128) B per (round, cluster)) plus each
+    (query, cluster) pair's digits and parameters (2 Dp + 48 B) and writes a 16-byte record
+    per survivor; the re-estimate reads each survivor's ex-code and two factors."""
+    a, g, q, idx, torch = r.a, r.g, r.q, r.idx, r.torch
+    D, EX, K = idx.D, idx.EX, idx.K
+    Dp = ((D + 511) // 512) * 512
+    qr = g.rotate_dev(q)
+    off, pid, pd, _ = g.probe_learned_dev(m, qr, mass, 4, max_np, fixed_np, 0.5)
+    st = torch.zeros(8, dtype=torch.int64, device="cuda")
+    g.search_learned_dev(m, qr, a.k, mass, 4, max_np, fixed_np, 0.5, stats=st)
+    # the exact ranks alone: the first 4 probes of every list
+    offs = off.long()
+    lens = offs[1:] - offs[:-1]
+    keep = torch.clamp(lens, max=4)
+    off4 = torch.zeros_like(offs)
+    off4[1:] = torch.cumsum(keep, 0)
+    rows = torch.repeat_interleave(torch.arange(a.nq, device="cuda"), keep)
+    pos = torch.arange(int(off4[-1]), device="cuda") - off4[:-1][rows] + offs[:-1][rows]
+    st4 = torch.zeros(8, dtype=torch.int64, device="cuda")
+    g.scan_dev(qr, a.k, off4.int(), pid[pos], pd[pos], stats=st4)
+    torch.cuda.synchronize()
+    st, st4 = st.cpu().numpy(), st4.cpu().numpy()
+    # (round, cluster) entry counts of the batched rounds: ranks [4,16) [16,64) [64,192) then
+    # 128 wide
+    offs_h = offs.cpu().numpy()
+    pid_h = pid.cpu().numpy().view(np.uint32)
+    rank = np.arange(pid_h.size) - np.repeat(offs_h[:-1], np.diff(offs_h))
+    sel = rank >= 4
+    rk = rank[sel]
+    rnd = np.where(rk < 16, 0, np.where(rk < 64, 1, np.where(rk < 192, 2, 3 + (rk - 192) // 128)))
+    cnt = np.bincount(rnd * K + pid_h[sel], minlength=(int(rnd.max(initial=0)) + 1) * K)
+    nblk = np.tile((idx.cluster_sizes.astype(np.int64) + 31) // 32, cnt.size // K)
+    item_blocks = int((((cnt + 15) // 16) * nblk).sum())
+    entries = int(cnt.sum())
+    surv_rounds = int(st[2] - st4[2])
+    per_surv = D * EX // 8 + 8
+    return {
+        "replay": int(st4[0] * (4 * D + 128) + st4[2] * per_surv),
+        "scan": int(item_blocks * (4 * D + 128) + entries * (2 * Dp + 48) + surv_rounds * 16),
+        "reest": int(surv_rounds * per_surv),
+        "counts": {"replay_blocks": int(st4[0]), "replay_survivors": int(st4[2]),
+                   "round_entries": entries, "round_item_blocks": item_blocks,
+                   "round_survivors": surv_rounds, "round_candidates": int(st[4]),
+                   "round_rescans": int(st[5])},
+    }
+
+
+KERNEL_OF_STAGE = {"replay": "scan_mfma_kernel", "scan": "batch_scan_kernel",
+                   "reest": "reest_kernel", "prep": "prep_kernel"}
+
+
+def roofline(stage, nbytes, ms, traffic=None, source=None):
+    achieved = nbytes / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": KERNEL_OF_STAGE.get(stage, stage), "ms_per_launch": round(ms, 4),
+            "algorithmic_bytes": int(nbytes), "traffic_source": source}
+
+
+def profiled_traffic(a, name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated) of this exact workload, or (None, None)."""
+    if a.nq != 20000 or a.n % 1_000_000:
+        return None, None
+    path = os.path.join(REPO, "profiles", name)
+    try:
+        mm = re.search(r"traffic_bytes_per_launch: (\d+)", open(path).read())
+    except OSError:
+        return None, None
+    return (int(mm.group(1)), "profiles/" + name) if mm else (None, None)
+
+
+def plain_roofline(r, nprobe):
+    """The per-query scan's roofline at one nprobe: its algorithmic bytes (every probed
+    block, every survivor's ex-code and factors) over its event-timed duration."""
+    a, g, q, idx, torch = r.a, r.g, r.q, r.idx, r.torch
+    qr = g.rotate_dev(q)
+    pid, pd = g.probe_dev(qr, nprobe)
+    off = torch.arange(0, (a.nq + 1) * nprobe, nprobe, dtype=torch.int32, device="cuda")
+    st = torch.zeros(8, dtype=torch.int64, device="cuda")
+    stages = r.staged(lambda: g.scan_dev(qr, a.k, off, pid, pd, stats=st), a.steps)
+    s = (st.cpu().numpy() // a.steps).tolist()
+    nbytes = s[0] * (4 * idx.D + 128) + s[2] * (idx.D * idx.EX // 8 + 8)
+    name = f"scan_mfma_kernel_plain_d{a.dim}_n{a.n // 1_000_000}m_b{a.bits}_nprobe{nprobe}.md"
+    traffic, src = profiled_traffic(a, name)
+    out = roofline("replay", nbytes, stages["replay"], traffic, src)
+    out["nprobe"] = nprobe
+    out["stats"] = s
+    return out
+
+
+def cpu_baseline(idx, q, a, nprobe):
+    """The CPU oracle (a scalar C port of the reference search), single thread, timed on
+    a bounded sample of the same queries at the same nprobe: QPS on this host."""
+    try:
+        from oracle import oracle as orc
+    except Exception as e:  # pragma: no cover
+        return {"error": str(e)}
+    oi = orc.OracleIndex(idx)
+    qr = q[:512].cpu().numpy().astype(np.float64)
+    qp = np.zeros((qr.shape[0], idx.D))
+    qp[:, :qr.shape[1]] = qr
+    qr = (qp @ idx.P.astype(np.float64)).astype(np.float32)
+    n = 0
+    t0 = time.perf_counter()
+    while n < qr.shape[0] and time.perf_counter() - t0 < a.cpu_seconds:
+        oi.search(qr[n], a.k, nprobe)
+        n += 1
+    dt = time.perf_counter() - t0
+    log(f"cpu baseline: {n} queries in {dt:.1f}s (1 thread)")
+    out = {"value": round(n / dt, 2), "unit": "queries/s", "cores": 1, "kind": "port",
+           "sample": f"{n} queries of the same batch, nprobe={nprobe}, k={a.k}, single thread, "
+                     f"scalar C oracle"}
+    threads = min(len(os.sched_getaffinity(0)),
+                  int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 10**6)
+    mm = int(min(qr.shape[0], max(threads, n / dt * threads * 2.0)))
+    t0 = time.perf_counter()
+    oi.search_batch(qr[:mm], a.k, nprobe, threads=threads)
+    dt = time.perf_counter() - t0
+    log(f"cpu baseline: {mm} queries in {dt:.1f}s ({threads} threads)")
+    out["all_cores"] = {"value": round(mm / dt, 2), "cores": threads,
+                        "sample": f"{mm} queries, OpenMP over queries"}
+    out["host"] = host_info()
+    return out
+
+
+def drop_in_latency(r, nprobe, nq=256):
+    """The IVF::search drop-in (erq_search, host pointers, one query per call, as
+    test_search.cpp's loop calls IVF::search): median and mean latency per query."""
+    qr = r.g.rotate_dev(r.q[:nq]).cpu().numpy()
+    for i in range(8):
+        r.g.search(qr[i], r.a.k, nprobe)
+    lat = []
+    for i in range(nq):
+        t0 = time.perf_counter()
+        r.g.search(qr[i], r.a.k, nprobe)
+        lat.append((time.perf_counter() - t0) * 1e6)
+    lat = np.asarray(lat)
+    return {"nprobe": nprobe, "queries": nq, "median_us": round(float(np.median(lat)), 1),
+            "mean_us": round(float(lat.mean()), 1), "p99_us": round(float(np.percentile(lat, 99)), 1)}
+
+
+def profile_plain(a, r):
+    out = plain_roofline(r, a.profile_plain)
+    qr = r.g.rotate_dev(r.q)
+    stages = r.staged(lambda: r.g.search_dev(r.g.rotate_dev(r.q), a.k, a.profile_plain), a.steps)
+    flop_rot = 2.0 * a.nq * r.idx.D * r.idx.D
+    del qr
+    print(json.dumps({"profile": "plain", "nprobe": a.profile_plain, "nq": a.nq,
+                      "steps": a.steps, "stages_ms": stages, "roofline": out,
+                      "rotation_flop": flop_rot,
+                      "centroid_gemm_flop": 2.0 * a.nq * r.idx.K * r.idx.D}), flush=True)
+
+
+def profile_learned(a, r, mix):
+    torch = r.torch
+    m, label_np, _, _, _ = train_probe_model(a, r.g, r.idx, mix, torch)
+
+    def step():
+        return r.g.search_learned_dev(m, r.g.rotate_dev(r.q), a.k, a.profile_learned, 4,
+                                      label_np, 192, 0.5)
+
+    ms = r.timed(step)
+    stages = r.staged(step, a.steps)
+    nb = stage_bytes(r, m, a.profile_learned, label_np, 192)
+    print(json.dumps({"profile": "learned", "mass": a.profile_learned, "nq": a.nq,
+                      "steps": a.steps, "ms_per_search": round(ms, 3),
+                      "qps": round(a.nq / ms * 1e3, 1), "stages_ms": stages,
+                      "stage_bytes": nb}), flush=True)
+    m.close()
+
+
+def main(argv=None):
+    a = parse(argv)
     if a.gpus != 1:
-        print(json.dumps({"error": "the search runs on one GPU (no multi-GPU path); --gpus must be 1"}))
+        print(json.dumps({"error": "the search runs on one GPU (no multi-GPU path); "
+                                   "--gpus must be 1"}))
         sys.exit(2)
     import torch
 
@@ -115,225 +512,123 @@ def main():
         del src, dst
     x, q, idx, mix = build_workload(a, torch)
     g = IVF.from_index(idx)
-    D, EX = idx.D, idx.EX
-    if a.profile_plain:
+    if a.profile_plain or a.profile_learned:
         del x
-        profile_plain(a, g, q, idx, torch)
-        g.close()
-        return
-    if a.profile_learned:
-        del x
-        profile_learned(a, g, q, idx, mix, torch)
+        r = Runner(a, g, idx, q, None, torch)
+        if a.profile_plain:
+            profile_plain(a, r)
+        else:
+            profile_learned(a, r, mix)
         g.close()
         return
 
     t0 = time.time()
     gt, _ = data.exact_knn_torch(x, q, a.k)
     gt = gt.cpu().numpy()
-    # FP64 spot check of the FP32 ground truth on 4 queries
-    xs = x.cpu().numpy()
-    qs = q[:4].cpu().numpy()
-    g64, _ = data.exact_knn_np(xs, qs, a.k)
+    xs = x.cpu().numpy()   # FP64 spot check of the FP32 ground truth on 4 queries
+    g64, _ = data.exact_knn_np(xs, q[:4].cpu().numpy(), a.k)
     spot = data.recall_at_k(gt[:4], g64, a.k)
     del xs
     log(f"ground truth (GPU FP32) in {time.time() - t0:.1f}s, FP64 spot-check agreement {spot:.4f}")
     del x
     torch.cuda.empty_cache()
-
-    stream = torch.cuda.current_stream()
-    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
-
-    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
-
-    def step(nprobe, path, timed=False):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-        ev[0].record(stream)
-        qr = g.rotate_dev(q)
-        ev[1].record(stream)
-        pid, pd = g.probe_dev(qr, nprobe)
-        off = torch.arange(0, (a.nq + 1) * nprobe, nprobe, dtype=torch.int32, device="cuda")
-        ev[2].record(stream)
-        st = stats if timed else None
-        if path == "plain":
-            ids, dist = g.scan_dev(qr, a.k, off, pid, pd, stats=st)
-        else:
-            ids, dist = g.scan_batched_dev(qr, a.k, off, pid, pd, nprobe, stats=st)
-        ev[3].record(stream)
-        return ids, ev
-
-    def measure(path):
-        sweep = []
-        for npb in [int(s) for s in a.nprobes.split(",")]:
-            if npb > idx.K:
-                break
-            ids, _ = step(npb, path)
-            torch.cuda.synchronize()
-            rec = data.recall_at_k(ids.cpu().numpy().view(np.uint32), gt, a.k)
-            sweep.append((npb, rec))
-            log(f"[{path}] nprobe={npb:4d} recall@{a.k}={rec:.4f}")
-            if rec >= a.target:
-                break
-        above = [s for s in sweep if s[1] >= a.target]
-        below = [s for s in sweep if s[1] < a.target]
-        if not above:
-            raise SystemExit(f"recall {a.target} not reached by nprobe {sweep[-1][0]}")
-        hi = above[0]
-        lo = below[-1] if below else None
-
-        def timed(npb):
-            for _ in range(a.warmup):
-                step(npb, path)
-            torch.cuda.synchronize()
-            stats.zero_()
-            t_rot = t_probe = t_scan = 0.0
-            evs = []
-            t0 = time.perf_counter()
-            for _ in range(a.steps):
-                _, ev = step(npb, path, timed=True)
-                evs.append(ev)
-            torch.cuda.synchronize()
-            wall = time.perf_counter() - t0
-            for ev in evs:
-                t_rot += ev[0].elapsed_time(ev[1])
-                t_probe += ev[1].elapsed_time(ev[2])
-                t_scan += ev[2].elapsed_time(ev[3])
-            st = (stats.cpu().numpy() // a.steps).tolist()
-            return dict(nprobe=npb, wall_ms=wall * 1e3 / a.steps, rot_ms=t_rot / a.steps,
-                        probe_ms=t_probe / a.steps, scan_ms=t_scan / a.steps, stats=st)
-
-        th = timed(hi[0])
-        th.update(scan_volume(g, q, idx, hi[0], th, torch))
-        log(f"[{path}] timed nprobe={hi[0]}: {th}")
-        qps_hi = a.nq / (th["wall_ms"] / 1e3)
-        tl = None
-        if lo is not None:
-            tl = timed(lo[0])
-            log(f"[{path}] timed nprobe={lo[0]}: {tl}")
-            qps_lo = a.nq / (tl["wall_ms"] / 1e3)
-            f = (a.target - lo[1]) / (hi[1] - lo[1])
-            qps = qps_lo + f * (qps_hi - qps_lo)
-        else:
-            qps = qps_hi
-        return dict(qps=qps, hi=hi, lo=lo, th=th, tl=tl, sweep=sweep)
+    r = Runner(a, g, idx, q, gt, torch)
 
     if a.sweep_only:
         for path in ("plain", "batched"):
-            for npb in [int(s) for s in a.nprobes.split(",")]:
-                if npb > idx.K:
-                    break
-                ids, _ = step(npb, path)
-                torch.cuda.synchronize()
-                rec = data.recall_at_k(ids.cpu().numpy().view(np.uint32), gt, a.k)
-                log(f"[{path}] sigma={a.sigma} nprobe={npb:4d} recall@{a.k}={rec:.4f}")
+            fixed_path(r, path)
         return
 
-    plain = measure("plain")
-    try:
-        batched = measure("batched")
-    except Exception as e:  # reported, never hidden
-        log(f"batched path failed: {e!r}")
-        batched = None
+    plain = fixed_path(r, "plain")
+    batched = fixed_path(r, "batched")
+    fixed_best = {}
+    for tgt in TARGETS:
+        cand = [p["targets"].get(str(tgt)) for p in (plain, batched)]
+        cand = [c for c in cand if c]
+        fixed_best[str(tgt)] = max((c["qps"] for c in cand), default=None)
+
     learned = None
+    headline = None
     if not a.no_learned:
-        try:
-            learned = measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch,
-                                      fixed_np=(batched or plain)["hi"][0])
-        except Exception as e:  # reported, never hidden
-            log(f"learned path failed: {e!r}")
+        m, label_np, losses, t_lab, t_train = train_probe_model(a, g, idx, mix, torch)
+        fixed_np = (batched["targets"].get(str(a.target)) or {}).get("nprobe_bracket",
+                                                                     [None, 192])[1]
+        fixed_np = min(fixed_np or 192, label_np)
+        learned = learned_path(r, m, label_np, fixed_np)
+        step = learned.pop("step")
+        head = learned["targets"].get(str(a.target))
+        if head:
+            stages = r.staged(lambda: step(head["mass"]), a.steps)
+            nb = stage_bytes(r, m, head["mass"], label_np, fixed_np)
+            dom = max(("replay", "scan", "reest"), key=lambda s: stages.get(s, 0.0))
+            name = f"{KERNEL_OF_STAGE[dom]}_learned_d{a.dim}_n{a.n // 1_000_000}m_b{a.bits}.md"
+            traffic, src = profiled_traffic(a, name)
+            headline = dict(head, stages_ms=stages, stage_bytes=nb,
+                            roofline=roofline(dom, nb[dom], stages[dom], traffic, src),
+                            stage_rooflines={s: roofline(s, nb[s], stages[s])["frac"]
+                                             for s in ("replay", "scan", "reest") if s in stages})
+        learned["train"] = {"queries": a.train_queries, "label_nprobe": label_np,
+                            "fallback_nprobe": fixed_np, "epochs": a.epochs,
+                            "loss": [round(v, 3) for v in losses[::5]],
+                            "label_s": round(t_lab, 1), "train_s": round(t_train, 1)}
+        m.close()
+
     uneven = None
     try:
         uneven = measure_uneven(a, g, q, gt, torch)
     except Exception as e:  # reported, never hidden
         log(f"uneven-nprobe leg failed: {e!r}")
-    best = batched if batched is not None and batched["qps"] > plain["qps"] else plain
-    qps = best["qps"]
-    hi, lo, th = plain["hi"], plain["lo"], plain["th"]
-    ms_per_step = a.nq / qps * 1e3
+    np_hi = (plain["targets"].get(str(a.target)) or {}).get("nprobe_bracket", [None, 192])[1]
+    plain_roof = plain_roofline(r, np_hi)
+    latency = drop_in_latency(r, np_hi)
+    log(f"[drop-in] erq_search per query at nprobe {np_hi}: {latency}")
+    cpu = None if a.no_cpu else cpu_baseline(idx, q, a, np_hi)
 
-    # roofline of the plain path's dominant kernel (the scan) at its upper bracketing nprobe
-    roofline = scan_roofline(th["stats"], th["scan_ms"], D, EX)
-    roofline["traffic"], roofline["traffic_source"] = profiled_traffic(a, hi[0])
-
-    # the drop-in's host-pointer entry (erq_search_batch): rotated queries from host memory,
-    # results back to host memory, device buffers allocated per call -- PCIe-inclusive
-    qr_host = g.rotate_dev(q).cpu().numpy()
-    g.search_batch(qr_host, a.k, hi[0])
-    t0 = time.perf_counter()
-    g.search_batch(qr_host, a.k, hi[0])
-    host_qps = a.nq / (time.perf_counter() - t0)
-    log(f"[host] erq_search_batch at nprobe {hi[0]}: {host_qps:.0f} queries/s")
-    del qr_host
-
-    cpu = cpu_baseline(idx, q, a, hi[0], torch)
+    if headline:
+        value, ms_step = headline["qps"], headline["ms"]
+        roof, path = headline["roofline"], "learned-probe batched"
+    else:   # no learned path: the best fixed path at the target (interpolated)
+        value = fixed_best[str(a.target)]
+        ms_step = a.nq / value * 1e3 if value else None
+        roof, path = plain_roof, "fixed nprobe"
+    vs = {}
+    for tgt in TARGETS:
+        lt = (learned or {}).get("targets", {}).get(str(tgt)) if learned else None
+        fb = fixed_best[str(tgt)]
+        vs[str(tgt)] = {"learned_qps": lt["qps"] if lt else None, "fixed_qps": fb,
+                        "learned_vs_fixed": round(lt["qps"] / fb, 3) if lt and fb else None,
+                        "plain_qps": (plain["targets"].get(str(tgt)) or {}).get("qps"),
+                        "learned_vs_plain": round(lt["qps"] / plain["targets"][str(tgt)]["qps"],
+                                                  3) if lt and plain["targets"].get(str(tgt))
+                        else None}
     out = {
         "metric": "QPS at recall@100=0.95, D1536 N1M B4: learned-probe vs fixed nprobe",
-        "value": round(learned["qps"] if learned else qps, 1),
+        "value": value,
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(ms_step, 3) if ms_step else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8 codes / int32 FastScan sums / f32 distances",
+        "dtype": "int8 MFMA FastScan sums / f32 distances",
         "data": "synthetic power-law Gaussian mixture (seeded), unit-norm",
-        "config": {"workload": "embedding-shaped IVF search", "N": a.n, "D": D, "clusters": idx.K,
-                   "bits": a.bits, "nq": a.nq, "k": a.k,
-                   "path": "learned-probe batched" if learned else
-                   ("batched" if best is batched else "plain"),
-                   "sigma": a.sigma},
-        "fixed_nprobe_qps_at_target": round(qps, 1),
-        "host_pointer_plain_qps": round(host_qps, 1),
-        "uneven_nprobe_graph": uneven,
-        "learned": None if learned is None else {
-            "qps_at_target": round(learned["qps"], 1),
-            "vs_fixed": round(learned["qps"] / qps, 3),
-            "mass_bracket": learned["bracket"], "train": learned["train"],
-            "sweep": learned["sweep"]},
-        "paths": {name: None if r is None else {
-            "qps_at_target": round(r["qps"], 1),
-            "nprobe_bracket": [r["lo"][0] if r["lo"] else None, r["hi"][0]],
-            "recall_bracket": [round(r["lo"][1], 4) if r["lo"] else None, round(r["hi"][1], 4)],
-            "stages_ms_at_hi": {"rotate": round(r["th"]["rot_ms"], 3),
-                                "probe": round(r["th"]["probe_ms"], 3),
-                                "scan": round(r["th"]["scan_ms"], 3)},
-            "stats_at_hi": r["th"]["stats"],
-            "scan_algorithmic_gbs_at_hi": r["th"].get("algorithmic_gbs"),
-            "scan_unique_bytes_at_hi": r["th"].get("unique_bytes"),
-            "sweep": [[n, round(x, 4)] for n, x in r["sweep"]]}
-            for name, r in (("plain", plain), ("batched", batched))},
-        "roofline": roofline,
+        "config": {"workload": "embedding-shaped IVF search", "N": a.n, "D": idx.D,
+                   "clusters": idx.K, "bits": a.bits, "nq": a.nq, "k": a.k, "path": path,
+                   "sigma": a.sigma, "recall_target": a.target,
+                   "recall_at_value": headline["recall"] if headline else None},
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "headline": headline,
+        "learned_vs_fixed": vs,
+        "paths": {"plain": plain, "batched": batched, "learned": learned},
+        "roofline_plain_scan": plain_roof,
+        "drop_in_latency": latency,
+        "uneven_nprobe_graph": uneven,
     }
     print(json.dumps(out), flush=True)
     g.close()
-
-
-def scan_roofline(stats, scan_ms, D, EX):
-    """Algorithmic bytes of the per-query scan (every probed block's 4D code bytes and 32
-    norms, plus each survivor's EX-bit code and two factors) over its measured time."""
-    scan_bytes = stats[0] * (4 * D + 128) + stats[2] * (D * EX // 8 + 8)
-    achieved = scan_bytes / (scan_ms / 1e3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "scan_mfma_kernel", "algorithmic_bytes": int(scan_bytes)}
-
-
-def scan_volume(g, q, idx, nprobe, th, torch):
-    """Algorithmic scan bytes of one step at this nprobe (every (query, cluster) pair's
-    blocks plus each survivor's EX-bit code and factors), the same bytes counted once per
-    distinct probed cluster, and the algorithmic rate over the measured scan time."""
-    D, EX = idx.D, idx.EX
-    pid, _ = g.probe_dev(g.rotate_dev(q), nprobe)
-    nblk = torch.from_numpy(((np.diff(idx.vec_start()) + 31) // 32).astype(np.int64)).cuda()
-    p = pid.long().reshape(-1)
-    blocks = int(nblk[p].sum())
-    unique_blocks = int(nblk[torch.unique(p)].sum())
-    alg = blocks * (4 * D + 128) + th["stats"][2] * (D * EX // 8 + 8)
-    return {"pair_blocks": blocks, "algorithmic_bytes": alg,
-            "algorithmic_gbs": round(alg / (th["scan_ms"] / 1e3) / 1e9, 1),
-            "unique_bytes": unique_blocks * (4 * D + 128)}
 
 
 def measure_uneven(a, g, q, gt, torch):
@@ -378,215 +673,8 @@ def measure_uneven(a, g, q, gt, torch):
         f"({a.nq / ms * 1e3:.0f} q/s), graph == eager: {same}")
     return {"qps": round(a.nq / ms * 1e3, 1), "recall": round(rec, 4), "ms": round(ms, 3),
             "mean_nprobe": round(float(npq.float().mean()), 1), "graph_equals_eager": same,
-            "autotune_ms": [round(x, 3) for x in tuned],
+            "autotune_ms": [round(v, 3) for v in tuned],
             "plain_waves": g.option(2), "batch_wg_per_cu": g.option(3)}
-
-
-def profiled_traffic(a, nprobe):
-    """HBM bytes per scan launch from the committed rocprofv3 counter summary of this exact
-    workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None if there is none."""
-    if a.nq != 20000 or a.n % 1_000_000:
-        return None, None
-    name = (f"scan_mfma_kernel_plain_d{a.dim}_n{a.n // 1_000_000}m_b{a.bits}_"
-            f"nprobe{nprobe}.md")
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
-    try:
-        m = re.search(r"traffic_bytes_per_launch: (\d+)", open(path).read())
-    except OSError:
-        return None, None
-    return (int(m.group(1)), "profiles/" + name) if m else (None, None)
-
-
-def profile_plain(a, g, q, idx, torch):
-    """The plain path alone at one nprobe: W untimed and K timed steps, HIP events around
-    each stage on the stream the kernels run on."""
-    nprobe = a.profile_plain
-    stream = torch.cuda.current_stream()
-    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
-    off = torch.arange(0, (a.nq + 1) * nprobe, nprobe, dtype=torch.int32, device="cuda")
-
-    def one(st):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-        ev[0].record(stream)
-        qr = g.rotate_dev(q)
-        ev[1].record(stream)
-        pid, pd = g.probe_dev(qr, nprobe)
-        ev[2].record(stream)
-        g.scan_dev(qr, a.k, off, pid, pd, stats=st)
-        ev[3].record(stream)
-        return ev
-
-    for _ in range(a.warmup):
-        one(None)
-    torch.cuda.synchronize()
-    evs = [one(stats) for _ in range(a.steps)]
-    torch.cuda.synchronize()
-    ms = [sum(e[i].elapsed_time(e[i + 1]) for e in evs) / a.steps for i in range(3)]
-    st = (stats.cpu().numpy() // a.steps).tolist()
-    D, EX = idx.D, idx.EX
-    flop_rot = 2.0 * a.nq * D * D
-    flop_cen = 2.0 * a.nq * idx.K * D
-    print(json.dumps({"profile": "plain", "nprobe": nprobe, "nq": a.nq, "steps": a.steps,
-                      "rotate_ms": round(ms[0], 3), "probe_ms": round(ms[1], 3),
-                      "scan_ms": round(ms[2], 3), "stats": st,
-                      "rotation_tflops": round(flop_rot / ms[0] / 1e9, 1),
-                      "roofline": scan_roofline(st, ms[2], D, EX),
-                      "centroid_gemm_flop": flop_cen}), flush=True)
-
-
-def train_probe_model(a, g, idx, mix, torch):
-    """Labels from the fixed-nprobe batched search over separate training queries, then the
-    probe MLP trained on them.  Returns (model, label_nprobe, losses, label_s, train_s)."""
-    from erq import data
-    from erq.probe_model import ProbeMLP, cluster_of_ids
-    basis, centres = mix
-    if a.load_mlp:
-        m = ProbeMLP.load(a.load_mlp)
-        log(f"probe MLP loaded from {a.load_mlp}")
-        return m, min(a.label_nprobe or 256, idx.K, 2048), [], 0.0, 0.0
-    t0 = time.time()
-    qt, _ = data.mixture_powerlaw_torch(a.train_queries, a.dim, a.clusters, a.sigma, seed=5,
-                                        alpha=1.0, centres=centres, basis=basis)
-    qt_r = g.rotate_dev(qt)
-    del qt
-    c_of = torch.from_numpy(cluster_of_ids(idx)).cuda()
-    label_np = min(a.label_nprobe or 256, idx.K, 2048)
-    lab = torch.empty((a.train_queries, a.k), dtype=torch.int32, device="cuda")
-    for s0 in range(0, a.train_queries, 20000):
-        ids, _ = g.search_batched_dev(qt_r[s0:s0 + 20000], a.k, label_np)
-        lab[s0:s0 + 20000] = c_of[ids.long()]
-    torch.cuda.synchronize()
-    t_lab = time.time() - t0
-    log(f"labels from fixed nprobe={label_np} over {a.train_queries} queries in {t_lab:.1f}s")
-    t0 = time.time()
-    # inputs scaled to unit variance per coordinate (rotated unit-norm queries)
-    m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=1024, seed=7,
-                 input_scale=float(np.sqrt(idx.D)))
-    losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=2e-3, pos_weight=4.0, log=log,
-                     schedule="cosine")
-    torch.cuda.synchronize()
-    if a.save_mlp:
-        m.save(a.save_mlp)
-    return m, label_np, losses, t_lab, time.time() - t0
-
-
-def profile_learned(a, g, q, idx, mix, torch):
-    m, label_np, _, _, _ = train_probe_model(a, g, idx, mix, torch)
-    stream = torch.cuda.current_stream()
-    stats = torch.zeros(8, dtype=torch.int64, device="cuda")
-
-    def one(st):
-        return g.search_learned_dev(m, g.rotate_dev(q), a.k, a.profile_learned, 4, label_np,
-                                    192, 0.5, stats=st)
-
-    for _ in range(a.warmup):
-        one(None)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(a.steps):
-        one(stats)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / a.steps
-    print(json.dumps({"profile": "learned", "mass": a.profile_learned, "nq": a.nq,
-                      "steps": a.steps, "ms_per_search": round(ms, 3),
-                      "qps": round(a.nq / ms * 1e3, 1),
-                      "stats": (stats.cpu().numpy() // a.steps).tolist()}), flush=True)
-    m.close()
-
-
-def measure_learned(a, g, idx, q, gt, mix, step, stats, stream, torch, fixed_np):
-    """Train the probe predictor on labels from fixed-nprobe search over separate training
-    queries, then sweep the mass threshold and report QPS at the recall target."""
-    from erq import data
-    m, label_np, losses, t_lab, t_train = train_probe_model(a, g, idx, mix, torch)
-    max_np = label_np
-
-    def lstep(mass, timed=False):
-        qr = g.rotate_dev(q)
-        return g.search_learned_dev(m, qr, a.k, mass, 4, max_np, min(fixed_np, max_np),
-                                    0.5, stats=stats if timed else None)
-
-    sweep = []
-    for mass in (0.9, 0.95, 0.97, 0.975, 0.98, 0.985, 0.99, 0.995, 0.998):
-        ids, _ = lstep(mass)
-        torch.cuda.synchronize()
-        rec = data.recall_at_k(ids.cpu().numpy().view(np.uint32), gt, a.k)
-        sweep.append((mass, rec))
-        log(f"[learned] mass={mass} recall@{a.k}={rec:.4f}")
-        if rec >= a.target:
-            break
-    above = [s for s in sweep if s[1] >= a.target]
-    below = [s for s in sweep if s[1] < a.target]
-    if not above:
-        raise RuntimeError("learned path did not reach the recall target")
-
-    def timed(mass):
-        for _ in range(a.warmup):
-            lstep(mass)
-        torch.cuda.synchronize()
-        stats.zero_()
-        t1 = time.perf_counter()
-        for _ in range(a.steps):
-            lstep(mass, timed=True)
-        torch.cuda.synchronize()
-        st = (stats.cpu().numpy() // a.steps).tolist()
-        return (time.perf_counter() - t1) * 1e3 / a.steps, st
-
-    hi = above[0]
-    ms_hi, st_hi = timed(hi[0])
-    qps = a.nq / (ms_hi / 1e3)
-    lo = below[-1] if below else None
-    if lo is not None:
-        ms_lo, _ = timed(lo[0])
-        qlo = a.nq / (ms_lo / 1e3)
-        f = (a.target - lo[1]) / (hi[1] - lo[1])
-        qps = qlo + f * (qps - qlo)
-    log(f"[learned] qps at target {qps:.0f} (mass {lo[0] if lo else None}..{hi[0]}), "
-        f"stats {st_hi}")
-    m.close()
-    return dict(qps=qps, bracket=[lo[0] if lo else None, hi[0]],
-                sweep=[[x, round(r, 4)] for x, r in sweep],
-                train={"queries": a.train_queries, "label_nprobe": label_np,
-                       "fallback_nprobe": fixed_np,
-                       "epochs": a.epochs, "loss": [round(x, 3) for x in losses],
-                       "label_s": round(t_lab, 1), "train_s": round(t_train, 1),
-                       "fallbacks_at_hi": st_hi[7]})
-
-
-def cpu_baseline(idx, q, a, nprobe, torch):
-    """The CPU oracle (a scalar C port of the reference search), single thread, timed on
-    a bounded sample of the same queries at the same nprobe: QPS on this host."""
-    try:
-        from oracle import oracle as orc
-    except Exception as e:  # pragma: no cover
-        return {"error": str(e)}
-    oi = orc.OracleIndex(idx)
-    qr = q[:512].cpu().numpy().astype(np.float64)
-    qp = np.zeros((qr.shape[0], idx.D))
-    qp[:, :qr.shape[1]] = qr
-    qr = (qp @ idx.P.astype(np.float64)).astype(np.float32)
-    n = 0
-    t0 = time.perf_counter()
-    while n < qr.shape[0] and time.perf_counter() - t0 < a.cpu_seconds:
-        oi.search(qr[n], a.k, nprobe)
-        n += 1
-    dt = time.perf_counter() - t0
-    log(f"cpu baseline: {n} queries in {dt:.1f}s (1 thread)")
-    out = {"value": round(n / dt, 2), "unit": "queries/s", "cores": 1, "kind": "port",
-           "sample": f"{n} queries of the same batch, nprobe={nprobe}, k={a.k}, single thread, "
-                     f"scalar C oracle"}
-    # all host cores (OpenMP over queries) on a bounded sample sized from the 1-thread rate
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 10**6)
-    m = int(min(qr.shape[0], max(threads, n / dt * threads * 2.0)))
-    t0 = time.perf_counter()
-    oi.search_batch(qr[:m], a.k, nprobe, threads=threads)
-    dt = time.perf_counter() - t0
-    log(f"cpu baseline: {m} queries in {dt:.1f}s ({threads} threads)")
-    out["all_cores"] = {"value": round(m / dt, 2), "cores": threads,
-                        "sample": f"{m} queries, OpenMP over queries"}
-    return out
 
 
 if __name__ == "__main__":

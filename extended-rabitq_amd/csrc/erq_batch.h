@@ -81,5 +81,8 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
 // One round's scan, re-estimate and merge (ba.n_items = rinfo + 4 r), with each query's
 // threshold read from pool_d (its k-th distance when the round starts).
 hipError_t launch_round_scan(const BatchArgs& ba, const RoundBufs& rb, int grid, hipStream_t s);
+// One stage of it: 0 the scan (main and inline passes), 1 the re-estimate, 2 the merge.
+hipError_t launch_round_stage(const BatchArgs& ba, const RoundBufs& rb, int grid, int stage,
+                              hipStream_t s);
 
 }  // namespace erq

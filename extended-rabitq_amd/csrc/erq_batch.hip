@@ -1281,16 +1281,25 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
     return hipGetLastError();
 }
 
+hipError_t launch_round_stage(const BatchArgs& ba, const RoundBufs& rb, int grid, int stage,
+                              hipStream_t s) {
+    switch (stage) {
+    case 0: return launch_batch_scan(ba, grid, s);
+    case 1: return launch_reest(ba, grid, s);
+    case 2:
+        hipLaunchKernelGGL(merge_kernel, dim3(ba.nq), dim3(256), 0, s, ba.pool_d, rb.pool_i, ba.k,
+                           ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.redo, ba.stats);
+        return hipGetLastError();
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_round_scan(const BatchArgs& ba, const RoundBufs& rb, int grid, hipStream_t s) {
-    hipError_t e = launch_batch_scan(ba, grid, s);
-    if (e != hipSuccess) return e;
-    e = launch_reest(ba, grid, s);
-    if (e != hipSuccess) return e;
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(merge_kernel, dim3(ba.nq), dim3(256), 0, s, ba.pool_d, rb.pool_i, ba.k,
-                       ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.redo, ba.stats);
-    return hipGetLastError();
+    for (int stage = 0; stage < 3; stage++) {
+        hipError_t e = launch_round_stage(ba, rb, grid, stage, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace erq

@@ -135,6 +135,33 @@ struct Call {
     }
 };
 
+// Records HIP events around one stage on the stream it runs on while timing is on (not under
+// graph capture, and not beyond the records reserved by erq_timing_begin).
+struct StageTimer {
+    erq_index* h;
+    hipStream_t s;
+    long slot = -1;
+    StageTimer(erq_index* h_, int stage, hipStream_t s_) : h(h_), s(s_) {
+        if (!h->tim_on || capturing(s)) return;
+        const size_t n = h->tim_stage.size();
+        if (2 * (n + 1) > h->tim_ev.size()) return;
+        if (hipEventRecord(h->tim_ev[2 * n], s) != hipSuccess) return;
+        h->tim_stage.push_back(stage);  // capacity reserved: no allocation
+        slot = (long)n;
+    }
+    ~StageTimer() {
+        if (slot >= 0) (void)hipEventRecord(h->tim_ev[2 * slot + 1], s);
+    }
+};
+
+void timing_release(erq_index* h) {
+    for (hipEvent_t e : h->tim_ev)
+        if (e) (void)hipEventDestroy(e);
+    h->tim_ev.clear();
+    h->tim_stage.clear();
+    h->tim_on = false;
+}
+
 template <typename T>
 int upload(T** dst, const T* src, size_t count, const char* what) {
     if (count == 0) {
@@ -165,6 +192,7 @@ void release(erq_index* h) {
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     if (h->ev_last) (void)hipEventDestroy(h->ev_last);
+    timing_release(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -488,6 +516,7 @@ static size_t probe_ws_bytes(size_t nq, size_t K) {
 // Probe selection with its scratch (probe_ws_bytes) at `base`.
 static int probe_into(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
                       uint32_t* pid, float* pd, hipStream_t s, char* base) {
+    StageTimer tim(h, ERQ_STAGE_SELECT, s);
     const size_t K = h->K, D = h->D;
     Ws ws{base};
     float* approx = ws.take<float>(nq * K);
@@ -624,6 +653,7 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
         if (e != hipSuccess) return hip_fail(e, "query order");
         a.qorder = ob + h->K + 1;
     }
+    StageTimer tim(h, ERQ_STAGE_REPLAY, s);
     hipError_t e = launch_scan(a, s);
     if (e != hipSuccess) return hip_fail(e, "scan");
     return ERQ_OK;
@@ -1016,16 +1046,21 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
                                 ids + cur_q0 * k, dists + cur_q0 * k, nullptr, 0);
         sa.bp = h->bp;
         sa.exl = h->exl;
+        static const int kStages[3] = {ERQ_STAGE_SCAN, ERQ_STAGE_REEST, ERQ_STAGE_MERGE};
         for (size_t r = 0; r < b.R; r++) {
             BatchArgs br = ba;
             br.n_items = b.rinfo + 4 * r;
             br.redo = b.redo + r * (cq + 1);
-            hipError_t e = launch_round_scan(br, rb, grid, s);
-            if (e != hipSuccess) return e;
+            for (int st = 0; st < 3; st++) {
+                StageTimer tim(h, kStages[st], s);
+                hipError_t e = launch_round_stage(br, rb, grid, st, s);
+                if (e != hipSuccess) return e;
+            }
             sa.qlist = br.redo;
             sa.r0 = (int)rlo[r];
             sa.r1 = (int)rhi[r];
-            e = launch_scan_round(sa, s);
+            StageTimer tim(h, ERQ_STAGE_REDO, s);
+            hipError_t e = launch_scan_round(sa, s);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -1035,7 +1070,10 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     setup(0, std::min(cq, nq));
     hipError_t e = hipEventRecord(h->ev_fork, s);
     if (e == hipSuccess) e = hipStreamWaitEvent(h->aux, h->ev_fork, 0);
-    if (e == hipSuccess) e = launch_prep_all(ba, rb, (int)b.R, (int)b.r_end, prep_grid, h->aux);
+    if (e == hipSuccess) {
+        StageTimer tim(h, ERQ_STAGE_PREP, h->aux);
+        e = launch_prep_all(ba, rb, (int)b.R, (int)b.r_end, prep_grid, h->aux);
+    }
     hipError_t ej = hipEventRecord(h->ev_join, h->aux);
     if (e != hipSuccess || ej != hipSuccess) {
         (void)hipStreamWaitEvent(s, h->ev_join, 0);
@@ -1049,6 +1087,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         const size_t n = std::min(cq, nq - q0);
         if (q0) {
             setup(q0, n);
+            StageTimer tim(h, ERQ_STAGE_PREP, s);
             e = launch_prep_all(ba, rb, (int)b.R, (int)b.r_end, prep_grid, s);
             if (e != hipSuccess) return hip_fail(e, "batched round preparation");
         }
@@ -1143,6 +1182,7 @@ static int learned_locked(erq_index* h, erq_mlp* m, const float* q_rot_dev, size
                           float mass, size_t min_np, size_t max_np, size_t fb_np, float min_conf,
                           uint32_t* off, uint32_t* ids, float* sqr_y, uint32_t* n_fallback,
                           hipStream_t s, char* base, size_t learned_off) {
+    StageTimer tim(h, ERQ_STAGE_SELECT, s);
     Ws ws{base, learned_off};
     float* prob = ws.take<float>(nq * h->K);
     uint32_t* pad_ids = ws.take<uint32_t>(nq * max_np);
@@ -1381,6 +1421,57 @@ int erq_rotate(erq_index* h, const float* q, size_t nq, float* q_rot) {
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return hip_fail(e, "copy rotated queries");
         return ERQ_OK;
+    });
+}
+
+int erq_timing_begin(erq_index* h, size_t max_records) {
+    return guarded([&]() -> int {
+        if (!h || max_records == 0 || max_records > (1u << 20)) {
+            set_error("erq_timing_begin: null handle or max_records outside [1, 2^20]");
+            return ERQ_EINVAL;
+        }
+        std::lock_guard<std::recursive_mutex> lk(h->mu);
+        timing_release(h);
+        h->tim_ev.assign(2 * max_records, nullptr);
+        for (auto& ev : h->tim_ev) {
+            hipError_t e = hipEventCreate(&ev);
+            if (e != hipSuccess) {
+                timing_release(h);
+                return hip_fail(e, "erq_timing_begin");
+            }
+        }
+        h->tim_stage.reserve(max_records);
+        h->tim_on = true;
+        return ERQ_OK;
+    });
+}
+
+int erq_timing_end(erq_index* h, double* ms_per_stage, uint64_t* count_per_stage) {
+    return guarded([&]() -> int {
+        if (!h || !ms_per_stage) {
+            set_error("erq_timing_end: null argument");
+            return ERQ_EINVAL;
+        }
+        std::lock_guard<std::recursive_mutex> lk(h->mu);
+        for (int i = 0; i < ERQ_NUM_STAGES; i++) {
+            ms_per_stage[i] = 0.0;
+            if (count_per_stage) count_per_stage[i] = 0;
+        }
+        int rc = ERQ_OK;
+        for (size_t r = 0; r < h->tim_stage.size() && !rc; r++) {
+            float ms = 0.f;
+            hipError_t e = hipEventSynchronize(h->tim_ev[2 * r + 1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->tim_ev[2 * r], h->tim_ev[2 * r + 1]);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "erq_timing_end");
+                break;
+            }
+            const int st = h->tim_stage[r];
+            ms_per_stage[st] += ms;
+            if (count_per_stage) count_per_stage[st]++;
+        }
+        timing_release(h);
+        return rc;
     });
 }
 

@@ -58,6 +58,10 @@ struct erq_index {
     uint32_t* hnp = nullptr;     // hq_cap + 1 (per-query probe counts, then CSR offsets)
     size_t hq_cap = 0, hk_cap = 0;
     int depth = 0;  // nesting of entry points on this thread (only the outermost orders)
+    // stage timing (erq_timing_begin / erq_timing_end): event pairs recorded around stages
+    std::vector<hipEvent_t> tim_ev;   // 2 per record
+    std::vector<int> tim_stage;       // stage of each record in use
+    bool tim_on = false;
     // recursive: the host-pointer calls hold it while they call the device-pointer ones
     std::recursive_mutex mu;
 };

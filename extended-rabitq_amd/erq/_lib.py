@@ -68,6 +68,8 @@ SIGNATURES = {
     "erq_search_learned_dev": (c_int, [c_vp, c_vp, c_vp, c_size, c_size, ctypes.c_float, c_size,
                                        c_size, c_size, ctypes.c_float, c_vp, c_vp, c_vp, c_vp]),
     "erq_device_array": (c_vp, [c_vp, c_int]),
+    "erq_timing_begin": (c_int, [c_vp, c_size]),
+    "erq_timing_end": (c_int, [c_vp, c_vp, c_vp]),
 }
 
 BUILDER_SIGNATURES = {

@@ -82,6 +82,19 @@ class IVF:
                                       _stream()))
         return float(out[0]), float(out[1])
 
+    STAGES = ("select", "replay", "prep", "scan", "reest", "merge", "redo")
+
+    def timing_begin(self, max_records=4096):
+        """Record HIP events around every search stage from now on (erq_timing_begin)."""
+        check(_lib.lib().erq_timing_begin(self._h, max_records))
+
+    def timing_end(self):
+        """{stage: (summed ms, recorded count)} since timing_begin (erq_timing_end)."""
+        ms = np.zeros(len(self.STAGES), np.float64)
+        cnt = np.zeros(len(self.STAGES), np.uint64)
+        check(_lib.lib().erq_timing_end(self._h, _ptr(ms), _ptr(cnt)))
+        return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(self.STAGES)}
+
     # ---- header (IVF.hpp:86-90) ----
     def padded_dim(self):
         return _lib.lib().erq_padded_dim(self._h)

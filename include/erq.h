@@ -235,6 +235,26 @@ int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, siz
                            size_t fallback_nprobe, float min_conf, uint32_t* ids_dev,
                            float* dists_dev, uint64_t* stats_dev, void* stream);
 
+/* ---- stage timing (no counterpart in the reference; for measurement) ----
+ * Between erq_timing_begin and erq_timing_end, every search call on the handle records a
+ * pair of HIP events around each of its stages, on the stream that stage runs on (up to
+ * max_records stages in all; later ones go unrecorded; nothing is recorded under HIP graph
+ * capture).  erq_timing_end waits for the recorded events and returns, per stage, the summed
+ * milliseconds and the number of recorded stages. */
+enum {
+    ERQ_STAGE_SELECT = 0, /* probe selection: centroid contraction + top-nprobe, or MLP +
+                             learned lists */
+    ERQ_STAGE_REPLAY = 1, /* the per-query exact scan (all of it, or ranks [0, 4)) */
+    ERQ_STAGE_PREP = 2,   /* batched rounds: work lists and per-pair digits */
+    ERQ_STAGE_SCAN = 3,   /* batched rounds: the FastScan pass over work items */
+    ERQ_STAGE_REEST = 4,  /* batched rounds: ex-bit re-estimate of the survivors */
+    ERQ_STAGE_MERGE = 5,  /* batched rounds: candidates into the top-k pools */
+    ERQ_STAGE_REDO = 6,   /* batched rounds: rescan of overflowed (query, round)s */
+    ERQ_NUM_STAGES = 7
+};
+int erq_timing_begin(erq_index* h, size_t max_records);
+int erq_timing_end(erq_index* h, double* ms_per_stage, uint64_t* count_per_stage);
+
 /* Device pointers of the resident index arrays (for tools that read the HBM layout).
  * which: 0 short blocks, 1 long codes, 2 ex factors, 3 ids, 4 rotated centroids,
  * 5 rotation P, 6 cluster sizes (u32), 7 first block per cluster (u32),

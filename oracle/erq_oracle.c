@@ -18,6 +18,19 @@
 #include <omp.h>
 #endif
 
+/* Fast-math variant (orc_set_fast_math(1)): the float rewrites GCC's -Ofast (the reference's
+ * CMakeLists.txt:21) is free to apply to the path, applied here one by one so their effect on
+ * the results can be measured against the literal source semantics (the default, 0):
+ *   F1 1/norm in float instead of a double quotient (normalize_query16, space.hpp:763)
+ *   F2 vmax / 8191 as vmax * (1/8191), u / width as u * (1/width) (space.hpp:705-708)
+ *   F3 1/sqrt(D) as an approximate reciprocal square root (12 bits) plus one Newton step
+ *   F4 (-5 * qnorm) * one_over_sqrtD reassociated as -5 * (qnorm * one_over_sqrtD)
+ *   F5 the lower bound t*onorm + qs + onorm*onorm as fma(onorm, fma(t, c2, onorm), qs)
+ *   F6 the re-estimated distance with its multiply-adds fused (HASearcher.hpp:225-232) */
+static int g_fast = 0;
+void orc_set_fast_math(int on) { g_fast = on; }
+int orc_fast_math(void) { return g_fast; }
+
 /* _mm512_reduce_add_ps (used by space.hpp:30, :120, :172, :774): halves added
  * pairwise -- lanes i and i+8, then j and j+4, then k and k+2, then 0 and 1. */
 float orc_reduce16(const float p[16]) {
@@ -44,7 +57,7 @@ float orc_l2sqr16(const float* x, const float* y, size_t D) {
 float orc_normalize_query(float* unit_q, const float* q, const float* c, float norm, size_t D) {
     const float eps = 1e-5f;
     if (norm > eps) {
-        float fac = (float)(1.0 / (double)norm);
+        float fac = g_fast ? 1.0f / norm : (float)(1.0 / (double)norm); /* F1 */
         float p[16] = {0};
         for (size_t i = 0; i < D; i += 16)
             for (int l = 0; l < 16; l++) {
@@ -76,9 +89,11 @@ void orc_quantize_query(int16_t* out, const float* u, size_t D, float* delta) {
     orc_data_range(u, D, &vl, &vr);
     float a = fabsf(vl), b = fabsf(vr);
     float vmax = a < b ? b : a;
-    float width = vmax / (float)((1 << 13) - 1);
+    float width = g_fast ? vmax * (1.0f / (float)((1 << 13) - 1)) /* F2 */
+                         : vmax / (float)((1 << 13) - 1);
+    const float rwidth = 1.0f / width;
     for (size_t i = 0; i < D; i++) {
-        float tmp = u[i] / width;
+        float tmp = g_fast ? u[i] * rwidth : u[i] / width; /* F2 */
         double r = (double)tmp + 0.5 - (double)(tmp < 0.0f);
         out[i] = (int16_t)(int)r;
     }
@@ -124,6 +139,15 @@ static inline int block_nibble(const uint8_t* codes, size_t m, int v) {
 
 float orc_one_over_sqrtD(size_t D) {
     /* HASearcher.hpp:56 -- 1.0 / std::sqrt((float)D), double quotient stored as float */
+    if (g_fast) { /* F3: a 12-bit estimate (rsqrtps' accuracy), one Newton-Raphson step */
+        const float x = (float)D;
+        float r0 = 1.0f / sqrtf(x);
+        uint32_t b;
+        memcpy(&b, &r0, 4);
+        b &= ~((1u << 11) - 1u); /* keep 12 significant bits */
+        memcpy(&r0, &b, 4);
+        return r0 * (1.5f - 0.5f * x * r0 * r0);
+    }
     return (float)(1.0 / (double)sqrtf((float)D));
 }
 
@@ -143,7 +167,8 @@ uint32_t orc_fastscan_block(const uint8_t* codes, const uint16_t* lut, const flo
     size_t M = D >> 2;
     const float const_bound = 0.58f;
     float c1 = (float)(0.5 * (double)sumq - (double)const_bound);
-    float c2 = (-5.0f * qnorm) * one_over_sqrtD;
+    float c2 = g_fast ? -5.0f * (qnorm * one_over_sqrtD) /* F4 */
+                      : (-5.0f * qnorm) * one_over_sqrtD;
     float qs = qnorm * qnorm;
     uint32_t mask = 0;
     for (int v = 0; v < 32; v++) {
@@ -158,10 +183,14 @@ uint32_t orc_fastscan_block(const uint8_t* codes, const uint16_t* lut, const flo
         int32_t I = s + shift;
         float ipq = (float)I * delta;
         float t = ipq - c1;
-        t = t * c2;
-        t = t * onorm[v];
-        t = t + qs;
-        t = t + onorm[v] * onorm[v];
+        if (g_fast) { /* F5 */
+            t = fmaf(onorm[v], fmaf(t, c2, onorm[v]), qs);
+        } else {
+            t = t * c2;
+            t = t * onorm[v];
+            t = t + qs;
+            t = t + onorm[v] * onorm[v];
+        }
         if (t < distk) mask |= 1u << v;
         if (ipq_out) ipq_out[v] = ipq;
         if (sum_out) sum_out[v] = s;
@@ -348,6 +377,24 @@ void orc_centroid_topn(const float* q, const float* centroids, size_t K, size_t 
     free(c);
 }
 
+/* The re-estimated distance, HASearcher.hpp:225-232, with C's float -> double promotions:
+ * the bracket becomes double at "- (FAC_RESCALE - 0.5) * sumq", the product at "* (...)".
+ * F6: its multiply-adds fused. */
+static float ex_distance(float onorm, float sqr_y, float xip, float y, int R, float ipq, float ip,
+                         float sumq) {
+    if (g_fast) {
+        const float cc = fmaf((float)R, ipq, ip);
+        const double g = fma(-((double)R - 0.5), (double)sumq, (double)cc);
+        return (float)fma(-(double)(xip * y), g, (double)fmaf(onorm, onorm, sqr_y));
+    }
+    float a = onorm * onorm + sqr_y;
+    float bb = xip * y;
+    float cc = (float)R * ipq + ip;
+    double g = (double)cc - ((double)R - 0.5) * (double)sumq;
+    double h = (double)bb * g;
+    return (float)((double)a - h);
+}
+
 /* HASearcher::search_cluster + scan_one_block, HASearcher.hpp:82-112, :187-237. */
 static void search_cluster(const orc_index* idx, size_t cid, const float* q, float sqr_y,
                            float one_over_sqrtD, float* unit_q, int16_t* qq, uint16_t* lut,
@@ -383,22 +430,52 @@ static void search_cluster(const orc_index* idx, size_t cid, const float* q, flo
             int j = __builtin_ctz(mask);
             mask &= mask - 1;
             size_t vi = v0 + b * 32 + (size_t)j;
-            float sqr_x = onorm[j] * onorm[j];
             float ip = orc_ip_ex(unit_q, idx->long_code + vi * code_len, D, (int)idx->EX);
-            float xip = idx->ex_factor[vi];
-            /* HASearcher.hpp:229-232 with C promotions: the bracket becomes double at
-             * "- (FAC_RESCALE - 0.5) * sumq", the product at "* (...)" */
-            float a = sqr_x + sqr_y;
-            float bb = xip * y;
-            float cc = (float)R * ipq[j] + ip;
-            double g = (double)cc - ((double)R - 0.5) * (double)sumq;
-            double h = (double)bb * g;
-            float ex_dist = (float)((double)a - h);
+            float ex_dist = ex_distance(onorm[j], sqr_y, idx->ex_factor[vi], y, R, ipq[j], ip,
+                                        sumq);
             orc_pool_insert(pool, idx->ids[vi], ex_dist);
             distk = orc_pool_distk(pool);
             if (st) st->survivors++;
         }
     }
+}
+
+/* Every vector of cluster cid against one query: the lower bound (FastScan.hpp:295-323) and
+ * the re-estimated distance (HASearcher.hpp:225-232), both under the current float mode. */
+void orc_pair_estimates(const orc_index* idx, const float* q_rot, size_t cid, float sqr_y,
+                        float* lower, float* exd) {
+    size_t D = idx->D;
+    size_t n = (size_t)idx->cluster_sizes[cid];
+    float* unit_q = (float*)malloc(D * sizeof(float));
+    int16_t* qq = (int16_t*)malloc(D * sizeof(int16_t));
+    uint16_t* lut = (uint16_t*)malloc((D / 4) * 16 * sizeof(uint16_t));
+    const float* c = idx->centroids + cid * D;
+    float y = sqrtf(sqr_y);
+    float sumq = orc_normalize_query(unit_q, q_rot, c, y, D);
+    float delta;
+    orc_quantize_query(qq, unit_q, D, &delta);
+    int32_t shift = orc_pack_lut(lut, qq, D);
+    int R = 1 << idx->EX;
+    size_t block_bytes = 4 * D + 128, code_len = D * idx->EX / 8;
+    const uint8_t* block = idx->short_data + idx->block_start[cid] * block_bytes;
+    size_t v0 = idx->vec_start[cid];
+    float ipq[32], lw[32];
+    for (size_t b = 0; b < (n + 31) / 32; b++, block += block_bytes) {
+        size_t np = n - b * 32 < 32 ? n - b * 32 : 32;
+        const float* onorm = (const float*)(block + 4 * D);
+        orc_fastscan_block(block, lut, onorm, sumq, y, delta, shift, orc_one_over_sqrtD(D),
+                           0.0f, D, ipq, NULL, lw);
+        for (size_t j = 0; j < np; j++) {
+            size_t vi = v0 + b * 32 + j;
+            float ip = orc_ip_ex(unit_q, idx->long_code + vi * code_len, D, (int)idx->EX);
+            lower[b * 32 + j] = lw[j];
+            exd[b * 32 + j] = ex_distance(onorm[j], sqr_y, idx->ex_factor[vi], y, R, ipq[j], ip,
+                                          sumq);
+        }
+    }
+    free(unit_q);
+    free(qq);
+    free(lut);
 }
 
 void orc_ivf_search_probes(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
@@ -494,11 +571,8 @@ void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, s
                     mask &= mask - 1;
                     size_t vi = v0 + b * 32 + (size_t)j;
                     float ip = orc_ip_ex(unit_q, idx->long_code + vi * code_len, D, (int)idx->EX);
-                    float a = onorm[j] * onorm[j] + sqr_y;
-                    float bb = idx->ex_factor[vi] * y;
-                    float cc = (float)R * ipq[j] + ip;
-                    double g = (double)cc - ((double)R - 0.5) * (double)sumq;
-                    float ex = (float)((double)a - (double)bb * g);
+                    float ex = ex_distance(onorm[j], sqr_y, idx->ex_factor[vi], y, R, ipq[j], ip,
+                                           sumq);
                     if (ex <= thr) {
                         if (st) st->inserts++;
                         if (nc == ncap) cs = (cand*)realloc(cs, (ncap *= 2) * sizeof(cand));

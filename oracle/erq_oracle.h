@@ -85,6 +85,15 @@ void orc_ivf_search_rounds(const orc_index* idx, const float* q_rot, size_t k, s
 void orc_ivf_search_batch(const orc_index* idx, const float* q_rot, size_t nq, size_t k,
                           const uint32_t* nprobe_per_q, uint32_t* ids, float* dists, int threads);
 
+/* Float mode: 0 (default) the source's literal IEEE semantics; 1 the -Ofast rewrites
+ * F1..F6 listed in erq_oracle.c, for measuring their effect.  Process-wide. */
+void orc_set_fast_math(int on);
+int orc_fast_math(void);
+/* Lower bounds and re-estimated distances of every vector of cluster cid for one query
+ * (arrays of cluster_sizes[cid] floats), under the current float mode. */
+void orc_pair_estimates(const orc_index* idx, const float* q_rot, size_t cid, float sqr_y,
+                        float* lower, float* exd);
+
 /* Format helpers the reference's builder defines (pack_codes.hpp:33-98,
  * Quantizer.hpp:371-384, :491-631). */
 void orc_pack_block_codes(const uint8_t* bits, size_t n, size_t D, uint8_t* block_codes);

@@ -73,6 +73,12 @@ def lib():
         L.orc_ivf_search_batch.restype = None
         L.orc_ivf_search_batch.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size, c_vp,
                                            c_vp, c_vp, ctypes.c_int]
+        L.orc_set_fast_math.restype = None
+        L.orc_set_fast_math.argtypes = [ctypes.c_int]
+        L.orc_fast_math.restype = ctypes.c_int
+        L.orc_pair_estimates.restype = None
+        L.orc_pair_estimates.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, ctypes.c_float,
+                                         c_vp, c_vp]
         L.orc_pack_block_codes.restype = None
         L.orc_pack_block_codes.argtypes = [c_vp, c_size, c_size, c_vp]
         _lib = L
@@ -162,6 +168,19 @@ def pack_block_codes(bits, D):
     out = np.empty(4 * D, np.uint8)
     lib().orc_pack_block_codes(_p(bits), bits.shape[0], D, _p(out))
     return out
+
+
+class fast_math:
+    """Context manager: the oracle's float rewrites of -Ofast (F1..F6 in erq_oracle.c) on
+    while inside; the literal source semantics otherwise."""
+
+    def __enter__(self):
+        lib().orc_set_fast_math(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_fast_math(0)
+        return False
 
 
 class Pool:
@@ -254,6 +273,16 @@ class OracleIndex:
         lib().orc_ivf_search_batch(ctypes.byref(self.view), _p(q), nq, k, _p(npq), _p(ids),
                                    _p(d), threads)
         return ids, d
+
+    def pair_estimates(self, q_rot, cid, sqr_y):
+        """(lower bounds, re-estimated distances) of every vector of cluster cid."""
+        q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)
+        n = int(self.idx.cluster_sizes[cid])
+        lower = np.empty(n, np.float32)
+        exd = np.empty(n, np.float32)
+        lib().orc_pair_estimates(ctypes.byref(self.view), _p(q), int(cid), float(sqr_y),
+                                 _p(lower), _p(exd))
+        return lower, exd
 
     def centroid_topn(self, q_rot, nprobe):
         q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)

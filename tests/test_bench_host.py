@@ -25,25 +25,45 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in line["error"]
 
 
-def test_scan_roofline_arithmetic(monkeypatch):
+def test_roofline_arithmetic(monkeypatch):
     bench, _ = _bench(monkeypatch)
-    # blocks x (4D + 128) + survivors x (D*EX/8 + 8), over the scan time, against 8 TB/s
-    stats = [68441459, 2131345010, 11944812, 3840000, 0, 0, 0, 0]
-    r = bench.scan_roofline(stats, 73.598, 1536, 3)
-    assert r["algorithmic_bytes"] == 68441459 * 6272 + 11944812 * 584 == 436240601056
+    # the per-query scan: blocks x (4D + 128) + survivors x (D*EX/8 + 8), over its time
+    nbytes = 68441459 * 6272 + 11944812 * 584
+    assert nbytes == 436240601056
+    r = bench.roofline("replay", nbytes, 73.598)
     assert r["achieved"] == pytest.approx(436240601056 / 0.073598 / 1e9, abs=0.1)
     assert r["frac"] == pytest.approx(r["achieved"] / 8000.0, abs=1e-4)
-    assert (r["bound"], r["unit"], r["peak"]) == ("hbm", "GB/s", 8000.0)
+    assert (r["bound"], r["unit"], r["peak"], r["kernel"]) == ("hbm", "GB/s", 8000.0,
+                                                              "scan_mfma_kernel")
+
+
+def test_interpolation_brackets_the_target():
+    sys.path.insert(0, REPO)
+    import bench
+    sweep = [[64, 0.90], [128, 0.94], [192, 0.96], [256, 0.97]]
+    lo, hi, f = bench.interpolate(sweep, 0.95)
+    assert (lo[0], hi[0]) == (128, 192) and f == pytest.approx(0.5)
+    lo, hi, f = bench.interpolate(sweep, 0.85)   # below the first point: no lower bracket
+    assert lo is None and hi[0] == 64 and f == 1.0
+    assert bench.interpolate(sweep, 0.99) is None
 
 
 def test_profiled_traffic_reads_committed_profile(monkeypatch):
     bench, a = _bench(monkeypatch)
-    traffic, src = bench.profiled_traffic(a, 192)
-    assert src == "profiles/scan_mfma_kernel_plain_d1536_n1m_b4_nprobe192.md"
+    name = "scan_mfma_kernel_plain_d1536_n1m_b4_nprobe192.md"
+    traffic, src = bench.profiled_traffic(a, name)
+    assert src == "profiles/" + name
     text = open(os.path.join(REPO, src)).read()
     assert f"traffic_bytes_per_launch: {traffic}" in text
     # FETCH_SIZE (KiB, x2 copy calibration) + WRITE_SIZE (KiB, x1) of the same launch
     assert traffic > 436240601056  # at least the algorithmic bytes
     # another workload has no committed profile
     monkeypatch.setattr(a, "nq", 1000)
-    assert bench.profiled_traffic(a, 192) == (None, None)
+    assert bench.profiled_traffic(a, name) == (None, None)
+
+
+def test_host_info_names_the_cpu():
+    sys.path.insert(0, REPO)
+    import bench
+    h = bench.host_info()
+    assert h["logical_cpus"] >= 1 and h["affinity_cpus"] >= 1 and h["cpu_model"]

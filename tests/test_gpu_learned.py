@@ -47,6 +47,48 @@ def test_mlp_forward_and_gradients_match_oracle(xs):
     m.close()
 
 
+def test_mlp_configured_size_matches_oracle():
+    """The configured network, 1536 -> 512 -> 512 -> 4096 (inputs scaled by sqrt(1536)), on a
+    batch of 32: forward and gradients against the FP64 oracle with the GPU's BF16 steps."""
+    import torch
+    from erq.probe_model import ProbeMLP
+    from oracle import mlp_oracle as mo
+    din, hid, dout, B = 1536, 512, 4096, 32
+    rng = np.random.default_rng(21)
+    sizes = [(hid, din), (hid,), (hid, hid), (hid,), (dout, hid), (dout,)]
+    parts = []
+    for shp in sizes:   # He-uniform weights, small random biases
+        fan = shp[1] if len(shp) == 2 else 1
+        a = np.sqrt(6.0 / fan) if len(shp) == 2 else 0.1
+        parts.append(rng.uniform(-a, a, int(np.prod(shp))))
+    theta = np.concatenate(parts).astype(np.float32)
+    xs = float(np.sqrt(din))
+    x = rng.standard_normal((B, din)).astype(np.float32) / xs   # unit-norm-like queries
+    y = (rng.random((B, dout)) < 0.02).astype(np.float32)
+    m = ProbeMLP(din, dout, hidden=hid, max_batch=B, seed=1, input_scale=xs)
+    m.set_params(theta)
+    xt, yt = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    p = m.forward_dev(xt)
+    loss = m.loss_grad_dev(xt, yt, 4.0)
+    torch.cuda.synchronize()
+    g = m.get_grads()
+    L_ref, g_ref, p_ref = mo.grad_bf16(theta.astype(np.float64), x, y, din, hid, dout, 4.0, xs)
+    np.testing.assert_allclose(p.cpu().numpy(), p_ref, rtol=5e-3, atol=1e-5)
+    assert abs(float(loss[0]) - L_ref) <= 1e-3 * abs(L_ref)
+    # per parameter block: FP32 accumulation order is the only difference from the oracle
+    o = 0
+    for shp in sizes:
+        n = int(np.prod(shp))
+        gb, rb = g[o:o + n], g_ref[o:o + n]
+        assert np.abs(gb - rb).max() <= 2e-2 * np.abs(rb).max(), (shp, np.abs(gb - rb).max())
+        o += n
+    g64 = mo.grad(theta.astype(np.float64), x, y, din, hid, dout, 4.0, xs)
+    cos = g @ g64 / np.linalg.norm(g) / np.linalg.norm(g64)
+    print(f"configured MLP: loss {float(loss[0]):.4f} (oracle {L_ref:.4f}), cos(g, g64) {cos:.5f}")
+    assert cos > 0.99
+    m.close()
+
+
 def test_mlp_non_multiple_batch_forward_and_chunking():
     import torch
     from erq.probe_model import ProbeMLP
@@ -112,19 +154,33 @@ def test_learned_probe_search_end_to_end():
     losses = m.train(qtr_r, labels, epochs=8, batch=512, lr=2e-3, pos_weight=4.0)
     assert losses[-1] < losses[0]
     gt, _ = data.exact_knn_np(x, qte, 100)
+    # the gain: the recall the learned lists reach at mass 0.8 costs fixed-nprobe search at
+    # least twice as many probes per query (interpolating its recall curve)
+    fixed = []
+    for npb in range(1, 9):
+        fi, _ = g.search_batched_dev(qte_r, 100, npb)
+        fixed.append((npb, data.recall_at_k(fi.cpu().numpy(), gt, 100)))
+    lg, _ = g.search_learned_dev(m, qte_r, 100, mass=0.8, min_nprobe=1, max_nprobe=48,
+                                 fallback_nprobe=16, min_conf=0.0)
+    off8, _, _, _ = g.probe_learned_dev(m, qte_r, 0.8, 1, 48, 16, 0.0)
+    torch.cuda.synchronize()
+    n_learned = float(np.diff(off8.cpu().numpy()).mean())
+    r_learned = data.recall_at_k(lg.cpu().numpy(), gt, 100)
+    n_fixed = None
+    for (n0, r0), (n1, r1) in zip(fixed, fixed[1:]):
+        if r0 < r_learned <= r1:
+            n_fixed = n0 + (r_learned - r0) / (r1 - r0) * (n1 - n0)
+            break
+    print(f"learned (mass 0.8): recall {r_learned:.4f} at {n_learned:.2f} probes per query; "
+          f"fixed needs {n_fixed} probes; fixed curve {[(n, round(r, 4)) for n, r in fixed]}")
+    assert r_learned > fixed[0][1], "the learned lists must beat one probe per query"
+    assert n_fixed is not None and n_fixed >= 2.0 * n_learned
     stats = torch.zeros(8, dtype=torch.int64, device="cuda")
     li, ld = g.search_learned_dev(m, qte_r, 100, mass=0.95, min_nprobe=2, max_nprobe=48,
                                   fallback_nprobe=16, min_conf=0.0, stats=stats)
     off, pid, pd, nfb = g.probe_learned_dev(m, qte_r, 0.95, 2, 48, 16, 0.0)
     torch.cuda.synchronize()
     offs = off.cpu().numpy()
-    nprobe_mean = float(np.diff(offs).mean())
-    r_learned = data.recall_at_k(li.cpu().numpy(), gt, 100)
-    fi, _ = g.search_batched_dev(qte_r, 100, max(1, int(round(nprobe_mean))))
-    r_fixed = data.recall_at_k(fi.cpu().numpy(), gt, 100)
-    print(f"mean nprobe {nprobe_mean:.1f}: recall learned {r_learned:.4f} fixed {r_fixed:.4f} "
-          f"fallbacks {int(nfb[0])}")
-    assert r_learned >= r_fixed - 0.01
     # probe lists are ordered by exact centroid distance and hold distinct clusters
     pid_h, pd_h = pid.cpu().numpy(), pd.cpu().numpy()
     from oracle import oracle as orc
@@ -146,6 +202,79 @@ def test_learned_probe_search_end_to_end():
     bi, bd = g.search_batched_dev(qte_r, 100, 16)
     torch.cuda.synchronize()
     assert torch.equal(fb_i, bi) and torch.equal(fb_d, bd)
+    m.close()
+    g.close()
+
+
+def _kept_by_rule(p, mass, min_np, max_np):
+    """The mass-prefix rule of erq_probe_learned_dev restated in numpy with the selection
+    kernel's float32 sums: clusters by probability (descending, ties by id), the first
+    T = min(max_np, K); thread t of 256 sums entries [t per, (t + 1) per) in order, the
+    thread sums are scanned (Hillis-Steele), the goal is float32(mass) times the total, and
+    n is the first prefix (thread's base plus its own entries, in order) reaching it; then
+    n is clamped to [max(min_np, 1), T]."""
+    K = p.size
+    order = np.lexsort((np.arange(K), -p.astype(np.float64)))
+    T = min(max_np, K)
+    top = order[:T]
+    pv = p[top].astype(np.float32)
+    L = T
+    per = (L + 255) // 256
+    own = np.zeros(256, np.float32)
+    for t in range(256):
+        acc = np.float32(0)
+        for i in range(t * per, min(L, (t + 1) * per)):
+            acc = np.float32(acc + pv[i])
+        own[t] = acc
+    s = own.copy()
+    o = 1
+    while o < 256:
+        s = np.concatenate([s[:o], (s[o:] + s[:-o]).astype(np.float32)])
+        o <<= 1
+    goal = np.float32(np.float32(mass) * s[255])
+    n = L
+    for t in range(256):
+        acc = np.float32(s[t - 1]) if t else np.float32(0)
+        for i in range(t * per, min(L, (t + 1) * per)):
+            acc = np.float32(acc + pv[i])
+            if acc >= goal:
+                n = min(n, i + 1)
+                break
+    n = max(min(max(n, min_np), T), 1)
+    return top[:n]
+
+
+@pytest.mark.parametrize("mass,min_np,max_np", [(0.5, 1, 48), (0.9, 2, 48), (0.99, 4, 30),
+                                                (1.0, 1, 64)])
+def test_learned_kept_set_is_the_mass_prefix_rule(mass, min_np, max_np):
+    """erq_probe_learned_dev keeps exactly the clusters the mass-prefix rule names (restated
+    in numpy from the model's own probabilities), ordered by (exact distance, id)."""
+    import torch
+    from erq.gpu import IVF
+    from erq.probe_model import ProbeMLP
+    from oracle import oracle as orc
+    idx, x, qtr, qte = _small_index()
+    g = IVF.from_index(idx)
+    m = ProbeMLP(idx.D, idx.K, hidden=128, max_batch=512, seed=4)
+    qte_np = rotate_np(idx, qte)
+    q = torch.from_numpy(qte_np).cuda()
+    prob = m.forward_dev(q).cpu().numpy()
+    off, pid, pd, nfb = g.probe_learned_dev(m, q, mass, min_np, max_np, 8, 0.0)
+    torch.cuda.synchronize()
+    off, pid, pd = off.cpu().numpy(), pid.cpu().numpy().view(np.uint32), pd.cpu().numpy()
+    oi = orc.OracleIndex(idx)
+    lens = []
+    for i in range(0, qte_np.shape[0], 5):
+        kept = _kept_by_rule(prob[i], mass, min_np, max_np)
+        all_i, all_d = oi.centroid_topn(qte_np[i], idx.K)
+        dist = np.empty(idx.K, np.float32)
+        dist[all_i] = all_d
+        want = kept[np.lexsort((kept, dist[kept]))]
+        seg = slice(off[i], off[i + 1])
+        np.testing.assert_array_equal(pid[seg], want.astype(np.uint32), err_msg=f"query {i}")
+        np.testing.assert_array_equal(pd[seg].view(np.uint32), dist[want].view(np.uint32))
+        lens.append(off[i + 1] - off[i])
+    print(f"mass {mass}: kept {np.mean(lens):.1f} clusters on average")
     m.close()
     g.close()
 
