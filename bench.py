@@ -345,7 +345,7 @@ def stage_bytes(r, m, mass, max_np, fixed_np):
     # (round, cluster) entry counts of the batched rounds: ranks [4,16) [16,64) [64,192) then
     # 128 wide
     offs_h = offs.cpu().numpy()
-    pid_h = pid.cpu().numpy().view(np.uint32)
+    pid_h = pid.cpu().numpy().view(np.uint32)[:offs_h[-1]]   # CSR: the used prefix
     rank = np.arange(pid_h.size) - np.repeat(offs_h[:-1], np.diff(offs_h))
     sel = rank >= 4
     rk = rank[sel]

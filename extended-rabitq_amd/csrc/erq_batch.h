@@ -31,7 +31,8 @@ struct BatchArgs {
     uint32_t* cand_cnt;        // nq
     float* cand_d;             // nq x cand_cap
     uint32_t* cand_i;          // nq x cand_cap
-    uint4* surv;               // survivor records: entry, vector, FastScan product, |o - c|,
+    uint4* surv;               // survivor records: entry, vector, FastScan product scaled by
+                               // delta (above 4 chunks: the exact integer), |o - c|,
                                // one slice of surv_cap / grid records per scan workgroup
     uint32_t* surv_count;      // per scan workgroup: records in its slice
     size_t surv_cap;

@@ -418,7 +418,8 @@ __device__ __forceinline__ void load_tile_codes(const uint8_t* bp, uint32_t blk,
 // and chunks accumulate into one 32 x 32 integer tile: lane (h, n) ends with the sums of
 // vector n for rows 8 b + 4 h + i.  Every vector whose lower bound is below its entry's
 // threshold goes to that entry's LDS list; the lists are written to the round's survivor
-// list (one record per (entry, vector): entry, vector, exact FastScan integer, |o - c|)
+// list (one record per (entry, vector): entry, vector, FastScan product scaled by delta --
+// above 4 chunks the exact integer instead, for the wrap correction --, |o - c|)
 // for reest_kernel.  An item with a list beyond kMainSurv, or whose records do not fit in
 // the workgroup's slice, is queued (both halves) for batch_inline_kernel instead.  Up to
 // 4 chunks the next item's digits and both blocks' codes are held in registers ahead of
@@ -573,7 +574,9 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                         const uint32_t sidx = atomicAdd(&s_cnt[qq], 1u);
                         if (sidx < (uint32_t)kMainSurv) {
                             s_vec[qq * kMainSurv + sidx] = (uint32_t)vloc;
-                            s_int[qq * kMainSurv + sidx] = I;
+                            // the exact integer where the wrap correction needs it, else the
+                            // scaled product the re-estimate uses
+                            s_int[qq * kMainSurv + sidx] = NCH > 4 ? I : __float_as_int(ipq);
                             s_on[qq * kMainSurv + sidx] = on;
                         }
                     }
@@ -811,7 +814,7 @@ __global__ void __launch_bounds__(kThreads) batch_inline_kernel(BatchArgs a) {
 // residual u once in its LDS slice (lane-major, from 16-byte loads of the rotated query and
 // the centroid) and re-estimates the run's records four at a time, one per 16-lane group.
 constexpr int kReestWG = 4;  // re-estimation workgroups per scan workgroup
-template <int EX>
+template <int EX, bool WRAP>
 __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int D = a.D;
@@ -819,8 +822,8 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
     const int l16 = lane & 15, lg = lane >> 4;
     const int S = ul_stride(D);  // lane-major residual rows (erq_device.h)
     float* uw = reinterpret_cast<float*>(smem) + (size_t)wid * 16 * S;
-    // above 4 chunks: each wave's digits of the staged pair, for the wrap correction
-    const bool wrap = a.nchunk > 4;
+    // above 4 chunks (WRAP): each wave's digits of the staged pair, for the wrap correction
+    constexpr bool wrap = WRAP;
     const size_t qv_bytes = wrap ? (size_t)a.nchunk * 1024 : 0;
     int16_t* qvw = reinterpret_cast<int16_t*>(smem + (size_t)256 * S + (size_t)wid * qv_bytes);
     uint32_t* pre = reinterpret_cast<uint32_t*>(smem + (size_t)256 * S + 4 * qv_bytes);
@@ -909,7 +912,7 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
                 for (int d = lane; d < D; d += 64) uw[ul_index(d, S)] = value;
             }
             wave_sync();
-            if (wrap) {
+            if constexpr (wrap) {
                 store_qv(uw, S, D, a.nchunk, 1.0 / (double)P.delta, qvw, lane);
                 wave_sync();
             }
@@ -919,12 +922,12 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             Pn = reinterpret_cast<const QParam*>(a.qparams)[e_next];
             pn_e = e_next;
         }
-        const uint32_t vi = act ? rec.y : a.vstart[P.cid];
+        const uint32_t vi = act ? rec.y : (wrap ? a.vstart[P.cid] : 0u);
         const float on = act ? __uint_as_float(rec.w) : 0.f;
-        const int I = act ? (int)rec.z : 0;
-        float ipq = (float)I * P.delta;
+        const int I = act ? (int)rec.z : 0;  // the integer (wrap) or the scaled product
+        float ipq = wrap ? (float)I * P.delta : __int_as_float(I);
         bool keep = act;
-        if (wrap)  // wave-uniform
+        if constexpr (wrap)
             keep = wrap_retest(a, P, qvw, (int)(vi - a.vstart[P.cid]), I, on, l16, ipq) && act;
         const float exd = reestimate<EX>(a, P, [&](int i) { return uw[l16 * S + i]; },
                                          a.exl + ((size_t)vi * 16 + l16) * a.exl_w, keep, ipq,
@@ -1211,39 +1214,44 @@ static hipError_t launch_batch_scan(const BatchArgs& ba, int grid, hipStream_t s
     }
 }
 
-static hipError_t launch_reest(const BatchArgs& ba, int scan_grid, hipStream_t s) {
+template <bool WRAP>
+static hipError_t launch_reest_w(const BatchArgs& ba, int scan_grid, hipStream_t s) {
     // residuals, digits (wrap correction above 4 chunks), slice prefix
     const size_t lds = (size_t)256 * ul_stride(ba.D) +
-                       (ba.nchunk > 4 ? 4 * (size_t)ba.nchunk * 1024 : 0) +
-                       4 * ((size_t)scan_grid + 1);
-    // as many workgroups as stay resident together (LDS-bound), a multiple of 8: each takes
-    // a fixed share of the records, so a second partial wave of workgroups would be a tail
+                       (WRAP ? 4 * (size_t)ba.nchunk * 1024 : 0) + 4 * ((size_t)scan_grid + 1);
     if (lds > 65536) {  // above the 64 KB default (large D): raise the kernel's limit
         hipError_t e = hipSuccess;
         switch (ba.EX) {
-        case 2: e = set_lds(reest_kernel<2>, lds); break;
-        case 3: e = set_lds(reest_kernel<3>, lds); break;
-        case 4: e = set_lds(reest_kernel<4>, lds); break;
-        case 6: e = set_lds(reest_kernel<6>, lds); break;
-        case 7: e = set_lds(reest_kernel<7>, lds); break;
-        case 8: e = set_lds(reest_kernel<8>, lds); break;
+        case 2: e = set_lds(reest_kernel<2, WRAP>, lds); break;
+        case 3: e = set_lds(reest_kernel<3, WRAP>, lds); break;
+        case 4: e = set_lds(reest_kernel<4, WRAP>, lds); break;
+        case 6: e = set_lds(reest_kernel<6, WRAP>, lds); break;
+        case 7: e = set_lds(reest_kernel<7, WRAP>, lds); break;
+        case 8: e = set_lds(reest_kernel<8, WRAP>, lds); break;
         default: return hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
     }
+    // as many workgroups as stay resident together (LDS-bound), a multiple of 8: each takes
+    // a fixed share of the records, so a second partial wave of workgroups would be a tail
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, 163840 / (lds + 1024)));
     const dim3 grid(ba.num_cu > 0 ? (unsigned)(per_cu * ((ba.num_cu + 7) / 8 * 8))
                                   : (unsigned)(scan_grid * kReestWG));
     switch (ba.EX) {
-    case 2: hipLaunchKernelGGL(reest_kernel<2>, grid, dim3(256), lds, s, ba, scan_grid); break;
-    case 3: hipLaunchKernelGGL(reest_kernel<3>, grid, dim3(256), lds, s, ba, scan_grid); break;
-    case 4: hipLaunchKernelGGL(reest_kernel<4>, grid, dim3(256), lds, s, ba, scan_grid); break;
-    case 6: hipLaunchKernelGGL(reest_kernel<6>, grid, dim3(256), lds, s, ba, scan_grid); break;
-    case 7: hipLaunchKernelGGL(reest_kernel<7>, grid, dim3(256), lds, s, ba, scan_grid); break;
-    case 8: hipLaunchKernelGGL(reest_kernel<8>, grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 2: hipLaunchKernelGGL((reest_kernel<2, WRAP>), grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 3: hipLaunchKernelGGL((reest_kernel<3, WRAP>), grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 4: hipLaunchKernelGGL((reest_kernel<4, WRAP>), grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 6: hipLaunchKernelGGL((reest_kernel<6, WRAP>), grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 7: hipLaunchKernelGGL((reest_kernel<7, WRAP>), grid, dim3(256), lds, s, ba, scan_grid); break;
+    case 8: hipLaunchKernelGGL((reest_kernel<8, WRAP>), grid, dim3(256), lds, s, ba, scan_grid); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+static hipError_t launch_reest(const BatchArgs& ba, int scan_grid, hipStream_t s) {
+    return ba.nchunk > 4 ? launch_reest_w<true>(ba, scan_grid, s)
+                         : launch_reest_w<false>(ba, scan_grid, s);
 }
 
 hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int r_end,
