@@ -72,6 +72,9 @@ def parse(argv=None):
                     help="save the trained probe predictor to this .npz file")
     ap.add_argument("--load-mlp", type=str, default="",
                     help="load the probe predictor from this .npz file instead of training it")
+    ap.add_argument("--overlap", type=int, default=0, choices=(0, 1),
+                    help="ERQ_OPT_BATCH_OVERLAP: round preparation beside the exact replay (1) "
+                         "or before it (0)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="print the recall sweep of both fixed paths and exit (no timing)")
     return ap.parse_args(argv)
@@ -488,7 +491,7 @@ def profile_learned(a, r, mix):
     print(json.dumps({"profile": "learned", "mass": a.profile_learned, "nq": a.nq,
                       "steps": a.steps, "ms_per_search": round(ms, 3),
                       "qps": round(a.nq / ms * 1e3, 1), "stages_ms": stages,
-                      "stage_bytes": nb}), flush=True)
+                      "stage_bytes": nb, "overlap": r.g.option(7)}), flush=True)
     m.close()
 
 
@@ -512,6 +515,7 @@ def main(argv=None):
         del src, dst
     x, q, idx, mix = build_workload(a, torch)
     g = IVF.from_index(idx)
+    g.set_option(7, a.overlap)
     if a.profile_plain or a.profile_learned:
         del x
         r = Runner(a, g, idx, q, None, torch)

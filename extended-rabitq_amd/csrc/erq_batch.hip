@@ -476,7 +476,36 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
             for (int i = 0; i < kStage; i++) dreg[i] = digit_unit(item, i);
         }
     };
-    if (it_first < it_hi) load_digits(ifirst + it_first);
+    // block blk's codes (16 bytes per (chunk, K half) per lane) and its |o - c| of vector n32
+    auto load_codes = [&](uint32_t blk, uint4 (&wc)[NCH][2], float& on) {
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const uint8_t* src = a.bp + (((size_t)blk * NCH + c) * 32 + n32) * 64 + 16 * h;
+            wc[c][0] = *reinterpret_cast<const uint4*>(src);
+            wc[c][1] = *reinterpret_cast<const uint4*>(src + 32);
+        }
+        on = reinterpret_cast<const float*>(a.short_data + (size_t)blk * BB + 4 * (size_t)D)[n32];
+    };
+    // this wave's blocks of an item (bfirst + wid and bfirst + wid + 4): up to 4 chunks their
+    // codes and factors are loaded one item ahead, while the previous item's vectors are
+    // tested and its lists written
+    uint4 w[kAhead ? 2 : 1][NCH][2];
+    float onp[2] = {0.f, 0.f};
+    auto fetch_codes = [&](uint32_t item) {
+        const uint32_t c = a.items[3 * item];
+        const int bf = (int)a.items[3 * item + 2] >> 1;
+        const int be = min((int)((a.csize[c] + 31) >> 5), bf + kItemTiles / 2);
+        const uint32_t bb = a.bstart[c];
+#pragma unroll
+        for (int j = 0; j < (kAhead ? 2 : 0); j++) {
+            const int bl = bf + wid + 4 * j;
+            if (bl < be) load_codes(bb + (uint32_t)bl, w[j], onp[j]);
+        }
+    };
+    if (it_first < it_hi) {
+        load_digits(ifirst + it_first);
+        if constexpr (kAhead) fetch_codes(ifirst + it_first);
+    }
     for (uint32_t it = it_first; it < it_hi; it += it_step) {
         __syncthreads();  // the previous item's LDS is no longer read
         const uint32_t item = ifirst + it;  // global index
@@ -486,25 +515,6 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         const int n = (int)a.csize[cid];
         const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
         const int bend = min((n + 31) >> 5, bfirst + kItemTiles / 2);
-        // block bl's codes: 16 bytes per (chunk, K half) per lane
-        auto load_codes = [&](int bl, uint4 (&wc)[NCH][2]) {
-#pragma unroll
-            for (int c = 0; c < NCH; c++) {
-                const uint8_t* src =
-                    a.bp + (((size_t)(b0 + (uint32_t)bl) * NCH + c) * 32 + n32) * 64 + 16 * h;
-                wc[c][0] = *reinterpret_cast<const uint4*>(src);
-                wc[c][1] = *reinterpret_cast<const uint4*>(src + 32);
-            }
-        };
-        // this wave's blocks bfirst + wid + 4 j
-        uint4 w[kAhead ? 2 : 1][NCH][2];
-        if constexpr (kAhead) {
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int bl = bfirst + wid + 4 * j;
-                if (bl < bend) load_codes(bl, w[j]);
-            }
-        }
         if (tid < kItemGroup) {
             s_cnt[tid] = 0;
             if (tid < (int)elen) {
@@ -524,37 +534,10 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         __syncthreads();
         if (it + it_step < it_hi) load_digits(item + it_step);
 
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int bl = bfirst + wid + 4 * j;
-            if (bl >= bend) break;
-            uint4 (&wj)[NCH][2] = w[kAhead ? j : 0];
-            if constexpr (!kAhead) load_codes(bl, wj);
-            // two independent accumulation chains (even and odd planes), summed after:
-            // integer addition, so the order does not matter
-            v16i acc4[2] = {};
-#pragma unroll
-            for (int c = 0; c < NCH; c++) {
-#pragma unroll
-                for (int sk = 0; sk < 2; sk++) {
-                    const uint4 cw = wj[c][sk];
-#pragma unroll
-                    for (int p = 0; p < 8; p++) {
-                        const v4i bv = v4i{(int)((cw.x >> p) & 0x01010101u),
-                                           (int)((cw.y >> p) & 0x01010101u),
-                                           (int)((cw.z >> p) & 0x01010101u),
-                                           (int)((cw.w >> p) & 0x01010101u)};
-                        const v4i av = *reinterpret_cast<const v4i*>(
-                            A + ((((c * 8 + p) * 4 + 2 * sk + h) * 32) + n32) * 16);
-                        acc4[p & 1] =
-                            __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc4[p & 1], 0, 0, 0);
-                    }
-                }
-            }
-            const v16i acc = acc4[0] + acc4[1];
+        // the lower-bound test of block bl's 32 vectors against the item's entries, from the
+        // block's integer tile (lane (h, n): rows 8 b + 4 h + i of vector n)
+        auto test_block = [&](int bl, const v16i& acc, float on) {
             const int vloc = 32 * bl + n32;
-            const float on = reinterpret_cast<const float*>(
-                a.short_data + (size_t)(b0 + (uint32_t)bl) * BB + 4 * (size_t)D)[n32];
             if (lane == 0) st_tiles += (n - 32 * bl > 16) ? 2 : 1;
 #pragma unroll
             for (int b = 0; b < 4; b++) {
@@ -581,6 +564,82 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                         }
                     }
                 }
+            }
+        };
+        // the A fragment of (chunk c, K half sk, plane p) and bit-plane p of a code dword set
+        auto afrag = [&](int c, int sk, int p) {
+            return *reinterpret_cast<const v4i*>(
+                A + ((((c * 8 + p) * 4 + 2 * sk + h) * 32) + n32) * 16);
+        };
+        auto plane = [](const uint4& cw, int p) {
+            return v4i{(int)((cw.x >> p) & 0x01010101u), (int)((cw.y >> p) & 0x01010101u),
+                       (int)((cw.z >> p) & 0x01010101u), (int)((cw.w >> p) & 0x01010101u)};
+        };
+        if constexpr (kAhead) {
+            const int bl0 = bfirst + wid, bl1 = bl0 + 4;
+            if (bl0 < bend) {
+                if (bl1 < bend) {
+                    // both of the wave's blocks: each A fragment is read from LDS once and
+                    // feeds one MFMA per block (two independent accumulation chains)
+                    v16i acc0 = {}, acc1 = {};
+#pragma unroll
+                    for (int c = 0; c < NCH; c++) {
+#pragma unroll
+                        for (int sk = 0; sk < 2; sk++) {
+#pragma unroll 2
+                            for (int p = 0; p < 8; p++) {
+                                const v4i av = afrag(c, sk, p);
+                                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                                    av, plane(w[0][c][sk], p), acc0, 0, 0, 0);
+                                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                                    av, plane(w[1][c][sk], p), acc1, 0, 0, 0);
+                            }
+                        }
+                    }
+                    const float on0 = onp[0], on1 = onp[1];
+                    if (it + it_step < it_hi) fetch_codes(item + it_step);
+                    test_block(bl0, acc0, on0);
+                    test_block(bl1, acc1, on1);
+                } else {
+                    // one block: even and odd planes in two chains, summed after (integer
+                    // addition, so the order does not matter)
+                    v16i acc4[2] = {};
+#pragma unroll
+                    for (int c = 0; c < NCH; c++) {
+#pragma unroll
+                        for (int sk = 0; sk < 2; sk++) {
+#pragma unroll
+                            for (int p = 0; p < 8; p++)
+                                acc4[p & 1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                                    afrag(c, sk, p), plane(w[0][c][sk], p), acc4[p & 1], 0, 0, 0);
+                        }
+                    }
+                    const float on0 = onp[0];
+                    if (it + it_step < it_hi) fetch_codes(item + it_step);
+                    test_block(bl0, acc4[0] + acc4[1], on0);
+                }
+            } else if (it + it_step < it_hi) {
+                fetch_codes(item + it_step);
+            }
+        } else {
+            for (int j = 0; j < 2; j++) {
+                const int bl = bfirst + wid + 4 * j;
+                if (bl >= bend) break;
+                uint4 (&wj)[NCH][2] = w[0];
+                float on;
+                load_codes(b0 + (uint32_t)bl, wj, on);
+                v16i acc4[2] = {};
+#pragma unroll
+                for (int c = 0; c < NCH; c++) {
+#pragma unroll
+                    for (int sk = 0; sk < 2; sk++) {
+#pragma unroll
+                        for (int p = 0; p < 8; p++)
+                            acc4[p & 1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                                afrag(c, sk, p), plane(wj[c][sk], p), acc4[p & 1], 0, 0, 0);
+                    }
+                }
+                test_block(bl, acc4[0] + acc4[1], on);
             }
         }
         __syncthreads();

@@ -690,6 +690,10 @@ int erq_set_option(erq_index* h, int option, long value) {
             h->batch_pair_budget = (size_t)value;
             return ERQ_OK;
         }
+        if (option == ERQ_OPT_BATCH_OVERLAP && (value == 0 || value == 1)) {
+            h->batch_overlap = (int)value;
+            return ERQ_OK;
+        }
         set_error("erq_set_option: unknown option or value");
         return ERQ_EINVAL;
     });
@@ -704,6 +708,7 @@ long erq_get_option(const erq_index* h, int option) {
     case ERQ_OPT_BATCH_SURVIVOR_CAP: return (long)h->batch_surv_cap;
     case ERQ_OPT_PLAIN_ORDER: return h->plain_order;
     case ERQ_OPT_BATCH_PAIR_BUDGET: return (long)h->batch_pair_budget;
+    case ERQ_OPT_BATCH_OVERLAP: return h->batch_overlap;
     default: return -1;
     }
 }
@@ -1066,8 +1071,34 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         return hipSuccess;
     };
     // the first chunk's rounds are all prepared on the second stream while the exact ranks
-    // of every query run on the caller's; joined before the first round's scan
+    // of every query run on the caller's; joined before the first round's scan (or, without
+    // the overlap, prepared on the caller's stream first)
     setup(0, std::min(cq, nq));
+    if (!h->batch_overlap) {
+        hipError_t e;
+        {
+            StageTimer tim(h, ERQ_STAGE_PREP, s);
+            e = launch_prep_all(ba, rb, (int)b.R, (int)b.r_end, prep_grid, s);
+        }
+        if (e != hipSuccess) return hip_fail(e, "batched round preparation");
+        rc = scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, s,
+                         (int)kExactRanks);
+        if (rc) return rc;
+        for (size_t q0 = 0; q0 < nq; q0 += cq) {
+            const size_t n = std::min(cq, nq - q0);
+            if (q0) {
+                setup(q0, n);
+                StageTimer tim(h, ERQ_STAGE_PREP, s);
+                e = launch_prep_all(ba, rb, (int)b.R, (int)b.r_end, prep_grid, s);
+                if (e != hipSuccess) return hip_fail(e, "batched round preparation");
+            }
+            e = launch_zero(b.cand_cnt, n, s);
+            if (e == hipSuccess) e = launch_zero(b.redo, b.R * (cq + 1), s);
+            if (e == hipSuccess) e = rounds();
+            if (e != hipSuccess) return hip_fail(e, "batched round");
+        }
+        return ERQ_OK;
+    }
     hipError_t e = hipEventRecord(h->ev_fork, s);
     if (e == hipSuccess) e = hipStreamWaitEvent(h->aux, h->ev_fork, 0);
     if (e == hipSuccess) {

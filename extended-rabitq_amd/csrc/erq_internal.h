@@ -41,6 +41,7 @@ struct erq_index {
     int batch_wg_per_cu = 2;  // ERQ_OPT_BATCH_WG_PER_CU
     size_t batch_surv_cap = size_t(1) << 26;  // ERQ_OPT_BATCH_SURVIVOR_CAP
     size_t batch_pair_budget = size_t(1) << 23;  // ERQ_OPT_BATCH_PAIR_BUDGET
+    int batch_overlap = 0;                       // ERQ_OPT_BATCH_OVERLAP
     // second stream of the batched search: the rounds' work lists and preparation run on it
     // while the exact ranks run on the caller's stream (fork and join by events)
     hipStream_t aux = nullptr;

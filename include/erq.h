@@ -9,9 +9,9 @@
  * Pointer conventions: functions without a _dev suffix take host pointers and run on the
  * handle's own stream, synchronously; _dev functions take device pointers, enqueue on
  * `stream` exactly as given (a hipStream_t; 0 is the null stream) and return without
- * synchronising.  The batched searches fork part of their work onto a second stream of the
- * handle and join it back into `stream` before returning (events, so a HIP graph capture
- * of `stream` takes both).
+ * synchronising.  With ERQ_OPT_BATCH_OVERLAP set, the batched searches fork part of their
+ * work onto a second stream of the handle and join it back into `stream` before returning
+ * (events, so a HIP graph capture of `stream` takes both).
  * Thread safety: calls on one handle are serialised by a per-handle mutex, and a call on one
  * stream is ordered after the device work of the previous call on the handle when that ran
  * on another stream (an event each call records when it returns).  Calls captured into a HIP
@@ -63,7 +63,12 @@ enum {
      * 3 KB of digits each at D = 1536).  Every round of a chunk of queries is prepared at
      * once, so a chunk's pairs over all ranks past the exact ones must fit it.  Results
      * are identical; only workspace size and speed differ. */
-    ERQ_OPT_BATCH_PAIR_BUDGET = 6
+    ERQ_OPT_BATCH_PAIR_BUDGET = 6,
+    /* 0 (default): the batched search prepares its rounds on the caller's stream before the
+     * exact replay of ranks [0, 4); 1: on a second stream beside it.  Results are identical;
+     * both are VALU-bound, so side by side they take about their sum (measured: 23.1 ms per
+     * 20k-query learned search without the overlap, 23.5 ms with it). */
+    ERQ_OPT_BATCH_OVERLAP = 7
 };
 int erq_set_option(erq_index* h, int option, long value);
 long erq_get_option(const erq_index* h, int option);
