@@ -363,6 +363,7 @@ def stage_bytes(r, m, mass, max_np, fixed_np):
         "replay": int(st4[0] * (4 * D + 128) + st4[2] * per_surv),
         "scan": int(item_blocks * (4 * D + 128) + entries * (2 * Dp + 48) + surv_rounds * 16),
         "reest": int(surv_rounds * per_surv),
+        "launches": {"replay": 1, "scan": cnt.size // K, "reest": cnt.size // K},
         "counts": {"replay_blocks": int(st4[0]), "replay_survivors": int(st4[2]),
                    "round_entries": entries, "round_item_blocks": item_blocks,
                    "round_survivors": surv_rounds, "round_candidates": int(st[4]),
@@ -374,12 +375,16 @@ KERNEL_OF_STAGE = {"replay": "scan_mfma_kernel", "scan": "batch_scan_kernel",
                    "reest": "reest_kernel", "prep": "prep_kernel"}
 
 
-def roofline(stage, nbytes, ms, traffic=None, source=None):
+def roofline(stage, nbytes, ms, traffic=None, source=None, launches=1):
+    """A stage's roofline per launch of its kernel: the stage runs `launches` dispatches per
+    search (one per batched round), so bytes and time are both divided by that count."""
+    nbytes, ms = nbytes / launches, ms / launches
     achieved = nbytes / (ms / 1e3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": KERNEL_OF_STAGE.get(stage, stage), "ms_per_launch": round(ms, 4),
-            "algorithmic_bytes": int(nbytes), "traffic_source": source}
+            "launches_per_search": launches, "algorithmic_bytes": int(nbytes),
+            "traffic_source": source}
 
 
 def profiled_traffic(a, name):
@@ -568,7 +573,8 @@ def main(argv=None):
             name = f"{KERNEL_OF_STAGE[dom]}_learned_d{a.dim}_n{a.n // 1_000_000}m_b{a.bits}.md"
             traffic, src = profiled_traffic(a, name)
             headline = dict(head, stages_ms=stages, stage_bytes=nb,
-                            roofline=roofline(dom, nb[dom], stages[dom], traffic, src),
+                            roofline=roofline(dom, nb[dom], stages[dom], traffic, src,
+                                              nb["launches"][dom]),
                             stage_rooflines={s: roofline(s, nb[s], stages[s])["frac"]
                                              for s in ("replay", "scan", "reest") if s in stages})
         learned["train"] = {"queries": a.train_queries, "label_nprobe": label_np,

@@ -35,6 +35,12 @@ def test_roofline_arithmetic(monkeypatch):
     assert r["frac"] == pytest.approx(r["achieved"] / 8000.0, abs=1e-4)
     assert (r["bound"], r["unit"], r["peak"], r["kernel"]) == ("hbm", "GB/s", 8000.0,
                                                               "scan_mfma_kernel")
+    # a batched stage of 4 round dispatches: bytes and time per dispatch, same fraction
+    s = bench.roofline("scan", 18046216368, 5.6057, launches=4)
+    assert s["algorithmic_bytes"] == 18046216368 // 4
+    assert s["ms_per_launch"] == pytest.approx(5.6057 / 4, abs=1e-4)
+    assert s["frac"] == pytest.approx(18046216368 / 5.6057e-3 / 1e9 / 8000.0, abs=1e-4)
+    assert (s["kernel"], s["launches_per_search"]) == ("batch_scan_kernel", 4)
 
 
 def test_interpolation_brackets_the_target():
