@@ -873,6 +873,7 @@ __global__ void __launch_bounds__(kThreads) batch_inline_kernel(BatchArgs a) {
 // residual u once in its LDS slice (lane-major, from 16-byte loads of the rotated query and
 // the centroid) and re-estimates the run's records four at a time, one per 16-lane group.
 constexpr int kReestWG = 4;  // re-estimation workgroups per scan workgroup
+constexpr int kReestSlices = 3;  // 256-dim slices of a pair's rows in flight while staging
 template <int EX, bool WRAP>
 __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -959,14 +960,7 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
             const float* cen = a.centroids + (size_t)P.cid * D;
             wave_sync();
             if (P.y > 1e-5f) {
-                for (int d = 4 * lane; d < D; d += 256) {
-                    const float4 qa = *reinterpret_cast<const float4*>(qv + d);
-                    const float4 ca = *reinterpret_cast<const float4*>(cen + d);
-                    uw[ul_index(d, S)] = (qa.x - ca.x) * P.fac;
-                    uw[ul_index(d + 1, S)] = (qa.y - ca.y) * P.fac;
-                    uw[ul_index(d + 2, S)] = (qa.z - ca.z) * P.fac;
-                    uw[ul_index(d + 3, S)] = (qa.w - ca.w) * P.fac;
-                }
+                stage_residual<kReestSlices>(qv, cen, P.fac, D, S, uw, lane);
             } else {
                 for (int d = lane; d < D; d += 64) uw[ul_index(d, S)] = value;
             }

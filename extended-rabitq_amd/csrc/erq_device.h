@@ -191,6 +191,33 @@ __device__ __forceinline__ bool residual_delta(const float* __restrict__ q,
     return res;
 }
 
+// u = (q - c) * fac stored lane-major in uq, as residual_delta stores it, with NL 256-dim
+// slices of q and c loaded before any is stored (one memory wait per NL slices).
+template <int NL>
+__device__ __forceinline__ void stage_residual(const float* __restrict__ q,
+                                               const float* __restrict__ c, float fac, int D,
+                                               int S, float* uq, int lane) {
+    for (int d0 = 4 * lane; d0 < D; d0 += 256 * NL) {
+        float4 qa[NL], ca[NL];
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            const int d = min(d0 + 256 * j, D - 4);
+            qa[j] = *reinterpret_cast<const float4*>(q + d);
+            ca[j] = *reinterpret_cast<const float4*>(c + d);
+        }
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            const int d = d0 + 256 * j;
+            if (d < D) {
+                uq[ul_index(d, S)] = (qa[j].x - ca[j].x) * fac;
+                uq[ul_index(d + 1, S)] = (qa[j].y - ca[j].y) * fac;
+                uq[ul_index(d + 2, S)] = (qa[j].z - ca[j].z) * fac;
+                uq[ul_index(d + 3, S)] = (qa[j].w - ca[j].w) * fac;
+            }
+        }
+    }
+}
+
 // sumq: the sum of u in _mm512 lane order (16 lane-strided chains over the lane-major rows,
 // then butterfly16), by the 16 lanes of a group (l = lane & 15); every lane of the group
 // gets it.  When y <= 1e-5 the reference takes D / sqrt(D) instead (prepare_residual).
@@ -208,11 +235,12 @@ __device__ __forceinline__ float residual_sum16(const float* uq, int S, int D, i
 }
 
 // Both parts by one wave (lanes 0..15 run the sum).
+template <int NL = 1>
 __device__ __forceinline__ void prepare_residual(const float* __restrict__ q,
                                                  const float* __restrict__ c, float y, int D,
                                                  int S, float* uq, int lane, float& fac,
                                                  float& sumq, float& delta) {
-    const bool res = residual_delta(q, c, y, D, S, uq, lane, fac, delta);
+    const bool res = residual_delta<NL>(q, c, y, D, S, uq, lane, fac, delta);
     wave_sync();
     if (res) {
         const float part = residual_sum16(uq, S, D, lane & 15);

@@ -518,6 +518,9 @@ __device__ __forceinline__ void pool_merge_did(const WaveLds& w, int& psize, int
 }
 
 enum { kModeReplay = 0, kModeRound = 1 };
+// 256-dim slices of q and c in flight while staging a pair's residual (fewer above 3 chunks,
+// where the code registers leave no room for them)
+template <int NCH> constexpr int kReplaySlices = NCH <= 3 ? 3 : 1;
 
 // LDS of one wave of scan_mfma_kernel: the lane-major residual, the A operand and its zero
 // twin, the digits for the wrap correction (NCH > 4), the block's integer sums, the pool and
@@ -635,21 +638,12 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
         const float sqr_y = a.probe_sqr_y[p];
         const float y = sqrtf(sqr_y);
         const float* c = a.centroids + (size_t)cid * D;
-        float fac, sumq, delta;
-        prepare_residual(q, c, y, D, S, uq, lane, fac, sumq, delta);
-        quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta, reinterpret_cast<uint32_t*>(A),
-                        lane, kWrap ? qv : nullptr);
-        wave_sync();
-
         const int n = (int)a.csize[cid];
         const int nb = (n + 31) >> 5;
         const uint32_t blk0 = a.bstart[cid];
         const uint32_t v0 = a.vstart[cid];
-        float distk = MODE == kModeReplay ? ((psize == k) ? w.pd[k - 1] : FLT_MAX) : thr;
-        const float c1 = (float)(0.5 * (double)sumq - (double)const_bound);
-        const float c2 = (-5.0f * y) * a.one_over_sqrtD;
-        const float qs = y * y;
-        // codes and norms of block b + 1 are loaded while block b is on the MFMA
+        // codes and norms of block b + 1 are loaded while block b is on the MFMA; block 0's
+        // while the residual is prepared
         uint4 wn0[NCH], wn1[NCH];
         float on_n = 0.f;
         auto load_block = [&](uint32_t blk) {
@@ -664,6 +658,16 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
                              : 0.f;
         };
         if (nb > 0) load_block(blk0);
+        float fac, sumq, delta;
+        prepare_residual<kReplaySlices<NCH>>(q, c, y, D, S, uq, lane, fac, sumq, delta);
+        quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta, reinterpret_cast<uint32_t*>(A),
+                        lane, kWrap ? qv : nullptr);
+        wave_sync();
+
+        float distk = MODE == kModeReplay ? ((psize == k) ? w.pd[k - 1] : FLT_MAX) : thr;
+        const float c1 = (float)(0.5 * (double)sumq - (double)const_bound);
+        const float c2 = (-5.0f * y) * a.one_over_sqrtD;
+        const float qs = y * y;
         for (int b = 0; b < nb; ++b) {
             const uint32_t blk = blk0 + (uint32_t)b;
             const int np = min(32, n - b * 32);
