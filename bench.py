@@ -81,7 +81,7 @@ def parse(argv=None):
 
 
 def build_workload(a, torch):
-    from erq import data, index
+    from erq import data, gpu, index
     t0 = time.time()
     ncomp = a.clusters
     x, (basis, centres) = data.mixture_powerlaw_torch(a.n, a.dim, ncomp, a.sigma, seed=1,
@@ -90,7 +90,8 @@ def build_workload(a, torch):
                                        centres=centres, basis=basis)
     log(f"data generated N={a.n} D={a.dim} nq={a.nq} in {time.time() - t0:.1f}s")
     t0 = time.time()
-    c, assign = data.kmeans_torch(x, a.clusters, a.kmeans_iters, seed=4)
+    c, assign, _ = gpu.kmeans_dev(x, a.clusters, a.kmeans_iters,
+                                  gpu.kmeans_init_rows(a.n, a.clusters, 4))
     log(f"k-means {a.clusters} clusters, {a.kmeans_iters} iters in {time.time() - t0:.1f}s")
     t0 = time.time()
     D = index.padded_dim(a.dim)
@@ -509,7 +510,7 @@ def main(argv=None):
     import torch
 
     from erq import data
-    from erq.gpu import IVF
+    from erq.gpu import IVF, knn_dev as gpu_knn
     torch.cuda.set_device(0)
     if a.calib_copy:
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
@@ -532,13 +533,14 @@ def main(argv=None):
         return
 
     t0 = time.time()
-    gt, _ = data.exact_knn_torch(x, q, a.k)
+    gt, _ = gpu_knn(x, q, a.k)
     gt = gt.cpu().numpy()
-    xs = x.cpu().numpy()   # FP64 spot check of the FP32 ground truth on 4 queries
+    xs = x.cpu().numpy()   # numpy FP64 spot check of the ground truth on 4 queries
     g64, _ = data.exact_knn_np(xs, q[:4].cpu().numpy(), a.k)
     spot = data.recall_at_k(gt[:4], g64, a.k)
     del xs
-    log(f"ground truth (GPU FP32) in {time.time() - t0:.1f}s, FP64 spot-check agreement {spot:.4f}")
+    log(f"ground truth (erq_knn_dev) in {time.time() - t0:.1f}s, FP64 spot-check agreement "
+        f"{spot:.4f}")
     del x
     torch.cuda.empty_cache()
     r = Runner(a, g, idx, q, gt, torch)

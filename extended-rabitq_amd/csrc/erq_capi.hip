@@ -91,24 +91,6 @@ void resolve_scratch(int device) {
 
 bool ex_ok(size_t ex) { return ex == 2 || ex == 3 || ex == 4 || ex == 6 || ex == 7 || ex == 8; }
 
-// Runs an entry point's body; a C++ exception becomes an error code (a failed host
-// allocation is ERQ_ENOMEM).
-template <typename F>
-int guarded(F&& f) noexcept {
-    try {
-        return f();
-    } catch (const std::bad_alloc&) {
-        set_error("host allocation failed");
-        return ERQ_ENOMEM;
-    } catch (const std::exception& ex) {
-        set_error(std::string("internal error: ") + ex.what());
-        return ERQ_EINVAL;
-    } catch (...) {
-        set_error("internal error");
-        return ERQ_EINVAL;
-    }
-}
-
 bool capturing(hipStream_t s) {
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
     return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;

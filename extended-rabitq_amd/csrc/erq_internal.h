@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <exception>
 #include <mutex>
 #include <new>
 
@@ -72,6 +73,24 @@ namespace erq {
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 int workspace(erq_index* h, size_t bytes, void** out);
+
+// Runs an entry point's body; a C++ exception becomes an error code (a failed host
+// allocation is ERQ_ENOMEM).
+template <typename F>
+int guarded(F&& f) noexcept {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return ERQ_ENOMEM;
+    } catch (const std::exception& ex) {
+        set_error(std::string("internal error: ") + ex.what());
+        return ERQ_EINVAL;
+    } catch (...) {
+        set_error("internal error");
+        return ERQ_EINVAL;
+    }
+}
 
 struct ScanArgs {
     const float* q_rot;

@@ -418,6 +418,12 @@ hipError_t launch_probe_select(const float* approx, const float* qnorm, const fl
     if (nq == 0) return hipSuccess;
     if (nprobe < 1 || nprobe > kCapC || nprobe > K) return hipErrorInvalidValue;
     size_t lds = probe_select_lds(K, D);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&probe_select_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(probe_select_kernel, dim3(nq), dim3(kThreads), lds, s, approx, qnorm, cnorm,
                        q_rot, centroids, K, D, nprobe, probe_ids, probe_d);
     return hipGetLastError();

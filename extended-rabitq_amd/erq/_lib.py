@@ -70,6 +70,14 @@ SIGNATURES = {
     "erq_device_array": (c_vp, [c_vp, c_int]),
     "erq_timing_begin": (c_int, [c_vp, c_size]),
     "erq_timing_end": (c_int, [c_vp, c_vp, c_vp]),
+    "erq_knn_dev": (c_int, [c_vp, ctypes.c_uint64, c_vp, ctypes.c_uint64, ctypes.c_uint32,
+                            ctypes.c_uint32, c_vp, c_vp, c_vp]),
+    "erq_kmeans_assign_dev": (c_int, [c_vp, ctypes.c_uint64, ctypes.c_uint32, c_vp,
+                                      ctypes.c_uint32, c_vp, c_vp, c_vp]),
+    "erq_kmeans_update_dev": (c_int, [c_vp, ctypes.c_uint64, ctypes.c_uint32, c_vp,
+                                      ctypes.c_uint32, c_vp, c_vp, c_vp]),
+    "erq_kmeans_dev": (c_int, [c_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.c_uint32, c_vp, c_vp, c_vp, c_vp, c_vp]),
 }
 
 BUILDER_SIGNATURES = {

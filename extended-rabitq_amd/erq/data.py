@@ -126,61 +126,6 @@ def mixture_decay_torch(n, dim, ncomp, sigma, seed, device="cuda", centres=None)
     return x, centres
 
 
-def kmeans_torch(x, k, iters, seed, chunk=65536):
-    """Lloyd k-means on the GPU (assignment by argmin of ||c||^2 - 2 x.c)."""
-    import torch
-    g = torch_generator(seed, x.device)
-    n = x.shape[0]
-    perm = torch.randperm(n, generator=g, device=x.device)[:k]
-    c = x[perm].clone()
-    assign = torch.empty(n, dtype=torch.int64, device=x.device)
-
-    def assign_all():
-        cn = (c * c).sum(1)
-        for s in range(0, n, chunk):
-            xs = x[s:s + chunk]
-            assign[s:s + chunk] = (cn[None, :] - 2.0 * (xs @ c.T)).argmin(1)
-
-    for _ in range(iters):
-        assign_all()
-        sums = torch.zeros_like(c, dtype=torch.float64)
-        sums.index_add_(0, assign, x.double())
-        cnt = torch.bincount(assign, minlength=k)
-        nz = cnt > 0
-        c[nz] = (sums[nz] / cnt[nz, None]).float()
-    assign_all()
-    return c, assign
-
-
-def exact_knn_torch(base, queries, k, chunk_q=1024, chunk_n=262144):
-    """Exact L2 top-k in FP32 on the GPU (ground truth), ids ascending by distance."""
-    import torch
-    nq = queries.shape[0]
-    bn = (base * base).sum(1)
-    out_i = torch.empty(nq, k, dtype=torch.int64, device=base.device)
-    out_d = torch.empty(nq, k, dtype=torch.float32, device=base.device)
-    for s in range(0, nq, chunk_q):
-        q = queries[s:s + chunk_q]
-        qn = (q * q).sum(1, keepdim=True)
-        best_d = None
-        best_i = None
-        for t in range(0, base.shape[0], chunk_n):
-            d = qn + bn[None, t:t + chunk_n] - 2.0 * (q @ base[t:t + chunk_n].T)
-            dk, ik = torch.topk(d, min(k, d.shape[1]), dim=1, largest=False)
-            ik = ik + t
-            if best_d is None:
-                best_d, best_i = dk, ik
-            else:
-                cd = torch.cat([best_d, dk], 1)
-                ci = torch.cat([best_i, ik], 1)
-                best_d, sel = torch.topk(cd, k, dim=1, largest=False)
-                best_i = torch.gather(ci, 1, sel)
-        order = torch.argsort(best_d, dim=1)
-        out_d[s:s + chunk_q] = torch.gather(best_d, 1, order)
-        out_i[s:s + chunk_q] = torch.gather(best_i, 1, order)
-    return out_i, out_d
-
-
 def exact_knn_np(base, queries, k):
     """Exact L2 top-k in FP64 on the CPU (spot checks)."""
     b = base.astype(np.float64)

@@ -237,3 +237,74 @@ class IVF:
         check(_lib.lib().erq_search_dev(self._h, _ptr(q_rot), nq, k, nprobe, _ptr(ids), _ptr(d),
                                         _ptr(stats), _stream()))
         return ids, d
+
+
+# ---------------------------------------------------------------- dataset preparation
+# python/ivf.py and python/compute_gt.py on liberq.so's kernels (erq_knn.hip); inputs are
+# torch CUDA float32 tensors of shape [rows, dim].
+
+def _rows_dev(x):
+    import torch
+    if not (torch.is_tensor(x) and x.is_cuda and x.dtype == torch.float32 and x.dim() == 2):
+        raise TypeError("expected a 2-D float32 CUDA tensor")
+    return x.contiguous()
+
+
+def knn_dev(base, queries, k):
+    """(ids int32 [nq, k], squared distances float32 [nq, k]): exact k nearest base rows of
+    each query, ordered by (distance, id) (erq_knn_dev)."""
+    import torch
+    b, q = _rows_dev(base), _rows_dev(queries)
+    if b.shape[1] != q.shape[1]:
+        raise ValueError("dimension mismatch")
+    ids = torch.empty((q.shape[0], k), dtype=torch.int32, device=q.device)
+    d = torch.empty((q.shape[0], k), dtype=torch.float32, device=q.device)
+    check(_lib.lib().erq_knn_dev(_ptr(b), b.shape[0], _ptr(q), q.shape[0], b.shape[1], k,
+                                 _ptr(ids), _ptr(d), _stream()))
+    return ids, d
+
+
+def kmeans_assign_dev(x, centroids):
+    """(assign int32 [n], squared distance float32 [n]): L2Sqr16-nearest centroid, ties by id."""
+    import torch
+    x, c = _rows_dev(x), _rows_dev(centroids)
+    a = torch.empty(x.shape[0], dtype=torch.int32, device=x.device)
+    d = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+    check(_lib.lib().erq_kmeans_assign_dev(_ptr(x), x.shape[0], x.shape[1], _ptr(c), c.shape[0],
+                                           _ptr(a), _ptr(d), _stream()))
+    return a, d
+
+
+def kmeans_update_dev(x, assign, centroids):
+    """One Lloyd update of `centroids` in place; returns the member counts (int32 [K])."""
+    import torch
+    x = _rows_dev(x)
+    if not centroids.is_contiguous():
+        raise ValueError("centroids must be contiguous")
+    a = assign.to(torch.int32).contiguous()
+    cnt = torch.empty(centroids.shape[0], dtype=torch.int32, device=x.device)
+    check(_lib.lib().erq_kmeans_update_dev(_ptr(x), x.shape[0], x.shape[1], _ptr(a),
+                                           centroids.shape[0], _ptr(centroids), _ptr(cnt),
+                                           _stream()))
+    return cnt
+
+
+def kmeans_dev(x, K, iters, init_rows):
+    """(centroids [K, dim], assign int32 [n], squared distance [n]) after `iters` Lloyd steps
+    from the rows init_rows (erq_kmeans_dev)."""
+    import torch
+    x = _rows_dev(x)
+    init = torch.as_tensor(np.ascontiguousarray(init_rows, np.int32)).to(x.device)
+    if init.numel() != K:
+        raise ValueError("init_rows must hold K row indices")
+    c = torch.empty((K, x.shape[1]), dtype=torch.float32, device=x.device)
+    a = torch.empty(x.shape[0], dtype=torch.int32, device=x.device)
+    d = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+    check(_lib.lib().erq_kmeans_dev(_ptr(x), x.shape[0], x.shape[1], K, iters, _ptr(init),
+                                    _ptr(c), _ptr(a), _ptr(d), _stream()))
+    return c, a, d
+
+
+def kmeans_init_rows(n, K, seed):
+    """K distinct row indices drawn by a seeded generator (the initial centroids)."""
+    return np.sort(np.random.default_rng(seed).choice(n, K, replace=False)).astype(np.int32)

@@ -1,55 +1,47 @@
-"""Dataset preparation of the reference's python/ scripts, on torch (GPU when one is there).
+"""Dataset preparation of the reference's python/ scripts, on liberq.so's HIP kernels.
 
 ivf_cluster      python/ivf.py:8-35 (faiss IVF{K},Flat training + nearest-centroid search):
-                 Lloyd k-means here (erq.data.kmeans_torch), then each vector's nearest
-                 centroid and the distance to it.  Clusters differ from faiss's (another
-                 trainer, another seed); the files have the same meaning and format.
+                 Lloyd k-means on the GPU (erq_kmeans_dev: L2Sqr16-nearest assignment, ties
+                 by id; cluster means summed in double in row order), then each vector's
+                 nearest centroid and the distance to it.  Clusters differ from faiss's
+                 (another trainer, another seed); the files have the same meaning and format.
 compute_gt       python/compute_gt.py:6-18: the ids of each query's k nearest base vectors
-                 by L2, nearest first.
+                 by L2, nearest first (erq_knn_dev: distances in double, ties by id).
 The *_files variants read and write the dataset directory as the scripts and
 src/create_index.cpp / src/test_search.cpp name its files:
   {name}_base.fvecs, {name}_query.fvecs, {name}_groundtruth.ivecs,
   {name}_centroid_{K}.fvecs, {name}_cluster_id_{K}.ivecs, {name}_dist_to_centroid_{K}.fvecs.
-This is setup work, not the search path: it runs on whatever device torch offers.
+Both need the GPU: there is no CPU fallback.  torch holds the device buffers.
 """
 import os
 
 import numpy as np
 
-from . import data, vecio
+from . import gpu, vecio
 
 
-def _device(device):
+def _dev(x):
     import torch
-    if device is None:
-        device = "cuda" if torch.cuda.is_available() else "cpu"
-    return torch.device(device)
+    if not torch.cuda.is_available():
+        raise RuntimeError("dataset preparation runs on the GPU (liberq.so); none is visible")
+    return torch.as_tensor(np.ascontiguousarray(x, np.float32)).cuda()
 
 
-def ivf_cluster(x, K, iters=20, seed=0, device=None, chunk=65536):
+def ivf_cluster(x, K, iters=20, seed=0):
     """(centroids [K, d] float32, cluster ids [N] int32, distance to the centroid [N] float32)."""
-    import torch
-    dev = _device(device)
-    xt = torch.as_tensor(np.ascontiguousarray(x, np.float32)).to(dev)
+    xt = _dev(x)
     if K > xt.shape[0]:
         raise ValueError("more clusters than vectors")
-    c, assign = data.kmeans_torch(xt, K, iters, seed, chunk=chunk)
-    dist = torch.empty(xt.shape[0], dtype=torch.float32, device=dev)
-    for s in range(0, xt.shape[0], chunk):
-        diff = xt[s:s + chunk] - c[assign[s:s + chunk]]
-        dist[s:s + chunk] = diff.norm(dim=1)
-    return (c.cpu().numpy().astype(np.float32), assign.cpu().numpy().astype(np.int32),
-            dist.cpu().numpy())
+    c, assign, d2 = gpu.kmeans_dev(xt, K, iters, gpu.kmeans_init_rows(xt.shape[0], K, seed))
+    return (c.cpu().numpy(), assign.cpu().numpy().astype(np.int32),
+            np.sqrt(d2.cpu().numpy()).astype(np.float32))
 
 
-def compute_gt(base, queries, k=1000, device=None):
-    """[nq, k] int32 ids of the exact L2 nearest neighbours, nearest first (FP32)."""
-    import torch
-    dev = _device(device)
-    b = torch.as_tensor(np.ascontiguousarray(base, np.float32)).to(dev)
-    q = torch.as_tensor(np.ascontiguousarray(queries, np.float32)).to(dev)
+def compute_gt(base, queries, k=1000):
+    """[nq, k] int32 ids of the exact L2 nearest neighbours, nearest first."""
+    b, q = _dev(base), _dev(queries)
     k = min(k, b.shape[0])
-    ids, _ = data.exact_knn_torch(b, q, k)
+    ids, _ = gpu.knn_dev(b, q, k)
     return ids.cpu().numpy().astype(np.int32)
 
 

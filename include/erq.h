@@ -266,6 +266,41 @@ int erq_timing_end(erq_index* h, double* ms_per_stage, uint64_t* count_per_stage
  * 8 first vector per cluster (u32). */
 const void* erq_device_array(const erq_index* h, int which);
 
+/* ---- dataset preparation (device pointers; row-major float rows of `dim` values; each
+ * call allocates its temporaries and returns after `stream` has drained, except
+ * erq_kmeans_update_dev, which only enqueues).  Rows are taken as zero-padded to a multiple
+ * of 16 dims, dim <= 8192, row counts < 2^31. */
+
+/* Ground truth, python/compute_gt.py:12-15 (numpy: argsort of the L2 norms, first k): per
+ * query, the k base rows with the smallest squared L2 distance -- evaluated in double as 16
+ * lane-strided fma chains over the exact float differences, reduced in
+ * _mm512_reduce_add_ps order -- ordered by (distance, id).  ids: nq x k; dists (nullable):
+ * nq x k, the distances rounded to float.  k <= min(n, 4096).  ERQ_EUNSUPPORTED if more
+ * than 65536 base rows lie within the FP32 GEMM's error bound of one query's k-th
+ * distance. */
+int erq_knn_dev(const float* base, uint64_t n, const float* queries, uint64_t nq, uint32_t dim,
+                uint32_t k, uint32_t* ids, float* dists, void* stream);
+
+/* Nearest centroid of each row, the index.quantizer.search(X, 1) of python/ivf.py:31: the
+ * centroid with the smallest L2Sqr16 (space.hpp:18-39, the search-time ranking of
+ * Initializer.hpp:55-68), ties by id; dist = that squared distance (ivf.py takes its square
+ * root for dist_to_centroid). */
+int erq_kmeans_assign_dev(const float* x, uint64_t n, uint32_t dim, const float* centroids,
+                          uint32_t K, uint32_t* assign, float* dist, void* stream);
+
+/* One Lloyd update: each cluster with members gets their mean (summed in double in
+ * increasing row order, rounded to float); a cluster without members keeps its centroid.
+ * counts (nullable): K member counts. */
+int erq_kmeans_update_dev(const float* x, uint64_t n, uint32_t dim, const uint32_t* assign,
+                          uint32_t K, float* centroids, uint32_t* counts, void* stream);
+
+/* IVF training, python/ivf.py:26-35 (faiss IVF{K},Flat train, then quantizer.search):
+ * centroids start as the rows init_rows[0..K), then `iters` rounds of assignment + update,
+ * then a final assignment into assign / dist. */
+int erq_kmeans_dev(const float* x, uint64_t n, uint32_t dim, uint32_t K, uint32_t iters,
+                   const uint32_t* init_rows, float* centroids, uint32_t* assign, float* dist,
+                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -639,3 +639,113 @@ void orc_pack_block_codes(const uint8_t* bits, size_t n, size_t D, uint8_t* bloc
         }
     }
 }
+
+/* ---------------------------------------------------------------- dataset preparation
+ * The reference trains its IVF with faiss (python/ivf.py:26-35: IVF{K},Flat, then each
+ * vector's nearest centroid) and its ground truth with numpy (python/compute_gt.py:12-15).
+ * This project's trainer, restated: Lloyd k-means whose assignment is the L2Sqr16-nearest
+ * centroid (ties by id, the ranking FlatInitializer applies at search time) and whose
+ * update is each cluster's mean, summed in double in increasing vector id order and
+ * rounded to float; an empty cluster keeps its centroid.  Rows shorter than a multiple of
+ * 16 are taken as zero-padded (a zero pair adds nothing to a lane's chain). */
+static float l2sqr16_any(const float* x, const float* y, size_t D) {
+    float p[16] = {0};
+    for (size_t i = 0; i < D; i += 16)
+        for (int l = 0; l < 16 && i + l < D; l++) {
+            float t = x[i + l] - y[i + l];
+            p[l] = fmaf(t, t, p[l]);
+        }
+    return orc_reduce16(p);
+}
+
+void orc_kmeans_assign(const float* x, size_t n, size_t D, const float* c, size_t K,
+                       uint32_t* assign, float* dist) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; i++) {
+        uint32_t best = 0;
+        float bd = l2sqr16_any(x + (size_t)i * D, c, D);
+        for (size_t j = 1; j < K; j++) {
+            float d = l2sqr16_any(x + (size_t)i * D, c + j * D, D);
+            if (d < bd) {
+                bd = d;
+                best = (uint32_t)j;
+            }
+        }
+        assign[i] = best;
+        dist[i] = bd;
+    }
+}
+
+void orc_kmeans_update(const float* x, size_t n, size_t D, const uint32_t* assign, size_t K,
+                       float* c, uint32_t* counts) {
+    double* acc = (double*)calloc(K * D, sizeof(double));
+    memset(counts, 0, K * sizeof(uint32_t));
+    for (size_t i = 0; i < n; i++) {
+        double* a = acc + (size_t)assign[i] * D;
+        for (size_t d = 0; d < D; d++) a[d] = a[d] + (double)x[i * D + d];
+        counts[assign[i]]++;
+    }
+    for (size_t j = 0; j < K; j++)
+        if (counts[j])
+            for (size_t d = 0; d < D; d++) c[j * D + d] = (float)(acc[j * D + d] / (double)counts[j]);
+    free(acc);
+}
+
+void orc_kmeans(const float* x, size_t n, size_t D, size_t K, size_t iters,
+                const uint32_t* init_rows, float* c, uint32_t* assign, float* dist) {
+    uint32_t* counts = (uint32_t*)malloc(K * sizeof(uint32_t));
+    for (size_t j = 0; j < K; j++) memcpy(c + j * D, x + (size_t)init_rows[j] * D, D * sizeof(float));
+    for (size_t it = 0; it < iters; it++) {
+        orc_kmeans_assign(x, n, D, c, K, assign, dist);
+        orc_kmeans_update(x, n, D, assign, K, c, counts);
+    }
+    orc_kmeans_assign(x, n, D, c, K, assign, dist);
+    free(counts);
+}
+
+/* The exact squared L2 distance of the ground truth: d_i = (double)x_i - (double)y_i (exact),
+ * 16 lane-strided chains of fma(d, d, p) in double, reduced in _mm512_reduce_add_ps order. */
+double orc_l2sqr16_f64(const float* x, const float* y, size_t D) {
+    double p[16] = {0};
+    for (size_t i = 0; i < D; i += 16)
+        for (int l = 0; l < 16 && i + l < D; l++) {
+            double t = (double)x[i + l] - (double)y[i + l];
+            p[l] = fma(t, t, p[l]);
+        }
+    double t3[8], t6[4];
+    for (int i = 0; i < 8; i++) t3[i] = p[i + 8] + p[i];
+    for (int j = 0; j < 4; j++) t6[j] = t3[j + 4] + t3[j];
+    return (t6[0] + t6[2]) + (t6[1] + t6[3]);
+}
+
+typedef struct {
+    uint32_t id;
+    double d;
+} cand64;
+
+static int cand64_cmp(const void* a, const void* b) {
+    const cand64 *x = (const cand64*)a, *y = (const cand64*)b;
+    if (x->d < y->d) return -1;
+    if (x->d > y->d) return 1;
+    return (x->id > y->id) - (x->id < y->id);
+}
+
+/* python/compute_gt.py:12-15: per query the k nearest base vectors by orc_l2sqr16_f64,
+ * ordered by (distance, id); dists (nullable) the distances rounded to float. */
+void orc_knn(const float* base, size_t n, const float* q, size_t nq, size_t D, size_t k,
+             uint32_t* ids, float* dists) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < (long)nq; i++) {
+        cand64* c = (cand64*)malloc(n * sizeof(cand64));
+        for (size_t j = 0; j < n; j++) {
+            c[j].id = (uint32_t)j;
+            c[j].d = orc_l2sqr16_f64(q + (size_t)i * D, base + j * D, D);
+        }
+        qsort(c, n, sizeof(cand64), cand64_cmp);
+        for (size_t j = 0; j < k; j++) {
+            ids[(size_t)i * k + j] = j < n ? c[j].id : 0xFFFFFFFFu;
+            if (dists) dists[(size_t)i * k + j] = j < n ? (float)c[j].d : FLT_MAX;
+        }
+        free(c);
+    }
+}

@@ -99,6 +99,18 @@ void orc_pair_estimates(const orc_index* idx, const float* q_rot, size_t cid, fl
 void orc_pack_block_codes(const uint8_t* bits, size_t n, size_t D, uint8_t* block_codes);
 int orc_ex_encode(const uint8_t* raw, size_t D, int ex_bits, uint8_t* out);
 
+/* Dataset preparation (python/ivf.py:26-35, python/compute_gt.py:12-15) as this project
+ * does it; see erq_oracle.c.  Row-major n x D floats (D any: zero padding implied). */
+void orc_kmeans_assign(const float* x, size_t n, size_t D, const float* c, size_t K,
+                       uint32_t* assign, float* dist);
+void orc_kmeans_update(const float* x, size_t n, size_t D, const uint32_t* assign, size_t K,
+                       float* c, uint32_t* counts);
+void orc_kmeans(const float* x, size_t n, size_t D, size_t K, size_t iters,
+                const uint32_t* init_rows, float* c, uint32_t* assign, float* dist);
+double orc_l2sqr16_f64(const float* x, const float* y, size_t D);
+void orc_knn(const float* base, size_t n, const float* q, size_t nq, size_t D, size_t k,
+             uint32_t* ids, float* dists);
+
 #ifdef __cplusplus
 }
 #endif

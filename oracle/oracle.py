@@ -81,6 +81,16 @@ def lib():
                                          c_vp, c_vp]
         L.orc_pack_block_codes.restype = None
         L.orc_pack_block_codes.argtypes = [c_vp, c_size, c_size, c_vp]
+        L.orc_kmeans_assign.restype = None
+        L.orc_kmeans_assign.argtypes = [c_vp, c_size, c_size, c_vp, c_size, c_vp, c_vp]
+        L.orc_kmeans_update.restype = None
+        L.orc_kmeans_update.argtypes = [c_vp, c_size, c_size, c_vp, c_size, c_vp, c_vp]
+        L.orc_kmeans.restype = None
+        L.orc_kmeans.argtypes = [c_vp, c_size, c_size, c_size, c_size, c_vp, c_vp, c_vp, c_vp]
+        L.orc_l2sqr16_f64.restype = ctypes.c_double
+        L.orc_l2sqr16_f64.argtypes = [c_vp, c_vp, c_size]
+        L.orc_knn.restype = None
+        L.orc_knn.argtypes = [c_vp, c_size, c_vp, c_size, c_size, c_size, c_vp, c_vp]
         _lib = L
     return _lib
 
@@ -291,3 +301,54 @@ class OracleIndex:
         lib().orc_centroid_topn(_p(q), _p(self.keep["C"]), self.idx.K, self.idx.D, nprobe,
                                 _p(ids), _p(d))
         return ids, d
+
+
+# ---------------------------------------------------------------- dataset preparation
+def _rows(x):
+    x = np.ascontiguousarray(x, np.float32)
+    return x.reshape(x.shape[0], -1)
+
+
+def kmeans_assign(x, c):
+    """(assign uint32 [n], squared L2Sqr16 distance float32 [n]): nearest centroid, ties by id."""
+    x, c = _rows(x), _rows(c)
+    a = np.empty(x.shape[0], np.uint32)
+    d = np.empty(x.shape[0], np.float32)
+    lib().orc_kmeans_assign(_p(x), x.shape[0], x.shape[1], _p(c), c.shape[0], _p(a), _p(d))
+    return a, d
+
+
+def kmeans_update(x, assign, c):
+    """(new centroids, counts): each non-empty cluster's mean (double sums in id order)."""
+    x = _rows(x)
+    c = np.array(c, np.float32, copy=True)
+    a = np.ascontiguousarray(assign, np.uint32)
+    cnt = np.empty(c.shape[0], np.uint32)
+    lib().orc_kmeans_update(_p(x), x.shape[0], x.shape[1], _p(a), c.shape[0], _p(c), _p(cnt))
+    return c, cnt
+
+
+def kmeans(x, K, iters, init_rows):
+    """(centroids [K, D], assign [n], squared distance [n]) after `iters` Lloyd steps."""
+    x = _rows(x)
+    init = np.ascontiguousarray(init_rows, np.uint32)
+    c = np.empty((K, x.shape[1]), np.float32)
+    a = np.empty(x.shape[0], np.uint32)
+    d = np.empty(x.shape[0], np.float32)
+    lib().orc_kmeans(_p(x), x.shape[0], x.shape[1], K, iters, _p(init), _p(c), _p(a), _p(d))
+    return c, a, d
+
+
+def l2sqr16_f64(x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return float(lib().orc_l2sqr16_f64(_p(x), _p(y), x.size))
+
+
+def knn(base, queries, k):
+    """(ids uint32 [nq, k], distances float32 [nq, k]) by orc_l2sqr16_f64, ties by id."""
+    b, q = _rows(base), _rows(queries)
+    ids = np.empty((q.shape[0], k), np.uint32)
+    d = np.empty((q.shape[0], k), np.float32)
+    lib().orc_knn(_p(b), b.shape[0], _p(q), q.shape[0], b.shape[1], k, _p(ids), _p(d))
+    return ids, d
