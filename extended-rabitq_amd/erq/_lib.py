@@ -78,6 +78,13 @@ SIGNATURES = {
                                       ctypes.c_uint32, c_vp, c_vp, c_vp]),
     "erq_kmeans_dev": (c_int, [c_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.c_uint32, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "erq_build_quantize_dev": (c_int, [c_vp, c_vp, ctypes.c_uint64, ctypes.c_uint32,
+                                       ctypes.c_uint32, c_vp, c_vp, c_vp]),
+    "erq_build_pack_dev": (c_int, [c_vp, c_vp, c_vp, ctypes.c_uint32, ctypes.c_uint32, c_vp,
+                                   c_vp]),
+    "erq_build_index_dev": (c_int, [c_vp, ctypes.c_uint64, ctypes.c_uint32, c_vp,
+                                    ctypes.c_uint32, c_vp, c_vp, ctypes.c_uint32, c_vp, c_vp,
+                                    c_vp, c_vp, c_vp, c_vp, c_vp]),
 }
 
 BUILDER_SIGNATURES = {

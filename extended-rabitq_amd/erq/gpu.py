@@ -308,3 +308,69 @@ def kmeans_dev(x, K, iters, init_rows):
 def kmeans_init_rows(n, K, seed):
     """K distinct row indices drawn by a seeded generator (the initial centroids)."""
     return np.sort(np.random.default_rng(seed).choice(n, K, replace=False)).astype(np.int32)
+
+
+# ---------------------------------------------------------------- index building
+# IVF::construct on liberq.so's kernels (erq_build.hip).
+
+def build_quantize_dev(resid, onorm, ex_bits):
+    """(long_code uint8 [n, D*EX/8], xipnorm float32 [n]) of rotated residuals [n, D] and
+    their norms (erq_build_quantize_dev)."""
+    import torch
+    r = _rows_dev(resid)
+    on = onorm.contiguous()
+    n, D = r.shape
+    lc = torch.empty((n, D * ex_bits // 8), dtype=torch.uint8, device=r.device)
+    xip = torch.empty(n, dtype=torch.float32, device=r.device)
+    check(_lib.lib().erq_build_quantize_dev(_ptr(r), _ptr(on), n, D, ex_bits, _ptr(lc),
+                                            _ptr(xip), _stream()))
+    return lc, xip
+
+
+def build_pack_dev(resid, onorm, cluster_sizes):
+    """Short blocks (uint8) of residuals in cluster order (erq_build_pack_dev)."""
+    import torch
+    r = _rows_dev(resid)
+    on = onorm.contiguous()
+    sizes = np.ascontiguousarray(cluster_sizes, np.uint64)
+    D = r.shape[1]
+    nb = int(((sizes + 31) // 32).sum())
+    out = torch.empty(nb * (4 * D + 128), dtype=torch.uint8, device=r.device)
+    check(_lib.lib().erq_build_pack_dev(_ptr(r), _ptr(on), _ptr(sizes), sizes.size, D,
+                                        _ptr(out), _stream()))
+    return out
+
+
+def build_index_dev(data, centroids, assign, bits, P):
+    """IVF::construct on the GPU (erq_build_index_dev): raw vectors, un-rotated centroids and
+    cluster ids (torch CUDA tensors) and the rotation P (numpy [D, D]) -> an IVFIndex with
+    host arrays, EX = bits - 1."""
+    import torch
+    from .index import IVFIndex, padded_dim
+    x, c = _rows_dev(data), _rows_dev(centroids)
+    N, DIM = x.shape
+    K = c.shape[0]
+    EX = bits - 1
+    D = padded_dim(DIM)
+    a = assign.to(torch.int32).contiguous()
+    if a.numel() != N:
+        raise ValueError("one cluster id per vector")
+    sizes = np.bincount(a.cpu().numpy(), minlength=K).astype(np.uint64)
+    if sizes.size != K:
+        raise ValueError("cluster id out of range")
+    Pd = torch.as_tensor(np.ascontiguousarray(P, np.float32)).to(x.device)
+    nb = int(((sizes + 31) // 32).sum())
+    crot = torch.empty((K, D), dtype=torch.float32, device=x.device)
+    short = torch.empty(nb * (4 * D + 128), dtype=torch.uint8, device=x.device)
+    lc = torch.empty(N * D * EX // 8, dtype=torch.uint8, device=x.device)
+    xip = torch.empty(N, dtype=torch.float32, device=x.device)
+    ids = torch.empty(N, dtype=torch.int32, device=x.device)
+    got = np.zeros(K, np.uint64)
+    check(_lib.lib().erq_build_index_dev(_ptr(x), N, DIM, _ptr(c), K, _ptr(a), _ptr(Pd), EX,
+                                         _ptr(got), _ptr(crot), _ptr(short), _ptr(lc), _ptr(xip),
+                                         _ptr(ids), _stream()))
+    if not np.array_equal(got, sizes):
+        raise RuntimeError("cluster sizes disagree")
+    return IVFIndex(N, DIM, K, EX, got, np.ascontiguousarray(P, np.float32), crot.cpu().numpy(),
+                    short.cpu().numpy(), lc.cpu().numpy(), xip.cpu().numpy(),
+                    ids.cpu().numpy().view(np.uint32))

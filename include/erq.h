@@ -301,6 +301,35 @@ int erq_kmeans_dev(const float* x, uint64_t n, uint32_t dim, uint32_t K, uint32_
                    const uint32_t* init_rows, float* centroids, uint32_t* assign, float* dist,
                    void* stream);
 
+/* ---- index building (device pointers; each call returns after `stream` has drained) */
+
+/* DataQuantizer::exrabitq_codes (Quantizer.hpp:457-489) with fast_quantize (:138-219) and
+ * store_compacted_code (:491-631) for n rotated residuals resid[n x D] (D a multiple of 64,
+ * <= 8192) and their norms ||o - c|| onorm[n]: the compacted EX-bit codes long_code[n x
+ * D*EX/8] and xipnorm ex_factor[n], byte for byte what the host builder (builder.cpp)
+ * writes from the same inputs.  ERQ_EINVAL on a non-finite input. */
+int erq_build_quantize_dev(const float* resid, const float* onorm, uint64_t n, uint32_t D,
+                           uint32_t ex_bits, uint8_t* long_code, float* ex_factor, void* stream);
+
+/* The short blocks (Quantizer.hpp:277-290, pack_codes.hpp:33-98): per cluster (sizes in
+ * host memory; resid/onorm in cluster order) ceil(n_k / 32) blocks of 4D code bytes -- the
+ * sign bits of resid -- and 32 float norms; unused slots of a tail block are zero. */
+int erq_build_pack_dev(const float* resid, const float* onorm, const uint64_t* cluster_sizes,
+                       uint32_t K, uint32_t D, uint8_t* short_data, void* stream);
+
+/* IVF::construct (IVF.hpp:116-153) on the GPU from raw vectors data[N x DIM], their
+ * centroids[K x DIM] (un-rotated), cluster ids assign[N] and the rotation P[D x D]
+ * (D = DIM rounded up to 64): cluster_sizes[K] (host), rotated centroids [K x D], ids[N]
+ * (cluster order, ascending within a cluster), short_data (sum ceil(n_k/32) (4D + 128)
+ * bytes, sized by the caller from the cluster sizes, e.g. a bincount of assign), long_code
+ * [N x D*EX/8] and ex_factor[N].  Rotations are one FP32 GEMM each; residual = rotated
+ * vector - rotated centroid; ||o - c|| = f32(sqrt((double)L2Sqr(o, c, DIM))) as
+ * Quantizer.hpp:399-401 computes it; then erq_build_quantize_dev and erq_build_pack_dev. */
+int erq_build_index_dev(const float* data, uint64_t N, uint32_t DIM, const float* centroids,
+                        uint32_t K, const uint32_t* assign, const float* P, uint32_t ex_bits,
+                        uint64_t* cluster_sizes, float* rotated_centroids, uint8_t* short_data,
+                        uint8_t* long_code, float* ex_factor, uint32_t* ids, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
