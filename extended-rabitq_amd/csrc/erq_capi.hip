@@ -9,7 +9,9 @@
 // work when the two run on different streams (an event recorded when each call returns), so
 // the workspace and the other shared buffers are never in use by two calls at once.  No
 // exception crosses the ABI: bodies run through guarded().
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -164,7 +166,7 @@ void release(erq_index* h) {
     void* ptrs[] = {h->short_data, h->long_code, h->ex_factor, h->ids,    h->centroids,
                     h->cnorm,      h->P,         h->csize,     h->bstart, h->vstart,
                     h->ws,         h->bp,        h->order,     h->exl,    h->hq,
-                    h->hids,       h->hdists,    h->hnp};
+                    h->hids,       h->hdists,    h->hnp,       h->lat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->aux) {
@@ -552,7 +554,7 @@ static size_t max_cluster_size(const erq_index* h) {
 static ScanArgs scan_args(const erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                           const uint32_t* off, const uint32_t* pid, const float* pd,
                           uint32_t* ids, float* dists, uint64_t* stats, int max_probes) {
-    ScanArgs a;
+    ScanArgs a{};
     a.q_rot = q_rot_dev;
     a.probe_off = off;
     a.probe_ids = pid;
@@ -591,10 +593,55 @@ static ScanArgs scan_args(const erq_index* h, const float* q_rot_dev, size_t nq,
 // given by the caller instead of the handle's; with it nothing is allocated, and the
 // bit-plane kernel runs only if its copies of the codes already exist (else the LUT kernel,
 // with identical results).
+// The low-latency scan of a small batch (launch_scan_latency): every vector of every probed
+// cluster estimated in parallel, then one wave per query replays the probes in order.  Only
+// for probe lists without repeated clusters (those of the library's own probe selection):
+// a query then has at most nblocks * 32 slots.  The slot buffers are per handle, grow-only.
+// Slot and pair bounds of a batch whose queries probe at most max_np clusters each.
+static size_t latency_bytes(erq_index* h, size_t nq, size_t max_np, size_t* slots_out) {
+    if (h->top_pad.empty()) {  // sums of the i largest cluster sizes rounded up to 32
+        std::vector<uint64_t> v(h->sizes.size());
+        for (size_t i = 0; i < v.size(); i++) v[i] = (h->sizes[i] + 31) / 32 * 32;
+        std::sort(v.begin(), v.end(), std::greater<uint64_t>());
+        h->top_pad.assign(v.size() + 1, 0);
+        for (size_t i = 0; i < v.size(); i++) h->top_pad[i + 1] = h->top_pad[i] + v[i];
+    }
+    const size_t slots = nq * h->top_pad[std::min(max_np, h->top_pad.size() - 1)];
+    if (slots_out) *slots_out = slots;
+    return slots * 12 + (nq * max_np + 1) * 8 + 256;  // slots; pair offsets and first items
+}
+
+static int scan_latency_locked(erq_index* h, ScanArgs& a, size_t nq, size_t max_np,
+                               hipStream_t s) {
+    size_t slots = 0;
+    const size_t need = latency_bytes(h, nq, max_np, &slots);
+    if (need > h->lat_cap) {
+        if (h->lat) {
+            hipError_t e = hipDeviceSynchronize();  // earlier calls may still read it
+            if (e != hipSuccess) return hip_fail(e, "latency buffer sync");
+            (void)hipFree(h->lat);
+            h->lat = nullptr;
+            h->lat_cap = 0;
+        }
+        hipError_t e = hipMalloc(&h->lat, need);
+        if (e != hipSuccess) return hip_fail(e, "latency buffer");
+        h->lat_cap = need;
+    }
+    char* p = static_cast<char*>(h->lat);
+    a.est_lower = reinterpret_cast<float*>(p);
+    a.est_exd = a.est_lower + slots;
+    a.est_id = reinterpret_cast<uint32_t*>(a.est_exd + slots);
+    a.est_off = a.est_id + slots;
+    StageTimer tim(h, ERQ_STAGE_REPLAY, s);
+    hipError_t e = launch_scan_latency(a, s);
+    if (e != hipSuccess) return hip_fail(e, "latency scan");
+    return ERQ_OK;
+}
+
 static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                        const uint32_t* off, const uint32_t* pid, const float* pd, uint32_t* ids,
                        float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0,
-                       uint32_t* order_buf = nullptr) {
+                       uint32_t* order_buf = nullptr, size_t own_lists = 0) {
     ScanArgs a = scan_args(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, max_probes);
     if (h->plain_kernel != 1) {  // auto or MFMA: bit-plane integer path
         if (!order_buf) {
@@ -606,6 +653,12 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
             a.exl = h->exl;
         }
     }
+    // own_lists > 0: the probe lists come from the library's selection (no repeated
+    // cluster), at most own_lists probes per query
+    if (own_lists > 0 && !order_buf && a.bp && a.nchunk <= 4 && h->latency_nq > 0 &&
+        nq <= (size_t)h->latency_nq &&
+        (!capturing(s) || latency_bytes(h, nq, own_lists, nullptr) <= h->lat_cap))
+        return scan_latency_locked(h, a, nq, own_lists, s);
     if (h->plain_order && nq > 1) {
         // queries by scan cost, heaviest first: the result of each query is unchanged
         const size_t need = nq + h->K + 1;
@@ -676,6 +729,10 @@ int erq_set_option(erq_index* h, int option, long value) {
             h->batch_overlap = (int)value;
             return ERQ_OK;
         }
+        if (option == ERQ_OPT_LATENCY_NQ && value >= 0 && value <= 4096) {
+            h->latency_nq = (int)value;
+            return ERQ_OK;
+        }
         set_error("erq_set_option: unknown option or value");
         return ERQ_EINVAL;
     });
@@ -691,6 +748,7 @@ long erq_get_option(const erq_index* h, int option) {
     case ERQ_OPT_PLAIN_ORDER: return h->plain_order;
     case ERQ_OPT_BATCH_PAIR_BUDGET: return (long)h->batch_pair_budget;
     case ERQ_OPT_BATCH_OVERLAP: return h->batch_overlap;
+    case ERQ_OPT_LATENCY_NQ: return h->latency_nq;
     default: return -1;
     }
 }
@@ -767,7 +825,8 @@ int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, si
                            nq, (uint32_t)nprobe);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "offsets");
-        return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s);
+        return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s, 0,
+                           nullptr, nprobe);
     });
 }
 
@@ -1347,7 +1406,7 @@ static int host_search(erq_index* h, const float* q_rot, size_t nq, size_t k, si
                                       h->hdists, nullptr, s, (char*)wsp,
                                       probe_part + pad_part + csr_part)
                      : scan_locked(h, h->hq, nq, k, h->hnp, cid, cd, h->hids, h->hdists, nullptr,
-                                   s);
+                                   s, 0, nullptr, max_np);
     }
     if (rc) return rc;
     e = hipMemcpyAsync(results, h->hids, nq * k * 4, hipMemcpyDeviceToHost, s);

@@ -139,6 +139,50 @@ __global__ void pad_rows_kernel(const float* __restrict__ src, int rows, int col
     }
 }
 
+// C = A B^T for at most kSkinnyRows rows of A (a few queries against the centroid rows): one
+// wave per column n, its lanes over k (16 bytes each), the rows' partial sums reduced across
+// the wave -- full occupancy where the tiled kernel would launch a handful of workgroups.
+// Any order of the FP32 sum keeps the error within the bound the probe selection assumes.
+constexpr int kSkinnyRows = 8;
+
+template <int EPI>
+__global__ void __launch_bounds__(256) skinny_kernel(const float* __restrict__ A, int lda,
+                                                     const float* __restrict__ B, int ldb,
+                                                     float* __restrict__ C, int ldc, int M, int N,
+                                                     int K, const float* __restrict__ rn,
+                                                     const float* __restrict__ cn) {
+    const int lane = threadIdx.x & 63;
+    const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (n >= N) return;  // wave-uniform
+    const float* b = B + (size_t)n * ldb;
+    float acc[kSkinnyRows];
+#pragma unroll
+    for (int m = 0; m < kSkinnyRows; m++) acc[m] = 0.f;
+    for (int k0 = 4 * lane; k0 < K; k0 += 256) {
+        const float4 bv = *reinterpret_cast<const float4*>(b + k0);
+#pragma unroll
+        for (int m = 0; m < kSkinnyRows; m++)
+            if (m < M) {
+                const float4 av = *reinterpret_cast<const float4*>(A + (size_t)m * lda + k0);
+                acc[m] = fmaf(av.x, bv.x, acc[m]);
+                acc[m] = fmaf(av.y, bv.y, acc[m]);
+                acc[m] = fmaf(av.z, bv.z, acc[m]);
+                acc[m] = fmaf(av.w, bv.w, acc[m]);
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < kSkinnyRows; m++)
+        for (int o = 32; o > 0; o >>= 1) acc[m] += __shfl_xor(acc[m], o);
+    if (lane < kSkinnyRows && lane < M) {
+        float v = acc[0];
+#pragma unroll
+        for (int m = 1; m < kSkinnyRows; m++)
+            if (lane == m) v = acc[m];
+        if (EPI == 1) v = fmaf(-2.0f, v, rn[lane] + cn[n]);
+        C[(size_t)lane * ldc + n] = v;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_gemm(const float* A, int lda, const float* B, int ldb, int b_trans, float* C,
@@ -147,6 +191,14 @@ hipError_t launch_gemm(const float* A, int lda, const float* B, int ldb, int b_t
     if (K % BK != 0 || lda % 4 != 0 || ldb % 4 != 0 || (!b_trans && N % 4 != 0))
         return hipErrorInvalidValue;
     if (M == 0 || N == 0) return hipSuccess;
+    if (b_trans && M <= kSkinnyRows) {
+        const dim3 g((N + 3) / 4), b(256);
+        if (epi)
+            hipLaunchKernelGGL(skinny_kernel<1>, g, b, 0, s, A, lda, B, ldb, C, ldc, M, N, K, rn, cn);
+        else
+            hipLaunchKernelGGL(skinny_kernel<0>, g, b, 0, s, A, lda, B, ldb, C, ldc, M, N, K, rn, cn);
+        return hipGetLastError();
+    }
     dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM), block(256);
     if (b_trans) {
         if (epi)

@@ -28,6 +28,11 @@ struct erq_index {
     uint32_t* csize = nullptr;      // K
     uint32_t* bstart = nullptr;     // K + 1
     uint32_t* vstart = nullptr;     // K + 1
+    // the low-latency scan's slot buffers (grow-only) and its batch-size limit
+    void* lat = nullptr;
+    size_t lat_cap = 0;
+    int latency_nq = 16;
+    std::vector<uint64_t> top_pad;  // sums of the i largest cluster sizes rounded up to 32
     // grow-only workspace
     void* ws = nullptr;
     size_t ws_bytes = 0;
@@ -123,10 +128,23 @@ struct ScanArgs {
     // estimate is at most that distance into the pool by (distance, id).
     const uint32_t* qlist;
     int r0, r1;
+    // Estimate mode (the low-latency search): waves take the (query, probe) pairs p <
+    // probe_off[nq] and write, for every vector of the probed cluster, its lower bound,
+    // re-estimated distance and id at slot est_off[p] + 32 b + j (block b, vector j; a tail
+    // block's unused slots get an infinite lower bound); the replay then walks each query's
+    // slots in order with the reference's pruning threshold.
+    float* est_lower;
+    float* est_exd;
+    uint32_t* est_id;
+    uint32_t* est_off;
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
 // The round-mode rescan (a.qlist set): a fixed grid of waves that take the listed queries.
 hipError_t launch_scan_round(const ScanArgs& a, hipStream_t s);
+// The low-latency scan (a.est_* set, a.bp set, nchunk <= 4): slot offsets, the per-vector
+// estimates of every (query, probe) pair in parallel, then one wave per query replays the
+// probes in order -- the same results as launch_scan.
+hipError_t launch_scan_latency(const ScanArgs& a, hipStream_t s);
 // Bytes of LDS the round-mode kernel needs per wave (D, k), and whether it fits.
 size_t scan_round_wave_bytes(int D, int k);
 // qorder[nq] = the queries by scan cost, heaviest first (cost: the 32-vector blocks of the

@@ -158,10 +158,12 @@ __device__ __forceinline__ void pool_insert_bulk(const WaveLds& w, int& psize, i
     if (lane < ns) {
         dt = nd[lane];
         it = nid[lane];
-        for (int u = 0; u < ns; u++) {
-            const float du = nd[u];
-            rt += (du < dt || (du == dt && u > lane)) ? 1 : 0;
-        }
+    }
+    for (int u = 0; u < ns; u++) {  // wave-uniform; the other lanes' values by shuffle
+        const float du = __shfl(dt, u);
+        rt += (du < dt || (du == dt && u > lane)) ? 1 : 0;
+    }
+    if (lane < ns) {
         int lo = 0, hi = psize;  // pool entries with pd < dt
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
@@ -517,7 +519,8 @@ __device__ __forceinline__ void pool_merge_did(const WaveLds& w, int& psize, int
     psize = min(k, psize + ns);
 }
 
-enum { kModeReplay = 0, kModeRound = 1 };
+enum { kModeReplay = 0, kModeRound = 1, kModeEstimate = 2 };
+constexpr int kEstBlocks = 2;  // blocks per work item of the estimate mode
 // 256-dim slices of q and c in flight while staging a pair's residual (fewer above 3 chunks,
 // where the code registers leave no room for them)
 template <int NCH> constexpr int kReplaySlices = NCH <= 3 ? 3 : 1;
@@ -533,8 +536,18 @@ __host__ __device__ inline size_t mfma_wave_bytes(int D, int nchunk, int k) {
 
 template <int EX, int NCH, int MODE>
 __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsigned char* base,
-                                                int lane) {
+                                                int lane, int blo = 0, int bhi = 0x7FFFFFFF) {
     const int D = a.D, k = a.k;
+    // estimate mode: qi is a (query, probe) pair; find its query
+    const int pair = qi;
+    if (MODE == kModeEstimate) {
+        int lo = 0, hi = a.nq - 1;  // the last query whose probes start at or before pair
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.probe_off[mid] <= (uint32_t)pair) lo = mid; else hi = mid - 1;
+        }
+        qi = lo;
+    }
     constexpr int nchunk = NCH;
     constexpr bool kWrap = NCH > 4;  // D > 2048
     const int kcap = (k + 63) & ~63;
@@ -567,7 +580,7 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
             w.pid[i] = 0;
         }
         wave_sync();
-    } else {
+    } else if (MODE == kModeRound) {
         // the pool at round start (sorted by distance, FLT_MAX-padded); equal distances are
         // put in id order, as the round's merge orders its pool
         const float* gd = a.out_dists + (size_t)qi * k;
@@ -627,6 +640,9 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
     uint32_t p1 = a.probe_off[qi + 1];
     if (MODE == kModeReplay) {
         if (a.max_probes > 0 && p1 - p0 > (uint32_t)a.max_probes) p1 = p0 + (uint32_t)a.max_probes;
+    } else if (MODE == kModeEstimate) {
+        p0 = (uint32_t)pair;
+        p1 = p0 + 1;
     } else {
         p1 = min(p1, p0 + (uint32_t)a.r1);
         p0 = p0 + (uint32_t)a.r0;
@@ -657,18 +673,21 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
                                                               4 * (size_t)D)[lane]
                              : 0.f;
         };
-        if (nb > 0) load_block(blk0);
+        const int b_lo = min(blo, nb), b_hi = min(bhi, nb);
+        if (b_lo < b_hi) load_block(blk0 + (uint32_t)b_lo);
         float fac, sumq, delta;
         prepare_residual<kReplaySlices<NCH>>(q, c, y, D, S, uq, lane, fac, sumq, delta);
         quantize_digits(uq, S, D, nchunk, 1.0 / (double)delta, reinterpret_cast<uint32_t*>(A),
                         lane, kWrap ? qv : nullptr);
         wave_sync();
 
-        float distk = MODE == kModeReplay ? ((psize == k) ? w.pd[k - 1] : FLT_MAX) : thr;
+        float distk = MODE == kModeReplay ? ((psize == k) ? w.pd[k - 1] : FLT_MAX)
+                      : (MODE == kModeEstimate ? INFINITY : thr);
+        const uint32_t est0 = MODE == kModeEstimate ? a.est_off[pair] : 0u;
         const float c1 = (float)(0.5 * (double)sumq - (double)const_bound);
         const float c2 = (-5.0f * y) * a.one_over_sqrtD;
         const float qs = y * y;
-        for (int b = 0; b < nb; ++b) {
+        for (int b = b_lo; b < b_hi; ++b) {
             const uint32_t blk = blk0 + (uint32_t)b;
             const int np = min(32, n - b * 32);
             uint4 wc0[NCH], wc1[NCH];
@@ -678,7 +697,7 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
                 wc1[ch] = wn1[ch];
             }
             const float on_l = on_n;  // lane < 32: norm of vector `lane` of this block
-            if (b + 1 < nb) load_block(blk + 1);
+            if (b + 1 < b_hi) load_block(blk + 1);
             // plane by plane: the NCH-chunk MFMA chain of plane pp for both 16-vector tiles,
             // then folded into the running sums (plane 7 carries the sign: 0x80 is -128)
             int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
@@ -715,7 +734,7 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
             }
             wave_sync();
             bool cand = false;
-            float ipq_l = 0.f;
+            float ipq_l = 0.f, low_l = INFINITY;
             auto lower_test = [&]() {
                 cand = false;
                 if (lane < 32) {
@@ -726,11 +745,13 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
                     t = t * on;
                     t = t + qs;
                     t = t + on * on;
+                    low_l = lane < np ? t : INFINITY;
                     cand = (lane < np) && (t < distk);
                 }
             };
             lower_test();
             uint64_t mask = ballot(cand);
+            if (MODE == kModeEstimate && lane < 32) a.est_lower[est0 + 32 * b + lane] = low_l;
             if constexpr (kWrap) {
                 // the reference's integer for the vectors that pass on the exact one (four at
                 // a time, one per 16-lane group), then the lower bound again
@@ -787,7 +808,20 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
                 const float ip = butterfly16(part), ip2 = butterfly16(part2);
                 const float ipq_j = __shfl(ipq_l, jj), ipq_j2 = __shfl(ipq_l, jj2);
                 const float on_j = __shfl(on_l, jj), on_j2 = __shfl(on_l, jj2);
-                if (MODE == kModeReplay) {
+                if (MODE == kModeEstimate) {
+                    if (l16 == 0) {
+                        if (j >= 0) {
+                            a.est_exd[est0 + 32 * b + j] =
+                                ex_distance<EX>(on_j, sqr_y, exf * y, ipq_j, ip, sumq);
+                            a.est_id[est0 + 32 * b + j] = idv;
+                        }
+                        if (j2 >= 0) {
+                            a.est_exd[est0 + 32 * b + j2] =
+                                ex_distance<EX>(on_j2, sqr_y, exf2 * y, ipq_j2, ip2, sumq);
+                            a.est_id[est0 + 32 * b + j2] = idv2;
+                        }
+                    }
+                } else if (MODE == kModeReplay) {
                     if (l16 == 0) {
                         if (j >= 0) {
                             w.nd[s0 + g] = ex_distance<EX>(on_j, sqr_y, exf * y, ipq_j, ip, sumq);
@@ -832,12 +866,13 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
                 }
                 st_surv += ns;
                 distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
-            } else if (nn) {
+            } else if (MODE == kModeRound && nn) {
                 wave_sync();
                 pool_merge_did(w, psize, k, lane, w.nd, w.nid, nn);
             }
         }
     }
+    if (MODE == kModeEstimate) return;
     uint32_t* oi = a.out_ids + (size_t)qi * k;
     float* od = a.out_dists ? a.out_dists + (size_t)qi * k : nullptr;
     for (int i = lane; i < k; i += kWave) {
@@ -863,6 +898,24 @@ scan_mfma_kernel(ScanArgs a, int per_wave_bytes) {
         const int qi = query_of_wave(a, wid);
         if (qi < 0) return;
         scan_query_mfma<EX, NCH, MODE>(a, qi, base, lane);
+    } else if (MODE == kModeEstimate) {
+        // items: kEstBlocks consecutive blocks of one (query, probe) pair; est_item (after
+        // the pairs' slot offsets) holds each pair's first item
+        const uint32_t np = a.probe_off[a.nq];
+        const uint32_t* est_item = a.est_off + np + 1;
+        const uint32_t ni = est_item[np];
+        const uint32_t wpb = blockDim.x >> 6;
+        for (uint32_t it = blockIdx.x * wpb + (uint32_t)wid; it < ni; it += gridDim.x * wpb) {
+            uint32_t lo = 0, hi = np - 1;  // the last pair whose items start at or before it
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (est_item[mid] <= it) lo = mid; else hi = mid - 1;
+            }
+            const int g = (int)(it - est_item[lo]);
+            scan_query_mfma<EX, NCH, MODE>(a, (int)lo, base, lane, g * kEstBlocks,
+                                           (g + 1) * kEstBlocks);
+            wave_sync();
+        }
     } else {
         const uint32_t nl = a.qlist[0];
         const uint32_t wpb = blockDim.x >> 6;
@@ -947,6 +1000,216 @@ hipError_t launch_scan_round(const ScanArgs& a, hipStream_t s) {
     case 8: return launch_round_ex<8>(a, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+namespace {
+
+// Slot offsets of the estimate mode: est_off[p] = sum over pairs p' < p of the probed
+// cluster's size rounded up to 32 (one workgroup; pairs < probe_off[nq]).
+__global__ void __launch_bounds__(1024) est_offsets_kernel(const uint32_t* __restrict__ probe_off,
+                                                           const uint32_t* __restrict__ probe_ids,
+                                                           const uint32_t* __restrict__ csize,
+                                                           int nq, int K,
+                                                           uint32_t* __restrict__ est_off) {
+    __shared__ uint32_t wsum[16], isum[16];
+    __shared__ uint32_t carry, icarry;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t np = probe_off[nq];
+    uint32_t* est_item = est_off + np + 1;  // np + 1 more entries: each pair's first item
+    if (tid == 0) {
+        carry = 0;
+        icarry = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < np; base += 1024) {  // block-uniform
+        const uint32_t p = base + (uint32_t)tid;
+        uint32_t v = 0, u = 0;
+        if (p < np) {
+            const uint32_t cid = probe_ids[p];
+            v = cid < (uint32_t)K ? ((csize[cid] + 31u) & ~31u) : 0u;
+            u = (v / 32 + kEstBlocks - 1) / kEstBlocks;
+        }
+        uint32_t incl = v, iincl = u;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o), ti = __shfl_up(iincl, o);
+            if (lane >= o) {
+                incl += t;
+                iincl += ti;
+            }
+        }
+        if (lane == 63) {
+            wsum[wid] = incl;
+            isum[wid] = iincl;
+        }
+        __syncthreads();
+        uint32_t pre = carry + incl - v, ipre = icarry + iincl - u;
+        for (int w = 0; w < wid; w++) {
+            pre += wsum[w];
+            ipre += isum[w];
+        }
+        if (p < np) {
+            est_off[p] = pre;
+            est_item[p] = ipre;
+        }
+        __syncthreads();
+        if (tid == 1023) {
+            carry = pre + v;
+            icarry = ipre + u;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        est_off[np] = carry;
+        est_item[np] = icarry;
+    }
+}
+
+constexpr int kReplayChunk = 1024;  // slots staged in LDS at a time
+
+// One wave per query: the query's slots in probe order, block by block (32 slots): the
+// vectors whose lower bound is below the pool's k-th distance, in slot order, go into the
+// pool by pool_insert_bulk -- the per-block step of scan_query_mfma.  The next chunk's slots
+// are loaded while the current one is replayed.
+__global__ void __launch_bounds__(64) replay_est_kernel(ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x, k = a.k, kcap = (k + 63) & ~63;
+    const int qi = blockIdx.x;
+    float* cl = reinterpret_cast<float*>(smem);            // 2 x kReplayChunk lower bounds
+    float* ce = cl + 2 * kReplayChunk;                     // estimates
+    uint32_t* ci = reinterpret_cast<uint32_t*>(ce + 2 * kReplayChunk);  // ids
+    WaveLds w;
+    w.uq = nullptr;
+    w.lut = nullptr;
+    w.sums = nullptr;
+    w.pd = reinterpret_cast<float*>(ci + 2 * kReplayChunk);
+    w.pid = reinterpret_cast<uint32_t*>(w.pd + kcap);
+    w.nd = reinterpret_cast<float*>(w.pid + kcap);
+    w.nid = reinterpret_cast<uint32_t*>(w.nd + 32);
+    for (int i = lane; i < kcap; i += kWave) {
+        w.pd[i] = FLT_MAX;
+        w.pid[i] = 0;
+    }
+    int psize = 0;
+    uint32_t p0 = a.probe_off[qi], p1 = a.probe_off[qi + 1];
+    if (a.max_probes > 0 && p1 - p0 > (uint32_t)a.max_probes) p1 = p0 + (uint32_t)a.max_probes;
+    const uint32_t s0 = a.est_off[p0], s1 = a.est_off[p1];
+    constexpr int kPer = kReplayChunk / kWave;
+    float rl[kPer], re[kPer];
+    uint32_t ri[kPer];
+    auto fetch = [&](uint32_t c0) {  // registers <- slots [c0, c0 + chunk) (clamped)
+#pragma unroll
+        for (int r = 0; r < kPer; r++) {
+            const uint32_t sl = c0 + (uint32_t)(r * kWave + lane);
+            const bool ok = sl < s1;
+            rl[r] = ok ? a.est_lower[sl] : INFINITY;
+            re[r] = ok ? a.est_exd[sl] : 0.f;
+            ri[r] = ok ? a.est_id[sl] : 0u;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < kPer; r++) {
+            cl[buf * kReplayChunk + r * kWave + lane] = rl[r];
+            ce[buf * kReplayChunk + r * kWave + lane] = re[r];
+            ci[buf * kReplayChunk + r * kWave + lane] = ri[r];
+        }
+    };
+    unsigned long long st_blocks = 0, st_vec = 0, st_surv = 0;
+    float distk = FLT_MAX;
+    int buf = 0;
+    if (s0 < s1) {
+        fetch(s0);
+        stash(0);
+    }
+    wave_sync();
+    for (uint32_t c0 = s0; c0 < s1; c0 += kReplayChunk, buf ^= 1) {  // wave-uniform
+        const bool more = c0 + kReplayChunk < s1;
+        if (more) fetch(c0 + kReplayChunk);
+        const int n = (int)min((uint32_t)kReplayChunk, s1 - c0);
+        const float* L = cl + buf * kReplayChunk;
+        const float* E = ce + buf * kReplayChunk;
+        const uint32_t* I = ci + buf * kReplayChunk;
+        for (int b0 = 0; b0 < n; b0 += 32) {
+            const float lv = lane < 32 ? L[b0 + lane] : INFINITY;
+            const uint64_t valid = ballot(lane < 32 && lv != INFINITY);
+            const uint64_t mask = ballot(lane < 32 && lv < distk);
+            if (valid) {
+                st_blocks++;
+                st_vec += __popcll(valid);
+            }
+            const int ns = __popcll(mask);
+            if (ns) {
+                if ((mask >> lane) & 1ull) {
+                    const int t = __popcll(mask & ((1ull << lane) - 1ull));
+                    w.nd[t] = E[b0 + lane];
+                    w.nid[t] = I[b0 + lane];
+                }
+                wave_sync();
+                pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
+                st_surv += ns;
+                distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+            }
+        }
+        if (more) stash(buf ^ 1);
+        wave_sync();
+    }
+    uint32_t* oi = a.out_ids + (size_t)qi * k;
+    float* od = a.out_dists ? a.out_dists + (size_t)qi * k : nullptr;
+    for (int i = lane; i < k; i += kWave) {
+        oi[i] = i < psize ? w.pid[i] : 0u;
+        if (od) od[i] = i < psize ? w.pd[i] : FLT_MAX;
+    }
+    if (a.stats && lane == 0) {
+        atomicAdd(&a.stats[0], st_blocks);
+        atomicAdd(&a.stats[1], st_vec);
+        atomicAdd(&a.stats[2], st_surv);
+        atomicAdd(&a.stats[3], (unsigned long long)(p1 - p0));
+    }
+}
+
+template <int EX>
+hipError_t launch_estimate_ex(const ScanArgs& a, hipStream_t s) {
+    const size_t per_wave = mfma_wave_bytes(a.D, a.nchunk, a.k);
+    const int waves = per_wave * 2 <= 65536 ? 2 : 1;
+    // a fixed grid: the pair count is only known on the device
+    dim3 grid(1024), block(64 * waves);
+    switch (a.nchunk) {
+    case 1: hipLaunchKernelGGL((scan_mfma_kernel<EX, 1, kModeEstimate>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
+    case 2: hipLaunchKernelGGL((scan_mfma_kernel<EX, 2, kModeEstimate>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
+    case 3: hipLaunchKernelGGL((scan_mfma_kernel<EX, 3, kModeEstimate>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
+    case 4: hipLaunchKernelGGL((scan_mfma_kernel<EX, 4, kModeEstimate>), grid, block, per_wave * waves, s, a, (int)per_wave); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t replay_est_lds(int k) {
+    const size_t kcap = (size_t)((k + 63) & ~63);
+    return (size_t)2 * kReplayChunk * 12 + 8 * kcap + 256;
+}
+
+hipError_t launch_scan_latency(const ScanArgs& a, hipStream_t s) {
+    if (!a.bp || a.nchunk > 4 || !a.est_off || !a.est_lower || !a.est_exd || !a.est_id)
+        return hipErrorInvalidValue;
+    if (a.nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(est_offsets_kernel, dim3(1), dim3(1024), 0, s, a.probe_off, a.probe_ids,
+                       a.csize, a.nq, a.K, a.est_off);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (a.EX) {
+    case 2: e = launch_estimate_ex<2>(a, s); break;
+    case 3: e = launch_estimate_ex<3>(a, s); break;
+    case 4: e = launch_estimate_ex<4>(a, s); break;
+    case 6: e = launch_estimate_ex<6>(a, s); break;
+    case 7: e = launch_estimate_ex<7>(a, s); break;
+    case 8: e = launch_estimate_ex<8>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(replay_est_kernel, dim3(a.nq), dim3(64), replay_est_lds(a.k), s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s) {

@@ -68,7 +68,16 @@ enum {
      * exact replay of ranks [0, 4); 1: on a second stream beside it.  Results are identical;
      * both are VALU-bound, so side by side they take about their sum (measured: 23.1 ms per
      * 20k-query learned search without the overlap, 23.5 ms with it). */
-    ERQ_OPT_BATCH_OVERLAP = 7
+    ERQ_OPT_BATCH_OVERLAP = 7,
+    /* Batches of at most this many queries (default 16; 0 off) through erq_search,
+     * erq_search_batch, erq_search_batch_nprobe (per-query scan) and erq_search_dev take the
+     * low-latency scan when D <= 2048: every vector of every probed cluster is estimated in
+     * parallel across the GPU, then one wave per query replays its probes in order with the
+     * reference's pruning threshold.  Results are identical to the per-query scan; it reads
+     * every probed vector's EX-bit code (the per-query scan reads the survivors' only), so it
+     * trades throughput for latency.  Its slot buffer (12 bytes per probed vector slot) is
+     * allocated on first use, outside HIP graph capture. */
+    ERQ_OPT_LATENCY_NQ = 8
 };
 int erq_set_option(erq_index* h, int option, long value);
 long erq_get_option(const erq_index* h, int option);
