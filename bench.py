@@ -453,18 +453,30 @@ def cpu_baseline(idx, q, a, nprobe):
 
 def drop_in_latency(r, nprobe, nq=256):
     """The IVF::search drop-in (erq_search, host pointers, one query per call, as
-    test_search.cpp's loop calls IVF::search): median and mean latency per query."""
+    test_search.cpp's loop calls IVF::search): median and mean latency per query, at the
+    bench's nprobe and at 5 (the published CSV row's), with the low-latency scan (the
+    default) and with the per-query scan (ERQ_OPT_LATENCY_NQ 0)."""
     qr = r.g.rotate_dev(r.q[:nq]).cpu().numpy()
-    for i in range(8):
-        r.g.search(qr[i], r.a.k, nprobe)
-    lat = []
-    for i in range(nq):
-        t0 = time.perf_counter()
-        r.g.search(qr[i], r.a.k, nprobe)
-        lat.append((time.perf_counter() - t0) * 1e6)
-    lat = np.asarray(lat)
-    return {"nprobe": nprobe, "queries": nq, "median_us": round(float(np.median(lat)), 1),
-            "mean_us": round(float(lat.mean()), 1), "p99_us": round(float(np.percentile(lat, 99)), 1)}
+    out = {"queries": nq}
+    for lat_nq, tag in ((16, "latency_scan"), (0, "per_query_scan")):
+        r.g.set_option(8, lat_nq)
+        for npb in (nprobe, 5):
+            for i in range(8):
+                r.g.search(qr[i], r.a.k, npb)
+            lat = []
+            for i in range(nq):
+                t0 = time.perf_counter()
+                r.g.search(qr[i], r.a.k, npb)
+                lat.append((time.perf_counter() - t0) * 1e6)
+            lat = np.asarray(lat)
+            out[f"{tag}_nprobe{npb}"] = {
+                "median_us": round(float(np.median(lat)), 1),
+                "mean_us": round(float(lat.mean()), 1),
+                "p99_us": round(float(np.percentile(lat, 99)), 1)}
+    r.g.set_option(8, 16)
+    out["nprobe"] = nprobe
+    out["median_us"] = out[f"latency_scan_nprobe{nprobe}"]["median_us"]
+    return out
 
 
 def profile_plain(a, r):

@@ -1129,7 +1129,27 @@ __global__ void __launch_bounds__(64) replay_est_kernel(ScanArgs a) {
         const float* L = cl + buf * kReplayChunk;
         const float* E = ce + buf * kReplayChunk;
         const uint32_t* I = ci + buf * kReplayChunk;
-        for (int b0 = 0; b0 < n; b0 += 32) {
+        for (int g0 = 0; g0 < n; g0 += 256) {
+            // eight blocks at a time: none of their slots below the k-th distance (the common
+            // case once the pool has filled) -- nothing to insert, only the counts
+            float lv8[4];
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int sl = g0 + 64 * r + lane;
+                lv8[r] = sl < n ? L[sl] : INFINITY;
+                any = any || lv8[r] < distk;
+            }
+            if (!ballot(any)) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint64_t v = ballot(lv8[r] != INFINITY);
+                    st_blocks += ((v & 0xFFFFFFFFull) != 0) + ((v >> 32) != 0);
+                    st_vec += __popcll(v);
+                }
+                continue;
+            }
+        for (int b0 = g0; b0 < min(n, g0 + 256); b0 += 32) {
             const float lv = lane < 32 ? L[b0 + lane] : INFINITY;
             const uint64_t valid = ballot(lane < 32 && lv != INFINITY);
             const uint64_t mask = ballot(lane < 32 && lv < distk);
@@ -1149,6 +1169,7 @@ __global__ void __launch_bounds__(64) replay_est_kernel(ScanArgs a) {
                 st_surv += ns;
                 distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
             }
+        }
         }
         if (more) stash(buf ^ 1);
         wave_sync();
