@@ -258,6 +258,77 @@ int erqb_quantize_cluster(const float* r, const float* onorm, size_t n, size_t D
     return 0;
 }
 
+// The non-HA format (Searcher.hpp; DataQuantizer::rabitq_factor, Quantizer.hpp:386-437): the
+// same codes, four factors per vector after a block's 4D code bytes -- ||o - c||^2 (x2[v],
+// given), fac_ip = 2 ||o - c|| / sum |u| / 2, the number of positive dims, and the error
+// factor 2 ||o - c|| FAC_ERR sqrt(1 - <u, o_bar>^2) / <u, o_bar> -- and xipnorm over
+// ||o - c|| = sqrt((double) x2).  Sums over dims run in dim order (float for <u, o_bar>,
+// double for the others).  blocks: ceil(n/32) * (4D + 512) bytes.
+int erqb_quantize_cluster_nonha(const float* r, const float* x2, size_t n, size_t D, int ex_bits,
+                                uint8_t* blocks, uint8_t* long_code, float* xipnorm) {
+    if (D % 64 != 0) return -1;
+    if (!(ex_bits == 2 || ex_bits == 3 || ex_bits == 4 || ex_bits == 6 || ex_bits == 7 ||
+          ex_bits == 8))
+        return -1;
+    const size_t M = D / 4;
+    const size_t block_bytes = 4 * D + 512;
+    const size_t code_len = D * (size_t)ex_bits / 8;
+    const int mask = (1 << ex_bits) - 1;
+    const size_t nblocks = (n + 31) / 32;
+    const double fac_norm = 1 / std::sqrt((double)D);
+    const double fac_err = 2.0 / std::sqrt((double)(D - 1));
+    std::memset(blocks, 0, nblocks * block_bytes);
+    std::vector<float> u(D), absu(D);
+    std::vector<uint8_t> raw(D);
+    Scratch scratch(D);
+    for (size_t v = 0; v < n * D; v++)
+        if (!std::isfinite(r[v])) return -2;
+    for (size_t v = 0; v < n; v++)
+        if (!std::isfinite(x2[v])) return -2;
+    for (size_t v = 0; v < n; v++) {
+        const float* x = r + v * D;
+        double ss = 0;
+        for (size_t d = 0; d < D; d++) ss += (double)x[d] * x[d];
+        const float inv = ss > 0 ? (float)(1.0 / std::sqrt(ss)) : 0.f;
+        for (size_t d = 0; d < D; d++) {
+            u[d] = x[d] * inv;
+            absu[d] = std::fabs(u[d]);
+        }
+        uint8_t* blk = blocks + (v / 32) * block_bytes;
+        const int vb = (int)(v % 32), j = byte_of_vec(vb);
+        for (size_t m = 0; m < M; m++) {
+            const float* xm = x + 4 * m;
+            int nib = ((xm[0] > 0) << 3) | ((xm[1] > 0) << 2) | ((xm[2] > 0) << 1) | (xm[3] > 0);
+            uint8_t* cb = blk + (m >> 1) * 32 + (m & 1) * 16;
+            cb[j] |= (uint8_t)(vb < 16 ? nib : nib << 4);
+        }
+        const float dist2c = (float)std::sqrt((double)x2[v]);
+        float o_obar = 0.f;
+        double ip = 0;
+        int sumxb = 0;
+        const float fn = (float)fac_norm;
+        for (size_t d = 0; d < D; d++) {
+            const bool pos = x[d] > 0;
+            o_obar += u[d] * (pos ? fn : -fn);
+            ip += std::abs(0.5 * u[d]);
+            sumxb += pos;
+        }
+        if (!std::isfinite(o_obar)) o_obar = 0.8f;
+        float* fac = reinterpret_cast<float*>(blk + 4 * D);
+        fac[vb] = x2[v];
+        fac[32 + vb] = (float)(1.0 / ip * 2 * dist2c);
+        fac[64 + vb] = (float)sumxb;
+        fac[96 + vb] = (float)(std::sqrt((1.0 - o_obar * o_obar) / (o_obar * o_obar)) * fac_err *
+                               2 * dist2c);
+        const float ipn = quantize_magnitudes(absu.data(), D, ex_bits, raw.data(), scratch);
+        xipnorm[v] = (float)(ipn * 2 * std::sqrt((double)x2[v]));
+        for (size_t d = 0; d < D; d++)
+            if (!(x[d] > 0)) raw[d] = (uint8_t)((~raw[d]) & mask);
+        compact_code(raw.data(), D, ex_bits, long_code + v * code_len);
+    }
+    return 0;
+}
+
 // All clusters, OpenMP over clusters.  r/onorm are in cluster order (cluster 0's
 // vectors first); outputs are the contiguous SHORT_DATA / LONG_CODE / EX_FACTOR arrays.
 int erqb_quantize_all(const float* r, const float* onorm, const uint64_t* cluster_sizes, size_t K,
@@ -280,6 +351,31 @@ int erqb_quantize_all(const float* r, const float* onorm, const uint64_t* cluste
         err |= erqb_quantize_cluster(r + v0 * D, onorm + v0, cluster_sizes[k], D, ex_bits,
                                      short_data + bstart[k] * block_bytes,
                                      long_code + v0 * code_len, xipnorm + v0) != 0;
+    }
+    return err ? -1 : 0;
+}
+
+// erqb_quantize_all for the non-HA format (x2 = ||o - c||^2 per vector).
+int erqb_quantize_all_nonha(const float* r, const float* x2, const uint64_t* cluster_sizes,
+                            size_t K, size_t D, int ex_bits, uint8_t* short_data,
+                            uint8_t* long_code, float* xipnorm, int threads) {
+    std::vector<size_t> vstart(K + 1, 0), bstart(K + 1, 0);
+    for (size_t k = 0; k < K; k++) {
+        vstart[k + 1] = vstart[k] + cluster_sizes[k];
+        bstart[k + 1] = bstart[k] + (cluster_sizes[k] + 31) / 32;
+    }
+    const size_t block_bytes = 4 * D + 512;
+    const size_t code_len = D * (size_t)ex_bits / 8;
+    int err = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+    for (long k = 0; k < (long)K; k++) {
+        size_t v0 = vstart[k];
+        err |= erqb_quantize_cluster_nonha(r + v0 * D, x2 + v0, cluster_sizes[k], D, ex_bits,
+                                           short_data + bstart[k] * block_bytes,
+                                           long_code + v0 * code_len, xipnorm + v0) != 0;
     }
     return err ? -1 : 0;
 }

@@ -92,6 +92,8 @@ BUILDER_SIGNATURES = {
     "erqb_quantize_all": (c_int, [c_vp, c_vp, c_vp, c_size, c_size, c_int, c_vp, c_vp, c_vp,
                                   c_int]),
     "erqb_quantize_magnitudes": (ctypes.c_float, [c_vp, c_size, c_int, c_vp]),
+    "erqb_quantize_all_nonha": (c_int, [c_vp, c_vp, c_vp, c_size, c_size, c_int, c_vp, c_vp, c_vp,
+                                        c_int]),
 }
 
 _lib = None

@@ -6,7 +6,10 @@ all little-endian:
   cluster_sizes[K]              size_t
   P[D x D]                      float32 row-major (Rotator.hpp:34-57)
   centroids[K x D]              float32, rotated (Initializer.hpp:70-76)
-  SHORT_DATA                    sum_k ceil(n_k/32) blocks of (4D + 128) bytes
+  SHORT_DATA                    sum_k ceil(n_k/32) blocks of (4D + 128 F) bytes: F = 1 factor
+                                per vector (HA, HIGH_ACC_FAST_SCAN) or 4 (the non-HA Searcher,
+                                Quantizer.hpp:107-121); the file does not record F, so load()
+                                infers it from the file size
   LONG_CODE                     N * D*EX/8 bytes
   EX_FACTOR                     N float32 (xipnorm)
   IDs                           N uint32
@@ -40,14 +43,19 @@ class IVFIndex:
     long_code: np.ndarray      # uint8 [N * D*EX/8]
     ex_factor: np.ndarray      # float32 [N]
     ids: np.ndarray            # uint32 [N]
+    factors: int = 1           # short factors per vector: 1 (HA) or 4 (non-HA)
 
     @property
     def D(self):
         return padded_dim(self.DIM)
 
     @property
+    def high_acc(self):
+        return self.factors == 1
+
+    @property
     def block_bytes(self):
-        return 4 * self.D + 128
+        return 4 * self.D + 128 * self.factors
 
     @property
     def code_len(self):
@@ -64,6 +72,8 @@ class IVFIndex:
         D = self.D
         if self.EX not in EX_BITS_SUPPORTED:
             raise ValueError(f"EX_BITS={self.EX} unsupported")
+        if self.factors not in (1, 4):
+            raise ValueError("factors per vector must be 1 (HA) or 4 (non-HA)")
         if int(self.cluster_sizes.sum()) != self.N:
             raise ValueError("cluster sizes do not sum to N")
         nblocks = int(self.block_start()[-1])
@@ -91,6 +101,7 @@ class IVFIndex:
 
     @staticmethod
     def load(path):
+        size = os.path.getsize(path)
         with open(path, "rb") as f:
             hdr = np.fromfile(f, dtype="<u8", count=4)
             if hdr.size != 4:
@@ -98,14 +109,19 @@ class IVFIndex:
             N, DIM, K, EX = (int(x) for x in hdr)
             D = padded_dim(DIM)
             cs = np.fromfile(f, dtype="<u8", count=K)
+            nblocks = int(((cs + 31) // 32).sum())
+            rest = 32 + 8 * K + 4 * D * D + 4 * K * D + N * D * EX // 8 + 8 * N
+            fits = [F for F in (1, 4) if rest + nblocks * (4 * D + 128 * F) == size]
+            if len(fits) != 1:
+                raise ValueError("file size matches neither the HA nor the non-HA block size")
+            F = fits[0]
             P = np.fromfile(f, dtype="<f4", count=D * D).reshape(D, D)
             C = np.fromfile(f, dtype="<f4", count=K * D).reshape(K, D)
-            nblocks = int(((cs + 31) // 32).sum())
-            sd = np.fromfile(f, dtype=np.uint8, count=nblocks * (4 * D + 128))
+            sd = np.fromfile(f, dtype=np.uint8, count=nblocks * (4 * D + 128 * F))
             lc = np.fromfile(f, dtype=np.uint8, count=N * D * EX // 8)
             ef = np.fromfile(f, dtype="<f4", count=N)
             ids = np.fromfile(f, dtype="<u4", count=N)
-        idx = IVFIndex(N, DIM, K, EX, cs, P, C, sd, lc, ef, ids)
+        idx = IVFIndex(N, DIM, K, EX, cs, P, C, sd, lc, ef, ids, F)
         idx.validate()
         return idx
 
@@ -124,8 +140,9 @@ def _np_matmul(a, b):
 
 
 def build_index(data, centroids, assign, bits, P=None, seed=3, matmul=None, threads=0,
-                chunk=65536):
-    """Build an HA index (IVF::construct, IVF.hpp:116-153) from raw vectors.
+                chunk=65536, high_acc=True):
+    """Build an HA index (IVF::construct, IVF.hpp:116-153) from raw vectors; high_acc=False
+    writes the non-HA format (four factors per vector, Quantizer.hpp:386-437).
 
     data [N, DIM] float32, centroids [K, DIM] float32 (un-rotated), assign [N] cluster ids,
     bits = total bits per dimension B (EX = B - 1).  matmul(a, b) computes a @ b in float32
@@ -151,6 +168,7 @@ def build_index(data, centroids, assign, bits, P=None, seed=3, matmul=None, thre
     crot = np.ascontiguousarray(matmul(cpad, P), dtype=np.float32)
     resid = np.empty((N, D), np.float32)
     onorm = np.empty(N, np.float32)
+    onorm2 = np.empty(N, np.float32)
     xp = np.zeros((chunk, D), np.float32)
     for s in range(0, N, chunk):
         sel = order[s:s + chunk]
@@ -158,13 +176,18 @@ def build_index(data, centroids, assign, bits, P=None, seed=3, matmul=None, thre
         xp[:n, :DIM] = data[sel]
         resid[s:s + n] = matmul(xp[:n], P) - crot[assign[sel]]
         diff = data[sel] - centroids[assign[sel]]
-        onorm[s:s + n] = np.sqrt(np.einsum("ij,ij->i", diff, diff, dtype=np.float32))
+        onorm2[s:s + n] = np.einsum("ij,ij->i", diff, diff, dtype=np.float32)
+        onorm[s:s + n] = np.sqrt(onorm2[s:s + n])
     nblocks = int(((sizes + 31) // 32).sum())
-    short = np.zeros(nblocks * (4 * D + 128), np.uint8)
+    F = 1 if high_acc else 4
+    short = np.zeros(nblocks * (4 * D + 128 * F), np.uint8)
     longc = np.zeros(N * D * EX // 8, np.uint8)
     xip = np.zeros(N, np.float32)
-    rc = _lib.builder().erqb_quantize_all(
-        resid.ctypes.data_as(ctypes.c_void_p), onorm.ctypes.data_as(ctypes.c_void_p),
+    fn = _lib.builder().erqb_quantize_all if high_acc else _lib.builder().erqb_quantize_all_nonha
+    # the non-HA blocks keep ||o - c||^2 (the HA ones ||o - c||)
+    fac = onorm if high_acc else np.ascontiguousarray(onorm2, np.float32)
+    rc = fn(
+        resid.ctypes.data_as(ctypes.c_void_p), fac.ctypes.data_as(ctypes.c_void_p),
         sizes.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(K), ctypes.c_size_t(D),
         ctypes.c_int(EX), short.ctypes.data_as(ctypes.c_void_p),
         longc.ctypes.data_as(ctypes.c_void_p), xip.ctypes.data_as(ctypes.c_void_p),
@@ -172,7 +195,7 @@ def build_index(data, centroids, assign, bits, P=None, seed=3, matmul=None, thre
     if rc != 0:
         raise ValueError("index build failed: non-finite vectors or unsupported shape")
     return IVFIndex(N, DIM, K, EX, sizes, P, crot, short, longc, xip,
-                    order.astype(np.uint32))
+                    order.astype(np.uint32), F)
 
 
 def index_bytes(idx):

@@ -749,3 +749,161 @@ void orc_knn(const float* base, size_t n, const float* q, size_t nq, size_t D, s
         free(c);
     }
 }
+
+/* ---------------------------------------------------------------- the non-HA searcher
+ * Searcher (Searcher.hpp:14-236), the searcher IVF::search uses when HIGH_ACC_FAST_SCAN is
+ * not defined (IVF.hpp:307-311).  Its blocks hold four factors per vector (Quantizer.hpp:
+ * 107-121): 4D code bytes, then ||o-c||^2, fac_ip, sumxb and fac_err, 32 floats each. */
+
+/* minus_sum16, space.hpp:714-736: r = q - c, and the sum of r in 16 lane-strided chains
+ * reduced by _mm512_reduce_add_ps. */
+float orc_minus_sum16(const float* q, const float* c, float* r, size_t D) {
+    float p[16] = {0};
+    for (size_t i = 0; i < D; i += 16)
+        for (int l = 0; l < 16; l++) {
+            r[i + l] = q[i + l] - c[i + l];
+            p[l] = p[l] + r[i + l];
+        }
+    return orc_reduce16(p);
+}
+
+/* scalar_quantize16<uint16_t, 16>, space.hpp:644-686 (AVX-512 branch): (r - vl) * f32(1 /
+ * width) + 0.5, truncated to int32, then to 16 bits. */
+void orc_scalar_quantize16(uint16_t* out, const float* r, float vl, float width, size_t D) {
+    const float o = (float)(1.0 / (double)width);
+    for (size_t i = 0; i < D; i++) {
+        float t = (r[i] - vl) * o;
+        t = t + 0.5f;
+        out[i] = (uint16_t)(int32_t)t;
+    }
+}
+
+/* Searcher::pack_LUT (Searcher.hpp:129-172): LUT_total[m][j] = sum of the digits of the dims
+ * whose bits j holds (bit 3 = dim 4m), in u16. */
+void orc_pack_lut_u16(uint16_t* lut, const uint16_t* q, size_t D) {
+    for (size_t m = 0; m < D / 4; m++)
+        for (int j = 0; j < 16; j++) {
+            uint16_t s = 0;
+            for (int b = 0; b < 4; b++)
+                if (j & (8 >> b)) s = (uint16_t)(s + q[4 * m + b]);
+            lut[m * 16 + j] = s;
+        }
+}
+
+/* accumulate_robust (FastScan.hpp:136-212) over LUT_lower and LUT_upper, combined as
+ * Searcher::scan_one_block does (Searcher.hpp:196-199): per byte plane, the even and the odd
+ * codebooks are summed in u16 lanes (mod 2^16) and the two sums added in 32 bits; then
+ * (upper << 8) + lower. */
+void orc_nonha_block_ints(const uint8_t* codes, const uint16_t* lut, size_t D, int32_t* out) {
+    for (int v = 0; v < 32; v++) {
+        uint32_t lo[2] = {0, 0}, hi[2] = {0, 0};
+        for (size_t m = 0; m < D / 4; m++) {
+            const uint16_t e = lut[m * 16 + block_nibble(codes, m, v)];
+            lo[m & 1] += e & 0xFFu;
+            hi[m & 1] += e >> 8;
+        }
+        const uint32_t lower = (lo[0] & 0xFFFFu) + (lo[1] & 0xFFFFu);
+        const uint32_t upper = (hi[0] & 0xFFFFu) + (hi[1] & 0xFFFFu);
+        out[v] = (int32_t)((upper << 8) + lower);
+    }
+}
+
+/* Searcher::search_cluster + scan_one_block (Searcher.hpp:85-236). */
+static void nonha_search_cluster(const orc_index* idx, size_t cid, const float* q, float sqr_y,
+                                 float* r, uint16_t* qq, uint16_t* lut, orc_pool* pool,
+                                 orc_stats* st) {
+    const size_t D = idx->D, n = (size_t)idx->cluster_sizes[cid];
+    const float* c = idx->centroids + cid * D;
+    /* preparing(): the residual, half its sum, the 14-bit unsigned digits, the LUT */
+    const float half_sum = (float)(0.5 * (double)orc_minus_sum16(q, c, r, D));
+    float vl, vr;
+    orc_data_range(r, D, &vl, &vr);
+    const float width = (vr - vl) / (float)((1 << 14) - 1);
+    orc_scalar_quantize16(qq, r, vl, width, D);
+    orc_pack_lut_u16(lut, qq, D);
+    const int R = 1 << idx->EX;
+    const size_t bb = 4 * D + 512, code_len = D * idx->EX / 8;
+    const uint8_t* block = idx->short_data + idx->block_start[cid] * bb;
+    const size_t v0 = idx->vec_start[cid];
+    const float y = sqrtf(sqr_y);
+    float distk = orc_pool_distk(pool);
+    int32_t ints[32];
+    for (size_t b = 0; b * 32 < n; b++, block += bb) {
+        const size_t np = n - b * 32 < 32 ? n - b * 32 : 32;
+        const float* x2 = (const float*)(block + 4 * D);
+        const float* fip = x2 + 32;
+        const float* sumxb = x2 + 64;
+        const float* ferr = x2 + 96;
+        orc_nonha_block_ints(block, lut, D, ints);
+        float lower[32], rip[32];
+        for (int j = 0; j < 32; j++) {
+            const float sum_sqr = x2[j] + sqr_y;
+            const float xbvl = sumxb[j] * vl;
+            const float resf = (float)ints[j];
+            float ip = resf * width + xbvl;
+            ip = ip - half_sum;
+            rip[j] = ip;
+            ip = ip * fip[j];
+            const float err = ferr[j] * y;
+            lower[j] = sum_sqr - (ip + err);
+        }
+        if (st) {
+            st->blocks++;
+            st->vectors += np;
+        }
+        for (size_t j = 0; j < np; j++) {
+            if (lower[j] > distk) continue;
+            const size_t vi = v0 + b * 32 + j;
+            const float ipx = orc_ip_ex(r, idx->long_code + vi * code_len, D, (int)idx->EX);
+            const float ex_dist = x2[j] + sqr_y -
+                                  idx->ex_factor[vi] *
+                                      ((float)R * rip[j] + ipx - (float)(R - 1) * half_sum);
+            orc_pool_insert(pool, idx->ids[vi], ex_dist);
+            distk = orc_pool_distk(pool);
+            if (st) st->survivors++;
+        }
+    }
+}
+
+void orc_nonha_search_probes(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
+                             const uint32_t* probe_ids, const float* probe_sqr_y, uint32_t* ids,
+                             float* dists, orc_stats* st) {
+    const size_t D = idx->D;
+    float* r = (float*)malloc(D * sizeof(float));
+    uint16_t* qq = (uint16_t*)malloc(D * sizeof(uint16_t));
+    uint16_t* lut = (uint16_t*)malloc((D / 4) * 16 * sizeof(uint16_t));
+    uint32_t* pid = (uint32_t*)malloc((k + 1) * sizeof(uint32_t));
+    float* pd = (float*)malloc((k + 1) * sizeof(float));
+    orc_pool pool;
+    orc_pool_init(&pool, pid, pd, k);
+    for (size_t i = 0; i < nprobe; i++)
+        nonha_search_cluster(idx, probe_ids[i], q_rot, probe_sqr_y[i], r, qq, lut, &pool, st);
+    for (size_t i = 0; i < k; i++) {
+        ids[i] = i < pool.size ? pid[i] : 0u;
+        if (dists) dists[i] = i < pool.size ? pd[i] : FLT_MAX;
+    }
+    free(r);
+    free(qq);
+    free(lut);
+    free(pid);
+    free(pd);
+}
+
+void orc_nonha_search_batch(const orc_index* idx, const float* q_rot, size_t nq, size_t k,
+                            size_t nprobe, uint32_t* ids, float* dists, int threads) {
+    if (nprobe > idx->K) nprobe = idx->K;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (long i = 0; i < (long)nq; i++) {
+        uint32_t* pids = (uint32_t*)malloc(nprobe * sizeof(uint32_t) + 1);
+        float* pdist = (float*)malloc(nprobe * sizeof(float) + 1);
+        const float* q = q_rot + (size_t)i * idx->D;
+        orc_centroid_topn(q, idx->centroids, idx->K, idx->D, nprobe, pids, pdist);
+        orc_nonha_search_probes(idx, q, k, nprobe, pids, pdist, ids + (size_t)i * k,
+                                dists ? dists + (size_t)i * k : NULL, NULL);
+        free(pids);
+        free(pdist);
+    }
+}

@@ -111,6 +111,18 @@ double orc_l2sqr16_f64(const float* x, const float* y, size_t D);
 void orc_knn(const float* base, size_t n, const float* q, size_t nq, size_t D, size_t k,
              uint32_t* ids, float* dists);
 
+/* The non-HA searcher (Searcher.hpp; IVF::search without HIGH_ACC_FAST_SCAN): blocks of
+ * 4D code bytes + 4 x 32 float factors (||o-c||^2, fac_ip, sumxb, fac_err). */
+float orc_minus_sum16(const float* q, const float* c, float* r, size_t D);
+void orc_scalar_quantize16(uint16_t* out, const float* r, float vl, float width, size_t D);
+void orc_pack_lut_u16(uint16_t* lut, const uint16_t* q, size_t D);
+void orc_nonha_block_ints(const uint8_t* codes, const uint16_t* lut, size_t D, int32_t* out);
+void orc_nonha_search_probes(const orc_index* idx, const float* q_rot, size_t k, size_t nprobe,
+                             const uint32_t* probe_ids, const float* probe_sqr_y, uint32_t* ids,
+                             float* dists, orc_stats* st);
+void orc_nonha_search_batch(const orc_index* idx, const float* q_rot, size_t nq, size_t k,
+                            size_t nprobe, uint32_t* ids, float* dists, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,6 +89,20 @@ def lib():
         L.orc_kmeans.argtypes = [c_vp, c_size, c_size, c_size, c_size, c_vp, c_vp, c_vp, c_vp]
         L.orc_l2sqr16_f64.restype = ctypes.c_double
         L.orc_l2sqr16_f64.argtypes = [c_vp, c_vp, c_size]
+        L.orc_minus_sum16.restype = f
+        L.orc_minus_sum16.argtypes = [c_vp, c_vp, c_vp, c_size]
+        L.orc_scalar_quantize16.restype = None
+        L.orc_scalar_quantize16.argtypes = [c_vp, c_vp, f, f, c_size]
+        L.orc_pack_lut_u16.restype = None
+        L.orc_pack_lut_u16.argtypes = [c_vp, c_vp, c_size]
+        L.orc_nonha_block_ints.restype = None
+        L.orc_nonha_block_ints.argtypes = [c_vp, c_vp, c_size, c_vp]
+        L.orc_nonha_search_probes.restype = None
+        L.orc_nonha_search_probes.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size,
+                                              c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(OrcStats)]
+        L.orc_nonha_search_batch.restype = None
+        L.orc_nonha_search_batch.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size,
+                                             c_size, c_vp, c_vp, ctypes.c_int]
         L.orc_knn.restype = None
         L.orc_knn.argtypes = [c_vp, c_size, c_vp, c_size, c_size, c_size, c_vp, c_vp]
         _lib = L
@@ -251,14 +265,19 @@ class OracleIndex:
             return ids, d, (st.blocks, st.vectors, st.survivors)
         return ids, d
 
-    def search_probes(self, q_rot, k, probe_ids, probe_d):
+    def search_probes(self, q_rot, k, probe_ids, probe_d, stats=False):
         q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)
         pid = np.ascontiguousarray(probe_ids, np.uint32)
         pd = np.ascontiguousarray(probe_d, np.float32)
         ids = np.empty(k, np.uint32)
         d = np.empty(k, np.float32)
-        lib().orc_ivf_search_probes(ctypes.byref(self.view), _p(q), k, pid.size, _p(pid), _p(pd),
-                                    _p(ids), _p(d), None)
+        st = OrcStats()
+        fn = (lib().orc_nonha_search_probes if getattr(self.idx, "factors", 1) == 4
+              else lib().orc_ivf_search_probes)
+        fn(ctypes.byref(self.view), _p(q), k, pid.size, _p(pid), _p(pd), _p(ids), _p(d),
+           ctypes.byref(st))
+        if stats:
+            return ids, d, (st.blocks, st.vectors, st.survivors)
         return ids, d
 
     def search_rounds(self, q_rot, k, probe_ids, probe_d, stats=False):
@@ -277,6 +296,12 @@ class OracleIndex:
     def search_batch(self, q_rot, k, nprobe, threads=0):
         q = np.ascontiguousarray(q_rot, np.float32)
         nq = q.shape[0]
+        if getattr(self.idx, "factors", 1) == 4:   # the non-HA Searcher, one nprobe
+            ids = np.empty((nq, k), np.uint32)
+            d = np.empty((nq, k), np.float32)
+            lib().orc_nonha_search_batch(ctypes.byref(self.view), _p(q), nq, k, int(nprobe),
+                                         _p(ids), _p(d), threads)
+            return ids, d
         npq = np.ascontiguousarray(np.broadcast_to(np.asarray(nprobe, np.uint32), (nq,)))
         ids = np.empty((nq, k), np.uint32)
         d = np.empty((nq, k), np.float32)
@@ -352,3 +377,33 @@ def knn(base, queries, k):
     d = np.empty((q.shape[0], k), np.float32)
     lib().orc_knn(_p(b), b.shape[0], _p(q), q.shape[0], b.shape[1], k, _p(ids), _p(d))
     return ids, d
+
+
+# ---------------------------------------------------------------- the non-HA searcher
+def minus_sum16(q, c):
+    q = np.ascontiguousarray(q, np.float32)
+    c = np.ascontiguousarray(c, np.float32)
+    r = np.empty_like(q)
+    return float(lib().orc_minus_sum16(_p(q), _p(c), _p(r), q.size)), r
+
+
+def scalar_quantize16(r, vl, width):
+    r = np.ascontiguousarray(r, np.float32)
+    out = np.empty(r.size, np.uint16)
+    lib().orc_scalar_quantize16(_p(out), _p(r), float(vl), float(width), r.size)
+    return out
+
+
+def pack_lut_u16(q):
+    q = np.ascontiguousarray(q, np.uint16)
+    lut = np.empty(q.size // 4 * 16, np.uint16)
+    lib().orc_pack_lut_u16(_p(lut), _p(q), q.size)
+    return lut
+
+
+def nonha_block_ints(codes, lut, D):
+    codes = np.ascontiguousarray(codes, np.uint8)
+    lut = np.ascontiguousarray(lut, np.uint16)
+    out = np.empty(32, np.int32)
+    lib().orc_nonha_block_ints(_p(codes), _p(lut), D, _p(out))
+    return out
