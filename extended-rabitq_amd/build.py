@@ -20,7 +20,7 @@ OBJ = os.path.join(PKG, "_build")
 INCLUDE = os.path.join(REPO, "include")
 
 HIP_SOURCES = ["erq_capi.hip", "erq_search.hip", "erq_gemm.hip", "erq_probe.hip", "erq_batch.hip",
-               "erq_mlp.hip", "erq_knn.hip", "erq_build.hip"]
+               "erq_mlp.hip", "erq_knn.hip", "erq_build.hip", "erq_nonha.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -227,10 +227,15 @@ extern "C" {
 
 const char* erq_last_error(void) { return g_err.c_str(); }
 
-int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint64_t* cluster_sizes,
-                     const float* P, const float* centroids, const uint8_t* short_data,
-                     const uint8_t* long_code, const float* ex_factor, const uint32_t* ids,
-                     erq_index** out) {
+}  // extern "C"
+
+namespace {
+
+// erq_index_create for either block format (factors per vector: 1 HA, 4 non-HA).
+int create_index(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint64_t* cluster_sizes,
+                 const float* P, const float* centroids, const uint8_t* short_data,
+                 const uint8_t* long_code, const float* ex_factor, const uint32_t* ids,
+                 int factors, erq_index** out) {
     return guarded([&]() -> int {
         if (!out || !cluster_sizes || !centroids || !short_data || !long_code || !ex_factor ||
             !ids) {
@@ -293,8 +298,9 @@ int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint6
         h->K = K;
         h->EX = EX_BITS;
         h->nblocks = nblocks;
+        h->factors = factors;
         h->sizes.assign(cluster_sizes, cluster_sizes + K);
-        const size_t BB = 4 * D + 128;
+        const size_t BB = 4 * D + 128 * (size_t)factors;
         int rc;
         if ((rc = upload(&h->short_data, short_data, nblocks * BB, "upload short data")) ||
             (rc = upload(&h->long_code, long_code, N * D * EX_BITS / 8, "upload long codes")) ||
@@ -325,6 +331,26 @@ int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint6
         *out = h;
         return ERQ_OK;
     });
+}
+
+}  // namespace
+
+extern "C" {
+
+int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint64_t* cluster_sizes,
+                     const float* P, const float* centroids, const uint8_t* short_data,
+                     const uint8_t* long_code, const float* ex_factor, const uint32_t* ids,
+                     erq_index** out) {
+    return create_index(N, DIM, K, EX_BITS, cluster_sizes, P, centroids, short_data, long_code,
+                        ex_factor, ids, 1, out);
+}
+
+int erq_index_create_nonha(size_t N, size_t DIM, size_t K, size_t EX_BITS,
+                           const uint64_t* cluster_sizes, const float* P, const float* centroids,
+                           const uint8_t* short_data, const uint8_t* long_code,
+                           const float* ex_factor, const uint32_t* ids, erq_index** out) {
+    return create_index(N, DIM, K, EX_BITS, cluster_sizes, P, centroids, short_data, long_code,
+                        ex_factor, ids, 4, out);
 }
 
 int erq_index_load(const char* path, erq_index** out) {
@@ -383,14 +409,24 @@ int erq_index_load(const char* path, erq_index** out) {
             set_error("erq_index_load: cluster sizes do not sum to N");
             return ERQ_EFORMAT;
         }
-        const uint64_t body = 4 * D * D + 4 * K * D + nblocks * (4 * D + 128) + N * D * EX / 8 +
-                              4 * N + 4 * N;
-        if (fsize < 32 + 8 * K + body) {
+        // The file does not record the block format: the HA one (1 factor per vector) or the
+        // non-HA one (4, Quantizer.hpp:107-121) is the one whose size the file has.
+        const uint64_t rest = 4 * D * D + 4 * K * D + N * D * EX / 8 + 4 * N + 4 * N;
+        const uint64_t body_ha = rest + nblocks * (4 * D + 128);
+        const uint64_t body_nonha = rest + nblocks * (4 * D + 512);
+        const uint64_t head = 32 + 8 * K;
+        int factors = 1;
+        if (fsize < head + body_ha) {
             set_error("erq_index_load: truncated index body");
             return ERQ_EIO;
         }
-        if (fsize > 32 + 8 * K + body) {
-            set_error("erq_index_load: bytes after the index body");
+        if (fsize == head + body_nonha && body_nonha != body_ha) {
+            factors = 4;
+        } else if (fsize != head + body_ha) {
+            set_error(fsize < head + body_nonha
+                          ? "erq_index_load: bytes after the index body, or a truncated non-HA "
+                            "index (the size matches neither block format)"
+                          : "erq_index_load: bytes after the index body");
             return ERQ_EFORMAT;
         }
         if (K > kMaxK || D > kMaxD) {
@@ -398,7 +434,7 @@ int erq_index_load(const char* path, erq_index** out) {
             return ERQ_EUNSUPPORTED;
         }
         std::vector<float> P(D * D), C(K * D), ef(N);
-        std::vector<uint8_t> sd(nblocks * (4 * D + 128)), lc(N * D * EX / 8);
+        std::vector<uint8_t> sd(nblocks * (4 * D + 128 * (uint64_t)factors)), lc(N * D * EX / 8);
         std::vector<uint32_t> ids(N);
         bool ok = in.read(reinterpret_cast<char*>(P.data()), P.size() * 4) &&
                   in.read(reinterpret_cast<char*>(C.data()), C.size() * 4) &&
@@ -410,8 +446,8 @@ int erq_index_load(const char* path, erq_index** out) {
             set_error("erq_index_load: truncated index body");
             return ERQ_EIO;
         }
-        return erq_index_create(N, DIM, K, EX, sizes.data(), P.data(), C.data(), sd.data(),
-                                lc.data(), ef.data(), ids.data(), out);
+        return create_index(N, DIM, K, EX, sizes.data(), P.data(), C.data(), sd.data(),
+                            lc.data(), ef.data(), ids.data(), factors, out);
     });
 }
 
@@ -422,6 +458,7 @@ size_t erq_num_clusters(const erq_index* h) { return h ? h->K : 0; }
 size_t erq_dim(const erq_index* h) { return h ? h->DIM : 0; }
 size_t erq_num_vectors(const erq_index* h) { return h ? h->N : 0; }
 size_t erq_ex_bits(const erq_index* h) { return h ? h->EX : 0; }
+size_t erq_short_factors(const erq_index* h) { return h ? (size_t)h->factors : 0; }
 
 const void* erq_device_array(const erq_index* h, int which) {
     if (!h) return nullptr;
@@ -643,7 +680,8 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
                        float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0,
                        uint32_t* order_buf = nullptr, size_t own_lists = 0) {
     ScanArgs a = scan_args(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, max_probes);
-    if (h->plain_kernel != 1) {  // auto or MFMA: bit-plane integer path
+    const bool nonha = h->factors != 1;  // Searcher's format: the LUT kernel of erq_nonha.hip
+    if (h->plain_kernel != 1 && !nonha) {  // auto or MFMA: bit-plane integer path
         if (!order_buf) {
             int rc = ensure_bp(h, s);
             if (rc) return rc;
@@ -689,7 +727,7 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
         a.qorder = ob + h->K + 1;
     }
     StageTimer tim(h, ERQ_STAGE_REPLAY, s);
-    hipError_t e = launch_scan(a, s);
+    hipError_t e = nonha ? launch_scan_nonha(a, s) : launch_scan(a, s);
     if (e != hipSuccess) return hip_fail(e, "scan");
     return ERQ_OK;
 }
@@ -1010,10 +1048,21 @@ static size_t batch_ws_bytes(const erq_index* h, size_t nq, size_t max_nprobe, i
     return ws.off + 256;
 }
 
+// The batched rounds read the HA block layout (and its bit-plane copy): a non-HA index is
+// searched by the per-query scan only.
+static int require_ha(const erq_index* h, const char* what) {
+    if (h->factors == 1) return ERQ_OK;
+    set_error(std::string(what) +
+              ": the batched search reads the HA block format; this index is non-HA (4 factors "
+              "per vector): use the per-query search");
+    return ERQ_EUNSUPPORTED;
+}
+
 static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k,
                           const uint32_t* off, const uint32_t* pid, const float* pd,
                           size_t max_nprobe, uint32_t* ids, float* dists, uint64_t* stats,
                           hipStream_t s, char* ws_base, size_t ws_off) {
+    if (int rc = require_ha(h, "batched search")) return rc;
     int rc = ensure_bp(h, s);
     if (rc) return rc;
     // ranks [0, kExactRanks): the exact per-query scan of each query's nearest clusters
@@ -1185,6 +1234,7 @@ int erq_scan_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t
             set_error("erq_scan_batched_dev: k must be in [1, 1024]");
             return ERQ_EINVAL;
         }
+        if (int rc = require_ha(h, "erq_scan_batched_dev")) return rc;
         if (nq == 0) return ERQ_OK;
         if (nq * std::max<size_t>(max_nprobe, 1) >= (1u << 27)) {
             set_error("erq_scan_batched_dev: nq * max_nprobe must stay below 2^27");
@@ -1212,6 +1262,7 @@ int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size
             set_error("erq_search_batched_dev: k must be in [1, 1024]");
             return ERQ_EINVAL;
         }
+        if (int rc = require_ha(h, "erq_search_batched_dev")) return rc;
         if (nprobe == 0 || nprobe > h->K || nprobe > 2048) {
             set_error("erq_search_batched_dev: nprobe must be in [1, min(K, 2048)]");
             return ERQ_EINVAL;
@@ -1325,6 +1376,7 @@ int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, siz
         int rc = learned_args_ok(h, m, min_nprobe, max_nprobe, fallback_nprobe,
                                  "erq_search_learned_dev");
         if (rc || nq == 0) return rc;
+        if ((rc = require_ha(h, "erq_search_learned_dev"))) return rc;
         if (nq * max_nprobe >= (1u << 27)) {
             set_error("erq_search_learned_dev: nq * max_nprobe must stay below 2^27");
             return ERQ_EUNSUPPORTED;
@@ -1362,13 +1414,15 @@ int erq_search_learned_dev(erq_index* h, erq_mlp* m, const float* q_rot_dev, siz
 // counts per query (np_host, nullable: every query takes `nprobe`).
 static int host_search(erq_index* h, const float* q_rot, size_t nq, size_t k, size_t nprobe,
                        const uint32_t* np_host, unsigned flags, uint32_t* results, float* dists) {
+    const bool batched = (flags & ERQ_SEARCH_BATCHED) != 0;
+    if (batched)
+        if (int rc = require_ha(h, "erq_search_batch (ERQ_SEARCH_BATCHED)")) return rc;
     hipStream_t s = h->stream;
     Call call(h, s);
     int rc = host_bufs(h, nq, k);
     if (rc) return rc;
     hipError_t e = hipMemcpyAsync(h->hq, q_rot, nq * h->D * 4, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "copy queries");
-    const bool batched = (flags & ERQ_SEARCH_BATCHED) != 0;
     if (!np_host) {
         rc = batched ? erq_search_batched_dev(h, h->hq, nq, k, nprobe, h->hids, h->hdists, nullptr, s)
                      : erq_search_dev(h, h->hq, nq, k, nprobe, h->hids, h->hdists, nullptr, s);
@@ -1597,7 +1651,7 @@ int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size
             }
         }
         h->plain_waves = bw;
-        for (int wg = 1; wg <= 4 && !rc; wg++) {
+        for (int wg = 1; wg <= 4 && !rc && h->factors == 1; wg++) {
             h->batch_wg_per_cu = wg;
             float ms;
             rc = time_it(true, &ms);

@@ -16,9 +16,12 @@ struct erq_index {
     hipStream_t stream = nullptr;
     size_t N = 0, DIM = 0, D = 0, K = 0, EX = 0;
     size_t nblocks = 0;
+    // short factors per vector: 1 (the HA format, HIGH_ACC_FAST_SCAN) or 4 (the non-HA
+    // format that Searcher reads, Quantizer.hpp:107-121); a block is 4D + 128 * factors bytes
+    int factors = 1;
     std::vector<uint64_t> sizes;
     // resident arrays (HA layout, reference inc/index/IVF.hpp:28-40)
-    uint8_t* short_data = nullptr;  // nblocks * (4D + 128)
+    uint8_t* short_data = nullptr;  // nblocks * (4D + 128 * factors)
     uint8_t* long_code = nullptr;   // N * D*EX/8
     float* ex_factor = nullptr;     // N
     uint32_t* ids = nullptr;        // N
@@ -139,6 +142,9 @@ struct ScanArgs {
     uint32_t* est_off;
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
+// The per-query scan of a non-HA index (Searcher.hpp:85-236): the LUT kernel of erq_nonha.hip
+// with a.short_data in the 4-factor block layout (a.bp unused).
+hipError_t launch_scan_nonha(const ScanArgs& a, hipStream_t s);
 // The round-mode rescan (a.qlist set): a fixed grid of waves that take the listed queries.
 hipError_t launch_scan_round(const ScanArgs& a, hipStream_t s);
 // The low-latency scan (a.est_* set, a.bp set, nchunk <= 4): slot offsets, the per-vector

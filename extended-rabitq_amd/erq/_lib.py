@@ -21,6 +21,8 @@ SIGNATURES = {
     "erq_index_load": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_vp)]),
     "erq_index_create": (c_int, [c_size, c_size, c_size, c_size, c_vp, c_vp, c_vp, c_vp, c_vp,
                                  c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    "erq_index_create_nonha": (c_int, [c_size, c_size, c_size, c_size, c_vp, c_vp, c_vp, c_vp,
+                                       c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
     "erq_index_free": (None, [c_vp]),
     "erq_set_option": (c_int, [c_vp, c_int, ctypes.c_long]),
     "erq_get_option": (ctypes.c_long, [c_vp, c_int]),
@@ -30,6 +32,7 @@ SIGNATURES = {
     "erq_dim": (c_size, [c_vp]),
     "erq_num_vectors": (c_size, [c_vp]),
     "erq_ex_bits": (c_size, [c_vp]),
+    "erq_short_factors": (c_size, [c_vp]),
     "erq_rotate": (c_int, [c_vp, c_vp, c_size, c_vp]),
     "erq_search": (c_int, [c_vp, c_vp, c_size, c_size, c_vp, c_vp]),
     "erq_search_batch": (c_int, [c_vp, c_vp, c_size, c_size, c_size, c_vp, c_vp]),

@@ -48,8 +48,9 @@ class IVF:
                 np.ascontiguousarray(idx.ex_factor, dtype=np.float32),
                 np.ascontiguousarray(idx.ids, dtype=np.uint32)]
         h = ctypes.c_void_p()
-        check(_lib.lib().erq_index_create(idx.N, idx.DIM, idx.K, idx.EX,
-                                          *[_ptr(a) for a in arrs], ctypes.byref(h)))
+        create = (_lib.lib().erq_index_create if getattr(idx, "factors", 1) == 1
+                  else _lib.lib().erq_index_create_nonha)
+        check(create(idx.N, idx.DIM, idx.K, idx.EX, *[_ptr(a) for a in arrs], ctypes.byref(h)))
         return IVF(h)
 
     def close(self):
@@ -107,6 +108,10 @@ class IVF:
 
     def ex_bits(self):
         return _lib.lib().erq_ex_bits(self._h)
+
+    def high_acc(self):
+        """True for the HA block format (HIGH_ACC_FAST_SCAN), False for the non-HA one."""
+        return _lib.lib().erq_short_factors(self._h) == 1
 
     # ---- host-pointer path (Rotator::rotate, IVF::search) ----
     def rotate(self, q):

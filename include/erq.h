@@ -91,9 +91,9 @@ int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size
 /* Message of the last error on the calling thread ("" if none). */
 const char* erq_last_error(void);
 
-/* Replaces IVF::load (IVF.hpp:254-293): reads an HA-format index file
- * (HIGH_ACC_FAST_SCAN layout, 1 factor per vector) and uploads it to device 0
- * (or the current HIP device). */
+/* Replaces IVF::load (IVF.hpp:254-293): reads an index file and uploads it to the current
+ * HIP device.  The file does not record its block format; the HA one (HIGH_ACC_FAST_SCAN,
+ * 1 factor per vector) or the non-HA one (4 factors) is the one whose size the file has. */
 int erq_index_load(const char* path, erq_index** out);
 
 /* The same arrays IVF::load reads (IVF.hpp:261-286), from host memory. */
@@ -101,6 +101,18 @@ int erq_index_create(size_t N, size_t DIM, size_t K, size_t EX_BITS, const uint6
                      const float* P, const float* centroids, const uint8_t* short_data,
                      const uint8_t* long_code, const float* ex_factor, const uint32_t* ids,
                      erq_index** out);
+
+/* erq_index_create for the non-HA format: the index IVF::construct writes when built without
+ * HIGH_ACC_FAST_SCAN (Quantizer.hpp:33-37,107-121), whose blocks hold 4D code bytes and four
+ * factors per vector (||o-c||^2, fac_ip, sumxb, fac_err; 4D + 512 bytes per 32 vectors), and
+ * which IVF::search scans with Searcher (Searcher.hpp:14-236).  The per-query searches
+ * (erq_search, erq_search_batch, erq_search_batch_nprobe without ERQ_SEARCH_BATCHED,
+ * erq_search_dev, erq_scan_dev, erq_search_dev_noalloc) run its scan; the batched and learned
+ * searches return ERQ_EUNSUPPORTED.  erq_index_load infers the format from the file size. */
+int erq_index_create_nonha(size_t N, size_t DIM, size_t K, size_t EX_BITS,
+                           const uint64_t* cluster_sizes, const float* P, const float* centroids,
+                           const uint8_t* short_data, const uint8_t* long_code,
+                           const float* ex_factor, const uint32_t* ids, erq_index** out);
 
 /* Replaces IVF::~IVF (IVF.hpp:107). */
 void erq_index_free(erq_index* h);
@@ -111,6 +123,8 @@ size_t erq_num_clusters(const erq_index* h);
 size_t erq_dim(const erq_index* h);
 size_t erq_num_vectors(const erq_index* h);
 size_t erq_ex_bits(const erq_index* h);
+/* Short factors per vector: 1 (HA format) or 4 (non-HA); Quantizer.hpp:33-37. */
+size_t erq_short_factors(const erq_index* h);
 
 /* Replaces Rotator::rotate on zero-padded queries (Rotator.hpp:60-63 as used by
  * test_search.cpp:83-92): q is nq x DIM, q_rot is nq x D (host pointers). */

@@ -256,6 +256,9 @@ class OracleIndex:
 
     def search(self, q_rot, k, nprobe, stats=False):
         q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)
+        if getattr(self.idx, "factors", 1) == 4:   # the non-HA Searcher (IVF.hpp:295-324)
+            pid, pd = self.centroid_topn(q, min(nprobe, self.idx.K))
+            return self.search_probes(q, k, pid, pd, stats=stats)
         ids = np.empty(k, np.uint32)
         d = np.empty(k, np.float32)
         st = OrcStats()
@@ -281,6 +284,8 @@ class OracleIndex:
         return ids, d
 
     def search_rounds(self, q_rot, k, probe_ids, probe_d, stats=False):
+        if getattr(self.idx, "factors", 1) != 1:
+            raise ValueError("the batched rounds are defined for the HA format only")
         q = np.ascontiguousarray(q_rot, np.float32).reshape(-1)
         pid = np.ascontiguousarray(probe_ids, np.uint32)
         pd = np.ascontiguousarray(probe_d, np.float32)

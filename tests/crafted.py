@@ -65,3 +65,63 @@ def wrap_index(n_per_cluster=70, K=6, ex=3, seed=11, nq=3):
     idx = index.IVFIndex(N, D, K, ex, sizes, P, centroids, short, longc, xip, ids)
     idx.validate()
     return idx, queries, bits
+
+
+def wrap_index_nonha(n_per_cluster=70, K=6, ex=3, seed=12, nq=3):
+    """A non-HA index (Searcher.hpp, four factors per vector) at D = 3072 whose FastScan
+    half-sums wrap for some vectors and not for others (accumulate_robust, FastScan.hpp:
+    136-212).  Every residual q - c_k is a positive multiple of one all-positive direction, so
+    every digit is (about) 16383 and a codebook whose nibble selects n >= 1 dims has LUT
+    entry n * 16383: low byte 256 - n, high byte 64 n - 1.  Per parity (384 codebooks) the
+    low-byte sum wraps once about 2/3 of the codebooks select something, the high-byte sum
+    once the selection rate passes 2/3; rates are drawn per vector and per parity.  Factors
+    are drawn in plausible ranges (sumxb is the vector's popcount)."""
+    from erq import index
+    from oracle import oracle as orc
+
+    rng = np.random.default_rng(seed)
+    D = 3072
+    direction = rng.uniform(0.5, 1.0, D)
+    direction = (direction / np.linalg.norm(direction)).astype(np.float32)
+    eps = 0.1 * np.arange(K)
+    centroids = (-eps[:, None] * direction[None, :]).astype(np.float32)
+    t = np.array([1.0, 0.7, 1.6][:nq])
+    queries = (t[:, None] * direction[None, :]).astype(np.float32)
+
+    N = n_per_cluster * K
+    bits = np.empty((N, D), np.uint8)
+    rates = np.array([0.05, 0.15, 0.25, 0.35, 0.6, 0.75, 0.9])
+    even = (np.arange(D) // 4) % 2 == 0
+    for v in range(N):
+        r_even, r_odd = rng.choice(rates, 2)
+        bits[v] = rng.random(D) < np.where(even, r_even, r_odd)
+    x2 = rng.uniform(0.05, 1.5, N).astype(np.float32)
+    fip = rng.uniform(0.5, 2.0, N).astype(np.float32)
+    ferr = rng.uniform(0.0, 0.05, N).astype(np.float32)
+    sumxb = bits.sum(1).astype(np.float32)
+    raw = rng.integers(0, 1 << ex, (N, D)).astype(np.uint8)
+    xip = rng.uniform(0.004, 0.012, N).astype(np.float32)
+
+    sizes = np.full(K, n_per_cluster, np.uint64)
+    nb = (n_per_cluster + 31) // 32
+    BB = 4 * D + 512
+    short = np.zeros(K * nb * BB, np.uint8)
+    clen = D * ex // 8
+    longc = np.empty(N * clen, np.uint8)
+    for k in range(K):
+        for b in range(nb):
+            v0 = k * n_per_cluster + 32 * b
+            n = min(32, n_per_cluster - 32 * b)
+            blk = (k * nb + b) * BB
+            short[blk:blk + 4 * D] = orc.pack_block_codes(bits[v0:v0 + n], D)
+            f = np.zeros((4, 32), np.float32)
+            for r, arr in enumerate((x2, fip, sumxb, ferr)):
+                f[r, :n] = arr[v0:v0 + n]
+            short[blk + 4 * D:blk + BB] = f.reshape(-1).view(np.uint8)
+    for v in range(N):
+        longc[v * clen:(v + 1) * clen] = orc.ex_encode(raw[v], ex)
+    ids = rng.permutation(N).astype(np.uint32) + 1000
+    P = np.eye(D, dtype=np.float32)
+    idx = index.IVFIndex(N, D, K, ex, sizes, P, centroids, short, longc, xip, ids, 4)
+    idx.validate()
+    return idx, queries, bits

@@ -124,3 +124,21 @@ def test_batch_nprobe_rejects_bad_counts():
     # a null handle is rejected before anything else is read
     assert L.erq_search_batch_nprobe(None, None, 2, 10, npq.ctypes.data_as(ctypes.c_void_p),
                                      None, None, 0) == 1
+
+
+def test_nonha_file_size_checks(small_data, tmp_path):
+    """The file does not record its block format: erq_index_load takes the one whose size the
+    file has.  A non-HA file cut short, or one with bytes appended, matches neither and is
+    refused before anything is uploaded."""
+    from erq import index
+    x, q, c, assign = small_data
+    idx = index.build_index(x[:2000], c, assign[:2000], bits=4, seed=3, high_acc=False)
+    p = tmp_path / "n.index"
+    idx.save(str(p))
+    data = p.read_bytes()
+    p.write_bytes(data[:-100])
+    rc, msg = _load_rc(str(p))
+    assert rc == 3 and "neither block format" in msg
+    p.write_bytes(data + b"\0" * 4)
+    rc, msg = _load_rc(str(p))
+    assert rc == 3 and "after the index body" in msg
