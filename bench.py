@@ -145,13 +145,33 @@ def interpolate(sweep, target):
 class Runner:
     """The workload on the GPU: the index handle, queries, ground truth and timing helpers."""
 
-    def __init__(self, a, g, idx, q, gt, torch):
+    def __init__(self, a, g, idx, q, gt, torch, x=None):
         self.a, self.g, self.idx, self.q, self.gt, self.torch = a, g, idx, q, gt, torch
+        self.x = x
+        self.gt_dev = torch.from_numpy(gt.astype(np.int64)).cuda() if gt is not None else None
         self.stream = torch.cuda.current_stream()
 
     def recall(self, ids):
         from erq import data
         return data.recall_at_k(ids.cpu().numpy().view(np.uint32), self.gt, self.a.k)
+
+    def ratio(self, ids, rows=None):
+        """test_search.cpp's average distance ratio of the results ids [nq, k] (the first
+        `rows` queries), true squared distances from the base vectors in HBM."""
+        from erq import data
+        torch = self.torch
+        n = ids.shape[0] if rows is None else rows
+        if isinstance(ids, np.ndarray):
+            ids = torch.from_numpy(ids.astype(np.int64))
+        ids = ids.to(device="cuda", dtype=torch.int64)
+        tot = 0.0
+        for s in range(0, n, 2000):
+            e = min(n, s + 2000)
+            qq = self.q[s:e, None, :]
+            ann = ((self.x[ids[s:e]] - qq) ** 2).sum(-1)
+            gtd = ((self.x[self.gt_dev[s:e]] - qq) ** 2).sum(-1)
+            tot += data.distance_ratio(gtd, ann) * (e - s)
+        return tot / n
 
     def timed(self, fn, steps=None, warmup=None):
         """W untimed and K timed calls of fn, synchronized on both sides of the timed
@@ -198,11 +218,13 @@ def fixed_path(r, path):
         ids, _ = step(npb)
         torch.cuda.synchronize()
         rec = r.recall(ids)
-        sweep.append([npb, rec, None])
-        log(f"[{path}] nprobe={npb:4d} recall@{a.k}={rec:.4f}")
+        rat = r.ratio(ids)
+        sweep.append([npb, rec, rat])
+        log(f"[{path}] nprobe={npb:4d} recall@{a.k}={rec:.4f} ratio={rat:.5f}")
         if rec >= max(TARGETS) or rec >= 0.9995:
             break
-    out = {"sweep": [[n, round(x, 4)] for n, x, _ in sweep], "targets": {}}
+    # the reference's CSV columns: nprobe, recall, ratio (QPS in the targets)
+    out = {"sweep": [[n, round(x, 4), round(t, 5)] for n, x, t in sweep], "targets": {}}
     timed = {}
 
     def ms_at(npb):
@@ -304,11 +326,13 @@ def learned_path(r, m, max_np, fixed_np):
             else:
                 lo = mid
         ms = r.timed(lambda: step(hi))
+        ids, _ = step(hi)
+        rat = r.ratio(ids)
         off, _, _, nfb = g.probe_learned_dev(m, g.rotate_dev(q), hi, 4, max_np, fixed_np, 0.5)
         mean_np = float(np.diff(off.cpu().numpy()).mean())
         out["targets"][str(tgt)] = {"qps": round(a.nq / ms * 1e3, 1), "ms": round(ms, 3),
                                     "mass": hi, "recall": round(cache[round(hi, 6)], 4),
-                                    "mean_nprobe": round(mean_np, 2),
+                                    "ratio": round(rat, 5), "mean_nprobe": round(mean_np, 2),
                                     "fallback_queries": int(nfb[0])}
         log(f"[learned] target {tgt}: mass {hi} recall {cache[round(hi, 6)]:.4f} "
             f"{ms:.3f} ms ({a.nq / ms * 1e3:.0f} q/s), mean nprobe {mean_np:.1f}")
@@ -416,39 +440,74 @@ def plain_roofline(r, nprobe):
     return out
 
 
-def cpu_baseline(idx, q, a, nprobe):
-    """The CPU oracle (a scalar C port of the reference search), single thread, timed on
-    a bounded sample of the same queries at the same nprobe: QPS on this host."""
+def cpu_baseline(r, plain):
+    """The reference's test_search protocol (test_search.cpp:83-150) on this host's CPU, run by
+    the AVX-512 restatement of the HA search (oracle/erq_cpu512.cpp, bit-identical to the
+    oracle): a bounded sample of the same queries, zero-padded and rotated by one single-thread
+    GEMM (timed); at each nprobe of the plain path's bracket around the recall target, every
+    query searched and timed on its own on one thread, QPS = n / (summed search time + rotation
+    time), over 3 rounds averaged; QPS at the target interpolated between the two nprobes with
+    the full batch's recall (the GPU per-query scan returns the same results).  Then the same
+    with OpenMP over queries on every host thread (wall clock)."""
+    from erq import data
     try:
         from oracle import oracle as orc
     except Exception as e:  # pragma: no cover
         return {"error": str(e)}
+    a, idx = r.a, r.idx
+    tgt = plain["targets"].get(str(a.target))
+    if not tgt:
+        return {"error": "the plain path does not reach the recall target"}
+    lo_np, hi_np = tgt["nprobe_bracket"]
+    lo_rec, hi_rec = tgt["recall_bracket"]
+    nps = [n for n in (lo_np, hi_np) if n]
+    f = 1.0 if lo_np is None else (a.target - lo_rec) / (hi_rec - lo_rec)
     oi = orc.OracleIndex(idx)
-    qr = q[:512].cpu().numpy().astype(np.float64)
-    qp = np.zeros((qr.shape[0], idx.D))
-    qp[:, :qr.shape[1]] = qr
-    qr = (qp @ idx.P.astype(np.float64)).astype(np.float32)
-    n = 0
-    t0 = time.perf_counter()
-    while n < qr.shape[0] and time.perf_counter() - t0 < a.cpu_seconds:
-        oi.search(qr[n], a.k, nprobe)
-        n += 1
-    dt = time.perf_counter() - t0
-    log(f"cpu baseline: {n} queries in {dt:.1f}s (1 thread)")
-    out = {"value": round(n / dt, 2), "unit": "queries/s", "cores": 1, "kind": "port",
-           "sample": f"{n} queries of the same batch, nprobe={nprobe}, k={a.k}, single thread, "
-                     f"scalar C oracle"}
+    if orc.cpu512() is None:
+        return {"error": "no AVX-512 on this host (or liberq_cpu512.so missing)",
+                "host": host_info()}
+    qs = r.q[:4000].cpu().numpy()
+    qr16, _ = orc.cpu512_rotate(qs[:16], idx.P)          # warm-up and sample sizing
+    _, _, us16 = orc.cpu512_search(oi, qr16, a.k, hi_np)
+    per_q = us16 / 16
+    n = int(min(qs.shape[0], max(64, a.cpu_seconds * 1e6 / (3 * len(nps) * per_q))))
+    qr, rot_us = orc.cpu512_rotate(qs[:n], idx.P)
+    at = {}
+    for npb in nps:
+        runs = []
+        for _ in range(3):
+            ids, _, us = orc.cpu512_search(oi, qr, a.k, npb, threads=1)
+            runs.append(n / ((us + rot_us) / 1e6))
+        at[npb] = {"qps": round(float(np.mean(runs)), 2),
+                   "qps_rounds": [round(v, 2) for v in runs],
+                   "recall_sample": round(data.recall_at_k(ids, r.gt[:n], a.k), 4),
+                   "ratio_sample": round(r.ratio(ids, n), 5)}
+        log(f"cpu baseline (AVX-512, 1 thread): nprobe {npb}, {n} queries x 3 rounds: "
+            f"{at[npb]['qps']} q/s")
+    qhi = at[hi_np]["qps"]
+    qlo = at[lo_np]["qps"] if lo_np else qhi
     threads = min(len(os.sched_getaffinity(0)),
                   int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 10**6)
-    mm = int(min(qr.shape[0], max(threads, n / dt * threads * 2.0)))
-    t0 = time.perf_counter()
-    oi.search_batch(qr[:mm], a.k, nprobe, threads=threads)
-    dt = time.perf_counter() - t0
-    log(f"cpu baseline: {mm} queries in {dt:.1f}s ({threads} threads)")
-    out["all_cores"] = {"value": round(mm / dt, 2), "cores": threads,
-                        "sample": f"{mm} queries, OpenMP over queries"}
-    out["host"] = host_info()
-    return out
+    mt = {}
+    for npb in nps:
+        ids, _, us = orc.cpu512_search(oi, qr, a.k, npb, threads=threads)
+        mt[npb] = round(n / ((us + rot_us / threads) / 1e6), 2)
+    mlo = mt[lo_np] if lo_np else mt[hi_np]
+    log(f"cpu baseline (AVX-512, {threads} threads): {mt}")
+    return {"value": round(qlo + f * (qhi - qlo), 2), "unit": "queries/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{n} queries of the same batch, test_search protocol (rotation timed, "
+                      f"per-query search times summed, 3 rounds averaged) at nprobe {nps}, "
+                      f"QPS interpolated to recall {a.target} with the full batch's recall "
+                      f"{[lo_rec, hi_rec]}; AVX-512 restatement of the reference search, "
+                      f"bit-identical to the oracle, one thread",
+           "rotation_us_per_query": round(rot_us / n, 2), "at_nprobe": at,
+           "all_cores": {"value": round(mlo + f * (mt[hi_np] - mlo), 2), "cores": threads,
+                         "at_nprobe": mt, "sample": f"the same {n} queries, OpenMP over "
+                                                    f"queries, wall clock; the rotation GEMM's "
+                                                    f"single-thread time divided by the thread "
+                                                    f"count"},
+           "host": host_info()}
 
 
 def drop_in_latency(r, nprobe, nq=256):
@@ -550,9 +609,8 @@ def main(argv=None):
     del xs
     log(f"ground truth (erq_knn_dev) in {a.gt_s:.1f}s, FP64 spot-check agreement "
         f"{spot:.4f}")
-    del x
     torch.cuda.empty_cache()
-    r = Runner(a, g, idx, q, gt, torch)
+    r = Runner(a, g, idx, q, gt, torch, x)   # x stays resident for the distance ratios
 
     if a.sweep_only:
         for path in ("plain", "batched"):
@@ -603,7 +661,7 @@ def main(argv=None):
     plain_roof = plain_roofline(r, np_hi)
     latency = drop_in_latency(r, np_hi)
     log(f"[drop-in] erq_search per query at nprobe {np_hi}: {latency}")
-    cpu = None if a.no_cpu else cpu_baseline(idx, q, a, np_hi)
+    cpu = None if a.no_cpu else cpu_baseline(r, plain)
 
     if headline:
         value, ms_step = headline["qps"], headline["ms"]

@@ -141,3 +141,20 @@ def recall_at_k(result_ids, gt_ids, k):
     for r, g in zip(np.asarray(result_ids), np.asarray(gt_ids)):
         tot += len(set(r[:k].tolist()) & set(g[:k].tolist()))
     return tot / (len(result_ids) * k)
+
+
+def distance_ratio(gt_sqr, ann_sqr):
+    """test_search.cpp's average distance ratio (get_ratio, tools.hpp:16-51; the harness divides
+    the summed per-query values by NQ * k): per query, its k true and k returned squared
+    distances each sorted and paired rank by rank, the mean over pairs whose true distance
+    exceeds 1e-5 of sqrt(returned / true) (1 when no pair counts), averaged over queries.
+    gt_sqr, ann_sqr: [nq, k] torch tensors (any device); returns a float."""
+    import torch
+    g = torch.sort(gt_sqr.double(), dim=1).values
+    a = torch.sort(ann_sqr.double(), dim=1).values
+    valid = g > 1e-5
+    r = torch.where(valid, torch.sqrt(a / torch.where(valid, g, torch.ones_like(g))),
+                    torch.zeros_like(g))
+    n = valid.sum(1)
+    per_q = torch.where(n > 0, r.sum(1) / n.clamp(min=1), torch.ones_like(r[:, 0]))
+    return float(per_q.mean())

@@ -412,3 +412,62 @@ def nonha_block_ints(codes, lut, D):
     out = np.empty(32, np.int32)
     lib().orc_nonha_block_ints(_p(codes), _p(lut), D, _p(out))
     return out
+
+
+# ---------------------------------------------------------------- the AVX-512 CPU baseline
+_c512 = None
+
+
+def cpu512():
+    """liberq_cpu512.so: the same HA search in AVX-512 (bench.py's CPU baseline), None when the
+    library is missing or this CPU lacks AVX-512 F/BW/DQ/VL."""
+    global _c512
+    if _c512 is None:
+        path = os.path.join(HERE, "liberq_cpu512.so")
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path)
+        L.c512_available.restype = ctypes.c_int
+        L.c512_available.argtypes = []
+        if not L.c512_available():
+            return None
+        L.c512_rotate.restype = ctypes.c_double
+        L.c512_rotate.argtypes = [c_vp, c_size, c_size, c_vp, c_vp]
+        L.c512_search_batch.restype = ctypes.c_double
+        L.c512_search_batch.argtypes = [ctypes.POINTER(OrcIndex), c_vp, c_size, c_size, c_size,
+                                        c_vp, c_vp, ctypes.c_int]
+        L.c512_ratio.restype = ctypes.c_double
+        L.c512_ratio.argtypes = [c_vp, c_vp, c_size]
+        _c512 = L
+    return _c512
+
+
+def cpu512_rotate(q, P):
+    """(q zero-padded to D) @ P on one thread; returns (rotated, microseconds)."""
+    D = P.shape[0]
+    qp = np.zeros((q.shape[0], D), np.float32)
+    qp[:, :q.shape[1]] = q
+    Pc = np.ascontiguousarray(P, np.float32)
+    out = np.empty_like(qp)
+    us = cpu512().c512_rotate(_p(qp), qp.shape[0], D, _p(Pc), _p(out))
+    return out, us
+
+
+def cpu512_search(oi, q_rot, k, nprobe, threads=1):
+    """IVF::search of every row of q_rot on the AVX-512 path: (ids, dists, microseconds) --
+    summed per-query times on one thread, wall time with threads > 1."""
+    q = np.ascontiguousarray(q_rot, np.float32)
+    nq = q.shape[0]
+    ids = np.empty((nq, k), np.uint32)
+    d = np.empty((nq, k), np.float32)
+    us = cpu512().c512_search_batch(ctypes.byref(oi.view), _p(q), nq, k, int(nprobe), _p(ids),
+                                    _p(d), int(threads))
+    return ids, d, us
+
+
+def ratio(gt_sqr, ann_sqr):
+    """get_ratio (tools.hpp:16-51) of one query from its k true and k returned squared
+    distances."""
+    g = np.ascontiguousarray(gt_sqr, np.float32)
+    a = np.ascontiguousarray(ann_sqr, np.float32)
+    return cpu512().c512_ratio(_p(g), _p(a), g.size)
