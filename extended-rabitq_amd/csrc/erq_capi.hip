@@ -1298,7 +1298,7 @@ int erq_search_batched_dev(erq_index* h, const float* q_rot_dev, size_t nq, size
 
 static size_t learned_ws_bytes(size_t nq, size_t K, size_t max_np) {
     return 8 * 256 + align_up(nq * K * 4, 256) + 2 * align_up(nq * max_np * 4, 256) +
-           align_up(nq * 4, 256);
+           2 * align_up(nq * 4, 256) + 256;
 }
 
 static int learned_locked(erq_index* h, erq_mlp* m, const float* q_rot_dev, size_t nq,
@@ -1311,12 +1311,14 @@ static int learned_locked(erq_index* h, erq_mlp* m, const float* q_rot_dev, size
     uint32_t* pad_ids = ws.take<uint32_t>(nq * max_np);
     float* pad_d = ws.take<float>(nq * max_np);
     uint32_t* len = ws.take<uint32_t>(nq);
+    uint32_t* fb_list = ws.take<uint32_t>(nq);
+    uint32_t* fb_count = ws.take<uint32_t>(1);
     int rc = erq_mlp_forward_dev(m, q_rot_dev, nq, prob, s);
     if (rc) return rc;
     hipError_t e = launch_adaptive_probe(prob, q_rot_dev, h->centroids, (int)nq, (int)h->K,
                                          (int)h->D, mass, (int)min_np, (int)max_np, min_conf,
                                          (int)fb_np, pad_ids, pad_d, len, off, ids, sqr_y,
-                                         n_fallback, s);
+                                         n_fallback, fb_list, fb_count, s);
     if (e != hipSuccess) return hip_fail(e, "learned probe selection");
     return ERQ_OK;
 }

@@ -180,10 +180,13 @@ hipError_t launch_compact_probes(const uint32_t* pad_ids, const float* pad_d, in
                                  const uint32_t* off, int nq, uint32_t* csr_ids, float* csr_d,
                                  hipStream_t s);
 
-hipError_t launch_adaptive_probe(const float* prob, const float* q_rot, const float* centroids,
+// prob (nq x K) is overwritten for the queries that take the fixed list (their exact
+// distances); fb_list (nq) and fb_count (1) are scratch.
+hipError_t launch_adaptive_probe(float* prob, const float* q_rot, const float* centroids,
                                  int nq, int K, int D, float mass, int min_np, int max_np,
                                  float min_conf, int fb_np, uint32_t* pad_ids, float* pad_d,
                                  uint32_t* len, uint32_t* off, uint32_t* csr_ids, float* csr_d,
-                                 uint32_t* n_fallback, hipStream_t s);
+                                 uint32_t* n_fallback, uint32_t* fb_list, uint32_t* fb_count,
+                                 hipStream_t s);
 
 }  // namespace erq

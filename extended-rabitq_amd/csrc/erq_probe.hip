@@ -10,6 +10,8 @@
 //   4. recomputes L2Sqr16 for those in the reference's order, sorts by (distance, id)
 //      and returns the first nprobe.
 // Equal distances are ordered by id (std::partial_sort leaves their order unspecified).
+#include <algorithm>
+
 #include "erq_device.h"
 #include "erq_internal.h"
 
@@ -245,7 +247,8 @@ __global__ void __launch_bounds__(kThreads)
                            const float* __restrict__ centroids, int K, int D, float mass,
                            int min_np, int max_np, float min_conf, int fb_np,
                            uint32_t* __restrict__ out_ids, float* __restrict__ out_d,
-                           uint32_t* __restrict__ out_len, uint32_t* __restrict__ n_fallback) {
+                           uint32_t* __restrict__ out_len, uint32_t* __restrict__ n_fallback,
+                           uint32_t* __restrict__ fb_list, uint32_t* __restrict__ fb_count) {
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* keys = reinterpret_cast<uint32_t*>(smem);
     float* cd = reinterpret_cast<float*>(smem + 4 * (size_t)K);
@@ -272,20 +275,11 @@ __global__ void __launch_bounds__(kThreads)
     for (int w = 0; w < kThreads / 64; w++) total += sh_sum[w];
 
     if (total < min_conf) {  // uniform
-        for (int base = 0; base < K; base += kThreads / 16) {
-            const int c = base + grp;
-            const bool act = c < K;
-            const float dist = l2sqr16_group(qs, centroids + (size_t)(act ? c : 0) * D, D, l, act);
-            if (act && l == 0) keys[c] = float_key(dist);
-        }
-        __syncthreads();
-        const uint32_t U = radix_select(keys, K, (uint32_t)fb_np, hist, &sh_sel, &sh_rank);
-        gather_sorted(keys, K, U, cd, ci, &sh_cnt);
-        for (int i = tid; i < fb_np; i += kThreads) {
-            out_ids[(size_t)qi * max_np + i] = ci[i];
-            out_d[(size_t)qi * max_np + i] = cd[i];
-        }
+        // the fixed list needs every cluster's exact distance: listed here, computed across
+        // the GPU by fallback_dist_kernel and selected by fallback_pick_kernel (one workgroup
+        // would take the K distances alone, long after the other queries are done)
         if (tid == 0) {
+            fb_list[atomicAdd(fb_count, 1u)] = (uint32_t)qi;
             out_len[qi] = (uint32_t)fb_np;
             atomicAdd(n_fallback, 1u);
         }
@@ -345,6 +339,63 @@ __global__ void __launch_bounds__(kThreads)
     if (tid == 0) out_len[qi] = (uint32_t)n;
 }
 
+// Exact L2Sqr16 distances of the listed (fallback) queries to every cluster, written over
+// their probability rows (no longer needed): workgroup (x, y) takes clusters
+// [kFbChunk y, kFbChunk (y + 1)) of list entries x, x + gridDim.x, ...
+constexpr int kFbChunk = 64;
+__global__ void __launch_bounds__(kThreads)
+    fallback_dist_kernel(float* __restrict__ prob, const float* __restrict__ q_rot,
+                         const float* __restrict__ centroids, int K, int D,
+                         const uint32_t* __restrict__ fb_list, const uint32_t* __restrict__ fb_count) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* qs = reinterpret_cast<float*>(smem);
+    const int tid = threadIdx.x, grp = tid >> 4, l = tid & 15;
+    const uint32_t cnt = *fb_count;
+    const int c0 = (int)blockIdx.y * kFbChunk;
+    for (uint32_t li = blockIdx.x; li < cnt; li += gridDim.x) {  // uniform
+        const uint32_t qi = fb_list[li];
+        __syncthreads();
+        for (int d = tid; d < D; d += kThreads) qs[d] = q_rot[(size_t)qi * D + d];
+        __syncthreads();
+        for (int base = c0; base < min(K, c0 + kFbChunk); base += kThreads / 16) {
+            const int c = base + grp;
+            const bool act = c < min(K, c0 + kFbChunk);
+            const float dist = l2sqr16_group(qs, centroids + (size_t)(act ? c : 0) * D, D, l, act);
+            if (act && l == 0) prob[(size_t)qi * K + c] = dist;
+        }
+    }
+}
+
+// The fixed list of each listed query: its fb_np nearest clusters by exact distance (ties by
+// id), from the distances fallback_dist_kernel wrote.
+__global__ void __launch_bounds__(kThreads)
+    fallback_pick_kernel(const float* __restrict__ dist, int K, int max_np, int fb_np,
+                         const uint32_t* __restrict__ fb_list, const uint32_t* __restrict__ fb_count,
+                         uint32_t* __restrict__ out_ids, float* __restrict__ out_d) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* keys = reinterpret_cast<uint32_t*>(smem);
+    float* cd = reinterpret_cast<float*>(smem + 4 * (size_t)K);
+    uint32_t* ci = reinterpret_cast<uint32_t*>(cd + kCapC);
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
+    const int tid = threadIdx.x;
+    const uint32_t cnt = *fb_count;
+    for (uint32_t li = blockIdx.x; li < cnt; li += gridDim.x) {  // uniform
+        const uint32_t qi = fb_list[li];
+        __syncthreads();
+        for (int c = tid; c < K; c += kThreads) keys[c] = float_key(dist[(size_t)qi * K + c]);
+        __syncthreads();
+        const uint32_t U = radix_select(keys, K, (uint32_t)fb_np, hist, &sh_sel, &sh_rank);
+        gather_sorted(keys, K, U, cd, ci, &sh_cnt);
+        for (int i = tid; i < fb_np; i += kThreads) {
+            out_ids[(size_t)qi * max_np + i] = ci[i];
+            out_d[(size_t)qi * max_np + i] = cd[i];
+        }
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0; }
+
 // CSR offsets from per-query lengths (single workgroup, chunked inclusive scan).
 __global__ void __launch_bounds__(1024) lens_to_offsets_kernel(const uint32_t* __restrict__ len,
                                                                int nq, uint32_t* __restrict__ off) {
@@ -385,16 +436,25 @@ __global__ void compact_probes_kernel(const uint32_t* __restrict__ pid, const fl
 
 }  // namespace
 
-hipError_t launch_adaptive_probe(const float* prob, const float* q_rot, const float* centroids,
+hipError_t launch_adaptive_probe(float* prob, const float* q_rot, const float* centroids,
                                  int nq, int K, int D, float mass, int min_np, int max_np,
                                  float min_conf, int fb_np, uint32_t* pad_ids, float* pad_d,
                                  uint32_t* len, uint32_t* off, uint32_t* csr_ids, float* csr_d,
-                                 uint32_t* n_fallback, hipStream_t s) {
+                                 uint32_t* n_fallback, uint32_t* fb_list, uint32_t* fb_count,
+                                 hipStream_t s) {
     if (nq == 0) return hipSuccess;
     const size_t lds = 4 * (size_t)K + 8 * (size_t)kCapC + 4 * (size_t)D;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, fb_count);
     hipLaunchKernelGGL(adaptive_select_kernel, dim3(nq), dim3(kThreads), lds, s, prob, q_rot,
                        centroids, K, D, mass, min_np, max_np, min_conf, fb_np, pad_ids, pad_d, len,
-                       n_fallback);
+                       n_fallback, fb_list, fb_count);
+    const unsigned fb_x = (unsigned)std::min(nq, 16);
+    hipLaunchKernelGGL(fallback_dist_kernel, dim3(fb_x, (K + kFbChunk - 1) / kFbChunk),
+                       dim3(kThreads), 4 * (size_t)D, s, prob, q_rot, centroids, K, D, fb_list,
+                       fb_count);
+    hipLaunchKernelGGL(fallback_pick_kernel, dim3((unsigned)std::min(nq, 512)), dim3(kThreads),
+                       4 * (size_t)K + 8 * (size_t)kCapC, s, prob, K, max_np, fb_np, fb_list,
+                       fb_count, pad_ids, pad_d);
     hipLaunchKernelGGL(lens_to_offsets_kernel, dim3(1), dim3(1024), 0, s, len, nq, off);
     hipLaunchKernelGGL(compact_probes_kernel, dim3(nq), dim3(64), 0, s, pad_ids, pad_d, max_np,
                        off, nq, csr_ids, csr_d);

@@ -293,3 +293,46 @@ def test_probe_model_save_load_round_trip(tmp_path):
     assert torch.equal(m.forward_dev(xt), m2.forward_dev(xt))
     m.close()
     m2.close()
+
+
+def test_learned_mixed_fallback_lists():
+    """min_conf between the queries' total predicted masses: the queries below it take the
+    fixed list (their fb_np nearest clusters by exact distance, ties by id), computed across
+    the GPU after the selection; the others keep the mass prefix.  Both against the oracle."""
+    import torch
+    from erq.gpu import IVF
+    from erq.probe_model import ProbeMLP
+    from oracle import oracle as orc
+    idx, x, qtr, qte = _small_index()
+    g = IVF.from_index(idx)
+    m = ProbeMLP(idx.D, idx.K, hidden=128, max_batch=512, seed=4)
+    qte_np = rotate_np(idx, qte)
+    q = torch.from_numpy(qte_np).cuda()
+    prob = m.forward_dev(q).cpu().numpy()
+    tot = prob.astype(np.float64).sum(1)
+    min_conf = float(np.median(tot))
+    fb_np, mass, min_np, max_np = 12, 0.9, 2, 48
+    off, pid, pd, nfb = g.probe_learned_dev(m, q, mass, min_np, max_np, fb_np, min_conf)
+    torch.cuda.synchronize()
+    off, pid, pd = off.cpu().numpy(), pid.cpu().numpy().view(np.uint32), pd.cpu().numpy()
+    oi = orc.OracleIndex(idx)
+    nf = 0
+    for i in range(qte_np.shape[0]):
+        all_i, all_d = oi.centroid_topn(qte_np[i], idx.K)
+        seg = slice(off[i], off[i + 1])
+        # the kernel's float32 total decides; keep clear of the boundary
+        if abs(tot[i] - min_conf) < 1e-3 * max(1.0, min_conf):
+            continue
+        if tot[i] < min_conf:
+            nf += 1
+            np.testing.assert_array_equal(pid[seg], all_i[:fb_np], err_msg=f"query {i}")
+            np.testing.assert_array_equal(pd[seg].view(np.uint32), all_d[:fb_np].view(np.uint32))
+        else:
+            kept = _kept_by_rule(prob[i], mass, min_np, max_np)
+            dist = np.empty(idx.K, np.float32)
+            dist[all_i] = all_d
+            want = kept[np.lexsort((kept, dist[kept]))]
+            np.testing.assert_array_equal(pid[seg], want.astype(np.uint32), err_msg=f"query {i}")
+    assert nf > 0 and int(nfb[0]) >= nf
+    m.close()
+    g.close()
