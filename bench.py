@@ -58,6 +58,10 @@ def parse(argv=None):
     ap.add_argument("--label-nprobe", type=int, default=0,
                     help="nprobe of the label search (0: 256) and the learned lists' maximum")
     ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--lr", type=float, default=2e-3, help="probe MLP peak learning rate")
+    ap.add_argument("--pos-weight", type=float, default=2.0,
+                    help="probe MLP loss weight of the positive labels")
+    ap.add_argument("--train-batch", type=int, default=1024, help="probe MLP minibatch")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calib-copy", action="store_true",
@@ -275,10 +279,10 @@ def train_probe_model(a, g, idx, mix, torch):
     log(f"labels from fixed nprobe={label_np} over {a.train_queries} queries in {t_lab:.1f}s")
     t0 = time.time()
     # inputs scaled to unit variance per coordinate (rotated unit-norm queries)
-    m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=1024, seed=7,
+    m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=max(1024, a.train_batch), seed=7,
                  input_scale=float(np.sqrt(idx.D)))
-    losses = m.train(qt_r, lab, epochs=a.epochs, batch=1024, lr=2e-3, pos_weight=4.0, log=log,
-                     schedule="cosine")
+    losses = m.train(qt_r, lab, epochs=a.epochs, batch=a.train_batch, lr=a.lr,
+                     pos_weight=a.pos_weight, log=log, schedule="cosine")
     torch.cuda.synchronize()
     if a.save_mlp:
         m.save(a.save_mlp)

@@ -261,10 +261,22 @@ __global__ void __launch_bounds__(kThreads)
     const int grp = tid >> 4, l = tid & 15;
     const float* prow = prob + (size_t)qi * K;
     float part = 0.f;
-    for (int c = tid; c < K; c += kThreads) {
-        float pc = prow[c];
-        part += pc;
-        keys[c] = float_key(-pc);
+    // 16 loads per thread in flight, then folded in the same order as one at a time
+    for (int c0 = 0; c0 < K; c0 += 16 * kThreads) {
+        float pv[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int c = c0 + tid + kThreads * j;
+            pv[j] = c < K ? prow[c] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int c = c0 + tid + kThreads * j;
+            if (c < K) {
+                part += pv[j];
+                keys[c] = float_key(-pv[j]);
+            }
+        }
     }
     for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
     if ((tid & 63) == 0) sh_sum[tid >> 6] = part;
