@@ -40,12 +40,13 @@ __device__ __forceinline__ int cvtt(double x) {  // (int)x as x86-64 computes it
 
 struct QuantArgs {
     const float* r;      // n x D residuals
-    const float* onorm;  // n
+    const float* onorm;  // n: ||o - c|| (HA), or ||o - c||^2 when fac is set (non-HA)
     uint64_t n;
     int D, EX;
     uint8_t* long_code;  // n x D*EX/8
     float* xipnorm;      // n
     uint32_t* flags;     // bit 0: a non-finite input
+    float* fac;          // non-HA (nullable): n x 3 -- fac_ip, sumxb, fac_err
 };
 
 // LDS of one wave: o[D] float, cur[D] and bl[D] bytes, then the window's events.
@@ -159,12 +160,38 @@ __global__ void __launch_bounds__(64) quantize_kernel(QuantArgs a) {
         ss = __shfl(ss, 0);
         const float inv = ss > 0 ? (float)(1.0 / sqrt(ss)) : 0.f;
         float mx = -1.f;
+        int npos = 0;
         for (int d = lane; d < D; d += kWave) {
             const float u = fabsf(o[d] * inv);
+            npos += o[d] > 0 ? 1 : 0;
             o[d] = u;
             mx = fmaxf(mx, u);
         }
         mx = wave_max(mx);
+        if (a.fac) {
+            // the non-HA factors (DataQuantizer::rabitq_factor, Quantizer.hpp:386-437) as the
+            // host builder evaluates them: <u, o_bar> = sum |u| f32(1/sqrt(D)) in float and
+            // sum |u / 2| in double, both in dim order (lane 0); sumxb = the positive dims
+            npos = wave_sum_i(npos);
+            wave_sync();
+            if (lane == 0) {
+                const float fn = (float)(1.0 / sqrt((double)D));
+                float oob = 0.f;
+                double ip = 0.0;
+                for (int d = 0; d < D; d++) {
+                    oob = oob + o[d] * fn;
+                    ip = ip + 0.5 * (double)o[d];
+                }
+                if (!isfinite(oob)) oob = 0.8f;
+                const float dist2c = (float)sqrt((double)on);
+                const double fac_err = 2.0 / sqrt((double)(D - 1));
+                const float oo = oob * oob;
+                a.fac[3 * v + 0] = (float)(1.0 / ip * 2 * (double)dist2c);
+                a.fac[3 * v + 1] = (float)npos;
+                a.fac[3 * v + 2] =
+                    (float)(sqrt((1.0 - (double)oo) / (double)oo) * fac_err * 2 * (double)dist2c);
+            }
+        }
         const double max_o = (double)mx;
         double best_t = 0.0;
         float ipn = 1.f;
@@ -358,17 +385,20 @@ __global__ void __launch_bounds__(64) quantize_kernel(QuantArgs a) {
         wave_sync();
         uint8_t* out = a.long_code + v * ((size_t)D * EX / 8);
         for (int p = lane; p < D * EX / 8; p += kWave) out[p] = compact_byte(cur, EX, p);
-        if (lane == 0) a.xipnorm[v] = ipn * 2 * on;
+        if (lane == 0)
+            a.xipnorm[v] = a.fac ? (float)((double)(ipn * 2) * sqrt((double)on)) : ipn * 2 * on;
         wave_sync();
     }
 }
 
-// One thread per byte of the packed blocks: 4D code bytes then 32 factors (as 128 bytes).
+// One thread per byte of the packed blocks: 4D code bytes then the factors, 32 floats each
+// (as 128 bytes): ||o - c|| (HA), or ||o - c||^2, fac_ip, sumxb and fac_err when fac is set
+// (non-HA, Quantizer.hpp:107-121).
 __global__ void pack_kernel(const float* __restrict__ r, const float* __restrict__ onorm, int D,
                             uint32_t v_base, const uint32_t* __restrict__ blk_first,
                             const uint8_t* __restrict__ blk_count, size_t nblocks,
-                            uint8_t* __restrict__ out) {
-    const size_t BB = 4 * (size_t)D + 128;
+                            const float* __restrict__ fac, uint8_t* __restrict__ out) {
+    const size_t BB = 4 * (size_t)D + (fac ? 512 : 128);
     const size_t total = nblocks * BB;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total;
          q += (size_t)gridDim.x * blockDim.x) {
@@ -391,8 +421,10 @@ __global__ void pack_kernel(const float* __restrict__ r, const float* __restrict
             }
             out[q] = (uint8_t)val;
         } else {
-            const int s = (byte - 4 * D) >> 2, bi = (byte - 4 * D) & 3;
-            const float f = s < cnt ? onorm[first + s] : 0.f;
+            const int fb = byte - 4 * D, row = fb >> 7;
+            const int s = (fb & 127) >> 2, bi = fb & 3;
+            float f = 0.f;
+            if (s < cnt) f = row == 0 ? onorm[first + s] : fac[3 * ((size_t)first + s) + row - 1];
             out[q] = (uint8_t)(__float_as_uint(f) >> (8 * bi));
         }
     }
@@ -401,7 +433,8 @@ __global__ void pack_kernel(const float* __restrict__ r, const float* __restrict
 }  // namespace
 
 hipError_t launch_quantize(const float* r, const float* onorm, uint64_t n, int D, int EX,
-                           uint8_t* long_code, float* xipnorm, uint32_t* flags, hipStream_t s) {
+                           uint8_t* long_code, float* xipnorm, uint32_t* flags, hipStream_t s,
+                           float* fac = nullptr) {
     if (n == 0) return hipSuccess;
     const size_t lds = quant_lds(D);
     if (lds > 64 * 1024) {
@@ -410,7 +443,7 @@ hipError_t launch_quantize(const float* r, const float* onorm, uint64_t n, int D
                                                  (int)lds);
         if (e != hipSuccess) return e;
     }
-    QuantArgs a{r, onorm, n, D, EX, long_code, xipnorm, flags};
+    QuantArgs a{r, onorm, n, D, EX, long_code, xipnorm, flags, fac};
     const unsigned grid = (unsigned)std::min<uint64_t>(n, 1u << 20);
     hipLaunchKernelGGL(quantize_kernel, dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
@@ -418,12 +451,12 @@ hipError_t launch_quantize(const float* r, const float* onorm, uint64_t n, int D
 
 hipError_t launch_pack(const float* r, const float* onorm, int D, uint32_t v_base,
                        const uint32_t* blk_first, const uint8_t* blk_count, size_t nblocks,
-                       uint8_t* out, hipStream_t s) {
+                       uint8_t* out, hipStream_t s, const float* fac = nullptr) {
     if (nblocks == 0) return hipSuccess;
-    const size_t total = nblocks * (4 * (size_t)D + 128);
+    const size_t total = nblocks * (4 * (size_t)D + (fac ? 512 : 128));
     const int grid = (int)std::min<size_t>((total + 255) / 256, 1 << 20);
     hipLaunchKernelGGL(pack_kernel, dim3(grid), dim3(256), 0, s, r, onorm, D, v_base, blk_first,
-                       blk_count, nblocks, out);
+                       blk_count, nblocks, fac, out);
     return hipGetLastError();
 }
 
@@ -505,11 +538,12 @@ __global__ void residual_kernel(float* __restrict__ xr, const uint32_t* __restri
 
 // ||o - c|| = f32(sqrt((double)L2Sqr(o, c, DIM))) (Quantizer.hpp:399-401; space.hpp:41-51:
 // L2Sqr16 over the largest multiple of 16 dims -- 16 lane-strided fma chains, butterfly --
-// then the remaining dims one by one), one 16-lane group per vector.
+// then the remaining dims one by one), one 16-lane group per vector; squared: L2Sqr itself
+// (the non-HA factor ||o - c||^2).
 __global__ void onorm_kernel(const float* __restrict__ data, int DIM,
                              const float* __restrict__ cent, const uint32_t* __restrict__ ids,
                              const uint32_t* __restrict__ assign, uint32_t rows,
-                             float* __restrict__ onorm) {
+                             float* __restrict__ onorm, int squared) {
     const int l = threadIdx.x & 15;
     const uint32_t g = (blockIdx.x * (uint32_t)blockDim.x + threadIdx.x) >> 4;
     const bool act = g < rows;
@@ -529,7 +563,7 @@ __global__ void onorm_kernel(const float* __restrict__ data, int DIM,
             const float t = x[d] - c[d];
             p = fmaf(t, t, p);
         }
-        onorm[g] = (float)sqrt((double)p);
+        onorm[g] = squared ? p : (float)sqrt((double)p);
     }
 }
 
@@ -628,12 +662,13 @@ extern "C" int erq_build_pack_dev(const float* resid, const float* onorm,
     });
 }
 
-extern "C" int erq_build_index_dev(const float* data, uint64_t N, uint32_t DIM,
-                                   const float* centroids, uint32_t K, const uint32_t* assign,
-                                   const float* P, uint32_t ex_bits, uint64_t* cluster_sizes,
-                                   float* rotated_centroids, uint8_t* short_data,
-                                   uint8_t* long_code, float* ex_factor, uint32_t* ids,
-                                   void* stream) {
+namespace {
+
+// IVF::construct on the GPU for either block format (nonha: four factors per vector).
+int build_index(const float* data, uint64_t N, uint32_t DIM, const float* centroids, uint32_t K,
+                const uint32_t* assign, const float* P, uint32_t ex_bits, uint64_t* cluster_sizes,
+                float* rotated_centroids, uint8_t* short_data, uint8_t* long_code,
+                float* ex_factor, uint32_t* ids, void* stream, bool nonha) {
     return guarded([&]() -> int {
         if (!data || !centroids || !assign || !P || !cluster_sizes || !rotated_centroids ||
             !short_data || !long_code || !ex_factor || !ids) {
@@ -717,10 +752,11 @@ extern "C" int erq_build_index_dev(const float* data, uint64_t N, uint32_t DIM,
         for (uint32_t k = 0; k < K; k++) maxc = std::max<uint64_t>(maxc, hc[k]);
         const uint64_t chunk = std::max<uint64_t>(maxc, std::max<uint64_t>(
                                    1024, ((uint64_t)1 << 30) / (8 * (uint64_t)D)));
-        float *xp = nullptr, *xr = nullptr, *on = nullptr;
+        float *xp = nullptr, *xr = nullptr, *on = nullptr, *fac = nullptr;
         ERQ_B(alloc(4 * chunk * D, &xp));
         ERQ_B(alloc(4 * chunk * D, &xr));
         ERQ_B(alloc(4 * chunk, &on));
+        if (nonha) ERQ_B(alloc(12 * chunk, &fac));
         std::vector<uint32_t> bfirst;
         std::vector<uint8_t> bcount;
         for (uint32_t k = 0; k < K; k++)
@@ -737,7 +773,7 @@ extern "C" int erq_build_index_dev(const float* data, uint64_t N, uint32_t DIM,
             ERQ_BH(hipMemcpyAsync(d_bfirst, bfirst.data(), 4 * nb, hipMemcpyHostToDevice, s), "blocks");
             ERQ_BH(hipMemcpyAsync(d_bcount, bcount.data(), nb, hipMemcpyHostToDevice, s), "blocks");
         }
-        const size_t BB = 4 * (size_t)D + 128, CL = (size_t)D * EX / 8;
+        const size_t BB = 4 * (size_t)D + (nonha ? 512 : 128), CL = (size_t)D * EX / 8;
         for (uint32_t k0 = 0; k0 < K;) {
             uint32_t k1 = k0 + 1;
             while (k1 < K && vstart[k1 + 1] - vstart[k0] <= chunk) k1++;
@@ -753,14 +789,15 @@ extern "C" int erq_build_index_dev(const float* data, uint64_t N, uint32_t DIM,
                                    ids + v0, assign, (uint32_t)rows, D, rotated_centroids);
                 ERQ_BH(hipGetLastError(), "residual");
                 hipLaunchKernelGGL(onorm_kernel, dim3(grid_for(rows * 16, 256)), dim3(256), 0, s,
-                                   data, (int)DIM, centroids, ids + v0, assign, (uint32_t)rows, on);
+                                   data, (int)DIM, centroids, ids + v0, assign, (uint32_t)rows, on,
+                                   nonha ? 1 : 0);
                 ERQ_BH(hipGetLastError(), "norms");
                 ERQ_BH(launch_quantize(xr, on, rows, D, EX, long_code + v0 * CL, ex_factor + v0,
-                                       flags, s),
+                                       flags, s, fac),
                        "quantize");
                 const size_t b0 = bstart[k0], b1 = bstart[k1];
                 ERQ_BH(launch_pack(xr, on, D, (uint32_t)v0, d_bfirst + b0, d_bcount + b0, b1 - b0,
-                                   short_data + b0 * BB, s),
+                                   short_data + b0 * BB, s, fac),
                        "pack");
             }
             k0 = k1;
@@ -775,4 +812,26 @@ extern "C" int erq_build_index_dev(const float* data, uint64_t N, uint32_t DIM,
 #undef ERQ_B
 #undef ERQ_BH
     });
+}
+
+}  // namespace
+
+extern "C" int erq_build_index_dev(const float* data, uint64_t N, uint32_t DIM,
+                                   const float* centroids, uint32_t K, const uint32_t* assign,
+                                   const float* P, uint32_t ex_bits, uint64_t* cluster_sizes,
+                                   float* rotated_centroids, uint8_t* short_data,
+                                   uint8_t* long_code, float* ex_factor, uint32_t* ids,
+                                   void* stream) {
+    return build_index(data, N, DIM, centroids, K, assign, P, ex_bits, cluster_sizes,
+                       rotated_centroids, short_data, long_code, ex_factor, ids, stream, false);
+}
+
+extern "C" int erq_build_index_dev_nonha(const float* data, uint64_t N, uint32_t DIM,
+                                         const float* centroids, uint32_t K,
+                                         const uint32_t* assign, const float* P, uint32_t ex_bits,
+                                         uint64_t* cluster_sizes, float* rotated_centroids,
+                                         uint8_t* short_data, uint8_t* long_code,
+                                         float* ex_factor, uint32_t* ids, void* stream) {
+    return build_index(data, N, DIM, centroids, K, assign, P, ex_bits, cluster_sizes,
+                       rotated_centroids, short_data, long_code, ex_factor, ids, stream, true);
 }

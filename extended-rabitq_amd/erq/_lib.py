@@ -88,6 +88,9 @@ SIGNATURES = {
     "erq_build_index_dev": (c_int, [c_vp, ctypes.c_uint64, ctypes.c_uint32, c_vp,
                                     ctypes.c_uint32, c_vp, c_vp, ctypes.c_uint32, c_vp, c_vp,
                                     c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "erq_build_index_dev_nonha": (c_int, [c_vp, ctypes.c_uint64, ctypes.c_uint32, c_vp,
+                                    ctypes.c_uint32, c_vp, c_vp, ctypes.c_uint32, c_vp, c_vp,
+                                    c_vp, c_vp, c_vp, c_vp, c_vp]),
 }
 
 BUILDER_SIGNATURES = {

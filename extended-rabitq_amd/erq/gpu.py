@@ -346,10 +346,11 @@ def build_pack_dev(resid, onorm, cluster_sizes):
     return out
 
 
-def build_index_dev(data, centroids, assign, bits, P):
+def build_index_dev(data, centroids, assign, bits, P, high_acc=True):
     """IVF::construct on the GPU (erq_build_index_dev): raw vectors, un-rotated centroids and
     cluster ids (torch CUDA tensors) and the rotation P (numpy [D, D]) -> an IVFIndex with
-    host arrays, EX = bits - 1."""
+    host arrays, EX = bits - 1.  high_acc=False builds the non-HA format
+    (erq_build_index_dev_nonha: four factors per vector)."""
     import torch
     from .index import IVFIndex, padded_dim
     x, c = _rows_dev(data), _rows_dev(centroids)
@@ -366,16 +367,17 @@ def build_index_dev(data, centroids, assign, bits, P):
     Pd = torch.as_tensor(np.ascontiguousarray(P, np.float32)).to(x.device)
     nb = int(((sizes + 31) // 32).sum())
     crot = torch.empty((K, D), dtype=torch.float32, device=x.device)
-    short = torch.empty(nb * (4 * D + 128), dtype=torch.uint8, device=x.device)
+    F = 1 if high_acc else 4
+    short = torch.empty(nb * (4 * D + 128 * F), dtype=torch.uint8, device=x.device)
     lc = torch.empty(N * D * EX // 8, dtype=torch.uint8, device=x.device)
     xip = torch.empty(N, dtype=torch.float32, device=x.device)
     ids = torch.empty(N, dtype=torch.int32, device=x.device)
     got = np.zeros(K, np.uint64)
-    check(_lib.lib().erq_build_index_dev(_ptr(x), N, DIM, _ptr(c), K, _ptr(a), _ptr(Pd), EX,
-                                         _ptr(got), _ptr(crot), _ptr(short), _ptr(lc), _ptr(xip),
-                                         _ptr(ids), _stream()))
+    fn = _lib.lib().erq_build_index_dev if high_acc else _lib.lib().erq_build_index_dev_nonha
+    check(fn(_ptr(x), N, DIM, _ptr(c), K, _ptr(a), _ptr(Pd), EX, _ptr(got), _ptr(crot),
+             _ptr(short), _ptr(lc), _ptr(xip), _ptr(ids), _stream()))
     if not np.array_equal(got, sizes):
         raise RuntimeError("cluster sizes disagree")
     return IVFIndex(N, DIM, K, EX, got, np.ascontiguousarray(P, np.float32), crot.cpu().numpy(),
                     short.cpu().numpy(), lc.cpu().numpy(), xip.cpu().numpy(),
-                    ids.cpu().numpy().view(np.uint32))
+                    ids.cpu().numpy().view(np.uint32), F)

@@ -353,6 +353,18 @@ int erq_build_index_dev(const float* data, uint64_t N, uint32_t DIM, const float
                         uint64_t* cluster_sizes, float* rotated_centroids, uint8_t* short_data,
                         uint8_t* long_code, float* ex_factor, uint32_t* ids, void* stream);
 
+/* erq_build_index_dev for the non-HA format (IVF::construct without HIGH_ACC_FAST_SCAN;
+ * DataQuantizer::rabitq_factor, Quantizer.hpp:386-437): short_data holds sum ceil(n_k/32)
+ * (4D + 512) bytes, four factors per vector after each block's codes -- ||o - c||^2 (L2Sqr),
+ * fac_ip = 2 ||o - c|| / sum |u / 2|, sumxb = the positive dims, fac_err = 2 ||o - c||
+ * FAC_ERR sqrt(1 - <u, o_bar>^2) / <u, o_bar> -- and ex_factor = xipnorm over
+ * f32(sqrt((double)||o - c||^2)), as the host builder writes them. */
+int erq_build_index_dev_nonha(const float* data, uint64_t N, uint32_t DIM,
+                              const float* centroids, uint32_t K, const uint32_t* assign,
+                              const float* P, uint32_t ex_bits, uint64_t* cluster_sizes,
+                              float* rotated_centroids, uint8_t* short_data, uint8_t* long_code,
+                              float* ex_factor, uint32_t* ids, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

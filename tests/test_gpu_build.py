@@ -134,3 +134,65 @@ def test_build_index_random_rotation_searches_like_host():
         ids = O.OracleIndex(idx).search_batch(qr, 100, np.full(64, 16, np.uint32))[0]
         rec[name] = data.recall_at_k(ids, gt, 100)
     assert abs(rec["dev"] - rec["host"]) < 0.01, rec
+
+
+def _host_quantize_nonha(r, x2, sizes, D, ex):
+    from erq import _lib
+    N = r.shape[0]
+    nb = int(((sizes + 31) // 32).sum())
+    short = np.zeros(nb * (4 * D + 512), np.uint8)
+    longc = np.zeros(N * D * ex // 8, np.uint8)
+    xip = np.zeros(N, np.float32)
+    vp = ctypes.c_void_p
+    rc = _lib.builder().erqb_quantize_all_nonha(
+        r.ctypes.data_as(vp), x2.ctypes.data_as(vp), sizes.ctypes.data_as(vp),
+        ctypes.c_size_t(sizes.size), ctypes.c_size_t(D), ctypes.c_int(ex), short.ctypes.data_as(vp),
+        longc.ctypes.data_as(vp), xip.ctypes.data_as(vp), ctypes.c_int(16))
+    assert rc == 0
+    return short, longc, xip
+
+
+@pytest.mark.parametrize("dim,bits", [(128, 4), (960, 9), (1536, 3)])
+def test_build_index_nonha_identity_rotation_matches_host(dim, bits):
+    """The non-HA format (four factors per vector) built on the GPU equals the host builder's
+    byte for byte when the residuals are exact (P = I): codes, ||o - c||^2, fac_ip, sumxb,
+    fac_err and xipnorm; and the index searches exactly like the oracle's non-HA searcher."""
+    from erq import data, gpu
+    from erq.index import padded_dim
+    from oracle import oracle as O
+    torch = _torch()
+    n = 4000
+    x, cen = data.mixture_isotropic(n, dim, 16, 0.5, seed=dim + 1)
+    c, assign = data.kmeans(x, 16, 4, seed=6)
+    D = padded_dim(dim)
+    P = np.eye(D, dtype=np.float32)
+    idx = gpu.build_index_dev(torch.from_numpy(x).cuda(), torch.from_numpy(c.astype(np.float32)).cuda(),
+                              torch.from_numpy(assign.astype(np.int32)).cuda(), bits, P,
+                              high_acc=False)
+    assert idx.factors == 4
+    order = np.argsort(assign, kind="stable")
+    np.testing.assert_array_equal(idx.ids, order.astype(np.uint32))
+    cpad = np.zeros((16, D), np.float32)
+    cpad[:, :dim] = c
+    xs = np.zeros((n, D), np.float32)
+    xs[:, :dim] = x[order]
+    r = xs - cpad[assign[order]]
+    x2 = np.array([O.l2sqr16(x[i], c[assign[i]]) for i in order], np.float32)
+    want_s, want_l, want_x = _host_quantize_nonha(r, x2, idx.cluster_sizes, D, bits - 1)
+    np.testing.assert_array_equal(idx.long_code, want_l)
+    np.testing.assert_array_equal(idx.ex_factor.view(np.uint32), want_x.view(np.uint32))
+    np.testing.assert_array_equal(idx.short_data, want_s)
+    # searched on the GPU, equal to the oracle's non-HA searcher
+    from erq.gpu import IVF
+    q, _ = data.mixture_isotropic(16, dim, 16, 0.5, seed=dim + 2, centres=cen)
+    qp = np.zeros((16, D), np.float32)
+    qp[:, :dim] = q
+    g = IVF.from_index(idx)
+    ids, d = g.search_dev(torch.from_numpy(qp).cuda(), 20, 4)
+    torch.cuda.synchronize()
+    oi = O.OracleIndex(idx)
+    for i in range(16):
+        oid, od = oi.search(qp[i], 20, 4)
+        np.testing.assert_array_equal(ids[i].cpu().numpy().view(np.uint32), oid)
+        np.testing.assert_array_equal(d[i].cpu().numpy().view(np.uint32), od.view(np.uint32))
+    g.close()
