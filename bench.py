@@ -103,6 +103,7 @@ def build_workload(a, torch):
     t0 = time.time()
     idx = gpu.build_index_dev(x, c, assign, a.bits, P)   # erq_build_index_dev
     a.build_s = time.time() - t0
+    a.build_inputs = (c, assign, P)   # for the non-HA index of the same data
     log(f"HA index built on the GPU (B={a.bits}) in {a.build_s:.2f}s (incl. copies to host); "
         f"short {idx.short_data.nbytes / 1e6:.0f} MB, long {idx.long_code.nbytes / 1e6:.0f} MB")
     return x, q, idx, (basis, centres)
@@ -514,6 +515,46 @@ def cpu_baseline(r, plain):
            "host": host_info()}
 
 
+def nonha_path(r, nprobe):
+    """The non-HA format (the reference built without HIGH_ACC_FAST_SCAN: Searcher, four
+    factors per vector) on the same data, clusters and rotation: built on the GPU
+    (erq_build_index_dev_nonha), searched per query (erq_search_dev) at the plain path's
+    nprobe; QPS, recall, ratio and the scan's roofline."""
+    from erq import gpu
+    from erq.gpu import IVF
+    a, torch = r.a, r.torch
+    c, assign, P = a.build_inputs
+    t0 = time.time()
+    idx2 = gpu.build_index_dev(r.x, c, assign, a.bits, P, high_acc=False)
+    build_s = time.time() - t0
+    g2 = IVF.from_index(idx2)
+    try:
+        qr = g2.rotate_dev(r.q)
+        st = torch.zeros(8, dtype=torch.int64, device="cuda")
+        ids, _ = g2.search_dev(qr, a.k, nprobe, stats=st)
+        torch.cuda.synchronize()
+        rec, rat = r.recall(ids), r.ratio(ids)
+        s = st.cpu().numpy().tolist()
+        ms = r.timed(lambda: g2.search_dev(g2.rotate_dev(r.q), a.k, nprobe), steps=3, warmup=1)
+        g2.timing_begin(64)
+        g2.search_dev(qr, a.k, nprobe)
+        torch.cuda.synchronize()
+        scan_ms = g2.timing_end()["replay"][0]
+        D, EX = idx2.D, idx2.EX
+        nbytes = s[0] * (4 * D + 512) + s[2] * (D * EX // 8 + 8)
+        log(f"[nonha] nprobe={nprobe} recall={rec:.4f} {ms:.2f} ms per batch, scan {scan_ms:.2f} ms")
+        return {"nprobe": nprobe, "qps": round(a.nq / ms * 1e3, 1), "ms": round(ms, 3),
+                "recall": round(rec, 4), "ratio": round(rat, 5), "build_gpu_s": round(build_s, 3),
+                "roofline": {"bound": "hbm", "kernel": "nonha_scan_kernel",
+                             "achieved": round(nbytes / scan_ms / 1e6, 1), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s",
+                             "frac": round(nbytes / scan_ms / 1e6 / HBM_PEAK_GBS, 4),
+                             "algorithmic_bytes": int(nbytes), "ms_per_launch": round(scan_ms, 3)},
+                "stats": s[:4]}
+    finally:
+        g2.close()
+
+
 def drop_in_latency(r, nprobe, nq=256):
     """The IVF::search drop-in (erq_search, host pointers, one query per call, as
     test_search.cpp's loop calls IVF::search): median and mean latency per query, at the
@@ -665,6 +706,12 @@ def main(argv=None):
     plain_roof = plain_roofline(r, np_hi)
     latency = drop_in_latency(r, np_hi)
     log(f"[drop-in] erq_search per query at nprobe {np_hi}: {latency}")
+    nonha = None
+    try:
+        nonha = nonha_path(r, np_hi)
+    except Exception as e:  # reported, never hidden
+        log(f"non-HA leg failed: {e!r}")
+        nonha = {"error": repr(e)}
     cpu = None if a.no_cpu else cpu_baseline(r, plain)
 
     if headline:
@@ -708,6 +755,7 @@ def main(argv=None):
         "paths": {"plain": plain, "batched": batched, "learned": learned},
         "roofline_plain_scan": plain_roof,
         "drop_in_latency": latency,
+        "nonha_per_query": nonha,
         "uneven_nprobe_graph": uneven,
         "setup_s": {"index_build_gpu": round(getattr(a, "build_s", 0.0), 3),
                     "ground_truth_gpu": round(getattr(a, "gt_s", 0.0), 3)},
