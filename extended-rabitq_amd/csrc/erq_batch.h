@@ -40,6 +40,7 @@ struct BatchArgs {
     uint32_t* redo;            // the round's queries whose candidates overflowed: [0] count,
                                // then the queries (rescanned by the round-mode scan)
     int cand_cap;
+    int item_tiles;            // 16-vector tiles per work item: 8 or 16 (4 or 8 blocks)
     int num_cu;                // compute units (sizes the re-estimate's grid)
     int D, EX, k, nq, nchunk;
     float one_over_sqrtD;
@@ -62,7 +63,7 @@ size_t batch_lds_bytes(int D);         // main scan pass
 size_t batch_inline_lds_bytes(int D);  // inline pass
 size_t prep_wave_bytes(int D);
 // Upper bound on the work items of one round of at most `pairs` (query, cluster) pairs.
-size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize);
+size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize, int item_tiles);
 hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
                            hipStream_t s);
 hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s);

@@ -40,7 +40,8 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 constexpr int kItemGroup = 16; // entries per work item (32 rows of the main pass's MFMA)
 constexpr int kGroup = 8;      // entries per inline pass over an item half (16 MFMA rows)
 constexpr int kThreads = 256;  // 4 waves
-constexpr int kItemTiles = 16; // 16-vector tiles (8 blocks of 32) per work item
+constexpr int kItemTiles = 16; // 16-vector tiles (8 blocks of 32) per work item, at most
+                               // (BatchArgs::item_tiles: 8 or 16)
 constexpr int kQSurv = kItemTiles * 16;  // survivors of one query in one work item, at most
 constexpr int kMainSurv = 128; // of them kept in LDS by the main pass (more: inline pass)
 
@@ -139,14 +140,14 @@ __global__ void count_kernel(const uint32_t* __restrict__ off, const uint32_t* _
 
 // Work items from the bin counts, by workgroups of 256 (small enough to start beside other
 // kernels).  A work item is (cluster, up to kItemGroup consecutive entries, up to
-// kItemTiles tiles of the cluster), three words: cluster, (first entry << 5) | entry count,
+// item_tiles tiles of the cluster), three words: cluster, (first entry << 5) | entry count,
 // first tile.  Items and entries are in (round, cluster) order; rinfo[4r + 0..3] = the
 // round's items, first item, entries, items queued for the inline pass (0); rinfo[4R + 0],
 // [4R + 2] = the totals.
 __device__ __forceinline__ uint32_t bin_items(const uint32_t* csize, int K, int bin, uint32_t cnt,
-                                              uint32_t& nsplit) {
+                                              uint32_t& nsplit, int item_tiles) {
     const uint32_t ntile = (csize[bin % K] + 15) / 16;
-    nsplit = (ntile + kItemTiles - 1) / kItemTiles;  // 0: empty cluster
+    nsplit = (ntile + item_tiles - 1) / item_tiles;  // 0: empty cluster
     return (cnt + kItemGroup - 1) / kItemGroup * nsplit;
 }
 
@@ -175,12 +176,13 @@ __device__ __forceinline__ void block_scan2(uint32_t& a, uint32_t& b, uint32_t* 
 // the totals.
 __global__ void __launch_bounds__(256) bin_scan_kernel(const uint32_t* __restrict__ counts,
                                                       const uint32_t* __restrict__ csize, int K,
-                                                      int nb, uint32_t* __restrict__ local,
+                                                      int nb, int item_tiles,
+                                                      uint32_t* __restrict__ local,
                                                       uint32_t* __restrict__ btot) {
     __shared__ uint32_t wa[4], wb[4];
     const int bin = blockIdx.x * 256 + threadIdx.x;
     uint32_t ns, cnt = bin < nb ? counts[bin] : 0;
-    uint32_t it = bin < nb ? bin_items(csize, K, bin, cnt, ns) : 0;
+    uint32_t it = bin < nb ? bin_items(csize, K, bin, cnt, ns, item_tiles) : 0;
     uint32_t ic = cnt, ii = it;
     block_scan2(ic, ii, wa, wb);
     if (bin < nb) {
@@ -238,7 +240,8 @@ __global__ void __launch_bounds__(256) bin_total_kernel(uint32_t* __restrict__ b
 // Per bin: the cursor of its entries and its items.
 __global__ void __launch_bounds__(256) bin_items_kernel(const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ csize, int K,
-                                                       int nb, const uint32_t* __restrict__ local,
+                                                       int nb, int item_tiles,
+                                                       const uint32_t* __restrict__ local,
                                                        const uint32_t* __restrict__ btot,
                                                        uint32_t* __restrict__ cursor,
                                                        uint32_t* __restrict__ items) {
@@ -246,7 +249,7 @@ __global__ void __launch_bounds__(256) bin_items_kernel(const uint32_t* __restri
     if (bin >= nb) return;
     const uint32_t cnt = counts[bin];
     uint32_t ns;
-    const uint32_t nit = bin_items(csize, K, bin, cnt, ns);
+    const uint32_t nit = bin_items(csize, K, bin, cnt, ns, item_tiles);
     const uint32_t c0 = btot[2 * blockIdx.x] + local[2 * bin];
     const uint32_t i0 = btot[2 * blockIdx.x + 1] + local[2 * bin + 1];
     cursor[bin] = c0;
@@ -255,7 +258,7 @@ __global__ void __launch_bounds__(256) bin_items_kernel(const uint32_t* __restri
         items[3 * w] = (uint32_t)(bin % K);
         items[3 * w + 1] = ((c0 + grp * kItemGroup) << 5) |
                            min((uint32_t)kItemGroup, cnt - grp * kItemGroup);
-        items[3 * w + 2] = sp * kItemTiles;
+        items[3 * w + 2] = sp * (uint32_t)item_tiles;
     }
 }
 
@@ -494,7 +497,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
     auto fetch_codes = [&](uint32_t item) {
         const uint32_t c = a.items[3 * item];
         const int bf = (int)a.items[3 * item + 2] >> 1;
-        const int be = min((int)((a.csize[c] + 31) >> 5), bf + kItemTiles / 2);
+        const int be = min((int)((a.csize[c] + 31) >> 5), bf + a.item_tiles / 2);
         const uint32_t bb = a.bstart[c];
 #pragma unroll
         for (int j = 0; j < (kAhead ? 2 : 0); j++) {
@@ -514,7 +517,7 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
         const int bfirst = (int)a.items[3 * item + 2] >> 1;
         const int n = (int)a.csize[cid];
         const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
-        const int bend = min((n + 31) >> 5, bfirst + kItemTiles / 2);
+        const int bend = min((n + 31) >> 5, bfirst + a.item_tiles / 2);
         if (tid < kItemGroup) {
             s_cnt[tid] = 0;
             if (tid < (int)elen) {
@@ -719,7 +722,7 @@ __global__ void __launch_bounds__(kThreads) batch_inline_kernel(BatchArgs a) {
         const int tfirst = (int)a.items[3 * item + 2];
         const int n = (int)a.csize[cid];
         const uint32_t b0 = a.bstart[cid], v0 = a.vstart[cid];
-        const int tend = min((n + 15) >> 4, tfirst + kItemTiles);
+        const int tend = min((n + 15) >> 4, tfirst + a.item_tiles);
         // this wave's tiles: tfirst + wid + 4j, j < kWaveTiles; codes loaded before staging
         uint4 w[kWaveTiles][NCH];
 #pragma unroll
@@ -1187,8 +1190,8 @@ size_t batch_inline_lds_bytes(int D) {
            (size_t)kGroup * kQSurv * 12;
 }
 
-size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize) {
-    const size_t tiles_per_item = (size_t)kItemTiles * 16;
+size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize, int item_tiles) {
+    const size_t tiles_per_item = (size_t)item_tiles * 16;
     const size_t max_split = (max_csize + tiles_per_item - 1) / tiles_per_item;
     return (pairs / kItemGroup + 1) * max_split + N / tiles_per_item + K + 8;
 }
@@ -1319,11 +1322,11 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
                        (int)kFirstRoundRank, r_end, K, rb.counts);
     const int nb = K * R, nblk = (nb + 255) / 256;
     hipLaunchKernelGGL(bin_scan_kernel, dim3(nblk), dim3(256), 0, s, rb.counts, ba.csize, K, nb,
-                       rb.local, rb.btot);
+                       ba.item_tiles, rb.local, rb.btot);
     hipLaunchKernelGGL(bin_total_kernel, dim3(1), dim3(256), 0, s, rb.btot, nblk, rb.local, K, R,
                        ba.n_items);
     hipLaunchKernelGGL(bin_items_kernel, dim3(nblk), dim3(256), 0, s, rb.counts, ba.csize, K, nb,
-                       rb.local, rb.btot, rb.cursor, (uint32_t*)ba.items);
+                       ba.item_tiles, rb.local, rb.btot, rb.cursor, (uint32_t*)ba.items);
     hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
                        (int)kFirstRoundRank, r_end, K, rb.cursor, (uint32_t*)ba.entries);
     e = hipGetLastError();

@@ -767,6 +767,14 @@ int erq_set_option(erq_index* h, int option, long value) {
             h->batch_overlap = (int)value;
             return ERQ_OK;
         }
+        if (option == ERQ_OPT_BATCH_ITEM_TILES && (value == 8 || value == 16)) {
+            h->batch_item_tiles = (int)value;
+            return ERQ_OK;
+        }
+        if (option == ERQ_OPT_PREP_WG_PER_CU && value >= 1 && value <= 6) {
+            h->prep_wg_per_cu = (int)value;
+            return ERQ_OK;
+        }
         if (option == ERQ_OPT_LATENCY_NQ && value >= 0 && value <= 4096) {
             h->latency_nq = (int)value;
             return ERQ_OK;
@@ -787,6 +795,8 @@ long erq_get_option(const erq_index* h, int option) {
     case ERQ_OPT_BATCH_PAIR_BUDGET: return (long)h->batch_pair_budget;
     case ERQ_OPT_BATCH_OVERLAP: return h->batch_overlap;
     case ERQ_OPT_LATENCY_NQ: return h->latency_nq;
+    case ERQ_OPT_BATCH_ITEM_TILES: return h->batch_item_tiles;
+    case ERQ_OPT_PREP_WG_PER_CU: return h->prep_wg_per_cu;
     default: return -1;
     }
 }
@@ -1005,7 +1015,7 @@ static void batch_layout(const erq_index* h, size_t nq, size_t max_nprobe, int c
     size_t pairs = 0, items = 0, max_items = 0;
     for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0)) {
         const size_t r1 = std::min(next_round(r0), max_nprobe), pr = b.cq * (r1 - r0) + 8;
-        const size_t it = batch_items_bound(pr, K, h->N, maxc);
+        const size_t it = batch_items_bound(pr, K, h->N, maxc, h->batch_item_tiles);
         pairs += pr;
         items += it;
         max_items = std::max(max_items, it);
@@ -1082,7 +1092,8 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
     const size_t cq = b.cq;
     const int grid = (int)std::min<size_t>(
         kMaxBatchGrid, (size_t)std::max(1, h->num_cu) * std::max(1, h->batch_wg_per_cu));
-    const int prep_grid = 6 * ((std::max(1, h->num_cu) + 7) / 8 * 8);  // 6 fit per CU (LDS)
+    // a multiple of 8 workgroups (the preparation's XCD-aware split); 6 fit per CU (LDS)
+    const int prep_grid = h->prep_wg_per_cu * ((std::max(1, h->num_cu) + 7) / 8 * 8);
     std::vector<size_t> rlo, rhi;  // each round's probe ranks
     for (size_t r0 = kExactRanks; r0 < max_nprobe; r0 = next_round(r0)) {
         rlo.push_back(r0);
@@ -1125,6 +1136,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.stats = reinterpret_cast<unsigned long long*>(stats);
         ba.redo = b.redo;
         ba.cand_cap = cap;
+        ba.item_tiles = h->batch_item_tiles;
         ba.num_cu = h->num_cu;
         ba.D = (int)h->D;
         ba.EX = (int)h->EX;
@@ -1605,8 +1617,9 @@ int erq_timing_end(erq_index* h, double* ms_per_stage, uint64_t* count_per_stage
 
 // Times the result-invariant launch shapes on the caller's queries (HIP events on the
 // caller's stream) and keeps the fastest: waves (queries) per workgroup of the per-query
-// scan, and workgroups per CU of the persistent batched kernel.  Holds the handle for the
-// whole sweep, so no other call sees an intermediate setting.
+// scan; workgroups per CU of the persistent batched kernel, tiles per work item and the
+// preparation's workgroups per CU.  Holds the handle for the whole sweep, so no other call
+// sees an intermediate setting.
 int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
                  float* best_ms_out, void* stream) {
     return guarded([&]() -> int {
@@ -1642,7 +1655,7 @@ int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size
             return x == hipSuccess ? ERQ_OK : hip_fail(x, "erq_autotune timing");
         };
         float best_plain = 0, best_batch = 0;
-        int bw = h->plain_waves, bg = h->batch_wg_per_cu;
+        int bw = h->plain_waves;
         for (int wv = 1; wv <= 4 && !rc; wv *= 2) {
             h->plain_waves = wv;
             float ms;
@@ -1653,16 +1666,27 @@ int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size
             }
         }
         h->plain_waves = bw;
-        for (int wg = 1; wg <= 4 && !rc && h->factors == 1; wg++) {
-            h->batch_wg_per_cu = wg;
-            float ms;
-            rc = time_it(true, &ms);
-            if (!rc && (best_batch == 0 || ms < best_batch)) {
-                best_batch = ms;
-                bg = wg;
+        // the batched search: scan workgroups per CU, then the work items' tiles, then the
+        // preparation's workgroups per CU, each kept at its fastest before the next is swept
+        auto sweep = [&](int* knob, const int* vals, int nv) {
+            int best = *knob;
+            float best_ms = 0;
+            for (int i = 0; i < nv && !rc && h->factors == 1; i++) {
+                *knob = vals[i];
+                float ms;
+                rc = time_it(true, &ms);
+                if (!rc && (best_ms == 0 || ms < best_ms)) {
+                    best_ms = ms;
+                    best = vals[i];
+                }
             }
-        }
-        h->batch_wg_per_cu = bg;
+            *knob = best;
+            if (best_ms > 0) best_batch = best_ms;
+        };
+        static const int kWg[] = {1, 2, 3, 4}, kTiles[] = {16, 8}, kPrep[] = {6, 4, 2};
+        sweep(&h->batch_wg_per_cu, kWg, 4);
+        sweep(&h->batch_item_tiles, kTiles, 2);
+        sweep(&h->prep_wg_per_cu, kPrep, 3);
         if (best_ms_out) {
             best_ms_out[0] = best_plain;
             best_ms_out[1] = best_batch;

@@ -48,6 +48,8 @@ struct erq_index {
     int plain_kernel = 0;   // ERQ_OPT_PLAIN_KERNEL
     int plain_waves = 1;    // ERQ_OPT_PLAIN_WAVES
     int batch_wg_per_cu = 2;  // ERQ_OPT_BATCH_WG_PER_CU
+    int batch_item_tiles = 16;  // ERQ_OPT_BATCH_ITEM_TILES
+    int prep_wg_per_cu = 6;     // ERQ_OPT_PREP_WG_PER_CU
     size_t batch_surv_cap = size_t(1) << 26;  // ERQ_OPT_BATCH_SURVIVOR_CAP
     size_t batch_pair_budget = size_t(1) << 23;  // ERQ_OPT_BATCH_PAIR_BUDGET
     int batch_overlap = 0;                       // ERQ_OPT_BATCH_OVERLAP

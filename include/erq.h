@@ -77,14 +77,23 @@ enum {
      * every probed vector's EX-bit code (the per-query scan reads the survivors' only), so it
      * trades throughput for latency.  Its slot buffer (12 bytes per probed vector slot) is
      * allocated on first use, outside HIP graph capture. */
-    ERQ_OPT_LATENCY_NQ = 8
+    ERQ_OPT_LATENCY_NQ = 8,
+    /* 16-vector tiles per work item of the batched rounds: 16 (default; 8 blocks of 32
+     * vectors) or 8.  Smaller items spread a large cluster over more workgroups; each item
+     * reads its entries' digits once.  Results are identical. */
+    ERQ_OPT_BATCH_ITEM_TILES = 9,
+    /* Workgroups per CU of the batched search's preparation kernel, 1..6 (default 6, what its
+     * LDS allows).  Results are identical. */
+    ERQ_OPT_PREP_WG_PER_CU = 10
 };
 int erq_set_option(erq_index* h, int option, long value);
 long erq_get_option(const erq_index* h, int option);
 
-/* Autotuner: times the result-invariant launch shapes (ERQ_OPT_PLAIN_WAVES,
- * ERQ_OPT_BATCH_WG_PER_CU) on these queries with HIP events and keeps the fastest;
- * best_ms_out (nullable, 2 floats) receives the best per-query and batched times. */
+/* Autotuner: times the result-invariant launch shapes (ERQ_OPT_PLAIN_WAVES, then for the
+ * batched search ERQ_OPT_BATCH_WG_PER_CU, ERQ_OPT_BATCH_ITEM_TILES and ERQ_OPT_PREP_WG_PER_CU,
+ * each kept at its fastest before the next is swept) on these queries with HIP events and
+ * keeps the fastest; best_ms_out (nullable, 2 floats) receives the best per-query and batched
+ * times.  It holds the handle for the whole sweep. */
 int erq_autotune(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, size_t nprobe,
                  float* best_ms_out, void* stream);
 

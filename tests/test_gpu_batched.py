@@ -41,10 +41,14 @@ def _probe_lists(idx, qr, nprobes):
     return [oi.centroid_topn(qr[i], int(nprobes[i])) for i in range(qr.shape[0])]
 
 
-def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None, pair_budget=None):
+def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None, pair_budget=None,
+                 options=()):
     import torch
     from erq.gpu import IVF
     g = IVF.from_index(idx)
+    for opt, val in options:
+        g.set_option(opt, val)
+        assert g.option(opt) == val
     if survivor_cap is not None:
         g.set_option(4, survivor_cap)   # ERQ_OPT_BATCH_SURVIVOR_CAP
     if pair_budget is not None:
@@ -64,12 +68,13 @@ def _gpu_batched(idx, qr, k, probe_lists, max_nprobe, survivor_cap=None, pair_bu
     return out
 
 
-def _check(idx, q, k, nprobes, label, survivor_cap=None, pair_budget=None, redo=0):
+def _check(idx, q, k, nprobes, label, survivor_cap=None, pair_budget=None, redo=0, options=()):
     qr = rotate_np(idx, q)
     pl = _probe_lists(idx, qr, nprobes)
     oid, od, ost = _oracle_rounds(idx, qr, k, pl)
     print(f"[{label}] k={k} nq={qr.shape[0]} max_nprobe={int(max(nprobes))}", flush=True)
-    gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)), survivor_cap, pair_budget)
+    gid, gd, st = _gpu_batched(idx, qr, k, pl, int(max(nprobes)), survivor_cap, pair_budget,
+                               options)
     print(f"[{label}] stats={st.tolist()} oracle blocks/survivors/candidates={ost.tolist()}",
           flush=True)
     # [5]: (query, round) rescans after a candidate overflow
@@ -210,6 +215,17 @@ def test_batched_search_in_hip_graph_replays_match_eager():
     g.close()
 
 
+@pytest.mark.parametrize("tiles,prep_wg", [(8, 6), (16, 1), (8, 3)])
+def test_batched_item_tiles_and_prep_grid_keep_results(tiles, prep_wg):
+    """ERQ_OPT_BATCH_ITEM_TILES (9) and ERQ_OPT_PREP_WG_PER_CU (10) change only how the
+    rounds are split into work items and how the preparation is spread: the results and the
+    work counters stay the oracle's."""
+    idx, q, _ = _build(("d128", 4), 20000, 128, 64, 4)
+    rng = np.random.default_rng(7)
+    nprobes = rng.integers(1, 64, q.shape[0])
+    _check(idx, q, 100, nprobes, f"tiles {tiles} prep {prep_wg}", options=((9, tiles), (10, prep_wg)))
+
+
 def test_autotuner_keeps_results():
     import torch
     from erq.gpu import IVF
@@ -219,8 +235,10 @@ def test_autotuner_keeps_results():
     pi, pd = g.search_dev(qr, 100, 16)
     bi, bd = g.search_batched_dev(qr, 100, 16)
     ms = g.autotune(qr, 100, 16)
-    print("autotune best ms", ms, "waves", g.option(2), "wg/cu", g.option(3))
+    print("autotune best ms", ms, "waves", g.option(2), "wg/cu", g.option(3), "tiles",
+          g.option(9), "prep wg/cu", g.option(10))
     assert 1 <= g.option(2) <= 4 and 1 <= g.option(3) <= 4
+    assert g.option(9) in (8, 16) and g.option(10) in (2, 4, 6)
     pi2, pd2 = g.search_dev(qr, 100, 16)
     bi2, bd2 = g.search_batched_dev(qr, 100, 16)
     torch.cuda.synchronize()
