@@ -64,7 +64,7 @@ size_t batch_inline_lds_bytes(int D);  // inline pass
 size_t prep_wave_bytes(int D);
 // Upper bound on the work items of one round of at most `pairs` (query, cluster) pairs.
 size_t batch_items_bound(size_t pairs, size_t K, size_t N, size_t max_csize, int item_tiles);
-hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
+hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, int factors, uint8_t* bp,
                            hipStream_t s);
 hipError_t launch_zero(uint32_t* p, size_t n, hipStream_t s);
 // exl (N x 16 x exl_words(D, EX) u32, erq_device.h) from long_code.

@@ -46,12 +46,13 @@ constexpr int kQSurv = kItemTiles * 16;  // survivors of one query in one work i
 constexpr int kMainSurv = 128; // of them kept in LDS by the main pass (more: inline pass)
 
 // ---------------------------------------------------------------- relayout
-// One thread per output byte: (block, chunk, vector, byte B).
+// One thread per output byte: (block, chunk, vector, byte B).  Blocks of BB bytes: 4D code
+// bytes and the factors (128 bytes per factor per 32 vectors, one factor in the HA format,
+// four in the non-HA one).
 __global__ void relayout_kernel(const uint8_t* __restrict__ short_data, size_t nblocks, int D,
-                                int nchunk, uint8_t* __restrict__ bp) {
+                                int nchunk, size_t BB, uint8_t* __restrict__ bp) {
     const size_t per_block = (size_t)nchunk * 32 * 64;
     const size_t total = nblocks * per_block;
-    const size_t BB = 4 * (size_t)D + 128;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
          i += (size_t)gridDim.x * blockDim.x) {
         size_t blk = i / per_block;
@@ -1170,14 +1171,14 @@ int bp_nchunk(int D) {
     return c == 5 ? 6 : (c == 7 ? 8 : c);
 }
 
-hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, uint8_t* bp,
-                           hipStream_t s) {
+hipError_t launch_relayout(const uint8_t* short_data, size_t nblocks, int D, int factors,
+                           uint8_t* bp, hipStream_t s) {
     if (nblocks == 0) return hipSuccess;
     const int nchunk = bp_nchunk(D);
     size_t total = nblocks * (size_t)nchunk * 32 * 64;
     int blocks = (int)std::min<size_t>((total + 255) / 256, 1 << 16);
     hipLaunchKernelGGL(relayout_kernel, dim3(blocks), dim3(256), 0, s, short_data, nblocks, D,
-                       nchunk, bp);
+                       nchunk, 4 * (size_t)D + 128 * (size_t)factors, bp);
     return hipGetLastError();
 }
 

@@ -680,8 +680,10 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
                        float* dists, uint64_t* stats, hipStream_t s, int max_probes = 0,
                        uint32_t* order_buf = nullptr, size_t own_lists = 0) {
     ScanArgs a = scan_args(h, q_rot_dev, nq, k, off, pid, pd, ids, dists, stats, max_probes);
-    const bool nonha = h->factors != 1;  // Searcher's format: the LUT kernel of erq_nonha.hip
-    if (h->plain_kernel != 1 && !nonha) {  // auto or MFMA: bit-plane integer path
+    const bool nonha = h->factors != 1;  // Searcher's format: the kernels of erq_nonha.hip
+    // auto or MFMA: bit-plane integer path (the non-HA format's up to 4 chunks, where its u16
+    // half-sums cannot wrap)
+    if (h->plain_kernel != 1 && (!nonha || a.nchunk <= 4)) {
         if (!order_buf) {
             int rc = ensure_bp(h, s);
             if (rc) return rc;
@@ -693,13 +695,16 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
     }
     // own_lists > 0: the probe lists come from the library's selection (no repeated
     // cluster), at most own_lists probes per query
-    if (own_lists > 0 && !order_buf && a.bp && a.nchunk <= 4 && h->latency_nq > 0 &&
+    if (own_lists > 0 && !order_buf && !nonha && a.bp && a.nchunk <= 4 && h->latency_nq > 0 &&
         nq <= (size_t)h->latency_nq &&
         (!capturing(s) || latency_bytes(h, nq, own_lists, nullptr) <= h->lat_cap))
         return scan_latency_locked(h, a, nq, own_lists, s);
-    if (h->plain_order && nq > 1) {
-        // queries by scan cost, heaviest first: the result of each query is unchanged
-        const size_t need = nq + h->K + 1;
+    const bool order = h->plain_order && nq > 1;
+    const bool redo = nonha && a.bp;  // the non-HA MFMA scan's hand-back list
+    if (order || redo) {
+        // queries by scan cost, heaviest first: the result of each query is unchanged;
+        // after them the hand-back list (nq + 1)
+        const size_t need = nq + h->K + 1 + (redo ? nq + 1 : 0);
         if (!order_buf && need > h->order_cap) {
             if (h->order) {
                 hipError_t e = hipDeviceSynchronize();  // earlier calls may still read it
@@ -721,10 +726,13 @@ static int scan_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t k
         int shift = 0;
         while ((bound >> shift) >= h->K) shift++;
         uint32_t* ob = order_buf ? order_buf : h->order;
-        hipError_t e = launch_query_order(off, pid, h->csize, (int)nq, (int)h->K, max_probes,
-                                          shift, ob, ob + h->K + 1, s);
-        if (e != hipSuccess) return hip_fail(e, "query order");
-        a.qorder = ob + h->K + 1;
+        if (order) {
+            hipError_t e = launch_query_order(off, pid, h->csize, (int)nq, (int)h->K,
+                                              max_probes, shift, ob, ob + h->K + 1, s);
+            if (e != hipSuccess) return hip_fail(e, "query order");
+            a.qorder = ob + h->K + 1;
+        }
+        if (redo) a.nonha_redo = ob + h->K + 1 + nq;
     }
     StageTimer tim(h, ERQ_STAGE_REPLAY, s);
     hipError_t e = nonha ? launch_scan_nonha(a, s) : launch_scan(a, s);
@@ -920,7 +928,7 @@ int erq_search_dev_noalloc(erq_index* h, const float* q_rot_dev, size_t nq, size
         uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
         float* pd = ws.take<float>(nq * nprobe);
         uint32_t* off = ws.take<uint32_t>(nq + 1);
-        uint32_t* order = ws.take<uint32_t>(nq + h->K + 1);
+        uint32_t* order = ws.take<uint32_t>(2 * nq + h->K + 2);  // order bins, hand-back list
         int rc = probe_into(h, q_rot_dev, nq, nprobe, pid, pd, s, base);
         if (rc) return rc;
         hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off,
@@ -938,7 +946,7 @@ size_t erq_search_noalloc_bytes(const erq_index* h, size_t nq, size_t nprobe) {
     ws.take<uint32_t>(nq * nprobe);
     ws.take<float>(nq * nprobe);
     ws.take<uint32_t>(nq + 1);
-    ws.take<uint32_t>(nq + h->K + 1);
+    ws.take<uint32_t>(2 * nq + h->K + 2);
     return ws.off;
 }
 
@@ -958,7 +966,7 @@ static int ensure_bp(erq_index* h, hipStream_t s) {
         h->bp = nullptr;
         return hip_fail(e, "bit-plane codes alloc");
     }
-    e = launch_relayout(h->short_data, h->nblocks, (int)h->D, h->bp, s);
+    e = launch_relayout(h->short_data, h->nblocks, (int)h->D, h->factors, h->bp, s);
     if (e != hipSuccess) return hip_fail(e, "bit-plane relayout");
     const size_t exl_bytes = h->N * 16 * (size_t)exl_words((int)h->D, (int)h->EX) * 4;
     e = hipMalloc(&h->exl, exl_bytes ? exl_bytes : 64);

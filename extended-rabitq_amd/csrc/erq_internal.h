@@ -142,6 +142,9 @@ struct ScanArgs {
     float* est_exd;
     uint32_t* est_id;
     uint32_t* est_off;
+    // Non-HA int8 MFMA scan (a.bp set, nchunk <= 4): [0] count, then the queries it handed
+    // back to the LUT kernel (a digit above 16383); nullptr: the LUT kernel alone.
+    uint32_t* nonha_redo;
 };
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
 // The per-query scan of a non-HA index (Searcher.hpp:85-236): the LUT kernel of erq_nonha.hip

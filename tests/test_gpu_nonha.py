@@ -3,6 +3,8 @@ reference is built without HIGH_ACC_FAST_SCAN): the HIP kernel of erq_nonha.hip 
 C-ABI against the oracle's orc_nonha_search_probes, on the same index, rotated queries and
 probe lists.
 
+Both integer kernels (the u16 LUT, and the int8 MFMA over the bit-plane codes up to D = 2048)
+are run where they apply.
 Bar: top-k ids and FP32 distances bit-identical (the FastScan integers of accumulate_robust,
 the unsigned 14-bit digits, the four-factor lower bound, the per-vector running threshold and
 the pool order), and the same block / vector / survivor counts.  Shapes: every EX at D = 128,
@@ -45,11 +47,12 @@ def _oracle(idx, qr, k, nprobe):
     return ids, ds, tot
 
 
-def _gpu(idx, qr, k, nprobe):
+def _gpu(idx, qr, k, nprobe, mode=0):
     import torch
     from erq.gpu import IVF
     g = IVF.from_index(idx)
     assert not g.high_acc()
+    g.set_plain_kernel(mode)   # 0 auto (int8 MFMA up to D = 2048), 1 LUT
     stats = torch.zeros(8, dtype=torch.int64, device="cuda")
     ids, d = g.search_dev(torch.from_numpy(qr).cuda(), k, nprobe, stats=stats)
     torch.cuda.synchronize()
@@ -58,13 +61,15 @@ def _gpu(idx, qr, k, nprobe):
     return out
 
 
-def _check(idx, qr, k, nprobe, tag):
-    print(f"[nonha {tag} D={idx.D} EX={idx.EX} k={k} nprobe={nprobe}]", flush=True)
+def _check(idx, qr, k, nprobe, tag, modes=(0,)):
     oid, od, ost = _oracle(idx, qr, k, nprobe)
-    gid, gd, gst = _gpu(idx, qr, k, nprobe)
-    np.testing.assert_array_equal(gid, oid, err_msg=tag)
-    np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32), err_msg=tag)
-    assert tuple(gst[:3]) == tuple(ost), (tag, gst.tolist(), ost.tolist())
+    for mode in modes:
+        print(f"[nonha {tag} D={idx.D} EX={idx.EX} k={k} nprobe={nprobe} kernel {mode}]",
+              flush=True)
+        gid, gd, gst = _gpu(idx, qr, k, nprobe, mode)
+        np.testing.assert_array_equal(gid, oid, err_msg=tag)
+        np.testing.assert_array_equal(gd.view(np.uint32), od.view(np.uint32), err_msg=tag)
+        assert tuple(gst[:3]) == tuple(ost), (tag, mode, gst.tolist(), ost.tolist())
 
 
 @pytest.mark.parametrize("bits", [3, 4, 5, 7, 8, 9])
@@ -72,18 +77,18 @@ def test_nonha_every_ex_d128(bits):
     idx, q = _build(("d128", bits), 6000, 128, 24, bits)
     qr = rotate_np(idx, q)
     for k, npb in ((1, 3), (10, 6), (100, 12)):
-        _check(idx, qr, k, npb, f"d128 b{bits}")
+        _check(idx, qr, k, npb, f"d128 b{bits}", modes=(0, 1))
 
 
 def test_nonha_d1000_ex7():
     idx, q = _build(("d1000",), 4000, 1000, 16, 8)
     assert idx.D == 1024  # zero-padded dims
-    _check(idx, rotate_np(idx, q), 50, 8, "d1000")
+    _check(idx, rotate_np(idx, q), 50, 8, "d1000", modes=(0, 1))
 
 
 def test_nonha_d1536_ex3():
     idx, q = _build(("d1536",), 4000, 1536, 16, 4)
-    _check(idx, rotate_np(idx, q), 100, 10, "d1536")
+    _check(idx, rotate_np(idx, q), 100, 10, "d1536", modes=(0, 1))
 
 
 def test_nonha_d3072_ex8_natural():
@@ -97,7 +102,7 @@ def test_nonha_ragged_clusters_and_large_k():
     assert list(idx.cluster_sizes) == sizes
     qr = rotate_np(idx, q)
     for k, npb in ((1, 8), (7, 3), (300, 8), (1024, 8)):
-        _check(idx, qr, k, npb, "ragged")
+        _check(idx, qr, k, npb, "ragged", modes=(0, 1))
 
 
 @pytest.mark.parametrize("k", [20, 512])
