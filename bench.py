@@ -69,6 +69,9 @@ def parse(argv=None):
     ap.add_argument("--profile-plain", type=int, default=0, metavar="NPROBE",
                     help="only time the plain path (rotate, probe, scan) at this nprobe and "
                          "print its roofline: the command the kernel profiles are taken with")
+    ap.add_argument("--profile-nonha", type=int, default=0, metavar="NPROBE",
+                    help="only build the non-HA index of the same data and time its per-query "
+                         "search at this nprobe")
     ap.add_argument("--profile-learned", type=float, default=0.0, metavar="MASS",
                     help="train (or --load-mlp) the probe predictor, then only time the learned "
                          "batched search at this mass and print its stages")
@@ -515,7 +518,7 @@ def cpu_baseline(r, plain):
            "host": host_info()}
 
 
-def nonha_path(r, nprobe):
+def nonha_path(r, nprobe, x=None, profile=False):
     """The non-HA format (the reference built without HIGH_ACC_FAST_SCAN: Searcher, four
     factors per vector) on the same data, clusters and rotation: built on the GPU
     (erq_build_index_dev_nonha), searched per query (erq_search_dev) at the plain path's
@@ -525,7 +528,7 @@ def nonha_path(r, nprobe):
     a, torch = r.a, r.torch
     c, assign, P = a.build_inputs
     t0 = time.time()
-    idx2 = gpu.build_index_dev(r.x, c, assign, a.bits, P, high_acc=False)
+    idx2 = gpu.build_index_dev(r.x if x is None else x, c, assign, a.bits, P, high_acc=False)
     build_s = time.time() - t0
     g2 = IVF.from_index(idx2)
     try:
@@ -533,7 +536,7 @@ def nonha_path(r, nprobe):
         st = torch.zeros(8, dtype=torch.int64, device="cuda")
         ids, _ = g2.search_dev(qr, a.k, nprobe, stats=st)
         torch.cuda.synchronize()
-        rec, rat = r.recall(ids), r.ratio(ids)
+        rec, rat = (None, None) if profile else (r.recall(ids), r.ratio(ids))
         s = st.cpu().numpy().tolist()
         ms = r.timed(lambda: g2.search_dev(g2.rotate_dev(r.q), a.k, nprobe), steps=3, warmup=1)
         g2.timing_begin(64)
@@ -542,10 +545,15 @@ def nonha_path(r, nprobe):
         scan_ms = g2.timing_end()["replay"][0]
         D, EX = idx2.D, idx2.EX
         nbytes = s[0] * (4 * D + 512) + s[2] * (D * EX // 8 + 8)
-        log(f"[nonha] nprobe={nprobe} recall={rec:.4f} {ms:.2f} ms per batch, scan {scan_ms:.2f} ms")
+        kern = "nonha_mfma_kernel" if D <= 2048 else "nonha_scan_kernel"
+        name = f"{kern}_d{a.dim}_n{a.n // 1_000_000}m_b{a.bits}_nprobe{nprobe}.md"
+        traffic, src = profiled_traffic(a, name)
+        log(f"[nonha] nprobe={nprobe} recall={rec} {ms:.2f} ms per batch, scan {scan_ms:.2f} ms")
         return {"nprobe": nprobe, "qps": round(a.nq / ms * 1e3, 1), "ms": round(ms, 3),
-                "recall": round(rec, 4), "ratio": round(rat, 5), "build_gpu_s": round(build_s, 3),
-                "roofline": {"bound": "hbm", "kernel": "nonha_scan_kernel",
+                "recall": None if rec is None else round(rec, 4),
+                "ratio": None if rat is None else round(rat, 5), "build_gpu_s": round(build_s, 3),
+                "roofline": {"bound": "hbm", "kernel": kern, "traffic": traffic,
+                             "traffic_source": src,
                              "achieved": round(nbytes / scan_ms / 1e6, 1), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s",
                              "frac": round(nbytes / scan_ms / 1e6 / HBM_PEAK_GBS, 4),
@@ -634,13 +642,16 @@ def main(argv=None):
     x, q, idx, mix = build_workload(a, torch)
     g = IVF.from_index(idx)
     g.set_option(7, a.overlap)
-    if a.profile_plain or a.profile_learned:
-        del x
+    if a.profile_plain or a.profile_learned or a.profile_nonha:
         r = Runner(a, g, idx, q, None, torch)
-        if a.profile_plain:
+        if a.profile_nonha:
+            print(json.dumps({"profile": "nonha", "nq": a.nq,
+                              **nonha_path(r, a.profile_nonha, x=x, profile=True)}), flush=True)
+        elif a.profile_plain:
             profile_plain(a, r)
         else:
             profile_learned(a, r, mix)
+        del x
         g.close()
         return
 
