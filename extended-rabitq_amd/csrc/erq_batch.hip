@@ -81,7 +81,7 @@ __global__ void relayout_kernel(const uint8_t* __restrict__ short_data, size_t n
 template <int EX>
 __global__ void exl_relayout_kernel(const uint8_t* __restrict__ long_code, size_t N, int D, int W,
                                     uint32_t* __restrict__ exl) {
-    constexpr int vpd = 32 / EX;
+    constexpr int vpd = exl_vpd(EX), sp = exl_sp(EX);
     const size_t CL = (size_t)D * EX / 8;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < N * 16;
          t += (size_t)gridDim.x * blockDim.x) {
@@ -100,7 +100,8 @@ __global__ void exl_relayout_kernel(const uint8_t* __restrict__ long_code, size_
                     out[w++] = cur;
                     cur = 0;
                 }
-                cur |= (uint32_t)e[s4] << ((i % vpd) * EX);
+                const int r = i % vpd;  // value r of the dword: byte r % 4, slot r / 4
+                cur |= (uint32_t)e[s4] << (8 * (r & 3) + sp * (r >> 2));
             }
         }
         out[w++] = cur;
@@ -341,7 +342,7 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
             P.c1 = (float)(0.5 * (double)sumq - (double)0.58f);
             P.c2 = (-5.0f * P.y) * a.one_over_sqrtD;
             P.qs = P.y * P.y;
-            P.thr = 0.f;  // the round's threshold is read when the round runs
+            P.thr = 0.f;  // the round's threshold: batch_scan_kernel writes it
             reinterpret_cast<QParam*>(a.qparams)[e] = P;
         }
     }
@@ -359,11 +360,11 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
 // 16 lane-strided FMA chains over u * code (ex_chain on the lane-major codes), reduced by
 // the xor butterfly, then the distance in double as C promotes it (HASearcher.hpp:183-231).
 // uf(i) gives u for the lane's dim 16 i + l.
-template <int EX, typename U>
-__device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P, const U& uf,
+template <int EX>
+__device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P, const float* ur,
                                             const uint32_t* lanecode, bool act, float ipq,
                                             float on, uint32_t vi) {
-    const float part = act ? ex_chain<EX>(uf, lanecode, a.D >> 4, a.exl_w) : 0.f;
+    const float part = act ? ex_chain<EX>(ur, lanecode, a.D >> 4, a.exl_w) : 0.f;
     const float ip = butterfly16(part);
     if (!act) return FLT_MAX;
     const float R = (float)(1 << EX);
@@ -525,6 +526,9 @@ __global__ void __launch_bounds__(kThreads) batch_scan_kernel(BatchArgs a) {
                 QParam P = reinterpret_cast<const QParam*>(a.qparams)[e0 + tid];
                 P.thr = a.pool_d[(size_t)P.q * a.k + a.k - 1];  // k-th distance at round start
                 qp[tid] = P;
+                // for reest_kernel, which then needs no dependent read of the pool: written
+                // by the entry's first item only (the pool does not change within the round)
+                if (bfirst == 0) reinterpret_cast<QParam*>(a.qparams)[e0 + tid].thr = P.thr;
             }
         }
         if constexpr (kAhead) {
@@ -853,7 +857,7 @@ __global__ void __launch_bounds__(kThreads) batch_inline_kernel(BatchArgs a) {
                 float ipq = (float)I * P.delta;
                 if constexpr (kWrap) act = wrap_retest(a, P, qvw, vloc, I, on, ln, ipq) && act;
                 const float exd = reestimate<EX>(
-                    a, P, [&](int i) { return uw[ln * S + i]; },
+                    a, P, uw + ln * S,
                     a.exl + ((size_t)vi * 16 + ln) * a.exl_w, act, ipq, on, vi);
                 if (act && ln == 0) {
                     if (exd <= P.thr) {
@@ -957,9 +961,9 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
         const uint4 rec_next = load_rec(r + m + lg);
         const uint32_t e_next = m < 4 ? __shfl(rec.x, 16 * m) : 0xFFFFFFFFu;
         if (e0 != staged) {
+            // the parameters carry the round's threshold (batch_scan_kernel wrote it)
             if (e0 != pn_e) P = reinterpret_cast<const QParam*>(a.qparams)[e0];
             else P = Pn;
-            P.thr = a.pool_d[(size_t)P.q * a.k + a.k - 1];
             const float* qv = a.q_rot + (size_t)P.q * D;
             const float* cen = a.centroids + (size_t)P.cid * D;
             wave_sync();
@@ -986,7 +990,7 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
         bool keep = act;
         if constexpr (wrap)
             keep = wrap_retest(a, P, qvw, (int)(vi - a.vstart[P.cid]), I, on, l16, ipq) && act;
-        const float exd = reestimate<EX>(a, P, [&](int i) { return uw[l16 * S + i]; },
+        const float exd = reestimate<EX>(a, P, uw + l16 * S,
                                          a.exl + ((size_t)vi * 16 + l16) * a.exl_w, keep, ipq,
                                          on, vi);
         if (keep && l16 == 0) {
@@ -1289,9 +1293,25 @@ static hipError_t launch_reest_w(const BatchArgs& ba, int scan_grid, hipStream_t
         }
         if (e != hipSuccess) return e;
     }
-    // as many workgroups as stay resident together (LDS-bound), a multiple of 8: each takes
-    // a fixed share of the records, so a second partial wave of workgroups would be a tail
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, 163840 / (lds + 1024)));
+    // as many workgroups as stay resident together (bound by LDS or registers), a multiple of
+    // 8: each takes a fixed share of the records, so a second partial wave of workgroups
+    // would be a tail
+    int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, 163840 / (lds + 1024)));
+    {
+        int occ = 0;
+        hipError_t e = hipSuccess;
+        switch (ba.EX) {
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reest_kernel<2, WRAP>, 256, lds); break;
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reest_kernel<3, WRAP>, 256, lds); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reest_kernel<4, WRAP>, 256, lds); break;
+        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reest_kernel<6, WRAP>, 256, lds); break;
+        case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reest_kernel<7, WRAP>, 256, lds); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reest_kernel<8, WRAP>, 256, lds); break;
+        default: return hipErrorInvalidValue;
+        }
+        if (e != hipSuccess) return e;
+        if (occ > 0) per_cu = std::min(per_cu, occ);
+    }
     const dim3 grid(ba.num_cu > 0 ? (unsigned)(per_cu * ((ba.num_cu + 7) / 8 * 8))
                                   : (unsigned)(scan_grid * kReestWG));
     switch (ba.EX) {

@@ -107,9 +107,13 @@ __device__ __forceinline__ void decode4(const uint8_t* __restrict__ c, int l, fl
 // Lane-major copy of the EX-bit codes ("exl"), for the 16-lane re-estimate: lane l of a
 // vector's group handles dims d = 16 i + l (i = 0 .. D/16 - 1), in increasing i, which is
 // the order of the reference's lane-strided FMA chains.  Per vector 16 x W dwords, lane l's
-// W dwords contiguous (16-byte aligned); value i sits in dword i / vpd at bit (i % vpd) * EX,
-// vpd = floor(32 / EX).
-__host__ __device__ constexpr int exl_vpd(int ex) { return 32 / ex; }
+// W dwords contiguous (16-byte aligned).  Values go four to a dword's bytes: dword w holds
+// values w vpd .. w vpd + vpd - 1, value w vpd + 4 h + t in byte t at bit 8 t + sp h, so the
+// four values 4 h .. 4 h + 3 of a dword are the bytes of (dword >> sp h) & mask and convert
+// with v_cvt_f32_ubyte0..3.  sp = 2 for EX = 2, 4 for EX = 3, 4 and 8 for EX = 6..8.
+__host__ __device__ constexpr int exl_sp(int ex) { return ex <= 2 ? 2 : (ex <= 4 ? 4 : 8); }
+__host__ __device__ constexpr int exl_npb(int ex) { return 8 / exl_sp(ex); }  // values per byte
+__host__ __device__ constexpr int exl_vpd(int ex) { return 4 * exl_npb(ex); }
 __host__ __device__ inline int exl_words(int D, int ex) {
     const int vpd = exl_vpd(ex), w = (D / 16 + vpd - 1) / vpd;
     return (w + 3) & ~3;
@@ -127,11 +131,16 @@ __device__ __forceinline__ int ul_index(int d, int S) { return (d & 15) * S + (d
 // float rounding boundary without being the float itself, so rounding the product to
 // float gives exactly the IEEE float division.
 // The reference's (int)(tmp + 0.5 - [tmp < 0]), evaluated in double, rounds half away from
-// zero: it equals roundf(tmp) for every float |tmp| <= 8192.5 (|tmp| <= 8191 here; checked
-// for every float in that range), and roundf needs no double arithmetic.
+// zero.  Here: (double)tmp + copysign(0.5, tmp), truncated toward zero by the conversion.
+// The sum is exact whenever |tmp| >= 2^-29 (tmp has 24 significant bits and |tmp| < 2^14, so
+// the sum needs at most 53); below that it rounds to a value of magnitude < 1, which
+// truncates to 0 as it should.  For |tmp| < 0.5 the exact sum has magnitude < 1 (0 after
+// truncation); otherwise trunc(|tmp| + 0.5) = floor(|tmp| + 0.5).  Four instructions after
+// tmp instead of roundf's seven.
 __device__ __forceinline__ int quantize_digit(float u, double rd) {
     const float tmp = (float)((double)u * rd);
-    return (int)roundf(tmp);
+    const double t = (double)tmp;
+    return (int)(t + copysign(0.5, t));
 }
 
 // Byte 0 of each of a, b, c, d as the bytes 0..3 of one word.
@@ -275,11 +284,17 @@ __device__ __forceinline__ void quantize_digits(const float* uq, int S, int D, i
                 q0[b] = quantize_digit(uq[ul_index(d, S)], rd);
                 q1[b] = quantize_digit(uq[ul_index(d + 1, S)], rd);
             }
-            // digit 0 = q & 127 (byte b of the word), digit 1 = floor(q / 128) (its low byte)
-            lo0 = pack_low_bytes(q0[0], q0[1], q0[2], q0[3]) & 0x7F7F7F7Fu;
-            lo1 = pack_low_bytes(q1[0], q1[1], q1[2], q1[3]) & 0x7F7F7F7Fu;
-            hi0 = pack_low_bytes(q0[0] >> 7, q0[1] >> 7, q0[2] >> 7, q0[3] >> 7);
-            hi1 = pack_low_bytes(q1[0] >> 7, q1[1] >> 7, q1[2] >> 7, q1[3] >> 7);
+            // digit 0 = q & 127 (byte b of the word), digit 1 = floor(q / 128) (its low byte:
+            // bits 7..14 of q, as |q| < 2^13).  The low halves of two q share a word, so one
+            // shift serves both high digits.
+            const uint32_t a01 = __builtin_amdgcn_perm((uint32_t)q0[1], (uint32_t)q0[0], 0x05040100u);
+            const uint32_t a23 = __builtin_amdgcn_perm((uint32_t)q0[3], (uint32_t)q0[2], 0x05040100u);
+            const uint32_t b01 = __builtin_amdgcn_perm((uint32_t)q1[1], (uint32_t)q1[0], 0x05040100u);
+            const uint32_t b23 = __builtin_amdgcn_perm((uint32_t)q1[3], (uint32_t)q1[2], 0x05040100u);
+            lo0 = __builtin_amdgcn_perm(a23, a01, 0x06040200u) & 0x7F7F7F7Fu;
+            lo1 = __builtin_amdgcn_perm(b23, b01, 0x06040200u) & 0x7F7F7F7Fu;
+            hi0 = __builtin_amdgcn_perm(a23 >> 7, a01 >> 7, 0x06040200u);
+            hi1 = __builtin_amdgcn_perm(b23 >> 7, b01 >> 7, 0x06040200u);
         }
         const int un = 32 * c + 8 * pp + g;  // plane 2 pp; plane 2 pp + 1 is un + 4
         dst[8 * un + jw] = lo0;
@@ -361,79 +376,110 @@ __device__ __forceinline__ void store_qv(const float* uq, int S, int D, int nchu
     }
 }
 
+// The values 4 h .. 4 h + 3 of an exl dword x as floats (h < exl_npb(EX)).  The byte mask
+// is applied by an opaque v_and so that each byte then converts with one v_cvt_f32_ubyteN
+// (the compiler would otherwise fold mask and shift into a v_bfe_u32 per value).
+template <int EX>
+__device__ __forceinline__ float4 exl_group(uint32_t x, int h) {
+    constexpr uint32_t m = ((1u << EX) - 1u) * 0x01010101u;
+    uint32_t y = x >> (exl_sp(EX) * h);
+    if constexpr (m != 0xFFFFFFFFu) asm("v_and_b32 %0, %1, %2" : "=v"(y) : "s"(m), "v"(y));
+    return make_float4((float)(y & 0xFFu), (float)((y >> 8) & 0xFFu), (float)((y >> 16) & 0xFFu),
+                       (float)(y >> 24));
+}
+
+// fma(u_i, e_i, part) for the four values of group h of dword x, in order.
+template <int EX>
+__device__ __forceinline__ void ex_group_fma(float4 u, uint32_t x, int h, float& part) {
+    const float4 e = exl_group<EX>(x, h);
+    part = fmaf(u.x, e.x, part);
+    part = fmaf(u.y, e.y, part);
+    part = fmaf(u.z, e.z, part);
+    part = fmaf(u.w, e.w, part);
+}
+
 // One lane's share of <u, e> for the re-estimate: part = fma(u_i, e_i, part) over
-// i = 0 .. nvals-1 in order; uf(i) gives u for dim 16 i + l, code = the lane's W dwords.
-template <int EX, typename UF>
-__device__ __forceinline__ float ex_chain(const UF& uf, const uint32_t* __restrict__ code,
-                                          int nvals, int W) {
-    constexpr int vpd = 32 / EX;
-    constexpr uint32_t mask = (1u << EX) - 1u;
-    float part = 0.f;
+// i = 0 .. nvals-1 in order (nvals a multiple of 4); ur = the lane's u row (u_i = ur[i], for
+// dim 16 i + l, 16-byte aligned), code = the lane's W exl dwords.  u is read four values at
+// a time; only the last 16-byte code word can be partly used.
+template <int EX>
+__device__ __forceinline__ float ex_chain(const float* __restrict__ ur,
+                                          const uint32_t* __restrict__ code, int nvals, int W) {
+    constexpr int npb = exl_npb(EX);
+    const float4* u4 = reinterpret_cast<const float4*>(ur);
     const uint4* cw = reinterpret_cast<const uint4*>(code);
-    const int n4 = W / 4;
-    for (int b4 = 0; b4 < n4; b4 += 4) {
+    const int ng = nvals >> 2;
+    const int nfd = ng / npb, rem = ng - nfd * npb;  // full dwords, groups of the last one
+    const int nq = (nfd + (rem ? 1 : 0) + 3) >> 2;   // 16-byte code words holding values
+    float part = 0.f;
+    for (int b4 = 0; b4 < nq; b4 += 4) {
         uint4 c4[4];  // up to four 16-byte loads in flight before the chain uses them
 #pragma unroll
         for (int j4 = 0; j4 < 4; j4++)
-            if (b4 + j4 < n4) c4[j4] = cw[b4 + j4];
+            if (b4 + j4 < nq) c4[j4] = cw[b4 + j4];
 #pragma unroll
         for (int j4 = 0; j4 < 4; j4++) {
-            if (b4 + j4 >= n4) break;
+            if (b4 + j4 >= nq) break;
             const uint32_t cs[4] = {c4[j4].x, c4[j4].y, c4[j4].z, c4[j4].w};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int i0 = (4 * (b4 + j4) + j) * vpd;
+                const int dw = 4 * (b4 + j4) + j;
+                const int nh = dw < nfd ? npb : (dw == nfd ? rem : 0);
 #pragma unroll
-                for (int t = 0; t < vpd; t++)
-                    if (i0 + t < nvals)
-                        part = fmaf(uf(i0 + t), (float)((cs[j] >> (t * EX)) & mask), part);
+                for (int h = 0; h < npb; h++)
+                    if (h < nh) ex_group_fma<EX>(u4[dw * npb + h], cs[j], h, part);
             }
         }
     }
+    (void)W;
     return part;
 }
 
 // ex_chain for two vectors at once (codes ca, cb; cb unused when !has_b): their loads are
 // issued together and each u value feeds both chains, each chain in the order of ex_chain.
-template <int EX, typename UF>
-__device__ __forceinline__ void ex_chain2(const UF& uf, const uint32_t* __restrict__ ca,
+template <int EX>
+__device__ __forceinline__ void ex_chain2(const float* __restrict__ ur,
+                                          const uint32_t* __restrict__ ca,
                                           const uint32_t* __restrict__ cb, bool has_b,
                                           int nvals, int W, float& pa, float& pb) {
-    constexpr int vpd = 32 / EX;
-    constexpr uint32_t mask = (1u << EX) - 1u;
-    pa = 0.f;
-    pb = 0.f;
+    constexpr int npb = exl_npb(EX);
+    const float4* u4 = reinterpret_cast<const float4*>(ur);
     const uint4* wa = reinterpret_cast<const uint4*>(ca);
     const uint4* wb = reinterpret_cast<const uint4*>(cb);
-    const int n4 = W / 4;
-    for (int b4 = 0; b4 < n4; b4 += 4) {
+    const int ng = nvals >> 2;
+    const int nfd = ng / npb, rem = ng - nfd * npb;
+    const int nq = (nfd + (rem ? 1 : 0) + 3) >> 2;
+    pa = 0.f;
+    pb = 0.f;
+    for (int b4 = 0; b4 < nq; b4 += 4) {
         uint4 xa[4], xb[4];
 #pragma unroll
         for (int j4 = 0; j4 < 4; j4++) {
-            if (b4 + j4 < n4) {
+            if (b4 + j4 < nq) {
                 xa[j4] = wa[b4 + j4];
                 xb[j4] = has_b ? wb[b4 + j4] : make_uint4(0, 0, 0, 0);
             }
         }
 #pragma unroll
         for (int j4 = 0; j4 < 4; j4++) {
-            if (b4 + j4 >= n4) break;
+            if (b4 + j4 >= nq) break;
             const uint32_t sa[4] = {xa[j4].x, xa[j4].y, xa[j4].z, xa[j4].w};
             const uint32_t sb[4] = {xb[j4].x, xb[j4].y, xb[j4].z, xb[j4].w};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int i0 = (4 * (b4 + j4) + j) * vpd;
+                const int dw = 4 * (b4 + j4) + j;
+                const int nh = dw < nfd ? npb : (dw == nfd ? rem : 0);
 #pragma unroll
-                for (int t = 0; t < vpd; t++) {
-                    if (i0 + t < nvals) {
-                        const float u = uf(i0 + t);
-                        pa = fmaf(u, (float)((sa[j] >> (t * EX)) & mask), pa);
-                        pb = fmaf(u, (float)((sb[j] >> (t * EX)) & mask), pb);
+                for (int h = 0; h < npb; h++)
+                    if (h < nh) {
+                        const float4 u = u4[dw * npb + h];
+                        ex_group_fma<EX>(u, sa[j], h, pa);
+                        ex_group_fma<EX>(u, sb[j], h, pb);
                     }
-                }
             }
         }
     }
+    (void)W;
 }
 
 // Float key with the same order as the float (for radix selection / sorting).

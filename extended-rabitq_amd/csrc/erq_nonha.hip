@@ -497,7 +497,7 @@ nonha_mfma_kernel(ScanArgs a, int per_wave_bytes) {
                 float part = 0.f, part2 = 0.f;
                 if (j >= 0) {
                     const float* ur = uq + l16 * S;
-                    ex_chain2<EX>([&](int i) { return ur[i]; },
+                    ex_chain2<EX>(ur,
                                   a.exl + ((size_t)vi * 16 + l16) * a.exl_w,
                                   a.exl + ((size_t)vi2 * 16 + l16) * a.exl_w, j2 >= 0, D >> 4,
                                   a.exl_w, part, part2);

@@ -687,7 +687,7 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
                         idv2 = a.ids[vi2];
                     }
                     const float* ur = uq + l16 * S;
-                    ex_chain2<EX>([&](int i) { return ur[i]; },
+                    ex_chain2<EX>(ur,
                                   a.exl + ((size_t)vi * 16 + l16) * a.exl_w,
                                   a.exl + ((size_t)vi2 * 16 + l16) * a.exl_w, j2 >= 0, D >> 4,
                                   a.exl_w, part, part2);
