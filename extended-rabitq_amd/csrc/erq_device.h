@@ -421,13 +421,36 @@ __device__ __forceinline__ float ex_chain(const float* __restrict__ ur,
         for (int j4 = 0; j4 < 4; j4++) {
             if (b4 + j4 >= nq) break;
             const uint32_t cs[4] = {c4[j4].x, c4[j4].y, c4[j4].z, c4[j4].w};
+            const int dw0 = 4 * (b4 + j4);
+            if (dw0 + 4 <= nfd) {
+                // every value of the word: no branch; the next dword's u is read while this
+                // one's FMAs run (scheduling barriers keep the compiler from hoisting every
+                // read of the word, which costs registers)
+                float4 ua[npb], ub[npb];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int dw = 4 * (b4 + j4) + j;
-                const int nh = dw < nfd ? npb : (dw == nfd ? rem : 0);
+                for (int h = 0; h < npb; h++) ua[h] = u4[dw0 * npb + h];
 #pragma unroll
-                for (int h = 0; h < npb; h++)
-                    if (h < nh) ex_group_fma<EX>(u4[dw * npb + h], cs[j], h, part);
+                for (int j = 0; j < 4; j++) {
+                    if (j < 3) {
+#pragma unroll
+                        for (int h = 0; h < npb; h++) ub[h] = u4[(dw0 + j + 1) * npb + h];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int h = 0; h < npb; h++) ex_group_fma<EX>(ua[h], cs[j], h, part);
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int h = 0; h < npb; h++) ua[h] = ub[h];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int dw = dw0 + j;
+                    const int nh = dw < nfd ? npb : (dw == nfd ? rem : 0);
+#pragma unroll
+                    for (int h = 0; h < npb; h++)
+                        if (h < nh) ex_group_fma<EX>(u4[dw * npb + h], cs[j], h, part);
+                }
             }
         }
     }
