@@ -39,6 +39,8 @@ struct BatchArgs {
     unsigned long long* stats; // 8 counters (see erq.h)
     uint32_t* redo;            // the round's queries whose candidates overflowed: [0] count,
                                // then the queries (rescanned by the round-mode scan)
+    uint32_t* slow;            // the round's queries for the general merge: [0] count, then
+                               // the queries (zero before the round)
     int cand_cap;
     int item_tiles;            // 16-vector tiles per work item: 8 or 16 (4 or 8 blocks)
     int num_cu;                // compute units (sizes the re-estimate's grid)

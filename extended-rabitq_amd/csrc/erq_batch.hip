@@ -1039,18 +1039,16 @@ __device__ __forceinline__ void merge_redo(int q, uint32_t* cand_cnt, uint32_t* 
     if (stats) atomicAdd(&stats[5], 1ull);
 }
 
-__global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
-                                                    uint32_t* __restrict__ pool_i, int k,
-                                                    uint32_t* __restrict__ cand_cnt,
-                                                    const float* __restrict__ cand_d,
-                                                    const uint32_t* __restrict__ cand_i,
-                                                    int cap, uint32_t* __restrict__ redo,
-                                                    unsigned long long* stats) {
-    __shared__ float sd[kMergeBuf];
-    __shared__ uint32_t si[kMergeBuf];
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
-    const int q = blockIdx.x, tid = threadIdx.x;
+// The general merge of one query by a workgroup (merge_fast_kernel hands it the queries it
+// does not take): a radix select of the k-th key over pool and candidates, then a bitonic
+// sort of the entries up to it.
+__device__ void merge_query(int q, float* __restrict__ pool_d, uint32_t* __restrict__ pool_i,
+                            int k, uint32_t* __restrict__ cand_cnt,
+                            const float* __restrict__ cand_d, const uint32_t* __restrict__ cand_i,
+                            int cap, uint32_t* __restrict__ redo, unsigned long long* stats,
+                            float* sd, uint32_t* si, uint32_t* hist, uint32_t& sh_sel,
+                            uint32_t& sh_rank, uint32_t& sh_cnt) {
+    const int tid = threadIdx.x;
     const uint32_t raw = cand_cnt[q];
     if (raw > (uint32_t)cap) {  // uniform
         if (tid == 0) merge_redo(q, cand_cnt, redo, stats);
@@ -1166,6 +1164,143 @@ __global__ void __launch_bounds__(256) merge_kernel(float* __restrict__ pool_d,
         pool_i[(size_t)q * k + i] = sd[i] < FLT_MAX ? si[i] : 0u;
     }
     if (tid == 0) cand_cnt[q] = 0;
+}
+
+// merge_query over the queries of list (list[0] = count, then the queries), a fixed grid.
+__global__ void __launch_bounds__(256) merge_list_kernel(float* __restrict__ pool_d,
+                                                         uint32_t* __restrict__ pool_i, int k,
+                                                         uint32_t* __restrict__ cand_cnt,
+                                                         const float* __restrict__ cand_d,
+                                                         const uint32_t* __restrict__ cand_i,
+                                                         int cap, const uint32_t* __restrict__ list,
+                                                         uint32_t* __restrict__ redo,
+                                                         unsigned long long* stats) {
+    __shared__ float sd[kMergeBuf];
+    __shared__ uint32_t si[kMergeBuf];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
+    const uint32_t n = list[0];
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {  // uniform
+        merge_query((int)list[1 + i], pool_d, pool_i, k, cand_cnt, cand_d, cand_i, cap, redo,
+                    stats, sd, si, hist, sh_sel, sh_rank, sh_cnt);
+        __syncthreads();
+    }
+}
+
+// (d, id) order of the merges.
+__device__ __forceinline__ bool did_less(float da, uint32_t ia, float db, uint32_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+constexpr int kFastCand = 64;  // candidates a one-wave merge takes
+
+// One wave per query: with at most kFastCand candidates and a pool in (distance, id) order
+// (no two finite entries of equal distance), the candidates are bitonic-sorted across the
+// wave's lanes and both sorted lists are merged by rank: a candidate ends at its index plus
+// the pool entries before it, a pool entry at its index plus the candidates before it;
+// positions below k are the new pool -- the result of merge_query.  Any other query goes
+// to `slow` for merge_list_kernel (or to the redo list when its candidates overflowed).
+__global__ void __launch_bounds__(256) merge_fast_kernel(float* __restrict__ pool_d,
+                                                         uint32_t* __restrict__ pool_i, int k,
+                                                         int nq, uint32_t* __restrict__ cand_cnt,
+                                                         const float* __restrict__ cand_d,
+                                                         const uint32_t* __restrict__ cand_i,
+                                                         int cap, uint32_t* __restrict__ slow,
+                                                         uint32_t* __restrict__ redo,
+                                                         unsigned long long* stats) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int q = blockIdx.x * 4 + wid;
+    if (q >= nq) return;  // wave-uniform; no workgroup barrier below
+    const int kcap = (k + 63) & ~63;
+    float* pd = reinterpret_cast<float*>(smem) + (size_t)wid * (2 * kcap + 2 * kFastCand);
+    uint32_t* pi = reinterpret_cast<uint32_t*>(pd + kcap);
+    float* cd = pd + 2 * kcap;
+    uint32_t* ci = reinterpret_cast<uint32_t*>(cd + kFastCand);
+    const uint32_t raw = cand_cnt[q];
+    if (raw > (uint32_t)cap) {
+        if (lane == 0) merge_redo(q, cand_cnt, redo, stats);
+        return;
+    }
+    const int nc = (int)raw;
+    // the pool: ids of FLT_MAX padding read as 0xFFFFFFFF (last in (d, id) order)
+    bool tie = false;
+    for (int i = lane; i < kcap; i += 64) {
+        const float d = i < k ? pool_d[(size_t)q * k + i] : FLT_MAX;
+        pd[i] = d;
+        pi[i] = d < FLT_MAX ? pool_i[(size_t)q * k + i] : 0xFFFFFFFFu;
+    }
+    wave_sync();
+    for (int i = lane; i + 1 < k; i += 64) tie = tie || (pd[i] < FLT_MAX && pd[i] == pd[i + 1]);
+    if (nc == 0 && !ballot(tie)) return;  // already in (distance, id) order
+    if (nc > kFastCand || ballot(tie)) {
+        if (lane == 0) slow[1 + atomicAdd(&slow[0], 1u)] = (uint32_t)q;
+        return;
+    }
+    float d = FLT_MAX;
+    uint32_t id = 0xFFFFFFFFu;
+    if (lane < nc) {
+        d = cand_d[(size_t)q * cap + lane];
+        id = cand_i[(size_t)q * cap + lane];
+    }
+    // bitonic sort of the 64 lanes' (d, id), ascending in lane order
+    for (int size = 2; size <= 64; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const float od = __shfl_xor(d, stride);
+            const uint32_t oi = __shfl_xor(id, stride);
+            const bool up = (lane & size) == 0, low = (lane & stride) == 0;
+            const bool other_less = did_less(od, oi, d, id);
+            // the lower lane of a pair keeps the smaller entry when ascending
+            if ((low == up) ? other_less : !other_less && !(od == d && oi == id)) {
+                d = od;
+                id = oi;
+            }
+        }
+    cd[lane] = d;
+    ci[lane] = id;
+    wave_sync();
+    // candidate lane (< nc): pool entries before it
+    int at = k;
+    if (lane < nc) {
+        int lo = 0, hi = k;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (did_less(pd[mid], pi[mid], d, id)) lo = mid + 1; else hi = mid;
+        }
+        at = lane + lo;
+    }
+    // pool entries: candidates before each
+    float od[16];
+    uint32_t oi[16];
+    int opos[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int i = lane + 64 * r;
+        opos[r] = k;
+        if (i < k) {
+            od[r] = pd[i];
+            oi[r] = pi[i];
+            int lo = 0, hi = nc;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (did_less(cd[mid], ci[mid], od[r], oi[r])) lo = mid + 1; else hi = mid;
+            }
+            opos[r] = i + lo;
+        }
+    }
+    float* gd = pool_d + (size_t)q * k;
+    uint32_t* gi = pool_i + (size_t)q * k;
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        if (opos[r] < k) {
+            gd[opos[r]] = od[r];
+            gi[opos[r]] = od[r] < FLT_MAX ? oi[r] : 0u;
+        }
+    if (at < k) {
+        gd[at] = d;
+        gi[at] = d < FLT_MAX ? id : 0u;
+    }
+    if (lane == 0) cand_cnt[q] = 0;
 }
 
 }  // namespace
@@ -1371,10 +1506,18 @@ hipError_t launch_round_stage(const BatchArgs& ba, const RoundBufs& rb, int grid
     switch (stage) {
     case 0: return launch_batch_scan(ba, grid, s);
     case 1: return launch_reest(ba, grid, s);
-    case 2:
-        hipLaunchKernelGGL(merge_kernel, dim3(ba.nq), dim3(256), 0, s, ba.pool_d, rb.pool_i, ba.k,
-                           ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.redo, ba.stats);
+    case 2: {
+        const size_t kcap = (size_t)((ba.k + 63) & ~63);
+        const size_t lds = 4 * (2 * kcap + 2 * kFastCand) * 4;
+        hipLaunchKernelGGL(merge_fast_kernel, dim3((ba.nq + 3) / 4), dim3(256), lds, s, ba.pool_d,
+                           rb.pool_i, ba.k, ba.nq, ba.cand_cnt, ba.cand_d, ba.cand_i,
+                           ba.cand_cap, ba.slow, ba.redo, ba.stats);
+        const unsigned lgrid = (unsigned)std::max(8, std::min(4 * std::max(1, ba.num_cu), ba.nq));
+        hipLaunchKernelGGL(merge_list_kernel, dim3(lgrid), dim3(256), 0, s, ba.pool_d, rb.pool_i,
+                           ba.k, ba.cand_cnt, ba.cand_d, ba.cand_i, ba.cand_cap, ba.slow, ba.redo,
+                           ba.stats);
         return hipGetLastError();
+    }
     default: return hipErrorInvalidValue;
     }
 }

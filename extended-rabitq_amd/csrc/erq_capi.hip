@@ -1007,7 +1007,7 @@ static size_t batch_chunk_nq(size_t nq, size_t max_nprobe, size_t budget) {
 struct BatchBufs {
     size_t cq = 0, R = 0, r_end = 0;
     uint32_t *counts, *cursor, *local, *btot, *rinfo, *items, *ovf, *entries, *cand_cnt, *cand_i,
-        *surv_count, *redo;
+        *surv_count, *redo, *slow;
     float* cand_d;
     int8_t* adig;
     char* qparams;
@@ -1042,6 +1042,7 @@ static void batch_layout(const erq_index* h, size_t nq, size_t max_nprobe, int c
     b.cand_d = ws.take<float>(b.cq * (size_t)cap);
     b.cand_i = ws.take<uint32_t>(b.cq * (size_t)cap);
     b.redo = ws.take<uint32_t>(b.R * (b.cq + 1));
+    b.slow = ws.take<uint32_t>(b.R * (b.cq + 1));
     b.adig = ws.take<int8_t>(pairs * 2 * Dp);
     b.qparams = ws.take<char>(pairs * 48);
     b.surv = ws.take<uint4>(h->batch_surv_cap);
@@ -1143,6 +1144,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.surv_cap = h->batch_surv_cap;
         ba.stats = reinterpret_cast<unsigned long long*>(stats);
         ba.redo = b.redo;
+        ba.slow = b.slow;
         ba.cand_cap = cap;
         ba.item_tiles = h->batch_item_tiles;
         ba.num_cu = h->num_cu;
@@ -1166,6 +1168,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
             BatchArgs br = ba;
             br.n_items = b.rinfo + 4 * r;
             br.redo = b.redo + r * (cq + 1);
+            br.slow = b.slow + r * (cq + 1);
             for (int st = 0; st < 3; st++) {
                 StageTimer tim(h, kStages[st], s);
                 hipError_t e = launch_round_stage(br, rb, grid, st, s);
@@ -1204,6 +1207,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
             }
             e = launch_zero(b.cand_cnt, n, s);
             if (e == hipSuccess) e = launch_zero(b.redo, b.R * (cq + 1), s);
+            if (e == hipSuccess) e = launch_zero(b.slow, b.R * (cq + 1), s);
             if (e == hipSuccess) e = rounds();
             if (e != hipSuccess) return hip_fail(e, "batched round");
         }
@@ -1234,6 +1238,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         }
         e = launch_zero(b.cand_cnt, n, s);
         if (e == hipSuccess) e = launch_zero(b.redo, b.R * (cq + 1), s);
+        if (e == hipSuccess) e = launch_zero(b.slow, b.R * (cq + 1), s);
         if (e == hipSuccess) e = rounds();
         if (e != hipSuccess) return hip_fail(e, "batched round");
     }
