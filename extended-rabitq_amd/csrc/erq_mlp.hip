@@ -4,9 +4,10 @@
 // FP32 master weights.  p = sigmoid(W3 gelu(W2 gelu(W1 x + b1) + b2) + b3).
 //
 // Weights are stored out-major ([out x in], as a Linear layer), so the forward GEMM reads
-// both operands k-contiguous.  GEMM tile: 64 x 64 per 256-thread workgroup (2 x 2 waves of
-// 32 x 32 = 2 x 2 MFMA tiles), K step 32 staged in LDS as [row][k] so each lane's MFMA
-// operand (8 consecutive k) is one 16-byte LDS read.
+// both operands k-contiguous; the backward products transpose what enters transposed and
+// run the same kernel.  GEMM tile: 128 (or 64) x 128 per 256-thread workgroup (2 x 2 waves
+// of 64 x 64 = 4 x 4 MFMA tiles), K step 32 double-buffered in LDS as [row][k] so each
+// lane's MFMA operand (8 consecutive k) is one 16-byte LDS read.
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -39,12 +40,13 @@ struct erq_mlp {
     __bf16* dz2 = nullptr;    // B x hid
     __bf16* dz1 = nullptr;
     float* tmp = nullptr;     // B x hid fp32 scratch
+    __bf16* tA = nullptr;     // transposed operands of the backward products:
+    __bf16* tB = nullptr;     //   max(out, hid) x B, max(hid, in) x B
+    __bf16* wt = nullptr;     //   and hid x max(out, hid) (a transposed weight)
     float* loss = nullptr;    // 1
     float* part = nullptr;    // partial sums of the loss and of the bias gradients
     // inference activations for batches above max_batch (inf_rows rows, made on first use)
     size_t inf_rows = 0;
-    __bf16* ixb = nullptr;
-    float* iz = nullptr;      // pre-activations of both hidden layers (not kept)
     __bf16* ih1 = nullptr;
     __bf16* ih2 = nullptr;
     std::mutex mu;
@@ -54,12 +56,17 @@ namespace erq {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int GBM = 64, GBN = 64, GBK = 32, GPAD = 8;
+constexpr int GBN = 128, GBK = 32, GPAD = 8;
 constexpr size_t kInferRows = 32768;  // rows per inference chunk above max_batch
 
-enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_F32 = 2, EPI_ACC_F32 = 3 };
+// EPI_BIAS_GELU keeps the pre-activation (training); EPI_BIAS_GELU_H stores only the BF16
+// activation and EPI_BIAS_SIGMOID the probability sigmoid(acc + bias) (inference).
+enum Epi {
+    EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2, EPI_BIAS_GELU_H = 3, EPI_BIAS_SIGMOID = 4
+};
 
 __device__ __forceinline__ float gelu(float z) {
     return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
@@ -68,107 +75,205 @@ __device__ __forceinline__ float gelu_grad(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
            z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
+__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
 
-// C(m, n) = sum_k A(m, k) B(k, n).  TA: A stored [K x M] (else [M x K]); TB: B stored
-// [N x K] (else [K x N]).  Edges in M, N and K are zero-filled; 16-byte loads need the
-// leading dimensions to be multiples of 8 and, along m (TA) or n (!TB), M or N too.
-template <int TA, int TB, int EPI>
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(const __bf16* __restrict__ A, int lda,
-                                                        const __bf16* __restrict__ B, int ldb,
-                                                        int M, int N, int K,
-                                                        float* __restrict__ Cf, int ldc,
-                                                        __bf16* __restrict__ Cb,
-                                                        const float* __restrict__ bias) {
-    __shared__ __align__(16) __bf16 As[GBM][GBK + GPAD];
-    __shared__ __align__(16) __bf16 Bs[GBN][GBK + GPAD];
+// C(m, n) = sum_k A(m, k) B(n, k): both operands k-contiguous, A [M x K] (lda), B [N x K]
+// (ldb).  Workgroup tile BM x 128 (BM = 64 or 128), K step 32, LDS double-buffered: the next
+// step's tiles are loaded into registers while the current step's feed the MFMAs, then
+// stored to the other buffer, one barrier per step.  4 waves as 2 x 2, wave tile BM/2 x 64
+// = TM x 4 tiles of v_mfma_f32_16x16x32_bf16.  AF: A is fp32 and enters as bf16(ascale * a)
+// (layer 1's input conversion fused into the load).  Edges in M, N and K are zero-filled;
+// 16-byte loads need lda and ldb to be multiples of 8 and 16-byte aligned bases.  Each
+// output sums its k in the same order whatever BM, so a row's result does not depend on
+// the batch it is computed in.
+template <int BM, int EPI, bool AF>
+__global__ void __launch_bounds__(256) gemm_nt_kernel(const void* __restrict__ Av, int lda,
+                                                      const __bf16* __restrict__ B, int ldb,
+                                                      int M, int N, int K, float ascale,
+                                                      float* __restrict__ Cf, int ldc,
+                                                      __bf16* __restrict__ Cb,
+                                                      const float* __restrict__ bias) {
+    constexpr int TM = BM / 32;               // 16-row tiles per wave along m
+    constexpr int ACH = BM * GBK / 8 / 256;   // 16-byte A chunks per thread per step
+    constexpr int BCH = GBN * GBK / 8 / 256;  // same for B
+    // one LDS buffer: the double-buffered A and B tiles, then the epilogue's staging
+    constexpr int kTileBytes = 2 * (BM + GBN) * (GBK + GPAD) * 2, kStageBytes = 4 * 16 * 68 * 4;
+    __shared__ __align__(16) unsigned char lds[kTileBytes > kStageBytes ? kTileBytes : kStageBytes];
+    auto As = reinterpret_cast<__bf16 (*)[BM][GBK + GPAD]>(lds);
+    auto Bs = reinterpret_cast<__bf16 (*)[GBN][GBK + GPAD]>(lds + 2 * BM * (GBK + GPAD) * 2);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
-    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * GBN;
     const __bf16 zero = (__bf16)0.0f;
-    f4 acc[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    f4 acc[TM][4];  // every loop over it is unrolled, so it stays in registers
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 ra[ACH], rb[BCH];
 
-    for (int k0 = 0; k0 < K; k0 += GBK) {
-        // ---- stage A (64 x 32) ----
-        if (!TA) {
-            int row = tid >> 2, kq = (tid & 3) * 8, gm = m0 + row;
-            bf16x8 v;
-            const __bf16* src = A + (size_t)gm * lda + k0 + kq;
-            if (gm < M && k0 + kq + 8 <= K) v = *reinterpret_cast<const bf16x8*>(src);
-            else
-                for (int e = 0; e < 8; e++) v[e] = (gm < M && k0 + kq + e < K) ? src[e] : zero;
-            *reinterpret_cast<bf16x8*>(&As[row][kq]) = v;
-        } else {
-            int kk = tid >> 3, mq = (tid & 7) * 8, gm = m0 + mq;
-            bf16x8 v;
-            if (gm < M && k0 + kk < K)
-                v = *reinterpret_cast<const bf16x8*>(A + (size_t)(k0 + kk) * lda + gm);
-            else for (int e = 0; e < 8; e++) v[e] = zero;
-            for (int e = 0; e < 8; e++) As[mq + e][kk] = v[e];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int c = tid + i * 256, gm = m0 + (c >> 2), k = k0 + (c & 3) * 8;
+            if (AF) {
+                const float* src = reinterpret_cast<const float*>(Av) + (size_t)gm * lda + k;
+                if (gm < M && k + 8 <= K) {
+                    const f4 a0 = *reinterpret_cast<const f4*>(src);
+                    const f4 a1 = *reinterpret_cast<const f4*>(src + 4);
+                    for (int e = 0; e < 4; e++) {
+                        ra[i][e] = (__bf16)(ascale * a0[e]);
+                        ra[i][4 + e] = (__bf16)(ascale * a1[e]);
+                    }
+                } else {
+                    for (int e = 0; e < 8; e++)
+                        ra[i][e] = (gm < M && k + e < K) ? (__bf16)(ascale * src[e]) : zero;
+                }
+            } else {
+                const __bf16* src = reinterpret_cast<const __bf16*>(Av) + (size_t)gm * lda + k;
+                if (gm < M && k + 8 <= K) ra[i] = *reinterpret_cast<const bf16x8*>(src);
+                else
+                    for (int e = 0; e < 8; e++) ra[i][e] = (gm < M && k + e < K) ? src[e] : zero;
+            }
         }
-        // ---- stage B (32 x 64) as [n][k] ----
-        if (TB) {
-            int row = tid >> 2, kq = (tid & 3) * 8, gn = n0 + row;
-            bf16x8 v;
-            const __bf16* src = B + (size_t)gn * ldb + k0 + kq;
-            if (gn < N && k0 + kq + 8 <= K) v = *reinterpret_cast<const bf16x8*>(src);
+#pragma unroll
+        for (int i = 0; i < BCH; i++) {
+            const int c = tid + i * 256, gn = n0 + (c >> 2), k = k0 + (c & 3) * 8;
+            const __bf16* src = B + (size_t)gn * ldb + k;
+            if (gn < N && k + 8 <= K) rb[i] = *reinterpret_cast<const bf16x8*>(src);
             else
-                for (int e = 0; e < 8; e++) v[e] = (gn < N && k0 + kq + e < K) ? src[e] : zero;
-            *reinterpret_cast<bf16x8*>(&Bs[row][kq]) = v;
-        } else {
-            int kk = tid >> 3, nq = (tid & 7) * 8, gn = n0 + nq;
-            bf16x8 v;
-            if (gn < N && k0 + kk < K)
-                v = *reinterpret_cast<const bf16x8*>(B + (size_t)(k0 + kk) * ldb + gn);
-            else for (int e = 0; e < 8; e++) v[e] = zero;
-            for (int e = 0; e < 8; e++) Bs[nq + e][kk] = v[e];
+                for (int e = 0; e < 8; e++) rb[i][e] = (gn < N && k + e < K) ? src[e] : zero;
         }
-        __syncthreads();
-        const int li = lane & 15, lg = lane >> 4;
-        bf16x8 af[2], bfr[2];
-        for (int i = 0; i < 2; i++)
-            af[i] = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + i * 16 + li][lg * 8]);
-        for (int j = 0; j < 2; j++)
-            bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + j * 16 + li][lg * 8]);
-        for (int i = 0; i < 2; i++)
-            for (int j = 0; j < 2; j++)
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int c = tid + i * 256;
+            *reinterpret_cast<bf16x8*>(&As[buf][c >> 2][(c & 3) * 8]) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < BCH; i++) {
+            const int c = tid + i * 256;
+            *reinterpret_cast<bf16x8*>(&Bs[buf][c >> 2][(c & 3) * 8]) = rb[i];
+        }
+    };
+
+    const int nk = (K + GBK - 1) / GBK;
+    load(0);
+    store(0);
+    __syncthreads();
+    const int li = lane & 15, lg = lane >> 4;
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) load((kt + 1) * GBK);  // in flight during this step's MFMAs
+        bf16x8 af[TM], bfr[4];
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+            af[i] = *reinterpret_cast<const bf16x8*>(&As[cur][wm * (BM / 2) + i * 16 + li][lg * 8]);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][wn * 64 + j * 16 + li][lg * 8]);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0,
                                                                     0, 0);
+        // the other buffer was last read in the previous step, before its closing barrier
+        if (kt + 1 < nk) store(cur ^ 1);
         __syncthreads();
     }
+    // Epilogue through LDS (free after the last barrier): per 16-row slice of the wave tile,
+    // the accumulators are staged as 16 x 64 floats, then each lane takes 4 consecutive
+    // columns of a row, so a row's 64 outputs leave as one contiguous segment.
+    float (*T)[68] = reinterpret_cast<float (*)[68]>(lds) + wid * 16;
     const int col = lane & 15, rg = (lane >> 4) * 4;
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) {
-            const int gn = n0 + wn * 32 + j * 16 + col;
-            if (gn >= N) continue;
-            for (int r = 0; r < 4; r++) {
-                const int gm = m0 + wm * 32 + i * 16 + rg + r;
-                if (gm >= M) continue;
-                float v = acc[i][j][r];
+    const int sr = lane >> 4, sc = (lane & 15) * 4, gn = n0 + wn * 64 + sc;
+    f4 bv = f4{0.f, 0.f, 0.f, 0.f};
+    if (EPI != EPI_F32) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) bv[e] = gn + e < N ? bias[gn + e] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < TM; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) T[rg + r][j * 16 + col] = acc[i][j][r];
+        __syncthreads();
+#pragma unroll
+        for (int pass = 0; pass < 4; pass++) {
+            const int row = pass * 4 + sr, gm = m0 + wm * (BM / 2) + i * 16 + row;
+            const f4 v = *reinterpret_cast<const f4*>(&T[row][sc]);
+            if (gm < M) {
+                f4 out;
+                bf16x4 hb;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const float z = EPI == EPI_F32 ? v[e] : v[e] + bv[e];
+                    out[e] = EPI == EPI_BIAS_SIGMOID ? sigmoid(z) : z;
+                    if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS_GELU_H) hb[e] = (__bf16)gelu(z);
+                }
                 const size_t o = (size_t)gm * ldc + gn;
-                if (EPI == EPI_F32) {
-                    Cf[o] = v;
-                } else if (EPI == EPI_ACC_F32) {
-                    Cf[o] += v;
-                } else if (EPI == EPI_BIAS_F32) {
-                    Cf[o] = v + bias[gn];
-                } else {  // bias + GELU: keep the pre-activation, store the BF16 activation
-                    float z = v + bias[gn];
-                    Cf[o] = z;
-                    Cb[o] = (__bf16)gelu(z);
+                if (gn + 4 <= N) {
+                    if (EPI != EPI_BIAS_GELU_H) *reinterpret_cast<f4*>(Cf + o) = out;
+                    if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS_GELU_H)
+                        *reinterpret_cast<bf16x4*>(Cb + o) = hb;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        if (gn + e < N) {
+                            if (EPI != EPI_BIAS_GELU_H) Cf[o + e] = out[e];
+                            if (EPI == EPI_BIAS_GELU || EPI == EPI_BIAS_GELU_H) Cb[o + e] = hb[e];
+                        }
                 }
             }
         }
+        __syncthreads();  // the slice is read before the next one is staged
+    }
 }
 
-template <int TA, int TB, int EPI>
-hipError_t gemm(const __bf16* A, int lda, const __bf16* B, int ldb, int M, int N, int K,
-                float* Cf, int ldc, __bf16* Cb, const float* bias, hipStream_t s) {
-    if ((lda | ldb) % 8) return hipErrorInvalidValue;
-    dim3 grid((N + GBN - 1) / GBN, (M + GBM - 1) / GBM);
-    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, EPI>), grid, dim3(256), 0, s, A, lda, B, ldb,
-                       M, N, K, Cf, ldc, Cb, bias);
+// C = A B^T with the operand layouts of gemm_nt_kernel; A is bf16 unless AF (fp32, scaled).
+// ldc is the row stride of whichever output the epilogue writes (Cf, Cb or both).
+template <int EPI, bool AF = false>
+hipError_t gemm(const void* A, int lda, const __bf16* B, int ldb, int M, int N, int K,
+                float* Cf, int ldc, __bf16* Cb, const float* bias, hipStream_t s,
+                float ascale = 1.f) {
+    if ((lda | ldb | ldc) % 8 || ((uintptr_t)A | (uintptr_t)B | (uintptr_t)Cf | (uintptr_t)Cb) % 16)
+        return hipErrorInvalidValue;
+    if (M <= 0 || N <= 0) return hipSuccess;
+    const unsigned gx = (unsigned)((N + GBN - 1) / GBN);
+    // 128-row tiles when they still give the chip several workgroups per CU, else 64
+    if ((size_t)gx * ((M + 127) / 128) >= 1024) {
+        hipLaunchKernelGGL((gemm_nt_kernel<128, EPI, AF>), dim3(gx, (M + 127) / 128), dim3(256), 0,
+                           s, A, lda, B, ldb, M, N, K, ascale, Cf, ldc, Cb, bias);
+    } else {
+        hipLaunchKernelGGL((gemm_nt_kernel<64, EPI, AF>), dim3(gx, (M + 63) / 64), dim3(256), 0,
+                           s, A, lda, B, ldb, M, N, K, ascale, Cf, ldc, Cb, bias);
+    }
+    return hipGetLastError();
+}
+
+// out [C x R] = in [R x C]^T, bf16, 64 x 64 tiles through LDS
+__global__ void __launch_bounds__(256) transpose_bf16_kernel(const __bf16* __restrict__ in, int R,
+                                                             int C, __bf16* __restrict__ out) {
+    __shared__ __bf16 t[64][66];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {
+        const int r = r0 + i, c = c0 + tx;
+        t[i][tx] = (r < R && c < C) ? in[(size_t)r * C + c] : (__bf16)0.0f;
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int c = c0 + i, r = r0 + tx;
+        if (c < C && r < R) out[(size_t)c * R + r] = t[tx][i];
+    }
+}
+
+hipError_t transpose(const __bf16* in, int R, int C, __bf16* out, hipStream_t s) {
+    hipLaunchKernelGGL(transpose_bf16_kernel, dim3((C + 63) / 64, (R + 63) / 64), dim3(256), 0, s,
+                       in, R, C, out);
     return hipGetLastError();
 }
 
@@ -177,18 +282,6 @@ __global__ void f32_to_bf16_kernel(const float* __restrict__ x, __bf16* __restri
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         y[i] = (__bf16)(scale * x[i]);
-}
-
-__global__ void sigmoid_kernel(const float* __restrict__ z, float* __restrict__ p, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 1.0f / (1.0f + expf(-z[i]));
-}
-
-__global__ void sigmoid_inplace_kernel(float* p, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 1.0f / (1.0f + expf(-p[i]));
 }
 
 // BCE with logits, positive weight w: l = w*y*softplus(-z) + (1-y)*softplus(z);
@@ -319,77 +412,87 @@ hipError_t refresh_bf16(erq_mlp* m, hipStream_t s) {
     return hipGetLastError();
 }
 
-// forward on B rows from fp32 inputs into the given activation buffers (B rows each);
-// logits (B x out, fp32) into logit
-struct Acts {
-    __bf16* xb;
-    float* z1;
-    __bf16* h1;
-    float* z2;
-    __bf16* h2;
-    float* logit;
-};
-
-hipError_t forward_into(erq_mlp* m, const Acts& A, const float* x, int B, hipStream_t s) {
+// training forward on B <= max_batch rows: xb = bf16(xscale x), z1, h1, z2, h2 and the
+// logits (fp32) kept for the backward pass
+hipError_t forward(erq_mlp* m, const float* x, int B, hipStream_t s) {
     Layout L = layout(m);
     const size_t n1 = (size_t)m->hid * m->in, n2 = (size_t)m->hid * m->hid;
     const __bf16 *W1 = m->wb, *W2 = m->wb + n1, *W3 = m->wb + n1 + n2;
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(grid_for((size_t)B * m->in)), dim3(256), 0, s, x,
-                       A.xb, (size_t)B * m->in, m->xscale);
-    hipError_t e = gemm<0, 1, EPI_BIAS_GELU>(A.xb, m->in, W1, m->in, B, m->hid, m->in, A.z1,
-                                             m->hid, A.h1, m->master + L.b1, s);
+                       m->xb, (size_t)B * m->in, m->xscale);
+    hipError_t e = gemm<EPI_BIAS_GELU>(m->xb, m->in, W1, m->in, B, m->hid, m->in, m->z1, m->hid,
+                                       m->h1, m->master + L.b1, s);
     if (e == hipSuccess)
-        e = gemm<0, 1, EPI_BIAS_GELU>(A.h1, m->hid, W2, m->hid, B, m->hid, m->hid, A.z2, m->hid,
-                                      A.h2, m->master + L.b2, s);
+        e = gemm<EPI_BIAS_GELU>(m->h1, m->hid, W2, m->hid, B, m->hid, m->hid, m->z2, m->hid,
+                                m->h2, m->master + L.b2, s);
     if (e == hipSuccess)
-        e = gemm<0, 1, EPI_BIAS_F32>(A.h2, m->hid, W3, m->hid, B, m->out, m->hid, A.logit,
-                                     m->out, nullptr, m->master + L.b3, s);
+        e = gemm<EPI_BIAS_F32>(m->h2, m->hid, W3, m->hid, B, m->out, m->hid, m->logit, m->out,
+                               nullptr, m->master + L.b3, s);
     return e;
 }
 
-// forward on batch B (B <= max_batch) from fp32 inputs; leaves z1,h1,z2,h2,logit
-hipError_t forward(erq_mlp* m, const float* x, int B, hipStream_t s) {
-    return forward_into(m, Acts{m->xb, m->z1, m->h1, m->z2, m->h2, m->logit}, x, B, s);
-}
-
-// backward after forward with labels y (fp32 0/1): grads into m->grad, loss into m->loss
-hipError_t backward(erq_mlp* m, const float* y, int B, float pos_weight, hipStream_t s) {
+// inference on B rows: probabilities straight into prob (B x out), the input conversion
+// fused into layer 1 and the sigmoid into layer 3 (the same values as the training
+// forward's bf16 input, activations and logits); h1, h2: B x hid scratch
+hipError_t infer(erq_mlp* m, const float* x, int B, __bf16* h1, __bf16* h2, float* prob,
+                 hipStream_t s) {
     Layout L = layout(m);
     const size_t n1 = (size_t)m->hid * m->in, n2 = (size_t)m->hid * m->hid;
+    const __bf16 *W1 = m->wb, *W2 = m->wb + n1, *W3 = m->wb + n1 + n2;
+    hipError_t e = gemm<EPI_BIAS_GELU_H, true>(x, m->in, W1, m->in, B, m->hid, m->in, nullptr,
+                                               m->hid, h1, m->master + L.b1, s, m->xscale);
+    if (e == hipSuccess)
+        e = gemm<EPI_BIAS_GELU_H>(h1, m->hid, W2, m->hid, B, m->hid, m->hid, nullptr, m->hid, h2,
+                                  m->master + L.b2, s);
+    if (e == hipSuccess)
+        e = gemm<EPI_BIAS_SIGMOID>(h2, m->hid, W3, m->hid, B, m->out, m->hid, prob, m->out,
+                                   nullptr, m->master + L.b3, s);
+    return e;
+}
+
+// backward after forward with labels y (fp32 0/1): grads into m->grad, loss into m->loss.
+// Every product runs as A B^T with k-contiguous operands: the activations and the weights
+// that enter transposed are transposed first (tA, tB, wt).
+hipError_t backward(erq_mlp* m, const float* y, int B, float pos_weight, hipStream_t s) {
+    Layout L = layout(m);
+    const int in = m->in, hid = m->hid, out = m->out;
+    const size_t n1 = (size_t)hid * in, n2 = (size_t)hid * hid;
     const __bf16 *W2 = m->wb + n1, *W3 = m->wb + n1 + n2;
-    const size_t no = (size_t)B * m->out, nh = (size_t)B * m->hid;
-    hipError_t e = hipSuccess;
+    const size_t no = (size_t)B * out, nh = (size_t)B * hid;
     const int gb = grid_for(no);
     hipLaunchKernelGGL(bce_kernel, dim3(gb), dim3(256), 0, s, m->logit, y, m->dlog, no,
                        pos_weight, 1.0f / B, m->part);
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, m->part, gb, m->loss);
-    // layer 3: dW3[out x hid] = dlog^T h2 ; db3 ; dh2 = dlog W3
-    e = gemm<1, 0, EPI_F32>(m->dlog, m->out, m->h2, m->hid, m->out, m->hid, B,
-                            m->grad + L.w3, m->hid, nullptr, nullptr, s);
-    if (e) return e;
-    e = colsum(m->dlog, B, m->out, m->part, m->grad + L.b3, s);
-    if (e) return e;
-    e = gemm<0, 0, EPI_F32>(m->dlog, m->out, W3, m->hid, B, m->hid, m->out, m->tmp, m->hid,
-                            nullptr, nullptr, s);
+    // layer 3: dW3 [out x hid] = dlog^T h2; db3; dh2 = dlog W3
+    hipError_t e = transpose(m->dlog, B, out, m->tA, s);
+    if (!e) e = transpose(m->h2, B, hid, m->tB, s);
+    if (!e) e = gemm<EPI_F32>(m->tA, B, m->tB, B, out, hid, B, m->grad + L.w3, hid, nullptr,
+                              nullptr, s);
+    if (!e) e = colsum(m->dlog, B, out, m->part, m->grad + L.b3, s);
+    if (!e) e = transpose(W3, out, hid, m->wt, s);
+    if (!e) e = gemm<EPI_F32>(m->dlog, out, m->wt, out, B, hid, out, m->tmp, hid, nullptr,
+                              nullptr, s);
     if (e) return e;
     hipLaunchKernelGGL(gelu_back_kernel, dim3(grid_for(nh)), dim3(256), 0, s, m->tmp, m->z2,
                        m->dz2, nh);
-    // layer 2
-    e = gemm<1, 0, EPI_F32>(m->dz2, m->hid, m->h1, m->hid, m->hid, m->hid, B, m->grad + L.w2,
-                            m->hid, nullptr, nullptr, s);
-    if (e) return e;
-    e = colsum(m->dz2, B, m->hid, m->part, m->grad + L.b2, s);
-    if (e) return e;
-    e = gemm<0, 0, EPI_F32>(m->dz2, m->hid, W2, m->hid, B, m->hid, m->hid, m->tmp, m->hid,
-                            nullptr, nullptr, s);
+    // layer 2: dW2 = dz2^T h1; db2; dh1 = dz2 W2
+    e = transpose(m->dz2, B, hid, m->tA, s);
+    if (!e) e = transpose(m->h1, B, hid, m->tB, s);
+    if (!e) e = gemm<EPI_F32>(m->tA, B, m->tB, B, hid, hid, B, m->grad + L.w2, hid, nullptr,
+                              nullptr, s);
+    if (!e) e = colsum(m->dz2, B, hid, m->part, m->grad + L.b2, s);
+    if (!e) e = transpose(W2, hid, hid, m->wt, s);
+    if (!e) e = gemm<EPI_F32>(m->dz2, hid, m->wt, hid, B, hid, hid, m->tmp, hid, nullptr,
+                              nullptr, s);
     if (e) return e;
     hipLaunchKernelGGL(gelu_back_kernel, dim3(grid_for(nh)), dim3(256), 0, s, m->tmp, m->z1,
                        m->dz1, nh);
-    // layer 1
-    e = gemm<1, 0, EPI_F32>(m->dz1, m->hid, m->xb, m->in, m->hid, m->in, B, m->grad + L.w1,
-                            m->in, nullptr, nullptr, s);
-    if (e) return e;
-    e = colsum(m->dz1, B, m->hid, m->part, m->grad + L.b1, s);
+    // layer 1: dW1 = dz1^T xb; db1
+    e = transpose(m->dz1, B, hid, m->tA, s);
+    if (!e) e = transpose(m->xb, B, in, m->tB, s);
+    if (!e) e = gemm<EPI_F32>(m->tA, B, m->tB, B, hid, in, B, m->grad + L.w1, in, nullptr,
+                              nullptr, s);
+    if (!e) e = colsum(m->dz1, B, hid, m->part, m->grad + L.b1, s);
     if (e) return e;
     return hipGetLastError();
 }
@@ -406,7 +509,7 @@ void mlp_release(erq_mlp* m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     void* ptrs[] = {m->master, m->grad, m->adam_m, m->adam_v, m->wb, m->xb, m->z1, m->h1,
                     m->z2, m->h2, m->logit, m->dlog, m->dz2, m->dz1, m->tmp, m->loss,
-                    m->part, m->ixb, m->iz, m->ih1, m->ih2};
+                    m->part, m->tA, m->tB, m->wt, m->ih1, m->ih2};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -462,6 +565,9 @@ int erq_mlp_create(size_t in_dim, size_t hidden, size_t out_dim, size_t max_batc
     if (e == hipSuccess) e = alloc(&m->dz2, B * hidden);
     if (e == hipSuccess) e = alloc(&m->dz1, B * hidden);
     if (e == hipSuccess) e = alloc(&m->tmp, B * hidden);
+    if (e == hipSuccess) e = alloc(&m->tA, B * std::max(out_dim, hidden));
+    if (e == hipSuccess) e = alloc(&m->tB, B * std::max(hidden, in_dim));
+    if (e == hipSuccess) e = alloc(&m->wt, hidden * std::max(out_dim, hidden));
     if (e == hipSuccess) e = alloc(&m->loss, 1);
     if (e == hipSuccess)
         e = alloc(&m->part, std::max<size_t>(8192, (size_t)kColSlices * std::max(out_dim, hidden)));
@@ -545,44 +651,29 @@ int erq_mlp_forward_dev(erq_mlp* m, const float* x_dev, size_t batch, float* pro
     }
     std::lock_guard<std::mutex> lk(m->mu);
     hipStream_t s = mlp_stream(m, stream);
+    if (batch == 0) return ERQ_OK;
     if (batch <= (size_t)m->max_batch) {
-        if (batch == 0) return ERQ_OK;
-        hipError_t e = forward(m, x_dev, (int)batch, s);
-        if (e != hipSuccess) return hip_fail(e, "mlp forward");
-        const size_t n = batch * m->out;
-        hipLaunchKernelGGL(sigmoid_kernel, dim3(grid_for(n)), dim3(256), 0, s, m->logit, prob_dev,
-                           n);
-        e = hipGetLastError();
-        return e == hipSuccess ? ERQ_OK : hip_fail(e, "mlp sigmoid");
+        hipError_t e = infer(m, x_dev, (int)batch, m->h1, m->h2, prob_dev, s);
+        return e == hipSuccess ? ERQ_OK : hip_fail(e, "mlp forward");
     }
-    // large batches: chunks of up to kInferRows rows through the inference buffers, the
-    // logits written straight into prob_dev and turned into probabilities there
+    // large batches: chunks of up to kInferRows rows through the inference buffers (the
+    // same per-row results: a row's sums do not depend on the rows beside it)
     const size_t rows = std::min<size_t>(batch, kInferRows);
     if (m->inf_rows < rows) {
-        void* old[] = {m->ixb, m->iz, m->ih1, m->ih2};
+        void* old[] = {m->ih1, m->ih2};
         for (void* p : old)
             if (p) (void)hipFree(p);
-        m->ixb = nullptr;
-        m->iz = nullptr;
         m->ih1 = m->ih2 = nullptr;
         m->inf_rows = 0;
-        hipError_t e = alloc(&m->ixb, rows * m->in);
-        if (e == hipSuccess) e = alloc(&m->iz, rows * m->hid);
-        if (e == hipSuccess) e = alloc(&m->ih1, rows * m->hid);
+        hipError_t e = alloc(&m->ih1, rows * m->hid);
         if (e == hipSuccess) e = alloc(&m->ih2, rows * m->hid);
         if (e != hipSuccess) return hip_fail(e, "mlp inference buffers");
         m->inf_rows = rows;
     }
     for (size_t b0 = 0; b0 < batch; b0 += rows) {
         const int B = (int)std::min<size_t>(rows, batch - b0);
-        float* pr = prob_dev + b0 * m->out;
-        hipError_t e = forward_into(m, Acts{m->ixb, m->iz, m->ih1, m->iz, m->ih2, pr},
-                                    x_dev + b0 * m->in, B, s);
+        hipError_t e = infer(m, x_dev + b0 * m->in, B, m->ih1, m->ih2, prob_dev + b0 * m->out, s);
         if (e != hipSuccess) return hip_fail(e, "mlp forward");
-        const size_t n = (size_t)B * m->out;
-        hipLaunchKernelGGL(sigmoid_inplace_kernel, dim3(grid_for(n)), dim3(256), 0, s, pr, n);
-        e = hipGetLastError();
-        if (e != hipSuccess) return hip_fail(e, "mlp sigmoid");
     }
     return ERQ_OK;
 }
