@@ -239,7 +239,8 @@ __global__ void __launch_bounds__(256) bin_total_kernel(uint32_t* __restrict__ b
     }
 }
 
-// Per bin: the cursor of its entries and its items.
+// Per bin, one wave each (the lanes take its items, so a bin with many entries does not
+// hold its workgroup): the cursor of its entries and its items.
 __global__ void __launch_bounds__(256) bin_items_kernel(const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ csize, int K,
                                                        int nb, int item_tiles,
@@ -247,15 +248,16 @@ __global__ void __launch_bounds__(256) bin_items_kernel(const uint32_t* __restri
                                                        const uint32_t* __restrict__ btot,
                                                        uint32_t* __restrict__ cursor,
                                                        uint32_t* __restrict__ items) {
-    const int bin = blockIdx.x * 256 + threadIdx.x;
+    const int bin = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (bin >= nb) return;
     const uint32_t cnt = counts[bin];
     uint32_t ns;
     const uint32_t nit = bin_items(csize, K, bin, cnt, ns, item_tiles);
-    const uint32_t c0 = btot[2 * blockIdx.x] + local[2 * bin];
-    const uint32_t i0 = btot[2 * blockIdx.x + 1] + local[2 * bin + 1];
-    cursor[bin] = c0;
-    for (uint32_t j = 0; j < nit; j++) {
+    const int blk = bin >> 8;  // the bin's workgroup of bin_scan_kernel
+    const uint32_t c0 = btot[2 * blk] + local[2 * bin];
+    const uint32_t i0 = btot[2 * blk + 1] + local[2 * bin + 1];
+    if (lane == 0) cursor[bin] = c0;
+    for (uint32_t j = lane; j < nit; j += 64) {
         const uint32_t grp = j / ns, sp = j % ns, w = i0 + j;
         items[3 * w] = (uint32_t)(bin % K);
         items[3 * w + 1] = ((c0 + grp * kItemGroup) << 5) |
@@ -1481,7 +1483,8 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
                        ba.item_tiles, rb.local, rb.btot);
     hipLaunchKernelGGL(bin_total_kernel, dim3(1), dim3(256), 0, s, rb.btot, nblk, rb.local, K, R,
                        ba.n_items);
-    hipLaunchKernelGGL(bin_items_kernel, dim3(nblk), dim3(256), 0, s, rb.counts, ba.csize, K, nb,
+    hipLaunchKernelGGL(bin_items_kernel, dim3((nb + 3) / 4), dim3(256), 0, s, rb.counts, ba.csize,
+                       K, nb,
                        ba.item_tiles, rb.local, rb.btot, rb.cursor, (uint32_t*)ba.items);
     hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
                        (int)kFirstRoundRank, r_end, K, rb.cursor, (uint32_t*)ba.entries);

@@ -408,28 +408,47 @@ __global__ void __launch_bounds__(kThreads)
 
 __global__ void zero_u32_kernel(uint32_t* p) { *p = 0; }
 
-// CSR offsets from per-query lengths (single workgroup, chunked inclusive scan).
+// CSR offsets from per-query lengths by one workgroup, 8192 lengths at a time: loaded into
+// LDS with coalesced reads, thread t sums lengths 8 t .. 8 t + 7, the sums are scanned across
+// the workgroup, and the offsets leave through LDS again.
 __global__ void __launch_bounds__(1024) lens_to_offsets_kernel(const uint32_t* __restrict__ len,
                                                                int nq, uint32_t* __restrict__ off) {
-    __shared__ uint32_t sc[1024];
-    __shared__ uint32_t carry;
-    const int t = threadIdx.x;
-    if (t == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nq; base += 1024) {
-        uint32_t v = base + t < nq ? len[base + t] : 0;
-        sc[t] = v;
+    constexpr int kPer = 8, kTile = 1024 * kPer;
+    __shared__ uint32_t tile[kTile];
+    __shared__ uint32_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint32_t carry = 0;
+    for (int base = 0; base < nq; base += kTile) {
+        for (int i = t; i < kTile; i += 1024) tile[i] = base + i < nq ? len[base + i] : 0u;
         __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            uint32_t a = t >= o ? sc[t - o] : 0;
-            __syncthreads();
-            sc[t] += a;
-            __syncthreads();
+        uint32_t v[kPer], s = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            v[j] = tile[kPer * t + j];
+            s += v[j];
         }
-        if (base + t < nq) off[base + t] = carry + sc[t] - v;
+        uint32_t x = s;  // inclusive scan over the wave, then over the waves
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
         __syncthreads();
-        if (t == 1023) carry += sc[1023];
+        uint32_t pre = carry, tot = 0;
+        for (int w = 0; w < 16; w++) {
+            pre += w < wv ? wsum[w] : 0u;
+            tot += wsum[w];
+        }
+        uint32_t run = pre + x - s;  // lengths before this thread's eight
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            tile[kPer * t + j] = run;
+            run += v[j];
+        }
         __syncthreads();
+        for (int i = t; i < kTile && base + i < nq; i += 1024) off[base + i] = tile[i];
+        carry += tot;
+        __syncthreads();  // the tile and the wave sums are reused by the next chunk
     }
     if (t == 0) off[nq] = carry;
 }
