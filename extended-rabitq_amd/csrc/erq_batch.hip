@@ -363,10 +363,8 @@ __global__ void __launch_bounds__(256) prep_kernel(BatchArgs a, int per_wave_byt
 // the xor butterfly, then the distance in double as C promotes it (HASearcher.hpp:183-231).
 // uf(i) gives u for the lane's dim 16 i + l.
 template <int EX>
-__device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P, const float* ur,
-                                            const uint32_t* lanecode, bool act, float ipq,
-                                            float on, uint32_t vi) {
-    const float part = act ? ex_chain<EX>(ur, lanecode, a.D >> 4, a.exl_w) : 0.f;
+__device__ __forceinline__ float reest_distance(const BatchArgs& a, const QParam& P, float part,
+                                                bool act, float ipq, float on, uint32_t vi) {
     const float ip = butterfly16(part);
     if (!act) return FLT_MAX;
     const float R = (float)(1 << EX);
@@ -377,6 +375,14 @@ __device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P,
     const double G = (double)Cc - ((double)R - 0.5) * (double)P.sumq;
     const double H = (double)Bb * G;
     return (float)((double)Aa - H);
+}
+
+template <int EX>
+__device__ __forceinline__ float reestimate(const BatchArgs& a, const QParam& P, const float* ur,
+                                            const uint32_t* lanecode, bool act, float ipq,
+                                            float on, uint32_t vi) {
+    const float part = act ? ex_chain<EX>(ur, lanecode, a.D >> 4, a.exl_w) : 0.f;
+    return reest_distance<EX>(a, P, part, act, ipq, on, vi);
 }
 
 // Above D = 2048 (nchunk > 4): a survivor found on the exact FastScan integer I gets the
@@ -950,8 +956,73 @@ __global__ void __launch_bounds__(256) reest_kernel(BatchArgs a, int scan_grid) 
         }
         return v;
     };
-    // Each step's records, and the next run's parameters when that run starts inside this
-    // step, are loaded one step ahead.
+    int nfd, rem, nqw;  // shape of a lane's exl codes
+    ex_chain_shape<EX>(D >> 4, nfd, rem, nqw);
+    if (!wrap && nqw <= 4) {
+        // Every lane's codes fit four 16-byte words: a step's code words are requested before
+        // its pair's residual is staged, so the two waits overlap.  Records and the next
+        // run's parameters are loaded one step ahead, as below.
+        uint4 rec = load_rec(r_begin + lg);
+        for (size_t r = r_begin; r < r_end;) {  // wave-uniform
+            const bool inr = r + lg < r_end;
+            const uint32_t e0 = __shfl(rec.x, 0);
+            const uint64_t same = __ballot(inr && rec.x == e0);
+            const int m = same == ~0ull ? 4 : __builtin_ctzll(~same) >> 4;
+            const bool act = lg < m;
+            uint4 cw[4];
+            if (act) {
+                const uint4* src =
+                    reinterpret_cast<const uint4*>(a.exl + ((size_t)rec.y * 16 + l16) * a.exl_w);
+#pragma unroll
+                for (int j4 = 0; j4 < 4; j4++)
+                    if (j4 < nqw) cw[j4] = src[j4];
+            }
+            const uint4 rec_next = load_rec(r + m + lg);
+            const uint32_t e_next = m < 4 ? __shfl(rec.x, 16 * m) : 0xFFFFFFFFu;
+            if (e0 != staged) {
+                if (e0 != pn_e) P = reinterpret_cast<const QParam*>(a.qparams)[e0];
+                else P = Pn;
+                const float* qv = a.q_rot + (size_t)P.q * D;
+                const float* cen = a.centroids + (size_t)P.cid * D;
+                wave_sync();
+                if (P.y > 1e-5f) {
+                    stage_residual<kReestSlices>(qv, cen, P.fac, D, S, uw, lane);
+                } else {
+                    for (int d = lane; d < D; d += 64) uw[ul_index(d, S)] = value;
+                }
+                wave_sync();
+                staged = e0;
+            }
+            if (e_next != 0xFFFFFFFFu) {  // the next run's first record is in this step
+                Pn = reinterpret_cast<const QParam*>(a.qparams)[e_next];
+                pn_e = e_next;
+            }
+            const uint32_t vi = act ? rec.y : 0u;
+            const float on = act ? __uint_as_float(rec.w) : 0.f;
+            const float ipq = act ? __int_as_float((int)rec.z) : 0.f;  // the scaled product
+            float part = 0.f;
+            if (act)
+                ex_chain_words<EX>(reinterpret_cast<const float4*>(uw + l16 * S), cw, 0, nqw,
+                                   nfd, rem, part);
+            const float exd = reest_distance<EX>(a, P, part, act, ipq, on, vi);
+            if (act && l16 == 0) {
+                if (exd <= P.thr) {
+                    add_candidate(a, P.q, exd, a.ids[vi]);
+                    st_cand++;
+                }
+                st_surv++;
+            }
+            rec = rec_next;
+            r += m;
+        }
+        if (a.stats) {
+            wave_stat_add(&a.stats[2], st_surv);
+            wave_stat_add(&a.stats[4], st_cand);
+        }
+        return;
+    }
+    // Otherwise each step's records, and the next run's parameters when that run starts
+    // inside this step, are loaded one step ahead.
     uint4 rec = load_rec(r_begin + lg);
     for (size_t r = r_begin; r < r_end;) {  // wave-uniform
         const bool inr = r + lg < r_end;

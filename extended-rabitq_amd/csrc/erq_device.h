@@ -398,6 +398,62 @@ __device__ __forceinline__ void ex_group_fma(float4 u, uint32_t x, int h, float&
     part = fmaf(u.w, e.w, part);
 }
 
+// The chain over 16-byte code words b4 .. b4 + 3 (those below nq) of a lane's exl codes,
+// held in c4: part = fma(u_i, e_i, part) for their values in order.  nfd = full dwords,
+// rem = 4-value groups of the last, partly used dword (ex_chain).
+template <int EX>
+__device__ __forceinline__ void ex_chain_words(const float4* __restrict__ u4, const uint4 (&c4)[4],
+                                               int b4, int nq, int nfd, int rem, float& part) {
+    constexpr int npb = exl_npb(EX);
+#pragma unroll
+    for (int j4 = 0; j4 < 4; j4++) {
+        if (b4 + j4 >= nq) break;
+        const uint32_t cs[4] = {c4[j4].x, c4[j4].y, c4[j4].z, c4[j4].w};
+        const int dw0 = 4 * (b4 + j4);
+        if (dw0 + 4 <= nfd) {
+            // every value of the word: no branch; the next dword's u is read while this
+            // one's FMAs run (scheduling barriers keep the compiler from hoisting every
+            // read of the word, which costs registers)
+            float4 ua[npb], ub[npb];
+#pragma unroll
+            for (int h = 0; h < npb; h++) ua[h] = u4[dw0 * npb + h];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (j < 3) {
+#pragma unroll
+                    for (int h = 0; h < npb; h++) ub[h] = u4[(dw0 + j + 1) * npb + h];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int h = 0; h < npb; h++) ex_group_fma<EX>(ua[h], cs[j], h, part);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int h = 0; h < npb; h++) ua[h] = ub[h];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int dw = dw0 + j;
+                const int nh = dw < nfd ? npb : (dw == nfd ? rem : 0);
+#pragma unroll
+                for (int h = 0; h < npb; h++)
+                    if (h < nh) ex_group_fma<EX>(u4[dw * npb + h], cs[j], h, part);
+            }
+        }
+    }
+}
+
+// The shape of a lane's exl codes for nvals values: full dwords, groups of the last dword,
+// 16-byte words holding values.
+template <int EX>
+__device__ __forceinline__ void ex_chain_shape(int nvals, int& nfd, int& rem, int& nq) {
+    constexpr int npb = exl_npb(EX);
+    const int ng = nvals >> 2;
+    nfd = ng / npb;
+    rem = ng - nfd * npb;
+    nq = (nfd + (rem ? 1 : 0) + 3) >> 2;
+}
+
 // One lane's share of <u, e> for the re-estimate: part = fma(u_i, e_i, part) over
 // i = 0 .. nvals-1 in order (nvals a multiple of 4); ur = the lane's u row (u_i = ur[i], for
 // dim 16 i + l, 16-byte aligned), code = the lane's W exl dwords.  u is read four values at
@@ -405,54 +461,17 @@ __device__ __forceinline__ void ex_group_fma(float4 u, uint32_t x, int h, float&
 template <int EX>
 __device__ __forceinline__ float ex_chain(const float* __restrict__ ur,
                                           const uint32_t* __restrict__ code, int nvals, int W) {
-    constexpr int npb = exl_npb(EX);
     const float4* u4 = reinterpret_cast<const float4*>(ur);
     const uint4* cw = reinterpret_cast<const uint4*>(code);
-    const int ng = nvals >> 2;
-    const int nfd = ng / npb, rem = ng - nfd * npb;  // full dwords, groups of the last one
-    const int nq = (nfd + (rem ? 1 : 0) + 3) >> 2;   // 16-byte code words holding values
+    int nfd, rem, nq;
+    ex_chain_shape<EX>(nvals, nfd, rem, nq);
     float part = 0.f;
     for (int b4 = 0; b4 < nq; b4 += 4) {
         uint4 c4[4];  // up to four 16-byte loads in flight before the chain uses them
 #pragma unroll
         for (int j4 = 0; j4 < 4; j4++)
             if (b4 + j4 < nq) c4[j4] = cw[b4 + j4];
-#pragma unroll
-        for (int j4 = 0; j4 < 4; j4++) {
-            if (b4 + j4 >= nq) break;
-            const uint32_t cs[4] = {c4[j4].x, c4[j4].y, c4[j4].z, c4[j4].w};
-            const int dw0 = 4 * (b4 + j4);
-            if (dw0 + 4 <= nfd) {
-                // every value of the word: no branch; the next dword's u is read while this
-                // one's FMAs run (scheduling barriers keep the compiler from hoisting every
-                // read of the word, which costs registers)
-                float4 ua[npb], ub[npb];
-#pragma unroll
-                for (int h = 0; h < npb; h++) ua[h] = u4[dw0 * npb + h];
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (j < 3) {
-#pragma unroll
-                        for (int h = 0; h < npb; h++) ub[h] = u4[(dw0 + j + 1) * npb + h];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int h = 0; h < npb; h++) ex_group_fma<EX>(ua[h], cs[j], h, part);
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int h = 0; h < npb; h++) ua[h] = ub[h];
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int dw = dw0 + j;
-                    const int nh = dw < nfd ? npb : (dw == nfd ? rem : 0);
-#pragma unroll
-                    for (int h = 0; h < npb; h++)
-                        if (h < nh) ex_group_fma<EX>(u4[dw * npb + h], cs[j], h, part);
-                }
-            }
-        }
+        ex_chain_words<EX>(u4, c4, b4, nq, nfd, rem, part);
     }
     (void)W;
     return part;
