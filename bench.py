@@ -67,6 +67,9 @@ def parse(argv=None):
                          "(any), or the share of the top-k it holds (share)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the uneven-nprobe graph, drop-in latency and non-HA legs "
+                         "(large configurations)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device copy first (HBM counter calibration)")
     ap.add_argument("--profile-plain", type=int, default=0, metavar="NPROBE",
@@ -713,21 +716,22 @@ def main(argv=None):
                             "label_s": round(t_lab, 1), "train_s": round(t_train, 1)}
         m.close()
 
-    uneven = None
-    try:
-        uneven = measure_uneven(a, g, q, gt, torch)
-    except Exception as e:  # reported, never hidden
-        log(f"uneven-nprobe leg failed: {e!r}")
+    uneven = latency = nonha = None
+    if not a.no_extras:
+        try:
+            uneven = measure_uneven(a, g, q, gt, torch)
+        except Exception as e:  # reported, never hidden
+            log(f"uneven-nprobe leg failed: {e!r}")
     np_hi = (plain["targets"].get(str(a.target)) or {}).get("nprobe_bracket", [None, 192])[1]
     plain_roof = plain_roofline(r, np_hi)
-    latency = drop_in_latency(r, np_hi)
-    log(f"[drop-in] erq_search per query at nprobe {np_hi}: {latency}")
-    nonha = None
-    try:
-        nonha = nonha_path(r, np_hi)
-    except Exception as e:  # reported, never hidden
-        log(f"non-HA leg failed: {e!r}")
-        nonha = {"error": repr(e)}
+    if not a.no_extras:
+        latency = drop_in_latency(r, np_hi)
+        log(f"[drop-in] erq_search per query at nprobe {np_hi}: {latency}")
+        try:
+            nonha = nonha_path(r, np_hi)
+        except Exception as e:  # reported, never hidden
+            log(f"non-HA leg failed: {e!r}")
+            nonha = {"error": repr(e)}
     cpu = None if a.no_cpu else cpu_baseline(r, plain)
 
     if headline:

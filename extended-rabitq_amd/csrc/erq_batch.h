@@ -54,6 +54,7 @@ struct RoundBufs {
     uint32_t* cursor;  // bins
     uint32_t* local;   // 2 per bin: entries and items before it in its block of 256 bins
     uint32_t* btot;    // 2 per block of 256 bins
+    uint32_t* wgbase;  // kCountGroups x bins: a counting workgroup's entries before its own in a bin
     uint32_t* pool_i;  // nq x k
     int K;
 };
@@ -74,6 +75,10 @@ hipError_t launch_exl_relayout(const uint8_t* long_code, size_t N, int D, int EX
                                hipStream_t s);
 constexpr int kFirstRoundRank = 4;  // ranks below it are the exact per-query replay's
 constexpr int kMaxRounds = 32;
+// Workgroups of the LDS-privatized bin counting and scatter (each a contiguous range of
+// queries); used while R x K bins of 4 bytes fit kCountLds.
+constexpr int kCountGroups = 16;
+constexpr size_t kCountLds = 128 * 1024;
 // The work lists of every round past the exact ranks (ranks [4, r_end), R rounds:
 // [4,16), [16,64), [64,192), then 128 wide) and the per-entry preparation (digits and
 // parameters, no threshold), which need only the probe lists.  ba.items / entries / adig /

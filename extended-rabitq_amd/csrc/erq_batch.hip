@@ -281,6 +281,62 @@ __global__ void scatter_kernel(const uint32_t* __restrict__ off, const uint32_t*
     }
 }
 
+// The same counts and scatter with the atomics of a workgroup's entries gathered in LDS:
+// kCountGroups workgroups of 1024 threads, workgroup g taking queries [nq g / G,
+// nq (g + 1) / G) (a wave per query).  count_priv_kernel counts its entries per bin in LDS,
+// then adds each nonzero count to the bin's total with one atomic and keeps what the atomic
+// returns (the entries of the bin placed before its own) in wgbase; scatter_priv_kernel,
+// after the bins' starts are known (cursor, from bin_items_kernel), places its entries
+// from there with LDS atomics.  A bin's entries come out grouped by workgroup, in any
+// order within it, as the global-atomic kernels leave them in any order.
+__global__ void __launch_bounds__(1024) count_priv_kernel(const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ pid,
+                                                          int nq, int r_first, int r_end, int K,
+                                                          int nb, uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ wgbase) {
+    extern __shared__ uint32_t hcnt[];  // nb
+    for (int i = threadIdx.x; i < nb; i += 1024) hcnt[i] = 0;
+    __syncthreads();
+    const int g = blockIdx.x, G = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int q0 = (int)((long long)nq * g / G), q1 = (int)((long long)nq * (g + 1) / G);
+    for (int q = q0 + wv; q < q1; q += 16) {
+        const uint32_t s = off[q], e = min(off[q + 1], s + (uint32_t)r_end);
+        for (uint32_t p = s + (uint32_t)r_first + lane; p < e; p += 64) {
+            const uint32_t c = pid[p];
+            if (c < (uint32_t)K) atomicAdd(&hcnt[round_of_rank(p - s) * K + c], 1u);
+        }
+    }
+    __syncthreads();
+    for (int bin = threadIdx.x; bin < nb; bin += 1024) {
+        const uint32_t cnt = hcnt[bin];
+        wgbase[(size_t)g * nb + bin] = cnt ? atomicAdd(&counts[bin], cnt) : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(1024) scatter_priv_kernel(const uint32_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ pid,
+                                                            int nq, int r_first, int r_end, int K,
+                                                            int nb,
+                                                            const uint32_t* __restrict__ cursor,
+                                                            const uint32_t* __restrict__ wgbase,
+                                                            uint32_t* __restrict__ entries) {
+    extern __shared__ uint32_t cur[];  // nb
+    const int g = blockIdx.x, G = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int bin = threadIdx.x; bin < nb; bin += 1024)
+        cur[bin] = cursor[bin] + wgbase[(size_t)g * nb + bin];
+    __syncthreads();
+    const int q0 = (int)((long long)nq * g / G), q1 = (int)((long long)nq * (g + 1) / G);
+    for (int q = q0 + wv; q < q1; q += 16) {
+        const uint32_t s = off[q], e = min(off[q + 1], s + (uint32_t)r_end);
+        for (uint32_t p = s + (uint32_t)r_first + lane; p < e; p += 64) {
+            const uint32_t c = pid[p];
+            if (c >= (uint32_t)K) continue;
+            const uint32_t pos = atomicAdd(&cur[round_of_rank(p - s) * K + c], 1u);
+            reinterpret_cast<uint2*>(entries)[pos] = make_uint2(p, (uint32_t)q);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- per-pair preparation
 // One wave per (query, cluster) pair of the round: the reference's query preparation
 // (normalize_query16 + high_acc_quantize16, space.hpp:695-778) and the int8 digits of the
@@ -1547,9 +1603,20 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
     hipError_t e = launch_zero(rb.counts, (size_t)K * R, s);
     if (e != hipSuccess) return e;
     dim3 qg((nq + 3) / 4), qb(256);  // a wave per query
-    hipLaunchKernelGGL(count_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
-                       (int)kFirstRoundRank, r_end, K, rb.counts);
     const int nb = K * R, nblk = (nb + 255) / 256;
+    const size_t priv_lds = 4 * (size_t)nb;
+    const bool priv = rb.wgbase && priv_lds <= kCountLds;
+    if (priv) {
+        e = set_lds(count_priv_kernel, priv_lds);
+        if (e == hipSuccess) e = set_lds(scatter_priv_kernel, priv_lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(count_priv_kernel, dim3(kCountGroups), dim3(1024), priv_lds, s,
+                           ba.probe_off, rb.probe_ids, nq, (int)kFirstRoundRank, r_end, K, nb,
+                           rb.counts, rb.wgbase);
+    } else {
+        hipLaunchKernelGGL(count_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
+                           (int)kFirstRoundRank, r_end, K, rb.counts);
+    }
     hipLaunchKernelGGL(bin_scan_kernel, dim3(nblk), dim3(256), 0, s, rb.counts, ba.csize, K, nb,
                        ba.item_tiles, rb.local, rb.btot);
     hipLaunchKernelGGL(bin_total_kernel, dim3(1), dim3(256), 0, s, rb.btot, nblk, rb.local, K, R,
@@ -1557,8 +1624,14 @@ hipError_t launch_prep_all(const BatchArgs& ba, const RoundBufs& rb, int R, int 
     hipLaunchKernelGGL(bin_items_kernel, dim3((nb + 3) / 4), dim3(256), 0, s, rb.counts, ba.csize,
                        K, nb,
                        ba.item_tiles, rb.local, rb.btot, rb.cursor, (uint32_t*)ba.items);
-    hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
-                       (int)kFirstRoundRank, r_end, K, rb.cursor, (uint32_t*)ba.entries);
+    if (priv) {
+        hipLaunchKernelGGL(scatter_priv_kernel, dim3(kCountGroups), dim3(1024), priv_lds, s,
+                           ba.probe_off, rb.probe_ids, nq, (int)kFirstRoundRank, r_end, K, nb,
+                           rb.cursor, rb.wgbase, (uint32_t*)ba.entries);
+    } else {
+        hipLaunchKernelGGL(scatter_kernel, qg, qb, 0, s, ba.probe_off, rb.probe_ids, nq,
+                           (int)kFirstRoundRank, r_end, K, rb.cursor, (uint32_t*)ba.entries);
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     // every entry of every round at once: ba.n_items points at the totals (rinfo + 4 R)

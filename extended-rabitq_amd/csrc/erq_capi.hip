@@ -1006,8 +1006,8 @@ static size_t batch_chunk_nq(size_t nq, size_t max_nprobe, size_t budget) {
 // The batched search's workspace, in take order (sizing with a null base gives the bytes).
 struct BatchBufs {
     size_t cq = 0, R = 0, r_end = 0;
-    uint32_t *counts, *cursor, *local, *btot, *rinfo, *items, *ovf, *entries, *cand_cnt, *cand_i,
-        *surv_count, *redo, *slow;
+    uint32_t *counts, *cursor, *local, *btot, *wgbase, *rinfo, *items, *ovf, *entries, *cand_cnt,
+        *cand_i, *surv_count, *redo, *slow;
     float* cand_d;
     int8_t* adig;
     char* qparams;
@@ -1034,6 +1034,7 @@ static void batch_layout(const erq_index* h, size_t nq, size_t max_nprobe, int c
     b.cursor = ws.take<uint32_t>(b.R * K);
     b.local = ws.take<uint32_t>(2 * b.R * K);
     b.btot = ws.take<uint32_t>(2 * ((b.R * K + 255) / 256));
+    b.wgbase = ws.take<uint32_t>((size_t)kCountGroups * b.R * K);
     b.rinfo = ws.take<uint32_t>(4 * (b.R + 1));
     b.items = ws.take<uint32_t>(3 * items);
     b.ovf = ws.take<uint32_t>(2 * max_items);
@@ -1154,7 +1155,7 @@ static int batched_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_
         ba.nq = (int)n;
         ba.nchunk = bp_nchunk((int)h->D);
         ba.one_over_sqrtD = (float)(1.0 / (double)std::sqrt((float)h->D));
-        rb = RoundBufs{pid, b.counts, b.cursor, b.local, b.btot, ids + q0 * k, (int)h->K};
+        rb = RoundBufs{pid, b.counts, b.cursor, b.local, b.btot, b.wgbase, ids + q0 * k, (int)h->K};
     };
     auto rounds = [&]() -> hipError_t {
         // the round-mode rescan of the queries whose candidates overflowed (none, normally:

@@ -279,6 +279,55 @@ def test_learned_kept_set_is_the_mass_prefix_rule(mass, min_np, max_np):
     g.close()
 
 
+def test_learned_lists_of_many_queries():
+    """More queries than one 8192-length tile of the offsets scan: the CSR offsets are the
+    running sum of the kept lengths, and lists past the first tile hold what the rule keeps."""
+    import torch
+    from erq.gpu import IVF
+    from erq.probe_model import ProbeMLP
+    from oracle import oracle as orc
+    idx, x, qtr, qte = _small_index()
+    g = IVF.from_index(idx)
+    m = ProbeMLP(idx.D, idx.K, hidden=128, max_batch=512, seed=4)
+    qr = rotate_np(idx, np.concatenate([qtr, qtr, qtr, qte]))   # 12544 queries
+    q = torch.from_numpy(qr).cuda()
+    prob = m.forward_dev(q).cpu().numpy()
+    mass, min_np, max_np = 0.9, 2, 48
+    off, pid, pd, nfb = g.probe_learned_dev(m, q, mass, min_np, max_np, 8, 0.0)
+    torch.cuda.synchronize()
+    off, pid = off.cpu().numpy().astype(np.int64), pid.cpu().numpy().view(np.uint32)
+    lens = np.diff(off)
+    assert off[0] == 0 and np.all(lens >= min_np) and np.all(lens <= max_np)
+    assert off[-1] <= pid.size
+    oi = orc.OracleIndex(idx)
+    for i in list(range(0, qr.shape[0], 997)) + [8191, 8192, 8193, qr.shape[0] - 1]:
+        kept = _kept_by_rule(prob[i], mass, min_np, max_np)
+        all_i, all_d = oi.centroid_topn(qr[i], idx.K)
+        dist = np.empty(idx.K, np.float32)
+        dist[all_i] = all_d
+        want = kept[np.lexsort((kept, dist[kept]))]
+        np.testing.assert_array_equal(pid[off[i]:off[i + 1]], want.astype(np.uint32),
+                                      err_msg=f"query {i}")
+    m.close()
+    g.close()
+
+
+def test_mlp_large_batch_tiles_match_small_batches():
+    """A batch large enough for the GEMM's 128-row tiles (layer 3 at 4096 rows x 4096 outputs)
+    gives every row the probabilities that batches of 64 rows (64-row tiles) give."""
+    import torch
+    from erq.probe_model import ProbeMLP
+    rng = np.random.default_rng(11)
+    din, hid, dout, rows = 256, 128, 4096, 4096
+    m = ProbeMLP(din, dout, hidden=hid, max_batch=64, seed=3, input_scale=16.0)
+    x = torch.from_numpy((rng.standard_normal((rows, din)) / 16.0).astype(np.float32)).cuda()
+    p_big = m.forward_dev(x)
+    p_small = torch.cat([m.forward_dev(x[i:i + 64]) for i in range(0, rows, 64)])
+    torch.cuda.synchronize()
+    assert torch.equal(p_big, p_small)
+    m.close()
+
+
 def test_probe_model_save_load_round_trip(tmp_path):
     import torch
     from erq.probe_model import ProbeMLP
