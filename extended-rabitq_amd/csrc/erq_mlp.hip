@@ -649,6 +649,10 @@ int erq_mlp_forward_dev(erq_mlp* m, const float* x_dev, size_t batch, float* pro
         set_error("erq_mlp_forward_dev: null argument");
         return ERQ_EINVAL;
     }
+    if (((uintptr_t)x_dev | (uintptr_t)prob_dev) % 16) {
+        set_error("erq_mlp_forward_dev: x_dev and prob_dev must be 16-byte aligned");
+        return ERQ_EINVAL;
+    }
     std::lock_guard<std::mutex> lk(m->mu);
     hipStream_t s = mlp_stream(m, stream);
     if (batch == 0) return ERQ_OK;

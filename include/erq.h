@@ -245,7 +245,9 @@ int erq_mlp_set_input_scale(erq_mlp* m, float scale);
 float erq_mlp_input_scale(const erq_mlp* m);
 /* Either output may be NULL; grads are those of the last erq_mlp_loss_grad_dev. */
 int erq_mlp_get_params(erq_mlp* m, float* host_params, float* host_grads);
-/* prob_dev = p for batch rows of x_dev (batch x in_dim), any batch size. */
+/* prob_dev = p for batch rows of x_dev (batch x in_dim), any batch size.  x_dev and prob_dev
+ * are read and written 16 bytes at a time: both must be 16-byte aligned (hipMalloc and torch
+ * allocations are), else ERQ_EINVAL. */
 int erq_mlp_forward_dev(erq_mlp* m, const float* x_dev, size_t batch, float* prob_dev,
                         void* stream);
 /* Forward + backward of the pos-weighted BCE (labels y_dev: batch x out_dim of 0/1, loss =

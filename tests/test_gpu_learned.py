@@ -328,6 +328,29 @@ def test_mlp_large_batch_tiles_match_small_batches():
     m.close()
 
 
+def test_mlp_forward_rejects_unaligned_buffers():
+    """Inference reads its input and writes its probabilities 16 bytes at a time: a buffer
+    off that alignment is refused with ERQ_EINVAL before anything is launched."""
+    import ctypes
+    import torch
+    from erq import _lib
+    from erq.probe_model import ProbeMLP
+    m = ProbeMLP(64, 40, hidden=64, max_batch=32, seed=1)
+    x = torch.zeros(8 * 64 + 1, dtype=torch.float32, device="cuda")
+    p = torch.zeros(8 * 40 + 1, dtype=torch.float32, device="cuda")
+    L = _lib.lib()
+    rc = L.erq_mlp_forward_dev(m.handle, ctypes.c_void_p(x.data_ptr() + 4), 8,
+                               ctypes.c_void_p(p.data_ptr()), None)
+    assert rc != 0 and b"16-byte aligned" in L.erq_last_error()
+    rc = L.erq_mlp_forward_dev(m.handle, ctypes.c_void_p(x.data_ptr()), 8,
+                               ctypes.c_void_p(p.data_ptr() + 4), None)
+    assert rc != 0
+    assert L.erq_mlp_forward_dev(m.handle, ctypes.c_void_p(x.data_ptr()), 8,
+                                 ctypes.c_void_p(p.data_ptr()), None) == 0
+    torch.cuda.synchronize()
+    m.close()
+
+
 def test_probe_model_save_load_round_trip(tmp_path):
     import torch
     from erq.probe_model import ProbeMLP
