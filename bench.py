@@ -595,6 +595,48 @@ def drop_in_latency(r, nprobe, nq=256):
     r.g.set_option(8, 16)
     out["nprobe"] = nprobe
     out["median_us"] = out[f"latency_scan_nprobe{nprobe}"]["median_us"]
+    try:
+        out["cpp_test_search"] = cpp_dropin(r, nprobe, nq)
+    except Exception as e:  # reported, never hidden
+        log(f"C++ drop-in leg failed: {e!r}")
+        out["cpp_test_search"] = {"error": repr(e)}
+    return out
+
+
+def cpp_dropin(r, nprobe, nq):
+    """test_search.cpp's loop through the C++ drop-in (host/ivf_gpu.hpp): the index saved to
+    a file and loaded by tests/cpp/test_search_dropin.cpp, nq raw queries zero-padded and
+    rotated by one call, each query searched and timed on its own, 3 rounds; QPS = nq /
+    (summed search time + rotation time), as the reference computes it."""
+    import shutil
+    import subprocess
+    import tempfile
+    gxx = shutil.which("g++")
+    if gxx is None:
+        return {"error": "no g++ on this host"}
+    lib = os.path.join(REPO, "extended-rabitq_amd", "lib")
+    out = {"queries": nq, "rounds": 3}
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "test_search_dropin")
+        subprocess.run([gxx, "-std=c++17", "-O2", "-I", os.path.join(REPO, "include"),
+                        "-I", os.path.join(REPO, "extended-rabitq_amd", "host"),
+                        os.path.join(REPO, "tests", "cpp", "test_search_dropin.cpp"),
+                        "-L", lib, "-lerq", f"-Wl,-rpath,{lib}", "-o", exe], check=True)
+        path = os.path.join(tmp, "index.bin")
+        r.idx.save(path)
+        qf = os.path.join(tmp, "q.f32")
+        r.q[:nq].cpu().numpy().astype(np.float32).tofile(qf)
+        for npb in (nprobe, 5):
+            tj = os.path.join(tmp, f"t{npb}.json")
+            subprocess.run([exe, path, qf, str(nq), str(r.a.k), str(npb),
+                            os.path.join(tmp, "ids.u32"), tj, "3"], check=True, timeout=600)
+            t = json.load(open(tj))
+            qps = [rd["qps"] for rd in t["rounds"]]
+            out[f"nprobe{npb}"] = {"qps": round(float(np.mean(qps)), 1),
+                                   "qps_rounds": [round(v, 1) for v in qps],
+                                   "median_us": t["median_us"], "mean_us": t["mean_us"],
+                                   "rotate_us": t["rotate_us"]}
+            log(f"[drop-in C++] nprobe {npb}: {out[f'nprobe{npb}']}")
     return out
 
 

@@ -48,3 +48,33 @@ def test_dropin_search_matches_python_path(tmp_path, small_data):
     want, _ = g.search_batch(qr, 100, 8)
     np.testing.assert_array_equal(got, want)
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a GPU")
+def test_dropin_timing_file(tmp_path, small_data):
+    """The timing mode searches every query `rounds` times with the same results and writes
+    test_search's QPS (queries over summed search time plus rotation time) per round."""
+    import json
+    from erq import index
+    x, q, c, assign = small_data
+    idx = index.build_index(x, c, assign, bits=4, seed=3)
+    path = tmp_path / "i.index"
+    idx.save(str(path))
+    qf = tmp_path / "q.f32"
+    q[:16].astype(np.float32).tofile(str(qf))
+    exe = _compile(tmp_path)
+    out1, out2, tj = tmp_path / "a.u32", tmp_path / "b.u32", tmp_path / "t.json"
+    r = subprocess.run([str(exe), str(path), str(qf), "16", "50", "6", str(out1)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe), str(path), str(qf), "16", "50", "6", str(out2), str(tj), "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert np.array_equal(np.fromfile(str(out1), np.uint32), np.fromfile(str(out2), np.uint32))
+    t = json.loads(tj.read_text())
+    assert t["nq"] == 16 and t["nprobe"] == 6 and len(t["rounds"]) == 2
+    for rd in t["rounds"]:
+        assert rd["search_us"] > 0
+        assert abs(rd["qps"] - 16 / ((rd["search_us"] + t["rotate_us"]) / 1e6)) <= 1e-3 * rd["qps"] + 0.1
+    assert 0 < t["median_us"] and 0 < t["mean_us"]
