@@ -119,4 +119,48 @@ __device__ __forceinline__ void pool_insert_bulk(const WaveLds& w, int& psize, i
     psize = min(k, psize + ns);
 }
 
+// pool_insert_bulk for k <= 128 without chains of dependent LDS reads: the pool's two
+// 64-entry halves (entry lane and lane + 64) and the new entries sit in registers, and every
+// count is one pass over the new entries read with v_readlane (a wave-uniform index): new
+// entry t's place = (pool entries below it) + (new entries before it), pool entry i's = i +
+// (new entries with a distance <= its own), as in pool_insert_bulk.  All entries are read
+// before any is written.  nd and nid are not changed.
+__device__ __forceinline__ void pool_insert_bulk128(const WaveLds& w, int& psize, int k, int lane,
+                                                    const float* nd, const uint32_t* nid, int ns) {
+    float dt = FLT_MAX;
+    uint32_t it = 0;
+    if (lane < ns) {
+        dt = nd[lane];
+        it = nid[lane];
+    }
+    const bool h0 = lane < psize, h1 = lane + kWave < psize;
+    const float v0 = h0 ? w.pd[lane] : 0.f, v1 = h1 ? w.pd[lane + kWave] : 0.f;
+    const uint32_t i0 = h0 ? w.pid[lane] : 0u, i1 = h1 ? w.pid[lane + kWave] : 0u;
+    int rt = 0, lo = 0, m0 = 0, m1 = 0;
+    for (int u = 0; u < ns; u++) {  // wave-uniform
+        const float du = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dt), u));
+        rt += (du < dt || (du == dt && u > lane)) ? 1 : 0;  // new entries placed before t
+        m0 += du <= v0 ? 1 : 0;                            // new entries placed before v0
+        m1 += du <= v1 ? 1 : 0;
+        const int below = __popcll(ballot(h0 && v0 < du)) + __popcll(ballot(h1 && v1 < du));
+        if (lane == u) lo = below;  // pool entries below new entry u
+    }
+    wave_sync();  // every lane has read its pool entries
+    const int d0 = lane + m0, d1 = lane + kWave + m1, at = lo + rt;
+    if (h0 && d0 < k) {
+        w.pd[d0] = v0;
+        w.pid[d0] = i0;
+    }
+    if (h1 && d1 < k) {
+        w.pd[d1] = v1;
+        w.pid[d1] = i1;
+    }
+    if (lane < ns && at < k) {
+        w.pd[at] = dt;
+        w.pid[at] = it;
+    }
+    wave_sync();
+    psize = min(k, psize + ns);
+}
+
 }  // namespace erq

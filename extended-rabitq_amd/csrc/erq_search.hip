@@ -749,7 +749,8 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
             if (MODE == kModeReplay) {
                 if (ns) {
                     wave_sync();
-                    pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
+                    if (k <= 128) pool_insert_bulk128(w, psize, k, lane, w.nd, w.nid, ns);
+                    else pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
                 }
                 st_surv += ns;
                 distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
@@ -955,7 +956,7 @@ constexpr int kReplayChunk = 1024;  // slots staged in LDS at a time
 
 // One wave per query: the query's slots in probe order, block by block (32 slots): the
 // vectors whose lower bound is below the pool's k-th distance, in slot order, go into the
-// pool by pool_insert_bulk -- the per-block step of scan_query_mfma.  The next chunk's slots
+// pool by pool_insert_bulk(128) -- the per-block step of scan_query_mfma.  The next chunk's slots
 // are loaded while the current one is replayed.
 __global__ void __launch_bounds__(64) replay_est_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1052,7 +1053,8 @@ __global__ void __launch_bounds__(64) replay_est_kernel(ScanArgs a) {
                     w.nid[t] = I[b0 + lane];
                 }
                 wave_sync();
-                pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
+                if (k <= 128) pool_insert_bulk128(w, psize, k, lane, w.nd, w.nid, ns);
+                else pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
                 st_surv += ns;
                 distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
             }
