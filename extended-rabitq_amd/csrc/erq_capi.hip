@@ -169,6 +169,7 @@ void release(erq_index* h) {
                     h->hids,       h->hdists,    h->hnp,       h->lat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (h->pin) (void)hipHostFree(h->pin);
     if (h->aux) {
         (void)hipStreamSynchronize(h->aux);
         (void)hipStreamDestroy(h->aux);
@@ -219,6 +220,27 @@ int host_bufs(erq_index* h, size_t nq, size_t k) {
     h->hq_cap = cq;
     h->hk_cap = ck;
     return ERQ_OK;
+}
+
+// Pinned staging of at least `bytes` for a host-pointer call's copies, or null when that is
+// above kPinMax or cannot be allocated (the copies then use the caller's memory).  The
+// previous call's copies have completed: every host-pointer call synchronises before it
+// returns, under the handle's lock.
+constexpr size_t kPinMax = 1u << 20;
+void* pinned(erq_index* h, size_t bytes) {
+    if (bytes > kPinMax) return nullptr;
+    if (bytes <= h->pin_cap) return h->pin;
+    if (h->pin) (void)hipHostFree(h->pin);
+    h->pin = nullptr;
+    h->pin_cap = 0;
+    const size_t want = std::max<size_t>(bytes, 64 * 1024);
+    if (hipHostMalloc(&h->pin, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();  // the pageable path still works
+        h->pin = nullptr;
+        return nullptr;
+    }
+    h->pin_cap = want;
+    return h->pin;
 }
 
 }  // namespace
@@ -1449,7 +1471,11 @@ static int host_search(erq_index* h, const float* q_rot, size_t nq, size_t k, si
     Call call(h, s);
     int rc = host_bufs(h, nq, k);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(h->hq, q_rot, nq * h->D * 4, hipMemcpyHostToDevice, s);
+    const size_t in_bytes = nq * h->D * 4, out_bytes = nq * k * 4;
+    char* pin = static_cast<char*>(pinned(h, std::max(in_bytes, 2 * out_bytes)));
+    if (pin) std::memcpy(pin, q_rot, in_bytes);
+    hipError_t e = hipMemcpyAsync(h->hq, pin ? (const void*)pin : (const void*)q_rot, in_bytes,
+                                  hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "copy queries");
     if (!np_host) {
         rc = batched ? erq_search_batched_dev(h, h->hq, nq, k, nprobe, h->hids, h->hdists, nullptr, s)
@@ -1491,11 +1517,18 @@ static int host_search(erq_index* h, const float* q_rot, size_t nq, size_t k, si
                                    s, 0, nullptr, max_np);
     }
     if (rc) return rc;
-    e = hipMemcpyAsync(results, h->hids, nq * k * 4, hipMemcpyDeviceToHost, s);
+    // the staging buffer is free again: the query copy precedes the kernels on the stream
+    e = hipMemcpyAsync(pin ? (void*)pin : (void*)results, h->hids, out_bytes,
+                       hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && dists)
-        e = hipMemcpyAsync(dists, h->hdists, nq * k * 4, hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(pin ? (void*)(pin + out_bytes) : (void*)dists, h->hdists, out_bytes,
+                           hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "copy results");
+    if (pin) {
+        std::memcpy(results, pin, out_bytes);
+        if (dists) std::memcpy(dists, pin + out_bytes, out_bytes);
+    }
     return ERQ_OK;
 }
 

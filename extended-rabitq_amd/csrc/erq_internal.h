@@ -69,6 +69,10 @@ struct erq_index {
     float* hdists = nullptr;     // hq_cap x hk
     uint32_t* hnp = nullptr;     // hq_cap + 1 (per-query probe counts, then CSR offsets)
     size_t hq_cap = 0, hk_cap = 0;
+    // pinned host staging of those calls' small copies (queries in, ids and distances out),
+    // grow-only up to kPinMax bytes; larger copies go to and from the caller's memory
+    void* pin = nullptr;
+    size_t pin_cap = 0;
     int depth = 0;  // nesting of entry points on this thread (only the outermost orders)
     // stage timing (erq_timing_begin / erq_timing_end): event pairs recorded around stages
     std::vector<hipEvent_t> tim_ev;   // 2 per record
