@@ -667,7 +667,8 @@ static size_t latency_bytes(erq_index* h, size_t nq, size_t max_np, size_t* slot
     }
     const size_t slots = nq * h->top_pad[std::min(max_np, h->top_pad.size() - 1)];
     if (slots_out) *slots_out = slots;
-    return slots * 12 + (nq * max_np + 1) * 8 + 256;  // slots; pair offsets and first items
+    // slots; pair offsets and first items; block minima
+    return align_up(slots * 12 + (nq * max_np + 1) * 8, 256) + slots / 32 * 4 + 256;
 }
 
 static int scan_latency_locked(erq_index* h, ScanArgs& a, size_t nq, size_t max_np,
@@ -691,6 +692,7 @@ static int scan_latency_locked(erq_index* h, ScanArgs& a, size_t nq, size_t max_
     a.est_exd = a.est_lower + slots;
     a.est_id = reinterpret_cast<uint32_t*>(a.est_exd + slots);
     a.est_off = a.est_id + slots;
+    a.est_bmin = reinterpret_cast<float*>(p + align_up(slots * 12 + (nq * max_np + 1) * 8, 256));
     StageTimer tim(h, ERQ_STAGE_REPLAY, s);
     hipError_t e = launch_scan_latency(a, s);
     if (e != hipSuccess) return hip_fail(e, "latency scan");

@@ -146,6 +146,7 @@ struct ScanArgs {
     float* est_exd;
     uint32_t* est_id;
     uint32_t* est_off;
+    float* est_bmin;  // per 32-slot block (slot / 32): the smallest lower bound of its slots
     // Non-HA int8 MFMA scan (a.bp set, nchunk <= 4): [0] count, then the queries it handed
     // back to the LUT kernel (a digit above 16383); nullptr: the LUT kernel alone.
     uint32_t* nonha_redo;

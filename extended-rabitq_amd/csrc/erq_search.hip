@@ -638,7 +638,11 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
             };
             lower_test();
             uint64_t mask = ballot(cand);
-            if (MODE == kModeEstimate && lane < 32) a.est_lower[est0 + 32 * b + lane] = low_l;
+            if (MODE == kModeEstimate) {
+                if (lane < 32) a.est_lower[est0 + 32 * b + lane] = low_l;
+                const float bm = wave_min(low_l);  // lanes past the block's vectors hold INFINITY
+                if (lane == 0) a.est_bmin[(est0 >> 5) + b] = bm;
+            }
             if constexpr (kWrap) {
                 // the reference's integer for the vectors that pass on the exact one (four at
                 // a time, one per 16-lane group), then the lower bound again
@@ -760,7 +764,14 @@ __device__ __forceinline__ void scan_query_mfma(const ScanArgs& a, int qi, unsig
             }
         }
     }
-    if (MODE == kModeEstimate) return;
+    if (MODE == kModeEstimate) {
+        // the blocks and vectors of the work item (the replay reads only candidate blocks)
+        if (a.stats && lane == 0 && st_blocks) {
+            atomicAdd(&a.stats[0], st_blocks);
+            atomicAdd(&a.stats[1], st_vec);
+        }
+        return;
+    }
     uint32_t* oi = a.out_ids + (size_t)qi * k;
     float* od = a.out_dists ? a.out_dists + (size_t)qi * k : nullptr;
     for (int i = lane; i < k; i += kWave) {
@@ -952,24 +963,23 @@ __global__ void __launch_bounds__(1024) est_offsets_kernel(const uint32_t* __res
     }
 }
 
-constexpr int kReplayChunk = 1024;  // slots staged in LDS at a time
-
-// One wave per query: the query's slots in probe order, block by block (32 slots): the
-// vectors whose lower bound is below the pool's k-th distance, in slot order, go into the
-// pool by pool_insert_bulk(128) -- the per-block step of scan_query_mfma.  The next chunk's slots
-// are loaded while the current one is replayed.
+// One wave per query: the query's blocks in probe order with the reference's per-block step
+// of scan_query_mfma (the vectors whose lower bound is below the pool's k-th distance at the
+// block's start, in slot order, into the pool).  A block whose smallest lower bound (est_bmin)
+// is not below the k-th distance has no such vector, and the k-th distance only falls, so the
+// replay reads the minima of 256 blocks at a time (the next 256 in flight) and fetches the
+// slots of the candidate blocks only, eight blocks per request.  The block and vector
+// counts come from the estimate kernel.
+constexpr int kReplayWin = 256;  // block minima per step (4 per lane)
 __global__ void __launch_bounds__(64) replay_est_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x, k = a.k, kcap = (k + 63) & ~63;
     const int qi = blockIdx.x;
-    float* cl = reinterpret_cast<float*>(smem);            // 2 x kReplayChunk lower bounds
-    float* ce = cl + 2 * kReplayChunk;                     // estimates
-    uint32_t* ci = reinterpret_cast<uint32_t*>(ce + 2 * kReplayChunk);  // ids
     WaveLds w;
     w.uq = nullptr;
     w.lut = nullptr;
     w.sums = nullptr;
-    w.pd = reinterpret_cast<float*>(ci + 2 * kReplayChunk);
+    w.pd = reinterpret_cast<float*>(smem);
     w.pid = reinterpret_cast<uint32_t*>(w.pd + kcap);
     w.nd = reinterpret_cast<float*>(w.pid + kcap);
     w.nid = reinterpret_cast<uint32_t*>(w.nd + 32);
@@ -977,91 +987,80 @@ __global__ void __launch_bounds__(64) replay_est_kernel(ScanArgs a) {
         w.pd[i] = FLT_MAX;
         w.pid[i] = 0;
     }
+    wave_sync();
     int psize = 0;
     uint32_t p0 = a.probe_off[qi], p1 = a.probe_off[qi + 1];
     if (a.max_probes > 0 && p1 - p0 > (uint32_t)a.max_probes) p1 = p0 + (uint32_t)a.max_probes;
-    const uint32_t s0 = a.est_off[p0], s1 = a.est_off[p1];
-    constexpr int kPer = kReplayChunk / kWave;
-    float rl[kPer], re[kPer];
-    uint32_t ri[kPer];
-    auto fetch = [&](uint32_t c0) {  // registers <- slots [c0, c0 + chunk) (clamped)
-#pragma unroll
-        for (int r = 0; r < kPer; r++) {
-            const uint32_t sl = c0 + (uint32_t)(r * kWave + lane);
-            const bool ok = sl < s1;
-            rl[r] = ok ? a.est_lower[sl] : INFINITY;
-            re[r] = ok ? a.est_exd[sl] : 0.f;
-            ri[r] = ok ? a.est_id[sl] : 0u;
-        }
-    };
-    auto stash = [&](int buf) {
-#pragma unroll
-        for (int r = 0; r < kPer; r++) {
-            cl[buf * kReplayChunk + r * kWave + lane] = rl[r];
-            ce[buf * kReplayChunk + r * kWave + lane] = re[r];
-            ci[buf * kReplayChunk + r * kWave + lane] = ri[r];
-        }
-    };
-    unsigned long long st_blocks = 0, st_vec = 0, st_surv = 0;
+    const uint32_t B0 = a.est_off[p0] >> 5, B1 = a.est_off[p1] >> 5;  // slots come in 32s
+    unsigned long long st_surv = 0;
     float distk = FLT_MAX;
-    int buf = 0;
-    if (s0 < s1) {
-        fetch(s0);
-        stash(0);
-    }
-    wave_sync();
-    for (uint32_t c0 = s0; c0 < s1; c0 += kReplayChunk, buf ^= 1) {  // wave-uniform
-        const bool more = c0 + kReplayChunk < s1;
-        if (more) fetch(c0 + kReplayChunk);
-        const int n = (int)min((uint32_t)kReplayChunk, s1 - c0);
-        const float* L = cl + buf * kReplayChunk;
-        const float* E = ce + buf * kReplayChunk;
-        const uint32_t* I = ci + buf * kReplayChunk;
-        for (int g0 = 0; g0 < n; g0 += 256) {
-            // eight blocks at a time: none of their slots below the k-th distance (the common
-            // case once the pool has filled) -- nothing to insert, only the counts
-            float lv8[4];
-            bool any = false;
+    float bm[4], bn[4];
+    auto minima = [&](uint32_t w0, float (&m)[4]) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int sl = g0 + 64 * r + lane;
-                lv8[r] = sl < n ? L[sl] : INFINITY;
-                any = any || lv8[r] < distk;
-            }
-            if (!ballot(any)) {
+        for (int r = 0; r < 4; r++) {
+            const uint32_t b = w0 + (uint32_t)(64 * r + lane);
+            m[r] = b < B1 ? a.est_bmin[b] : INFINITY;
+        }
+    };
+    if (B0 < B1) minima(B0, bm);
+    for (uint32_t w0 = B0; w0 < B1; w0 += kReplayWin) {  // wave-uniform
+        if (w0 + kReplayWin < B1) minima(w0 + kReplayWin, bn);
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const uint64_t v = ballot(lv8[r] != INFINITY);
-                    st_blocks += ((v & 0xFFFFFFFFull) != 0) + ((v >> 32) != 0);
-                    st_vec += __popcll(v);
+        for (int r = 0; r < 4; r++) {
+            const uint32_t base = w0 + 64 * r;
+            uint64_t cand = ballot(bm[r] < distk);
+            while (cand) {  // wave-uniform: up to 8 candidate blocks per request
+                uint32_t blk[8];
+                int nb = 0;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    blk[j] = base;
+                    if (cand) {
+                        blk[j] = base + (uint32_t)(__ffsll((unsigned long long)cand) - 1);
+                        cand &= cand - 1;
+                        nb++;
+                    }
                 }
-                continue;
-            }
-        for (int b0 = g0; b0 < min(n, g0 + 256); b0 += 32) {
-            const float lv = lane < 32 ? L[b0 + lane] : INFINITY;
-            const uint64_t valid = ballot(lane < 32 && lv != INFINITY);
-            const uint64_t mask = ballot(lane < 32 && lv < distk);
-            if (valid) {
-                st_blocks++;
-                st_vec += __popcll(valid);
-            }
-            const int ns = __popcll(mask);
-            if (ns) {
-                if ((mask >> lane) & 1ull) {
-                    const int t = __popcll(mask & ((1ull << lane) - 1ull));
-                    w.nd[t] = E[b0 + lane];
-                    w.nid[t] = I[b0 + lane];
+                // lane l holds slot l % 32 of blocks 2 i + l / 32, i < 4
+                float lv[4], ev[4];
+                uint32_t iv[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int j = 2 * i + (lane >> 5);
+                    const bool ok = j < nb;
+                    const uint32_t bj = (lane >> 5) ? blk[2 * i + 1] : blk[2 * i];
+                    const size_t sl = (size_t)bj * 32 + (lane & 31);
+                    lv[i] = ok ? a.est_lower[sl] : INFINITY;
+                    ev[i] = ok ? a.est_exd[sl] : 0.f;
+                    iv[i] = ok ? a.est_id[sl] : 0u;
                 }
-                wave_sync();
-                if (k <= 128) pool_insert_bulk128(w, psize, k, lane, w.nd, w.nid, ns);
-                else pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
-                st_surv += ns;
-                distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    if (j >= nb) break;
+                    const int i = j >> 1, half = j & 1;
+                    const uint64_t m64 = ballot(lv[i] < distk);
+                    const uint32_t mask = half ? (uint32_t)(m64 >> 32) : (uint32_t)m64;
+                    const int ns = __popc(mask);
+                    if (ns) {
+                        const int l32 = lane & 31;
+                        if ((lane >> 5) == half && ((mask >> l32) & 1u)) {
+                            const int t = __popc(mask & ((1u << l32) - 1u));
+                            w.nd[t] = ev[i];
+                            w.nid[t] = iv[i];
+                        }
+                        wave_sync();
+                        if (k <= 128) pool_insert_bulk128(w, psize, k, lane, w.nd, w.nid, ns);
+                        else pool_insert_bulk(w, psize, k, lane, w.nd, w.nid, ns);
+                        st_surv += ns;
+                        distk = (psize == k) ? w.pd[k - 1] : FLT_MAX;
+                    }
+                }
+                // the rest of this group's candidates against the lowered k-th distance
+                cand &= ballot(bm[r] < distk);
             }
         }
-        }
-        if (more) stash(buf ^ 1);
-        wave_sync();
+#pragma unroll
+        for (int r = 0; r < 4; r++) bm[r] = bn[r];
     }
     uint32_t* oi = a.out_ids + (size_t)qi * k;
     float* od = a.out_dists ? a.out_dists + (size_t)qi * k : nullptr;
@@ -1070,8 +1069,6 @@ __global__ void __launch_bounds__(64) replay_est_kernel(ScanArgs a) {
         if (od) od[i] = i < psize ? w.pd[i] : FLT_MAX;
     }
     if (a.stats && lane == 0) {
-        atomicAdd(&a.stats[0], st_blocks);
-        atomicAdd(&a.stats[1], st_vec);
         atomicAdd(&a.stats[2], st_surv);
         atomicAdd(&a.stats[3], (unsigned long long)(p1 - p0));
     }
@@ -1097,11 +1094,12 @@ hipError_t launch_estimate_ex(const ScanArgs& a, hipStream_t s) {
 
 size_t replay_est_lds(int k) {
     const size_t kcap = (size_t)((k + 63) & ~63);
-    return (size_t)2 * kReplayChunk * 12 + 8 * kcap + 256;
+    return 8 * kcap + 256 + 256;
 }
 
 hipError_t launch_scan_latency(const ScanArgs& a, hipStream_t s) {
-    if (!a.bp || a.nchunk > 4 || !a.est_off || !a.est_lower || !a.est_exd || !a.est_id)
+    if (!a.bp || a.nchunk > 4 || !a.est_off || !a.est_lower || !a.est_exd || !a.est_id ||
+        !a.est_bmin)
         return hipErrorInvalidValue;
     if (a.nq == 0) return hipSuccess;
     hipLaunchKernelGGL(est_offsets_kernel, dim3(1), dim3(1024), 0, s, a.probe_off, a.probe_ids,
