@@ -553,7 +553,8 @@ int erq_centroid_dist_dev(erq_index* h, const float* q_rot_dev, size_t nq, float
 
 // Workspace bytes of probe_locked (approximate distances and query norms), at offset 0.
 static size_t probe_ws_bytes(size_t nq, size_t K) {
-    return align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + 512;
+    const size_t cand = nq <= kProbeSplitNq ? align_up(nq * kProbeCandWords * 4, 256) : 0;
+    return align_up(nq * K * 4, 256) + align_up(nq * 4, 256) + cand + 512;
 }
 
 // Probe selection with its scratch (probe_ws_bytes) at `base`.
@@ -564,13 +565,14 @@ static int probe_into(erq_index* h, const float* q_rot_dev, size_t nq, size_t np
     Ws ws{base};
     float* approx = ws.take<float>(nq * K);
     float* qn = ws.take<float>(nq);
+    uint32_t* cand = nq <= kProbeSplitNq ? ws.take<uint32_t>(nq * kProbeCandWords) : nullptr;
     hipError_t e = launch_row_sqnorm(q_rot_dev, (int)nq, (int)D, (int)D, qn, s);
     if (e == hipSuccess)
         e = launch_gemm(q_rot_dev, (int)D, h->centroids, (int)D, 1, approx, (int)K, (int)nq,
                         (int)K, (int)D, 1, qn, h->cnorm, s);
     if (e != hipSuccess) return hip_fail(e, "centroid gemm");
     e = launch_probe_select(approx, qn, h->cnorm, q_rot_dev, h->centroids, (int)nq, (int)K,
-                            (int)D, (int)nprobe, pid, pd, s);
+                            (int)D, (int)nprobe, pid, pd, s, cand);
     if (e != hipSuccess) return hip_fail(e, "probe selection");
     return ERQ_OK;
 }

@@ -179,10 +179,16 @@ hipError_t launch_row_sqnorm(const float* X, int rows, int cols, int ld, float* 
 hipError_t launch_pad_rows(const float* src, int rows, int cols, float* dst, int ldd,
                            hipStream_t s);
 
-// Ranked probe lists from approximate distances (see erq_probe.hip).
-hipError_t launch_probe_select(const float* approx, const float* qnorm, const float* cnorm,
+// Ranked probe lists from approximate distances (see erq_probe.hip).  With cand (nq x
+// kProbeCandWords words) and nq <= kProbeSplitNq the selection runs as three kernels that
+// spread the candidates' exact distances over many workgroups, overwriting approx.
+constexpr int kProbeCap = 2048;  // candidates kept per query
+constexpr size_t kProbeSplitNq = 16;
+constexpr size_t kProbeCandWords = kProbeCap + 1;
+hipError_t launch_probe_select(float* approx, const float* qnorm, const float* cnorm,
                                const float* q_rot, const float* centroids, int nq, int K, int D,
-                               int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s);
+                               int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s,
+                               uint32_t* cand = nullptr);
 
 // CSR probe lists (ids and distances) from padded per-query rows of `stride` entries: query q
 // keeps the first off[q + 1] - off[q] of its row.

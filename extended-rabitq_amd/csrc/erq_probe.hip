@@ -20,7 +20,7 @@ namespace erq {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kCapC = 2048;  // candidate capacity per query
+constexpr int kCapC = kProbeCap;  // candidate capacity per query
 
 __device__ __forceinline__ bool cand_less(float da, uint32_t ia, float db, uint32_t ib) {
     return da < db || (da == db && ia < ib);
@@ -406,6 +406,114 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// Small batches: probe_select_kernel's steps over three kernels, so that the candidates'
+// exact distances -- most of one query's selection -- spread over many workgroups.  cand
+// holds per query kProbeCandWords words: the candidate count (above kCapC: every cluster is
+// a candidate), then the candidates.  The same candidates, distances and order as
+// probe_select_kernel, so the same lists.
+__global__ void __launch_bounds__(kThreads)
+    probe_cand_kernel(const float* __restrict__ approx, const float* __restrict__ qnorm,
+                      const float* __restrict__ cnorm, int K, int D, int nprobe,
+                      uint32_t* __restrict__ cand) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* keys = reinterpret_cast<uint32_t*>(smem);  // K
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
+    const int tid = threadIdx.x, qi = blockIdx.x;
+    const float* arow = approx + (size_t)qi * K;
+    const float qn = qnorm[qi];
+    const float coef = (float)(4 * D + 64) * 5.9604644775390625e-8f;  // 2^-24
+    for (int c = tid; c < K; c += kThreads) keys[c] = float_key(arow[c] + coef * (qn + cnorm[c]));
+    if (tid == 0) sh_cnt = 0;
+    __syncthreads();
+    const float U = key_float(radix_select(keys, K, (uint32_t)nprobe, hist, &sh_sel, &sh_rank));
+    uint32_t* cq = cand + (size_t)qi * kProbeCandWords;
+    for (int c = tid; c < K; c += kThreads) {
+        if (arow[c] - coef * (qn + cnorm[c]) <= U) {
+            const uint32_t slot = atomicAdd(&sh_cnt, 1u);
+            if (slot < (uint32_t)kCapC) cq[1 + slot] = (uint32_t)c;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) cq[0] = sh_cnt;
+}
+
+// Workgroup (q, y): the exact L2Sqr16 distances of query q's candidates 16 y .. 16 y + 15
+// (one per 16-lane group), into dist[q K + i] -- for cluster i when every cluster is one.
+__global__ void __launch_bounds__(kThreads)
+    cand_dist_kernel(const uint32_t* __restrict__ cand, const float* __restrict__ q_rot,
+                     const float* __restrict__ centroids, int K, int D,
+                     float* __restrict__ dist) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* qs = reinterpret_cast<float*>(smem);  // D
+    const int tid = threadIdx.x, qi = blockIdx.x, grp = tid >> 4, l = tid & 15;
+    const uint32_t* cq = cand + (size_t)qi * kProbeCandWords;
+    const uint32_t cnt = cq[0];
+    const bool all = cnt > (uint32_t)kCapC;
+    const int n = all ? K : (int)cnt;
+    const int i0 = (int)blockIdx.y * (kThreads / 16);
+    if (i0 >= n) return;  // uniform
+    for (int d = tid; d < D; d += kThreads) qs[d] = q_rot[(size_t)qi * D + d];
+    __syncthreads();
+    const int i = i0 + grp;
+    const bool act = i < n;
+    const uint32_t c = act ? (all ? (uint32_t)i : cq[1 + i]) : 0u;
+    const float p = l2sqr16_group(qs, centroids + (size_t)c * D, D, l, act);
+    if (act && l == 0) dist[(size_t)qi * K + i] = p;
+}
+
+// One workgroup per query: the first nprobe candidates by (distance, id), as
+// probe_select_kernel orders them (every cluster a candidate: the radix-selected nprobe
+// nearest, as its fallback branch).
+__global__ void __launch_bounds__(kThreads)
+    cand_sort_kernel(const uint32_t* __restrict__ cand, const float* __restrict__ dist, int K,
+                     int nprobe, uint32_t* __restrict__ probe_ids, float* __restrict__ probe_d) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* keys = reinterpret_cast<uint32_t*>(smem);          // K
+    float* cd = reinterpret_cast<float*>(smem + 4 * (size_t)K);  // kCapC
+    uint32_t* ci = reinterpret_cast<uint32_t*>(cd + kCapC);      // kCapC
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_sel, sh_rank, sh_cnt;
+    const int tid = threadIdx.x, qi = blockIdx.x;
+    const uint32_t* cq = cand + (size_t)qi * kProbeCandWords;
+    const float* drow = dist + (size_t)qi * K;
+    int cnt = (int)cq[0];
+    if (cnt <= kCapC) {
+        for (int i = tid; i < cnt; i += kThreads) {
+            ci[i] = cq[1 + i];
+            cd[i] = drow[i];
+        }
+    } else {
+        for (int c = tid; c < K; c += kThreads) keys[c] = float_key(drow[c]);
+        if (tid == 0) sh_cnt = 0;
+        __syncthreads();
+        const uint32_t U2 = radix_select(keys, K, (uint32_t)nprobe, hist, &sh_sel, &sh_rank);
+        for (int c = tid; c < K; c += kThreads) {
+            if (keys[c] <= U2) {
+                const uint32_t slot = atomicAdd(&sh_cnt, 1u);
+                if (slot < (uint32_t)kCapC) {
+                    ci[slot] = (uint32_t)c;
+                    cd[slot] = key_float(keys[c]);
+                }
+            }
+        }
+        __syncthreads();
+        cnt = min((int)sh_cnt, kCapC);
+    }
+    int P2 = 1;
+    while (P2 < cnt) P2 <<= 1;
+    for (int i = cnt + tid; i < P2; i += kThreads) {
+        cd[i] = FLT_MAX;
+        ci[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    bitonic_by_key(cd, ci, P2);
+    for (int i = tid; i < nprobe; i += kThreads) {
+        probe_ids[(size_t)qi * nprobe + i] = ci[i];
+        probe_d[(size_t)qi * nprobe + i] = cd[i];
+    }
+}
+
 __global__ void zero_u32_kernel(uint32_t* p) { *p = 0; }
 
 // CSR offsets from per-query lengths by one workgroup, 8192 lengths at a time: loaded into
@@ -503,11 +611,28 @@ hipError_t launch_compact_probes(const uint32_t* pad_ids, const float* pad_d, in
 
 size_t probe_select_lds(int K, int D) { return 4 * (size_t)K + 8 * (size_t)kCapC + 4 * (size_t)D; }
 
-hipError_t launch_probe_select(const float* approx, const float* qnorm, const float* cnorm,
+hipError_t launch_probe_select(float* approx, const float* qnorm, const float* cnorm,
                                const float* q_rot, const float* centroids, int nq, int K, int D,
-                               int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s) {
+                               int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s,
+                               uint32_t* cand) {
     if (nq == 0) return hipSuccess;
     if (nprobe < 1 || nprobe > kCapC || nprobe > K) return hipErrorInvalidValue;
+    if (cand && (size_t)nq <= kProbeSplitNq) {
+        const size_t sort_lds = 4 * (size_t)K + 8 * (size_t)kCapC;
+        if (sort_lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(
+                reinterpret_cast<const void*>(&cand_sort_kernel),
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sort_lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(probe_cand_kernel, dim3(nq), dim3(kThreads), 4 * (size_t)K, s, approx,
+                           qnorm, cnorm, K, D, nprobe, cand);
+        hipLaunchKernelGGL(cand_dist_kernel, dim3(nq, (K + kThreads / 16 - 1) / (kThreads / 16)),
+                           dim3(kThreads), 4 * (size_t)D, s, cand, q_rot, centroids, K, D, approx);
+        hipLaunchKernelGGL(cand_sort_kernel, dim3(nq), dim3(kThreads), sort_lds, s, cand, approx, K,
+                           nprobe, probe_ids, probe_d);
+        return hipGetLastError();
+    }
     size_t lds = probe_select_lds(K, D);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (lds > 64 * 1024) {
