@@ -62,6 +62,7 @@ def parse(argv=None):
     ap.add_argument("--pos-weight", type=float, default=2.0,
                     help="probe MLP loss weight of the positive labels")
     ap.add_argument("--train-batch", type=int, default=1024, help="probe MLP minibatch")
+    ap.add_argument("--hidden", type=int, default=1024, help="probe MLP hidden width")
     ap.add_argument("--labels", choices=("any", "share"), default="any",
                     help="probe MLP targets: 1 if the cluster holds any of the query's top-k "
                          "(any), or the share of the top-k it holds (share)")
@@ -289,7 +290,7 @@ def train_probe_model(a, g, idx, mix, torch):
     log(f"labels from fixed nprobe={label_np} over {a.train_queries} queries in {t_lab:.1f}s")
     t0 = time.time()
     # inputs scaled to unit variance per coordinate (rotated unit-norm queries)
-    m = ProbeMLP(idx.D, idx.K, hidden=512, max_batch=max(1024, a.train_batch), seed=7,
+    m = ProbeMLP(idx.D, idx.K, hidden=a.hidden, max_batch=max(1024, a.train_batch), seed=7,
                  input_scale=float(np.sqrt(idx.D)))
     losses = m.train(qt_r, lab, epochs=a.epochs, batch=a.train_batch, lr=a.lr,
                      pos_weight=a.pos_weight, log=log, schedule="cosine",
@@ -753,7 +754,7 @@ def main(argv=None):
                                              for s in ("replay", "scan", "reest") if s in stages})
         learned["train"] = {"queries": a.train_queries, "label_nprobe": label_np,
                             "fallback_nprobe": fixed_np, "epochs": a.epochs,
-                            "labels": a.labels, "pos_weight": a.pos_weight,
+                            "labels": a.labels, "pos_weight": a.pos_weight, "hidden": a.hidden,
                             "loss": [round(v, 3) for v in losses[::5]],
                             "label_s": round(t_lab, 1), "train_s": round(t_train, 1)}
         m.close()
