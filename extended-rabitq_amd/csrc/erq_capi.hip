@@ -566,10 +566,8 @@ static int probe_into(erq_index* h, const float* q_rot_dev, size_t nq, size_t np
     float* approx = ws.take<float>(nq * K);
     float* qn = ws.take<float>(nq);
     uint32_t* cand = nq <= kProbeSplitNq ? ws.take<uint32_t>(nq * kProbeCandWords) : nullptr;
-    hipError_t e = launch_row_sqnorm(q_rot_dev, (int)nq, (int)D, (int)D, qn, s);
-    if (e == hipSuccess)
-        e = launch_gemm(q_rot_dev, (int)D, h->centroids, (int)D, 1, approx, (int)K, (int)nq,
-                        (int)K, (int)D, 1, qn, h->cnorm, s);
+    hipError_t e = launch_query_centroid_dist(q_rot_dev, (int)nq, (int)D, h->centroids, (int)K,
+                                              h->cnorm, approx, qn, s);
     if (e != hipSuccess) return hip_fail(e, "centroid gemm");
     e = launch_probe_select(approx, qn, h->cnorm, q_rot_dev, h->centroids, (int)nq, (int)K,
                             (int)D, (int)nprobe, pid, pd, s, cand);

@@ -145,19 +145,23 @@ __global__ void pad_rows_kernel(const float* __restrict__ src, int rows, int col
 // Any order of the FP32 sum keeps the error within the bound the probe selection assumes.
 constexpr int kSkinnyRows = 8;
 
+// EPI 0: store; 1: rn[m] + cn[n] - 2 acc; 2: as 1 with rn[m] = ||A_m||^2 summed by every wave
+// in the same order (so every column uses the same value), the wave of column 0 storing it to
+// rn_out for the caller.
 template <int EPI>
 __global__ void __launch_bounds__(256) skinny_kernel(const float* __restrict__ A, int lda,
                                                      const float* __restrict__ B, int ldb,
                                                      float* __restrict__ C, int ldc, int M, int N,
                                                      int K, const float* __restrict__ rn,
-                                                     const float* __restrict__ cn) {
+                                                     const float* __restrict__ cn,
+                                                     float* __restrict__ rn_out = nullptr) {
     const int lane = threadIdx.x & 63;
     const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (n >= N) return;  // wave-uniform
     const float* b = B + (size_t)n * ldb;
-    float acc[kSkinnyRows];
+    float acc[kSkinnyRows], sq[kSkinnyRows];
 #pragma unroll
-    for (int m = 0; m < kSkinnyRows; m++) acc[m] = 0.f;
+    for (int m = 0; m < kSkinnyRows; m++) acc[m] = sq[m] = 0.f;
     for (int k0 = 4 * lane; k0 < K; k0 += 256) {
         const float4 bv = *reinterpret_cast<const float4*>(b + k0);
 #pragma unroll
@@ -168,22 +172,52 @@ __global__ void __launch_bounds__(256) skinny_kernel(const float* __restrict__ A
                 acc[m] = fmaf(av.y, bv.y, acc[m]);
                 acc[m] = fmaf(av.z, bv.z, acc[m]);
                 acc[m] = fmaf(av.w, bv.w, acc[m]);
+                if (EPI == 2) {
+                    sq[m] = fmaf(av.x, av.x, sq[m]);
+                    sq[m] = fmaf(av.y, av.y, sq[m]);
+                    sq[m] = fmaf(av.z, av.z, sq[m]);
+                    sq[m] = fmaf(av.w, av.w, sq[m]);
+                }
             }
     }
 #pragma unroll
     for (int m = 0; m < kSkinnyRows; m++)
-        for (int o = 32; o > 0; o >>= 1) acc[m] += __shfl_xor(acc[m], o);
+        for (int o = 32; o > 0; o >>= 1) {
+            acc[m] += __shfl_xor(acc[m], o);
+            if (EPI == 2) sq[m] += __shfl_xor(sq[m], o);
+        }
     if (lane < kSkinnyRows && lane < M) {
-        float v = acc[0];
+        float v = acc[0], r = sq[0];
 #pragma unroll
         for (int m = 1; m < kSkinnyRows; m++)
-            if (lane == m) v = acc[m];
+            if (lane == m) {
+                v = acc[m];
+                r = sq[m];
+            }
         if (EPI == 1) v = fmaf(-2.0f, v, rn[lane] + cn[n]);
+        if (EPI == 2) {
+            v = fmaf(-2.0f, v, r + cn[n]);
+            if (n == 0) rn_out[lane] = r;
+        }
         C[(size_t)lane * ldc + n] = v;
     }
 }
 
 }  // namespace
+
+hipError_t launch_query_centroid_dist(const float* Q, int nq, int D, const float* centroids,
+                                      int K, const float* cnorm, float* approx, float* qnorm,
+                                      hipStream_t s) {
+    if (nq == 0 || K == 0) return hipSuccess;
+    if (nq > kSkinnyRows || D % BK != 0) {
+        hipError_t e = launch_row_sqnorm(Q, nq, D, D, qnorm, s);
+        if (e != hipSuccess) return e;
+        return launch_gemm(Q, D, centroids, D, 1, approx, K, nq, K, D, 1, qnorm, cnorm, s);
+    }
+    hipLaunchKernelGGL(skinny_kernel<2>, dim3((K + 3) / 4), dim3(256), 0, s, Q, D, centroids, D,
+                       approx, K, nq, K, D, nullptr, cnorm, qnorm);
+    return hipGetLastError();
+}
 
 hipError_t launch_gemm(const float* A, int lda, const float* B, int ldb, int b_trans, float* C,
                        int ldc, int M, int N, int K, int epi, const float* rn, const float* cn,

@@ -175,6 +175,11 @@ hipError_t launch_query_order(const uint32_t* probe_off, const uint32_t* probe_i
 hipError_t launch_gemm(const float* A, int lda, const float* B, int ldb, int b_trans, float* C,
                        int ldc, int M, int N, int K, int epi, const float* rn, const float* cn,
                        hipStream_t s);
+// approx[q][c] = ||Q_q||^2 + cnorm[c] - 2 Q_q . centroid_c and qnorm[q] = ||Q_q||^2 (Q: nq rows
+// of D floats); one kernel for up to 8 queries, else launch_row_sqnorm and launch_gemm.
+hipError_t launch_query_centroid_dist(const float* Q, int nq, int D, const float* centroids,
+                                      int K, const float* cnorm, float* approx, float* qnorm,
+                                      hipStream_t s);
 hipError_t launch_row_sqnorm(const float* X, int rows, int cols, int ld, float* out, hipStream_t s);
 hipError_t launch_pad_rows(const float* src, int rows, int cols, float* dst, int ldd,
                            hipStream_t s);
