@@ -559,7 +559,8 @@ static size_t probe_ws_bytes(size_t nq, size_t K) {
 
 // Probe selection with its scratch (probe_ws_bytes) at `base`.
 static int probe_into(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
-                      uint32_t* pid, float* pd, hipStream_t s, char* base) {
+                      uint32_t* pid, float* pd, hipStream_t s, char* base,
+                      uint32_t* fixed_off = nullptr) {
     StageTimer tim(h, ERQ_STAGE_SELECT, s);
     const size_t K = h->K, D = h->D;
     Ws ws{base};
@@ -570,17 +571,17 @@ static int probe_into(erq_index* h, const float* q_rot_dev, size_t nq, size_t np
                                               h->cnorm, approx, qn, s);
     if (e != hipSuccess) return hip_fail(e, "centroid gemm");
     e = launch_probe_select(approx, qn, h->cnorm, q_rot_dev, h->centroids, (int)nq, (int)K,
-                            (int)D, (int)nprobe, pid, pd, s, cand);
+                            (int)D, (int)nprobe, pid, pd, s, cand, cand ? fixed_off : nullptr);
     if (e != hipSuccess) return hip_fail(e, "probe selection");
     return ERQ_OK;
 }
 
 static int probe_locked(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
-                        uint32_t* pid, float* pd, hipStream_t s) {
+                        uint32_t* pid, float* pd, hipStream_t s, uint32_t* fixed_off = nullptr) {
     void* wsp;
     int rc = workspace(h, probe_ws_bytes(nq, h->K), &wsp);
     if (rc) return rc;
-    return probe_into(h, q_rot_dev, nq, nprobe, pid, pd, s, (char*)wsp);
+    return probe_into(h, q_rot_dev, nq, nprobe, pid, pd, s, (char*)wsp, fixed_off);
 }
 
 int erq_probe_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t nprobe,
@@ -899,12 +900,16 @@ int erq_search_dev(erq_index* h, const float* q_rot_dev, size_t nq, size_t k, si
         uint32_t* pid = ws.take<uint32_t>(nq * nprobe);
         float* pd = ws.take<float>(nq * nprobe);
         uint32_t* off = ws.take<uint32_t>(nq + 1);
-        rc = probe_locked(h, q_rot_dev, nq, nprobe, pid, pd, s);
+        // up to kProbeSplitNq queries: the selection's last kernel writes the offsets too
+        const bool split = nq <= kProbeSplitNq;
+        rc = probe_locked(h, q_rot_dev, nq, nprobe, pid, pd, s, split ? off : nullptr);
         if (rc) return rc;
-        hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s, off,
-                           nq, (uint32_t)nprobe);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return hip_fail(e, "offsets");
+        if (!split) {
+            hipLaunchKernelGGL(fill_offsets, dim3((unsigned)((nq + 256) / 256)), dim3(256), 0, s,
+                               off, nq, (uint32_t)nprobe);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return hip_fail(e, "offsets");
+        }
         return scan_locked(h, q_rot_dev, nq, k, off, pid, pd, ids_dev, dists_dev, stats_dev, s, 0,
                            nullptr, nprobe);
     });

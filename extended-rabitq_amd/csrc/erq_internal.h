@@ -186,14 +186,15 @@ hipError_t launch_pad_rows(const float* src, int rows, int cols, float* dst, int
 
 // Ranked probe lists from approximate distances (see erq_probe.hip).  With cand (nq x
 // kProbeCandWords words) and nq <= kProbeSplitNq the selection runs as three kernels that
-// spread the candidates' exact distances over many workgroups, overwriting approx.
+// spread the candidates' exact distances over many workgroups, overwriting approx; it can
+// then also write the lists' CSR offsets (fixed_off, nq + 1 entries: q * nprobe).
 constexpr int kProbeCap = 2048;  // candidates kept per query
 constexpr size_t kProbeSplitNq = 16;
 constexpr size_t kProbeCandWords = kProbeCap + 1;
 hipError_t launch_probe_select(float* approx, const float* qnorm, const float* cnorm,
                                const float* q_rot, const float* centroids, int nq, int K, int D,
                                int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s,
-                               uint32_t* cand = nullptr);
+                               uint32_t* cand = nullptr, uint32_t* fixed_off = nullptr);
 
 // CSR probe lists (ids and distances) from padded per-query rows of `stride` entries: query q
 // keeps the first off[q + 1] - off[q] of its row.

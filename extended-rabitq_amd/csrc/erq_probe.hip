@@ -467,7 +467,8 @@ __global__ void __launch_bounds__(kThreads)
 // nearest, as its fallback branch).
 __global__ void __launch_bounds__(kThreads)
     cand_sort_kernel(const uint32_t* __restrict__ cand, const float* __restrict__ dist, int K,
-                     int nprobe, uint32_t* __restrict__ probe_ids, float* __restrict__ probe_d) {
+                     int nprobe, uint32_t* __restrict__ probe_ids, float* __restrict__ probe_d,
+                     uint32_t* __restrict__ fixed_off) {
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* keys = reinterpret_cast<uint32_t*>(smem);          // K
     float* cd = reinterpret_cast<float*>(smem + 4 * (size_t)K);  // kCapC
@@ -477,6 +478,8 @@ __global__ void __launch_bounds__(kThreads)
     const int tid = threadIdx.x, qi = blockIdx.x;
     const uint32_t* cq = cand + (size_t)qi * kProbeCandWords;
     const float* drow = dist + (size_t)qi * K;
+    if (fixed_off && qi == 0)  // the lists' CSR offsets, every query nprobe entries
+        for (int i = tid; i <= (int)gridDim.x; i += kThreads) fixed_off[i] = (uint32_t)(i * nprobe);
     int cnt = (int)cq[0];
     if (cnt <= kCapC) {
         for (int i = tid; i < cnt; i += kThreads) {
@@ -614,9 +617,10 @@ size_t probe_select_lds(int K, int D) { return 4 * (size_t)K + 8 * (size_t)kCapC
 hipError_t launch_probe_select(float* approx, const float* qnorm, const float* cnorm,
                                const float* q_rot, const float* centroids, int nq, int K, int D,
                                int nprobe, uint32_t* probe_ids, float* probe_d, hipStream_t s,
-                               uint32_t* cand) {
+                               uint32_t* cand, uint32_t* fixed_off) {
     if (nq == 0) return hipSuccess;
     if (nprobe < 1 || nprobe > kCapC || nprobe > K) return hipErrorInvalidValue;
+    if (fixed_off && !(cand && (size_t)nq <= kProbeSplitNq)) return hipErrorInvalidValue;
     if (cand && (size_t)nq <= kProbeSplitNq) {
         const size_t sort_lds = 4 * (size_t)K + 8 * (size_t)kCapC;
         if (sort_lds > 64 * 1024) {
@@ -630,7 +634,7 @@ hipError_t launch_probe_select(float* approx, const float* qnorm, const float* c
         hipLaunchKernelGGL(cand_dist_kernel, dim3(nq, (K + kThreads / 16 - 1) / (kThreads / 16)),
                            dim3(kThreads), 4 * (size_t)D, s, cand, q_rot, centroids, K, D, approx);
         hipLaunchKernelGGL(cand_sort_kernel, dim3(nq), dim3(kThreads), sort_lds, s, cand, approx, K,
-                           nprobe, probe_ids, probe_d);
+                           nprobe, probe_ids, probe_d, fixed_off);
         return hipGetLastError();
     }
     size_t lds = probe_select_lds(K, D);
